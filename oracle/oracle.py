@@ -1,0 +1,410 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+The oracle is a FP64 C++ restatement of FastMultipole's FMM path
+(src/tree.jl, src/interaction_list.jl, src/bodytomultipole.jl, src/rotate.jl,
+src/translate.jl, src/evaluate_expansions.jl, src/fmm.jl and the user kernels
+of test/gravitational.jl and test/vortex.jl). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only as the checker; the product path never touches it.
+
+Parity status of each restated function is recorded in the header of
+tests/test_oracle_golden.py (which fixture or closed form pins it).
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KIND_SOURCE = 0   # Point{Source}, test/gravitational.jl
+KIND_VORTEX = 1   # Point{Vortex}, test/vortex.jl
+METHOD_BARBA, METHOD_SELFTUNING, METHOD_SELFTUNING_TREESTOP = 0, 1, 2
+ERR_NONE, ERR_ROTATED_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_ip32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_ic = np.ctypeslib.ndpointer(dtype=np.intc, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        L = C.CDLL(path)
+        L.orc_run_create.restype = C.c_void_p
+        L.orc_run_create.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ic, _dp]
+        for f in ("orc_run_free", "orc_run_fmm", "orc_run_trees", "orc_run_lists"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = None
+        for f in ("orc_n_branches", "orc_n_levels", "orc_n_leaves", "orc_n_list"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int]
+            getattr(L, f).restype = C.c_int64
+        L.orc_get_branches.argtypes = [C.c_void_p, C.c_int, _ip64, _dp]
+        L.orc_get_levels.argtypes = [C.c_void_p, C.c_int, _ip64]
+        L.orc_get_leaf_index.argtypes = [C.c_void_p, C.c_int, _ip64]
+        L.orc_get_sort_index.argtypes = [C.c_void_p, C.c_int, _ip64]
+        L.orc_get_buffer.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_get_expansions.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_get_list.argtypes = [C.c_void_p, C.c_int, _ip32]
+        L.orc_get_m2l_orders.argtypes = [C.c_void_p, _ip32]
+        L.orc_get_targets.argtypes = [C.c_void_p, _dp]
+        L.orc_pmax.argtypes = [C.c_void_p]
+        L.orc_pmax.restype = C.c_int
+        L.orc_error_success.argtypes = [C.c_void_p]
+        L.orc_error_success.restype = C.c_int
+        L.orc_direct.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int, C.c_int]
+        L.orc_direct_sampled.argtypes = [C.c_int, C.c_int64, _ip64, _dp, _dp, C.c_int64, _dp, C.c_int, C.c_int, C.c_int]
+        L.orc_tables.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_hs_fresh.argtypes = [C.c_int, _dp]
+        L.orc_pnms.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_update_Ts.argtypes = [C.c_double, C.c_int, _dp]
+        L.orc_eimphis.argtypes = [C.c_double, C.c_int, _dp]
+        L.orc_zeta_sign.argtypes = [C.c_double, C.c_int, C.c_int, _dp]
+        L.orc_eta_sign.argtypes = [C.c_double, C.c_int, C.c_int, _dp]
+        L.orc_regular_harmonics.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_irregular_harmonics.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_c2s.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_factorial.argtypes = [C.c_int]
+        L.orc_factorial.restype = C.c_double
+        for f in ("orc_rotate_multipole_y", "orc_rotate_local_y", "orc_rotate_z", "orc_translate_multipole_z",
+                  "orc_translate_m2l_z", "orc_translate_local_z"):
+            getattr(L, f).argtypes = [_dp, C.c_double, C.c_int, C.c_int, _dp]
+        L.orc_transform_lh_multipole.argtypes = [_dp, C.c_double, C.c_int]
+        L.orc_transform_lh_local.argtypes = [_dp, C.c_double, C.c_int]
+        L.orc_m2m.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_l2l.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_m2l.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
+                              C.POINTER(C.c_int)]
+        L.orc_m2l.restype = C.c_int
+        L.orc_l2p.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_p2m.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp]
+        _LIB = L
+    return _LIB
+
+
+def nh(p):
+    return (p + 1) * (p + 2) // 2
+
+
+def length_Ts(p):
+    return (p + 1) * (p + 2) * (2 * p + 3) // 6
+
+
+def length_Hs(p):
+    n = p + 1
+    return n * (n + 1) * (n + 2) // 6
+
+
+def data_per_body(kind):
+    return 5 if kind == KIND_SOURCE else 7
+
+
+class Run:
+    """One oracle FMM evaluation (trees, lists, passes) on one target and one source system.
+
+    positions: (N,3); source_data: (N, data_per_body) in the reference's source
+    buffer row order [x,y,z,radius,strength...] (src/compatibility.jl:5-20).
+    """
+
+    def __init__(self, kind, target_xyz, source_data, p=5, leaf_size_source=20, leaf_size_target=None,
+                 method=METHOD_SELFTUNING_TREESTOP, shrink_recenter=True, error_kind=ERR_NONE, relative=False,
+                 eps=0.0, BE=True, scalar_potential=False, gradient=True, hessian=False, farfield=True,
+                 nearfield=True, self_induced=True, multipole_acceptance=0.4):
+        L = lib()
+        self.kind = kind
+        t = np.zeros((len(target_xyz), 16))
+        t[:, :3] = target_xyz
+        s = np.ascontiguousarray(source_data, dtype=np.float64)
+        assert s.shape[1] == data_per_body(kind)
+        self.nt, self.ns = len(t), len(s)
+        if leaf_size_target is None:
+            leaf_size_target = leaf_size_source
+        oi = np.array([p, leaf_size_source, leaf_size_target, method, int(shrink_recenter), error_kind, int(relative),
+                       int(BE), int(scalar_potential), int(gradient), int(hessian), int(farfield), int(nearfield),
+                       int(self_induced)], dtype=np.intc)
+        od = np.array([multipole_acceptance, eps], dtype=np.float64)
+        self._h = L.orc_run_create(kind, self.nt, np.ascontiguousarray(t.ravel()), self.ns,
+                                   np.ascontiguousarray(s.ravel()), oi, od)
+        self.p = p
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_run_free(self._h)
+            self._h = None
+
+    def fmm(self):
+        lib().orc_run_fmm(self._h)
+        return self
+
+    def trees(self):
+        lib().orc_run_trees(self._h)
+        return self
+
+    def lists(self):
+        lib().orc_run_lists(self._h)
+        return self
+
+    def branches(self, which):
+        L = lib()
+        nb = L.orc_n_branches(self._h, which)
+        ints = np.zeros(nb * 6, dtype=np.int64)
+        reals = np.zeros(nb * 17)
+        L.orc_get_branches(self._h, which, ints, reals)
+        return ints.reshape(nb, 6), reals.reshape(nb, 17)
+
+    def levels(self, which):
+        L = lib()
+        n = L.orc_n_levels(self._h, which)
+        out = np.zeros(2 * n, dtype=np.int64)
+        L.orc_get_levels(self._h, which, out)
+        return out.reshape(n, 2)
+
+    def leaf_index(self, which):
+        L = lib()
+        out = np.zeros(L.orc_n_leaves(self._h, which), dtype=np.int64)
+        L.orc_get_leaf_index(self._h, which, out)
+        return out
+
+    def sort_index(self, which):
+        out = np.zeros(self.nt if which == 0 else self.ns, dtype=np.int64)
+        lib().orc_get_sort_index(self._h, which, out)
+        return out
+
+    def buffer(self, which):
+        rows = 16 if which == 0 else data_per_body(self.kind)
+        n = self.nt if which == 0 else self.ns
+        out = np.zeros(rows * n)
+        lib().orc_get_buffer(self._h, which, out)
+        return out.reshape(n, rows)
+
+    def expansions(self, which):
+        nb = lib().orc_n_branches(self._h, which)
+        out = np.zeros(nb * 4 * nh(self.p))
+        lib().orc_get_expansions(self._h, which, out)
+        return out.reshape(nb, nh(self.p), 2, 2)   # [branch, i, phi/chi, re/im]
+
+    def list(self, which):
+        L = lib()
+        n = L.orc_n_list(self._h, which)
+        out = np.zeros(2 * n, dtype=np.int32)
+        L.orc_get_list(self._h, which, out)
+        return out.reshape(n, 2)
+
+    def m2l_orders(self):
+        n = lib().orc_n_list(self._h, 0)
+        out = np.zeros(n, dtype=np.int32)
+        lib().orc_get_m2l_orders(self._h, out)
+        return out
+
+    def targets(self):
+        out = np.zeros(self.nt * 16)
+        lib().orc_get_targets(self._h, out)
+        return out.reshape(self.nt, 16)
+
+    @property
+    def pmax(self):
+        return lib().orc_pmax(self._h)
+
+    @property
+    def error_success(self):
+        return bool(lib().orc_error_success(self._h))
+
+
+def direct(kind, target_xyz, source_data, scalar_potential=False, gradient=True, hessian=False):
+    """direct!(targets, sources) of src/direct.jl:43-75; returns the (N,16) target buffer."""
+    t = np.zeros((len(target_xyz), 16))
+    t[:, :3] = target_xyz
+    tb = np.ascontiguousarray(t.ravel())
+    s = np.ascontiguousarray(np.asarray(source_data, dtype=np.float64).ravel())
+    lib().orc_direct(kind, len(t), tb, len(source_data), s, int(scalar_potential), int(gradient), int(hessian))
+    return tb.reshape(-1, 16)
+
+
+def direct_sampled(kind, target_xyz, idx, source_data, scalar_potential=False, gradient=True, hessian=False):
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = np.zeros(len(idx) * 16)
+    lib().orc_direct_sampled(kind, len(idx), idx, np.ascontiguousarray(np.asarray(target_xyz, dtype=np.float64).ravel()),
+                             out, len(source_data), np.ascontiguousarray(np.asarray(source_data, np.float64).ravel()),
+                             int(scalar_potential), int(gradient), int(hessian))
+    return out.reshape(-1, 16)
+
+
+# ---------------------------------------------------------------- operators
+def _exp(p):
+    return np.zeros(4 * nh(p))
+
+
+def tables(p):
+    Hs, z, e, M, Lt = np.zeros(length_Hs(p)), np.zeros(length_Ts(p)), np.zeros(length_Ts(p)), np.zeros(nh(p)), np.zeros(nh(p))
+    lib().orc_tables(p, Hs, z, e, M, Lt)
+    return Hs, z, e, M, Lt
+
+
+def update_Ts(beta, p):
+    out = np.zeros(length_Ts(p))
+    lib().orc_update_Ts(beta, p, out)
+    return out
+
+
+def pnms(p, s, c):
+    out = np.zeros(nh(p))
+    lib().orc_pnms(p, s, c, out)
+    return out
+
+
+def eimphis(phi, P):
+    out = np.zeros(2 * (P + 1))
+    lib().orc_eimphis(phi, P, out)
+    return out.reshape(P + 1, 2)
+
+
+def zeta_sign(mag, mp, m):
+    out = np.zeros(2)
+    lib().orc_zeta_sign(mag, mp, m, out)
+    return out
+
+
+def eta_sign(mag, mp, m):
+    out = np.zeros(2)
+    lib().orc_eta_sign(mag, mp, m, out)
+    return out
+
+
+def regular_harmonics(rho, theta, phi, p):
+    out = _exp(p)
+    lib().orc_regular_harmonics(rho, theta, phi, p, out)
+    return out.reshape(nh(p), 2, 2)[:, 0, 0] + 1j * out.reshape(nh(p), 2, 2)[:, 0, 1]
+
+
+def irregular_harmonics(rho, theta, phi, p):
+    out = _exp(p)
+    lib().orc_irregular_harmonics(rho, theta, phi, p, out)
+    return out.reshape(nh(p), 2, 2)[:, 0, 0] + 1j * out.reshape(nh(p), 2, 2)[:, 0, 1]
+
+
+def cartesian_to_spherical(x, y, z):
+    out = np.zeros(3)
+    lib().orc_c2s(x, y, z, out)
+    return tuple(out)
+
+
+def factorial(k):
+    return lib().orc_factorial(k)
+
+
+def _op(name, e, arg, p, LH):
+    out = _exp(p)
+    getattr(lib(), name)(np.ascontiguousarray(e, dtype=np.float64), arg, p, int(LH), out)
+    return out
+
+
+def rotate_multipole_y(e, theta, p, LH=False):
+    return _op("orc_rotate_multipole_y", e, theta, p, LH)
+
+
+def rotate_local_y(e, theta, p, LH=False):
+    return _op("orc_rotate_local_y", e, theta, p, LH)
+
+
+def rotate_z(e, phi, p, LH=False):
+    return _op("orc_rotate_z", e, phi, p, LH)
+
+
+def translate_multipole_z(e, t, p, LH=False):
+    return _op("orc_translate_multipole_z", e, t, p, LH)
+
+
+def translate_m2l_z(e, t, p, LH=False):
+    return _op("orc_translate_m2l_z", e, t, p, LH)
+
+
+def translate_local_z(e, t, p, LH=False):
+    return _op("orc_translate_local_z", e, t, p, LH)
+
+
+def transform_lh_multipole(e, r, p):
+    out = np.array(e, dtype=np.float64)
+    lib().orc_transform_lh_multipole(out, r, p)
+    return out
+
+
+def transform_lh_local(e, r, p):
+    out = np.array(e, dtype=np.float64)
+    lib().orc_transform_lh_local(out, r, p)
+    return out
+
+
+def m2m(target_center, source_center, e, p, LH=False, out=None):
+    out = _exp(p) if out is None else out
+    lib().orc_m2m(np.asarray(target_center, float), np.asarray(source_center, float), np.ascontiguousarray(e), p, int(LH), out)
+    return out
+
+
+def l2l(target_center, source_center, e, p, LH=False, out=None):
+    out = _exp(p) if out is None else out
+    lib().orc_l2l(np.asarray(target_center, float), np.asarray(source_center, float), np.ascontiguousarray(e), p, int(LH), out)
+    return out
+
+
+def geom(center, radius=0.0, box=(0.0, 0.0, 0.0), max_influence=0.0):
+    g = np.zeros(15)
+    g[0:3] = center
+    g[3:6] = center
+    g[6] = radius
+    g[7] = radius
+    g[8:11] = box
+    g[11:14] = box
+    g[14] = max_influence
+    return g
+
+
+def m2l(tgeom, sgeom, e, p, LH=False, err_kind=ERR_NONE, relative=False, eps=0.0, BE=True, out=None):
+    out = _exp(p) if out is None else out
+    ok = C.c_int(0)
+    P = lib().orc_m2l(np.ascontiguousarray(tgeom, float), np.ascontiguousarray(sgeom, float), np.ascontiguousarray(e),
+                      p, int(LH), err_kind, int(relative), eps, int(BE), out, C.byref(ok))
+    return out, P, bool(ok.value)
+
+
+def l2p(dx, local, p, LH=False, PS=True, GS=True, HS=True):
+    out = np.zeros(13)
+    lib().orc_l2p(np.asarray(dx, float), np.ascontiguousarray(local), p, int(LH), int(PS), int(GS), int(HS), out)
+    return out[0], out[1:4], out[4:13]
+
+
+def p2m(kind, dx, strength, p, out=None):
+    out = _exp(p) if out is None else out
+    lib().orc_p2m(kind, np.asarray(dx, float), np.atleast_1d(np.asarray(strength, float)), p, out)
+    return out
+
+
+def exp_to_complex(e, p, index=0):
+    """[re/im, phi/chi, H] flat -> complex coefficients of potential `index` (m >= 0 layout)."""
+    a = np.asarray(e).reshape(nh(p), 2, 2)
+    return a[:, index, 0] + 1j * a[:, index, 1]
+
+
+def complex_to_exp(c_phi, p, c_chi=None):
+    a = np.zeros((nh(p), 2, 2))
+    a[:, 0, 0] = np.real(c_phi)
+    a[:, 0, 1] = np.imag(c_phi)
+    if c_chi is not None:
+        a[:, 1, 0] = np.real(c_chi)
+        a[:, 1, 1] = np.imag(c_chi)
+    return a.ravel()
+
+
+def full_to_compressed(v, p):
+    """Reference test layout (n, m = -n..n) -> (n, m >= 0) as in test/runtests.jl:23-37."""
+    out = []
+    i = 0
+    for n in range(p + 1):
+        for m in range(-n, n + 1):
+            if m >= 0:
+                out.append(v[i])
+            i += 1
+    return np.array(out)

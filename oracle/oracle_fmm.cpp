@@ -1,0 +1,800 @@
+// Oracle (test infrastructure only): CPU FP64 restatement of FastMultipole's
+// octree (src/tree.jl), interaction lists (src/interaction_list.jl), pass
+// drivers (src/fmm.jl) and the user kernels of test/gravitational.jl and
+// test/vortex.jl. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker.
+//
+// Index convention: branches, bodies and leaves are 0-based here; the
+// reference's 1-based values are these plus one (root parent 0 <-> -1).
+#include "oracle_math.hpp"
+#include <cstring>
+#include <memory>
+#include <array>
+
+namespace orc {
+
+struct OBranch {
+    int64_t n_bodies, first;           // bodies_index = first : first+n_bodies-1
+    int64_t n_branches, child_first;   // branch_index = child_first : child_first+n_branches-1
+    int64_t i_parent, i_leaf;          // -1 when absent
+    double source_center[3], target_center[3];
+    double source_radius, target_radius;
+    double source_box[3], target_box[3];
+    double max_influence;
+};
+
+enum { METHOD_BARBA = 0, METHOD_SELFTUNING = 1, METHOD_SELFTUNING_TREESTOP = 2 };
+enum { KIND_SOURCE = 0, KIND_VORTEX = 1 };
+
+struct OTree {
+    std::vector<OBranch> branches;
+    std::vector<std::array<int64_t, 2>> levels;   // [first, last] 0-based inclusive
+    std::vector<int64_t> leaf_index, sort_index, inverse_sort_index;
+    std::vector<double> buffer;   // rows x N, column-major, sorted order
+    int rows = 0;
+    std::vector<double> expansions;
+    int p = 0;
+};
+
+// -------------------------------------------------------------- octree
+static bool exceeds(int64_t n, int64_t ls, int method) {   // src/tree.jl:492-569 (single system)
+    if (method == METHOD_BARBA) return n > ls;
+    double fraction = 0.0;
+    fraction += (double)n / (double)(ls * ls);
+    fraction *= (double)ls;
+    if (method == METHOD_SELFTUNING) return fraction > 0.5 && n > 1;
+    return fraction > 2.8284271247461903 && n > 1;
+}
+
+static inline int get_octant(const double* pos, const double* c) {   // src/tree.jl:474-480
+    int o = 1;
+    if (pos[0] > c[0]) o += 1;
+    if (pos[1] > c[1]) o += 2;
+    if (pos[2] > c[2]) o += 4;
+    return o;
+}
+
+static void get_child_center(const double* pc, const double* pbox, int oct, double* cc) {   // src/tree.jl:581-592
+    double delta = pbox[0] * 0.5;
+    oct -= 1;
+    double dx = (oct % 2 == 0) ? -delta : delta;
+    delta = pbox[1] * 0.5;
+    oct >>= 1;
+    double dy = (oct % 2 == 0) ? -delta : delta;
+    delta = pbox[2] * 0.5;
+    oct >>= 1;
+    double dz = (oct % 2 == 0) ? -delta : delta;
+    cc[0] = pc[0] + dx; cc[1] = pc[1] + dy; cc[2] = pc[2] + dz;
+}
+
+struct TreeBuilder {
+    OTree& T;
+    int64_t N;
+    int64_t ls;
+    int method;
+    std::vector<double> small;        // 3 x N
+    std::vector<int64_t> sort_buf;
+    TreeBuilder(OTree& t, int64_t n, int64_t l, int m) : T(t), N(n), ls(l), method(m), small(3 * n), sort_buf(n) {}
+    double* pos(int64_t i) { return &T.buffer[(size_t)i * T.rows]; }
+
+    void census(int64_t first, int64_t count, const double* c, int64_t* cum) {   // :608-625
+        for (int k = 0; k < 8; ++k) cum[k] = 0;
+        for (int64_t i = first; i < first + count; ++i) cum[get_octant(pos(i), c) - 1] += 1;
+        for (int k = 1; k < 8; ++k) cum[k] += cum[k - 1];
+    }
+
+    // branch! (src/tree.jl:433-463)
+    OBranch branch(int64_t i_first_branch, int64_t first, int64_t count, const double* sc, const double* tc,
+                   double sr, double tr, const double* sbox, const double* tbox, int64_t i_parent, int64_t& i_leaf,
+                   int64_t& n_children) {
+        int64_t cum[8];
+        census(first, count, tc, cum);
+        n_children = 0;
+        if (exceeds(cum[7], ls, method))
+            for (int o = 0; o < 8; ++o) if ((o == 0 ? cum[0] : cum[o] - cum[o - 1]) > 0) n_children += 1;
+        if (n_children > 0) {
+            int64_t oi[8];   // octant_beginning_index! (0-based)
+            for (int o = 7; o >= 1; --o) oi[o] = cum[o - 1] + first;
+            oi[0] = first;
+            for (int64_t i = first; i < first + count; ++i) {   // sort_bodies! :712-737
+                int o = get_octant(pos(i), tc) - 1;
+                int64_t ti = oi[o];
+                for (int d = 0; d < 3; ++d) small[3 * ti + d] = pos(i)[d];
+                sort_buf[ti] = T.sort_index[i];
+                oi[o] += 1;
+            }
+            for (int64_t i = first; i < first + count; ++i) {
+                for (int d = 0; d < 3; ++d) pos(i)[d] = small[3 * i + d];
+                T.sort_index[i] = sort_buf[i];
+            }
+        }
+        OBranch b;
+        b.n_bodies = count;
+        b.first = first;
+        b.n_branches = n_children;
+        b.child_first = i_first_branch;
+        b.i_parent = i_parent;
+        if (n_children == 0) { b.i_leaf = i_leaf; i_leaf += 1; } else b.i_leaf = -1;
+        for (int d = 0; d < 3; ++d) {
+            b.source_center[d] = sc[d]; b.target_center[d] = tc[d];
+            b.source_box[d] = sbox[d]; b.target_box[d] = tbox[d];
+        }
+        b.source_radius = sr;
+        b.target_radius = tr;
+        b.max_influence = 0.0;
+        return b;
+    }
+
+    // child_branches! (src/tree.jl:346-376)
+    void child_branches(int64_t& pfirst, int64_t& plast, int64_t& n_children, int64_t& i_leaf) {
+        int64_t i_first_branch = plast + n_children + 1;
+        for (int64_t ip = pfirst; ip <= plast; ++ip) {
+            OBranch parent = T.branches[ip];
+            if (parent.n_branches > 0) {
+                double child_radius = parent.target_radius * 0.5;
+                double child_box[3] = {parent.target_box[0] * 0.5, parent.target_box[1] * 0.5, parent.target_box[2] * 0.5};
+                int64_t cum[8];
+                census(parent.first, parent.n_bodies, parent.target_center, cum);
+                if (exceeds(cum[7], ls, method)) {
+                    for (int o = 1; o <= 8; ++o) {
+                        int64_t pop = (o == 1) ? cum[0] : cum[o - 1] - cum[o - 2];
+                        if (pop > 0) {
+                            int64_t off = (o == 1) ? 0 : cum[o - 2];
+                            double cc[3];
+                            get_child_center(parent.target_center, parent.target_box, o, cc);
+                            int64_t ngc;
+                            OBranch child = branch(i_first_branch, parent.first + off, pop, cc, cc, child_radius, child_radius,
+                                                   child_box, child_box, ip, i_leaf, ngc);
+                            i_first_branch += ngc;
+                            T.branches.push_back(child);
+                        }
+                    }
+                }
+            }
+        }
+        n_children = i_first_branch - (int64_t)T.branches.size();
+        pfirst = plast + 1;
+        plast = (int64_t)T.branches.size() - 1;
+    }
+};
+
+// center_box / source_center_box / shrink_radius (src/tree.jl:845-1139)
+static void shrink_recenter(OTree& T, bool use_radius) {   // src/tree.jl:1176-1188
+    const int rows = T.rows;
+    auto P = [&](int64_t i) { return &T.buffer[(size_t)i * rows]; };
+    auto R = [&](int64_t i) { return use_radius ? T.buffer[(size_t)i * rows + 3] : 0.0; };
+    for (int64_t lv = (int64_t)T.levels.size() - 1; lv >= 0; --lv) {
+        for (int64_t ib = T.levels[lv][0]; ib <= T.levels[lv][1]; ++ib) {
+            OBranch& b = T.branches[ib];
+            double tc[3], tb[3], sc[3], sbx[3], tr = 0.0, sr = 0.0;
+            if (b.n_branches == 0) {   // shrink_leaf!
+                double mn[3], mx[3], smn[3], smx[3];
+                for (int d = 0; d < 3; ++d) { mn[d] = mx[d] = smn[d] = smx[d] = P(b.first)[d]; }
+                for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+                    double r = R(i);
+                    for (int d = 0; d < 3; ++d) {
+                        double x = P(i)[d];
+                        mn[d] = std::min(mn[d], x); mx[d] = std::max(mx[d], x);
+                        smn[d] = std::min(smn[d], x - r); smx[d] = std::max(smx[d], x + r);
+                    }
+                }
+                for (int d = 0; d < 3; ++d) {
+                    tc[d] = (mx[d] + mn[d]) * 0.5; tb[d] = mx[d] - tc[d];
+                    sc[d] = (smx[d] + smn[d]) * 0.5; sbx[d] = smx[d] - sc[d];
+                }
+                for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+                    double dx = P(i)[0] - tc[0], dy = P(i)[1] - tc[1], dz = P(i)[2] - tc[2];
+                    double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    tr = std::max(tr, dist);
+                    dx = P(i)[0] - sc[0]; dy = P(i)[1] - sc[1]; dz = P(i)[2] - sc[2];
+                    dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    sr = std::max(sr, dist + R(i));
+                }
+            } else {   // shrink_branch!
+                const OBranch& f = T.branches[b.child_first];
+                double mn[3], mx[3], smn[3], smx[3];
+                for (int d = 0; d < 3; ++d) { mn[d] = mx[d] = f.target_center[d]; smn[d] = smx[d] = f.source_center[d]; }
+                for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c) {
+                    const OBranch& ch = T.branches[c];
+                    for (int d = 0; d < 3; ++d) {
+                        mn[d] = std::min(mn[d], ch.target_center[d] - ch.target_box[d]);
+                        mx[d] = std::max(mx[d], ch.target_center[d] + ch.target_box[d]);
+                        smn[d] = std::min(smn[d], ch.source_center[d] - ch.source_box[d]);
+                        smx[d] = std::max(smx[d], ch.source_center[d] + ch.source_box[d]);
+                    }
+                }
+                for (int d = 0; d < 3; ++d) {
+                    tc[d] = (mn[d] + mx[d]) * 0.5; tb[d] = mx[d] - tc[d];
+                    sc[d] = (smn[d] + smx[d]) * 0.5; sbx[d] = smx[d] - sc[d];
+                }
+                for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c) {
+                    const OBranch& ch = T.branches[c];
+                    double dx = ch.target_center[0] - tc[0], dy = ch.target_center[1] - tc[1], dz = ch.target_center[2] - tc[2];
+                    double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    tr = std::max(tr, dist + ch.target_radius);
+                    dx = ch.source_center[0] - sc[0]; dy = ch.source_center[1] - sc[1]; dz = ch.source_center[2] - sc[2];
+                    dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    sr = std::max(sr, dist + ch.source_radius);
+                }
+            }
+            for (int d = 0; d < 3; ++d) {
+                b.target_center[d] = tc[d]; b.target_box[d] = tb[d];
+                b.source_center[d] = sc[d]; b.source_box[d] = sbx[d];
+            }
+            b.target_radius = tr;
+            b.source_radius = sr;
+        }
+    }
+}
+
+// Tree(systems, target; ...) for one system (src/tree.jl:3-122).
+// `full` is the system in its original order: rows x N (target: 16 rows with
+// positions only; source: data_per_body rows).
+static void build_tree(OTree& T, int64_t N, int rows, const double* full, bool target, int64_t ls, int method,
+                       bool shrink, int p, int n_divisions = 20) {
+    T = OTree();
+    T.rows = rows;
+    T.p = p;
+    if (N == 0) return;
+    T.buffer.assign((size_t)rows * N, 0.0);
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) T.buffer[(size_t)i * rows + d] = full[(size_t)i * rows + d];   // target_to_buffer!
+    T.sort_index.resize(N);
+    for (int64_t i = 0; i < N; ++i) T.sort_index[i] = i;
+    // center_box (:894-907)
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) mn[d] = mx[d] = T.buffer[d];
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) {
+            double x = T.buffer[(size_t)i * rows + d];
+            mn[d] = std::min(mn[d], x);
+            mx[d] = std::max(mx[d], x);
+        }
+    double center[3], box[3];
+    for (int d = 0; d < 3; ++d) { center[d] = (mx[d] + mn[d]) * 0.5; box[d] = mx[d] - center[d]; }
+    double bmax = std::max(std::max(box[0], box[1]), box[2]);
+    double radius = std::sqrt(bmax * bmax * 3.0);
+    double cbox[3] = {bmax, bmax, bmax};
+    TreeBuilder B(T, N, ls, method);
+    int64_t i_leaf = 0, n_children = 0;
+    OBranch root = B.branch(1, 0, N, center, center, radius, radius, cbox, cbox, -1, i_leaf, n_children);
+    T.branches.push_back(root);
+    int64_t pf = 0, pl = 0;
+    T.levels.push_back({0, 0});
+    for (int k = 0; k < n_divisions; ++k)
+        if (n_children > 0) { B.child_branches(pf, pl, n_children, i_leaf); T.levels.push_back({pf, pl}); }
+    while (n_children > 0) { B.child_branches(pf, pl, n_children, i_leaf); T.levels.push_back({pf, pl}); }
+    if (!target)   // system_to_buffer!(buffers, systems, sort_index)
+        for (int64_t i = 0; i < N; ++i)
+            for (int r = 0; r < rows; ++r) T.buffer[(size_t)i * rows + r] = full[(size_t)T.sort_index[i] * rows + r];
+    T.inverse_sort_index.resize(N);
+    for (int64_t i = 0; i < N; ++i) T.inverse_sort_index[T.sort_index[i]] = i;
+    if (shrink) shrink_recenter(T, !target);
+    for (size_t ib = 0; ib < T.branches.size(); ++ib)
+        if (T.branches[ib].n_branches == 0) T.leaf_index.push_back((int64_t)ib);
+    T.expansions.assign((size_t)4 * n_harmonics(p) * T.branches.size(), 0.0);
+}
+
+// ----------------------------------------------------- interaction lists
+struct ListBuilder {   // src/interaction_list.jl:3-166
+    const OTree& Tt;
+    const OTree& Ts;
+    int64_t ls;
+    double mac;
+    bool ff, nf, si;
+    int method;
+    std::vector<std::array<int32_t, 2>> m2l, direct;
+    void run(int32_t it, int32_t js) {
+        const OBranch& s = Ts.branches[js];
+        const OBranch& t = Tt.branches[it];
+        if (method == METHOD_SELFTUNING_TREESTOP || method == METHOD_BARBA) {
+            bool both_leaves = (s.n_branches == 0 && t.n_branches == 0);
+            if (method == METHOD_SELFTUNING_TREESTOP && both_leaves) {
+                if ((nf || (it == js && si)) && (it != js || si)) direct.push_back({it, js});
+                return;
+            }
+            double dx = t.target_center[0] - s.source_center[0];
+            double dy = t.target_center[1] - s.source_center[1];
+            double dz = t.target_center[2] - s.source_center[2];
+            double sep2 = dx * dx + dy * dy + dz * dz;
+            double sr = s.source_radius + t.target_radius;
+            if (sep2 * mac * mac > sr * sr) {
+                if (ff) m2l.push_back({it, js});
+                return;
+            }
+            if (method == METHOD_BARBA && both_leaves) {
+                if ((nf || (it == js && si)) && (it != js || si)) direct.push_back({it, js});
+                return;
+            }
+            bool split_target;
+            if (method == METHOD_BARBA)
+                split_target = s.n_branches == 0 || (t.target_radius >= s.source_radius && t.n_branches != 0);
+            else
+                split_target = s.n_branches == 0 || (t.n_bodies >= s.n_bodies && t.n_branches != 0);
+            if (split_target)
+                for (int64_t c = t.child_first; c < t.child_first + t.n_branches; ++c) run((int32_t)c, js);
+            else
+                for (int64_t c = s.child_first; c < s.child_first + s.n_branches; ++c) run(it, (int32_t)c);
+            return;
+        }
+        // SelfTuning (:66-120)
+        double fraction = 0.0;
+        fraction += (double)s.n_bodies / (double)(ls * ls);
+        fraction *= (double)t.n_bodies;
+        if (fraction < 1.0 || (s.n_branches == 0 && t.n_branches == 0)) {
+            if (nf && (it != js || si)) direct.push_back({it, js});
+            return;
+        }
+        double dx = t.target_center[0] - s.source_center[0];
+        double dy = t.target_center[1] - s.source_center[1];
+        double dz = t.target_center[2] - s.source_center[2];
+        double sep2 = dx * dx + dy * dy + dz * dz;
+        double sr = s.source_radius + t.target_radius;
+        if (sep2 * mac * mac > sr * sr) {
+            if (ff) m2l.push_back({it, js});
+            return;
+        }
+        if (s.n_branches == 0 || (t.n_bodies >= s.n_bodies && t.n_branches != 0))
+            for (int64_t c = t.child_first; c < t.child_first + t.n_branches; ++c) run((int32_t)c, js);
+        else
+            for (int64_t c = s.child_first; c < s.child_first + s.n_branches; ++c) run(it, (int32_t)c);
+    }
+};
+
+static std::vector<std::array<int32_t, 2>> sort_by_target(const std::vector<std::array<int32_t, 2>>& list, size_t nt) {  // :183-213
+    std::vector<int64_t> cnt(nt + 1, 0);
+    for (auto& e : list) cnt[e[0] + 1] += 1;
+    for (size_t i = 1; i <= nt; ++i) cnt[i] += cnt[i - 1];
+    std::vector<std::array<int32_t, 2>> out(list.size());
+    for (auto& e : list) out[cnt[e[0]]++] = e;
+    return out;
+}
+
+// ----------------------------------------------------------- user kernels
+// test/gravitational.jl:80-104 (ignores the derivative switches, as the reference kernel does)
+static void direct_gravitational(double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns) {
+    for (int64_t js = s0; js < s0 + ns; ++js) {
+        const double* s = sbuf + (size_t)js * sr;
+        double sx = s[0], sy = s[1], sz = s[2], q = s[4];
+        for (int64_t it = t0; it < t0 + nt; ++it) {
+            double* t = tbuf + (size_t)it * 16;
+            double dx = t[0] - sx, dy = t[1] - sy, dz = t[2] - sz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 0) {
+                double r = std::sqrt(r2);
+                double dphi = q / r * ONE_OVER_4PI;
+                t[3] += dphi;
+                double r3 = r2 * r;
+                t[4] += dx * q / r3 * ONE_OVER_4PI;
+                t[5] += dy * q / r3 * ONE_OVER_4PI;
+                t[6] += dz * q / r3 * ONE_OVER_4PI;
+            }
+        }
+    }
+}
+
+// test/vortex.jl:83-128
+static void direct_vortex(double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns, bool V, bool VG) {
+    for (int64_t js = s0; js < s0 + ns; ++js) {
+        const double* s = sbuf + (size_t)js * sr;
+        double sx = s[0], sy = s[1], sz = s[2], Gx = s[4], Gy = s[5], Gz = s[6];
+        for (int64_t it = t0; it < t0 + nt; ++it) {
+            double* t = tbuf + (size_t)it * 16;
+            double dx = t[0] - sx, dy = t[1] - sy, dz = t[2] - sz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 0) {
+                double r = std::sqrt(r2);
+                double rinv = 1 / r;
+                double rinv2 = rinv * rinv;
+                double denom = ONE_OVER_4PI * rinv * rinv2;
+                if (V) {
+                    t[4] += (dz * Gy - dy * Gz) * denom;
+                    t[5] += (dx * Gz - dz * Gx) * denom;
+                    t[6] += (dy * Gx - dx * Gy) * denom;
+                }
+                if (VG) {
+                    denom *= rinv2;
+                    double h[9];
+                    h[0] = -3 * dx * (Gy * dz - Gz * dy) * denom;
+                    h[1] = (-3 * dx * (Gz * dx - Gx * dz) + Gz * r2) * denom;
+                    h[2] = (-3 * dx * (Gx * dy - Gy * dx) - Gy * r2) * denom;
+                    h[3] = (-3 * dy * (Gy * dz - Gz * dy) - Gz * r2) * denom;
+                    h[4] = -3 * dy * (Gz * dx - Gx * dz) * denom;
+                    h[5] = (-3 * dy * (Gx * dy - Gy * dx) + Gx * r2) * denom;
+                    h[6] = (-3 * dz * (Gy * dz - Gz * dy) + Gy * r2) * denom;
+                    h[7] = (-3 * dz * (Gz * dx - Gx * dz) - Gx * r2) * denom;
+                    h[8] = -3 * dz * (Gx * dy - Gy * dx) * denom;
+                    for (int k = 0; k < 9; ++k) t[7 + k] += h[k];
+                }
+            }
+        }
+    }
+}
+
+static void user_direct(int kind, double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
+                        bool PS, bool GS, bool HS) {
+    (void)PS;
+    if (kind == KIND_SOURCE) direct_gravitational(tbuf, t0, nt, sbuf, sr, s0, ns);
+    else direct_vortex(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
+}
+
+static int data_per_body(int kind) { return kind == KIND_SOURCE ? 5 : 7; }
+
+// ------------------------------------------------------------ fmm driver
+struct Options {
+    int p = 5, ls_source = 20, ls_target = 20, method = METHOD_SELFTUNING_TREESTOP;
+    bool shrink = true;
+    ErrorMethod em;
+    bool PS = false, GS = true, HS = false;
+    bool ff = true, nf = true, si = true;
+    double theta = 0.4;
+};
+
+struct Run {
+    int kind;
+    int64_t nt, ns;
+    std::vector<double> tfull, sfull;   // original order: 16 x nt, dpb x ns
+    Options opt;
+    OTree tt, st;
+    std::vector<std::array<int32_t, 2>> m2l, direct;
+    int Pmax = 0;
+    bool error_success = true;
+    std::vector<int32_t> m2l_P;   // expansion order chosen per M2L interaction
+};
+
+static BranchGeom geom(const OBranch& b) {
+    BranchGeom g;
+    for (int d = 0; d < 3; ++d) {
+        g.source_center[d] = b.source_center[d]; g.target_center[d] = b.target_center[d];
+        g.source_box[d] = b.source_box[d]; g.target_box[d] = b.target_box[d];
+    }
+    g.source_radius = b.source_radius; g.target_radius = b.target_radius; g.max_influence = b.max_influence;
+    return g;
+}
+
+static void fmm_run(Run& R, bool estimate_pass);
+
+// estimate_influence! (src/fmm.jl:1193-1276)
+static void estimate_influence(Run& R) {
+    Run E;
+    E.kind = R.kind; E.nt = R.nt; E.ns = R.ns; E.tfull = R.tfull; E.sfull = R.sfull;
+    E.opt = R.opt;
+    E.opt.PS = true; E.opt.GS = true; E.opt.HS = false;
+    E.opt.ls_source = 5; E.opt.ls_target = 5;
+    E.opt.p = 3; E.opt.theta = 0.6;
+    E.opt.em = ErrorMethod();
+    E.opt.shrink = true;
+    E.opt.method = METHOD_SELFTUNING_TREESTOP;
+    E.opt.ff = true; E.opt.nf = true; E.opt.si = true;
+    fmm_run(E, true);
+    // map estimate buffer (sorted by the estimate tree) onto the main target tree buffer
+    std::vector<double> infl(R.nt, 0.0);
+    for (int64_t je = 0; je < E.nt; ++je) {
+        int64_t jsys = E.tt.sort_index[je];
+        int64_t jb = R.tt.inverse_sort_index[jsys];
+        const double* e = &E.tt.buffer[(size_t)je * 16];
+        double pot = e[3], vx = e[4], vy = e[5], vz = e[6];
+        (void)pot;
+        if (R.opt.em.kind == 2) infl[jb] = pot;   // PowerRelativePotential reads the scalar potential
+        else infl[jb] = std::sqrt(vx * vx + vy * vy + vz * vz);
+    }
+    for (auto& b : R.tt.branches) {
+        double mi = b.max_influence;
+        for (int64_t j = b.first; j < b.first + b.n_bodies; ++j) mi = std::max(mi, infl[j]);
+        b.max_influence = mi;
+    }
+}
+
+static void fmm_run(Run& R, bool estimate_pass) {
+    const Options& o = R.opt;
+    const bool LH = (R.kind == KIND_VORTEX);   // has_vector_potential
+    const int dpb = data_per_body(R.kind);
+    build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p);
+    build_tree(R.st, R.ns, dpb, R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p);
+    R.m2l.clear(); R.direct.clear();
+    if (!R.tt.branches.empty() && !R.st.branches.empty()) {
+        ListBuilder L{R.tt, R.st, o.ls_source, o.theta, o.ff, o.nf, o.si, o.method, {}, {}};
+        L.run(0, 0);
+        R.m2l = sort_by_target(L.m2l, R.tt.branches.size());
+        R.direct = sort_by_target(L.direct, R.tt.branches.size());
+    }
+    if (!estimate_pass && o.em.kind != 0 && o.em.relative) estimate_influence(R);
+    R.Pmax = 0;
+    R.error_success = true;
+    if (R.nt == 0 || R.ns == 0) return;
+    tables().ensure(o.p);
+    int p = o.p;
+    int Hn = n_harmonics(p);
+    // nearfield (src/fmm.jl:3-40)
+    for (auto& pr : R.direct) {
+        const OBranch& s = R.st.branches[pr[1]];
+        const OBranch& t = R.tt.branches[pr[0]];
+        user_direct(R.kind, R.tt.buffer.data(), t.first, t.n_bodies, R.st.buffer.data(), dpb, s.first, s.n_bodies, o.PS, o.GS, o.HS);
+    }
+    Scratch S;
+    S.ensure(p);
+    std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
+    // upward pass: P2M (src/bodytomultipole.jl:19-26) then M2M (src/fmm.jl:262-284)
+    for (int64_t il : R.st.leaf_index) {
+        const OBranch& b = R.st.branches[il];
+        Exp M{&R.st.expansions[(size_t)il * 4 * Hn]};
+        for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+            const double* x = &R.st.buffer[(size_t)i * dpb];
+            double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
+            if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
+            else p2m_point_vortex(M, Exp{harm.data()}, dx, x + 4, p);
+        }
+    }
+    for (int64_t ib = (int64_t)R.st.branches.size() - 1; ib >= 0; --ib) {
+        const OBranch& b = R.st.branches[ib];
+        if (b.n_branches != 0)
+            for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
+                multipole_to_multipole(Exp{&R.st.expansions[(size_t)ib * 4 * Hn]}, b.source_center,
+                                       Exp{&R.st.expansions[(size_t)c * 4 * Hn]}, R.st.branches[c].source_center, S, p, LH);
+    }
+    // horizontal pass (src/fmm.jl:455-490)
+    R.m2l_P.assign(R.m2l.size(), 0);
+    for (size_t k = 0; k < R.m2l.size(); ++k) {
+        int32_t it = R.m2l[k][0], js = R.m2l[k][1];
+        bool ok = true;
+        int P = multipole_to_local(Exp{&R.tt.expansions[(size_t)it * 4 * Hn]}, geom(R.tt.branches[it]),
+                                   Exp{&R.st.expansions[(size_t)js * 4 * Hn]}, geom(R.st.branches[js]), S, p, LH, o.em, ok);
+        R.m2l_P[k] = P;
+        R.Pmax = std::max(P, R.Pmax);
+        R.error_success = R.error_success && ok;
+    }
+    if (!R.error_success) R.Pmax += 1;
+    // downward pass: L2L (src/fmm.jl:602-622) then L2P (src/evaluate_expansions.jl:1-43)
+    for (size_t ib = 0; ib < R.tt.branches.size(); ++ib) {
+        const OBranch& b = R.tt.branches[ib];
+        if (b.n_branches > 0)
+            for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
+                local_to_local(Exp{&R.tt.expansions[(size_t)c * 4 * Hn]}, R.tt.branches[c].target_center,
+                               Exp{&R.tt.expansions[ib * 4 * Hn]}, b.target_center, S, p, LH);
+    }
+    std::vector<double> gnm;
+    for (int64_t il : R.tt.leaf_index) {
+        const OBranch& b = R.tt.branches[il];
+        Exp L{&R.tt.expansions[(size_t)il * 4 * Hn]};
+        for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+            double* t = &R.tt.buffer[(size_t)i * 16];
+            double dx[3] = {t[0] - b.target_center[0], t[1] - b.target_center[1], t[2] - b.target_center[2]};
+            double phi, v[3], h[9];
+            evaluate_local(dx, Exp{harm.data()}, gnm, L, p, LH, o.PS, o.GS, o.HS, phi, v, h);
+            if (o.PS) t[3] += phi;
+            if (o.GS) { t[4] += v[0]; t[5] += v[1]; t[6] += v[2]; }
+            if (o.HS) for (int k = 0; k < 9; ++k) t[7 + k] += h[k];
+        }
+    }
+}
+
+}  // namespace orc
+
+// ======================================================================
+// Flat C API used by tests/ (ctypes) and bench.py's cpu_baseline leg.
+// ======================================================================
+using namespace orc;
+
+extern "C" {
+
+// opts_i: [p, ls_source, ls_target, method, shrink, err_kind, err_relative, err_BE,
+//          PS, GS, HS, farfield, nearfield, self_induced]; opts_d: [theta, eps]
+// tfull: 16 x nt (positions in rows 1:3), sfull: data_per_body x ns, both original order.
+void* orc_run_create(int kind, int64_t nt, const double* tfull, int64_t ns, const double* sfull, const int* oi, const double* od) {
+    Run* R = new Run();
+    R->kind = kind; R->nt = nt; R->ns = ns;
+    R->tfull.assign(tfull, tfull + (size_t)16 * nt);
+    R->sfull.assign(sfull, sfull + (size_t)data_per_body(kind) * ns);
+    Options& o = R->opt;
+    o.p = oi[0]; o.ls_source = oi[1]; o.ls_target = oi[2]; o.method = oi[3]; o.shrink = oi[4] != 0;
+    o.em.kind = oi[5]; o.em.relative = oi[6] != 0; o.em.BE = oi[7] != 0;
+    o.PS = oi[8] != 0; o.GS = oi[9] != 0; o.HS = oi[10] != 0;
+    o.ff = oi[11] != 0; o.nf = oi[12] != 0; o.si = oi[13] != 0;
+    o.theta = od[0]; o.em.eps = od[1];
+    return R;
+}
+
+void orc_run_free(void* h) { delete (Run*)h; }
+void orc_run_fmm(void* h) { fmm_run(*(Run*)h, false); }
+
+// tree only (no lists, no evaluation): which = 0 target tree, 1 source tree
+void orc_run_trees(void* h) {
+    Run& R = *(Run*)h;
+    build_tree(R.tt, R.nt, 16, R.tfull.data(), true, R.opt.ls_target, R.opt.method, R.opt.shrink, R.opt.p);
+    build_tree(R.st, R.ns, data_per_body(R.kind), R.sfull.data(), false, R.opt.ls_source, R.opt.method, R.opt.shrink, R.opt.p);
+}
+
+void orc_run_lists(void* h) {
+    Run& R = *(Run*)h;
+    R.m2l.clear(); R.direct.clear();
+    if (R.tt.branches.empty() || R.st.branches.empty()) return;
+    ListBuilder L{R.tt, R.st, R.opt.ls_source, R.opt.theta, R.opt.ff, R.opt.nf, R.opt.si, R.opt.method, {}, {}};
+    L.run(0, 0);
+    R.m2l = sort_by_target(L.m2l, R.tt.branches.size());
+    R.direct = sort_by_target(L.direct, R.tt.branches.size());
+}
+
+static OTree& tree_of(Run& R, int which) { return which == 0 ? R.tt : R.st; }
+
+int64_t orc_n_branches(void* h, int which) { return (int64_t)tree_of(*(Run*)h, which).branches.size(); }
+int64_t orc_n_levels(void* h, int which) { return (int64_t)tree_of(*(Run*)h, which).levels.size(); }
+int64_t orc_n_leaves(void* h, int which) { return (int64_t)tree_of(*(Run*)h, which).leaf_index.size(); }
+
+// ints: nb x 6 [n_bodies, first, n_branches, child_first, i_parent, i_leaf]
+// reals: nb x 17 [source_center(3), target_center(3), source_radius, target_radius,
+//                 source_box(3), target_box(3), max_influence, pad, pad]
+void orc_get_branches(void* h, int which, int64_t* ints, double* reals) {
+    OTree& T = tree_of(*(Run*)h, which);
+    for (size_t i = 0; i < T.branches.size(); ++i) {
+        const OBranch& b = T.branches[i];
+        int64_t* I = ints + 6 * i;
+        I[0] = b.n_bodies; I[1] = b.first; I[2] = b.n_branches; I[3] = b.child_first; I[4] = b.i_parent; I[5] = b.i_leaf;
+        double* D = reals + 17 * i;
+        for (int d = 0; d < 3; ++d) {
+            D[d] = b.source_center[d]; D[3 + d] = b.target_center[d];
+            D[8 + d] = b.source_box[d]; D[11 + d] = b.target_box[d];
+        }
+        D[6] = b.source_radius; D[7] = b.target_radius; D[14] = b.max_influence; D[15] = 0; D[16] = 0;
+    }
+}
+
+void orc_get_levels(void* h, int which, int64_t* out) {
+    OTree& T = tree_of(*(Run*)h, which);
+    for (size_t i = 0; i < T.levels.size(); ++i) { out[2 * i] = T.levels[i][0]; out[2 * i + 1] = T.levels[i][1]; }
+}
+void orc_get_leaf_index(void* h, int which, int64_t* out) {
+    OTree& T = tree_of(*(Run*)h, which);
+    std::memcpy(out, T.leaf_index.data(), T.leaf_index.size() * sizeof(int64_t));
+}
+void orc_get_sort_index(void* h, int which, int64_t* out) {
+    OTree& T = tree_of(*(Run*)h, which);
+    std::memcpy(out, T.sort_index.data(), T.sort_index.size() * sizeof(int64_t));
+}
+// sorted buffer of a tree (rows x N)
+void orc_get_buffer(void* h, int which, double* out) {
+    OTree& T = tree_of(*(Run*)h, which);
+    std::memcpy(out, T.buffer.data(), T.buffer.size() * sizeof(double));
+}
+void orc_get_expansions(void* h, int which, double* out) {
+    OTree& T = tree_of(*(Run*)h, which);
+    std::memcpy(out, T.expansions.data(), T.expansions.size() * sizeof(double));
+}
+int64_t orc_n_list(void* h, int which) { Run& R = *(Run*)h; return (int64_t)(which == 0 ? R.m2l.size() : R.direct.size()); }
+void orc_get_list(void* h, int which, int32_t* out) {
+    Run& R = *(Run*)h;
+    auto& L = which == 0 ? R.m2l : R.direct;
+    for (size_t i = 0; i < L.size(); ++i) { out[2 * i] = L[i][0]; out[2 * i + 1] = L[i][1]; }
+}
+void orc_get_m2l_orders(void* h, int32_t* out) {
+    Run& R = *(Run*)h;
+    std::memcpy(out, R.m2l_P.data(), R.m2l_P.size() * sizeof(int32_t));
+}
+// target results in the ORIGINAL body order (16 x nt)
+void orc_get_targets(void* h, double* out) {
+    Run& R = *(Run*)h;
+    for (int64_t i = 0; i < R.nt; ++i)
+        std::memcpy(out + (size_t)R.tt.sort_index[i] * 16, &R.tt.buffer[(size_t)i * 16], 16 * sizeof(double));
+}
+int orc_pmax(void* h) { return ((Run*)h)->Pmax; }
+int orc_error_success(void* h) { return ((Run*)h)->error_success ? 1 : 0; }
+
+// direct! over all pairs (src/direct.jl:43-75); tbuf 16 x nt (positions set, rest accumulates)
+void orc_direct(int kind, int64_t nt, double* tbuf, int64_t ns, const double* sbuf, int PS, int GS, int HS) {
+    user_direct(kind, tbuf, 0, nt, sbuf, data_per_body(kind), 0, ns, PS != 0, GS != 0, HS != 0);
+}
+// direct over a subset of targets (index list) against all sources: for sampled-error checks
+void orc_direct_sampled(int kind, int64_t nsamp, const int64_t* idx, const double* txyz, double* out16,
+                        int64_t ns, const double* sbuf, int PS, int GS, int HS) {
+    for (int64_t k = 0; k < nsamp; ++k) {
+        double t[16] = {0};
+        for (int d = 0; d < 3; ++d) t[d] = txyz[3 * idx[k] + d];
+        user_direct(kind, t, 0, 1, sbuf, data_per_body(kind), 0, ns, PS != 0, GS != 0, HS != 0);
+        std::memcpy(out16 + 16 * k, t, sizeof(t));
+    }
+}
+
+// ---------------------------------------------- unit-level operator entry points
+void orc_tables(int p, double* Hs, double* zetas, double* etas, double* Mt, double* Lt) {
+    Tables& T = tables();
+    T.ensure(p);
+    if (Hs) std::memcpy(Hs, T.Hs_pi2.data(), length_Hs(p) * sizeof(double));
+    if (zetas) std::memcpy(zetas, T.zetas.data(), length_Ts(p) * sizeof(double));
+    if (etas) std::memcpy(etas, T.etas.data(), length_Ts(p) * sizeof(double));
+    if (Mt) std::memcpy(Mt, T.Mt.data(), n_harmonics(p) * sizeof(double));
+    if (Lt) std::memcpy(Lt, T.Lt.data(), n_harmonics(p) * sizeof(double));
+}
+void orc_hs_fresh(int p, double* out) {   // update_Hs_pi2! starting from [1.0]
+    std::vector<double> Hs(length_Hs(p), 0.0);
+    Hs[0] = 1.0;
+    if (p >= 1) compute_Hs(Hs, PI / 2, 1, p);
+    std::memcpy(out, Hs.data(), Hs.size() * sizeof(double));
+}
+void orc_pnms(int p, double s, double c, double* out) { auto v = compute_Pnms(p, s, c); std::memcpy(out, v.data(), v.size() * sizeof(double)); }
+void orc_update_Ts(double beta, int p, double* out) {
+    tables().ensure(p);
+    update_Ts(out, tables().Hs_pi2.data(), beta, p);
+}
+void orc_eimphis(double phi, int P, double* out) { update_eimphis(out, phi, P); }
+void orc_zeta_sign(double mag, int mp, int m, double* out) { zeta_sign(mag, mp, m, out[0], out[1]); }
+void orc_eta_sign(double mag, int mp, int m, double* out) { eta_sign(mag, mp, m, out[0], out[1]); }
+void orc_regular_harmonics(double rho, double th, double ph, int p, double* out) { regular_harmonics(Exp{out}, rho, th, ph, p); }
+void orc_irregular_harmonics(double rho, double th, double ph, int p, double* out) { irregular_harmonics(Exp{out}, rho, th, ph, p); }
+void orc_c2s(double x, double y, double z, double* out) { cartesian_to_spherical(x, y, z, out[0], out[1], out[2]); }
+double orc_factorial(int k) { return factorial_to_double(k); }
+
+void orc_rotate_multipole_y(const double* in, double theta, int p, int LH, double* out) {
+    tables().ensure(p);
+    std::vector<double> Ts(length_Ts(p)), tmp(in, in + 4 * n_harmonics(p));
+    update_Ts(Ts.data(), tables().Hs_pi2.data(), theta, p);
+    rotate_multipole_y_full(Exp{out}, Exp{tmp.data()}, Ts.data(), tables().zetas.data(), p, LH != 0);
+}
+void orc_rotate_local_y(const double* in, double theta, int p, int LH, double* out) {
+    tables().ensure(p);
+    std::vector<double> Ts(length_Ts(p)), tmp(in, in + 4 * n_harmonics(p));
+    update_Ts(Ts.data(), tables().Hs_pi2.data(), theta, p);
+    rotate_local_y_full(Exp{out}, Exp{tmp.data()}, Ts.data(), tables().etas.data(), p, LH != 0);
+}
+void orc_rotate_z(const double* in, double phi, int p, int LH, double* out) {
+    std::vector<double> e(2 * (p + 1)), tmp(in, in + 4 * n_harmonics(p));
+    rotate_z(Exp{out}, Exp{tmp.data()}, e.data(), phi, p, LH != 0);
+}
+void orc_translate_multipole_z(const double* in, double t, int p, int LH, double* out) {
+    std::vector<double> tmp(in, in + 4 * n_harmonics(p));
+    translate_multipole_z(Exp{out}, Exp{tmp.data()}, t, p, LH != 0);
+}
+void orc_translate_m2l_z(const double* in, double t, int p, int LH, double* out) {
+    std::vector<double> tmp(in, in + 4 * n_harmonics(p));
+    translate_multipole_to_local_z(Exp{out}, Exp{tmp.data()}, t, p, LH != 0);
+}
+void orc_translate_local_z(const double* in, double t, int p, int LH, double* out) {
+    std::vector<double> tmp(in, in + 4 * n_harmonics(p));
+    translate_local_z(Exp{out}, Exp{tmp.data()}, t, p, LH != 0);
+}
+void orc_transform_lh_multipole(double* e, double r, int p) { transform_lh_multipole(Exp{e}, r, p); }
+void orc_transform_lh_local(double* e, double r, int p) { transform_lh_local(Exp{e}, r, p); }
+void orc_m2m(const double* tc, const double* sc, const double* in, int p, int LH, double* out) {
+    Scratch s;
+    std::vector<double> tmp(in, in + 4 * n_harmonics(p));
+    multipole_to_multipole(Exp{out}, tc, Exp{tmp.data()}, sc, s, p, LH != 0);
+}
+void orc_l2l(const double* tc, const double* sc, const double* in, int p, int LH, double* out) {
+    Scratch s;
+    std::vector<double> tmp(in, in + 4 * n_harmonics(p));
+    local_to_local(Exp{out}, tc, Exp{tmp.data()}, sc, s, p, LH != 0);
+}
+// tgeom/sgeom: [source_center(3), target_center(3), source_radius, target_radius, source_box(3), target_box(3), max_influence]
+int orc_m2l(const double* tgeom, const double* sgeom, const double* in, int p, int LH, int err_kind, int err_rel,
+            double eps, int BE, double* out, int* success) {
+    auto G = [](const double* g) {
+        BranchGeom b;
+        for (int d = 0; d < 3; ++d) { b.source_center[d] = g[d]; b.target_center[d] = g[3 + d]; b.source_box[d] = g[8 + d]; b.target_box[d] = g[11 + d]; }
+        b.source_radius = g[6]; b.target_radius = g[7]; b.max_influence = g[14];
+        return b;
+    };
+    ErrorMethod em;
+    em.kind = err_kind; em.relative = err_rel != 0; em.eps = eps; em.BE = BE != 0;
+    Scratch s;
+    std::vector<double> tmp(in, in + 4 * n_harmonics(p));
+    bool ok = true;
+    int P = multipole_to_local(Exp{out}, G(tgeom), Exp{tmp.data()}, G(sgeom), s, p, LH != 0, em, ok);
+    *success = ok ? 1 : 0;
+    return P;
+}
+void orc_l2p(const double* dx, const double* local, int p, int LH, int PS, int GS, int HS, double* out13) {
+    std::vector<double> harm(4 * n_harmonics(p + 2), 0.0), gnm;
+    std::vector<double> tmp(local, local + 4 * n_harmonics(p));
+    double phi, v[3], h[9];
+    evaluate_local(dx, Exp{harm.data()}, gnm, Exp{tmp.data()}, p, LH != 0, PS != 0, GS != 0, HS != 0, phi, v, h);
+    out13[0] = phi;
+    for (int k = 0; k < 3; ++k) out13[1 + k] = v[k];
+    for (int k = 0; k < 9; ++k) out13[4 + k] = h[k];
+}
+// kind 0 source (strength[0]), 1 vortex (strength[0..2]), 2 dipole (strength[0..2]); accumulates into out
+void orc_p2m(int kind, const double* dx, const double* strength, int p, double* out) {
+    std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
+    if (kind == 0) p2m_point_source(Exp{out}, Exp{harm.data()}, dx, strength[0], p);
+    else if (kind == 1) p2m_point_vortex(Exp{out}, Exp{harm.data()}, dx, strength, p);
+    else p2m_point_dipole(Exp{out}, Exp{harm.data()}, dx, strength, p);
+}
+
+}  // extern "C"

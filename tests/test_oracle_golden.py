@@ -1,0 +1,545 @@
+"""Pins the CPU oracle against FastMultipole's own golden vectors and closed forms.
+
+Which check pins which restated function (reference file:line in brackets):
+
+* compute_Pnms [src/rotate.jl:95-117]      -> scipy associated Legendre (Condon-Shortley)
+* Hs_pi2 [rotate.jl:119-223]               -> H_pi2 fixture n<=4 (test/rotate_test.jl:28-51)
+* update_Ts [rotate.jl:260-346]            -> T(3pi/7) fixture (test/rotate_test.jl:80-104)
+* update_eimphis [rotate.jl:25-36]         -> exp(i m phi)
+* zeta/eta magnitudes + signs [:625-1004]  -> factorial closed forms (test/rotate_test.jl:141-169, 607-635)
+* multipole/local y and z rotations        -> fixtures (test/rotate_test.jl:173-815) + back-rotation identities
+* regular/irregular harmonics [src/harmonics.jl] -> closed forms (test/harmonics_test.jl)
+* M~/L~ [src/translate.jl:366-448]         -> factorial closed forms (test/multipole_power_test.jl)
+* P2M point source [src/bodytomultipole.jl:403-429] -> (-1)^(n+m) q conj(R_n^m) closed form
+* P2M point dipole [:431-463]              -> fixture (test/bodytomultipole_test.jl:464)
+* P2M point vortex [:465-489, :283-334]    -> fixture (test/bodytomultipole_test.jl:524-525) and Biot-Savart
+* z translations M2M/M2L/L2L               -> fixtures (translate_*_test.jl)
+* M2M/M2L/L2L point-and-shoot              -> fixtures
+* Lamb-Helmholtz transforms, LH M2M/M2L/L2L -> fixtures (test/lamb_helmholtz_test.jl)
+* L2P [src/evaluate_expansions.jl]         -> analytic 1/r, gradient, Hessian; vorton vs Biot-Savart
+* tree [src/tree.jl]                       -> 5-body known answer (test/tree_test.jl:1-131) + invariants
+* lists [src/interaction_list.jl]          -> sortedness + coverage (every pair of leaves counted once)
+* full FMM                                 -> O(N^2) direct at p=24 (test/fmm_test.jl:70-109)
+* error-controlled M2L [translate.jl:555-1412] -> error bands eps*0.1 < max err < eps*{10,30,100}
+  (test/dynamic_expansion_order_test.jl), on this repository's own seeded inputs
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.special import lpmv
+
+from oracle import oracle as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+
+
+def cfull(d):
+    return np.array(d["re"]) + 1j * np.array(d["im"])
+
+
+def compressed(d, p):
+    return O.full_to_compressed(cfull(d), p)
+
+
+def R_closed(n, m, rho, th, ph):
+    # Gumerov normalisation with the Condon-Shortley phase (src/harmonics.jl:9-12)
+    return (-1) ** n * (1j) ** abs(m) * rho ** n * lpmv(abs(m), n, math.cos(th)) * np.exp(1j * m * ph) / math.factorial(n + abs(m))
+
+
+def S_closed(n, m, rho, th, ph):
+    return (-1) ** m * (1j) ** abs(m) * rho ** (-n - 1) * lpmv(abs(m), n, math.cos(th)) * np.exp(1j * m * ph) * math.factorial(n - abs(m))
+
+
+# ------------------------------------------------------------------ tables
+def test_legendre_pnms():
+    P, th = 10, math.pi / 7
+    v = O.pnms(P, math.sin(th), math.cos(th))
+    for n in range(P + 1):
+        for m in range(n + 1):
+            assert v[n * (n + 1) // 2 + m] == pytest.approx(lpmv(m, n, math.cos(th)), rel=1e-12, abs=1e-14)
+
+
+def test_H_pi2_fixture():
+    Hs, *_ = O.tables(10)
+    g = GOLD["rotate_test.jl:H_pi2"]
+    off = 1
+    for n in range(1, 5):
+        H = np.array(g[f"H{n}"])
+        for m in range(n + 1):
+            for mp in range(m + 1):
+                assert Hs[off + m * (m + 1) // 2 + mp] == pytest.approx(H[n + mp, n + m], abs=1e-6)
+        off += (n + 1) * (n + 2) // 2
+
+
+def test_T_fixture():
+    g = GOLD["rotate_test.jl:T_3pi7"]
+    Ts = O.update_Ts(3 * math.pi / 7, 5)
+    off = 1
+    for n in range(1, 5):
+        T = np.array(g[f"T{n}"])
+        for m in range(n + 1):
+            for mp in range(-m, m + 1):
+                assert Ts[off + m * m + mp + m] == pytest.approx(T[n + mp, n + m], abs=1e-6)
+        off += (n + 1) ** 2
+
+
+def test_eimphi():
+    e = O.eimphis(math.pi / 7, 10)
+    for m in range(11):
+        z = np.exp(1j * m * math.pi / 7)
+        assert e[m, 0] == pytest.approx(z.real, abs=1e-12) and e[m, 1] == pytest.approx(z.imag, abs=1e-12)
+
+
+def test_zeta_eta_normalisation():
+    _, z, e, _, _ = O.tables(4)
+    f = math.factorial
+    for m in range(5):
+        for mp in range(5):
+            zs = O.zeta_sign(1.0, mp, m)
+            c = (1j) ** (1.0 * mp - m)
+            assert zs[0] == pytest.approx(c.real, abs=1e-12) and zs[1] == pytest.approx(c.imag, abs=1e-12)
+            es = O.eta_sign(1.0, mp, m)
+            c = (1j) ** (1.0 * m - mp)
+            assert es[0] == pytest.approx(c.real, abs=1e-12) and es[1] == pytest.approx(c.imag, abs=1e-12)
+    for n in range(5):
+        off = O.length_Ts(n - 1) if n > 0 else 0
+        i = 0
+        for m in range(n + 1):
+            for mp in range(n + 1):
+                zc = (1j) ** (1.0 * (mp - m)) * math.sqrt(f(n - mp) * f(n + mp) / f(n - m) / f(n + m))
+                got = O.zeta_sign(z[off + i], mp, m)
+                assert got[0] == pytest.approx(zc.real, abs=1e-12) and got[1] == pytest.approx(zc.imag, abs=1e-12)
+                ec = (1j) ** (1.0 * (m - mp)) * math.sqrt(f(n - m) * f(n + m) / f(n - mp) / f(n + mp))
+                got = O.eta_sign(e[off + i], mp, m)
+                assert got[0] == pytest.approx(ec.real, abs=1e-12) and got[1] == pytest.approx(ec.imag, abs=1e-12)
+                i += 1
+
+
+def test_multipole_local_powers():
+    _, _, _, M, L = O.tables(5)
+    f = math.factorial
+    i = 0
+    for n in range(6):
+        for m in range(n + 1):
+            assert M[i] == pytest.approx(math.sqrt(f(n + m) * f(n - m) / (2 * n + 1)), abs=1e-12)
+            assert L[i] == pytest.approx(math.sqrt(1 / (f(n + m) * f(n - m) * (2 * n + 1))), abs=1e-12)
+            i += 1
+    # correctly rounded big factorials (Float64(factorial(big(k))))
+    for k in (20, 25, 30, 40, 42):
+        assert O.factorial(k) == float(math.factorial(k))
+
+
+# --------------------------------------------------------------- rotations
+def test_multipole_y_rotation_fixture():
+    g = GOLD["rotate_test.jl:multipole_y"]
+    p = g["p"]
+    c = np.array(g["in_re"]) + 1j * np.array(g["in_im"])
+    e = O.complex_to_exp(c, p)
+    out = O.exp_to_complex(O.rotate_multipole_y(e, g["theta"], p), p)
+    exp = compressed(g["out_full"], p)
+    np.testing.assert_allclose(out.real, exp.real, atol=3e-12)
+    np.testing.assert_allclose(out.imag, exp.imag, atol=3e-12)
+    # back rotation (same T) is the identity (test/rotate_test.jl:347-515)
+    back = O.exp_to_complex(O.rotate_multipole_y(O.rotate_multipole_y(e, g["theta"], p), g["theta"], p), p)
+    np.testing.assert_allclose(back, c, atol=1e-12)
+
+
+def test_multipole_z_rotation_fixture():
+    g = GOLD["rotate_test.jl:multipole_z"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    out = O.exp_to_complex(O.rotate_z(e, g["phi"], p), p)
+    exp = compressed(g["out_full"], p)
+    np.testing.assert_allclose(out, exp, atol=1e-12)
+
+
+def test_local_y_rotation_fixture():
+    g = GOLD["rotate_test.jl:local_y"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    out = O.rotate_local_y(e, g["theta"], p)
+    np.testing.assert_allclose(O.exp_to_complex(out, p), compressed(g["out_full"], p), atol=1e-12)
+    back = O.rotate_local_y(out, g["theta"], p)   # back_rotate_local_y! reuses T
+    np.testing.assert_allclose(O.exp_to_complex(back, p), compressed(g["in_full"], p), atol=1e-12)
+
+
+def test_local_z_rotation_fixture():
+    g = GOLD["rotate_test.jl:local_z"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    out = O.exp_to_complex(O.rotate_z(e, g["phi"], p), p)
+    np.testing.assert_allclose(out, compressed(g["out_full"], p), atol=1e-12)
+
+
+# --------------------------------------------------------------- harmonics
+def test_regular_irregular_harmonics():
+    p, rho, th, ph = 7, 1.7, 3 * math.pi / 8, 5 * math.pi / 3
+    R = O.regular_harmonics(rho, th, ph, p)
+    S = O.irregular_harmonics(rho, th, ph, p)
+    for n in range(p + 1):
+        for m in range(n + 1):
+            i = n * (n + 1) // 2 + m
+            assert abs(R[i] - R_closed(n, m, rho, th, ph)) < 1e-12
+            assert abs(S[i] - S_closed(n, m, rho, th, ph)) < 1e-12
+
+
+def test_cartesian_to_spherical():
+    r, t, f = O.cartesian_to_spherical(math.sin(math.pi / 4) * math.cos(math.pi / 2), math.sin(math.pi / 4) * math.sin(math.pi / 2),
+                                       math.cos(math.pi / 4))
+    assert (r, t, f) == pytest.approx((1.0, math.pi / 4, math.pi / 2), abs=1e-10)
+    assert O.cartesian_to_spherical(0.0, 0.0, -2.0) == pytest.approx((2.0, math.pi, 0.0))
+    assert O.cartesian_to_spherical(0.0, 0.0, 0.0) == (0.0, 0.0, 0.0)
+
+
+# -------------------------------------------------------------------- P2M
+def test_p2m_point_source_closed_form():
+    x = np.array([0.1, 0.2, -0.3])
+    xs = x + np.array([-0.2, 0.07, -0.1])
+    p = 10
+    M = O.exp_to_complex(O.p2m(O.KIND_SOURCE, xs - x, [0.7], p), p)
+    rho, th, ph = O.cartesian_to_spherical(*(xs - x))
+    for n in range(p + 1):
+        for m in range(n + 1):
+            chk = (-1) ** (n + m) * 0.7 * np.conj(R_closed(n, m, rho, th, ph))
+            assert abs(M[n * (n + 1) // 2 + m] + chk) < 1e-12
+
+
+def test_p2m_point_dipole_fixture():
+    g = GOLD["bodytomultipole_test.jl:point_dipole"]
+    p = g["p"]
+    M = O.exp_to_complex(O.p2m(2, np.array(g["x_source"]) - np.array(g["center"]), g["q"], p), p)
+    np.testing.assert_allclose(M, -compressed(g["multipole_full"], p), atol=1e-12)
+
+
+def test_p2m_point_vortex_fixture():
+    # these weights are the inputs of the vorton M2L test; they are the P2M of a unit z-vortex at the origin
+    # expanded about (0.01, 0.02, -0.03) (test/lamb_helmholtz_test.jl:271-291)
+    g = GOLD["bodytomultipole_test.jl:point_vortex"]
+    p = g["p"]
+    e = O.p2m(O.KIND_VORTEX, -np.array([0.01, 0.02, -0.03]), [0.0, 0.0, 1.0], p)
+    np.testing.assert_allclose(O.exp_to_complex(e, p, 0), compressed(g["w1"], p), atol=1e-12)
+    np.testing.assert_allclose(O.exp_to_complex(e, p, 1), compressed(g["w2"], p), atol=1e-12)
+
+
+# ------------------------------------------------------------- translations
+def _check_pair(out, g_out, p, n_checked=None, atol=1e-12, index=0):
+    got = O.exp_to_complex(out, p, index)
+    exp = compressed(g_out, p)
+    k = len(got) if n_checked is None else O.nh(n_checked)
+    np.testing.assert_allclose(got[:k].real, exp[:k].real, atol=atol, rtol=1e-12)
+    np.testing.assert_allclose(got[:k].imag, exp[:k].imag, atol=atol, rtol=1e-12)
+
+
+def test_m2m_z_fixture():
+    g = GOLD["translate_multipole_test.jl:z"]
+    p = g["p"]
+    e = np.zeros(4 * O.nh(p))
+    for i, s in enumerate(g["in_slices"]):   # slice [re_phi re_chi; im_phi im_chi]
+        e[4 * i + 0], e[4 * i + 2] = s[0][0], s[0][1]
+        e[4 * i + 1], e[4 * i + 3] = s[1][0], s[1][1]
+    _check_pair(O.translate_multipole_z(e, g["r"], p), g["out_full"], p)
+
+
+def test_m2m_point_and_shoot_fixture():
+    g = GOLD["translate_multipole_test.jl:point_and_shoot"]
+    p = g["p"]
+    x = np.array(g["x"])
+    xs = x + np.array(g["dxs"])
+    M = O.p2m(O.KIND_SOURCE, xs - x, [g["strength"]], p)
+    out = O.m2m(x + np.array(g["target_offset"]), x, M, p)
+    exp = compressed(g["out_full"], p)   # the extractor already applied the literal's leading minus
+    np.testing.assert_allclose(O.exp_to_complex(out, p), exp, atol=1e-12)
+
+
+def test_m2l_z_fixture():
+    g = GOLD["translate_multipole_to_local_test.jl:z"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    _check_pair(O.translate_m2l_z(e, g["r"], p), g["out_full"], p)
+
+
+def test_m2l_point_and_shoot_fixture():
+    g = GOLD["translate_multipole_to_local_test.jl:point_and_shoot"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    out, P, ok = O.m2l(O.geom(g["target_center"]), O.geom(g["source_center"]), e, p)
+    assert P == p and ok
+    _check_pair(out, g["out_full"], p)
+
+
+def test_l2l_z_fixture():
+    g = GOLD["translate_local_test.jl:z"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    _check_pair(O.translate_local_z(e, g["r"], p), g["out_full"], p)
+
+
+def test_l2l_point_and_shoot_fixture():
+    g = GOLD["translate_local_test.jl:point_and_shoot"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["in_full"], p), p)
+    _check_pair(O.l2l(g["target_center"], g["source_center"], e, p), g["out_full"], p)
+
+
+def test_l2p_analytic():
+    g = GOLD["evaluate_expansions_test.jl:l2p"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["local_full"], p), p)
+    xt, xs, c = np.array(g["x_target"]), np.array(g["x_source"]), np.array(g["center"])
+    phi, v, h = O.l2p(xt - c, e, p)
+    dx = xt - xs
+    r = np.linalg.norm(dx)
+    mass = g["mass"]
+    assert phi == pytest.approx(mass / r / 4 / math.pi, abs=1e-12)
+    np.testing.assert_allclose(v, dx / r ** 3 * mass / 4 / math.pi, atol=1e-11)
+    H = -np.array([[2 * dx[0] ** 2 - dx[1] ** 2 - dx[2] ** 2, 3 * dx[0] * dx[1], 3 * dx[0] * dx[2]],
+                   [3 * dx[0] * dx[1], -dx[0] ** 2 + 2 * dx[1] ** 2 - dx[2] ** 2, 3 * dx[1] * dx[2]],
+                   [3 * dx[0] * dx[2], 3 * dx[1] * dx[2], -dx[0] ** 2 - dx[1] ** 2 + 2 * dx[2] ** 2]]) / r ** 5 * mass / 4 / math.pi
+    np.testing.assert_allclose(np.asarray(h).reshape(3, 3).T, H, atol=1e-9)
+
+
+def test_source_chain_p2m_m2l_l2p_vs_analytic():
+    # test/bodytomultipole_test.jl:393-441: P2M -> M2L -> L2P of one source against 1/r
+    x = np.array([0.1, 0.2, -0.3])
+    xs = x + np.array([-0.2, 0.07, -0.1])
+    p = 10
+    M = O.p2m(O.KIND_SOURCE, xs - x, [0.7], p)
+    xt = np.array([2.3, -4.1, 0.4])
+    tc = xt + np.array([0.05, -0.1, -0.2])
+    L, _, _ = O.m2l(O.geom(tc), O.geom(x), M, p)
+    _, v, _ = O.l2p(xt - tc, L, p)
+    r = xt - xs
+    np.testing.assert_allclose(v, r / np.linalg.norm(r) ** 3 / 4 / math.pi * 0.7, atol=1e-12)
+
+
+# --------------------------------------------------------- Lamb-Helmholtz
+def test_lh_transform_multipole_fixture():
+    g = GOLD["lamb_helmholtz_test.jl:transform_multipole"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["w1"], p), p, compressed(g["w2"], p))
+    out = O.transform_lh_multipole(e, g["r"], p)
+    _check_pair(out, g["new_w1"], p, index=0)
+    _check_pair(out, g["new_w2"], p, index=1)
+
+
+def test_lh_transform_local_fixture():
+    g = GOLD["lamb_helmholtz_test.jl:transform_local"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["w1"], p), p, compressed(g["w2"], p))
+    out = O.transform_lh_local(e, g["r"], p)
+    _check_pair(out, g["new_w1"], p, index=0)
+    _check_pair(out, g["new_w2"], p, index=1)
+
+
+def test_lh_m2m_fixture():
+    g = GOLD["lamb_helmholtz_test.jl:m2m"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["w1"], p), p, compressed(g["w2"], p))
+    out = O.m2m(g["target_center"], g["source_center"], e, p, LH=True)
+    _check_pair(out, g["out1"], p, index=0)
+    _check_pair(out, g["out2"], p, index=1)
+
+
+def test_lh_m2l_fixture():
+    g = GOLD["lamb_helmholtz_test.jl:m2l"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["w1"], p), p, compressed(g["w2"], p))
+    out, _, _ = O.m2l(O.geom(g["target_center"]), O.geom(g["source_center"]), e, p, LH=True)
+    _check_pair(out, g["out1"], p, n_checked=g["n_checked"], index=0)
+    _check_pair(out, g["out2"], p, n_checked=g["n_checked"], index=1)
+
+
+def test_lh_l2l_fixture():
+    g = GOLD["lamb_helmholtz_test.jl:l2l"]
+    p = g["p"]
+    e = O.complex_to_exp(compressed(g["w1"], p), p, compressed(g["w2"], p))
+    out = O.l2l(g["target_center"], g["source_center"], e, p, LH=True)
+    _check_pair(out, g["out1"], p, n_checked=g["n_checked"], index=0, atol=1e-11)
+    _check_pair(out, g["out2"], p, n_checked=g["n_checked"], index=1, atol=1e-11)
+
+
+def test_vorton_chain_vs_biot_savart():
+    # test/lamb_helmholtz_test.jl:271-339, with P2M computed here instead of read from the fixture
+    g = GOLD["lamb_helmholtz_test.jl:vorton"]
+    p = g["p"]
+    c = np.array(g["center"])
+    xs, xt, G = np.array(g["x_source"]), np.array(g["x_target"]), np.array(g["gamma"])
+    M = O.p2m(O.KIND_VORTEX, xs - c, G, p)
+    lc = xt + np.array(g["local_center_offset"])
+    L, _, _ = O.m2l(O.geom(lc), O.geom(c), M, p, LH=True)
+    _, v, h = O.l2p(xt - lc, L, p, LH=True)
+    d = xt - xs
+    r = np.linalg.norm(d)
+    np.testing.assert_allclose(v, -np.cross(d, G) / r ** 3 / (4 * math.pi), atol=1e-11)
+    # velocity gradient against the P2P kernel of test/vortex.jl:111-122
+    ref = O.direct(O.KIND_VORTEX, xt[None, :], np.array([[*xs, 0.0, *G]]), gradient=True, hessian=True)
+    np.testing.assert_allclose(h, ref[0, 7:16], atol=1e-9)
+
+
+# ------------------------------------------------------------------- tree
+def _five_bodies():
+    g = GOLD["tree_test.jl:five_bodies"]
+    pos = np.array(g["positions"])
+    src = np.zeros((5, 5))
+    src[:, :3] = pos
+    src[:, 4] = 1.0
+    return g, pos, src
+
+
+def test_tree_five_bodies_known_answer():
+    g, pos, src = _five_bodies()
+    run = O.Run(O.KIND_SOURCE, pos, src, p=2, leaf_size_source=1, method=O.METHOD_SELFTUNING, shrink_recenter=False).trees()
+    ints, reals = run.branches(1)
+    assert list(ints[:, 0]) == g["n_bodies"]
+    c = np.array(g["center"])
+    np.testing.assert_allclose(reals[0, 3:6], c, atol=1e-12)
+    r1 = reals[0, 11:14].max()
+    r2 = reals[1, 11:14].max()
+    np.testing.assert_allclose(reals[0, 7], math.sqrt(3) * r1, atol=1e-7)
+    for k, s in enumerate(g["octant_signs"][1:], start=1):
+        np.testing.assert_allclose(reals[k, 3:6], c + np.array(s) * r1 / 2, atol=1e-7)
+        np.testing.assert_allclose(reals[k, 7], math.sqrt(3) * r2, atol=1e-7)
+    c2 = reals[1, 3:6]
+    np.testing.assert_allclose(reals[5, 3:6], c2 - r2 / 2, atol=1e-7)
+    np.testing.assert_allclose(reals[6, 3:6], c2 + np.array([-1, -1, 1]) * r2 / 2, atol=1e-7)
+    np.testing.assert_allclose(reals[5:7, 7], math.sqrt(3) * r2 / 2, atol=1e-7)
+
+
+def test_tree_shrink_invariants():
+    # test/tree_test.jl:134-250: leaves of single bodies recentre on the body; source box = body radius
+    g, pos, src = _five_bodies()
+    rng = np.random.default_rng(123)
+    src[:, 3] = rng.random(5)
+    run = O.Run(O.KIND_SOURCE, pos, src, p=2, leaf_size_source=1, method=O.METHOD_SELFTUNING, shrink_recenter=True).trees()
+    ints, reals = run.branches(1)
+    buf = run.buffer(1)
+    for ib in range(2, 7):
+        i = ints[ib, 1]
+        np.testing.assert_allclose(reals[ib, 3:6], buf[i, :3], atol=1e-12)
+        np.testing.assert_allclose(reals[ib, 0:3], buf[i, :3], atol=1e-12)
+        assert reals[ib, 7] == 0.0
+        assert reals[ib, 6] == pytest.approx(buf[i, 3], abs=1e-12)
+        np.testing.assert_allclose(reals[ib, 8:11], buf[i, 3], atol=1e-12)
+    b = ints[1]
+    cs = buf[b[1]:b[1] + b[0], :3]
+    rs = buf[b[1]:b[1] + b[0], 3]
+    lo, hi = (cs - rs[:, None]).min(0), (cs + rs[:, None]).max(0)
+    np.testing.assert_allclose(reals[1, 0:3], (lo + hi) / 2, atol=1e-12)
+    np.testing.assert_allclose(reals[1, 8:11], (hi - lo) / 2, atol=1e-12)
+    assert reals[1, 7] == pytest.approx(np.linalg.norm(cs - reals[1, 3:6], axis=1).max(), abs=1e-12)
+    assert reals[1, 6] == pytest.approx((np.linalg.norm(cs - reals[1, 0:3], axis=1) + rs).max(), abs=1e-12)
+
+
+def test_tree_leaf_numbering_and_sort():
+    rng = np.random.default_rng(7)
+    pos = rng.random((101, 3))
+    src = np.zeros((101, 5))
+    src[:, :3] = pos
+    run = O.Run(O.KIND_SOURCE, pos, src, leaf_size_source=5, method=O.METHOD_SELFTUNING).trees()
+    ints, reals = run.branches(1)
+    leaves = [k for k in range(len(ints)) if ints[k, 2] == 0]
+    assert list(run.leaf_index(1)) == leaves
+    assert list(ints[leaves, 5]) == list(range(len(leaves)))
+    si = run.sort_index(1)
+    assert sorted(si) == list(range(101))
+    np.testing.assert_array_equal(run.buffer(1)[:, :3], pos[si])
+    # every leaf's bodies keep their original relative order (stable partitions)
+    for k in leaves:
+        seg = si[ints[k, 1]:ints[k, 1] + ints[k, 0]]
+        assert list(seg) == sorted(seg)
+
+
+def test_interaction_lists_sorted_and_complete():
+    rng = np.random.default_rng(456)
+    n = 20000
+    pos = rng.random((n, 3))
+    src = np.zeros((n, 5))
+    src[:, :3] = pos
+    run = O.Run(O.KIND_SOURCE, pos, src, leaf_size_source=50, multipole_acceptance=0.5).trees().lists()
+    for which in (0, 1):
+        L = run.list(which)
+        assert np.all(np.diff(L[:, 0]) >= 0)
+    # every (target leaf, source leaf) body pair is covered exactly once by direct or by an M2L ancestor pair
+    ints, _ = run.branches(0)
+    nb = len(ints)
+    parent = ints[:, 4]
+    leaves = run.leaf_index(0)
+    cover = {}
+    m2l = {tuple(x) for x in run.list(0).tolist()}
+    direct = {tuple(x) for x in run.list(1).tolist()}
+    assert len(direct) == len(run.list(1)) and len(m2l) == len(run.list(0))
+
+    def ancestors(b):
+        out = [b]
+        while parent[b] >= 0:
+            b = parent[b]
+            out.append(b)
+        return out
+
+    anc = {int(l): ancestors(int(l)) for l in leaves}
+    sample = rng.choice(leaves, size=30, replace=False)
+    for tl in sample:
+        for sl in leaves:
+            hits = int((int(tl), int(sl)) in direct)
+            for a in anc[int(tl)]:
+                for b in anc[int(sl)]:
+                    hits += int((a, b) in m2l)
+            assert hits == 1, (tl, sl, hits)
+
+
+# -------------------------------------------------------------- full FMM
+def test_fmm_vs_direct_five_masses():
+    g = GOLD["fmm_test.jl:scalar_potential"]
+    pos = np.array(g["positions"])
+    src = np.zeros((5, 5))
+    src[:, :3] = pos
+    src[:, 4] = g["masses"]
+    run = O.Run(O.KIND_SOURCE, pos, src, p=g["p"], multipole_acceptance=g["multipole_acceptance"],
+                scalar_potential=True).fmm()
+    ref = O.direct(O.KIND_SOURCE, pos, src, scalar_potential=True)
+    np.testing.assert_allclose(run.targets()[:, 3], ref[:, 3], atol=1e-12)
+
+
+def test_direct_seven_masses():
+    g = GOLD["direct_test.jl:direct"]
+    x = np.array(g["x"]).T
+    m = np.array(g["m"])
+    src = np.zeros((7, 5))
+    src[:, :3] = x
+    src[:, 4] = m
+    got = O.direct(O.KIND_SOURCE, x, src, scalar_potential=True)[:, 3]
+    for i in range(7):
+        exp = sum(m[j] / np.linalg.norm(x[i] - x[j]) / 4 / math.pi for j in range(7) if j != i)
+        assert got[i] == pytest.approx(exp, abs=1e-12)
+
+
+def _uniform(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 3))
+    if kind == O.KIND_SOURCE:
+        src = np.zeros((n, 5))
+        src[:, :3] = pos
+        src[:, 4] = rng.random(n) / n
+    else:
+        src = np.zeros((n, 7))
+        src[:, :3] = pos
+        src[:, 4:7] = (2 * rng.random((n, 3)) - 1.0 - 0.5) * 2 / n
+    return pos, src
+
+
+@pytest.mark.parametrize("kind,err_kind,upper", [
+    (O.KIND_SOURCE, O.ERR_ROTATED_GRADIENT, 30), (O.KIND_VORTEX, O.ERR_ROTATED_GRADIENT, 100),
+    (O.KIND_SOURCE, O.ERR_POWER_GRADIENT, 10), (O.KIND_VORTEX, O.ERR_POWER_GRADIENT, 10)])
+def test_error_controlled_bands(kind, err_kind, upper):
+    """test/dynamic_expansion_order_test.jl:1-164 bands, BE=false, eps=1e-5, leaf 10, mac 0.5, p_max 20."""
+    n = 4000
+    pos, src = _uniform(kind, n, 123)
+    eps = 1e-5
+    run = O.Run(kind, pos, src, p=20, leaf_size_source=10, multipole_acceptance=0.5, error_kind=err_kind,
+                eps=eps, BE=False, gradient=True).fmm()
+    ref = O.direct(kind, pos, src)
+    err = np.linalg.norm(run.targets()[:, 4:7] - ref[:, 4:7], axis=1).max()
+    assert eps * 0.1 < err < eps * upper, err
