@@ -1,0 +1,18 @@
+"""fastmultipole_amd: MI355X-native FMM evaluator behind FastMultipole's fmm!/direct! interface.
+
+The GPU library (libfmmhip.so, C ABI in include/fmmhip.h) does the tree build,
+interaction lists, P2P, P2M/M2M/M2L/L2L/L2P; this package is the host-side
+mirror of the reference's entry points and user-system interface.
+"""
+from . import _lib
+from .core import (Context, Evaluation, DerivativesSwitch, ErrorMethod, InteractionListMethod, PowerAbsoluteGradient,
+                   PowerAbsolutePotential, PowerRelativeGradient, PowerRelativePotential,
+                   RotatedCoefficientsAbsoluteGradient, RotatedCoefficientsRelativeGradient, Barba, SelfTuning,
+                   SelfTuningTreeStop, default_context, direct, fmm, make_options)
+from .systems import Gravitational, VortexParticles
+
+__all__ = ["Context", "Evaluation", "DerivativesSwitch", "ErrorMethod", "InteractionListMethod",
+           "PowerAbsoluteGradient", "PowerAbsolutePotential", "PowerRelativeGradient", "PowerRelativePotential",
+           "RotatedCoefficientsAbsoluteGradient", "RotatedCoefficientsRelativeGradient", "Barba", "SelfTuning",
+           "SelfTuningTreeStop", "default_context", "direct", "fmm", "make_options", "Gravitational",
+           "VortexParticles"]

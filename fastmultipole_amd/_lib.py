@@ -1,0 +1,105 @@
+"""ctypes binding of libfmmhip.so (the C ABI declared in include/fmmhip.h).
+
+The library is built in-tree (fastmultipole_amd/libfmmhip.so) by
+__graft_entry__.build() or `make -C fastmultipole_amd/csrc`. There is no CPU
+fallback: every call below fails loudly if the library or the GPU is missing.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfmmhip.so")
+
+FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMHIP_EDEPTH = range(6)
+POINT_SOURCE, POINT_VORTEX = 0, 1
+FP64, MIXED = 0, 1
+BARBA, SELFTUNING, SELFTUNING_TREESTOP = 0, 1, 2
+ERR_NONE, ERR_ROTATED_COEFFICIENTS_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
+
+EXPORTED = [
+    "fmmhip_options_default", "fmmhip_ctx_create", "fmmhip_ctx_destroy", "fmmhip_last_error", "fmmhip_sync",
+    "fmmhip_eval_create", "fmmhip_eval_destroy", "fmmhip_eval_set_systems", "fmmhip_eval_build_trees",
+    "fmmhip_eval_build_lists", "fmmhip_eval_run", "fmmhip_fmm", "fmmhip_eval_get_targets", "fmmhip_eval_n_branches",
+    "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_get_branches", "fmmhip_eval_get_levels",
+    "fmmhip_eval_get_leaf_index", "fmmhip_eval_get_sort_index", "fmmhip_eval_n_list", "fmmhip_eval_get_list",
+    "fmmhip_eval_get_m2l_orders", "fmmhip_eval_get_expansions", "fmmhip_direct",
+]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("expansion_order", C.c_int32), ("leaf_size_source", C.c_int32), ("leaf_size_target", C.c_int32),
+        ("multipole_acceptance", C.c_double), ("list_method", C.c_int32), ("shrink_recenter", C.c_int32),
+        ("error_kind", C.c_int32), ("error_relative", C.c_int32), ("error_tolerance", C.c_double),
+        ("error_bonus", C.c_int32), ("scalar_potential", C.c_int32), ("gradient", C.c_int32), ("hessian", C.c_int32),
+        ("farfield", C.c_int32), ("nearfield", C.c_int32), ("self_induced", C.c_int32), ("precision", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("pmax", C.c_int32), ("error_success", C.c_int32), ("n_target_branches", C.c_int64),
+        ("n_source_branches", C.c_int64), ("n_m2l", C.c_int64), ("n_direct", C.c_int64), ("n_p2p_pairs", C.c_int64),
+        ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_p2p", C.c_double), ("ms_upward", C.c_double),
+        ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FmmHipError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"libfmmhip status {status}: {msg}")
+        self.status = status
+
+
+_LIB = None
+
+
+def load():
+    """Load libfmmhip.so (no compute call; usable without a GPU)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or `make -C fastmultipole_amd/csrc` (there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
+    L.fmmhip_options_default.argtypes = [C.POINTER(Options)]
+    L.fmmhip_options_default.restype = None
+    L.fmmhip_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.fmmhip_ctx_destroy.argtypes = [vp]
+    L.fmmhip_ctx_destroy.restype = None
+    L.fmmhip_last_error.argtypes = [vp]
+    L.fmmhip_last_error.restype = C.c_char_p
+    L.fmmhip_sync.argtypes = [vp]
+    L.fmmhip_eval_create.argtypes = [vp, C.POINTER(vp)]
+    L.fmmhip_eval_destroy.argtypes = [vp]
+    L.fmmhip_eval_destroy.restype = None
+    L.fmmhip_eval_set_systems.argtypes = [vp, i32, i64, dp, i64, dp, i32]
+    for f in ("fmmhip_eval_build_trees", "fmmhip_eval_build_lists"):
+        getattr(L, f).argtypes = [vp, C.POINTER(Options)]
+    for f in ("fmmhip_eval_run", "fmmhip_fmm"):
+        getattr(L, f).argtypes = [vp, C.POINTER(Options), C.POINTER(Stats)]
+    L.fmmhip_eval_get_targets.argtypes = [vp, dp, i32]
+    for f in ("fmmhip_eval_n_branches", "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_n_list"):
+        getattr(L, f).argtypes = [vp, i32]
+        getattr(L, f).restype = i64
+    L.fmmhip_eval_get_branches.argtypes = [vp, i32, dp, dp]
+    L.fmmhip_eval_get_levels.argtypes = [vp, i32, dp]
+    L.fmmhip_eval_get_leaf_index.argtypes = [vp, i32, dp]
+    L.fmmhip_eval_get_sort_index.argtypes = [vp, i32, dp]
+    L.fmmhip_eval_get_list.argtypes = [vp, i32, dp]
+    L.fmmhip_eval_get_m2l_orders.argtypes = [vp, dp]
+    L.fmmhip_eval_get_expansions.argtypes = [vp, i32, dp]
+    L.fmmhip_direct.argtypes = [vp, i32, i64, dp, i64, dp, i32, i32, i32, i32, dp, i32]
+    _LIB = L
+    return L
+
+
+def default_options():
+    o = Options()
+    load().fmmhip_options_default(C.byref(o))
+    return o

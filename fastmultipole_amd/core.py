@@ -1,0 +1,336 @@
+"""Host-side mirror of FastMultipole's fmm!/direct! entry points over libfmmhip.
+
+Mirrors src/fmm.jl:807-1176 (fmm! and its keyword arguments and 8-tuple
+return), src/direct.jl:20-75 (direct!), the error-method and interaction-list
+types of src/containers.jl:64-123, and the user-system interface of
+src/compatibility.jl (source_system_to_buffer!, data_per_body, get_position,
+strength_dims, get_n_bodies, body_to_multipole! kernel tag, direct!,
+buffer_to_target_system!, has_vector_potential).
+
+All evaluation runs on the GPU through libfmmhip; there is no CPU path.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ------------------------------------------------------------ error methods
+@dataclass(frozen=True)
+class ErrorMethod:
+    eps: float
+    BE: bool = True
+    kind: int = L.ERR_NONE
+    relative: bool = False
+
+
+def PowerAbsolutePotential(eps, BE=True):            # src/containers.jl:85-86
+    return ErrorMethod(float(eps), bool(BE), L.ERR_POWER_POTENTIAL, False)
+
+
+def PowerAbsoluteGradient(eps, BE=True):             # :88-89
+    return ErrorMethod(float(eps), bool(BE), L.ERR_POWER_GRADIENT, False)
+
+
+def RotatedCoefficientsAbsoluteGradient(eps, BE=True):   # :91-92
+    return ErrorMethod(float(eps), bool(BE), L.ERR_ROTATED_COEFFICIENTS_GRADIENT, False)
+
+
+def PowerRelativePotential(eps, BE=True):            # :97-98
+    return ErrorMethod(float(eps), bool(BE), L.ERR_POWER_POTENTIAL, True)
+
+
+def PowerRelativeGradient(eps, BE=True):             # :100-101
+    return ErrorMethod(float(eps), bool(BE), L.ERR_POWER_GRADIENT, True)
+
+
+def RotatedCoefficientsRelativeGradient(eps, BE=True):   # :103-104
+    return ErrorMethod(float(eps), bool(BE), L.ERR_ROTATED_COEFFICIENTS_GRADIENT, True)
+
+
+# ------------------------------------------------- interaction list methods
+@dataclass(frozen=True)
+class InteractionListMethod:
+    code: int
+
+
+def Barba():                 # src/containers.jl:110-112
+    return InteractionListMethod(L.BARBA)
+
+
+def SelfTuning():            # :118-120
+    return InteractionListMethod(L.SELFTUNING)
+
+
+def SelfTuningTreeStop():    # :122-123
+    return InteractionListMethod(L.SELFTUNING_TREESTOP)
+
+
+@dataclass(frozen=True)
+class DerivativesSwitch:     # src/containers.jl:50
+    scalar_potential: bool = True
+    gradient: bool = True
+    hessian: bool = True
+
+
+# ----------------------------------------------------------------- context
+class Context:
+    """One HIP device + stream (fmmhip_ctx). Not thread-safe."""
+
+    def __init__(self, device=-1):
+        lib = L.load()
+        h = C.c_void_p()
+        st = lib.fmmhip_ctx_create(int(device), C.byref(h))
+        if st != L.FMMHIP_OK:
+            raise L.FmmHipError(st, "fmmhip_ctx_create failed (no usable GPU?)")
+        self._h = h
+        self.lib = lib
+
+    def check(self, st):
+        if st != L.FMMHIP_OK:
+            raise L.FmmHipError(st, self.lib.fmmhip_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.fmmhip_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT = None
+
+
+def default_context():
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context()
+    return _DEFAULT
+
+
+def make_options(expansion_order=5, leaf_size_source=20, leaf_size_target=None, multipole_acceptance=0.4,
+                 interaction_list_method=None, shrink_recenter=True, error_tolerance=None, scalar_potential=False,
+                 gradient=True, hessian=False, farfield=True, nearfield=True, self_induced=True, precision="fp64"):
+    o = L.default_options()
+    o.expansion_order = int(expansion_order)
+    o.leaf_size_source = int(leaf_size_source)
+    o.leaf_size_target = int(leaf_size_source if leaf_size_target is None else leaf_size_target)
+    o.multipole_acceptance = float(multipole_acceptance)
+    o.list_method = (interaction_list_method or SelfTuningTreeStop()).code
+    o.shrink_recenter = int(bool(shrink_recenter))
+    if error_tolerance is not None:
+        o.error_kind = error_tolerance.kind
+        o.error_relative = int(error_tolerance.relative)
+        o.error_tolerance = error_tolerance.eps
+        o.error_bonus = int(error_tolerance.BE)
+    o.scalar_potential, o.gradient, o.hessian = int(scalar_potential), int(gradient), int(hessian)
+    o.farfield, o.nearfield, o.self_induced = int(farfield), int(nearfield), int(self_induced)
+    o.precision = {"fp64": L.FP64, "mixed": L.MIXED}[precision] if isinstance(precision, str) else int(precision)
+    return o
+
+
+class TreeView:
+    """Host view of a device tree (Tree, src/containers.jl:182-195); indices are 0-based."""
+
+    def __init__(self, ev, which):
+        lib, h = ev.ctx.lib, ev._h
+        nb = lib.fmmhip_eval_n_branches(h, which)
+        self.n_branches = nb
+        ints = np.zeros((max(nb, 1), 6), dtype=np.int64)
+        reals = np.zeros((max(nb, 1), 17))
+        if nb:
+            ev.ctx.check(lib.fmmhip_eval_get_branches(h, which, ints.ctypes.data, reals.ctypes.data))
+        self.ints, self.reals = ints[:nb], reals[:nb]
+        nl = lib.fmmhip_eval_n_levels(h, which)
+        lv = np.zeros((max(nl, 1), 2), dtype=np.int64)
+        if nl:
+            ev.ctx.check(lib.fmmhip_eval_get_levels(h, which, lv.ctypes.data))
+        self.levels_index = lv[:nl]
+        nleaf = lib.fmmhip_eval_n_leaves(h, which)
+        li = np.zeros(max(nleaf, 1), dtype=np.int64)
+        if nleaf:
+            ev.ctx.check(lib.fmmhip_eval_get_leaf_index(h, which, li.ctypes.data))
+        self.leaf_index = li[:nleaf]
+        n = ev.nt if which == 0 else ev.ns
+        si = np.zeros(max(n, 1), dtype=np.int64)
+        if n and nb:
+            ev.ctx.check(lib.fmmhip_eval_get_sort_index(h, which, si.ctypes.data))
+        self.sort_index = si[:n]
+        self.inverse_sort_index = np.empty_like(self.sort_index)
+        self.inverse_sort_index[self.sort_index] = np.arange(n)
+        self._ev, self._which = ev, which
+
+    def expansions(self, p):
+        H = (p + 1) * (p + 2) // 2
+        out = np.zeros(self.n_branches * 4 * H)
+        self._ev.ctx.check(self._ev.ctx.lib.fmmhip_eval_get_expansions(self._ev._h, self._which, out.ctypes.data))
+        return out.reshape(self.n_branches, H, 2, 2)
+
+
+class Evaluation:
+    """Device-resident trees, lists and expansions of one fmm! call (the reference's Cache + trees)."""
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.fmmhip_eval_create(self.ctx._h, C.byref(h)))
+        self._h = h
+        self.nt = self.ns = 0
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx.lib.fmmhip_eval_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_systems(self, kernel, target_xyz, source_buffer, on_device=False):
+        """target_xyz: (nt,3) float64 or None (targets = sources); source_buffer: (ns, data_per_body).
+
+        With on_device=True both are device pointers (ints) or objects with data_ptr() (torch tensors)."""
+        if on_device:
+            tp = 0 if target_xyz is None else (target_xyz.data_ptr() if hasattr(target_xyz, "data_ptr") else int(target_xyz))
+            sp = source_buffer.data_ptr() if hasattr(source_buffer, "data_ptr") else int(source_buffer)
+            ns = source_buffer.shape[0]
+            nt = ns if target_xyz is None else target_xyz.shape[0]
+            self._keep = (target_xyz, source_buffer)
+        else:
+            sb = np.ascontiguousarray(source_buffer, dtype=np.float64)
+            ns = sb.shape[0]
+            tx = None if target_xyz is None else np.ascontiguousarray(target_xyz, dtype=np.float64)
+            nt = ns if tx is None else tx.shape[0]
+            self._keep = (tx, sb)
+            tp = 0 if tx is None else tx.ctypes.data
+            sp = sb.ctypes.data
+        self.kernel, self.nt, self.ns = kernel, nt, ns
+        self.ctx.check(self.ctx.lib.fmmhip_eval_set_systems(self._h, kernel, nt, tp or None, ns, sp or None, int(on_device)))
+
+    def build_trees(self, opt):
+        self.ctx.check(self.ctx.lib.fmmhip_eval_build_trees(self._h, C.byref(opt)))
+
+    def build_lists(self, opt):
+        self.ctx.check(self.ctx.lib.fmmhip_eval_build_lists(self._h, C.byref(opt)))
+
+    def run(self, opt):
+        st = L.Stats()
+        self.ctx.check(self.ctx.lib.fmmhip_eval_run(self._h, C.byref(opt), C.byref(st)))
+        return st
+
+    def fmm(self, opt):
+        st = L.Stats()
+        self.ctx.check(self.ctx.lib.fmmhip_fmm(self._h, C.byref(opt), C.byref(st)))
+        return st
+
+    def targets(self, out=None):
+        """16 x nt target buffer rows in original order, as an (nt, 16) array (or into a device pointer)."""
+        if out is not None:
+            ptr = out.data_ptr() if hasattr(out, "data_ptr") else int(out)
+            self.ctx.check(self.ctx.lib.fmmhip_eval_get_targets(self._h, ptr, 1))
+            return out
+        a = np.zeros((self.nt, 16))
+        if self.nt:
+            self.ctx.check(self.ctx.lib.fmmhip_eval_get_targets(self._h, a.ctypes.data, 0))
+        return a
+
+    def tree(self, which):
+        return TreeView(self, which)
+
+    def list(self, which):
+        n = self.ctx.lib.fmmhip_eval_n_list(self._h, which)
+        out = np.zeros((max(n, 1), 2), dtype=np.int32)
+        if n:
+            self.ctx.check(self.ctx.lib.fmmhip_eval_get_list(self._h, which, out.ctypes.data))
+        return out[:n]
+
+    def m2l_orders(self):
+        n = self.ctx.lib.fmmhip_eval_n_list(self._h, 0)
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        if n:
+            self.ctx.check(self.ctx.lib.fmmhip_eval_get_m2l_orders(self._h, out.ctypes.data))
+        return out[:n]
+
+
+# ------------------------------------------------------------------ fmm!
+def _as_tuple(x):
+    return x if isinstance(x, tuple) else (x,)
+
+
+def _kernel_of(system):
+    tag = getattr(system, "fmm_kernel", None)
+    if tag is None:
+        raise TypeError(f"{type(system).__name__} does not declare a built-in kernel (fmm_kernel); "
+                        "only Point{Source} and Point{Vortex} systems run on libfmmhip")
+    return tag
+
+
+def fmm(target_systems, source_systems=None, *, leaf_size=None, leaf_size_source=None, leaf_size_target=None,
+        expansion_order=5, multipole_acceptance=None, error_tolerance=None, shrink_recenter=True,
+        interaction_list_method=None, scalar_potential=False, gradient=True, hessian=False, farfield=True,
+        nearfield=True, self_induced=True, update_target_systems=True, precision="fp64", evaluation=None):
+    """fmm!(systems) / fmm!(targets, sources) of src/fmm.jl:807-915.
+
+    Returns the reference's 8-tuple (optimized_args, cache, target_tree, source_tree,
+    m2l_list, direct_list, derivatives_switches, error_success); `cache` is the
+    device-resident Evaluation, the trees are host views with 0-based indices.
+    """
+    single = source_systems is None
+    if single:
+        source_systems = target_systems
+        ls = 20 if leaf_size is None else leaf_size        # fmm!(system; leaf_size=20) (src/fmm.jl:807)
+        leaf_size_source = ls if leaf_size_source is None else leaf_size_source
+        leaf_size_target = ls if leaf_size_target is None else leaf_size_target
+    if leaf_size_source is None:
+        leaf_size_source = 20
+    if multipole_acceptance is None:
+        multipole_acceptance = 0.4
+    targets, sources = _as_tuple(target_systems), _as_tuple(source_systems)
+    if len(targets) != 1 or len(sources) != 1:
+        raise NotImplementedError("one target system and one source system per call")
+    tsys, ssys = targets[0], sources[0]
+    kernel = _kernel_of(ssys)
+    ev = evaluation or Evaluation()
+    sbuf = ssys.source_buffer()
+    txyz = None if (single or tsys is ssys) else tsys.positions()
+    ev.set_systems(kernel, txyz, sbuf)
+    opt = make_options(expansion_order, leaf_size_source, leaf_size_target, multipole_acceptance,
+                       interaction_list_method, shrink_recenter, error_tolerance, scalar_potential, gradient, hessian,
+                       farfield, nearfield, self_induced, precision)
+    stats = ev.fmm(opt)
+    out = ev.targets()
+    switch = DerivativesSwitch(bool(scalar_potential), bool(gradient), bool(hessian))
+    if update_target_systems:
+        tsys.buffer_to_target_system(out, switch)
+    optimized_args = {"leaf_size_source": opt.leaf_size_source, "expansion_order": max(opt.expansion_order, 1),
+                      "multipole_acceptance": opt.multipole_acceptance}
+    ev.last_stats = stats
+    return (optimized_args, ev, ev.tree(0), ev.tree(1), ev.list(0), ev.list(1), (switch,), bool(stats.error_success))
+
+
+def direct(target_systems, source_systems=None, *, scalar_potential=False, gradient=True, hessian=False, precision="fp64",
+           ctx=None):
+    """direct!(targets, sources; scalar_potential, gradient, hessian) of src/direct.jl:20-75, on the GPU."""
+    if source_systems is None:
+        source_systems = target_systems
+    tsys, ssys = _as_tuple(target_systems)[0], _as_tuple(source_systems)[0]
+    ctx = ctx or default_context()
+    kernel = _kernel_of(ssys)
+    txyz = np.ascontiguousarray(tsys.positions(), dtype=np.float64)
+    sbuf = np.ascontiguousarray(ssys.source_buffer(), dtype=np.float64)
+    out = np.zeros((txyz.shape[0], 16))
+    out[:, :3] = txyz
+    prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
+    ctx.check(ctx.lib.fmmhip_direct(ctx._h, kernel, txyz.shape[0], txyz.ctypes.data, sbuf.shape[0], sbuf.ctypes.data,
+                                    int(scalar_potential), int(gradient), int(hessian), prec, out.ctypes.data, 0))
+    tsys.buffer_to_target_system(out, DerivativesSwitch(bool(scalar_potential), bool(gradient), bool(hessian)))
+    return out

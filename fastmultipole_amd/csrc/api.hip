@@ -1,0 +1,519 @@
+// C ABI of libfmmhip (include/fmmhip.h): contexts, evaluations and the fmm!
+// pipeline (src/fmm.jl:862-1176) on one MI355X.
+#include "fmmhip_internal.hpp"
+#include <cstring>
+#include <algorithm>
+#include <cmath>
+
+using namespace fmmhip;
+
+struct fmmhip_ctx { Context c; };
+struct fmmhip_eval { Eval e; };
+
+namespace fmmhip {
+
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)std::max<int64_t>(1, (n + block - 1) / block); }
+
+__global__ void k_gather_outputs(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                                 const int64_t* __restrict__ sort_index, const double* __restrict__ tout, int64_t nt,
+                                 double* __restrict__ aos) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    double* o = aos + sort_index[i] * 16;
+    o[0] = x[i]; o[1] = y[i]; o[2] = z[i];
+    for (int k = 0; k < 13; ++k) o[3 + k] = tout[(int64_t)k * nt + i];
+}
+
+void gather_outputs(Context& C, const Tree& tt, const double* tout, double* aos, int64_t nt) {
+    if (nt == 0) return;
+    k_gather_outputs<<<grid_for(nt, 256), 256, 0, C.stream>>>(tt.x.data(), tt.y.data(), tt.z.data(), tt.sort_index.data(), tout,
+                                                              nt, aos);
+    LAUNCH_CHECK();
+}
+
+// influence of each main-tree target from an estimate evaluation (src/fmm.jl:1206-1227)
+__global__ void k_map_influence(const int64_t* __restrict__ est_sort, const int64_t* __restrict__ main_inverse,
+                                const double* __restrict__ est_tout, int64_t nt, int use_potential, double* __restrict__ infl) {
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    int64_t jm = main_inverse[est_sort[j]];
+    if (use_potential) infl[jm] = est_tout[j];
+    else {
+        double vx = est_tout[nt + j], vy = est_tout[2 * nt + j], vz = est_tout[3 * nt + j];
+        infl[jm] = sqrt(vx * vx + vy * vy + vz * vz);
+    }
+}
+
+// max_influence of every branch over its bodies (src/fmm.jl:1231-1262)
+__global__ void k_branch_max(Branch* __restrict__ br, int64_t nb, const double* __restrict__ infl) {
+    __shared__ double s[256];
+    const int64_t ib = blockIdx.x;
+    if (ib >= nb) return;
+    Branch& B = br[ib];
+    double m = B.max_influence;
+    for (int64_t i = B.first + threadIdx.x; i < B.first + B.n_bodies; i += blockDim.x) m = fmax(m, infl[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) B.max_influence = s[0];
+}
+
+static void check_options(const fmmhip_options* o) {
+    if (!o) throw Error(FMMHIP_EINVAL, "options is NULL");
+    if (o->expansion_order < 1 || o->expansion_order > 20) throw Error(FMMHIP_EINVAL, "expansion_order must be in 1..20");
+    if (o->leaf_size_source < 1 || o->leaf_size_target < 1) throw Error(FMMHIP_EINVAL, "leaf sizes must be positive");
+    if (!(o->multipole_acceptance > 0.0)) throw Error(FMMHIP_EINVAL, "multipole_acceptance must be positive");
+    if (o->list_method < 0 || o->list_method > 2) throw Error(FMMHIP_EINVAL, "unknown interaction list method");
+    if (o->error_kind < 0 || o->error_kind > 3) throw Error(FMMHIP_EINVAL, "unknown error method");
+    if (o->precision != FMMHIP_FP64 && o->precision != FMMHIP_MIXED) throw Error(FMMHIP_EINVAL, "unknown precision");
+}
+
+struct Timer {
+    Context& C;
+    hipEvent_t a, b;
+    explicit Timer(Context& c) : C(c) {
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        HIP_CHECK(hipEventRecord(a, C.stream));
+    }
+    double stop() {
+        HIP_CHECK(hipEventRecord(b, C.stream));
+        HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        HIP_CHECK(hipEventDestroy(a));
+        HIP_CHECK(hipEventDestroy(b));
+        return ms;
+    }
+};
+
+static void eval_build_trees(Eval& E, const fmmhip_options& o) {
+    Context& C = *E.ctx;
+    const int dpb = data_per_body(E.kernel);
+    const int p = o.expansion_order;
+    if (E.t_xyz) build_tree(C, E.tt, E.nt, E.t_xyz, 3, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
+    else build_tree(C, E.tt, E.nt, E.s_buf, dpb, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
+    build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
+               strength_dims(E.kernel));
+    E.trees_built = true;
+    E.lists_built = false;
+    E.p_built = p;
+}
+
+static void eval_build_lists(Eval& E, const fmmhip_options& o) {
+    if (!E.trees_built) throw Error(FMMHIP_EINVAL, "trees are not built");
+    build_lists(*E.ctx, E.tt, E.st, E.lists, o.leaf_size_source, o.multipole_acceptance, o.list_method, o.farfield != 0,
+                o.nearfield != 0, o.self_induced != 0);
+    E.lists_built = true;
+}
+
+static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, bool is_estimate);
+
+// estimate_influence! for relative tolerances (src/fmm.jl:1193-1276): a full
+// low-order evaluation (p = 3, leaf 5, MAC 0.6, potential and gradient) on its
+// own trees; max |grad| (or potential) per main-tree branch
+static void estimate_influence(Eval& E, const fmmhip_options& o) {
+    Context& C = *E.ctx;
+    Eval X;
+    X.ctx = E.ctx;
+    X.kernel = E.kernel;
+    X.nt = E.nt;
+    X.ns = E.ns;
+    X.t_xyz = E.t_xyz;
+    X.s_buf = E.s_buf;
+    fmmhip_options eo = o;
+    eo.scalar_potential = 1; eo.gradient = 1; eo.hessian = 0;
+    eo.leaf_size_source = 5; eo.leaf_size_target = 5;
+    eo.expansion_order = 3; eo.multipole_acceptance = 0.6;
+    eo.error_kind = FMMHIP_ERR_NONE; eo.error_relative = 0;
+    eo.shrink_recenter = 1;
+    eo.list_method = FMMHIP_SELFTUNING_TREESTOP;
+    eo.farfield = eo.nearfield = eo.self_induced = 1;
+    eval_build_trees(X, eo);
+    eval_build_lists(X, eo);
+    eval_core(X, eo, nullptr, true);
+    DBuf<double> infl;
+    infl.resize(std::max<int64_t>(1, E.nt));
+    k_map_influence<<<grid_for(E.nt, 256), 256, 0, C.stream>>>(X.tt.sort_index.data(), E.tt.inverse_sort.data(), X.tout.data(),
+                                                              E.nt, o.error_kind == FMMHIP_ERR_POWER_POTENTIAL ? 1 : 0,
+                                                              infl.data());
+    LAUNCH_CHECK();
+    k_branch_max<<<(unsigned)E.tt.n_branches, 256, 0, C.stream>>>(E.tt.branches.data(), E.tt.n_branches, infl.data());
+    LAUNCH_CHECK();
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, bool is_estimate) {
+    Context& C = *E.ctx;
+    if (!E.lists_built) throw Error(FMMHIP_EINVAL, "interaction lists are not built");
+    const int p = o.expansion_order;
+    if (p != E.p_built) throw Error(FMMHIP_EINVAL, "expansion_order differs from the one the trees were built with");
+    const bool LH = E.kernel == FMMHIP_POINT_VORTEX;   // has_vector_potential
+    const bool PS = o.scalar_potential != 0, GS = o.gradient != 0, HS = o.hessian != 0;
+    Timer total(C);
+    if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) estimate_influence(E, o);
+    E.tout.resize((size_t)13 * std::max<int64_t>(1, E.nt));
+    HIP_CHECK(hipMemsetAsync(E.tout.data(), 0, E.tout.size() * sizeof(double), C.stream));
+    if (E.tt.n_branches) HIP_CHECK(hipMemsetAsync(E.tt.expansions.data(), 0, E.tt.expansions.size() * sizeof(double), C.stream));
+    if (E.st.n_branches) HIP_CHECK(hipMemsetAsync(E.st.expansions.data(), 0, E.st.expansions.size() * sizeof(double), C.stream));
+    E.pmax = 0;
+    E.error_success = 1;
+    double t_p2p = 0, t_up = 0, t_m2l = 0, t_down = 0;
+    if (E.nt > 0 && E.ns > 0) {
+        ensure_tables(C, p);
+        {
+            Timer t(C);
+            run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision);
+            t_p2p = t.stop();
+        }
+        {
+            Timer t(C);
+            run_p2m(C, E.kernel, E.st, p);
+            run_m2m(C, E.st, p, LH);
+            t_up = t.stop();
+        }
+        {
+            Timer t(C);
+            int32_t pmax = 0, ok = 1;
+            run_m2l(C, E.tt, E.st, E.tt, E.lists, p, LH, o.error_kind, o.error_relative != 0, o.error_tolerance,
+                    o.error_bonus != 0, o.precision, &pmax, &ok);
+            E.pmax = pmax;
+            E.error_success = ok;
+            if (!ok) E.pmax += 1;
+            t_m2l = t.stop();
+        }
+        {
+            Timer t(C);
+            run_l2l(C, E.tt, p, LH);
+            run_l2p(C, E.tt, E.tout.data(), E.nt, p, LH, PS, GS, HS);
+            t_down = t.stop();
+        }
+    }
+    if (!is_estimate) {
+        E.tout_aos.resize((size_t)16 * std::max<int64_t>(1, E.nt));
+        gather_outputs(C, E.tt, E.tout.data(), E.tout_aos.data(), E.nt);
+    }
+    const double t_tot = total.stop();
+    E.ran = true;
+    if (stats) {
+        stats->pmax = E.pmax;
+        stats->error_success = E.error_success;
+        stats->n_target_branches = E.tt.n_branches;
+        stats->n_source_branches = E.st.n_branches;
+        stats->n_m2l = E.lists.n_m2l;
+        stats->n_direct = E.lists.n_direct;
+        stats->ms_p2p = t_p2p;
+        stats->ms_upward = t_up;
+        stats->ms_m2l = t_m2l;
+        stats->ms_downward = t_down;
+        stats->ms_total = t_tot;
+    }
+}
+
+}  // namespace fmmhip
+
+// ====================================================================== ABI
+#define ABI_TRY(ctxptr, ...)                                             \
+    try {                                                                \
+        __VA_ARGS__;                                                     \
+        return FMMHIP_OK;                                                \
+    } catch (const Error& err) {                                         \
+        if (ctxptr) (ctxptr)->last_error = err.what();                   \
+        return err.status;                                               \
+    } catch (const std::exception& err) {                                \
+        if (ctxptr) (ctxptr)->last_error = err.what();                   \
+        return FMMHIP_EDEVICE;                                           \
+    }
+
+extern "C" {
+
+void fmmhip_options_default(fmmhip_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->expansion_order = 5;
+    o->leaf_size_source = 20;
+    o->leaf_size_target = 20;
+    o->multipole_acceptance = 0.4;
+    o->list_method = FMMHIP_SELFTUNING_TREESTOP;
+    o->shrink_recenter = 1;
+    o->error_kind = FMMHIP_ERR_NONE;
+    o->error_relative = 0;
+    o->error_tolerance = 0.0;
+    o->error_bonus = 1;
+    o->scalar_potential = 0;
+    o->gradient = 1;
+    o->hessian = 0;
+    o->farfield = o->nearfield = o->self_induced = 1;
+    o->precision = FMMHIP_FP64;
+}
+
+fmmhip_status fmmhip_ctx_create(int device, fmmhip_ctx** out) {
+    if (!out) return FMMHIP_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return FMMHIP_EDEVICE;
+    fmmhip_ctx* h = new fmmhip_ctx();
+    try {
+        if (device >= 0) HIP_CHECK(hipSetDevice(device));
+        HIP_CHECK(hipGetDevice(&h->c.device));
+        HIP_CHECK(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
+    } catch (const Error& e) {
+        delete h;
+        return e.status;
+    }
+    *out = h;
+    return FMMHIP_OK;
+}
+
+void fmmhip_ctx_destroy(fmmhip_ctx* h) {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->c.stream);
+    free_tables(h->c.tables);
+    (void)hipStreamDestroy(h->c.stream);
+    delete h;
+}
+
+const char* fmmhip_last_error(const fmmhip_ctx* h) { return h ? h->c.last_error.c_str() : "null context"; }
+
+fmmhip_status fmmhip_sync(fmmhip_ctx* h) {
+    if (!h) return FMMHIP_EINVAL;
+    Context* C = &h->c;
+    ABI_TRY(C, HIP_CHECK(hipStreamSynchronize(C->stream)));
+}
+
+fmmhip_status fmmhip_eval_create(fmmhip_ctx* h, fmmhip_eval** out) {
+    if (!h || !out) return FMMHIP_EINVAL;
+    fmmhip_eval* e = new fmmhip_eval();
+    e->e.ctx = &h->c;
+    *out = e;
+    return FMMHIP_OK;
+}
+
+void fmmhip_eval_destroy(fmmhip_eval* e) {
+    if (!e) return;
+    (void)hipStreamSynchronize(e->e.ctx->stream);
+    delete e;
+}
+
+fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t nt, const double* t_xyz, int64_t ns,
+                                      const double* s_buf, int32_t on_device) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (kernel != FMMHIP_POINT_SOURCE && kernel != FMMHIP_POINT_VORTEX) throw Error(FMMHIP_EINVAL, "unknown kernel");
+        if (nt < 0 || ns < 0) throw Error(FMMHIP_EINVAL, "negative system size");
+        if ((ns > 0 && !s_buf) || (nt > 0 && !t_xyz && nt != ns)) throw Error(FMMHIP_EINVAL, "missing system buffer");
+        E.kernel = kernel;
+        E.nt = nt;
+        E.ns = ns;
+        const int dpb = data_per_body(kernel);
+        if (on_device) {
+            E.t_xyz = t_xyz;
+            E.s_buf = s_buf;
+        } else {
+            E.s_buf_in.resize(std::max<int64_t>(1, (int64_t)dpb * ns));
+            if (ns) HIP_CHECK(hipMemcpyAsync(E.s_buf_in.data(), s_buf, sizeof(double) * dpb * ns, hipMemcpyHostToDevice, C->stream));
+            E.s_buf = E.s_buf_in.data();
+            if (t_xyz) {
+                E.t_xyz_in.resize(std::max<int64_t>(1, 3 * nt));
+                if (nt) HIP_CHECK(hipMemcpyAsync(E.t_xyz_in.data(), t_xyz, sizeof(double) * 3 * nt, hipMemcpyHostToDevice, C->stream));
+                E.t_xyz = E.t_xyz_in.data();
+            } else E.t_xyz = nullptr;
+            HIP_CHECK(hipStreamSynchronize(C->stream));
+        }
+        E.trees_built = E.lists_built = E.ran = false;
+    });
+}
+
+fmmhip_status fmmhip_eval_build_trees(fmmhip_eval* ev, const fmmhip_options* o) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        check_options(o);
+        eval_build_trees(ev->e, *o);
+        HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
+fmmhip_status fmmhip_eval_build_lists(fmmhip_eval* ev, const fmmhip_options* o) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        check_options(o);
+        eval_build_lists(ev->e, *o);
+        HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
+fmmhip_status fmmhip_eval_run(fmmhip_eval* ev, const fmmhip_options* o, fmmhip_stats* stats) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        check_options(o);
+        eval_core(ev->e, *o, stats, false);
+        HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
+fmmhip_status fmmhip_fmm(fmmhip_eval* ev, const fmmhip_options* o, fmmhip_stats* stats) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        check_options(o);
+        Timer t(*C);
+        double t_tree, t_lists;
+        {
+            Timer a(*C);
+            eval_build_trees(ev->e, *o);
+            t_tree = a.stop();
+        }
+        {
+            Timer a(*C);
+            eval_build_lists(ev->e, *o);
+            t_lists = a.stop();
+        }
+        eval_core(ev->e, *o, stats, false);
+        double tt = t.stop();
+        if (stats) {
+            stats->ms_tree = t_tree;
+            stats->ms_lists = t_lists;
+            stats->ms_total = tt;
+        }
+    });
+}
+
+fmmhip_status fmmhip_eval_get_targets(fmmhip_eval* ev, double* out, int32_t on_device) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (!E.ran) throw Error(FMMHIP_EINVAL, "no evaluation has run");
+        if (E.nt)
+            HIP_CHECK(hipMemcpyAsync(out, E.tout_aos.data(), sizeof(double) * 16 * E.nt,
+                                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, C->stream));
+        HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
+static Tree& which_tree(Eval& E, int32_t which) { return which == 0 ? E.tt : E.st; }
+
+int64_t fmmhip_eval_n_branches(fmmhip_eval* ev, int32_t which) { return ev ? which_tree(ev->e, which).n_branches : -1; }
+int64_t fmmhip_eval_n_levels(fmmhip_eval* ev, int32_t which) { return ev ? (int64_t)which_tree(ev->e, which).level_first.size() : -1; }
+int64_t fmmhip_eval_n_leaves(fmmhip_eval* ev, int32_t which) { return ev ? which_tree(ev->e, which).n_leaves : -1; }
+
+fmmhip_status fmmhip_eval_get_branches(fmmhip_eval* ev, int32_t which, int64_t* ints, double* reals) {
+    if (!ev || !ints || !reals) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        Tree& T = which_tree(ev->e, which);
+        std::vector<Branch> h(T.n_branches);
+        if (T.n_branches)
+            HIP_CHECK(hipMemcpy(h.data(), T.branches.data(), sizeof(Branch) * T.n_branches, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < T.n_branches; ++i) {
+            const Branch& b = h[i];
+            int64_t* I = ints + 6 * i;
+            I[0] = b.n_bodies; I[1] = b.first; I[2] = b.n_branches; I[3] = b.child_first; I[4] = b.parent; I[5] = b.leaf;
+            double* D = reals + 17 * i;
+            for (int d = 0; d < 3; ++d) {
+                D[d] = b.source_center[d]; D[3 + d] = b.target_center[d];
+                D[8 + d] = b.source_box[d]; D[11 + d] = b.target_box[d];
+            }
+            D[6] = b.source_radius; D[7] = b.target_radius; D[14] = b.max_influence; D[15] = 0; D[16] = 0;
+        }
+    });
+}
+
+fmmhip_status fmmhip_eval_get_levels(fmmhip_eval* ev, int32_t which, int64_t* out) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Tree& T = which_tree(ev->e, which);
+    for (size_t i = 0; i < T.level_first.size(); ++i) { out[2 * i] = T.level_first[i]; out[2 * i + 1] = T.level_last[i]; }
+    return FMMHIP_OK;
+}
+
+fmmhip_status fmmhip_eval_get_leaf_index(fmmhip_eval* ev, int32_t which, int64_t* out) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        Tree& T = which_tree(ev->e, which);
+        if (T.n_leaves) HIP_CHECK(hipMemcpy(out, T.leaf_index.data(), sizeof(int64_t) * T.n_leaves, hipMemcpyDeviceToHost));
+    });
+}
+
+fmmhip_status fmmhip_eval_get_sort_index(fmmhip_eval* ev, int32_t which, int64_t* out) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        Tree& T = which_tree(ev->e, which);
+        if (T.n) HIP_CHECK(hipMemcpy(out, T.sort_index.data(), sizeof(int64_t) * T.n, hipMemcpyDeviceToHost));
+    });
+}
+
+int64_t fmmhip_eval_n_list(fmmhip_eval* ev, int32_t which) {
+    if (!ev) return -1;
+    return which == 0 ? ev->e.lists.n_m2l : ev->e.lists.n_direct;
+}
+
+fmmhip_status fmmhip_eval_get_list(fmmhip_eval* ev, int32_t which, int32_t* out) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        Lists& L = ev->e.lists;
+        int64_t n = which == 0 ? L.n_m2l : L.n_direct;
+        if (n) {
+            std::vector<int32_t> t(n), s(n);
+            HIP_CHECK(hipMemcpy(t.data(), which == 0 ? L.m2l_t.data() : L.dir_t.data(), 4 * n, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(s.data(), which == 0 ? L.m2l_s.data() : L.dir_s.data(), 4 * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) { out[2 * i] = t[i]; out[2 * i + 1] = s[i]; }
+        }
+    });
+}
+
+fmmhip_status fmmhip_eval_get_m2l_orders(fmmhip_eval* ev, int32_t* out) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        Lists& L = ev->e.lists;
+        if (L.n_m2l) HIP_CHECK(hipMemcpy(out, L.m2l_order.data(), 4 * L.n_m2l, hipMemcpyDeviceToHost));
+    });
+}
+
+fmmhip_status fmmhip_eval_get_expansions(fmmhip_eval* ev, int32_t which, double* out) {
+    if (!ev || !out) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        Tree& T = which_tree(ev->e, which);
+        size_t n = (size_t)T.n_branches * 4 * n_harmonics(T.p);
+        if (n) HIP_CHECK(hipMemcpy(out, T.expansions.data(), sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+fmmhip_status fmmhip_direct(fmmhip_ctx* h, int32_t kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
+                            int32_t PS, int32_t GS, int32_t HS, int32_t precision, double* out, int32_t on_device) {
+    if (!h) return FMMHIP_EINVAL;
+    Context* C = &h->c;
+    ABI_TRY(C, {
+        if (kernel != FMMHIP_POINT_SOURCE && kernel != FMMHIP_POINT_VORTEX) throw Error(FMMHIP_EINVAL, "unknown kernel");
+        if (nt < 0 || ns < 0 || (nt && (!t_xyz || !out)) || (ns && !s_buf)) throw Error(FMMHIP_EINVAL, "bad direct arguments");
+        const int dpb = data_per_body(kernel);
+        if (on_device) {
+            run_direct_all(*C, kernel, nt, t_xyz, ns, s_buf, out, PS, GS, HS, precision);
+        } else {
+            DBuf<double> dt, ds, dout;
+            dt.resize(std::max<int64_t>(1, 3 * nt));
+            ds.resize(std::max<int64_t>(1, (int64_t)dpb * ns));
+            dout.resize(std::max<int64_t>(1, 16 * nt));
+            if (nt) HIP_CHECK(hipMemcpyAsync(dt.data(), t_xyz, sizeof(double) * 3 * nt, hipMemcpyHostToDevice, C->stream));
+            if (ns) HIP_CHECK(hipMemcpyAsync(ds.data(), s_buf, sizeof(double) * dpb * ns, hipMemcpyHostToDevice, C->stream));
+            if (nt) HIP_CHECK(hipMemcpyAsync(dout.data(), out, sizeof(double) * 16 * nt, hipMemcpyHostToDevice, C->stream));
+            run_direct_all(*C, kernel, nt, dt.data(), ns, ds.data(), dout.data(), PS, GS, HS, precision);
+            if (nt) HIP_CHECK(hipMemcpyAsync(out, dout.data(), sizeof(double) * 16 * nt, hipMemcpyDeviceToHost, C->stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
+}  // extern "C"

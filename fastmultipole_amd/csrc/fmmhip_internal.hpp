@@ -1,0 +1,153 @@
+// Internal declarations of libfmmhip (MI355X / gfx950). Host orchestration and
+// device kernels share these structures; nothing here crosses the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "../../include/fmmhip.h"
+
+namespace fmmhip {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+    fmmhip_status status;
+    Error(fmmhip_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            throw ::fmmhip::Error(_e == hipErrorOutOfMemory ? FMMHIP_ENOMEM : FMMHIP_EDEVICE,    \
+                                  std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+// -------------------------------------------------------- device buffers
+// Grow-only device allocation owned by an evaluation; contents survive calls.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    void resize(size_t count) {
+        n = count;
+        if (count <= cap) return;
+        if (p) HIP_CHECK(hipFree(p));
+        p = nullptr;
+        size_t c = count + count / 8 + 64;
+        HIP_CHECK(hipMalloc(&p, c * sizeof(T)));
+        cap = c;
+    }
+    T* data() const { return p; }
+    size_t size() const { return n; }
+};
+
+// ---------------------------------------------------------------- branch
+// Device record of a tree branch (Branch, src/containers.jl:150-164), 0-based.
+struct Branch {
+    int64_t n_bodies, first, n_branches, child_first, parent, leaf;
+    double source_center[3], target_center[3];
+    double source_radius, target_radius;
+    double source_box[3], target_box[3];
+    double max_influence;
+    double pad;
+};
+
+struct Tree {
+    int64_t n = 0;              // bodies
+    int rows = 0;               // data rows per body (16 target / data_per_body source)
+    int64_t n_branches = 0;
+    int depth = 0;
+    std::vector<int64_t> level_first, level_last;   // host copy of levels_index
+    std::vector<int64_t> leaf_index_h;               // host copy of leaf indices (lazy)
+    int64_t n_leaves = 0;
+    DBuf<Branch> branches;
+    DBuf<int64_t> sort_index;        // sorted -> original
+    DBuf<int64_t> inverse_sort;      // original -> sorted
+    DBuf<int64_t> leaf_index;        // leaf k -> branch
+    DBuf<double> x, y, z;            // sorted positions (SoA)
+    DBuf<double> radius;             // sorted body radius (source trees)
+    DBuf<double> strength;           // sorted strengths, SoA [dim][n]
+    int strength_dims = 0;
+    DBuf<double> expansions;         // [branch][H][2][2] (reference layout per branch)
+    int p = 0;
+};
+
+struct Lists {
+    int64_t n_m2l = 0, n_direct = 0;
+    DBuf<int32_t> m2l_t, m2l_s, dir_t, dir_s;        // sorted by target, DFS order within target
+    DBuf<int32_t> m2l_order;                         // expansion order used per M2L
+    // CSR over target branches for the direct list: unique targets and offsets
+    int64_t n_direct_targets = 0;
+    DBuf<int32_t> dir_targets;                       // unique target branches in the direct list
+    DBuf<int64_t> dir_offsets;                       // n_direct_targets + 1
+    int64_t n_m2l_targets = 0;
+    DBuf<int32_t> m2l_targets;
+    DBuf<int64_t> m2l_offsets;
+};
+
+struct Tables;   // device constant tables (rotation normalisations), expansions.hip
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    Tables* tables = nullptr;
+    DBuf<unsigned char> temp;   // scratch for sorts/scans
+    hipEvent_t ev[8];
+};
+
+struct Eval {
+    Context* ctx = nullptr;
+    int kernel = 0;
+    int64_t nt = 0, ns = 0;
+    DBuf<double> t_xyz_in, s_buf_in;          // host-copied inputs (when on_device == 0)
+    const double* t_xyz = nullptr;            // device pointer to 3 x nt
+    const double* s_buf = nullptr;            // device pointer to dpb x ns
+    Tree tt, st;
+    Lists lists;
+    DBuf<double> tout;                        // sorted target outputs, SoA [13][nt]: phi, v(3), h(9)
+    DBuf<double> tout_aos;                    // 16 x nt original order
+    bool trees_built = false, lists_built = false, ran = false;
+    int32_t pmax = 0, error_success = 1;
+    int p_built = -1;
+};
+
+inline int data_per_body(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 5 : 7; }
+inline int strength_dims(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 1 : 3; }
+inline int n_harmonics(int p) { return ((p + 1) * (p + 2)) / 2; }
+
+// ---------------------------------------------------------- module entry
+// tree.hip
+void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, bool target, int64_t leaf_size,
+                int method, bool shrink, int p, int strength_dims);
+// lists.hip
+void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t leaf_size_source, double mac,
+                 int method, bool ff, bool nf, bool si);
+// p2p.hip
+void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
+             bool PS, bool GS, bool HS, int precision);
+void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
+                    double* tout_aos, bool PS, bool GS, bool HS, int precision);
+// expansions.hip
+void ensure_tables(Context& C, int p);
+void free_tables(Tables* t);
+void run_p2m(Context& C, int kernel, Tree& st, int p);
+void run_m2m(Context& C, Tree& st, int p, bool LH);
+void run_m2l(Context& C, const Tree& tt, Tree& st_src, Tree& tt_dst, Lists& L, int p, bool LH, int err_kind,
+             bool err_rel, double eps, bool BE, int precision, int32_t* pmax, int32_t* success);
+void run_l2l(Context& C, Tree& tt, int p, bool LH);
+void run_l2p(Context& C, Tree& tt, double* tout, int64_t nt, int p, bool LH, bool PS, bool GS, bool HS);
+// api.hip helpers
+void gather_outputs(Context& C, const Tree& tt, const double* tout, double* aos, int64_t nt);
+
+}  // namespace fmmhip

@@ -1,0 +1,537 @@
+// Octree construction on the GPU, bit-exact with FastMultipole's Tree()
+// (src/tree.jl:3-122, 346-463, 581-592, 608-743, 845-1188).
+//
+// The reference partitions each node's bodies stably into octants, level by
+// level, so the final body order is: lexicographic by the octant path down to
+// the body's leaf, then by original index. Here every body gets its full octant
+// path (up to 42 levels, two 63-bit words) in one pass, computed with exactly
+// the reference's child-centre arithmetic (centre +/- box*0.5, box halved per
+// level). A stable radix sort by path groups bodies by node at every depth; the
+// node structure is then read off the sorted keys level by level with binary
+// searches, using the reference's leaf test on the node counts. A second sort
+// by (leaf, original index) restores the reference's in-leaf order.
+//
+// This file is compiled with -ffp-contract=off: every floating-point result
+// here (centres, boxes, radii, the leaf test, the MAC) must match the
+// reference's IEEE double arithmetic bit for bit.
+#include "fmmhip_internal.hpp"
+#include <cstring>   // host memset for rocprim headers
+#include <rocprim/rocprim.hpp>
+#include <algorithm>
+#include <cmath>
+
+namespace fmmhip {
+
+static constexpr int LEVELS_PER_WORD = 21;
+static constexpr int MAX_LEVELS = 2 * LEVELS_PER_WORD;
+
+// Julia's min/max on Float64 (order -0.0 < 0.0; inputs are never NaN here)
+__device__ __host__ inline double jmin(double a, double b) { return (b < a || (b == a && std::signbit(b))) ? b : a; }
+__device__ __host__ inline double jmax(double a, double b) { return (b > a || (b == a && !std::signbit(b))) ? b : a; }
+
+__device__ inline int digit_at(uint64_t hi, uint64_t lo, int level) {
+    return level < LEVELS_PER_WORD ? (int)((hi >> (3 * (LEVELS_PER_WORD - 1 - level))) & 7u)
+                                   : (int)((lo >> (3 * (MAX_LEVELS - 1 - level))) & 7u);
+}
+
+// exceeds (src/tree.jl:492-569), single system
+__device__ __host__ inline bool exceeds(int64_t n, int64_t ls, int method) {
+    if (method == FMMHIP_BARBA) return n > ls;
+    double fraction = 0.0;
+    fraction += (double)n / (double)(ls * ls);
+    fraction *= (double)ls;
+    if (method == FMMHIP_SELFTUNING) return fraction > 0.5 && n > 1;
+    return fraction > 2.8284271247461903 && n > 1;
+}
+
+// ------------------------------------------------------------- bounding box
+__global__ void k_bbox_partial(const double* __restrict__ buf, int rows, int64_t n, double* __restrict__ part) {
+    __shared__ double s[6][256];
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) { mn[d] = buf[d]; mx[d] = buf[d]; }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            double v = buf[i * rows + d];
+            mn[d] = jmin(mn[d], v);
+            mx[d] = jmax(mx[d], v);
+        }
+    for (int d = 0; d < 3; ++d) { s[d][threadIdx.x] = mn[d]; s[3 + d][threadIdx.x] = mx[d]; }
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] = jmin(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] = jmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 6; ++d) part[blockIdx.x * 6 + d] = s[d][0];
+}
+
+// ---------------------------------------------------------------- path keys
+__global__ void k_path_keys(const double* __restrict__ buf, int rows, int64_t n, double c0, double c1, double c2,
+                            double bmax, uint64_t* __restrict__ khi, uint64_t* __restrict__ klo,
+                            uint32_t* __restrict__ idx) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = buf[i * rows + 0], y = buf[i * rows + 1], z = buf[i * rows + 2];
+    double cx = c0, cy = c1, cz = c2, box = bmax;
+    uint64_t w[2] = {0, 0};
+    for (int word = 0; word < 2; ++word)
+        for (int l = 0; l < LEVELS_PER_WORD; ++l) {
+            int o = (x > cx ? 1 : 0) | (y > cy ? 2 : 0) | (z > cz ? 4 : 0);   // get_octant - 1
+            w[word] = (w[word] << 3) | (uint64_t)o;
+            double delta = box * 0.5;                                       // get_child_center
+            cx = cx + ((o & 1) ? delta : -delta);
+            cy = cy + ((o & 2) ? delta : -delta);
+            cz = cz + ((o & 4) ? delta : -delta);
+            box = box * 0.5;                                                // child_box = box * 0.5
+        }
+    khi[i] = w[0];
+    klo[i] = w[1];
+    idx[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx, int64_t n,
+                             uint64_t* __restrict__ dst) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// ------------------------------------------------------ structure per level
+// one thread per (active node, octant): body sub-range with that digit
+__global__ void k_split(const uint64_t* __restrict__ khi, const uint64_t* __restrict__ klo, int level, int64_t n_active,
+                        const int64_t* __restrict__ act_first, const int64_t* __restrict__ act_count,
+                        int64_t* __restrict__ sub_first, int64_t* __restrict__ sub_count, int32_t* __restrict__ nonempty) {
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n_active * 8) return;
+    int64_t a = t >> 3;
+    int o = (int)(t & 7);
+    int64_t f = act_first[a], e = f + act_count[a];
+    auto lower = [&](int d) {   // first position in [f,e) whose digit >= d
+        int64_t lo = f, hi = e;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (digit_at(khi[mid], klo[mid], level) < d) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    int64_t s0 = lower(o), s1 = (o == 7) ? e : lower(o + 1);
+    sub_first[t] = s0;
+    sub_count[t] = s1 - s0;
+    nonempty[t] = (s1 > s0) ? 1 : 0;
+}
+
+// child branches (child_branches!, src/tree.jl:346-376 and branch!, :433-463)
+__global__ void k_make_children(int64_t n_active, const int64_t* __restrict__ act_branch, const int64_t* __restrict__ sub_first,
+                                const int64_t* __restrict__ sub_count, const int32_t* __restrict__ rank,
+                                int64_t child_base, Branch* __restrict__ br, int64_t ls, int method,
+                                int32_t* __restrict__ split_flag) {
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n_active * 8) return;
+    int64_t cnt = sub_count[t];
+    if (cnt == 0) return;
+    int64_t a = t >> 3;
+    int o = (int)(t & 7);
+    const Branch& P = br[act_branch[a]];
+    int64_t ci = child_base + rank[t];
+    Branch c;
+    c.n_bodies = cnt;
+    c.first = sub_first[t];
+    c.n_branches = 0;
+    c.child_first = -1;
+    c.parent = act_branch[a];
+    c.leaf = -1;
+    const double child_radius = P.target_radius * 0.5;
+    for (int d = 0; d < 3; ++d) {
+        double delta = P.target_box[d] * 0.5;
+        double cc = P.target_center[d] + (((o >> d) & 1) ? delta : -delta);
+        c.source_center[d] = cc;
+        c.target_center[d] = cc;
+        c.source_box[d] = P.target_box[d] * 0.5;
+        c.target_box[d] = P.target_box[d] * 0.5;
+    }
+    c.source_radius = child_radius;
+    c.target_radius = child_radius;
+    c.max_influence = 0.0;
+    c.pad = 0.0;
+    br[ci] = c;
+    split_flag[rank[t]] = exceeds(cnt, ls, method) ? 1 : 0;
+}
+
+// parents' child ranges: first nonempty octant's child index, number of children
+__global__ void k_link_parents(int64_t n_active, const int64_t* __restrict__ act_branch, const int32_t* __restrict__ nonempty,
+                               const int32_t* __restrict__ rank, int64_t child_base, Branch* __restrict__ br) {
+    int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (a >= n_active) return;
+    int nb = 0;
+    int64_t first = -1;
+    for (int o = 0; o < 8; ++o)
+        if (nonempty[a * 8 + o]) {
+            if (first < 0) first = child_base + rank[a * 8 + o];
+            ++nb;
+        }
+    Branch& P = br[act_branch[a]];
+    P.n_branches = nb;
+    P.child_first = first;
+}
+
+// next active list from split flags
+__global__ void k_next_active(int64_t n_children, const int32_t* __restrict__ split_flag, const int32_t* __restrict__ srank,
+                              int64_t child_base, const Branch* __restrict__ br, int64_t* __restrict__ act_branch,
+                              int64_t* __restrict__ act_first, int64_t* __restrict__ act_count) {
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= n_children || !split_flag[c]) return;
+    int64_t k = srank[c];
+    const Branch& B = br[child_base + c];
+    act_branch[k] = child_base + c;
+    act_first[k] = B.first;
+    act_count[k] = B.n_bodies;
+}
+
+// ------------------------------------------------ leaves, final body order
+__global__ void k_leaf_flags(const Branch* __restrict__ br, int64_t nb, int32_t* __restrict__ flag) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < nb) flag[i] = br[i].n_branches == 0 ? 1 : 0;
+}
+
+__global__ void k_leaf_number(Branch* __restrict__ br, int64_t nb, const int32_t* __restrict__ flag,
+                              const int32_t* __restrict__ rank, int64_t* __restrict__ leaf_index) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nb || !flag[i]) return;
+    br[i].leaf = rank[i];
+    leaf_index[rank[i]] = i;
+}
+
+// key (leaf first body << 32 | original index): sorting it keeps leaves in place and
+// orders each leaf's bodies by original index
+__global__ void k_leaf_keys(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index, int64_t n_leaves,
+                            const uint32_t* __restrict__ idx_sorted, uint64_t* __restrict__ key2) {
+    int64_t l = blockIdx.x;
+    if (l >= n_leaves) return;
+    const Branch& B = br[leaf_index[l]];
+    for (int64_t i = B.first + threadIdx.x; i < B.first + B.n_bodies; i += blockDim.x)
+        key2[i] = ((uint64_t)B.first << 32) | (uint64_t)idx_sorted[i];
+}
+
+__global__ void k_finish_sort(const uint64_t* __restrict__ key2, int64_t n, int64_t* __restrict__ sort_index,
+                              int64_t* __restrict__ inverse) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t o = (int64_t)(key2[i] & 0xffffffffull);
+    sort_index[i] = o;
+    inverse[o] = i;
+}
+
+__global__ void k_gather_bodies(const double* __restrict__ buf, int rows, int64_t n, const int64_t* __restrict__ sort_index,
+                                bool target, int sdims, double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+                                double* __restrict__ radius, double* __restrict__ strength) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* b = buf + sort_index[i] * rows;
+    x[i] = b[0];
+    y[i] = b[1];
+    z[i] = b[2];
+    if (!target) {
+        radius[i] = b[3];
+        for (int d = 0; d < sdims; ++d) strength[d * n + i] = b[4 + d];
+    }
+}
+
+// ---------------------------------------------------------- shrink_recenter!
+// one thread per branch of a level, deepest level first (src/tree.jl:1141-1188)
+__global__ void k_shrink_level(Branch* __restrict__ br, int64_t b0, int64_t b1, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ radius,
+                               bool use_radius) {
+    int64_t ib = b0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (ib > b1) return;
+    Branch& b = br[ib];
+    double tc[3], tb[3], sc[3], sbx[3], tr = 0.0, sr = 0.0;
+    if (b.n_branches == 0) {
+        int64_t f = b.first, e = b.first + b.n_bodies;
+        double mn[3] = {x[f], y[f], z[f]}, mx[3] = {x[f], y[f], z[f]};
+        double smn[3] = {x[f], y[f], z[f]}, smx[3] = {x[f], y[f], z[f]};
+        for (int64_t i = f; i < e; ++i) {
+            double r = use_radius ? radius[i] : 0.0;
+            double p[3] = {x[i], y[i], z[i]};
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = jmin(mn[d], p[d]);
+                mx[d] = jmax(mx[d], p[d]);
+                smn[d] = jmin(smn[d], p[d] - r);
+                smx[d] = jmax(smx[d], p[d] + r);
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            tc[d] = (mx[d] + mn[d]) * 0.5;
+            tb[d] = mx[d] - tc[d];
+            sc[d] = (smx[d] + smn[d]) * 0.5;
+            sbx[d] = smx[d] - sc[d];
+        }
+        for (int64_t i = f; i < e; ++i) {
+            double r = use_radius ? radius[i] : 0.0;
+            double dx = x[i] - tc[0], dy = y[i] - tc[1], dz = z[i] - tc[2];
+            tr = jmax(tr, sqrt(dx * dx + dy * dy + dz * dz));
+            dx = x[i] - sc[0]; dy = y[i] - sc[1]; dz = z[i] - sc[2];
+            sr = jmax(sr, sqrt(dx * dx + dy * dy + dz * dz) + r);
+        }
+    } else {
+        const Branch& f = br[b.child_first];
+        double mn[3], mx[3], smn[3], smx[3];
+        for (int d = 0; d < 3; ++d) { mn[d] = mx[d] = f.target_center[d]; smn[d] = smx[d] = f.source_center[d]; }
+        for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c) {
+            const Branch& ch = br[c];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = jmin(mn[d], ch.target_center[d] - ch.target_box[d]);
+                mx[d] = jmax(mx[d], ch.target_center[d] + ch.target_box[d]);
+                smn[d] = jmin(smn[d], ch.source_center[d] - ch.source_box[d]);
+                smx[d] = jmax(smx[d], ch.source_center[d] + ch.source_box[d]);
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            tc[d] = (mn[d] + mx[d]) * 0.5;
+            tb[d] = mx[d] - tc[d];
+            sc[d] = (smn[d] + smx[d]) * 0.5;
+            sbx[d] = smx[d] - sc[d];
+        }
+        for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c) {
+            const Branch& ch = br[c];
+            double dx = ch.target_center[0] - tc[0], dy = ch.target_center[1] - tc[1], dz = ch.target_center[2] - tc[2];
+            tr = jmax(tr, sqrt(dx * dx + dy * dy + dz * dz) + ch.target_radius);
+            dx = ch.source_center[0] - sc[0]; dy = ch.source_center[1] - sc[1]; dz = ch.source_center[2] - sc[2];
+            sr = jmax(sr, sqrt(dx * dx + dy * dy + dz * dz) + ch.source_radius);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        b.target_center[d] = tc[d];
+        b.target_box[d] = tb[d];
+        b.source_center[d] = sc[d];
+        b.source_box[d] = sbx[d];
+    }
+    b.target_radius = tr;
+    b.source_radius = sr;
+}
+
+__global__ void k_init_root(Branch r, Branch* __restrict__ br, int64_t* __restrict__ act_branch, int64_t* __restrict__ act_first,
+                            int64_t* __restrict__ act_count, int64_t n) {
+    br[0] = r;
+    act_branch[0] = 0;
+    act_first[0] = 0;
+    act_count[0] = n;
+}
+
+// ------------------------------------------------------------ host helpers
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)std::max<int64_t>(1, (n + block - 1) / block); }
+
+template <class T>
+static void exclusive_scan_i32(Context& C, const T* in, T* out, int64_t n) {
+    size_t bytes = 0;
+    HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), C.stream));
+    C.temp.resize(bytes + 16);
+    HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), C.stream));
+}
+
+static void sort_pairs_u64(Context& C, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, int64_t n) {
+    size_t bytes = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, 63, C.stream));
+    C.temp.resize(bytes + 16);
+    HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, kin, kout, vin, vout, (size_t)n, 0, 63, C.stream));
+}
+
+static void sort_keys_u64(Context& C, const uint64_t* kin, uint64_t* kout, int64_t n) {
+    size_t bytes = 0;
+    HIP_CHECK(rocprim::radix_sort_keys(nullptr, bytes, kin, kout, (size_t)n, 0, 64, C.stream));
+    C.temp.resize(bytes + 16);
+    HIP_CHECK(rocprim::radix_sort_keys(C.temp.data(), bytes, kin, kout, (size_t)n, 0, 64, C.stream));
+}
+
+template <class T>
+static T read_back(Context& C, const T* dptr) {
+    T v;
+    HIP_CHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, C.stream));
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+    return v;
+}
+
+struct TreeScratch {
+    DBuf<uint64_t> khi, klo, khi_s, klo_s, key2, key2_s;
+    DBuf<uint32_t> idx, idx_s;
+    DBuf<int64_t> act_branch, act_first, act_count, sub_first, sub_count;
+    DBuf<int32_t> nonempty, rank, split, srank, flag, frank;
+    DBuf<double> part;
+};
+
+static TreeScratch& scratch() {
+    // one build at a time per process; intentionally never freed so that no device
+    // memory is released after the HIP runtime has shut down at process exit
+    static TreeScratch* s = new TreeScratch();
+    return *s;
+}
+
+// Level-by-level structure from sorted keys; returns false if deeper levels than
+// max_level would be needed.
+static bool build_structure(Context& C, Tree& T, TreeScratch& S, double center[3], double bmax, double radius,
+                            int64_t ls, int method, int max_level) {
+    const int64_t n = T.n;
+    Branch r;
+    r.n_bodies = n; r.first = 0; r.n_branches = 0; r.child_first = -1; r.parent = -1; r.leaf = -1;
+    for (int d = 0; d < 3; ++d) {
+        r.source_center[d] = r.target_center[d] = center[d];
+        r.source_box[d] = r.target_box[d] = bmax;
+    }
+    r.source_radius = r.target_radius = radius;
+    r.max_influence = 0.0; r.pad = 0.0;
+    // capacity: every split node adds at most 8 children; grow on demand
+    size_t cap = std::max<size_t>(1024, (size_t)(2 * n / std::max<int64_t>(1, ls) + 64));
+    T.branches.resize(cap);
+    T.level_first.assign(1, 0);
+    T.level_last.assign(1, 0);
+    int64_t n_branches = 1;
+    int64_t n_active = exceeds(n, ls, method) ? 1 : 0;
+    S.act_branch.resize(1); S.act_first.resize(1); S.act_count.resize(1);
+    k_init_root<<<1, 1, 0, C.stream>>>(r, T.branches.data(), S.act_branch.data(), S.act_first.data(), S.act_count.data(), n);
+    LAUNCH_CHECK();
+    int level = 0;
+    while (n_active > 0) {
+        if (level >= max_level) return false;
+        const int64_t m = n_active * 8;
+        S.sub_first.resize(m); S.sub_count.resize(m); S.nonempty.resize(m); S.rank.resize(m + 1);
+        k_split<<<grid_for(m, 256), 256, 0, C.stream>>>(S.khi_s.data(), S.klo_s.data(), level, n_active, S.act_first.data(),
+                                                        S.act_count.data(), S.sub_first.data(), S.sub_count.data(), S.nonempty.data());
+        LAUNCH_CHECK();
+        exclusive_scan_i32(C, S.nonempty.data(), S.rank.data(), m);
+        int32_t last_rank = read_back(C, S.rank.data() + (m - 1));
+        int32_t last_ne = read_back(C, S.nonempty.data() + (m - 1));
+        const int64_t n_children = (int64_t)last_rank + last_ne;
+        const int64_t child_base = n_branches;
+        if ((size_t)(n_branches + n_children) > T.branches.cap) {
+            DBuf<Branch> bigger;
+            bigger.resize(2 * (n_branches + n_children));
+            HIP_CHECK(hipMemcpyAsync(bigger.data(), T.branches.data(), n_branches * sizeof(Branch), hipMemcpyDeviceToDevice, C.stream));
+            std::swap(bigger.p, T.branches.p);
+            std::swap(bigger.cap, T.branches.cap);
+        }
+        S.split.resize(n_children + 1); S.srank.resize(n_children + 1);
+        k_make_children<<<grid_for(m, 256), 256, 0, C.stream>>>(n_active, S.act_branch.data(), S.sub_first.data(),
+                                                                S.sub_count.data(), S.rank.data(), child_base, T.branches.data(),
+                                                                ls, method, S.split.data());
+        LAUNCH_CHECK();
+        k_link_parents<<<grid_for(n_active, 256), 256, 0, C.stream>>>(n_active, S.act_branch.data(), S.nonempty.data(),
+                                                                      S.rank.data(), child_base, T.branches.data());
+        LAUNCH_CHECK();
+        exclusive_scan_i32(C, S.split.data(), S.srank.data(), n_children);
+        int32_t ls_rank = read_back(C, S.srank.data() + (n_children - 1));
+        int32_t ls_flag = read_back(C, S.split.data() + (n_children - 1));
+        const int64_t next_active = (int64_t)ls_rank + ls_flag;
+        S.act_branch.resize(std::max<int64_t>(1, next_active));
+        S.act_first.resize(std::max<int64_t>(1, next_active));
+        S.act_count.resize(std::max<int64_t>(1, next_active));
+        k_next_active<<<grid_for(n_children, 256), 256, 0, C.stream>>>(n_children, S.split.data(), S.srank.data(), child_base,
+                                                                      T.branches.data(), S.act_branch.data(), S.act_first.data(),
+                                                                      S.act_count.data());
+        LAUNCH_CHECK();
+        n_branches += n_children;
+        T.level_first.push_back(child_base);
+        T.level_last.push_back(child_base + n_children - 1);
+        n_active = next_active;
+        ++level;
+    }
+    T.n_branches = n_branches;
+    T.branches.n = n_branches;
+    T.depth = level;
+    return true;
+}
+
+void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, bool target, int64_t ls, int method,
+                bool shrink, int p, int sdims) {
+    T.n = n;
+    T.rows = rows;
+    T.p = p;
+    T.strength_dims = target ? 0 : sdims;
+    T.n_branches = 0;
+    T.level_first.clear();
+    T.level_last.clear();
+    T.leaf_index_h.clear();
+    if (n == 0) return;
+    if (n >= (int64_t)1 << 32) throw Error(FMMHIP_EINVAL, "more than 2^32 bodies in one system");
+    if (ls < 1) throw Error(FMMHIP_EINVAL, "leaf size must be positive");
+    TreeScratch& S = scratch();
+    // root cube (center_box, src/tree.jl:894-907; :20-26)
+    const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    S.part.resize(6 * nblk);
+    k_bbox_partial<<<nblk, 256, 0, C.stream>>>(d_buf, rows, n, S.part.data());
+    LAUNCH_CHECK();
+    std::vector<double> part(6 * nblk);
+    HIP_CHECK(hipMemcpyAsync(part.data(), S.part.data(), part.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+    double mn[3] = {part[0], part[1], part[2]}, mx[3] = {part[3], part[4], part[5]};
+    for (int b = 1; b < nblk; ++b)
+        for (int d = 0; d < 3; ++d) { mn[d] = jmin(mn[d], part[6 * b + d]); mx[d] = jmax(mx[d], part[6 * b + 3 + d]); }
+    double center[3], box[3];
+    for (int d = 0; d < 3; ++d) { center[d] = (mx[d] + mn[d]) * 0.5; box[d] = mx[d] - center[d]; }
+    const double bmax = std::max(std::max(box[0], box[1]), box[2]);
+    const double radius = std::sqrt(bmax * bmax * 3.0);
+    // full octant paths, then a stable sort by the first 21 levels
+    S.khi.resize(n); S.klo.resize(n); S.idx.resize(n); S.khi_s.resize(n); S.klo_s.resize(n); S.idx_s.resize(n);
+    k_path_keys<<<grid_for(n, 256), 256, 0, C.stream>>>(d_buf, rows, n, center[0], center[1], center[2], bmax, S.khi.data(),
+                                                       S.klo.data(), S.idx.data());
+    LAUNCH_CHECK();
+    sort_pairs_u64(C, S.khi.data(), S.khi_s.data(), S.idx.data(), S.idx_s.data(), n);
+    k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.klo.data(), S.idx_s.data(), n, S.klo_s.data());
+    LAUNCH_CHECK();
+    if (!build_structure(C, T, S, center, bmax, radius, ls, method, LEVELS_PER_WORD)) {
+        // deeper than 21 levels: order by the second word first, then stably by the first
+        DBuf<uint64_t> tmpk;
+        DBuf<uint32_t> tmpi;
+        tmpk.resize(n); tmpi.resize(n);
+        sort_pairs_u64(C, S.klo.data(), tmpk.data(), S.idx.data(), tmpi.data(), n);
+        k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.khi.data(), tmpi.data(), n, S.khi_s.data());
+        LAUNCH_CHECK();
+        // khi_s now holds hi keys in lo-sorted order; stable sort by hi carrying the index
+        DBuf<uint64_t> hi2;
+        hi2.resize(n);
+        sort_pairs_u64(C, S.khi_s.data(), hi2.data(), tmpi.data(), S.idx_s.data(), n);
+        HIP_CHECK(hipMemcpyAsync(S.khi_s.data(), hi2.data(), n * sizeof(uint64_t), hipMemcpyDeviceToDevice, C.stream));
+        k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.klo.data(), S.idx_s.data(), n, S.klo_s.data());
+        LAUNCH_CHECK();
+        if (!build_structure(C, T, S, center, bmax, radius, ls, method, 2 * LEVELS_PER_WORD))
+            throw Error(FMMHIP_EDEPTH, "octree deeper than 42 levels (coincident bodies?)");
+    }
+    const int64_t nb = T.n_branches;
+    // leaf numbering in branch order (src/tree.jl:93-100)
+    S.flag.resize(nb + 1); S.frank.resize(nb + 1);
+    k_leaf_flags<<<grid_for(nb, 256), 256, 0, C.stream>>>(T.branches.data(), nb, S.flag.data());
+    LAUNCH_CHECK();
+    exclusive_scan_i32(C, S.flag.data(), S.frank.data(), nb);
+    T.n_leaves = (int64_t)read_back(C, S.frank.data() + (nb - 1)) + read_back(C, S.flag.data() + (nb - 1));
+    T.leaf_index.resize(T.n_leaves);
+    k_leaf_number<<<grid_for(nb, 256), 256, 0, C.stream>>>(T.branches.data(), nb, S.flag.data(), S.frank.data(), T.leaf_index.data());
+    LAUNCH_CHECK();
+    // in-leaf order = original index order
+    S.key2.resize(n); S.key2_s.resize(n);
+    k_leaf_keys<<<(unsigned)T.n_leaves, 256, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves, S.idx_s.data(),
+                                                           S.key2.data());
+    LAUNCH_CHECK();
+    sort_keys_u64(C, S.key2.data(), S.key2_s.data(), n);
+    T.sort_index.resize(n); T.inverse_sort.resize(n);
+    k_finish_sort<<<grid_for(n, 256), 256, 0, C.stream>>>(S.key2_s.data(), n, T.sort_index.data(), T.inverse_sort.data());
+    LAUNCH_CHECK();
+    // sorted body data (target_to_buffer! / system_to_buffer!, src/compatibility.jl:340-399)
+    T.x.resize(n); T.y.resize(n); T.z.resize(n);
+    if (!target) { T.radius.resize(n); T.strength.resize((size_t)sdims * n); }
+    k_gather_bodies<<<grid_for(n, 256), 256, 0, C.stream>>>(d_buf, rows, n, T.sort_index.data(), target, sdims, T.x.data(),
+                                                           T.y.data(), T.z.data(), target ? nullptr : T.radius.data(),
+                                                           target ? nullptr : T.strength.data());
+    LAUNCH_CHECK();
+    if (shrink)
+        for (int64_t l = (int64_t)T.level_first.size() - 1; l >= 0; --l) {
+            int64_t b0 = T.level_first[l], b1 = T.level_last[l];
+            k_shrink_level<<<grid_for(b1 - b0 + 1, 128), 128, 0, C.stream>>>(T.branches.data(), b0, b1, T.x.data(), T.y.data(),
+                                                                             T.z.data(), target ? nullptr : T.radius.data(),
+                                                                             !target);
+            LAUNCH_CHECK();
+        }
+    T.expansions.resize((size_t)nb * 4 * n_harmonics(p));
+}
+
+}  // namespace fmmhip

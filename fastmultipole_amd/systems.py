@@ -1,0 +1,107 @@
+"""User systems with the built-in point kernels, mirroring the reference's test systems.
+
+Gravitational follows test/gravitational.jl (Point{Source}, data_per_body 5,
+buffer rows [x, y, z, radius, strength]); VortexParticles follows
+test/vortex.jl (Point{Vortex}, data_per_body 7, rows [x, y, z, sigma,
+Gamma_x, Gamma_y, Gamma_z]). Each implements the reference's user-system
+interface (src/compatibility.jl): source_system_to_buffer!, data_per_body,
+get_position, strength_dims, get_n_bodies, has_vector_potential,
+buffer_to_target_system! and the body_to_multipole! kernel tag (fmm_kernel).
+"""
+import numpy as np
+
+from . import _lib as L
+
+
+class Gravitational:
+    """test/gravitational.jl:16-119. bodies: (N, >=5) rows [x, y, z, radius, strength, ...]."""
+
+    fmm_kernel = L.POINT_SOURCE       # body_to_multipole!(::Gravitational) = Point{Source} (:74)
+
+    def __init__(self, bodies):
+        b = np.asarray(bodies, dtype=np.float64)
+        self.position = np.ascontiguousarray(b[:, 0:3])
+        self.radius = np.ascontiguousarray(b[:, 3])
+        self.strength = np.ascontiguousarray(b[:, 4])
+        self.potential = np.zeros((b.shape[0], 16))   # columns 0 phi, 4:7 gradient, 7:16 hessian (reference rows 1..16)
+
+    # --- user-system interface (src/compatibility.jl:21-286)
+    def get_n_bodies(self):
+        return self.position.shape[0]
+
+    def data_per_body(self):
+        return 5
+
+    def strength_dims(self):
+        return 1
+
+    def has_vector_potential(self):
+        return False
+
+    def get_position(self, i):
+        return self.position[i]
+
+    def positions(self):
+        return self.position
+
+    def source_buffer(self):
+        """All bodies through source_system_to_buffer! (:50-54), as an (N, 5) array."""
+        return np.column_stack([self.position, self.radius, self.strength])
+
+    def buffer_to_target_system(self, target_buffer, switch):
+        """buffer_to_target_system! (:106-119): assigns phi, gradient and hessian (zero when switched off)."""
+        n = self.get_n_bodies()
+        self.potential[:, 0] = target_buffer[:n, 3] if switch.scalar_potential else 0.0
+        self.potential[:, 4:7] = target_buffer[:n, 4:7] if switch.gradient else 0.0
+        self.potential[:, 7:16] = target_buffer[:n, 7:16] if switch.hessian else 0.0
+
+    def reset(self):
+        self.potential[:] = 0.0
+
+
+class VortexParticles:
+    """test/vortex.jl:20-147. position (N,3), strength (N,3), sigma (N,)."""
+
+    fmm_kernel = L.POINT_VORTEX       # body_to_multipole!(::VortexParticles) = Point{Vortex} (:78)
+
+    def __init__(self, position, strength, sigma=None):
+        self.position = np.ascontiguousarray(position, dtype=np.float64)
+        self.strength = np.ascontiguousarray(strength, dtype=np.float64)
+        n = self.position.shape[0]
+        self.sigma = np.zeros(n) if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        self.potential = np.zeros((n, 13))            # reference: potential rows 5:13 hold the velocity gradient
+        self.gradient_stretching = np.zeros((n, 6))   # rows 1:3 velocity
+
+    def get_n_bodies(self):
+        return self.position.shape[0]
+
+    def data_per_body(self):
+        return 7
+
+    def strength_dims(self):
+        return 3
+
+    def has_vector_potential(self):
+        return True
+
+    def get_position(self, i):
+        return self.position[i]
+
+    def positions(self):
+        return self.position
+
+    def source_buffer(self):
+        """source_system_to_buffer! (:56-60): rows [x, y, z, sigma, Gamma]."""
+        return np.column_stack([self.position, self.sigma, self.strength])
+
+    def buffer_to_target_system(self, target_buffer, switch):
+        """buffer_to_target_system! (:130-143): accumulates velocity and velocity gradient."""
+        n = self.get_n_bodies()
+        if switch.gradient:
+            self.gradient_stretching[:, 0:3] += target_buffer[:n, 4:7]
+        if switch.hessian:
+            self.potential[:, 4:13] += target_buffer[:n, 7:16]
+
+    def reset(self):
+        self.potential[:] = 0.0
+        self.gradient_stretching[:] = 0.0

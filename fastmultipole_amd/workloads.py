@@ -1,0 +1,75 @@
+"""Seeded synthetic particle sets for the benchmark configurations.
+
+All inputs are generated here (numpy PCG64), never from Julia's RNG, so the
+GPU path and the oracle consume identical data.
+"""
+import math
+
+import numpy as np
+
+
+def uniform_sources(n, seed=1, strength_scale=None):
+    """N point sources, positions U[0,1]^3, strengths U[0,1] (scaled by strength_scale if given).
+
+    Returns an (n, 5) source buffer [x, y, z, radius=0, q]."""
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 5))
+    b[:, :3] = rng.random((n, 3))
+    b[:, 4] = rng.random(n) * (1.0 if strength_scale is None else strength_scale)
+    return b
+
+
+def uniform_vortices(n, seed=1):
+    """test/vortex.jl:32-41 distribution: positions U[0,1]^3, strengths (2U-1-0.5)*2/n. (n, 7) buffer."""
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 7))
+    b[:, :3] = rng.random((n, 3))
+    b[:, 4:7] = (2.0 * rng.random((n, 3)) - 1.0 - 0.5) * 2.0 / n
+    return b
+
+
+def vortex_ring(n, seed=2, radius=1.0, core_sigma=0.1, noise=1e-3):
+    """Vortex ring of radius `radius`: angle U[0, 2pi), cross-section offsets N(0, sigma^2) in the
+    radial and axial directions (a Gaussian core), vector strength tangential with magnitude
+    2*pi*radius/n (unit total circulation) plus relative N(0,1) noise of size `noise` per component.
+    Particles are points (buffer radius 0) and interact through the singular vortex kernel of
+    test/vortex.jl. Returns an (n, 7) buffer [x, y, z, 0, Gamma]."""
+    rng = np.random.default_rng(seed)
+    phi = rng.random(n) * 2.0 * math.pi
+    a = rng.normal(0.0, core_sigma, n)
+    b = rng.normal(0.0, core_sigma, n)
+    rr = radius + a
+    out = np.zeros((n, 7))
+    out[:, 0] = rr * np.cos(phi)
+    out[:, 1] = rr * np.sin(phi)
+    out[:, 2] = b
+    g = 2.0 * math.pi * radius / n
+    t = np.stack([-np.sin(phi), np.cos(phi), np.zeros(n)], axis=1)
+    out[:, 4:7] = g * (t + noise * rng.normal(0.0, 1.0, (n, 3)))
+    return out
+
+
+def plummer_sources(n, seed=3, scale=1.0, rmax=20.0):
+    """Plummer sphere: r = a / sqrt(u^(-2/3) - 1) (rejected beyond rmax), isotropic direction, q ~ N(0,1)."""
+    rng = np.random.default_rng(seed)
+    r = np.empty(0)
+    while r.size < n:
+        u = rng.random(2 * n)
+        rs = scale / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+        r = np.concatenate([r, rs[rs <= rmax]])
+    r = r[:n]
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    b = np.zeros((n, 5))
+    b[:, :3] = v * r[:, None]
+    b[:, 4] = rng.normal(size=n)
+    return b
+
+
+def relative_error(v, v_ref):
+    """Per-target ||v - v_ref|| / ||v_ref|| (guarded by max ||v_ref||); returns (max, p99, median)."""
+    num = np.linalg.norm(v - v_ref, axis=1)
+    den = np.linalg.norm(v_ref, axis=1)
+    den = np.maximum(den, 1e-300 * max(den.max(), 1e-300))
+    e = num / den
+    return float(e.max()), float(np.percentile(e, 99)), float(np.median(e))

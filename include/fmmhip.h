@@ -1,0 +1,147 @@
+/*
+ * fmmhip.h - C ABI of the MI355X FMM evaluator (libfmmhip.so).
+ *
+ * This boundary replaces the FMM evaluation core of FastMultipole.jl:
+ *   - Tree(systems, target; ...)                     src/tree.jl:3-122
+ *   - build_interaction_lists + sort_by_target        src/interaction_list.jl:3-19, 183-213
+ *   - fmm!(targets, target_tree, sources, source_tree, ..., m2l_list, direct_list, ...)
+ *                                                     src/fmm.jl:914-1176
+ *   - direct!(targets, sources; ...)                  src/direct.jl:20-75
+ * for systems whose body_to_multipole! / direct! are the built-in point kernels
+ * Point{Source} (test/gravitational.jl:74-104) and Point{Vortex} (test/vortex.jl:78-128).
+ *
+ * Data layout across the ABI follows the reference's buffers (src/compatibility.jl:5-20,
+ * 155-172): column-major double matrices, i.e. one record per body.
+ *   target positions : 3 x N   [x, y, z]
+ *   source buffer    : data_per_body x N  [x, y, z, radius, strength...]
+ *                      (5 rows for POINT_SOURCE, 7 rows for POINT_VORTEX)
+ *   target results   : 16 x N  [x, y, z, phi, grad(3), hessian(9)] in the ORIGINAL body
+ *                      order (the reference's target buffer after unsorting through
+ *                      sort_index, src/compatibility.jl:324-338). Hessian entries are in
+ *                      the reference's linear order (vxx, vxy, vxz, vyx, ... ).
+ * Indices returned by the getters are 0-based; the reference's are these plus one.
+ *
+ * Errors never unwind across the ABI: every call returns an fmmhip_status and
+ * fmmhip_last_error() describes the last failure. Each context owns one HIP
+ * stream and is not thread-safe; use one context per host thread.
+ */
+#ifndef FMMHIP_H
+#define FMMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FMMHIP_OK = 0,
+    FMMHIP_EINVAL = 1,   /* bad argument (reference: @assert / ArgumentError) */
+    FMMHIP_ENOMEM = 2,   /* device allocation failed */
+    FMMHIP_EDEVICE = 3,  /* HIP runtime or kernel launch error, or no GPU */
+    FMMHIP_ETOL = 4,     /* error tolerance not met (reference: error_success == false) */
+    FMMHIP_EDEPTH = 5    /* octree deeper than 42 levels (coincident bodies) */
+} fmmhip_status;
+
+typedef enum { FMMHIP_POINT_SOURCE = 0, FMMHIP_POINT_VORTEX = 1 } fmmhip_kernel;
+typedef enum { FMMHIP_FP64 = 0, FMMHIP_MIXED = 1 } fmmhip_precision;
+typedef enum { FMMHIP_BARBA = 0, FMMHIP_SELFTUNING = 1, FMMHIP_SELFTUNING_TREESTOP = 2 } fmmhip_list_method;
+typedef enum {
+    FMMHIP_ERR_NONE = 0,
+    FMMHIP_ERR_ROTATED_COEFFICIENTS_GRADIENT = 1, /* RotatedCoefficients{Absolute,Relative}Gradient */
+    FMMHIP_ERR_POWER_POTENTIAL = 2,               /* Power{Absolute,Relative}Potential */
+    FMMHIP_ERR_POWER_GRADIENT = 3                 /* Power{Absolute,Relative}Gradient */
+} fmmhip_error_kind;
+
+/* Keyword arguments of fmm! (src/fmm.jl:862-923); defaults from fmmhip_options_default(). */
+typedef struct {
+    int32_t expansion_order;      /* 5 */
+    int32_t leaf_size_source;     /* 20 */
+    int32_t leaf_size_target;     /* same as source */
+    double multipole_acceptance;  /* 0.4 */
+    int32_t list_method;          /* FMMHIP_SELFTUNING_TREESTOP */
+    int32_t shrink_recenter;      /* 1 */
+    int32_t error_kind;           /* FMMHIP_ERR_NONE */
+    int32_t error_relative;       /* 0: Absolute, 1: Relative (runs estimate_influence!) */
+    double error_tolerance;       /* epsilon */
+    int32_t error_bonus;          /* BE, 1 */
+    int32_t scalar_potential;     /* 0 */
+    int32_t gradient;             /* 1 */
+    int32_t hessian;              /* 0 */
+    int32_t farfield, nearfield, self_induced;  /* 1, 1, 1 */
+    int32_t precision;            /* FMMHIP_FP64 */
+} fmmhip_options;
+
+typedef struct {
+    int32_t pmax;                 /* max expansion order used by M2L (+1 when a tolerance was missed) */
+    int32_t error_success;        /* reference's error_success */
+    int64_t n_target_branches, n_source_branches;
+    int64_t n_m2l, n_direct;      /* interaction list lengths */
+    int64_t n_p2p_pairs;          /* body-body pairs evaluated by the nearfield */
+    double ms_tree, ms_lists, ms_p2p, ms_upward, ms_m2l, ms_downward, ms_total;  /* device time per phase */
+} fmmhip_stats;
+
+typedef struct fmmhip_ctx fmmhip_ctx;
+typedef struct fmmhip_eval fmmhip_eval;
+
+void fmmhip_options_default(fmmhip_options* opt);
+
+/* Context: one HIP device and one stream. device < 0 selects the current device. */
+fmmhip_status fmmhip_ctx_create(int device, fmmhip_ctx** out);
+void fmmhip_ctx_destroy(fmmhip_ctx* ctx);
+const char* fmmhip_last_error(const fmmhip_ctx* ctx);
+fmmhip_status fmmhip_sync(fmmhip_ctx* ctx);
+
+/* An evaluation keeps trees, lists, expansions and buffers resident in HBM between
+ * calls (the reference's Cache + Tree objects, src/containers.jl:264-292). */
+fmmhip_status fmmhip_eval_create(fmmhip_ctx* ctx, fmmhip_eval** out);
+void fmmhip_eval_destroy(fmmhip_eval* ev);
+
+/* Hand over one target system and one source system (fmm!(targets, sources)).
+ * on_device = 0: host pointers, copied in. on_device = 1: device pointers (HBM),
+ * read in place during the following calls; they must stay valid. */
+fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n_targets, const double* target_xyz,
+                                      int64_t n_sources, const double* source_buffer, int32_t on_device);
+
+/* Stages of fmm!: Tree() for both systems (src/tree.jl:3-122) */
+fmmhip_status fmmhip_eval_build_trees(fmmhip_eval* ev, const fmmhip_options* opt);
+/* build_interaction_lists + sort_by_target (src/fmm.jl:904-907) */
+fmmhip_status fmmhip_eval_build_lists(fmmhip_eval* ev, const fmmhip_options* opt);
+/* the fmm! core: estimate_influence!, nearfield, upward, horizontal, downward passes (src/fmm.jl:914-1152) */
+fmmhip_status fmmhip_eval_run(fmmhip_eval* ev, const fmmhip_options* opt, fmmhip_stats* stats);
+/* all three stages (fmm!(targets, sources; kwargs...), src/fmm.jl:862-884) */
+fmmhip_status fmmhip_fmm(fmmhip_eval* ev, const fmmhip_options* opt, fmmhip_stats* stats);
+
+/* Results: 16 x n_targets in original body order. on_device selects a device or host destination. */
+fmmhip_status fmmhip_eval_get_targets(fmmhip_eval* ev, double* out, int32_t on_device);
+
+/* Tree and list inspection (host copies). which = 0 target tree, 1 source tree. */
+int64_t fmmhip_eval_n_branches(fmmhip_eval* ev, int32_t which);
+int64_t fmmhip_eval_n_levels(fmmhip_eval* ev, int32_t which);
+int64_t fmmhip_eval_n_leaves(fmmhip_eval* ev, int32_t which);
+/* ints: n x 6 [n_bodies, first_body, n_branches, first_child, parent, leaf]
+ * reals: n x 17 [source_center(3), target_center(3), source_radius, target_radius,
+ *                source_box(3), target_box(3), max_influence, 0, 0]          (Branch, src/containers.jl:150-164) */
+fmmhip_status fmmhip_eval_get_branches(fmmhip_eval* ev, int32_t which, int64_t* ints, double* reals);
+fmmhip_status fmmhip_eval_get_levels(fmmhip_eval* ev, int32_t which, int64_t* first_last);
+fmmhip_status fmmhip_eval_get_leaf_index(fmmhip_eval* ev, int32_t which, int64_t* out);
+fmmhip_status fmmhip_eval_get_sort_index(fmmhip_eval* ev, int32_t which, int64_t* out);
+/* which = 0 m2l list, 1 direct list; n x 2 int32 (target, source) sorted by target */
+int64_t fmmhip_eval_n_list(fmmhip_eval* ev, int32_t which);
+fmmhip_status fmmhip_eval_get_list(fmmhip_eval* ev, int32_t which, int32_t* out);
+/* expansion order chosen for each M2L interaction, in m2l-list order */
+fmmhip_status fmmhip_eval_get_m2l_orders(fmmhip_eval* ev, int32_t* out);
+/* expansions of a tree, [branch][H][phi/chi][re/im] doubles (src/tree.jl:1334-1339 layout per branch) */
+fmmhip_status fmmhip_eval_get_expansions(fmmhip_eval* ev, int32_t which, double* out);
+
+/* direct!(targets, sources; scalar_potential, gradient, hessian): O(N^2) on the GPU (src/direct.jl:43-75).
+ * target_out: 16 x n_targets, accumulated (+=) like the reference's set_* accessors. */
+fmmhip_status fmmhip_direct(fmmhip_ctx* ctx, int32_t kernel, int64_t n_targets, const double* target_xyz,
+                            int64_t n_sources, const double* source_buffer, int32_t scalar_potential,
+                            int32_t gradient, int32_t hessian, int32_t precision, double* target_out,
+                            int32_t on_device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMMHIP_H */

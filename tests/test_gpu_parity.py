@@ -1,0 +1,240 @@
+"""GPU parity tests: libfmmhip (through the C ABI) against the CPU oracle.
+
+Bars:
+* tree and interaction lists: bit-exact (every branch field, body order, list order);
+* FP64 evaluation: potential/gradient/hessian within 1e-11 of the oracle's FMM
+  relative to the largest target value (same expansion order, leaf size, MAC);
+* error-controlled expansion order: the order chosen per M2L interaction equals
+  the oracle's for at least 99.9% of interactions (1-ulp differences in acos/atan
+  can flip a threshold) and the results still meet 1e-11 relative;
+* mixed precision: the stated error bound against the FP64 O(N^2) direct sum.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import fastmultipole_amd as F
+from fastmultipole_amd import _lib, workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _opts(**kw):
+    return F.make_options(**kw)
+
+
+def _gpu_eval(kernel, sbuf, txyz=None, **kw):
+    ev = F.Evaluation()
+    ev.set_systems(kernel, txyz, sbuf)
+    opt = _opts(**kw)
+    return ev, opt
+
+
+def _oracle_run(kind, sbuf, txyz=None, **kw):
+    pos = sbuf[:, :3] if txyz is None else txyz
+    m = {"interaction_list_method": "method"}
+    okw = {}
+    for k, v in kw.items():
+        if k == "interaction_list_method":
+            okw["method"] = v.code
+        elif k == "error_tolerance":
+            if v is not None:
+                okw.update(error_kind=v.kind, relative=v.relative, eps=v.eps, BE=v.BE)
+        elif k == "precision":
+            pass
+        else:
+            okw[k] = v
+    return O.Run(kind, pos, sbuf, **okw)
+
+
+def _compare_trees(ev, run, which):
+    g = ev.tree(which)
+    ints, reals = run.branches(which)
+    assert g.n_branches == len(ints), (g.n_branches, len(ints))
+    gi = g.ints.copy()
+    oi = ints.copy()
+    leaves = oi[:, 2] == 0
+    gi[leaves, 3] = oi[leaves, 3] = -1          # child range start is meaningless for leaves
+    np.testing.assert_array_equal(gi, oi)
+    assert np.array_equal(g.reals[:, :15], reals[:, :15]), "branch geometry differs bitwise"
+    np.testing.assert_array_equal(g.levels_index, run.levels(which))
+    np.testing.assert_array_equal(g.leaf_index, run.leaf_index(which))
+    np.testing.assert_array_equal(g.sort_index, run.sort_index(which))
+
+
+TREE_CASES = [
+    ("uniform-20k-ls20", lambda: W.uniform_sources(20000, seed=11), O.KIND_SOURCE, 20, O.METHOD_SELFTUNING_TREESTOP, True),
+    ("uniform-5k-ls1-selftuning", lambda: W.uniform_sources(5000, seed=12), O.KIND_SOURCE, 1, O.METHOD_SELFTUNING, False),
+    ("ring-40k-ls64", lambda: W.vortex_ring(40000, seed=13), O.KIND_VORTEX, 64, O.METHOD_SELFTUNING_TREESTOP, True),
+    ("plummer-30k-barba", lambda: W.plummer_sources(30000, seed=14), O.KIND_SOURCE, 30, O.METHOD_BARBA, True),
+]
+
+
+@pytest.mark.parametrize("name,gen,kind,ls,method,shrink", TREE_CASES, ids=[c[0] for c in TREE_CASES])
+def test_tree_and_lists_bit_exact(name, gen, kind, ls, method, shrink):
+    _say(f"[gpu] tree+lists {name}")
+    sbuf = gen()
+    if kind == O.KIND_SOURCE and name.startswith("plummer"):
+        sbuf[:, 3] = 0.001 * np.abs(np.sin(np.arange(len(sbuf))))   # nonzero body radii exercise the source boxes
+    lm = {O.METHOD_BARBA: F.Barba(), O.METHOD_SELFTUNING: F.SelfTuning(), O.METHOD_SELFTUNING_TREESTOP: F.SelfTuningTreeStop()}[method]
+    ev, opt = _gpu_eval(kind, sbuf, expansion_order=4, leaf_size_source=ls, multipole_acceptance=0.5,
+                        interaction_list_method=lm, shrink_recenter=shrink)
+    ev.build_trees(opt)
+    ev.build_lists(opt)
+    run = O.Run(kind, sbuf[:, :3], sbuf, p=4, leaf_size_source=ls, method=method, shrink_recenter=shrink,
+                multipole_acceptance=0.5).trees().lists()
+    _compare_trees(ev, run, 0)
+    _compare_trees(ev, run, 1)
+    for which in (0, 1):
+        np.testing.assert_array_equal(ev.list(which), run.list(which))
+
+
+FMM_CASES = [
+    ("laplace-4k-p8", lambda: W.uniform_sources(4000, seed=21), O.KIND_SOURCE, dict(expansion_order=8, leaf_size_source=50,
+                                                                                    multipole_acceptance=0.5,
+                                                                                    scalar_potential=True, hessian=False)),
+    ("laplace-3k-p6-noshrink", lambda: W.uniform_sources(3000, seed=22), O.KIND_SOURCE,
+     dict(expansion_order=6, leaf_size_source=20, multipole_acceptance=0.4, shrink_recenter=False, scalar_potential=True,
+          hessian=True)),
+    ("vortex-4k-p8", lambda: W.uniform_vortices(4000, seed=23), O.KIND_VORTEX,
+     dict(expansion_order=8, leaf_size_source=30, multipole_acceptance=0.5, hessian=True)),
+    ("ring-6k-p10", lambda: W.vortex_ring(6000, seed=24), O.KIND_VORTEX,
+     dict(expansion_order=10, leaf_size_source=64, multipole_acceptance=0.4, hessian=True)),
+]
+
+
+@pytest.mark.parametrize("name,gen,kind,kw", FMM_CASES, ids=[c[0] for c in FMM_CASES])
+def test_fmm_fp64_matches_oracle(name, gen, kind, kw):
+    _say(f"[gpu] fmm fp64 {name}")
+    sbuf = gen()
+    ev, opt = _gpu_eval(kind, sbuf, **kw)
+    st = ev.fmm(opt)
+    got = ev.targets()
+    run = _oracle_run(kind, sbuf, p=kw["expansion_order"], **{k: v for k, v in kw.items() if k != "expansion_order"}).fmm()
+    ref = run.targets()
+    assert st.pmax == run.pmax and bool(st.error_success) == run.error_success
+    for lo, hi in ((3, 4), (4, 7), (7, 16)):
+        scale = np.abs(ref[:, lo:hi]).max()
+        if scale == 0:
+            continue
+        err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / scale
+        assert err < 1e-11, (name, lo, hi, err)
+    # expansions themselves (multipoles of the source tree, locals of the target tree)
+    p = kw["expansion_order"]
+    for which in (1, 0):
+        e_g = ev.tree(which).expansions(p)
+        e_o = run.expansions(which)
+        sc = np.abs(e_o).max()
+        assert np.abs(e_g - e_o).max() <= 1e-11 * sc
+
+
+ERR_CASES = [
+    ("laplace-rotated-abs", lambda: W.uniform_sources(5000, seed=31, strength_scale=1 / 5000), O.KIND_SOURCE,
+     F.RotatedCoefficientsAbsoluteGradient(1e-5, False), 10),
+    ("vortex-power-abs", lambda: W.uniform_vortices(5000, seed=32), O.KIND_VORTEX, F.PowerAbsoluteGradient(1e-5, False), 10),
+    ("laplace-power-potential-abs", lambda: W.uniform_sources(4000, seed=33, strength_scale=1 / 4000), O.KIND_SOURCE,
+     F.PowerAbsolutePotential(1e-6), 16),
+    ("ring-power-rel", lambda: W.vortex_ring(8000, seed=34), O.KIND_VORTEX, F.PowerRelativeGradient(1e-6), 64),
+    ("laplace-rotated-rel", lambda: W.uniform_sources(5000, seed=35), O.KIND_SOURCE,
+     F.RotatedCoefficientsRelativeGradient(1e-6), 20),
+]
+
+
+@pytest.mark.parametrize("name,gen,kind,em,ls", ERR_CASES, ids=[c[0] for c in ERR_CASES])
+def test_error_controlled_order_matches_oracle(name, gen, kind, em, ls):
+    _say(f"[gpu] error-controlled {name}")
+    sbuf = gen()
+    kw = dict(expansion_order=20, leaf_size_source=ls, multipole_acceptance=0.5, error_tolerance=em,
+              scalar_potential=(em.kind == _lib.ERR_POWER_POTENTIAL))
+    ev, opt = _gpu_eval(kind, sbuf, **kw)
+    st = ev.fmm(opt)
+    run = _oracle_run(kind, sbuf, p=20, **{k: v for k, v in kw.items() if k != "expansion_order"}).fmm()
+    if em.relative:   # estimate_influence!: max_influence per target branch
+        mi_g = ev.tree(0).reals[:, 14]
+        mi_o = run.branches(0)[1][:, 14]
+        np.testing.assert_allclose(mi_g, mi_o, rtol=1e-11, atol=0)
+    Pg, Po = ev.m2l_orders(), run.m2l_orders()
+    assert len(Pg) == len(Po)
+    match = float(np.mean(Pg == Po))
+    _say(f"    same order for {match * 100:.4f}% of {len(Po)} M2L interactions; pmax gpu {st.pmax} oracle {run.pmax}")
+    assert match >= 0.999
+    got, ref = ev.targets(), run.targets()
+    lo, hi = (3, 4) if em.kind == _lib.ERR_POWER_POTENTIAL else (4, 7)
+    scale = np.abs(ref[:, lo:hi]).max()
+    if match == 1.0:
+        assert np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / scale < 1e-11
+
+
+def test_direct_fp64_matches_oracle():
+    _say("[gpu] direct fp64")
+    for kind, sbuf in ((O.KIND_SOURCE, W.uniform_sources(3000, seed=41)), (O.KIND_VORTEX, W.vortex_ring(3000, seed=42))):
+        ctx = F.default_context()
+        txyz = np.ascontiguousarray(sbuf[:, :3])
+        out = np.zeros((len(sbuf), 16))
+        out[:, :3] = txyz
+        ctx.check(ctx.lib.fmmhip_direct(ctx._h, kind, len(sbuf), txyz.ctypes.data, len(sbuf),
+                                        np.ascontiguousarray(sbuf).ctypes.data, 1, 1, 1, _lib.FP64, out.ctypes.data, 0))
+        ref = O.direct(kind, txyz, sbuf, scalar_potential=True, gradient=True, hessian=True)
+        for lo, hi in ((3, 4), (4, 7), (7, 16)):
+            sc = np.abs(ref[:, lo:hi]).max()
+            if sc:
+                assert np.abs(out[:, lo:hi] - ref[:, lo:hi]).max() / sc < 1e-13
+
+
+def test_fmm_api_mirror_and_edge_cases():
+    _say("[gpu] api mirror + edge cases")
+    # fmm!(system) through the user-system interface, vs direct! on the GPU
+    g = W.uniform_sources(2000, seed=51)
+    sysA = F.Gravitational(np.column_stack([g, np.zeros((len(g), 3))]))
+    out = F.fmm(sysA, expansion_order=12, leaf_size=20, multipole_acceptance=0.5, scalar_potential=True)
+    assert len(out) == 8 and out[7] is True
+    sysB = F.Gravitational(np.column_stack([g, np.zeros((len(g), 3))]))
+    F.direct(sysB, scalar_potential=True)
+    rel = np.abs(sysA.potential[:, 0] - sysB.potential[:, 0]).max() / np.abs(sysB.potential[:, 0]).max()
+    assert rel < 1e-8
+    # a single body, two coincident bodies, a root leaf, and an empty target set
+    for n in (1, 2, 5):
+        b = np.zeros((n, 5))
+        b[:, 4] = 1.0
+        if n == 5:
+            b[:, :3] = np.random.default_rng(5).random((5, 3))
+        ev, opt = _gpu_eval(O.KIND_SOURCE, b, expansion_order=4, leaf_size_source=20, scalar_potential=True)
+        ev.fmm(opt)
+        got = ev.targets()
+        ref = O.direct(O.KIND_SOURCE, b[:, :3], b, scalar_potential=True)
+        np.testing.assert_allclose(got[:, 3:7], ref[:, 3:7], rtol=1e-12, atol=1e-14)
+    # unsupported expansion order is an error, not a fallback
+    ev, opt = _gpu_eval(O.KIND_SOURCE, W.uniform_sources(100, seed=6), expansion_order=25)
+    with pytest.raises(_lib.FmmHipError):
+        ev.fmm(opt)
+
+
+def test_mixed_precision_ring_error():
+    """Mixed form (FP32 P2P with compensated sums) on a 20k vortex ring at p=16 vs FP64 direct."""
+    _say("[gpu] mixed precision ring")
+    sbuf = W.vortex_ring(20000, seed=61)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ev, opt = _gpu_eval(O.KIND_VORTEX, sbuf, expansion_order=16, leaf_size_source=64, multipole_acceptance=0.4,
+                            precision=prec)
+        ev.fmm(opt)
+        res[prec] = ev.targets()
+    ctx = F.default_context()
+    txyz = np.ascontiguousarray(sbuf[:, :3])
+    ref = np.zeros((len(sbuf), 16))
+    ref[:, :3] = txyz
+    ctx.check(ctx.lib.fmmhip_direct(ctx._h, O.KIND_VORTEX, len(sbuf), txyz.ctypes.data, len(sbuf),
+                                    np.ascontiguousarray(sbuf).ctypes.data, 0, 1, 0, _lib.FP64, ref.ctypes.data, 0))
+    e64 = W.relative_error(res["fp64"][:, 4:7], ref[:, 4:7])
+    e32 = W.relative_error(res["mixed"][:, 4:7], ref[:, 4:7])
+    _say(f"    fp64 form rel err max {e64[0]:.3e} p99 {e64[1]:.3e};  mixed max {e32[0]:.3e} p99 {e32[1]:.3e}")
+    assert e64[1] < 1e-6
+    assert e32[1] < 1e-5
