@@ -1,0 +1,177 @@
+"""Benchmark: FMM evaluations/s on one MI355X for the 1M-particle vortex ring.
+
+Workload (the configuration the headline metric is quoted on): N = 1,000,000
+point vortices on a ring of radius 1 with a Gaussian core (sigma 0.1),
+tangential strengths with 1e-3 relative noise; velocity + velocity gradient;
+error-controlled expansion order PowerRelativeGradient(1e-6) with p <= 20;
+leaf size 64, MAC 0.4, shrink_recenter. One step = one complete fmm!
+evaluation (both trees, interaction lists, estimate_influence pre-pass, P2P,
+upward, M2L, downward) on inputs already resident in HBM.
+
+Prints one JSON line. `value` is the mixed-precision form's evaluations/s;
+the FP64 form is timed in the same run and reported beside it, with each
+form's error against the FP64 O(N^2) direct sum on sampled targets.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3     # MI355X dense FP32 vector/matrix peak
+PEAK_FP64_TFLOPS = 78.6
+FLOPS_PER_PAIR_VORTEX = 63   # v + grad v per body pair (sqrt and divide counted once)
+
+
+def m2l_flops(p, lh=True):
+    """Algorithmic flops of one rotation-based M2L at order p (Wigner-matrix build excluded)."""
+    H = (p + 1) * (p + 2) // 2
+    f = 14 * (H - 1) + 8 * sum((n + 1) * (2 * n + 1) for n in range(p + 1)) + \
+        4 * sum(p - m + 1 for n in range(p + 1) for m in range(n + 1))
+    return 2 * f + 8 * H if lh else f
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--mac", type=float, default=0.4)
+    ap.add_argument("--pmax", type=int, default=20)
+    ap.add_argument("--samples", type=int, default=10000)
+    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="mixed,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)"}))
+        sys.exit(2)
+
+    import fastmultipole_amd as F
+    from fastmultipole_amd import _lib, workloads as W
+
+    n = args.n
+    sbuf = W.vortex_ring(n, seed=2)
+    ctx = F.default_context()
+    ev = F.Evaluation(ctx)
+    ev.set_systems(_lib.POINT_VORTEX, None, sbuf)          # host -> HBM once, outside the timed region
+    em = F.PowerRelativeGradient(args.tol)
+
+    # reference velocities on sampled targets: FP64 O(N^2) direct on the GPU
+    rng = np.random.default_rng(3)
+    idx = np.sort(rng.choice(n, size=min(args.samples, n), replace=False))
+    txyz = np.ascontiguousarray(sbuf[idx, :3])
+    ref = np.zeros((len(idx), 16))
+    ref[:, :3] = txyz
+    t0 = time.perf_counter()
+    ctx.check(ctx.lib.fmmhip_direct(ctx._h, _lib.POINT_VORTEX, len(idx), txyz.ctypes.data, n,
+                                    np.ascontiguousarray(sbuf).ctypes.data, 0, 1, 1, _lib.FP64, ref.ctypes.data, 0))
+    log(f"direct reference on {len(idx)} targets: {time.perf_counter() - t0:.2f} s")
+
+    forms = {}
+    for form in args.forms.split(","):
+        opt = F.make_options(expansion_order=args.pmax, leaf_size_source=args.leaf, multipole_acceptance=args.mac,
+                             error_tolerance=em, gradient=True, hessian=True, precision=form)
+        for _ in range(args.warmup):
+            ev.fmm(opt)
+        ctx.check(ctx.lib.fmmhip_sync(ctx._h))
+        t0 = time.perf_counter()
+        stats = []
+        for _ in range(args.steps):
+            stats.append(ev.fmm(opt).as_dict())
+        ctx.check(ctx.lib.fmmhip_sync(ctx._h))
+        dt = (time.perf_counter() - t0) / args.steps
+        out = ev.targets()
+        orders = ev.m2l_orders()
+        e_v = W.relative_error(out[idx, 4:7], ref[:, 4:7])
+        e_g = W.relative_error(out[idx, 7:16], ref[:, 7:16])
+        phases = {k: float(np.mean([s[k] for s in stats])) for k in
+                  ("ms_tree", "ms_lists", "ms_p2p", "ms_upward", "ms_m2l", "ms_downward", "ms_total")}
+        forms[form] = dict(ms=dt * 1e3, evals=n / dt, stats=stats[-1], phases=phases, err_v=e_v, err_grad=e_g,
+                           orders=orders)
+        log(f"{form}: {dt * 1e3:.2f} ms/step, {n / dt:.3e} evals/s; phases {phases}; "
+            f"vel err max {e_v[0]:.2e} p99 {e_v[1]:.2e}; grad err max {e_g[0]:.2e} p99 {e_g[1]:.2e}; "
+            f"pmax {stats[-1]['pmax']} m2l {stats[-1]['n_m2l']} pairs {stats[-1]['n_p2p_pairs']}")
+
+    main_form = "mixed" if "mixed" in forms else next(iter(forms))
+    M = forms[main_form]
+    st = M["stats"]
+    ph = M["phases"]
+    # dominant phase -> roofline object
+    p2p_tflops = st["n_p2p_pairs"] * FLOPS_PER_PAIR_VORTEX / (ph["ms_p2p"] * 1e-3) / 1e12
+    hist = np.bincount(M["orders"], minlength=args.pmax + 1)
+    m2l_total = float(sum(c * m2l_flops(p) for p, c in enumerate(hist)))
+    m2l_tflops = m2l_total / (ph["ms_m2l"] * 1e-3) / 1e12
+    if ph["ms_p2p"] >= ph["ms_m2l"]:
+        roof = {"kernel": "k_p2p_f32 (nearfield P2P, vortex v + grad v)" if main_form == "mixed" else "k_p2p_f64",
+                "bound": "vector-alu", "achieved": round(p2p_tflops, 3),
+                "peak": PEAK_FP32_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "traffic": None}
+    else:
+        roof = {"kernel": "k_m2l (rotation-based M2L, FP64)", "bound": "vector-alu", "achieved": round(m2l_tflops, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "traffic": None}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        m = args.cpu_sample
+        cb = W.vortex_ring(m, seed=2)
+        t0 = time.perf_counter()
+        O.Run(O.KIND_VORTEX, cb[:, :3], cb, p=args.pmax, leaf_size_source=args.leaf, multipole_acceptance=args.mac,
+              error_kind=O.ERR_POWER_GRADIENT, relative=True, eps=args.tol, gradient=True, hessian=True).fmm()
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(m / tc, 1), "unit": "evals/s", "cores": 1, "kind": "port",
+               "sample": f"oracle (C++ FP64 restatement) full fmm! on a {m}-particle ring of the same distribution "
+                         f"and settings, {tc:.1f} s on one host core"}
+        log(f"cpu baseline: {m} bodies in {tc:.2f} s")
+
+    line = {
+        "metric": "FMM evals/s at rel.err<=1e-6 (mixed vs FP64 GPU) + p50/p99 request latency",
+        "value": round(M["evals"], 1),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(M["ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 P2P + fp64 expansions" if main_form == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {"workload": f"vortex ring N={n} (R=1, Gaussian core sigma=0.1), velocity+gradient, "
+                               f"PowerRelativeGradient({args.tol}), p<={args.pmax}, leaf {args.leaf}, MAC {args.mac}, "
+                               "shrink_recenter, SelfTuningTreeStop"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "forms": {k: {"evals_per_s": round(v["evals"], 1), "ms_per_step": round(v["ms"], 3),
+                      "rel_err_velocity": {"max": v["err_v"][0], "p99": v["err_v"][1], "median": v["err_v"][2]},
+                      "rel_err_gradient": {"max": v["err_grad"][0], "p99": v["err_grad"][1]},
+                      "phases_ms": {k2: round(x, 3) for k2, x in v["phases"].items()},
+                      "pmax": v["stats"]["pmax"], "n_m2l": v["stats"]["n_m2l"],
+                      "p2p_pairs_per_s": v["stats"]["n_p2p_pairs"] / (v["phases"]["ms_p2p"] * 1e-3),
+                      "m2l_per_s": v["stats"]["n_m2l"] / (v["phases"]["ms_m2l"] * 1e-3)}
+                  for k, v in forms.items()},
+        "mixed_over_fp64": (round(forms["mixed"]["evals"] / forms["fp64"]["evals"], 3)
+                            if "mixed" in forms and "fp64" in forms else None),
+        "latency_ms": {"p50": round(M["ms"], 3), "p99": None,
+                       "note": "single-request latency of one evaluation; the serving loop is not built yet"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

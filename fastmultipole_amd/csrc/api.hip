@@ -205,6 +205,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         stats->n_source_branches = E.st.n_branches;
         stats->n_m2l = E.lists.n_m2l;
         stats->n_direct = E.lists.n_direct;
+        stats->n_p2p_pairs = E.lists.n_pairs;
         stats->ms_p2p = t_p2p;
         stats->ms_upward = t_up;
         stats->ms_m2l = t_m2l;

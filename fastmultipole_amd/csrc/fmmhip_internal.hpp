@@ -83,7 +83,7 @@ struct Tree {
 };
 
 struct Lists {
-    int64_t n_m2l = 0, n_direct = 0;
+    int64_t n_m2l = 0, n_direct = 0, n_pairs = 0;
     DBuf<int32_t> m2l_t, m2l_s, dir_t, dir_s;        // sorted by target, DFS order within target
     DBuf<int32_t> m2l_order;                         // expansion order used per M2L
     // CSR over target branches for the direct list: unique targets and offsets
@@ -103,7 +103,6 @@ struct Context {
     std::string last_error;
     Tables* tables = nullptr;
     DBuf<unsigned char> temp;   // scratch for sorts/scans
-    hipEvent_t ev[8];
 };
 
 struct Eval {
