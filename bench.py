@@ -3,8 +3,11 @@
 Workload (the configuration the headline metric is quoted on): N = 1,000,000
 point vortices on a ring of radius 1 with a Gaussian core (sigma 0.1),
 tangential strengths with 1e-3 relative noise; velocity + velocity gradient;
-error-controlled expansion order PowerRelativeGradient(1e-6) with p <= 20;
-leaf size 64, MAC 0.4, shrink_recenter. One step = one complete fmm!
+error-controlled expansion order PowerRelativeGradient with p <= 20;
+leaf size 64, MAC 0.4, shrink_recenter. The reference's tolerance is a
+per-interaction heuristic, so the default tolerance (1e-8) is the one whose
+measured error against the FP64 direct sum stays <= 1e-6 on every sampled
+target; the line reports the measured max / p99 error and whether the bar held. One step = one complete fmm!
 evaluation (both trees, interaction lists, estimate_influence pre-pass, P2P,
 upward, M2L, downward) on inputs already resident in HBM.
 
@@ -47,7 +50,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--mac", type=float, default=0.4)
     ap.add_argument("--pmax", type=int, default=20)
@@ -89,9 +92,11 @@ def main():
             ev.fmm(opt)
         ctx.check(ctx.lib.fmmhip_sync(ctx._h))
         t0 = time.perf_counter()
-        stats = []
+        stats, lat = [], []
         for _ in range(args.steps):
-            stats.append(ev.fmm(opt).as_dict())
+            t1 = time.perf_counter()
+            stats.append(ev.fmm(opt).as_dict())     # returns after the evaluation's stream synchronisation
+            lat.append(time.perf_counter() - t1)
         ctx.check(ctx.lib.fmmhip_sync(ctx._h))
         dt = (time.perf_counter() - t0) / args.steps
         out = ev.targets()
@@ -99,9 +104,10 @@ def main():
         e_v = W.relative_error(out[idx, 4:7], ref[:, 4:7])
         e_g = W.relative_error(out[idx, 7:16], ref[:, 7:16])
         phases = {k: float(np.mean([s[k] for s in stats])) for k in
-                  ("ms_tree", "ms_lists", "ms_p2p", "ms_upward", "ms_m2l", "ms_downward", "ms_total")}
+                  ("ms_tree", "ms_lists", "ms_estimate", "ms_p2p", "ms_upward", "ms_m2l", "ms_downward",
+                   "ms_gather", "ms_total")}
         forms[form] = dict(ms=dt * 1e3, evals=n / dt, stats=stats[-1], phases=phases, err_v=e_v, err_grad=e_g,
-                           orders=orders)
+                           orders=orders, lat=lat)
         log(f"{form}: {dt * 1e3:.2f} ms/step, {n / dt:.3e} evals/s; phases {phases}; "
             f"vel err max {e_v[0]:.2e} p99 {e_v[1]:.2e}; grad err max {e_g[0]:.2e} p99 {e_g[1]:.2e}; "
             f"pmax {stats[-1]['pmax']} m2l {stats[-1]['n_m2l']} pairs {stats[-1]['n_p2p_pairs']}")
@@ -167,8 +173,10 @@ def main():
                   for k, v in forms.items()},
         "mixed_over_fp64": (round(forms["mixed"]["evals"] / forms["fp64"]["evals"], 3)
                             if "mixed" in forms and "fp64" in forms else None),
-        "latency_ms": {"p50": round(M["ms"], 3), "p99": None,
-                       "note": "single-request latency of one evaluation; the serving loop is not built yet"},
+        "meets_error_bar": bool(M["err_v"][0] <= 1e-6),
+        "latency_ms": {"p50": round(float(np.percentile(M["lat"], 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(M["lat"], 99)) * 1e3, 3),
+                       "note": "one request = one full evaluation issued back to back; no serving loop"},
     }
     print(json.dumps(line), flush=True)
 

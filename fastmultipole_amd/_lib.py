@@ -42,6 +42,7 @@ class Stats(C.Structure):
         ("n_source_branches", C.c_int64), ("n_m2l", C.c_int64), ("n_direct", C.c_int64), ("n_p2p_pairs", C.c_int64),
         ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_p2p", C.c_double), ("ms_upward", C.c_double),
         ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_total", C.c_double),
+        ("ms_estimate", C.c_double), ("ms_gather", C.c_double),
     ]
 
     def as_dict(self):

@@ -103,10 +103,10 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
     E.p_built = p;
 }
 
-static void eval_build_lists(Eval& E, const fmmhip_options& o) {
+static void eval_build_lists(Eval& E, const fmmhip_options& o, bool count_pairs = true) {
     if (!E.trees_built) throw Error(FMMHIP_EINVAL, "trees are not built");
     build_lists(*E.ctx, E.tt, E.st, E.lists, o.leaf_size_source, o.multipole_acceptance, o.list_method, o.farfield != 0,
-                o.nearfield != 0, o.self_induced != 0);
+                o.nearfield != 0, o.self_induced != 0, count_pairs);
     E.lists_built = true;
 }
 
@@ -117,7 +117,8 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
 // own trees; max |grad| (or potential) per main-tree branch
 static void estimate_influence(Eval& E, const fmmhip_options& o) {
     Context& C = *E.ctx;
-    Eval X;
+    if (!E.estimate) E.estimate = new Eval();
+    Eval& X = *E.estimate;
     X.ctx = E.ctx;
     X.kernel = E.kernel;
     X.nt = E.nt;
@@ -133,9 +134,9 @@ static void estimate_influence(Eval& E, const fmmhip_options& o) {
     eo.list_method = FMMHIP_SELFTUNING_TREESTOP;
     eo.farfield = eo.nearfield = eo.self_induced = 1;
     eval_build_trees(X, eo);
-    eval_build_lists(X, eo);
+    eval_build_lists(X, eo, false);
     eval_core(X, eo, nullptr, true);
-    DBuf<double> infl;
+    DBuf<double>& infl = E.influence;
     infl.resize(std::max<int64_t>(1, E.nt));
     k_map_influence<<<grid_for(E.nt, 256), 256, 0, C.stream>>>(X.tt.sort_index.data(), E.tt.inverse_sort.data(), X.tout.data(),
                                                               E.nt, o.error_kind == FMMHIP_ERR_POWER_POTENTIAL ? 1 : 0,
@@ -143,7 +144,6 @@ static void estimate_influence(Eval& E, const fmmhip_options& o) {
     LAUNCH_CHECK();
     k_branch_max<<<(unsigned)E.tt.n_branches, 256, 0, C.stream>>>(E.tt.branches.data(), E.tt.n_branches, infl.data());
     LAUNCH_CHECK();
-    HIP_CHECK(hipStreamSynchronize(C.stream));
 }
 
 static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, bool is_estimate) {
@@ -154,7 +154,12 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
     const bool LH = E.kernel == FMMHIP_POINT_VORTEX;   // has_vector_potential
     const bool PS = o.scalar_potential != 0, GS = o.gradient != 0, HS = o.hessian != 0;
     Timer total(C);
-    if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) estimate_influence(E, o);
+    double t_est = 0, t_gather = 0;
+    if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) {
+        Timer t(C);
+        estimate_influence(E, o);
+        t_est = t.stop();
+    }
     E.tout.resize((size_t)13 * std::max<int64_t>(1, E.nt));
     HIP_CHECK(hipMemsetAsync(E.tout.data(), 0, E.tout.size() * sizeof(double), C.stream));
     if (E.tt.n_branches) HIP_CHECK(hipMemsetAsync(E.tt.expansions.data(), 0, E.tt.expansions.size() * sizeof(double), C.stream));
@@ -193,8 +198,10 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         }
     }
     if (!is_estimate) {
+        Timer t(C);
         E.tout_aos.resize((size_t)16 * std::max<int64_t>(1, E.nt));
         gather_outputs(C, E.tt, E.tout.data(), E.tout_aos.data(), E.nt);
+        t_gather = t.stop();
     }
     const double t_tot = total.stop();
     E.ran = true;
@@ -211,6 +218,8 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         stats->ms_m2l = t_m2l;
         stats->ms_downward = t_down;
         stats->ms_total = t_tot;
+        stats->ms_estimate = t_est;
+        stats->ms_gather = t_gather;
     }
 }
 

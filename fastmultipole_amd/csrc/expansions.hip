@@ -4,14 +4,15 @@
 // src/evaluate_expansions.jl). Expansions keep the reference layout per branch:
 // [H][phi/chi][re/im] doubles.
 //
-// Translations use the reference's rotation scheme: z-rotation, y-rotation by
-// a Wigner matrix assembled from the pi/2 tables, a translation along z, the
-// Lamb-Helmholtz correction, then the inverse rotations. One wavefront runs one
-// translation at a time with all intermediates in LDS: lanes split the (n, m)
-// coefficients for rotations and translations and the (n, m, m') entries for
-// the Wigner matrix. A wavefront owns one destination expansion (target box for
-// M2L, parent for M2M, child for L2L) and accumulates in LDS, so results are
-// written once and summation order is the reference's list order.
+// Translations use the reference's rotation scheme: z-rotation, y-rotation, a
+// translation along z, the Lamb-Helmholtz correction, then the inverse
+// rotations. The y-rotations apply the reference's Wigner matrices in factorised
+// form (two products with the fixed pi/2 tables around a diagonal phase), so no
+// per-translation matrix is assembled: O(p^3) work per translation. One
+// wavefront runs one translation at a time with its intermediates in LDS and
+// its destination expansion in registers; workgroups of four wavefronts share
+// the pi/2 tables in LDS. M2L: one workgroup per target box, its wavefronts
+// take the box's sources in turn and the partial locals are summed once.
 //
 // P2M: one thread per output coefficient of a leaf, bodies staged in LDS;
 // L2P: per-leaf coefficient sets for potential, field and field gradient, then
@@ -19,7 +20,8 @@
 // Cartesian recurrences (no trigonometry per target).
 //
 // Compiled with -ffp-contract=off so that the error estimates that choose the
-// expansion order follow the reference's arithmetic.
+// expansion order follow the reference's arithmetic; the rotation and
+// translation products use explicit fma.
 #include "fmmhip_internal.hpp"
 #include <algorithm>
 #include <cmath>
@@ -32,9 +34,9 @@ static constexpr double ONE_OVER_4PI = 1.0 / (4.0 * 3.141592653589793);
 static constexpr double SQRT3_D = 1.7320508075688772;
 static constexpr double LOCAL_ERROR_SAFETY = 2.0;
 
-__host__ __device__ inline int nh(int p) { return ((p + 1) * (p + 2)) >> 1; }
-__host__ __device__ inline int len_Hs(int p) { int n = p + 1; return (n * (n + 1) * (n + 2)) / 6; }
-__host__ __device__ inline int len_Ts(int p) { return p < 0 ? 0 : ((p + 1) * (p + 2) * (2 * p + 3)) / 6; }
+__host__ __device__ constexpr int nh(int p) { return ((p + 1) * (p + 2)) >> 1; }
+__host__ __device__ constexpr int len_Hs(int p) { return ((p + 1) * (p + 2) * (p + 3)) / 6; }
+__host__ __device__ constexpr int len_Ts(int p) { return p < 0 ? 0 : ((p + 1) * (p + 2) * (2 * p + 3)) / 6; }
 
 // idx -> (n, m) for the packed (n, m>=0) ordering
 __device__ inline void idx_nm(int idx, int& n, int& m) {
@@ -49,7 +51,6 @@ __device__ inline void idx_nm(int idx, int& n, int& m) {
 struct Tables {
     int p = -1;
     DBuf<double> Hs, zeta, eta, Mt, Lt, inv_fact;
-    DBuf<uint32_t> tent;   // Wigner entries (n | m<<8 | mp<<16), ordered by n
 };
 
 // Host computation of the pi/2 Wigner tables, zeta/eta magnitudes and power
@@ -176,17 +177,11 @@ void ensure_tables(Context& C, int p) {
             Lt[i] = std::sqrt(1.0 / (a * b * (double)(2 * n + 1)));
         }
     for (int k = 0; k < (int)invf.size(); ++k) invf[k] = 1.0 / host_factorial(k);
-    std::vector<uint32_t> tent;
-    for (int n = 1; n <= pm; ++n)
-        for (int m = 0; m <= n; ++m)
-            for (int mp = 0; mp <= m; ++mp) tent.push_back((uint32_t)n | ((uint32_t)m << 8) | ((uint32_t)mp << 16));
     auto up = [&](DBuf<double>& d, const std::vector<double>& h) {
         d.resize(h.size());
         HIP_CHECK(hipMemcpy(d.data(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     };
     up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf);
-    T.tent.resize(tent.size());
-    HIP_CHECK(hipMemcpy(T.tent.data(), tent.data(), tent.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     T.p = pm;
 }
 
@@ -194,38 +189,122 @@ void free_tables(Tables* t) { delete t; }
 
 struct TablePtrs {
     const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact;
-    const uint32_t* tent;
 };
 static TablePtrs ptrs(Context& C) {
     Tables& T = *C.tables;
-    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data(), T.tent.data()};
+    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data()};
 }
 
-// ======================================================= wavefront engine
-// LDS expansion arrays use the per-branch global layout: [idx][pot][reim]
-#define EX(a, i, pot, ri) (a)[4 * (i) + 2 * (pot) + (ri)]
+// ====================================================== rotation engine
+// y-rotations use the reference's matrices T (update_Ts!, src/rotate.jl:260-441)
+// with the zeta / eta magnitudes and the i^k signs of _rotate_multipole_y! /
+// _rotate_local_y! (:699-1128), evaluated in factorised form instead of
+// assembling T for each translation. With A_n(m, v) the degree-n pi/2 table
+// (Hs_pi2, read as a symmetric matrix), w_0 = 1, w_v = 2 (v >= 1) and
+// s_v = (-1)^(n+v), the entries of T are
+//   T(m, +-m') = sum_v w_v A(m, v) A(m', v) (i^(m+m') e^{i v b} + (-i)^(m+m') e^{-i v b}) / 2 * (1 | s_v),
+// and the magnitudes factor as zeta(m, m') = g(m') / g(m) = 1 / eta(m, m'),
+// g(k) = sqrt((n+k)! (n-k)!) / n!. Substituting both into the rotation sum,
+// the i^k signs cancel, and one rotation of degree n becomes
+//   U_m'   = g(m')^(+-1) X_m'                                   (input scale)
+//   SE_v   = sum_{m' even} A(m', v) U_m',   SO_v = sum_{m' odd} A(m', v) U_m'
+//   P+_v   = (SE_v - A(0, v) U_0) - SO_v + s_v conj(SE_v + SO_v)
+//   P-_v   = (SE_v - A(0, v) U_0) + SO_v + s_v conj(SE_v - SO_v)
+//   Q+-_v  = w_v e^{+-i sigma v b} P+-_v,  R_v^(e,o) = Q+_v +- Q-_v
+//   out_m  = g(m)^(-+1) / 2 * sum_v A(m, v) R_v^(parity of m)
+// with sigma = +1 and g^+1 in / g^-1 out for zeta (multipole) rotations,
+// sigma = -1 and g^-1 in / g^+1 out for eta (local) rotations. The back
+// rotations of the reference reuse the same T (rotate.jl:927-930, 1115-1128), so
+// the same algorithm serves both directions. Work per degree: two real
+// (n+1) x (n+1) matrix-vector products per complex potential, no T assembly.
+//
+// One wavefront runs one translation. Lanes own work items of a "chunk" of
+// consecutive degrees: item (n, j) covers the column pair v = 2j, 2j+1 in the
+// first product and the output pair m = 2j, 2j+1 in the second. A chunk holds at
+// most 64 items, so each product is a single pass of the wavefront.
 
-struct Lds {
-    double* acc;   // 4H
-    double* w1;    // 4H
-    double* w2;    // 4H
-    double* T;     // len_Ts(p)
-    double* eim;   // 2(p+1)
-    double* misc;  // 16 + 4(p+1)
+static constexpr int WPB = 4;          // wavefronts per workgroup in the translation kernels
+static constexpr int RBUF = 1024;      // doubles of per-wavefront item buffer (64 items x 16)
+static constexpr int MAXCH = 2;        // back-rotation chunks held in registers: orders up to 20
+static constexpr int FWD_DEGREES = 4;  // degrees per forward chunk of the error-controlled M2L
+static constexpr int SMALL_P = 4;      // fixed orders up to this run one M2L per lane (k_m2l_lane)
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+__device__ inline int wave_id() { return (int)(threadIdx.x >> 6); }
+
+// LDS ordering between lanes of one wavefront
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// items of degrees < n (ceil((k+1)/2) per degree k)
+__host__ __device__ inline int items_below(int n) { return ((n + 1) * (n + 1)) >> 2; }
+
+// end degree of the chunk starting at n0: at most 64 items and at most `maxdeg` degrees
+__host__ __device__ inline int chunk_end(int n0, int P, int maxdeg) {
+    int n1 = n0;
+    while (n1 + 1 <= P && n1 + 1 - n0 < maxdeg && items_below(n1 + 2) - items_below(n0) <= 64) ++n1;
+    return n1;
+}
+
+// global item index g -> (n, j)
+__device__ inline void item_nj(int g, int& n, int& j) {
+    int k = (int)sqrtf(4.0f * (float)g + 3.0f) - 1;
+    while (items_below(k + 1) <= g) ++k;
+    while (k > 0 && items_below(k) > g) --k;
+    n = k;
+    j = g - items_below(k);
+}
+
+__device__ inline double hat(const double* Hn, int a, int b) {   // A_n(a, b) = Hs_pi2 entry (min, max)
+    int lo = min(a, b), hi = max(a, b);
+    return Hn[((hi * (hi + 1)) >> 1) + lo];
+}
+
+struct WaveLds {
+    double* X;     // 4H: expansion workspace [idx][phi/chi][re/im]
+    double* R;     // max(RBUF, 4H): item buffer / translated expansion
+    double* eim;   // 2(P+1): e^{i m phi}
+    double* cs;    // 2(P+1): cos(v beta), sin(v beta)
 };
 
-__device__ inline Lds carve(double* base, int p) {
-    const int H4 = 4 * nh(p);
-    Lds L;
-    L.acc = base;
-    L.w1 = L.acc + H4;
-    L.w2 = L.w1 + H4;
-    L.T = L.w2 + H4;
-    L.eim = L.T + len_Ts(p);
-    L.misc = L.eim + 2 * (p + 1);
-    return L;
+__host__ __device__ inline int wave_lds_doubles(int P) {
+    const int H4 = 4 * (((P + 1) * (P + 2)) >> 1);
+    return H4 + (H4 > RBUF ? H4 : RBUF) + 4 * (P + 1);
 }
-static size_t lds_bytes(int p) { return sizeof(double) * (3 * 4 * nh(p) + len_Ts(p) + 2 * (p + 1) + 16 + 4 * (p + 1)); }
+__host__ __device__ inline int hs_lds_doubles(int P) { return (len_Hs(P) + 1) & ~1; }
+static size_t translate_lds_bytes(int P) { return sizeof(double) * (size_t)(hs_lds_doubles(P) + WPB * wave_lds_doubles(P)); }
+
+// launch set-up shared by the translation kernels: order check and the LDS limit
+template <class K>
+static size_t translate_setup(K kernel, int P) {
+    int chunks = 0;
+    for (int n0 = 0; n0 <= P; n0 = chunk_end(n0, P, 64) + 1) ++chunks;
+    if (chunks > MAXCH) throw Error(FMMHIP_EINVAL, "expansion order too large for the translation kernels");
+    const size_t sh = translate_lds_bytes(P);
+    HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    return sh;
+}
+
+__device__ inline WaveLds carve_wave(double* base, int P) {
+    WaveLds W;
+    W.X = base + (size_t)wave_id() * wave_lds_doubles(P);
+    const int H4 = 4 * nh(P);
+    W.R = W.X + H4;
+    W.eim = W.R + (H4 > RBUF ? H4 : RBUF);
+    W.cs = W.eim + 2 * (P + 1);
+    return W;
+}
+
+// copies the pi/2 tables into LDS (whole workgroup, ends with a barrier)
+__device__ inline double* load_hs(double* lds, const double* Hs_g, int P) {
+    const int n = len_Hs(P);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = Hs_g[i];
+    __syncthreads();
+    return lds;
+}
 
 __device__ inline void c2s(double x, double y, double z, double& r, double& th, double& ph) {   // src/rotate.jl:9-21
     double x2y2 = x * x + y * y;
@@ -237,115 +316,241 @@ __device__ inline void c2s(double x, double y, double z, double& r, double& th, 
     ph = (x2y2 == 0.0) ? 0.0 : atan2(y, x);
 }
 
-// e^{i m phi}, m = 0..P by repeated multiplication (src/rotate.jl:25-36); lane 0 only
-__device__ inline void eim_fill(double* eim, double phi, int P) {
-    if (threadIdx.x == 0) {
-        double ei_i = sin(phi), ei_r = cos(phi);
-        double er = 1.0, e_i = 0.0;
-        for (int m = 0; m <= P; ++m) {
-            eim[2 * m] = er;
-            eim[2 * m + 1] = e_i;
+// e^{i m phi} by repeated multiplication (src/rotate.jl:25-36) and cos/sin(v beta)
+// by the angle-addition recurrence of update_Ts! (:380-435); lane m / lane 32+v
+__device__ inline void fill_angles(const WaveLds& W, double ph, double th, int P) {
+    const int l = lane_id();
+    if (l <= P) {
+        const double s = sin(ph), c = cos(ph);
+        double er = 1.0, ei = 0.0;
+        for (int k = 0; k < l; ++k) {
             double t = er;
-            er = er * ei_r - e_i * ei_i;
-            e_i = t * ei_i + e_i * ei_r;
+            er = er * c - ei * s;
+            ei = t * s + ei * c;
         }
+        W.eim[2 * l] = er;
+        W.eim[2 * l + 1] = ei;
+    } else if (l >= 32 && l - 32 <= P) {
+        const int v = l - 32;
+        const double sb = sin(th), cb = cos(th);
+        double c = 1.0, s = 0.0;
+        for (int k = 0; k < v; ++k) {
+            double cn = cb * c - sb * s;
+            double sn = sb * c + cb * s;
+            c = cn;
+            s = sn;
+        }
+        W.cs[2 * v] = c;
+        W.cs[2 * v + 1] = s;
     }
 }
 
-__device__ inline double get_scalar(int m_mp) {
-    if (m_mp & 1) m_mp += 1;
-    return ((m_mp >> 1) & 1) ? -1.0 : 1.0;
-}
-
-// one Wigner entry (n, m, mp): the (m, +-mp) pair of T for angle beta (src/rotate.jl:380-435)
-__device__ inline void wigner_entry(double* T, const double* Hs, double sb, double cb, int n, int m, int mp) {
-    const double* Hp = Hs + len_Hs(n - 1);
-    auto Hat = [&](int a, int b) { return Hp[(b * (b + 1)) / 2 + a]; };
-    const bool even = ((m + mp) & 1) == 0;
-    double scalar = get_scalar(m + mp);
-    double vpos = 0.0, vneg = 0.0, s_prev = 0.0, c_prev = 1.0;
-    double sgn = ((n + mp + 1) & 1) ? -1.0 : 1.0;           // (-1)^(n+mp+nu) at nu = 1
-    const double one_mp_odd = (mp & 1) ? -1.0 : 1.0;
-    for (int nu = 1; nu <= n; ++nu) {
-        double c_nb = cb * c_prev - sb * s_prev;
-        double s_nb = sb * c_prev + cb * s_prev;
-        double sc = even ? scalar * c_nb : scalar * s_nb;
-        double val = Hat(min(mp, nu), max(mp, nu)) * Hat(min(m, nu), max(m, nu)) * sc;
-        vpos += val;
-        vneg += val * sgn * one_mp_odd;
-        c_prev = c_nb;
-        s_prev = s_nb;
-        sgn = -sgn;
-    }
-    vpos *= 2.0;
-    vneg *= 2.0;
-    scalar = even ? scalar : 0.0;
-    double val = Hat(0, m) * Hat(0, mp) * scalar;
-    const double one_n_mp = ((n + mp) & 1) ? -1.0 : 1.0;
-    vpos += val;
-    vneg += val * one_n_mp * one_mp_odd;
-    double* Tn = T + len_Ts(n - 1);
-    Tn[m * m + mp + m] = vpos;
-    Tn[m * m - mp + m] = vneg;
-}
-
-// all Wigner entries for degrees 1..P (lane-parallel); T[0] = 1
-__device__ inline void wigner_all(double* T, const TablePtrs& tb, double beta, int P) {
-    const double sb = sin(beta), cb = cos(beta);
-    if (threadIdx.x == 0) T[0] = 1.0;
-    const int ne = len_Hs(P) - 1;
-    for (int k = threadIdx.x; k < ne; k += 64) {
-        uint32_t e = tb.tent[k];
-        wigner_entry(T, tb.Hs, sb, cb, (int)(e & 0xff), (int)((e >> 8) & 0xff), (int)(e >> 16));
-    }
-}
-
-// y-rotation of coefficient (n, m) using magnitudes `mags` (zeta: multipole, eta: local)
-template <bool ETA, bool LH>
-__device__ inline void rotate_y_one(const double* in, const double* T, const double* mags, int n, int m, double out[4]) {
-    const double* Tn = T + len_Ts(n - 1);
-    const double* mg = mags + len_Ts(n - 1) + m * (n + 1);
-    double v1r = 0, v1i = 0, v2r = 0, v2i = 0;
-    auto sign4 = [&](int mp, double mag, double& zr, double& zi) {
-        int mod = ETA ? (m - abs(mp)) % 4 : (abs(mp) - m) % 4;
-        if (mod < 0) mod += 4;
-        zr = (mod == 0) ? mag : (mod == 2) ? -mag : 0.0;
-        zi = (mod == 1) ? mag : (mod == 3) ? -mag : 0.0;
-    };
-    const int base = (n * (n + 1)) / 2;
-    for (int mp = -n; mp <= 0; ++mp) {
-        int i1 = min(-mp, m), i2 = max(-mp, m);
-        double Tv = (n == 0) ? T[0] : Tn[i2 * i2 - i1 + i2];
-        double zr, zi;
-        sign4(mp, mg[-mp] * Tv, zr, zi);
-        int iw = base - mp;
-        double Mr = EX(in, iw, 0, 0), Mi = -EX(in, iw, 0, 1);
-        if (mp & 1) { Mr = -Mr; Mi = -Mi; }
-        v1r += Mr * zr - Mi * zi;
-        v1i += Mr * zi + Mi * zr;
+// X[n][m] = g_n(m)^(+-1) * (src[n][m] e^{i m phi}) for degrees n0..n1 (rotate_z!, src/rotate.jl:448-483)
+// mags row m = 0 of zeta (g) or eta (1/g)
+template <bool LH>
+__device__ inline void zrot_in(const double* __restrict__ src, const WaveLds& W, const double* mags, int n0, int n1) {
+    const int i0 = nh(n0 - 1), i1 = nh(n1);
+    for (int i = i0 + lane_id(); i < i1; i += 64) {
+        int n, m;
+        idx_nm(i, n, m);
+        const double g = mags[len_Ts(n - 1) + m];
+        const double er = W.eim[2 * m], ei = W.eim[2 * m + 1];
+        double xr = src[4 * i], xi = src[4 * i + 1];
+        W.X[4 * i] = g * (xr * er - xi * ei);
+        W.X[4 * i + 1] = g * (xr * ei + xi * er);
         if (LH) {
-            Mr = EX(in, iw, 1, 0); Mi = -EX(in, iw, 1, 1);
-            if (mp & 1) { Mr = -Mr; Mi = -Mi; }
-            v2r += Mr * zr - Mi * zi;
-            v2i += Mr * zi + Mi * zr;
+            xr = src[4 * i + 2]; xi = src[4 * i + 3];
+            W.X[4 * i + 2] = g * (xr * er - xi * ei);
+            W.X[4 * i + 3] = g * (xr * ei + xi * er);
         }
     }
-    for (int mp = 1; mp <= n; ++mp) {
-        int i1 = min(mp, m), i2 = max(mp, m);
-        double Tv = Tn[i2 * i2 + i1 + i2];
-        double zr, zi;
-        sign4(mp, mg[mp] * Tv, zr, zi);
-        int iw = base + mp;
-        double Mr = EX(in, iw, 0, 0), Mi = EX(in, iw, 0, 1);
-        v1r += Mr * zr - Mi * zi;
-        v1i += Mr * zi + Mi * zr;
-        if (LH) {
-            Mr = EX(in, iw, 1, 0); Mi = EX(in, iw, 1, 1);
-            v2r += Mr * zr - Mi * zi;
-            v2i += Mr * zi + Mi * zr;
+}
+
+// first product of the chunk [n0, n1] (degrees <= nlim): X -> R
+template <bool LH>
+__device__ inline void rot_step1(const WaveLds& W, const double* Hs, double sigma, int n0, int n1, int nlim) {
+    const int k = lane_id();
+    const int base = items_below(n0);
+    if (k >= items_below(n1 + 1) - base) return;
+    int n, j;
+    item_nj(base + k, n, j);
+    if (n > nlim) return;
+    const double* Hn = Hs + len_Hs(n - 1);
+    const double* Xn = W.X + 4 * nh(n - 1);
+    const int v0 = 2 * j, v1 = 2 * j + 1;
+    const bool has1 = v1 <= n;
+    constexpr int NP = LH ? 4 : 2;
+    double se0[NP], so0[NP], se1[NP], so1[NP];
+#pragma unroll
+    for (int c = 0; c < NP; ++c) se0[c] = so0[c] = se1[c] = so1[c] = 0.0;
+    for (int mp = 0; mp <= n; mp += 2) {   // even m'
+        const double a0 = hat(Hn, mp, v0), a1 = has1 ? hat(Hn, mp, v1) : 0.0;
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            const double u = Xn[4 * mp + c];
+            se0[c] = fma(a0, u, se0[c]);
+            se1[c] = fma(a1, u, se1[c]);
         }
     }
-    out[0] = v1r; out[1] = v1i; out[2] = v2r; out[3] = v2i;
+    for (int mp = 1; mp <= n; mp += 2) {   // odd m'
+        const double a0 = hat(Hn, mp, v0), a1 = has1 ? hat(Hn, mp, v1) : 0.0;
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            const double u = Xn[4 * mp + c];
+            so0[c] = fma(a0, u, so0[c]);
+            so1[c] = fma(a1, u, so1[c]);
+        }
+    }
+    double* out = W.R + 16 * k;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int v = h ? v1 : v0;
+        if (h && !has1) break;
+        const double* se = h ? se1 : se0;
+        const double* so = h ? so1 : so0;
+        const double a00 = hat(Hn, 0, v);
+        const double s = ((n + v) & 1) ? -1.0 : 1.0;
+        const double w = v ? 2.0 : 1.0;
+        const double c = w * W.cs[2 * v], sn = w * sigma * W.cs[2 * v + 1];
+#pragma unroll
+        for (int pot = 0; pot < (LH ? 2 : 1); ++pot) {
+            const double ser = se[2 * pot], sei = se[2 * pot + 1], sor = so[2 * pot], soi = so[2 * pot + 1];
+            const double dr = ser - a00 * Xn[2 * pot], di = sei - a00 * Xn[2 * pot + 1];
+            // P+ = (SE - S0) - SO + s conj(SE + SO),  P- = (SE - S0) + SO + s conj(SE - SO)
+            const double ppr = dr - sor + s * (ser + sor), ppi = di - soi - s * (sei + soi);
+            const double pmr = dr + sor + s * (ser - sor), pmi = di + soi - s * (sei - soi);
+            // Q+ = (c + i sn) P+,  Q- = (c - i sn) P-
+            const double qpr = c * ppr - sn * ppi, qpi = c * ppi + sn * ppr;
+            const double qmr = c * pmr + sn * pmi, qmi = c * pmi - sn * pmr;
+            double* o = out + 8 * h + 4 * pot;
+            o[0] = qpr + qmr; o[1] = qpi + qmi;   // even m
+            o[2] = qpr - qmr; o[3] = qpi - qmi;   // odd m
+        }
+    }
+}
+
+// second product for item (n, j): outputs m0 = 2j, m1 = 2j+1 (scaled), o[0..3] and o[4..7]
+template <bool LH>
+__device__ inline void rot_step2(const WaveLds& W, const double* Hs, const double* mags, int n0, int n, int j,
+                                 double o[8]) {
+    const double* Hn = Hs + len_Hs(n - 1);
+    const int m0 = 2 * j, m1 = 2 * j + 1;
+    const bool has1 = m1 <= n;
+    constexpr int NP = LH ? 4 : 2;
+    double e[NP], d[NP];
+#pragma unroll
+    for (int c = 0; c < NP; ++c) e[c] = d[c] = 0.0;
+    const double* Rn = W.R + 16 * (items_below(n) - items_below(n0));
+    for (int v = 0; v <= n; ++v) {
+        const double* rv = Rn + 8 * v;   // item v/2, half v&1: 16*(v>>1) + 8*(v&1) = 8v
+        const double a0 = hat(Hn, m0, v), a1 = has1 ? hat(Hn, m1, v) : 0.0;
+#pragma unroll
+        for (int pot = 0; pot < NP / 2; ++pot) {
+            e[2 * pot] = fma(a0, rv[4 * pot], e[2 * pot]);
+            e[2 * pot + 1] = fma(a0, rv[4 * pot + 1], e[2 * pot + 1]);
+            d[2 * pot] = fma(a1, rv[4 * pot + 2], d[2 * pot]);
+            d[2 * pot + 1] = fma(a1, rv[4 * pot + 3], d[2 * pot + 1]);
+        }
+    }
+    const double* mg = mags + len_Ts(n - 1);
+    const double g0 = 0.5 * mg[m0 * (n + 1)], g1 = has1 ? 0.5 * mg[m1 * (n + 1)] : 0.0;
+    o[0] = g0 * e[0]; o[1] = g0 * e[1];
+    o[2] = LH ? g0 * e[NP - 2] : 0.0; o[3] = LH ? g0 * e[NP - 1] : 0.0;
+    o[4] = g1 * d[0]; o[5] = g1 * d[1];
+    o[6] = LH ? g1 * d[NP - 2] : 0.0; o[7] = LH ? g1 * d[NP - 1] : 0.0;
+}
+
+// forward rotation of the chunk [n0, n1]: X (scaled input) -> X (rotated)
+template <bool LH>
+__device__ inline void rot_chunk_store(const WaveLds& W, const double* Hs, const double* mags, double sigma, int n0, int n1) {
+    rot_step1<LH>(W, Hs, sigma, n0, n1, n1);
+    wave_sync();
+    const int k = lane_id();
+    const int base = items_below(n0);
+    if (k < items_below(n1 + 1) - base) {
+        int n, j;
+        item_nj(base + k, n, j);
+        double o[8];
+        rot_step2<LH>(W, Hs, mags, n0, n, j, o);
+        double* x = W.X + 4 * (nh(n - 1) + 2 * j);
+        x[0] = o[0]; x[1] = o[1];
+        if (LH) { x[2] = o[2]; x[3] = o[3]; }
+        if (2 * j + 1 <= n) {
+            x[4] = o[4]; x[5] = o[5];
+            if (LH) { x[6] = o[6]; x[7] = o[7]; }
+        }
+    }
+    wave_sync();
+}
+
+// back rotation (degrees <= nlim) + back z-rotation, accumulated into registers:
+// acc[c] holds item (chunk c, lane) = outputs (n, 2j) and (n, 2j+1)
+template <bool LH>
+__device__ inline void rot_back_acc(const WaveLds& W, const double* Hs, const double* mags, double sigma, int P, int nlim,
+                                    double (&acc)[MAXCH][8]) {
+    int n0 = 0;
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+        if (n0 <= nlim) {
+            const int n1 = chunk_end(n0, P, 64);
+            rot_step1<LH>(W, Hs, sigma, n0, n1, nlim);
+            wave_sync();
+            const int k = lane_id();
+            const int base = items_below(n0);
+            if (k < items_below(n1 + 1) - base) {
+                int n, j;
+                item_nj(base + k, n, j);
+                if (n <= nlim) {
+                    double o[8];
+                    rot_step2<LH>(W, Hs, mags, n0, n, j, o);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int m = 2 * j + h;
+                        if (m > n) break;
+                        const double er = W.eim[2 * m], ei = -W.eim[2 * m + 1];   // back_rotate_z! (src/rotate.jl:584-616)
+                        acc[c][4 * h] += o[4 * h] * er - o[4 * h + 1] * ei;
+                        acc[c][4 * h + 1] += o[4 * h] * ei + o[4 * h + 1] * er;
+                        if (LH) {
+                            acc[c][4 * h + 2] += o[4 * h + 2] * er - o[4 * h + 3] * ei;
+                            acc[c][4 * h + 3] += o[4 * h + 2] * ei + o[4 * h + 3] * er;
+                        }
+                    }
+                }
+            }
+            wave_sync();
+            n0 = n1 + 1;
+        }
+    }
+}
+
+// register accumulators -> expansion layout (every (n, m) <= P written exactly once)
+template <bool ADD>
+__device__ inline void acc_store(double* dst, int P, const double (&acc)[MAXCH][8]) {
+    int n0 = 0;
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+        if (n0 <= P) {
+            const int n1 = chunk_end(n0, P, 64);
+            const int k = lane_id();
+            const int base = items_below(n0);
+            if (k < items_below(n1 + 1) - base) {
+                int n, j;
+                item_nj(base + k, n, j);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int m = 2 * j + h;
+                    if (m > n) break;
+                    double* d = dst + 4 * (nh(n - 1) + m);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (ADD) d[q] += acc[c][4 * h + q];
+                        else d[q] = acc[c][4 * h + q];
+                    }
+                }
+            }
+            n0 = n1 + 1;
+        }
+    }
 }
 
 enum { OP_M2M = 0, OP_M2L = 1, OP_L2L = 2 };
@@ -361,8 +566,8 @@ __device__ inline void translate_one(const double* in, double t, int P, int n, i
         for (int np = n; np >= m; --np) {
             double tmp = t_n_np / n_np_f;
             int iw = (np * (np + 1)) / 2 + m;
-            v1r += tmp * EX(in, iw, 0, 0); v1i += tmp * EX(in, iw, 0, 1);
-            if (LH) { v2r += tmp * EX(in, iw, 1, 0); v2i += tmp * EX(in, iw, 1, 1); }
+            v1r = fma(tmp, in[4 * iw], v1r); v1i = fma(tmp, in[4 * iw + 1], v1i);
+            if (LH) { v2r = fma(tmp, in[4 * iw + 2], v2r); v2i = fma(tmp, in[4 * iw + 3], v2i); }
             t_n_np *= _t;
             n_np_f *= (double)n_np;
             n_np += 1;
@@ -371,14 +576,13 @@ __device__ inline void translate_one(const double* in, double t, int P, int n, i
         const double one_over_t = 1.0 / t;
         double nf = one_over_t;   // n!/t^(n+1)
         for (int k = 0; k < n; ++k) nf *= (double)(k + 1) * one_over_t;
-        double nmf = nf;          // (n+m)!/t^(n+m+1)
-        for (int k = 0; k < m; ++k) nmf *= (double)(n + k + 1) * one_over_t;
-        double f = nmf;
+        double f = nf;            // (n+m)!/t^(n+m+1)
+        for (int k = 0; k < m; ++k) f *= (double)(n + k + 1) * one_over_t;
         int n_np = n + m;
         for (int np = m; np <= P; ++np) {
             int iw = (np * (np + 1)) / 2 + m;
-            v1r += f * EX(in, iw, 0, 0); v1i += f * EX(in, iw, 0, 1);
-            if (LH) { v2r += f * EX(in, iw, 1, 0); v2i += f * EX(in, iw, 1, 1); }
+            v1r = fma(f, in[4 * iw], v1r); v1i = fma(f, in[4 * iw + 1], v1i);
+            if (LH) { v2r = fma(f, in[4 * iw + 2], v2r); v2i = fma(f, in[4 * iw + 3], v2i); }
             n_np += 1;
             f *= (double)n_np * one_over_t;
         }
@@ -388,8 +592,8 @@ __device__ inline void translate_one(const double* in, double t, int P, int n, i
         int n_np = 1;
         for (int np = n; np <= P; ++np) {
             int iw = (np * (np + 1)) / 2 + m;
-            v1r += t_n_nf * EX(in, iw, 0, 0); v1i += t_n_nf * EX(in, iw, 0, 1);
-            if (LH) { v2r += t_n_nf * EX(in, iw, 1, 0); v2i += t_n_nf * EX(in, iw, 1, 1); }
+            v1r = fma(t_n_nf, in[4 * iw], v1r); v1i = fma(t_n_nf, in[4 * iw + 1], v1i);
+            if (LH) { v2r = fma(t_n_nf, in[4 * iw + 2], v2r); v2i = fma(t_n_nf, in[4 * iw + 3], v2i); }
             t_n_nf *= _t / (double)n_np;
             n_np += 1;
         }
@@ -402,14 +606,14 @@ __device__ inline void translate_one(const double* in, double t, int P, int n, i
 template <bool LOCAL>
 __device__ inline void lh_one(const double* in, double r, int P, int n, int m, double out[4]) {
     const int i = (n * (n + 1)) / 2 + m;
-    double pr = EX(in, i, 0, 0), pi = EX(in, i, 0, 1), cr = EX(in, i, 1, 0), ci = EX(in, i, 1, 1);
+    double pr = in[4 * i], pi = in[4 * i + 1], cr = in[4 * i + 2], ci = in[4 * i + 3];
     if (!LOCAL) {
         if (n >= 1) {
             double r_m_np1 = r * (double)m / (double)(n + 1);
             out[0] = pr + r_m_np1 * ci;
             out[1] = pi - r_m_np1 * cr;
             double c1r = 0, c1i = 0;
-            if (m < n) { c1r = EX(in, i - n, 1, 0); c1i = EX(in, i - n, 1, 1); }
+            if (m < n) { c1r = in[4 * (i - n) + 2]; c1i = in[4 * (i - n) + 3]; }
             double r_n = r / (double)n;
             out[2] = cr + r_n * c1r;
             out[3] = ci + r_n * c1i;
@@ -421,7 +625,7 @@ __device__ inline void lh_one(const double* in, double r, int P, int n, int m, d
             out[1] = pi + r_m_n * cr;
         } else { out[0] = pr; out[1] = pi; }
         if (n < P) {
-            double c1r = EX(in, i + n + 1, 1, 0), c1i = EX(in, i + n + 1, 1, 1);
+            double c1r = in[4 * (i + n + 1) + 2], c1i = in[4 * (i + n + 1) + 3];
             double r_np1 = r / (double)(n + 1);
             out[2] = cr - r_np1 * c1r;
             out[3] = ci - r_np1 * c1i;
@@ -429,150 +633,127 @@ __device__ inline void lh_one(const double* in, double r, int P, int n, int m, d
     }
 }
 
-// full rotate-translate-rotate of one expansion into acc (+=), all lanes
+// z-translation X -> R, then Lamb-Helmholtz transform and input scale of the back
+// rotation R -> X, for orders <= Pu
 template <int OP, bool LH>
-__device__ void translate_full(const double* __restrict__ src, double dx, double dy, double dz, const Lds& L, int P,
-                               const TablePtrs& tb) {
-    const int H = nh(P);
-    double r, th, ph;
-    c2s(dx, dy, dz, r, th, ph);
-    eim_fill(L.eim, ph, P);
-    wigner_all(L.T, tb, th, P);
-    __syncthreads();
-    for (int i = threadIdx.x; i < H; i += 64) {   // rotate_z (src/rotate.jl:448-483)
+__device__ inline void translate_and_transform(const WaveLds& W, double r, int Pu, const double* back_mags) {
+    const int H = nh(Pu);
+    for (int i = lane_id(); i < H; i += 64) {
         int n, m;
         idx_nm(i, n, m);
-        double er = L.eim[2 * m], ei = L.eim[2 * m + 1];
-        double xr = src[4 * i], xi = src[4 * i + 1];
-        EX(L.w1, i, 0, 0) = xr * er - xi * ei;
-        EX(L.w1, i, 0, 1) = xr * ei + xi * er;
+        double o[4];
+        translate_one<OP, LH>(W.X, r, Pu, n, m, o);
+        W.R[4 * i] = o[0]; W.R[4 * i + 1] = o[1];
+        if (LH) { W.R[4 * i + 2] = o[2]; W.R[4 * i + 3] = o[3]; }
+    }
+    wave_sync();
+    for (int i = lane_id(); i < H; i += 64) {
+        int n, m;
+        idx_nm(i, n, m);
+        const double g = back_mags[len_Ts(n - 1) + m];
         if (LH) {
-            xr = src[4 * i + 2]; xi = src[4 * i + 3];
-            EX(L.w1, i, 1, 0) = xr * er - xi * ei;
-            EX(L.w1, i, 1, 1) = xr * ei + xi * er;
-        }
-    }
-    __syncthreads();
-    const double* mag_fwd = (OP == OP_L2L) ? tb.eta : tb.zeta;
-    for (int i = threadIdx.x; i < H; i += 64) {
-        int n, m;
-        idx_nm(i, n, m);
-        double o[4];
-        if (OP == OP_L2L) rotate_y_one<true, LH>(L.w1, L.T, mag_fwd, n, m, o);
-        else rotate_y_one<false, LH>(L.w1, L.T, mag_fwd, n, m, o);
-        EX(L.w2, i, 0, 0) = o[0]; EX(L.w2, i, 0, 1) = o[1];
-        if (LH) { EX(L.w2, i, 1, 0) = o[2]; EX(L.w2, i, 1, 1) = o[3]; }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < H; i += 64) {
-        int n, m;
-        idx_nm(i, n, m);
-        double o[4];
-        translate_one<OP, LH>(L.w2, r, P, n, m, o);
-        EX(L.w1, i, 0, 0) = o[0]; EX(L.w1, i, 0, 1) = o[1];
-        if (LH) { EX(L.w1, i, 1, 0) = o[2]; EX(L.w1, i, 1, 1) = o[3]; }
-    }
-    __syncthreads();
-    const double* back_in = L.w1;
-    if (LH) {
-        for (int i = threadIdx.x; i < H; i += 64) {
-            int n, m;
-            idx_nm(i, n, m);
             double o[4];
-            if (OP == OP_M2M) lh_one<false>(L.w1, r, P, n, m, o);
-            else lh_one<true>(L.w1, r, P, n, m, o);
-            EX(L.w2, i, 0, 0) = o[0]; EX(L.w2, i, 0, 1) = o[1]; EX(L.w2, i, 1, 0) = o[2]; EX(L.w2, i, 1, 1) = o[3];
-        }
-        __syncthreads();
-        back_in = L.w2;
-    }
-    const double* mag_back = (OP == OP_M2M) ? tb.zeta : tb.eta;
-    for (int i = threadIdx.x; i < H; i += 64) {   // back y-rotation, back z-rotation, accumulate
-        int n, m;
-        idx_nm(i, n, m);
-        double o[4];
-        if (OP == OP_M2M) rotate_y_one<false, LH>(back_in, L.T, mag_back, n, m, o);
-        else rotate_y_one<true, LH>(back_in, L.T, mag_back, n, m, o);
-        double er = L.eim[2 * m], ei = -L.eim[2 * m + 1];
-        EX(L.acc, i, 0, 0) += o[0] * er - o[1] * ei;
-        EX(L.acc, i, 0, 1) += o[0] * ei + o[1] * er;
-        if (LH) {
-            EX(L.acc, i, 1, 0) += o[2] * er - o[3] * ei;
-            EX(L.acc, i, 1, 1) += o[2] * ei + o[3] * er;
+            lh_one<OP != OP_M2M>(W.R, r, Pu, n, m, o);
+            W.X[4 * i] = g * o[0]; W.X[4 * i + 1] = g * o[1]; W.X[4 * i + 2] = g * o[2]; W.X[4 * i + 3] = g * o[3];
+        } else {
+            W.X[4 * i] = g * W.R[4 * i]; W.X[4 * i + 1] = g * W.R[4 * i + 1];
         }
     }
-    __syncthreads();
+    wave_sync();
 }
 
-__device__ inline void acc_zero(const Lds& L, int P) {
-    for (int i = threadIdx.x; i < 4 * nh(P); i += 64) L.acc[i] = 0.0;
-    __syncthreads();
+// complete fixed-order translation of one expansion, accumulated into acc
+template <int OP, bool LH>
+__device__ void translate_fixed(const double* __restrict__ src, double dx, double dy, double dz, const WaveLds& W,
+                                const double* Hs, int P, const TablePtrs& tb, double (&acc)[MAXCH][8]) {
+    double r, th, ph;
+    c2s(dx, dy, dz, r, th, ph);
+    fill_angles(W, ph, th, P);
+    wave_sync();
+    const double* fwd_mags = (OP == OP_L2L) ? tb.eta : tb.zeta;
+    const double* back_mags = (OP == OP_M2M) ? tb.zeta : tb.eta;
+    const double sf = (OP == OP_L2L) ? -1.0 : 1.0, sb = (OP == OP_M2M) ? 1.0 : -1.0;
+    zrot_in<LH>(src, W, fwd_mags, 0, P);
+    wave_sync();
+    for (int n0 = 0; n0 <= P;) {
+        const int n1 = chunk_end(n0, P, 64);
+        rot_chunk_store<LH>(W, Hs, fwd_mags, sf, n0, n1);
+        n0 = n1 + 1;
+    }
+    translate_and_transform<OP, LH>(W, r, P, back_mags);
+    rot_back_acc<LH>(W, Hs, back_mags, sb, P, P, acc);
 }
 
 // ================================================================= M2M
-__global__ __launch_bounds__(64) void k_m2m_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0, int64_t b1,
-                                                  int P, int LHi, TablePtrs tb) {
+template <bool LH>
+__global__ __launch_bounds__(64 * WPB) void k_m2m_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
+                                                        int64_t b1, int P, TablePtrs tb) {
     extern __shared__ double lds[];
-    const int64_t ib = b0 + blockIdx.x;
+    const double* Hs = load_hs(lds, tb.Hs, P);
+    const WaveLds W = carve_wave(lds + hs_lds_doubles(P), P);
+    const int64_t ib = b0 + (int64_t)blockIdx.x * WPB + wave_id();
     if (ib > b1) return;
     const Branch B = br[ib];
     if (B.n_branches == 0) return;
-    Lds L = carve(lds, P);
-    acc_zero(L, P);
     const int H4 = 4 * nh(P);
+    double acc[MAXCH][8] = {};
     for (int64_t c = B.child_first; c < B.child_first + B.n_branches; ++c) {
         const Branch& Ch = br[c];
         double dx = B.source_center[0] - Ch.source_center[0], dy = B.source_center[1] - Ch.source_center[1],
                dz = B.source_center[2] - Ch.source_center[2];
-        if (LHi) translate_full<OP_M2M, true>(ex + c * H4, dx, dy, dz, L, P, tb);
-        else translate_full<OP_M2M, false>(ex + c * H4, dx, dy, dz, L, P, tb);
+        translate_fixed<OP_M2M, LH>(ex + c * H4, dx, dy, dz, W, Hs, P, tb, acc);
     }
-    for (int i = threadIdx.x; i < H4; i += 64) ex[ib * H4 + i] += L.acc[i];
+    acc_store<true>(ex + ib * H4, P, acc);
 }
 
 void run_m2m(Context& C, Tree& st, int p, bool LH) {
     ensure_tables(C, p);
     TablePtrs tb = ptrs(C);
-    const size_t sh = lds_bytes(p);
+    const size_t sh = LH ? translate_setup(k_m2m_level<true>, p) : translate_setup(k_m2m_level<false>, p);
     for (int64_t l = (int64_t)st.level_first.size() - 2; l >= 0; --l) {
         int64_t b0 = st.level_first[l], b1 = st.level_last[l];
-        k_m2m_level<<<(unsigned)(b1 - b0 + 1), 64, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, LH ? 1 : 0, tb);
+        const unsigned grid = (unsigned)((b1 - b0 + WPB) / WPB);
+        if (LH) k_m2m_level<true><<<grid, 64 * WPB, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, tb);
+        else k_m2m_level<false><<<grid, 64 * WPB, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, tb);
         LAUNCH_CHECK();
     }
 }
 
 // ================================================================= L2L
-__global__ __launch_bounds__(64) void k_l2l_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0, int64_t b1,
-                                                  int P, int LHi, TablePtrs tb) {
+template <bool LH>
+__global__ __launch_bounds__(64 * WPB) void k_l2l_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
+                                                        int64_t b1, int P, TablePtrs tb) {
     extern __shared__ double lds[];
-    const int64_t ib = b0 + blockIdx.x;
+    const double* Hs = load_hs(lds, tb.Hs, P);
+    const WaveLds W = carve_wave(lds + hs_lds_doubles(P), P);
+    const int64_t ib = b0 + (int64_t)blockIdx.x * WPB + wave_id();
     if (ib > b1) return;
     const Branch B = br[ib];
     const Branch& Pa = br[B.parent];
-    Lds L = carve(lds, P);
-    acc_zero(L, P);
     const int H4 = 4 * nh(P);
+    double acc[MAXCH][8] = {};
     double dx = B.target_center[0] - Pa.target_center[0], dy = B.target_center[1] - Pa.target_center[1],
            dz = B.target_center[2] - Pa.target_center[2];
-    if (LHi) translate_full<OP_L2L, true>(ex + B.parent * H4, dx, dy, dz, L, P, tb);
-    else translate_full<OP_L2L, false>(ex + B.parent * H4, dx, dy, dz, L, P, tb);
-    for (int i = threadIdx.x; i < H4; i += 64) ex[ib * H4 + i] += L.acc[i];
+    translate_fixed<OP_L2L, LH>(ex + B.parent * H4, dx, dy, dz, W, Hs, P, tb, acc);
+    acc_store<true>(ex + ib * H4, P, acc);
 }
 
 void run_l2l(Context& C, Tree& tt, int p, bool LH) {
     ensure_tables(C, p);
     TablePtrs tb = ptrs(C);
-    const size_t sh = lds_bytes(p);
+    const size_t sh = LH ? translate_setup(k_l2l_level<true>, p) : translate_setup(k_l2l_level<false>, p);
     for (size_t l = 1; l < tt.level_first.size(); ++l) {
         int64_t b0 = tt.level_first[l], b1 = tt.level_last[l];
-        k_l2l_level<<<(unsigned)(b1 - b0 + 1), 64, sh, C.stream>>>(tt.branches.data(), tt.expansions.data(), b0, b1, p, LH ? 1 : 0, tb);
+        const unsigned grid = (unsigned)((b1 - b0 + WPB) / WPB);
+        if (LH) k_l2l_level<true><<<grid, 64 * WPB, sh, C.stream>>>(tt.branches.data(), tt.expansions.data(), b0, b1, p, tb);
+        else k_l2l_level<false><<<grid, 64 * WPB, sh, C.stream>>>(tt.branches.data(), tt.expansions.data(), b0, b1, p, tb);
         LAUNCH_CHECK();
     }
 }
 
 // ================================================================= M2L
 struct ErrParams {
-    int kind;        // 1 rotated coefficients (gradient), 2 power potential, 3 power gradient
+    int kind;        // 0 fixed order, 1 rotated coefficients (gradient), 2 power potential, 3 power gradient
     int relative;
     double eps;
     int BE;
@@ -585,166 +766,153 @@ __device__ inline double md1(double dx, double rm, double rp) {   // src/error.j
     return same ? mn : 0.0;
 }
 
-// error-controlled M2L of one pair into acc; returns the expansion order used (src/translate.jl:555-1412)
-template <bool LH>
-__device__ int m2l_error_pair(const double* __restrict__ src, const Branch& tb_, const Branch& sb_, const Lds& L, int P,
-                              const TablePtrs& tb, const ErrParams& E, int& success) {
-    const int H = nh(P);
-    double* misc = L.misc;             // [0] decision flag, [1..] scratch
-    double* tmp = L.misc + 16;         // 4(P+1) translated degree-n coefficients
-    const double eps = E.relative ? tb_.max_influence * E.eps : E.eps;
-    double dx = tb_.target_center[0] - sb_.source_center[0], dy = tb_.target_center[1] - sb_.source_center[1],
-           dz = tb_.target_center[2] - sb_.source_center[2];
-    double r, th, ph;
-    c2s(dx, dy, dz, r, th, ph);
-    double d0 = md1(tb_.target_center[0] - sb_.source_center[0], -tb_.target_box[0], tb_.target_box[0]);
-    double d1 = md1(tb_.target_center[1] - sb_.source_center[1], -tb_.target_box[1], tb_.target_box[1]);
-    double d2 = md1(tb_.target_center[2] - sb_.source_center[2], -tb_.target_box[2], tb_.target_box[2]);
-    const double r_mp = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    const double r_l = tb_.target_radius;
-    const double rinv = 1.0 / r, r_mp_inv = 1.0 / r_mp;
-    double r_mp_inv_np2 = r_mp_inv * r_mp_inv * r_mp_inv;
-    double r_l_nm1 = 1.0;
-    const double sbeta = sin(th), cbeta = cos(th);
-    eim_fill(L.eim, ph, P);
-    if (threadIdx.x == 0) {
-        L.T[0] = 1.0;
-        EX(L.w1, 0, 0, 0) = src[0]; EX(L.w1, 0, 0, 1) = src[1];
-        if (LH) { EX(L.w1, 0, 1, 0) = src[2]; EX(L.w1, 0, 1, 1) = src[3]; }
+// Geometry and running factors of the degree loop of the error-controlled
+// multipole_to_local! methods (src/translate.jl:555-1412). The reference walks
+// the degrees in order, updating running products after each degree; here each
+// lane evaluates the test of one degree, replaying the same product sequence.
+struct ErrGeo {
+    double limit, rinv, r, r_mp, r_mp_inv, r_l;
+};
+
+struct DegFactors {
+    double n_f_t_np1, r_l_nm1, r_mp_inv_np2, nm1_f_inv, np1_f;
+};
+
+__device__ inline DegFactors deg_factors(const ErrGeo& G, int n) {   // values at the start of iteration n
+    DegFactors F;
+    F.n_f_t_np1 = G.rinv * G.rinv;
+    F.r_l_nm1 = 1.0;
+    F.r_mp_inv_np2 = G.r_mp_inv * G.r_mp_inv * G.r_mp_inv;
+    F.nm1_f_inv = 1.0;
+    F.np1_f = 2.0;
+    for (int k = 1; k < n; ++k) {
+        F.n_f_t_np1 *= (double)(k + 1) * G.rinv;
+        F.r_l_nm1 *= G.r_l;
+        F.r_mp_inv_np2 *= G.r_mp_inv;
+        F.nm1_f_inv /= (double)k;
+        F.np1_f *= (double)(k + 1);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {   // degree 0 rotation (T[0] = 1)
-        double o[4];
-        rotate_y_one<false, LH>(L.w1, L.T, tb.zeta, 0, 0, o);
-        EX(L.w2, 0, 0, 0) = o[0]; EX(L.w2, 0, 0, 1) = o[1]; EX(L.w2, 0, 1, 0) = o[2]; EX(L.w2, 0, 1, 1) = o[3];
-    }
-    double n_f_t_np1 = rinv * rinv, nm1_f_inv = 1.0, np1_f = 2.0;
-    const bool power = E.kind >= 2;
-    double Mt_n0 = tb.Mt[0], mp_pow = 0.0;
-    if (power) { double a = src[0], b = src[1]; mp_pow = sqrt((a * a + b * b) * Mt_n0 * Mt_n0); }
-    int Pused = P;
-    success = 0;
-    const double limit = eps * 4 * PI_D;
-    for (int n = 1; n <= P; ++n) {
-        const int in0 = (n * (n + 1)) / 2;
-        // rotate_z degree n (src/rotate.jl:503-579)
-        for (int m = threadIdx.x; m <= n; m += 64) {
-            double er = L.eim[2 * m], ei = L.eim[2 * m + 1];
-            int i = in0 + m;
+    return F;
+}
+
+// power of the unrotated multipole of degree k: sqrt(sum_m |M_k^m|^2 Mt_k^m^2) (phi, chi)
+__device__ inline void degree_power(const double* __restrict__ src, const double* Mt, int k, bool LH, double& pw, double& pc) {
+    const int i0 = nh(k - 1);
+    double a = 0.0, b = 0.0;
+    if (k == 0) {
+        double x = src[0], y = src[1];
+        a = (x * x + y * y) * Mt[0] * Mt[0];
+    } else {
+        for (int m = 0; m <= k; ++m) {
+            const int i = i0 + m;
+            const double Nt = Mt[i];
             double xr = src[4 * i], xi = src[4 * i + 1];
-            EX(L.w1, i, 0, 0) = xr * er - xi * ei;
-            EX(L.w1, i, 0, 1) = xr * ei + xi * er;
-            if (LH) {
-                xr = src[4 * i + 2]; xi = src[4 * i + 3];
-                EX(L.w1, i, 1, 0) = xr * er - xi * ei;
-                EX(L.w1, i, 1, 1) = xr * ei + xi * er;
+            a += (xr * xr + xi * xi) * Nt * Nt;
+            if (LH) { xr = src[4 * i + 2]; xi = src[4 * i + 3]; b += (xr * xr + xi * xi) * Nt * Nt; }
+        }
+    }
+    pw = sqrt(a);
+    pc = sqrt(b);
+}
+
+// Tests of degrees nA..nB (rotated multipole of degrees <= nB in W.X). Returns
+// the first degree that meets the tolerance, or -1.
+template <bool LH>
+__device__ int m2l_test_chunk(const double* __restrict__ src, const WaveLds& W, const ErrGeo& G, const ErrParams& E,
+                              const TablePtrs& tb, int nA, int nB) {
+    if (E.kind >= 2) {
+        // degree-n part of the z-translation, all m (translate_multipole_to_local_z_n!, :238-295) -> R
+        const int i0 = nh(nA - 1), cnt = nh(nB) - i0;
+        for (int k = lane_id(); k < cnt; k += 64) {
+            int n, m;
+            idx_nm(i0 + k, n, m);
+            const DegFactors F = deg_factors(G, n);
+            double n_m_f = F.n_f_t_np1;
+            for (int q = 0; q < m; ++q) n_m_f *= (double)(n + q + 1) * G.rinv;
+            double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
+            int n_np = n + m;
+            for (int np = m; np <= n; ++np) {
+                const int iw = (np * (np + 1)) / 2 + m;
+                v1r += f * W.X[4 * iw]; v1i += f * W.X[4 * iw + 1];
+                if (LH) { v2r += f * W.X[4 * iw + 2]; v2i += f * W.X[4 * iw + 3]; }
+                n_np += 1;
+                f *= (double)n_np * G.rinv;
             }
+            double* t = W.R + 4 * k;
+            t[0] = v1r; t[1] = v1i; t[2] = v2r; t[3] = v2i;
         }
-        // Wigner entries of degree n
-        for (int k = threadIdx.x; k < nh(n); k += 64) {
-            int m, mp;
-            idx_nm(k, m, mp);
-            wigner_entry(L.T, tb.Hs, sbeta, cbeta, n, m, mp);
-        }
-        __syncthreads();
-        for (int m = threadIdx.x; m <= n; m += 64) {
-            double o[4];
-            rotate_y_one<false, LH>(L.w1, L.T, tb.zeta, n, m, o);
-            int i = in0 + m;
-            EX(L.w2, i, 0, 0) = o[0]; EX(L.w2, i, 0, 1) = o[1];
-            EX(L.w2, i, 1, 0) = LH ? o[2] : 0.0; EX(L.w2, i, 1, 1) = LH ? o[3] : 0.0;
-        }
-        __syncthreads();
-        // multipole error (uniform across lanes: every lane reads the same LDS values)
-        double Mt_next = tb.Mt[in0], pow_next = 0.0, pow_c_next = 0.0;
-        if (power) {
-            for (int m = 0; m <= n; ++m) {
-                int i = in0 + m;
-                double Nt = tb.Mt[i];
-                double xr = src[4 * i], xi = src[4 * i + 1];
-                pow_next += (xr * xr + xi * xi) * Nt * Nt;
-                if (LH) { xr = src[4 * i + 2]; xi = src[4 * i + 3]; pow_c_next += (xr * xr + xi * xi) * Nt * Nt; }
-            }
-            pow_next = sqrt(pow_next);
-            pow_c_next = sqrt(pow_c_next);
-        }
+        wave_sync();
+    }
+    bool pass = false;
+    const int n = nA + lane_id();
+    if (n <= nB) {
+        const DegFactors F = deg_factors(G, n);
+        const int in0 = nh(n - 1);
         double eps_mp;
         if (E.kind == 1) {
-            double p0 = EX(L.w2, in0, 0, 0), p1r = EX(L.w2, in0 + 1, 0, 0), p1i = EX(L.w2, in0 + 1, 0, 1);
-            eps_mp = (sqrt(p1r * p1r + p1i * p1i) + fabs(p0)) * np1_f * r_mp_inv_np2;
+            const double p0 = W.X[4 * in0], p1r = W.X[4 * (in0 + 1)], p1i = W.X[4 * (in0 + 1) + 1];
+            eps_mp = (sqrt(p1r * p1r + p1i * p1i) + fabs(p0)) * F.np1_f * F.r_mp_inv_np2;
             if (LH) {
-                double c1r = EX(L.w2, in0 + 1, 1, 0), c1i = EX(L.w2, in0 + 1, 1, 1);
-                eps_mp += sqrt(c1r * c1r + c1i * c1i) * np1_f * r_mp_inv_np2 * r_mp;
+                const double c1r = W.X[4 * (in0 + 1) + 2], c1i = W.X[4 * (in0 + 1) + 3];
+                eps_mp += sqrt(c1r * c1r + c1i * c1i) * F.np1_f * F.r_mp_inv_np2 * G.r_mp;
             }
-        } else if (E.kind == 3) {
-            eps_mp = SQRT3_D * mp_pow * r_mp_inv_np2 * np1_f / Mt_n0;
-            if (LH) eps_mp += SQRT3_D * (double)n * pow_c_next * r_mp_inv_np2 * np1_f / Mt_next;
         } else {
-            eps_mp = SQRT3_D * pow_next * r_mp_inv_np2 * r_mp * np1_f / (Mt_n0 * (double)(n + 1));
-            if (LH) eps_mp += SQRT3_D * (double)n * pow_c_next * r_mp_inv_np2 * r_mp * np1_f / (Mt_next * (double)(n + 1));
+            double pw_prev, pc_prev, pw_n, pc_n;
+            degree_power(src, tb.Mt, n - 1, LH, pw_prev, pc_prev);
+            degree_power(src, tb.Mt, n, LH, pw_n, pc_n);
+            const double Mt_n0 = tb.Mt[nh(n - 2)], Mt_next = tb.Mt[in0];
+            if (E.kind == 3) {
+                eps_mp = SQRT3_D * pw_prev * F.r_mp_inv_np2 * F.np1_f / Mt_n0;
+                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * F.r_mp_inv_np2 * F.np1_f / Mt_next;
+            } else {
+                eps_mp = SQRT3_D * pw_n * F.r_mp_inv_np2 * G.r_mp * F.np1_f / (Mt_n0 * (double)(n + 1));
+                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * F.r_mp_inv_np2 * G.r_mp * F.np1_f / (Mt_next * (double)(n + 1));
+            }
         }
-        bool done = false;
-        if (eps_mp <= limit) {
+        if (eps_mp <= G.limit) {
             double eps_l;
-            if (E.kind == 1) {   // translate_multipole_to_local_z_m01_n (:302-362), all lanes redundantly
-                const double one_over_t = rinv;
-                double n_m_f = n_f_t_np1;
+            if (E.kind == 1) {   // translate_multipole_to_local_z_m01_n (:302-362)
+                double n_m_f = F.n_f_t_np1;
                 int n_m = n;
                 double p0r = 0.0, n_np_f = n_m_f;
                 int n_np = n_m;
                 for (int np = 0; np <= n; ++np) {
-                    p0r += n_np_f * EX(L.w2, (np * (np + 1)) / 2, 0, 0);
+                    p0r += n_np_f * W.X[4 * ((np * (np + 1)) / 2)];
                     n_np += 1;
-                    n_np_f *= (double)n_np * one_over_t;
+                    n_np_f *= (double)n_np * G.rinv;
                 }
                 n_m += 1;
-                n_m_f *= (double)n_m * one_over_t;
+                n_m_f *= (double)n_m * G.rinv;
                 double p1r = 0, p1i = 0, c1r = 0, c1i = 0;
                 n_np_f = n_m_f;
                 n_np = n_m;
                 for (int np = 1; np <= n; ++np) {
-                    int iw = (np * (np + 1)) / 2 + 1;
-                    p1r += n_np_f * EX(L.w2, iw, 0, 0);
-                    p1i += n_np_f * EX(L.w2, iw, 0, 1);
-                    if (LH) { c1r += n_np_f * EX(L.w2, iw, 1, 0); c1i += n_np_f * EX(L.w2, iw, 1, 1); }
+                    const int iw = (np * (np + 1)) / 2 + 1;
+                    p1r += n_np_f * W.X[4 * iw];
+                    p1i += n_np_f * W.X[4 * iw + 1];
+                    if (LH) { c1r += n_np_f * W.X[4 * iw + 2]; c1i += n_np_f * W.X[4 * iw + 3]; }
                     n_np += 1;
-                    n_np_f *= (double)n_np * one_over_t;
+                    n_np_f *= (double)n_np * G.rinv;
                 }
                 if (LH) {
-                    double t_over_n = r / (double)n;
+                    const double t_over_n = G.r / (double)n;
                     p1r -= t_over_n * c1i;
                     p1i += t_over_n * c1r;
                 }
-                eps_l = (fabs(p1r) + fabs(p1i) + fabs(p0r)) * r_l_nm1 * nm1_f_inv;
-                if (LH) eps_l += sqrt(c1r * c1r + c1i * c1i) * r_l_nm1 * r_l * nm1_f_inv;
-            } else {
-                // translate_multipole_to_local_z_n! (:238-295) for degree n, lanes over m
-                for (int m = threadIdx.x; m <= n; m += 64) {
-                    double n_m_f = n_f_t_np1;
-                    for (int k = 0; k < m; ++k) n_m_f *= (double)(n + k + 1) * rinv;
-                    double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
-                    int n_np = n + m;
-                    for (int np = m; np <= n; ++np) {
-                        int iw = (np * (np + 1)) / 2 + m;
-                        v1r += f * EX(L.w2, iw, 0, 0); v1i += f * EX(L.w2, iw, 0, 1);
-                        if (LH) { v2r += f * EX(L.w2, iw, 1, 0); v2i += f * EX(L.w2, iw, 1, 1); }
-                        n_np += 1;
-                        f *= (double)n_np * rinv;
-                    }
-                    tmp[4 * m] = v1r; tmp[4 * m + 1] = v1i; tmp[4 * m + 2] = v2r; tmp[4 * m + 3] = v2i;
-                }
-                __syncthreads();
-                // local_power (:455-502)
+                eps_l = (fabs(p1r) + fabs(p1i) + fabs(p0r)) * F.r_l_nm1 * F.nm1_f_inv;
+                if (LH) eps_l += sqrt(c1r * c1r + c1i * c1i) * F.r_l_nm1 * G.r_l * F.nm1_f_inv;
+            } else {   // local_power (:455-502) of the degree-n translation
+                const double* tmp = W.R + 4 * (in0 - nh(nA - 1));
                 double lp = 0, lc = 0, Nt = 0, cnr = 0, cni = 0;
                 for (int m = n; m >= 0; --m) {
                     Nt = tb.Lt[in0 + m];
                     double pr = tmp[4 * m], pi = tmp[4 * m + 1];
                     if (LH) {
-                        double cor = tmp[4 * m + 2], coi = tmp[4 * m + 3];
-                        pr = pr - coi * r * (double)m / (double)n;
-                        pi = pi + cor * r * (double)m / (double)n;
-                        double cr = cor - cnr * r / (double)(n + 1);
-                        double ci = coi - cni * r / (double)(n + 1);
+                        const double cor = tmp[4 * m + 2], coi = tmp[4 * m + 3];
+                        pr = pr - coi * G.r * (double)m / (double)n;
+                        pi = pi + cor * G.r * (double)m / (double)n;
+                        const double cr = cor - cnr * G.r / (double)(n + 1);
+                        const double ci = coi - cni * G.r / (double)(n + 1);
                         cnr = cor; cni = coi;
                         lc += (cr * cr + ci * ci) * Nt * Nt;
                     }
@@ -753,109 +921,337 @@ __device__ int m2l_error_pair(const double* __restrict__ src, const Branch& tb_,
                 lp = sqrt(lp);
                 if (LH) lc = sqrt(lc);
                 if (E.kind == 3) {
-                    eps_l = SQRT3_D * lp * r_l_nm1 * nm1_f_inv / Nt;
-                    if (LH) eps_l += SQRT3_D * (double)n * lc * r_l_nm1 * r_l * nm1_f_inv / (Nt * (double)n);
+                    eps_l = SQRT3_D * lp * F.r_l_nm1 * F.nm1_f_inv / Nt;
+                    if (LH) eps_l += SQRT3_D * (double)n * lc * F.r_l_nm1 * G.r_l * F.nm1_f_inv / (Nt * (double)n);
                 } else {
-                    eps_l = SQRT3_D * lp * r_l_nm1 * r_l * nm1_f_inv / (Nt * (double)n);
-                    if (LH) eps_l += SQRT3_D * (double)n * lc * r_l_nm1 * r_l * nm1_f_inv / (Nt * (double)n);
+                    eps_l = SQRT3_D * lp * F.r_l_nm1 * G.r_l * F.nm1_f_inv / (Nt * (double)n);
+                    if (LH) eps_l += SQRT3_D * (double)n * lc * F.r_l_nm1 * G.r_l * F.nm1_f_inv / (Nt * (double)n);
                 }
-                __syncthreads();
             }
-            if (eps_mp + eps_l * LOCAL_ERROR_SAFETY <= limit) {
-                Pused = n - 1 + (E.BE ? 1 : 0);
-                success = 1;
-                done = true;
-            }
-        }
-        if (done) break;
-        if (E.kind != 1 || n < P) {
-            n_f_t_np1 *= (double)(n + 1) * rinv;
-            r_l_nm1 *= r_l;
-            r_mp_inv_np2 *= r_mp_inv;
-            nm1_f_inv /= (double)n;
-            np1_f *= (double)(n + 1);
-        }
-        if (power) { mp_pow = pow_next; Mt_n0 = Mt_next; }
-    }
-    __syncthreads();
-    // translate, LH correction, back rotations into acc at order Pused
-    const int Hu = nh(Pused);
-    for (int i = threadIdx.x; i < Hu; i += 64) {
-        int n, m;
-        idx_nm(i, n, m);
-        double o[4];
-        translate_one<OP_M2L, LH>(L.w2, r, Pused, n, m, o);
-        EX(L.w1, i, 0, 0) = o[0]; EX(L.w1, i, 0, 1) = o[1];
-        if (LH) { EX(L.w1, i, 1, 0) = o[2]; EX(L.w1, i, 1, 1) = o[3]; }
-    }
-    __syncthreads();
-    const double* back_in = L.w1;
-    if (LH) {
-        for (int i = threadIdx.x; i < Hu; i += 64) {
-            int n, m;
-            idx_nm(i, n, m);
-            double o[4];
-            lh_one<true>(L.w1, r, Pused, n, m, o);
-            EX(L.w2, i, 0, 0) = o[0]; EX(L.w2, i, 0, 1) = o[1]; EX(L.w2, i, 1, 0) = o[2]; EX(L.w2, i, 1, 1) = o[3];
-        }
-        __syncthreads();
-        back_in = L.w2;
-    }
-    for (int i = threadIdx.x; i < Hu; i += 64) {
-        int n, m;
-        idx_nm(i, n, m);
-        double o[4];
-        rotate_y_one<true, LH>(back_in, L.T, tb.eta, n, m, o);
-        double er = L.eim[2 * m], ei = -L.eim[2 * m + 1];
-        EX(L.acc, i, 0, 0) += o[0] * er - o[1] * ei;
-        EX(L.acc, i, 0, 1) += o[0] * ei + o[1] * er;
-        if (LH) {
-            EX(L.acc, i, 1, 0) += o[2] * er - o[3] * ei;
-            EX(L.acc, i, 1, 1) += o[2] * ei + o[3] * er;
+            pass = (eps_mp + eps_l * LOCAL_ERROR_SAFETY <= G.limit);
         }
     }
-    __syncthreads();
-    (void)H;
-    (void)misc;
-    return Pused;
+    const unsigned long long mask = __ballot(pass);
+    wave_sync();
+    return mask ? nA + __ffsll((long long)mask) - 1 : -1;
 }
 
-__global__ __launch_bounds__(64) void k_m2l(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
-                                            const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
-                                            const Branch* __restrict__ sbr, const double* __restrict__ sex,
-                                            double* __restrict__ tex, int P, int LHi, ErrParams E, int32_t* __restrict__ orders,
-                                            int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
+// one M2L interaction (fixed order when E.kind == 0) accumulated into acc; returns the order used
+template <bool LH>
+__device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const Branch& S, const WaveLds& W,
+                        const double* Hs, int P, const TablePtrs& tb, const ErrParams& E, int& success,
+                        double (&acc)[MAXCH][8]) {
+    const double dx = T.target_center[0] - S.source_center[0], dy = T.target_center[1] - S.source_center[1],
+                 dz = T.target_center[2] - S.source_center[2];
+    double r, th, ph;
+    c2s(dx, dy, dz, r, th, ph);
+    fill_angles(W, ph, th, P);
+    wave_sync();
+    int Pu = P;
+    success = 1;
+    if (E.kind == 0) {
+        zrot_in<LH>(src, W, tb.zeta, 0, P);
+        wave_sync();
+        for (int n0 = 0; n0 <= P;) {
+            const int n1 = chunk_end(n0, P, 64);
+            rot_chunk_store<LH>(W, Hs, tb.zeta, 1.0, n0, n1);
+            n0 = n1 + 1;
+        }
+    } else {
+        ErrGeo G;
+        const double eps = E.relative ? T.max_influence * E.eps : E.eps;
+        G.limit = eps * 4 * PI_D;
+        const double d0 = md1(dx, -T.target_box[0], T.target_box[0]);
+        const double d1 = md1(dy, -T.target_box[1], T.target_box[1]);
+        const double d2 = md1(dz, -T.target_box[2], T.target_box[2]);
+        G.r = r;
+        G.r_mp = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        G.r_l = T.target_radius;
+        G.rinv = 1.0 / r;
+        G.r_mp_inv = 1.0 / G.r_mp;
+        success = 0;
+        for (int n0 = 0; n0 <= P;) {
+            const int n1 = chunk_end(n0, P, FWD_DEGREES);
+            zrot_in<LH>(src, W, tb.zeta, n0, n1);
+            wave_sync();
+            rot_chunk_store<LH>(W, Hs, tb.zeta, 1.0, n0, n1);
+            const int nA = n0 < 1 ? 1 : n0;
+            if (nA <= n1) {
+                const int hit = m2l_test_chunk<LH>(src, W, G, E, tb, nA, n1);
+                if (hit >= 0) {
+                    Pu = hit - 1 + (E.BE ? 1 : 0);
+                    success = 1;
+                    break;
+                }
+            }
+            n0 = n1 + 1;
+        }
+    }
+    translate_and_transform<OP_M2L, LH>(W, r, Pu, tb.eta);
+    rot_back_acc<LH>(W, Hs, tb.eta, -1.0, P, Pu, acc);
+    return Pu;
+}
+
+// One workgroup per target box: its wavefronts take the box's M2L sources in
+// turn, accumulate in registers, and the partial locals are summed through LDS
+// and added to the box's expansion once.
+template <bool LH>
+__global__ __launch_bounds__(64 * WPB) void k_m2l(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
+                                                  const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
+                                                  const Branch* __restrict__ sbr, const double* __restrict__ sex,
+                                                  double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
+                                                  int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
     extern __shared__ double lds[];
+    const double* Hs = load_hs(lds, tb.Hs, P);
+    const WaveLds W = carve_wave(lds + hs_lds_doubles(P), P);
     const int32_t it = m2l_targets[blockIdx.x];
     const int64_t l0 = m2l_offsets[blockIdx.x], l1 = m2l_offsets[blockIdx.x + 1];
-    Lds L = carve(lds, P);
-    acc_zero(L, P);
     const int H4 = 4 * nh(P);
     const Branch T = tbr[it];
+    double acc[MAXCH][8] = {};
     int local_max = 0, local_ok = 1;
-    for (int64_t li = l0; li < l1; ++li) {
+    for (int64_t li = l0 + wave_id(); li < l1; li += WPB) {
         const int32_t js = m2l_s[li];
         const Branch S = sbr[js];
-        const double* src = sex + (int64_t)js * H4;
-        int Pu = P, ok = 1;
-        if (E.kind == 0) {
-            double dx = T.target_center[0] - S.source_center[0], dy = T.target_center[1] - S.source_center[1],
-                   dz = T.target_center[2] - S.source_center[2];
-            if (LHi) translate_full<OP_M2L, true>(src, dx, dy, dz, L, P, tb);
-            else translate_full<OP_M2L, false>(src, dx, dy, dz, L, P, tb);
-        } else {
-            if (LHi) Pu = m2l_error_pair<true>(src, T, S, L, P, tb, E, ok);
-            else Pu = m2l_error_pair<false>(src, T, S, L, P, tb, E, ok);
-        }
-        if (threadIdx.x == 0) orders[li] = Pu;
+        int ok = 1;
+        const int Pu = m2l_pair<LH>(sex + (int64_t)js * H4, T, S, W, Hs, P, tb, E, ok, acc);
+        if (lane_id() == 0) orders[li] = Pu;
         local_max = max(local_max, Pu);
         local_ok = local_ok && ok;
     }
-    for (int i = threadIdx.x; i < H4; i += 64) tex[(int64_t)it * H4 + i] += L.acc[i];
-    if (threadIdx.x == 0) {
+    acc_store<false>(W.X, P, acc);
+    __syncthreads();
+    const double* base = lds + hs_lds_doubles(P);
+    const int stride = wave_lds_doubles(P);
+    for (int i = threadIdx.x; i < H4; i += 64 * WPB) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < WPB; ++w) s += base[(size_t)w * stride + i];
+        tex[(int64_t)it * H4 + i] += s;
+    }
+    if (lane_id() == 0) {
         atomicMax(pmax, local_max);
         if (!local_ok) atomicAnd(all_ok, 0);
     }
+}
+
+// ================================================= small orders: lane per M2L
+// For p <= 4 an expansion (at most 15 coefficients x 2 potentials) fits in
+// registers, so one lane runs one M2L interaction end to end with the same
+// factorised rotations, and the table reads are wavefront-uniform (scalar
+// loads). A wavefront owns one target box and takes 64 of its sources at a
+// time; the partial locals are summed across lanes at the end. This is the
+// order used by estimate_influence (p = 3, src/fmm.jl:1193-1276). Angles come
+// from the Cartesian components (cos b = z/r, sin b = rho/r, e^{i phi} = (x + iy)/rho),
+// which equal the reference's acos/atan2 route up to rounding.
+
+__host__ __device__ constexpr int hidx(int a, int b) { return a <= b ? ((b * (b + 1)) >> 1) + a : ((a * (a + 1)) >> 1) + b; }
+
+template <int P, bool LH>
+__device__ inline void lane_rotate(double (&x)[nh(P)][4], const double* __restrict__ Hs, const double* __restrict__ mags,
+                                   double sigma, const double (&cv)[P + 1], const double (&sv)[P + 1]) {
+    constexpr int NQ = LH ? 4 : 2;
+#pragma unroll
+    for (int n = 0; n <= P; ++n) {
+        const int i0 = nh(n - 1);
+        const double* Hn = Hs + len_Hs(n - 1);
+        const double* mg = mags + len_Ts(n - 1);
+        double U[P + 1][4];
+#pragma unroll
+        for (int mp = 0; mp <= n; ++mp) {
+            const double g = mg[mp];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) U[mp][q] = g * x[i0 + mp][q];
+        }
+        double Re[P + 1][4], Ro[P + 1][4];
+#pragma unroll
+        for (int v = 0; v <= n; ++v) {
+            double se[4] = {0.0, 0.0, 0.0, 0.0}, so[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int mp = 0; mp <= n; ++mp) {
+                const double a = Hn[hidx(mp, v)];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    if (mp & 1) so[q] = fma(a, U[mp][q], so[q]);
+                    else se[q] = fma(a, U[mp][q], se[q]);
+                }
+            }
+            const double a00 = Hn[hidx(0, v)];
+            const double sg = ((n + v) & 1) ? -1.0 : 1.0;
+            const double w = v ? 2.0 : 1.0;
+            const double c = w * cv[v], sn = w * sigma * sv[v];
+#pragma unroll
+            for (int pot = 0; pot < NQ / 2; ++pot) {
+                const double ser = se[2 * pot], sei = se[2 * pot + 1], sor = so[2 * pot], soi = so[2 * pot + 1];
+                const double dr = ser - a00 * U[0][2 * pot], di = sei - a00 * U[0][2 * pot + 1];
+                const double ppr = dr - sor + sg * (ser + sor), ppi = di - soi - sg * (sei + soi);
+                const double pmr = dr + sor + sg * (ser - sor), pmi = di + soi - sg * (sei - soi);
+                const double qpr = c * ppr - sn * ppi, qpi = c * ppi + sn * ppr;
+                const double qmr = c * pmr + sn * pmi, qmi = c * pmi - sn * pmr;
+                Re[v][2 * pot] = qpr + qmr; Re[v][2 * pot + 1] = qpi + qmi;
+                Ro[v][2 * pot] = qpr - qmr; Ro[v][2 * pot + 1] = qpi - qmi;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m <= n; ++m) {
+            double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int v = 0; v <= n; ++v) {
+                const double a = Hn[hidx(m, v)];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) a4[q] = fma(a, (m & 1) ? Ro[v][q] : Re[v][q], a4[q]);
+            }
+            const double g = 0.5 * mg[m * (n + 1)];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) x[i0 + m][q] = g * a4[q];
+        }
+    }
+}
+
+template <int P, bool LH>
+__global__ __launch_bounds__(256) void k_m2l_lane(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
+                                                  const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
+                                                  const Branch* __restrict__ sbr, const double* __restrict__ sex,
+                                                  double* __restrict__ tex, int64_t n_targets, int32_t* __restrict__ orders,
+                                                  int32_t* __restrict__ pmax, TablePtrs tb) {
+    constexpr int H = nh(P);
+    constexpr int NQ = LH ? 4 : 2;
+    const int64_t wt = (int64_t)blockIdx.x * 4 + wave_id();
+    if (wt >= n_targets) return;   // uniform per wavefront
+    const int lane = lane_id();
+    const int32_t it = m2l_targets[wt];
+    const int64_t l0 = m2l_offsets[wt], l1 = m2l_offsets[wt + 1];
+    const double tx = tbr[it].target_center[0], ty = tbr[it].target_center[1], tz = tbr[it].target_center[2];
+    double acc[H][4];
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
+    for (int64_t li = l0 + lane; li < l1; li += 64) {
+        const int32_t js = m2l_s[li];
+        const Branch& S = sbr[js];
+        const double dx = tx - S.source_center[0], dy = ty - S.source_center[1], dz = tz - S.source_center[2];
+        const double x2y2 = dx * dx + dy * dy;
+        const double r = sqrt(x2y2 + dz * dz);
+        double cb, sb, ce, se;
+        if (x2y2 > 0.0) {
+            const double rho = sqrt(x2y2);
+            cb = dz / r; sb = rho / r; ce = dx / rho; se = dy / rho;
+        } else {
+            cb = dz < 0.0 ? -1.0 : 1.0; sb = 0.0; ce = 1.0; se = 0.0;
+        }
+        double cv[P + 1], sv[P + 1], er[P + 1], ei[P + 1];
+        cv[0] = 1.0; sv[0] = 0.0; er[0] = 1.0; ei[0] = 0.0;
+#pragma unroll
+        for (int k = 1; k <= P; ++k) {
+            cv[k] = cb * cv[k - 1] - sb * sv[k - 1];
+            sv[k] = sb * cv[k - 1] + cb * sv[k - 1];
+            er[k] = er[k - 1] * ce - ei[k - 1] * se;
+            ei[k] = er[k - 1] * se + ei[k - 1] * ce;
+        }
+        double x[H][4];
+        const double* src = sex + (int64_t)js * 4 * H;
+#pragma unroll
+        for (int n = 0; n <= P; ++n)
+#pragma unroll
+            for (int m = 0; m <= n; ++m) {   // rotate_z! (src/rotate.jl:448-483)
+                const int i = nh(n - 1) + m;
+#pragma unroll
+                for (int pot = 0; pot < NQ / 2; ++pot) {
+                    const double a = src[4 * i + 2 * pot], b = src[4 * i + 2 * pot + 1];
+                    x[i][2 * pot] = a * er[m] - b * ei[m];
+                    x[i][2 * pot + 1] = a * ei[m] + b * er[m];
+                }
+            }
+        lane_rotate<P, LH>(x, tb.Hs, tb.zeta, 1.0, cv, sv);
+        // translate_multipole_to_local_z! (src/translate.jl:181-232): F[k] = k!/t^(k+1)
+        double F[2 * P + 1];
+        const double ot = 1.0 / r;
+        F[0] = ot;
+#pragma unroll
+        for (int k = 1; k <= 2 * P; ++k) F[k] = F[k - 1] * ((double)k * ot);
+        double y[H][4];
+#pragma unroll
+        for (int n = 0; n <= P; ++n)
+#pragma unroll
+            for (int m = 0; m <= n; ++m) {
+                double v4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int np = m; np <= P; ++np)
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) v4[q] = fma(F[n + np], x[nh(np - 1) + m][q], v4[q]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y[nh(n - 1) + m][q] = v4[q];
+            }
+        // Lamb-Helmholtz local transform (:98-142) and the back rotation's input scale
+#pragma unroll
+        for (int n = 0; n <= P; ++n)
+#pragma unroll
+            for (int m = 0; m <= n; ++m) {
+                const int i = nh(n - 1) + m;
+                double o0 = y[i][0], o1 = y[i][1], o2 = 0.0, o3 = 0.0;
+                if (LH) {
+                    const double cr = y[i][2], ci = y[i][3];
+                    if (n > 0) {
+                        const double r_m_n = r * (double)m / (double)n;
+                        o0 = y[i][0] - r_m_n * ci;
+                        o1 = y[i][1] + r_m_n * cr;
+                    }
+                    o2 = cr; o3 = ci;
+                    if (n < P) {
+                        const double r_np1 = r / (double)(n + 1);
+                        o2 = cr - r_np1 * y[i + n + 1][2];
+                        o3 = ci - r_np1 * y[i + n + 1][3];
+                    }
+                }
+                x[i][0] = o0; x[i][1] = o1;
+                if (LH) { x[i][2] = o2; x[i][3] = o3; }
+            }
+        lane_rotate<P, LH>(x, tb.Hs, tb.eta, -1.0, cv, sv);
+#pragma unroll
+        for (int n = 0; n <= P; ++n)
+#pragma unroll
+            for (int m = 0; m <= n; ++m) {   // back_rotate_z! (src/rotate.jl:584-616)
+                const int i = nh(n - 1) + m;
+#pragma unroll
+                for (int pot = 0; pot < NQ / 2; ++pot) {
+                    const double a = x[i][2 * pot], b = x[i][2 * pot + 1];
+                    acc[i][2 * pot] += a * er[m] + b * ei[m];
+                    acc[i][2 * pot + 1] += b * er[m] - a * ei[m];
+                }
+            }
+        orders[li] = P;
+    }
+    // sum the lanes' partial locals and add them to the target box's expansion
+    double* dst = tex + (int64_t)it * 4 * H;
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double v = acc[i][q];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == ((i * NQ + q) & 63)) dst[4 * i + q] += v;
+        }
+    if (lane == 0 && l1 > l0) atomicMax(pmax, P);
+}
+
+template <bool LH>
+static void launch_m2l_lane(Context& C, int p, const Lists& Ls, const Tree& st, Tree& tt_dst, int32_t* pmax, const TablePtrs& tb) {
+    const unsigned grid = (unsigned)((Ls.n_m2l_targets + 3) / 4);
+#define FMMHIP_M2L_LANE(PP)                                                                                          \
+    k_m2l_lane<PP, LH><<<grid, 256, 0, C.stream>>>(Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(),    \
+                                                   tt_dst.branches.data(), st.branches.data(), st.expansions.data(), \
+                                                   tt_dst.expansions.data(), Ls.n_m2l_targets, Ls.m2l_order.data(),  \
+                                                   pmax, tb)
+    switch (p) {
+        case 1: FMMHIP_M2L_LANE(1); break;
+        case 2: FMMHIP_M2L_LANE(2); break;
+        case 3: FMMHIP_M2L_LANE(3); break;
+        default: FMMHIP_M2L_LANE(4); break;
+    }
+#undef FMMHIP_M2L_LANE
+    LAUNCH_CHECK();
 }
 
 __global__ void k_init_flags(int32_t* pmax, int32_t* ok) { *pmax = 0; *ok = 1; }
@@ -870,12 +1266,20 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
     flags.resize(2);
     k_init_flags<<<1, 1, 0, C.stream>>>(flags.data(), flags.data() + 1);
     LAUNCH_CHECK();
-    if (Ls.n_m2l_targets > 0) {
+    if (Ls.n_m2l_targets > 0 && err_kind == FMMHIP_ERR_NONE && p <= SMALL_P) {
+        if (LH) launch_m2l_lane<true>(C, p, Ls, st, tt_dst, flags.data(), tb);
+        else launch_m2l_lane<false>(C, p, Ls, st, tt_dst, flags.data(), tb);
+    } else if (Ls.n_m2l_targets > 0) {
         ErrParams E{err_kind, err_rel ? 1 : 0, eps, BE ? 1 : 0};
-        k_m2l<<<(unsigned)Ls.n_m2l_targets, 64, lds_bytes(p), C.stream>>>(
-            Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
-            st.expansions.data(), tt_dst.expansions.data(), p, LH ? 1 : 0, E, Ls.m2l_order.data(), flags.data(),
-            flags.data() + 1, tb);
+        const size_t sh = LH ? translate_setup(k_m2l<true>, p) : translate_setup(k_m2l<false>, p);
+        if (LH)
+            k_m2l<true><<<(unsigned)Ls.n_m2l_targets, 64 * WPB, sh, C.stream>>>(
+                Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
+                st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags.data(), flags.data() + 1, tb);
+        else
+            k_m2l<false><<<(unsigned)Ls.n_m2l_targets, 64 * WPB, sh, C.stream>>>(
+                Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
+                st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags.data(), flags.data() + 1, tb);
         LAUNCH_CHECK();
     }
     int32_t h[2];

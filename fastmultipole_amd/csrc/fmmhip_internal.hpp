@@ -119,6 +119,9 @@ struct Eval {
     bool trees_built = false, lists_built = false, ran = false;
     int32_t pmax = 0, error_success = 1;
     int p_built = -1;
+    Eval* estimate = nullptr;                 // persistent low-order evaluation of estimate_influence
+    DBuf<double> influence;                   // per-target max |grad| (or |phi|) of the estimate
+    ~Eval() { delete estimate; }
 };
 
 inline int data_per_body(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 5 : 7; }
@@ -131,7 +134,7 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
                 int method, bool shrink, int p, int strength_dims);
 // lists.hip
 void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t leaf_size_source, double mac,
-                 int method, bool ff, bool nf, bool si);
+                 int method, bool ff, bool nf, bool si, bool count_pairs);
 // p2p.hip
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
              bool PS, bool GS, bool HS, int precision);

@@ -140,14 +140,22 @@ static T read_back(Context& C, const T* dptr) {
     return v;
 }
 
-// number of body-body pairs of the direct list (nearfield work)
-__global__ void k_count_pairs(const int32_t* __restrict__ t, const int32_t* __restrict__ s, int64_t n, const Branch* __restrict__ tb,
-                              const Branch* __restrict__ sb, unsigned long long* __restrict__ out) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// number of body-body pairs of the direct list (nearfield work): grid-stride
+// partial sums, one atomic per workgroup
+__global__ __launch_bounds__(256) void k_count_pairs(const int32_t* __restrict__ t, const int32_t* __restrict__ s, int64_t n,
+                                                     const Branch* __restrict__ tb, const Branch* __restrict__ sb,
+                                                     unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[4];
     unsigned long long v = 0;
-    if (i < n) v = (unsigned long long)tb[t[i]].n_bodies * (unsigned long long)sb[s[i]].n_bodies;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += (unsigned long long)tb[t[i]].n_bodies * (unsigned long long)sb[s[i]].n_bodies;
     for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long tot = part[0] + part[1] + part[2] + part[3];
+        if (tot) atomicAdd(out, tot);
+    }
 }
 
 __global__ void k_init_frontier(int32_t* ft, int32_t* fs, int32_t* fst, unsigned long long* counter) {
@@ -211,7 +219,7 @@ static void extract_list(Context& C, ListScratch& S, int cur, int64_t n, int32_t
 }
 
 void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t ls, double mac, int method, bool ff, bool nf,
-                 bool si) {
+                 bool si, bool count_pairs) {
     L.n_m2l = L.n_direct = L.n_pairs = 0;
     L.n_direct_targets = L.n_m2l_targets = 0;
     if (tt.n_branches == 0 || st.n_branches == 0) return;
@@ -249,9 +257,9 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
     extract_list(C, S, cur, n, ST_DIRECT, L.dir_t, L.dir_s, L.n_direct, L.dir_targets, L.dir_offsets, L.n_direct_targets);
     L.m2l_order.resize(std::max<int64_t>(1, L.n_m2l));
     L.n_pairs = 0;
-    if (L.n_direct > 0) {
+    if (count_pairs && L.n_direct > 0) {
         k_zero_counter<<<1, 1, 0, C.stream>>>(S.counter.data());
-        k_count_pairs<<<grid_for(L.n_direct, 256), 256, 0, C.stream>>>(L.dir_t.data(), L.dir_s.data(), L.n_direct,
+        k_count_pairs<<<std::min(1024u, grid_for(L.n_direct, 256)), 256, 0, C.stream>>>(L.dir_t.data(), L.dir_s.data(), L.n_direct,
                                                                        tt.branches.data(), st.branches.data(), S.counter.data());
         LAUNCH_CHECK();
         L.n_pairs = (int64_t)read_back(C, S.counter.data());

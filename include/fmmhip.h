@@ -79,6 +79,8 @@ typedef struct {
     int64_t n_m2l, n_direct;      /* interaction list lengths */
     int64_t n_p2p_pairs;          /* body-body pairs evaluated by the nearfield */
     double ms_tree, ms_lists, ms_p2p, ms_upward, ms_m2l, ms_downward, ms_total;  /* device time per phase */
+    double ms_estimate;           /* estimate_influence pre-pass (relative tolerances only) */
+    double ms_gather;             /* scatter of sorted results back to the caller's order */
 } fmmhip_stats;
 
 typedef struct fmmhip_ctx fmmhip_ctx;

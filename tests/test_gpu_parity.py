@@ -108,6 +108,16 @@ FMM_CASES = [
      dict(expansion_order=8, leaf_size_source=30, multipole_acceptance=0.5, hessian=True)),
     ("ring-6k-p10", lambda: W.vortex_ring(6000, seed=24), O.KIND_VORTEX,
      dict(expansion_order=10, leaf_size_source=64, multipole_acceptance=0.4, hessian=True)),
+    # orders <= 4 take the lane-per-interaction M2L kernel
+    ("vortex-5k-p3-leaf5", lambda: W.uniform_vortices(5000, seed=25), O.KIND_VORTEX,
+     dict(expansion_order=3, leaf_size_source=5, multipole_acceptance=0.6, scalar_potential=True, hessian=True)),
+    ("laplace-4k-p4", lambda: W.uniform_sources(4000, seed=26), O.KIND_SOURCE,
+     dict(expansion_order=4, leaf_size_source=10, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("laplace-2k-p1", lambda: W.uniform_sources(2000, seed=27), O.KIND_SOURCE,
+     dict(expansion_order=1, leaf_size_source=8, multipole_acceptance=0.5, scalar_potential=True)),
+    # order 20: two back-rotation chunks in the wavefront engine
+    ("ring-3k-p20", lambda: W.vortex_ring(3000, seed=28), O.KIND_VORTEX,
+     dict(expansion_order=20, leaf_size_source=40, multipole_acceptance=0.5, hessian=True)),
 ]
 
 
