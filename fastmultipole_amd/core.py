@@ -9,7 +9,9 @@ buffer_to_target_system!, has_vector_potential).
 
 All evaluation runs on the GPU through libfmmhip; there is no CPU path.
 """
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -87,6 +89,7 @@ class Context:
             raise L.FmmHipError(st, "fmmhip_ctx_create failed (no usable GPU?)")
         self._h = h
         self.lib = lib
+        _LIVE_CTXS.add(self)
 
     def check(self, st):
         if st != L.FMMHIP_OK:
@@ -105,6 +108,17 @@ class Context:
 
 
 _DEFAULT = None
+_LIVE_EVALS = weakref.WeakSet()
+_LIVE_CTXS = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    """Release device state before interpreter teardown: evaluations first, then contexts."""
+    for ev in list(_LIVE_EVALS):
+        ev.close()
+    for ctx in list(_LIVE_CTXS):
+        ctx.close()
 
 
 def default_context():
@@ -183,6 +197,7 @@ class Evaluation:
         self._h = h
         self.nt = self.ns = 0
         self._keep = None
+        _LIVE_EVALS.add(self)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -51,6 +51,7 @@ __device__ inline void idx_nm(int idx, int& n, int& m) {
 struct Tables {
     int p = -1;
     DBuf<double> Hs, zeta, eta, Mt, Lt, inv_fact;
+    DBuf<double> Ad;   // dense, even-padded pi/2 tables per degree (rotation kernels)
 };
 
 // Host computation of the pi/2 Wigner tables, zeta/eta magnitudes and power
@@ -181,18 +182,29 @@ void ensure_tables(Context& C, int p) {
         d.resize(h.size());
         HIP_CHECK(hipMemcpy(d.data(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     };
-    up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf);
+    // dense symmetric copy: A_n(a, b) = Hs_n(min(a, b), max(a, b)), rows padded to an even length
+    std::vector<double> Ad;
+    for (int n = 0; n <= pm; ++n) {
+        const int s = (n + 2) & ~1;
+        const double* Hn = &Hs[((n) * (n + 1) * (n + 2)) / 6];
+        for (int a = 0; a <= n; ++a)
+            for (int b = 0; b < s; ++b) {
+                const int lo = std::min(a, b), hi = std::max(a, b);
+                Ad.push_back(b <= n ? Hn[(hi * (hi + 1)) / 2 + lo] : 0.0);
+            }
+    }
+    up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf); up(T.Ad, Ad);
     T.p = pm;
 }
 
 void free_tables(Tables* t) { delete t; }
 
 struct TablePtrs {
-    const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact;
+    const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact, *Ad;
 };
 static TablePtrs ptrs(Context& C) {
     Tables& T = *C.tables;
-    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data()};
+    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data(), T.Ad.data()};
 }
 
 // ====================================================== rotation engine
@@ -223,11 +235,11 @@ static TablePtrs ptrs(Context& C) {
 // first product and the output pair m = 2j, 2j+1 in the second. A chunk holds at
 // most 64 items, so each product is a single pass of the wavefront.
 
-static constexpr int WPB = 4;          // wavefronts per workgroup in the translation kernels
 static constexpr int RBUF = 1024;      // doubles of per-wavefront item buffer (64 items x 16)
 static constexpr int MAXCH = 2;        // back-rotation chunks held in registers: orders up to 20
 static constexpr int FWD_DEGREES = 4;  // degrees per forward chunk of the error-controlled M2L
 static constexpr int SMALL_P = 4;      // fixed orders up to this run one M2L per lane (k_m2l_lane)
+static constexpr size_t LDS_LIMIT = 160 * 1024;
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
 __device__ inline int wave_id() { return (int)(threadIdx.x >> 6); }
@@ -258,50 +270,62 @@ __device__ inline void item_nj(int g, int& n, int& j) {
     j = g - items_below(k);
 }
 
-__device__ inline double hat(const double* Hn, int a, int b) {   // A_n(a, b) = Hs_pi2 entry (min, max)
-    int lo = min(a, b), hi = max(a, b);
-    return Hn[((hi * (hi + 1)) >> 1) + lo];
-}
+// Dense pi/2 tables: degree n is an (n+1) x s_n row-major matrix A_n(a, b), with
+// the row length s_n = n+1 rounded up to even (zero padding), so a column pair
+// (2j, 2j+1) is one aligned 16-byte read.
+__host__ __device__ constexpr int a_stride(int n) { return (n + 2) & ~1; }
+__host__ __device__ constexpr int a_base(int n) { return (n * (n + 1) * (2 * n + 1)) / 6 + ((n + 1) / 2) * ((n + 1) / 2); }
+__host__ __device__ constexpr int len_A(int P) { return a_base(P + 1); }
 
 struct WaveLds {
     double* X;     // 4H: expansion workspace [idx][phi/chi][re/im]
     double* R;     // max(RBUF, 4H): item buffer / translated expansion
     double* eim;   // 2(P+1): e^{i m phi}
     double* cs;    // 2(P+1): cos(v beta), sin(v beta)
+    double* fac;   // 6(P+1): running factors of the error tests per degree
+    double* pw;    // 2(P+1): multipole power per degree (phi, chi)
 };
 
 __host__ __device__ inline int wave_lds_doubles(int P) {
     const int H4 = 4 * (((P + 1) * (P + 2)) >> 1);
-    return H4 + (H4 > RBUF ? H4 : RBUF) + 4 * (P + 1);
+    return H4 + (H4 > RBUF ? H4 : RBUF) + 12 * (P + 1);
 }
-__host__ __device__ inline int hs_lds_doubles(int P) { return (len_Hs(P) + 1) & ~1; }
-static size_t translate_lds_bytes(int P) { return sizeof(double) * (size_t)(hs_lds_doubles(P) + WPB * wave_lds_doubles(P)); }
+static size_t translate_lds_bytes(int P, int nw) { return sizeof(double) * (size_t)(len_A(P) + nw * wave_lds_doubles(P)); }
+
+// wavefronts per workgroup: as many (8, 6 or 4) as the LDS holds
+static int translate_waves(int P) {
+    for (int nw : {8, 6, 4})
+        if (translate_lds_bytes(P, nw) <= LDS_LIMIT) return nw;
+    throw Error(FMMHIP_EINVAL, "expansion order too large for the translation kernels");
+}
 
 // launch set-up shared by the translation kernels: order check and the LDS limit
 template <class K>
-static size_t translate_setup(K kernel, int P) {
+static size_t translate_setup(K kernel, int P, int nw) {
     int chunks = 0;
     for (int n0 = 0; n0 <= P; n0 = chunk_end(n0, P, 64) + 1) ++chunks;
     if (chunks > MAXCH) throw Error(FMMHIP_EINVAL, "expansion order too large for the translation kernels");
-    const size_t sh = translate_lds_bytes(P);
+    const size_t sh = translate_lds_bytes(P, nw);
     HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     return sh;
 }
 
-__device__ inline WaveLds carve_wave(double* base, int P) {
+__device__ inline WaveLds carve_wave(double* lds, int P) {
     WaveLds W;
-    W.X = base + (size_t)wave_id() * wave_lds_doubles(P);
+    W.X = lds + len_A(P) + (size_t)wave_id() * wave_lds_doubles(P);
     const int H4 = 4 * nh(P);
     W.R = W.X + H4;
     W.eim = W.R + (H4 > RBUF ? H4 : RBUF);
     W.cs = W.eim + 2 * (P + 1);
+    W.fac = W.cs + 2 * (P + 1);
+    W.pw = W.fac + 6 * (P + 1);
     return W;
 }
 
-// copies the pi/2 tables into LDS (whole workgroup, ends with a barrier)
-__device__ inline double* load_hs(double* lds, const double* Hs_g, int P) {
-    const int n = len_Hs(P);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = Hs_g[i];
+// copies the dense pi/2 tables into LDS (whole workgroup, ends with a barrier)
+__device__ inline const double* load_tables(double* lds, const double* Ad, int P) {
+    const int n = len_A(P);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = Ad[i];
     __syncthreads();
     return lds;
 }
@@ -346,70 +370,89 @@ __device__ inline void fill_angles(const WaveLds& W, double ph, double th, int P
 }
 
 // X[n][m] = g_n(m)^(+-1) * (src[n][m] e^{i m phi}) for degrees n0..n1 (rotate_z!, src/rotate.jl:448-483)
-// mags row m = 0 of zeta (g) or eta (1/g)
-template <bool LH>
-__device__ inline void zrot_in(const double* __restrict__ src, const WaveLds& W, const double* mags, int n0, int n1) {
+// mags row m = 0 of zeta (g) or eta (1/g). With POW, also the power terms
+// |src_n^m|^2 Mt_n^m^2 (phi, chi) of the unrotated coefficients into R.
+template <bool LH, bool POW>
+__device__ inline void zrot_in(const double* __restrict__ src, const WaveLds& W, const double* mags, const double* Mt,
+                               int n0, int n1) {
     const int i0 = nh(n0 - 1), i1 = nh(n1);
     for (int i = i0 + lane_id(); i < i1; i += 64) {
         int n, m;
         idx_nm(i, n, m);
         const double g = mags[len_Ts(n - 1) + m];
         const double er = W.eim[2 * m], ei = W.eim[2 * m + 1];
-        double xr = src[4 * i], xi = src[4 * i + 1];
-        W.X[4 * i] = g * (xr * er - xi * ei);
-        W.X[4 * i + 1] = g * (xr * ei + xi * er);
+        const double2 s01 = *(const double2*)(src + 4 * i);
+        double2 o01;
+        o01.x = g * (s01.x * er - s01.y * ei);
+        o01.y = g * (s01.x * ei + s01.y * er);
+        *(double2*)(W.X + 4 * i) = o01;
+        double2 s23 = make_double2(0.0, 0.0);
         if (LH) {
-            xr = src[4 * i + 2]; xi = src[4 * i + 3];
-            W.X[4 * i + 2] = g * (xr * er - xi * ei);
-            W.X[4 * i + 3] = g * (xr * ei + xi * er);
+            s23 = *(const double2*)(src + 4 * i + 2);
+            double2 o23;
+            o23.x = g * (s23.x * er - s23.y * ei);
+            o23.y = g * (s23.x * ei + s23.y * er);
+            *(double2*)(W.X + 4 * i + 2) = o23;
+        }
+        if (POW) {
+            const double Nt = Mt[i];
+            W.R[2 * (i - i0)] = (s01.x * s01.x + s01.y * s01.y) * Nt * Nt;
+            W.R[2 * (i - i0) + 1] = LH ? (s23.x * s23.x + s23.y * s23.y) * Nt * Nt : 0.0;
         }
     }
 }
 
 // first product of the chunk [n0, n1] (degrees <= nlim): X -> R
 template <bool LH>
-__device__ inline void rot_step1(const WaveLds& W, const double* Hs, double sigma, int n0, int n1, int nlim) {
+__device__ inline void rot_step1(const WaveLds& W, const double* A, double sigma, int n0, int n1, int nlim) {
     const int k = lane_id();
     const int base = items_below(n0);
     if (k >= items_below(n1 + 1) - base) return;
     int n, j;
     item_nj(base + k, n, j);
     if (n > nlim) return;
-    const double* Hn = Hs + len_Hs(n - 1);
+    const int s = a_stride(n);
+    const double* An = A + a_base(n) + 2 * j;   // row m' at An + m' s: (A(m', 2j), A(m', 2j+1))
     const double* Xn = W.X + 4 * nh(n - 1);
-    const int v0 = 2 * j, v1 = 2 * j + 1;
-    const bool has1 = v1 <= n;
-    constexpr int NP = LH ? 4 : 2;
-    double se0[NP], so0[NP], se1[NP], so1[NP];
-#pragma unroll
-    for (int c = 0; c < NP; ++c) se0[c] = so0[c] = se1[c] = so1[c] = 0.0;
-    for (int mp = 0; mp <= n; mp += 2) {   // even m'
-        const double a0 = hat(Hn, mp, v0), a1 = has1 ? hat(Hn, mp, v1) : 0.0;
-#pragma unroll
-        for (int c = 0; c < NP; ++c) {
-            const double u = Xn[4 * mp + c];
-            se0[c] = fma(a0, u, se0[c]);
-            se1[c] = fma(a1, u, se1[c]);
+    double se0[4] = {0.0, 0.0, 0.0, 0.0}, so0[4] = {0.0, 0.0, 0.0, 0.0};
+    double se1[4] = {0.0, 0.0, 0.0, 0.0}, so1[4] = {0.0, 0.0, 0.0, 0.0};
+    int mp = 0;
+    for (; mp + 1 <= n; mp += 2) {
+        const double2 ae = *(const double2*)(An + mp * s);
+        const double2 ao = *(const double2*)(An + (mp + 1) * s);
+        const double2 xe0 = *(const double2*)(Xn + 4 * mp), xo0 = *(const double2*)(Xn + 4 * mp + 4);
+        se0[0] = fma(ae.x, xe0.x, se0[0]); se0[1] = fma(ae.x, xe0.y, se0[1]);
+        se1[0] = fma(ae.y, xe0.x, se1[0]); se1[1] = fma(ae.y, xe0.y, se1[1]);
+        so0[0] = fma(ao.x, xo0.x, so0[0]); so0[1] = fma(ao.x, xo0.y, so0[1]);
+        so1[0] = fma(ao.y, xo0.x, so1[0]); so1[1] = fma(ao.y, xo0.y, so1[1]);
+        if (LH) {
+            const double2 xe1 = *(const double2*)(Xn + 4 * mp + 2), xo1 = *(const double2*)(Xn + 4 * mp + 6);
+            se0[2] = fma(ae.x, xe1.x, se0[2]); se0[3] = fma(ae.x, xe1.y, se0[3]);
+            se1[2] = fma(ae.y, xe1.x, se1[2]); se1[3] = fma(ae.y, xe1.y, se1[3]);
+            so0[2] = fma(ao.x, xo1.x, so0[2]); so0[3] = fma(ao.x, xo1.y, so0[3]);
+            so1[2] = fma(ao.y, xo1.x, so1[2]); so1[3] = fma(ao.y, xo1.y, so1[3]);
         }
     }
-    for (int mp = 1; mp <= n; mp += 2) {   // odd m'
-        const double a0 = hat(Hn, mp, v0), a1 = has1 ? hat(Hn, mp, v1) : 0.0;
-#pragma unroll
-        for (int c = 0; c < NP; ++c) {
-            const double u = Xn[4 * mp + c];
-            so0[c] = fma(a0, u, so0[c]);
-            so1[c] = fma(a1, u, so1[c]);
+    if (mp <= n) {   // last (even) m'
+        const double2 ae = *(const double2*)(An + mp * s);
+        const double2 xe0 = *(const double2*)(Xn + 4 * mp);
+        se0[0] = fma(ae.x, xe0.x, se0[0]); se0[1] = fma(ae.x, xe0.y, se0[1]);
+        se1[0] = fma(ae.y, xe0.x, se1[0]); se1[1] = fma(ae.y, xe0.y, se1[1]);
+        if (LH) {
+            const double2 xe1 = *(const double2*)(Xn + 4 * mp + 2);
+            se0[2] = fma(ae.x, xe1.x, se0[2]); se0[3] = fma(ae.x, xe1.y, se0[3]);
+            se1[2] = fma(ae.y, xe1.x, se1[2]); se1[3] = fma(ae.y, xe1.y, se1[3]);
         }
     }
     double* out = W.R + 16 * k;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int v = h ? v1 : v0;
-        if (h && !has1) break;
+        const int v = 2 * j + h;
+        if (v > n) break;
         const double* se = h ? se1 : se0;
         const double* so = h ? so1 : so0;
-        const double a00 = hat(Hn, 0, v);
-        const double s = ((n + v) & 1) ? -1.0 : 1.0;
+        const double a00 = An[h];   // A(0, v)
+        const double sg = ((n + v) & 1) ? -1.0 : 1.0;
         const double w = v ? 2.0 : 1.0;
         const double c = w * W.cs[2 * v], sn = w * sigma * W.cs[2 * v + 1];
 #pragma unroll
@@ -417,53 +460,48 @@ __device__ inline void rot_step1(const WaveLds& W, const double* Hs, double sigm
             const double ser = se[2 * pot], sei = se[2 * pot + 1], sor = so[2 * pot], soi = so[2 * pot + 1];
             const double dr = ser - a00 * Xn[2 * pot], di = sei - a00 * Xn[2 * pot + 1];
             // P+ = (SE - S0) - SO + s conj(SE + SO),  P- = (SE - S0) + SO + s conj(SE - SO)
-            const double ppr = dr - sor + s * (ser + sor), ppi = di - soi - s * (sei + soi);
-            const double pmr = dr + sor + s * (ser - sor), pmi = di + soi - s * (sei - soi);
+            const double ppr = dr - sor + sg * (ser + sor), ppi = di - soi - sg * (sei + soi);
+            const double pmr = dr + sor + sg * (ser - sor), pmi = di + soi - sg * (sei - soi);
             // Q+ = (c + i sn) P+,  Q- = (c - i sn) P-
             const double qpr = c * ppr - sn * ppi, qpi = c * ppi + sn * ppr;
             const double qmr = c * pmr + sn * pmi, qmi = c * pmi - sn * pmr;
             double* o = out + 8 * h + 4 * pot;
-            o[0] = qpr + qmr; o[1] = qpi + qmi;   // even m
-            o[2] = qpr - qmr; o[3] = qpi - qmi;   // odd m
+            *(double2*)o = make_double2(qpr + qmr, qpi + qmi);         // even m
+            *(double2*)(o + 2) = make_double2(qpr - qmr, qpi - qmi);   // odd m
         }
     }
 }
 
 // second product for item (n, j): outputs m0 = 2j, m1 = 2j+1 (scaled), o[0..3] and o[4..7]
 template <bool LH>
-__device__ inline void rot_step2(const WaveLds& W, const double* Hs, const double* mags, int n0, int n, int j,
-                                 double o[8]) {
-    const double* Hn = Hs + len_Hs(n - 1);
-    const int m0 = 2 * j, m1 = 2 * j + 1;
-    const bool has1 = m1 <= n;
-    constexpr int NP = LH ? 4 : 2;
-    double e[NP], d[NP];
-#pragma unroll
-    for (int c = 0; c < NP; ++c) e[c] = d[c] = 0.0;
+__device__ inline void rot_step2(const WaveLds& W, const double* A, const double* mags, int n0, int n, int j, double o[8]) {
+    const int s = a_stride(n);
+    const double* An = A + a_base(n) + 2 * j;   // row v: (A(v, 2j), A(v, 2j+1)) = (A(2j, v), A(2j+1, v))
     const double* Rn = W.R + 16 * (items_below(n) - items_below(n0));
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, d[4] = {0.0, 0.0, 0.0, 0.0};
     for (int v = 0; v <= n; ++v) {
-        const double* rv = Rn + 8 * v;   // item v/2, half v&1: 16*(v>>1) + 8*(v&1) = 8v
-        const double a0 = hat(Hn, m0, v), a1 = has1 ? hat(Hn, m1, v) : 0.0;
-#pragma unroll
-        for (int pot = 0; pot < NP / 2; ++pot) {
-            e[2 * pot] = fma(a0, rv[4 * pot], e[2 * pot]);
-            e[2 * pot + 1] = fma(a0, rv[4 * pot + 1], e[2 * pot + 1]);
-            d[2 * pot] = fma(a1, rv[4 * pot + 2], d[2 * pot]);
-            d[2 * pot + 1] = fma(a1, rv[4 * pot + 3], d[2 * pot + 1]);
+        const double2 a = *(const double2*)(An + v * s);
+        const double* rv = Rn + 8 * v;   // item v/2, half v&1
+        const double2 re0 = *(const double2*)rv, ro0 = *(const double2*)(rv + 2);
+        e[0] = fma(a.x, re0.x, e[0]); e[1] = fma(a.x, re0.y, e[1]);
+        d[0] = fma(a.y, ro0.x, d[0]); d[1] = fma(a.y, ro0.y, d[1]);
+        if (LH) {
+            const double2 re1 = *(const double2*)(rv + 4), ro1 = *(const double2*)(rv + 6);
+            e[2] = fma(a.x, re1.x, e[2]); e[3] = fma(a.x, re1.y, e[3]);
+            d[2] = fma(a.y, ro1.x, d[2]); d[3] = fma(a.y, ro1.y, d[3]);
         }
     }
     const double* mg = mags + len_Ts(n - 1);
-    const double g0 = 0.5 * mg[m0 * (n + 1)], g1 = has1 ? 0.5 * mg[m1 * (n + 1)] : 0.0;
-    o[0] = g0 * e[0]; o[1] = g0 * e[1];
-    o[2] = LH ? g0 * e[NP - 2] : 0.0; o[3] = LH ? g0 * e[NP - 1] : 0.0;
-    o[4] = g1 * d[0]; o[5] = g1 * d[1];
-    o[6] = LH ? g1 * d[NP - 2] : 0.0; o[7] = LH ? g1 * d[NP - 1] : 0.0;
+    const int m0 = 2 * j, m1 = 2 * j + 1;
+    const double g0 = 0.5 * mg[m0 * (n + 1)], g1 = m1 <= n ? 0.5 * mg[m1 * (n + 1)] : 0.0;
+    o[0] = g0 * e[0]; o[1] = g0 * e[1]; o[2] = g0 * e[2]; o[3] = g0 * e[3];
+    o[4] = g1 * d[0]; o[5] = g1 * d[1]; o[6] = g1 * d[2]; o[7] = g1 * d[3];
 }
 
-// forward rotation of the chunk [n0, n1]: X (scaled input) -> X (rotated)
+// rotation of the chunk [n0, n1]: X (scaled input) -> X (rotated)
 template <bool LH>
-__device__ inline void rot_chunk_store(const WaveLds& W, const double* Hs, const double* mags, double sigma, int n0, int n1) {
-    rot_step1<LH>(W, Hs, sigma, n0, n1, n1);
+__device__ inline void rot_chunk_store(const WaveLds& W, const double* A, const double* mags, double sigma, int n0, int n1) {
+    rot_step1<LH>(W, A, sigma, n0, n1, n1);
     wave_sync();
     const int k = lane_id();
     const int base = items_below(n0);
@@ -471,13 +509,13 @@ __device__ inline void rot_chunk_store(const WaveLds& W, const double* Hs, const
         int n, j;
         item_nj(base + k, n, j);
         double o[8];
-        rot_step2<LH>(W, Hs, mags, n0, n, j, o);
+        rot_step2<LH>(W, A, mags, n0, n, j, o);
         double* x = W.X + 4 * (nh(n - 1) + 2 * j);
-        x[0] = o[0]; x[1] = o[1];
-        if (LH) { x[2] = o[2]; x[3] = o[3]; }
+        *(double2*)x = make_double2(o[0], o[1]);
+        if (LH) *(double2*)(x + 2) = make_double2(o[2], o[3]);
         if (2 * j + 1 <= n) {
-            x[4] = o[4]; x[5] = o[5];
-            if (LH) { x[6] = o[6]; x[7] = o[7]; }
+            *(double2*)(x + 4) = make_double2(o[4], o[5]);
+            if (LH) *(double2*)(x + 6) = make_double2(o[6], o[7]);
         }
     }
     wave_sync();
@@ -486,14 +524,14 @@ __device__ inline void rot_chunk_store(const WaveLds& W, const double* Hs, const
 // back rotation (degrees <= nlim) + back z-rotation, accumulated into registers:
 // acc[c] holds item (chunk c, lane) = outputs (n, 2j) and (n, 2j+1)
 template <bool LH>
-__device__ inline void rot_back_acc(const WaveLds& W, const double* Hs, const double* mags, double sigma, int P, int nlim,
+__device__ inline void rot_back_acc(const WaveLds& W, const double* A, const double* mags, double sigma, int P, int nlim,
                                     double (&acc)[MAXCH][8]) {
     int n0 = 0;
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
         if (n0 <= nlim) {
             const int n1 = chunk_end(n0, P, 64);
-            rot_step1<LH>(W, Hs, sigma, n0, n1, nlim);
+            rot_step1<LH>(W, A, sigma, n0, n1, nlim);
             wave_sync();
             const int k = lane_id();
             const int base = items_below(n0);
@@ -502,7 +540,7 @@ __device__ inline void rot_back_acc(const WaveLds& W, const double* Hs, const do
                 item_nj(base + k, n, j);
                 if (n <= nlim) {
                     double o[8];
-                    rot_step2<LH>(W, Hs, mags, n0, n, j, o);
+                    rot_step2<LH>(W, A, mags, n0, n, j, o);
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const int m = 2 * j + h;
@@ -561,41 +599,46 @@ __device__ inline void translate_one(const double* in, double t, int P, int n, i
     double v1r = 0, v1i = 0, v2r = 0, v2i = 0;
     if (OP == OP_M2M) {
         const double _t = -t;
-        double n_np_f = 1.0, t_n_np = 1.0;
-        int n_np = 1;
+        double n_np_f = 1.0, t_n_np = 1.0, d_np = 1.0;
+        int iw = nh(n - 1) + m;
         for (int np = n; np >= m; --np) {
-            double tmp = t_n_np / n_np_f;
-            int iw = (np * (np + 1)) / 2 + m;
-            v1r = fma(tmp, in[4 * iw], v1r); v1i = fma(tmp, in[4 * iw + 1], v1i);
-            if (LH) { v2r = fma(tmp, in[4 * iw + 2], v2r); v2i = fma(tmp, in[4 * iw + 3], v2i); }
+            const double tmp = t_n_np / n_np_f;
+            const double2 a = *(const double2*)(in + 4 * iw);
+            v1r = fma(tmp, a.x, v1r); v1i = fma(tmp, a.y, v1i);
+            if (LH) { const double2 b = *(const double2*)(in + 4 * iw + 2); v2r = fma(tmp, b.x, v2r); v2i = fma(tmp, b.y, v2i); }
             t_n_np *= _t;
-            n_np_f *= (double)n_np;
-            n_np += 1;
+            n_np_f *= d_np;
+            d_np += 1.0;
+            iw -= np;
         }
     } else if (OP == OP_M2L) {
         const double one_over_t = 1.0 / t;
-        double nf = one_over_t;   // n!/t^(n+1)
-        for (int k = 0; k < n; ++k) nf *= (double)(k + 1) * one_over_t;
-        double f = nf;            // (n+m)!/t^(n+m+1)
-        for (int k = 0; k < m; ++k) f *= (double)(n + k + 1) * one_over_t;
-        int n_np = n + m;
+        double f = one_over_t;   // (n+np)!/t^(n+np+1) by the reference's product chain
+        double dk = 1.0;
+        for (int k = 0; k < n + m; ++k) {
+            f *= dk * one_over_t;
+            dk += 1.0;
+        }
+        int iw = nh(m - 1) + m;
         for (int np = m; np <= P; ++np) {
-            int iw = (np * (np + 1)) / 2 + m;
-            v1r = fma(f, in[4 * iw], v1r); v1i = fma(f, in[4 * iw + 1], v1i);
-            if (LH) { v2r = fma(f, in[4 * iw + 2], v2r); v2i = fma(f, in[4 * iw + 3], v2i); }
-            n_np += 1;
-            f *= (double)n_np * one_over_t;
+            const double2 a = *(const double2*)(in + 4 * iw);
+            v1r = fma(f, a.x, v1r); v1i = fma(f, a.y, v1i);
+            if (LH) { const double2 b = *(const double2*)(in + 4 * iw + 2); v2r = fma(f, b.x, v2r); v2i = fma(f, b.y, v2i); }
+            f *= dk * one_over_t;
+            dk += 1.0;
+            iw += np + 1;
         }
     } else {
         const double _t = -t;
-        double t_n_nf = 1.0;
-        int n_np = 1;
+        double t_n_nf = 1.0, d_np = 1.0;
+        int iw = nh(n - 1) + m;
         for (int np = n; np <= P; ++np) {
-            int iw = (np * (np + 1)) / 2 + m;
-            v1r = fma(t_n_nf, in[4 * iw], v1r); v1i = fma(t_n_nf, in[4 * iw + 1], v1i);
-            if (LH) { v2r = fma(t_n_nf, in[4 * iw + 2], v2r); v2i = fma(t_n_nf, in[4 * iw + 3], v2i); }
-            t_n_nf *= _t / (double)n_np;
-            n_np += 1;
+            const double2 a = *(const double2*)(in + 4 * iw);
+            v1r = fma(t_n_nf, a.x, v1r); v1i = fma(t_n_nf, a.y, v1i);
+            if (LH) { const double2 b = *(const double2*)(in + 4 * iw + 2); v2r = fma(t_n_nf, b.x, v2r); v2i = fma(t_n_nf, b.y, v2i); }
+            t_n_nf *= _t / d_np;
+            d_np += 1.0;
+            iw += np + 1;
         }
     }
     out[0] = v1r; out[1] = v1i; out[2] = v2r; out[3] = v2i;
@@ -643,8 +686,8 @@ __device__ inline void translate_and_transform(const WaveLds& W, double r, int P
         idx_nm(i, n, m);
         double o[4];
         translate_one<OP, LH>(W.X, r, Pu, n, m, o);
-        W.R[4 * i] = o[0]; W.R[4 * i + 1] = o[1];
-        if (LH) { W.R[4 * i + 2] = o[2]; W.R[4 * i + 3] = o[3]; }
+        *(double2*)(W.R + 4 * i) = make_double2(o[0], o[1]);
+        if (LH) *(double2*)(W.R + 4 * i + 2) = make_double2(o[2], o[3]);
     }
     wave_sync();
     for (int i = lane_id(); i < H; i += 64) {
@@ -654,9 +697,11 @@ __device__ inline void translate_and_transform(const WaveLds& W, double r, int P
         if (LH) {
             double o[4];
             lh_one<OP != OP_M2M>(W.R, r, Pu, n, m, o);
-            W.X[4 * i] = g * o[0]; W.X[4 * i + 1] = g * o[1]; W.X[4 * i + 2] = g * o[2]; W.X[4 * i + 3] = g * o[3];
+            *(double2*)(W.X + 4 * i) = make_double2(g * o[0], g * o[1]);
+            *(double2*)(W.X + 4 * i + 2) = make_double2(g * o[2], g * o[3]);
         } else {
-            W.X[4 * i] = g * W.R[4 * i]; W.X[4 * i + 1] = g * W.R[4 * i + 1];
+            const double2 a = *(const double2*)(W.R + 4 * i);
+            *(double2*)(W.X + 4 * i) = make_double2(g * a.x, g * a.y);
         }
     }
     wave_sync();
@@ -665,7 +710,7 @@ __device__ inline void translate_and_transform(const WaveLds& W, double r, int P
 // complete fixed-order translation of one expansion, accumulated into acc
 template <int OP, bool LH>
 __device__ void translate_fixed(const double* __restrict__ src, double dx, double dy, double dz, const WaveLds& W,
-                                const double* Hs, int P, const TablePtrs& tb, double (&acc)[MAXCH][8]) {
+                                const double* A, int P, const TablePtrs& tb, double (&acc)[MAXCH][8]) {
     double r, th, ph;
     c2s(dx, dy, dz, r, th, ph);
     fill_angles(W, ph, th, P);
@@ -673,25 +718,25 @@ __device__ void translate_fixed(const double* __restrict__ src, double dx, doubl
     const double* fwd_mags = (OP == OP_L2L) ? tb.eta : tb.zeta;
     const double* back_mags = (OP == OP_M2M) ? tb.zeta : tb.eta;
     const double sf = (OP == OP_L2L) ? -1.0 : 1.0, sb = (OP == OP_M2M) ? 1.0 : -1.0;
-    zrot_in<LH>(src, W, fwd_mags, 0, P);
+    zrot_in<LH, false>(src, W, fwd_mags, tb.Mt, 0, P);
     wave_sync();
     for (int n0 = 0; n0 <= P;) {
         const int n1 = chunk_end(n0, P, 64);
-        rot_chunk_store<LH>(W, Hs, fwd_mags, sf, n0, n1);
+        rot_chunk_store<LH>(W, A, fwd_mags, sf, n0, n1);
         n0 = n1 + 1;
     }
     translate_and_transform<OP, LH>(W, r, P, back_mags);
-    rot_back_acc<LH>(W, Hs, back_mags, sb, P, P, acc);
+    rot_back_acc<LH>(W, A, back_mags, sb, P, P, acc);
 }
 
 // ================================================================= M2M
-template <bool LH>
-__global__ __launch_bounds__(64 * WPB) void k_m2m_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
-                                                        int64_t b1, int P, TablePtrs tb) {
+template <bool LH, int NW>
+__global__ __launch_bounds__(64 * NW) void k_m2m_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
+                                                       int64_t b1, int P, TablePtrs tb) {
     extern __shared__ double lds[];
-    const double* Hs = load_hs(lds, tb.Hs, P);
-    const WaveLds W = carve_wave(lds + hs_lds_doubles(P), P);
-    const int64_t ib = b0 + (int64_t)blockIdx.x * WPB + wave_id();
+    const double* A = load_tables(lds, tb.Ad, P);
+    const WaveLds W = carve_wave(lds, P);
+    const int64_t ib = b0 + (int64_t)blockIdx.x * NW + wave_id();
     if (ib > b1) return;
     const Branch B = br[ib];
     if (B.n_branches == 0) return;
@@ -701,32 +746,38 @@ __global__ __launch_bounds__(64 * WPB) void k_m2m_level(Branch* __restrict__ br,
         const Branch& Ch = br[c];
         double dx = B.source_center[0] - Ch.source_center[0], dy = B.source_center[1] - Ch.source_center[1],
                dz = B.source_center[2] - Ch.source_center[2];
-        translate_fixed<OP_M2M, LH>(ex + c * H4, dx, dy, dz, W, Hs, P, tb, acc);
+        translate_fixed<OP_M2M, LH>(ex + c * H4, dx, dy, dz, W, A, P, tb, acc);
     }
     acc_store<true>(ex + ib * H4, P, acc);
+}
+
+template <bool LH, int NW>
+static void launch_m2m(Context& C, Tree& st, int p, const TablePtrs& tb) {
+    const size_t sh = translate_setup(k_m2m_level<LH, NW>, p, NW);
+    for (int64_t l = (int64_t)st.level_first.size() - 2; l >= 0; --l) {
+        int64_t b0 = st.level_first[l], b1 = st.level_last[l];
+        const unsigned grid = (unsigned)((b1 - b0 + NW) / NW);
+        k_m2m_level<LH, NW><<<grid, 64 * NW, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, tb);
+        LAUNCH_CHECK();
+    }
 }
 
 void run_m2m(Context& C, Tree& st, int p, bool LH) {
     ensure_tables(C, p);
     TablePtrs tb = ptrs(C);
-    const size_t sh = LH ? translate_setup(k_m2m_level<true>, p) : translate_setup(k_m2m_level<false>, p);
-    for (int64_t l = (int64_t)st.level_first.size() - 2; l >= 0; --l) {
-        int64_t b0 = st.level_first[l], b1 = st.level_last[l];
-        const unsigned grid = (unsigned)((b1 - b0 + WPB) / WPB);
-        if (LH) k_m2m_level<true><<<grid, 64 * WPB, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, tb);
-        else k_m2m_level<false><<<grid, 64 * WPB, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, tb);
-        LAUNCH_CHECK();
-    }
+    const int nw = translate_waves(p);
+    if (LH) { if (nw == 8) launch_m2m<true, 8>(C, st, p, tb); else if (nw == 6) launch_m2m<true, 6>(C, st, p, tb); else launch_m2m<true, 4>(C, st, p, tb); }
+    else { if (nw == 8) launch_m2m<false, 8>(C, st, p, tb); else if (nw == 6) launch_m2m<false, 6>(C, st, p, tb); else launch_m2m<false, 4>(C, st, p, tb); }
 }
 
 // ================================================================= L2L
-template <bool LH>
-__global__ __launch_bounds__(64 * WPB) void k_l2l_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
-                                                        int64_t b1, int P, TablePtrs tb) {
+template <bool LH, int NW>
+__global__ __launch_bounds__(64 * NW) void k_l2l_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
+                                                       int64_t b1, int P, TablePtrs tb) {
     extern __shared__ double lds[];
-    const double* Hs = load_hs(lds, tb.Hs, P);
-    const WaveLds W = carve_wave(lds + hs_lds_doubles(P), P);
-    const int64_t ib = b0 + (int64_t)blockIdx.x * WPB + wave_id();
+    const double* A = load_tables(lds, tb.Ad, P);
+    const WaveLds W = carve_wave(lds, P);
+    const int64_t ib = b0 + (int64_t)blockIdx.x * NW + wave_id();
     if (ib > b1) return;
     const Branch B = br[ib];
     const Branch& Pa = br[B.parent];
@@ -734,21 +785,27 @@ __global__ __launch_bounds__(64 * WPB) void k_l2l_level(Branch* __restrict__ br,
     double acc[MAXCH][8] = {};
     double dx = B.target_center[0] - Pa.target_center[0], dy = B.target_center[1] - Pa.target_center[1],
            dz = B.target_center[2] - Pa.target_center[2];
-    translate_fixed<OP_L2L, LH>(ex + B.parent * H4, dx, dy, dz, W, Hs, P, tb, acc);
+    translate_fixed<OP_L2L, LH>(ex + B.parent * H4, dx, dy, dz, W, A, P, tb, acc);
     acc_store<true>(ex + ib * H4, P, acc);
+}
+
+template <bool LH, int NW>
+static void launch_l2l(Context& C, Tree& tt, int p, const TablePtrs& tb) {
+    const size_t sh = translate_setup(k_l2l_level<LH, NW>, p, NW);
+    for (size_t l = 1; l < tt.level_first.size(); ++l) {
+        int64_t b0 = tt.level_first[l], b1 = tt.level_last[l];
+        const unsigned grid = (unsigned)((b1 - b0 + NW) / NW);
+        k_l2l_level<LH, NW><<<grid, 64 * NW, sh, C.stream>>>(tt.branches.data(), tt.expansions.data(), b0, b1, p, tb);
+        LAUNCH_CHECK();
+    }
 }
 
 void run_l2l(Context& C, Tree& tt, int p, bool LH) {
     ensure_tables(C, p);
     TablePtrs tb = ptrs(C);
-    const size_t sh = LH ? translate_setup(k_l2l_level<true>, p) : translate_setup(k_l2l_level<false>, p);
-    for (size_t l = 1; l < tt.level_first.size(); ++l) {
-        int64_t b0 = tt.level_first[l], b1 = tt.level_last[l];
-        const unsigned grid = (unsigned)((b1 - b0 + WPB) / WPB);
-        if (LH) k_l2l_level<true><<<grid, 64 * WPB, sh, C.stream>>>(tt.branches.data(), tt.expansions.data(), b0, b1, p, tb);
-        else k_l2l_level<false><<<grid, 64 * WPB, sh, C.stream>>>(tt.branches.data(), tt.expansions.data(), b0, b1, p, tb);
-        LAUNCH_CHECK();
-    }
+    const int nw = translate_waves(p);
+    if (LH) { if (nw == 8) launch_l2l<true, 8>(C, tt, p, tb); else if (nw == 6) launch_l2l<true, 6>(C, tt, p, tb); else launch_l2l<true, 4>(C, tt, p, tb); }
+    else { if (nw == 8) launch_l2l<false, 8>(C, tt, p, tb); else if (nw == 6) launch_l2l<false, 6>(C, tt, p, tb); else launch_l2l<false, 4>(C, tt, p, tb); }
 }
 
 // ================================================================= M2L
@@ -766,77 +823,69 @@ __device__ inline double md1(double dx, double rm, double rp) {   // src/error.j
     return same ? mn : 0.0;
 }
 
-// Geometry and running factors of the degree loop of the error-controlled
-// multipole_to_local! methods (src/translate.jl:555-1412). The reference walks
-// the degrees in order, updating running products after each degree; here each
-// lane evaluates the test of one degree, replaying the same product sequence.
+// Geometry of the error-controlled multipole_to_local! methods
+// (src/translate.jl:555-1412). The reference walks the degrees in order and
+// updates running products after each degree; here lane n replays the same
+// product sequence for its degree once per interaction (fac), and each degree's
+// test is evaluated by its own lane.
 struct ErrGeo {
     double limit, rinv, r, r_mp, r_mp_inv, r_l;
 };
 
-struct DegFactors {
-    double n_f_t_np1, r_l_nm1, r_mp_inv_np2, nm1_f_inv, np1_f;
-};
-
-__device__ inline DegFactors deg_factors(const ErrGeo& G, int n) {   // values at the start of iteration n
-    DegFactors F;
-    F.n_f_t_np1 = G.rinv * G.rinv;
-    F.r_l_nm1 = 1.0;
-    F.r_mp_inv_np2 = G.r_mp_inv * G.r_mp_inv * G.r_mp_inv;
-    F.nm1_f_inv = 1.0;
-    F.np1_f = 2.0;
+// running factors at the start of iteration n: n!/t^(n+1)-type chain, r_l^(n-1),
+// r_mp^-(n+2), 1/(n-1)!, (n+1)!  -> fac[6n .. 6n+4]
+__device__ inline void fill_factors(const WaveLds& W, const ErrGeo& G, int P) {
+    const int n = lane_id();
+    if (n < 1 || n > P) return;
+    double n_f_t_np1 = G.rinv * G.rinv, r_l_nm1 = 1.0, r_mp_inv_np2 = G.r_mp_inv * G.r_mp_inv * G.r_mp_inv;
+    double nm1_f_inv = 1.0, np1_f = 2.0;
     for (int k = 1; k < n; ++k) {
-        F.n_f_t_np1 *= (double)(k + 1) * G.rinv;
-        F.r_l_nm1 *= G.r_l;
-        F.r_mp_inv_np2 *= G.r_mp_inv;
-        F.nm1_f_inv /= (double)k;
-        F.np1_f *= (double)(k + 1);
+        n_f_t_np1 *= (double)(k + 1) * G.rinv;
+        r_l_nm1 *= G.r_l;
+        r_mp_inv_np2 *= G.r_mp_inv;
+        nm1_f_inv /= (double)k;
+        np1_f *= (double)(k + 1);
     }
-    return F;
+    double* f = W.fac + 6 * n;
+    f[0] = n_f_t_np1; f[1] = r_l_nm1; f[2] = r_mp_inv_np2; f[3] = nm1_f_inv; f[4] = np1_f;
 }
 
-// power of the unrotated multipole of degree k: sqrt(sum_m |M_k^m|^2 Mt_k^m^2) (phi, chi)
-__device__ inline void degree_power(const double* __restrict__ src, const double* Mt, int k, bool LH, double& pw, double& pc) {
-    const int i0 = nh(k - 1);
+// degree powers of the chunk from the terms zrot_in left in R (summed in m order)
+template <bool LH>
+__device__ inline void chunk_powers(const WaveLds& W, int n0, int n1) {
+    const int n = n0 + lane_id();
+    if (n > n1) return;
+    const double* t = W.R + 2 * (nh(n - 1) - nh(n0 - 1));
     double a = 0.0, b = 0.0;
-    if (k == 0) {
-        double x = src[0], y = src[1];
-        a = (x * x + y * y) * Mt[0] * Mt[0];
-    } else {
-        for (int m = 0; m <= k; ++m) {
-            const int i = i0 + m;
-            const double Nt = Mt[i];
-            double xr = src[4 * i], xi = src[4 * i + 1];
-            a += (xr * xr + xi * xi) * Nt * Nt;
-            if (LH) { xr = src[4 * i + 2]; xi = src[4 * i + 3]; b += (xr * xr + xi * xi) * Nt * Nt; }
-        }
+    for (int m = 0; m <= n; ++m) {
+        a += t[2 * m];
+        if (LH) b += t[2 * m + 1];
     }
-    pw = sqrt(a);
-    pc = sqrt(b);
+    W.pw[2 * n] = sqrt(a);
+    W.pw[2 * n + 1] = sqrt(b);
 }
 
 // Tests of degrees nA..nB (rotated multipole of degrees <= nB in W.X). Returns
 // the first degree that meets the tolerance, or -1.
 template <bool LH>
-__device__ int m2l_test_chunk(const double* __restrict__ src, const WaveLds& W, const ErrGeo& G, const ErrParams& E,
-                              const TablePtrs& tb, int nA, int nB) {
+__device__ int m2l_test_chunk(const WaveLds& W, const ErrGeo& G, const ErrParams& E, const TablePtrs& tb, int nA, int nB) {
     if (E.kind >= 2) {
         // degree-n part of the z-translation, all m (translate_multipole_to_local_z_n!, :238-295) -> R
         const int i0 = nh(nA - 1), cnt = nh(nB) - i0;
         for (int k = lane_id(); k < cnt; k += 64) {
             int n, m;
             idx_nm(i0 + k, n, m);
-            const DegFactors F = deg_factors(G, n);
-            double n_m_f = F.n_f_t_np1;
+            double n_m_f = W.fac[6 * n];
             for (int q = 0; q < m; ++q) n_m_f *= (double)(n + q + 1) * G.rinv;
             double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
             int n_np = n + m;
+            int iw = nh(m - 1) + m;
             for (int np = m; np <= n; ++np) {
-                const int iw = (np * (np + 1)) / 2 + m;
                 v1r += f * W.X[4 * iw]; v1i += f * W.X[4 * iw + 1];
                 if (LH) { v2r += f * W.X[4 * iw + 2]; v2i += f * W.X[4 * iw + 3]; }
                 n_np += 1;
                 f *= (double)n_np * G.rinv;
+                iw += np + 1;
             }
             double* t = W.R + 4 * k;
             t[0] = v1r; t[1] = v1i; t[2] = v2r; t[3] = v2i;
@@ -846,33 +895,32 @@ __device__ int m2l_test_chunk(const double* __restrict__ src, const WaveLds& W, 
     bool pass = false;
     const int n = nA + lane_id();
     if (n <= nB) {
-        const DegFactors F = deg_factors(G, n);
+        const double* F = W.fac + 6 * n;
+        const double n_f_t_np1 = F[0], r_l_nm1 = F[1], r_mp_inv_np2 = F[2], nm1_f_inv = F[3], np1_f = F[4];
         const int in0 = nh(n - 1);
         double eps_mp;
         if (E.kind == 1) {
             const double p0 = W.X[4 * in0], p1r = W.X[4 * (in0 + 1)], p1i = W.X[4 * (in0 + 1) + 1];
-            eps_mp = (sqrt(p1r * p1r + p1i * p1i) + fabs(p0)) * F.np1_f * F.r_mp_inv_np2;
+            eps_mp = (sqrt(p1r * p1r + p1i * p1i) + fabs(p0)) * np1_f * r_mp_inv_np2;
             if (LH) {
                 const double c1r = W.X[4 * (in0 + 1) + 2], c1i = W.X[4 * (in0 + 1) + 3];
-                eps_mp += sqrt(c1r * c1r + c1i * c1i) * F.np1_f * F.r_mp_inv_np2 * G.r_mp;
+                eps_mp += sqrt(c1r * c1r + c1i * c1i) * np1_f * r_mp_inv_np2 * G.r_mp;
             }
         } else {
-            double pw_prev, pc_prev, pw_n, pc_n;
-            degree_power(src, tb.Mt, n - 1, LH, pw_prev, pc_prev);
-            degree_power(src, tb.Mt, n, LH, pw_n, pc_n);
+            const double pw_prev = W.pw[2 * (n - 1)], pw_n = W.pw[2 * n], pc_n = W.pw[2 * n + 1];
             const double Mt_n0 = tb.Mt[nh(n - 2)], Mt_next = tb.Mt[in0];
             if (E.kind == 3) {
-                eps_mp = SQRT3_D * pw_prev * F.r_mp_inv_np2 * F.np1_f / Mt_n0;
-                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * F.r_mp_inv_np2 * F.np1_f / Mt_next;
+                eps_mp = SQRT3_D * pw_prev * r_mp_inv_np2 * np1_f / Mt_n0;
+                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * r_mp_inv_np2 * np1_f / Mt_next;
             } else {
-                eps_mp = SQRT3_D * pw_n * F.r_mp_inv_np2 * G.r_mp * F.np1_f / (Mt_n0 * (double)(n + 1));
-                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * F.r_mp_inv_np2 * G.r_mp * F.np1_f / (Mt_next * (double)(n + 1));
+                eps_mp = SQRT3_D * pw_n * r_mp_inv_np2 * G.r_mp * np1_f / (Mt_n0 * (double)(n + 1));
+                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * r_mp_inv_np2 * G.r_mp * np1_f / (Mt_next * (double)(n + 1));
             }
         }
         if (eps_mp <= G.limit) {
             double eps_l;
             if (E.kind == 1) {   // translate_multipole_to_local_z_m01_n (:302-362)
-                double n_m_f = F.n_f_t_np1;
+                double n_m_f = n_f_t_np1;
                 int n_m = n;
                 double p0r = 0.0, n_np_f = n_m_f;
                 int n_np = n_m;
@@ -899,8 +947,8 @@ __device__ int m2l_test_chunk(const double* __restrict__ src, const WaveLds& W, 
                     p1r -= t_over_n * c1i;
                     p1i += t_over_n * c1r;
                 }
-                eps_l = (fabs(p1r) + fabs(p1i) + fabs(p0r)) * F.r_l_nm1 * F.nm1_f_inv;
-                if (LH) eps_l += sqrt(c1r * c1r + c1i * c1i) * F.r_l_nm1 * G.r_l * F.nm1_f_inv;
+                eps_l = (fabs(p1r) + fabs(p1i) + fabs(p0r)) * r_l_nm1 * nm1_f_inv;
+                if (LH) eps_l += sqrt(c1r * c1r + c1i * c1i) * r_l_nm1 * G.r_l * nm1_f_inv;
             } else {   // local_power (:455-502) of the degree-n translation
                 const double* tmp = W.R + 4 * (in0 - nh(nA - 1));
                 double lp = 0, lc = 0, Nt = 0, cnr = 0, cni = 0;
@@ -921,11 +969,11 @@ __device__ int m2l_test_chunk(const double* __restrict__ src, const WaveLds& W, 
                 lp = sqrt(lp);
                 if (LH) lc = sqrt(lc);
                 if (E.kind == 3) {
-                    eps_l = SQRT3_D * lp * F.r_l_nm1 * F.nm1_f_inv / Nt;
-                    if (LH) eps_l += SQRT3_D * (double)n * lc * F.r_l_nm1 * G.r_l * F.nm1_f_inv / (Nt * (double)n);
+                    eps_l = SQRT3_D * lp * r_l_nm1 * nm1_f_inv / Nt;
+                    if (LH) eps_l += SQRT3_D * (double)n * lc * r_l_nm1 * G.r_l * nm1_f_inv / (Nt * (double)n);
                 } else {
-                    eps_l = SQRT3_D * lp * F.r_l_nm1 * G.r_l * F.nm1_f_inv / (Nt * (double)n);
-                    if (LH) eps_l += SQRT3_D * (double)n * lc * F.r_l_nm1 * G.r_l * F.nm1_f_inv / (Nt * (double)n);
+                    eps_l = SQRT3_D * lp * r_l_nm1 * G.r_l * nm1_f_inv / (Nt * (double)n);
+                    if (LH) eps_l += SQRT3_D * (double)n * lc * r_l_nm1 * G.r_l * nm1_f_inv / (Nt * (double)n);
                 }
             }
             pass = (eps_mp + eps_l * LOCAL_ERROR_SAFETY <= G.limit);
@@ -939,22 +987,22 @@ __device__ int m2l_test_chunk(const double* __restrict__ src, const WaveLds& W, 
 // one M2L interaction (fixed order when E.kind == 0) accumulated into acc; returns the order used
 template <bool LH>
 __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const Branch& S, const WaveLds& W,
-                        const double* Hs, int P, const TablePtrs& tb, const ErrParams& E, int& success,
+                        const double* A, int P, const TablePtrs& tb, const ErrParams& E, int& success,
                         double (&acc)[MAXCH][8]) {
     const double dx = T.target_center[0] - S.source_center[0], dy = T.target_center[1] - S.source_center[1],
                  dz = T.target_center[2] - S.source_center[2];
     double r, th, ph;
     c2s(dx, dy, dz, r, th, ph);
     fill_angles(W, ph, th, P);
-    wave_sync();
     int Pu = P;
     success = 1;
     if (E.kind == 0) {
-        zrot_in<LH>(src, W, tb.zeta, 0, P);
+        wave_sync();
+        zrot_in<LH, false>(src, W, tb.zeta, tb.Mt, 0, P);
         wave_sync();
         for (int n0 = 0; n0 <= P;) {
             const int n1 = chunk_end(n0, P, 64);
-            rot_chunk_store<LH>(W, Hs, tb.zeta, 1.0, n0, n1);
+            rot_chunk_store<LH>(W, A, tb.zeta, 1.0, n0, n1);
             n0 = n1 + 1;
         }
     } else {
@@ -969,15 +1017,24 @@ __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const B
         G.r_l = T.target_radius;
         G.rinv = 1.0 / r;
         G.r_mp_inv = 1.0 / G.r_mp;
+        fill_factors(W, G, P);
+        wave_sync();
         success = 0;
+        const bool power = E.kind >= 2;
         for (int n0 = 0; n0 <= P;) {
             const int n1 = chunk_end(n0, P, FWD_DEGREES);
-            zrot_in<LH>(src, W, tb.zeta, n0, n1);
+            if (power) {
+                zrot_in<LH, true>(src, W, tb.zeta, tb.Mt, n0, n1);
+                wave_sync();
+                chunk_powers<LH>(W, n0, n1);
+            } else {
+                zrot_in<LH, false>(src, W, tb.zeta, tb.Mt, n0, n1);
+            }
             wave_sync();
-            rot_chunk_store<LH>(W, Hs, tb.zeta, 1.0, n0, n1);
+            rot_chunk_store<LH>(W, A, tb.zeta, 1.0, n0, n1);
             const int nA = n0 < 1 ? 1 : n0;
             if (nA <= n1) {
-                const int hit = m2l_test_chunk<LH>(src, W, G, E, tb, nA, n1);
+                const int hit = m2l_test_chunk<LH>(W, G, E, tb, nA, n1);
                 if (hit >= 0) {
                     Pu = hit - 1 + (E.BE ? 1 : 0);
                     success = 1;
@@ -988,51 +1045,61 @@ __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const B
         }
     }
     translate_and_transform<OP_M2L, LH>(W, r, Pu, tb.eta);
-    rot_back_acc<LH>(W, Hs, tb.eta, -1.0, P, Pu, acc);
+    rot_back_acc<LH>(W, A, tb.eta, -1.0, P, Pu, acc);
     return Pu;
 }
 
 // One workgroup per target box: its wavefronts take the box's M2L sources in
 // turn, accumulate in registers, and the partial locals are summed through LDS
 // and added to the box's expansion once.
-template <bool LH>
-__global__ __launch_bounds__(64 * WPB) void k_m2l(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
-                                                  const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
-                                                  const Branch* __restrict__ sbr, const double* __restrict__ sex,
-                                                  double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
-                                                  int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
+template <bool LH, int NW>
+__global__ __launch_bounds__(64 * NW) void k_m2l(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
+                                                 const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
+                                                 const Branch* __restrict__ sbr, const double* __restrict__ sex,
+                                                 double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
+                                                 int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
     extern __shared__ double lds[];
-    const double* Hs = load_hs(lds, tb.Hs, P);
-    const WaveLds W = carve_wave(lds + hs_lds_doubles(P), P);
+    const double* A = load_tables(lds, tb.Ad, P);
+    const WaveLds W = carve_wave(lds, P);
     const int32_t it = m2l_targets[blockIdx.x];
     const int64_t l0 = m2l_offsets[blockIdx.x], l1 = m2l_offsets[blockIdx.x + 1];
     const int H4 = 4 * nh(P);
     const Branch T = tbr[it];
     double acc[MAXCH][8] = {};
     int local_max = 0, local_ok = 1;
-    for (int64_t li = l0 + wave_id(); li < l1; li += WPB) {
+    for (int64_t li = l0 + wave_id(); li < l1; li += NW) {
         const int32_t js = m2l_s[li];
         const Branch S = sbr[js];
         int ok = 1;
-        const int Pu = m2l_pair<LH>(sex + (int64_t)js * H4, T, S, W, Hs, P, tb, E, ok, acc);
+        const int Pu = m2l_pair<LH>(sex + (int64_t)js * H4, T, S, W, A, P, tb, E, ok, acc);
         if (lane_id() == 0) orders[li] = Pu;
         local_max = max(local_max, Pu);
         local_ok = local_ok && ok;
     }
     acc_store<false>(W.X, P, acc);
     __syncthreads();
-    const double* base = lds + hs_lds_doubles(P);
+    const double* base = lds + len_A(P);
     const int stride = wave_lds_doubles(P);
-    for (int i = threadIdx.x; i < H4; i += 64 * WPB) {
+    for (int i = threadIdx.x; i < H4; i += 64 * NW) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < WPB; ++w) s += base[(size_t)w * stride + i];
+        for (int w = 0; w < NW; ++w) s += base[(size_t)w * stride + i];
         tex[(int64_t)it * H4 + i] += s;
     }
     if (lane_id() == 0) {
         atomicMax(pmax, local_max);
         if (!local_ok) atomicAnd(all_ok, 0);
     }
+}
+
+template <bool LH, int NW>
+static void launch_m2l(Context& C, int p, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E, int32_t* flags,
+                       const TablePtrs& tb) {
+    const size_t sh = translate_setup(k_m2l<LH, NW>, p, NW);
+    k_m2l<LH, NW><<<(unsigned)Ls.n_m2l_targets, 64 * NW, sh, C.stream>>>(
+        Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
+        st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags, flags + 1, tb);
+    LAUNCH_CHECK();
 }
 
 // ================================================= small orders: lane per M2L
@@ -1271,16 +1338,16 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
         else launch_m2l_lane<false>(C, p, Ls, st, tt_dst, flags.data(), tb);
     } else if (Ls.n_m2l_targets > 0) {
         ErrParams E{err_kind, err_rel ? 1 : 0, eps, BE ? 1 : 0};
-        const size_t sh = LH ? translate_setup(k_m2l<true>, p) : translate_setup(k_m2l<false>, p);
-        if (LH)
-            k_m2l<true><<<(unsigned)Ls.n_m2l_targets, 64 * WPB, sh, C.stream>>>(
-                Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
-                st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags.data(), flags.data() + 1, tb);
-        else
-            k_m2l<false><<<(unsigned)Ls.n_m2l_targets, 64 * WPB, sh, C.stream>>>(
-                Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
-                st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags.data(), flags.data() + 1, tb);
-        LAUNCH_CHECK();
+        const int nw = translate_waves(p);
+        if (LH) {
+            if (nw == 8) launch_m2l<true, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            else if (nw == 6) launch_m2l<true, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            else launch_m2l<true, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+        } else {
+            if (nw == 8) launch_m2l<false, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            else if (nw == 6) launch_m2l<false, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            else launch_m2l<false, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+        }
     }
     int32_t h[2];
     HIP_CHECK(hipMemcpyAsync(h, flags.data(), sizeof(h), hipMemcpyDeviceToHost, C.stream));
