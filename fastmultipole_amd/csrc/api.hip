@@ -151,7 +151,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
     if (!E.lists_built) throw Error(FMMHIP_EINVAL, "interaction lists are not built");
     const int p = o.expansion_order;
     if (p != E.p_built) throw Error(FMMHIP_EINVAL, "expansion_order differs from the one the trees were built with");
-    const bool LH = E.kernel == FMMHIP_POINT_VORTEX;   // has_vector_potential
+    const bool LH = has_vector_potential(E.kernel);
     const bool PS = o.scalar_potential != 0, GS = o.gradient != 0, HS = o.hessian != 0;
     Timer total(C);
     double t_est = 0, t_gather = 0;
@@ -314,7 +314,7 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
     Eval& E = ev->e;
     Context* C = E.ctx;
     ABI_TRY(C, {
-        if (kernel != FMMHIP_POINT_SOURCE && kernel != FMMHIP_POINT_VORTEX) throw Error(FMMHIP_EINVAL, "unknown kernel");
+        if (!valid_kernel(kernel)) throw Error(FMMHIP_EINVAL, "unknown kernel");
         if (nt < 0 || ns < 0) throw Error(FMMHIP_EINVAL, "negative system size");
         if ((ns > 0 && !s_buf) || (nt > 0 && !t_xyz && nt != ns)) throw Error(FMMHIP_EINVAL, "missing system buffer");
         E.kernel = kernel;
@@ -506,7 +506,7 @@ fmmhip_status fmmhip_direct(fmmhip_ctx* h, int32_t kernel, int64_t nt, const dou
     if (!h) return FMMHIP_EINVAL;
     Context* C = &h->c;
     ABI_TRY(C, {
-        if (kernel != FMMHIP_POINT_SOURCE && kernel != FMMHIP_POINT_VORTEX) throw Error(FMMHIP_EINVAL, "unknown kernel");
+        if (!valid_kernel(kernel)) throw Error(FMMHIP_EINVAL, "unknown kernel");
         if (nt < 0 || ns < 0 || (nt && (!t_xyz || !out)) || (ns && !s_buf)) throw Error(FMMHIP_EINVAL, "bad direct arguments");
         const int dpb = data_per_body(kernel);
         if (on_device) {

@@ -1418,7 +1418,7 @@ __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const
                 const int64_t j = B.first + b0 + threadIdx.x;
                 lb[0][threadIdx.x] = x[j] - cx; lb[1][threadIdx.x] = y[j] - cy; lb[2][threadIdx.x] = z[j] - cz;
                 lb[3][threadIdx.x] = q[j];
-                if (KERNEL == FMMHIP_POINT_VORTEX) { lb[4][threadIdx.x] = q[ns + j]; lb[5][threadIdx.x] = q[2 * ns + j]; }
+                if (KERNEL != FMMHIP_POINT_SOURCE) { lb[4][threadIdx.x] = q[ns + j]; lb[5][threadIdx.x] = q[2 * ns + j]; }
             }
             __syncthreads();
             if (i >= H) continue;
@@ -1430,6 +1430,23 @@ __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const
                     double f = (((n + m) & 1) ? 1.0 : -1.0) * lb[3][s];
                     a0 += f * R[0];
                     a1 -= f * R[1];
+                } else if (KERNEL == FMMHIP_POINT_DIPOLE) {
+                    // source_to_dipole! of -q on the harmonics of dx (src/bodytomultipole.jl:251-278, 431-463):
+                    // h_n^m from q_{n-1}^{m-1}, q_{n-1}^m, q_{n-1}^{m+1} (get_nm1 rules), M += (-1)^(n+m) conj(h)
+                    if (n >= 1) {
+                        const double sxq = -lb[3][s], syq = -lb[4][s], szq = -lb[5][s];
+                        double Rm1[4], R0[4], Rp1[4];
+                        solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m > 0 ? m - 1 : 1, inv_fact, Rm1);
+                        solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m, inv_fact, R0);
+                        solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m + 1, inv_fact, Rp1);
+                        const double q0 = m > 0 ? Rm1[2] : -Rm1[2], q1 = Rm1[3];   // q_{n-1}^{-1} = -conj(q_{n-1}^1)
+                        const double q2 = R0[2], q3 = R0[3], q4 = Rp1[2], q5 = Rp1[3];
+                        const double hr = -sxq * 0.5 * (q5 + q1) + syq * 0.5 * (q4 - q0) - szq * q2;
+                        const double hi = sxq * 0.5 * (q4 + q0) + syq * 0.5 * (q5 - q1) - szq * q3;
+                        const double f = ((n + m) & 1) ? -1.0 : 1.0;
+                        a0 += f * hr;
+                        a1 -= f * hi;
+                    }
                 } else {
                     // mirrored_source_to_vortex! with q = R(-dx) (src/bodytomultipole.jl:283-334, 465-489)
                     const double vx = lb[3][s], vy = lb[4][s], vz = lb[5][s];
@@ -1467,7 +1484,7 @@ __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const
         if (i < H) {
             double* e = ex + ib * 4 * H + 4 * i;
             e[0] += a0; e[1] += a1;
-            if (KERNEL == FMMHIP_POINT_VORTEX) { e[2] += a2; e[3] += a3; }
+            if (KERNEL == FMMHIP_POINT_VORTEX) { e[2] += a2; e[3] += a3; }   // Lamb-Helmholtz chi part
         }
     }
 }
@@ -1476,12 +1493,13 @@ void run_p2m(Context& C, int kernel, Tree& st, int p) {
     ensure_tables(C, p);
     if (st.n_leaves == 0) return;
     TablePtrs tb = ptrs(C);
-    if (kernel == FMMHIP_POINT_SOURCE)
-        k_p2m<FMMHIP_POINT_SOURCE><<<(unsigned)st.n_leaves, 64, 0, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(),
-                                                                              st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact);
-    else
-        k_p2m<FMMHIP_POINT_VORTEX><<<(unsigned)st.n_leaves, 64, 0, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(),
-                                                                              st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact);
+#define FMMHIP_P2M(K)                                                                                                  \
+    k_p2m<K><<<(unsigned)st.n_leaves, 64, 0, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(), \
+                                                        st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact)
+    if (kernel == FMMHIP_POINT_SOURCE) FMMHIP_P2M(FMMHIP_POINT_SOURCE);
+    else if (kernel == FMMHIP_POINT_DIPOLE) FMMHIP_P2M(FMMHIP_POINT_DIPOLE);
+    else FMMHIP_P2M(FMMHIP_POINT_VORTEX);   // the Gaussian core changes only the nearfield
+#undef FMMHIP_P2M
     LAUNCH_CHECK();
 }
 

@@ -124,8 +124,11 @@ struct Eval {
     ~Eval() { delete estimate; }
 };
 
-inline int data_per_body(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 5 : 7; }
+__host__ __device__ inline int data_per_body(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 5 : 7; }
 inline int strength_dims(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 1 : 3; }
+inline bool valid_kernel(int kernel) { return kernel >= FMMHIP_POINT_SOURCE && kernel <= FMMHIP_VORTEX_GAUSSIAN; }
+// has_vector_potential (test/vortex.jl:145-147): Lamb-Helmholtz chi expansions
+inline bool has_vector_potential(int kernel) { return kernel == FMMHIP_POINT_VORTEX || kernel == FMMHIP_VORTEX_GAUSSIAN; }
 inline int n_harmonics(int p) { return ((p + 1) * (p + 2)) / 2; }
 
 // ---------------------------------------------------------- module entry

@@ -1,6 +1,11 @@
 // Nearfield P2P on the GPU for the built-in point kernels:
 //   Point{Source}: Laplace 1/r potential + gradient  (test/gravitational.jl:80-104)
 //   Point{Vortex}: singular vortex velocity + velocity gradient (test/vortex.jl:83-128)
+//   Point{Dipole}: phi = q.d/(4 pi r^3), the potential of the reference's dipole
+//                  expansion (src/bodytomultipole.jl:431-463), with gradient and
+//                  Hessian slots in the expansion evaluation's convention
+//   Gaussian-core vortex: the singular vortex kernel times the core factor
+//                  g(r/sigma) = erf(rho/sqrt 2) - sqrt(2/pi) rho exp(-rho^2/2)
 // driven by the direct list sorted by target (src/fmm.jl:18-40).
 //
 // One workgroup per target leaf. Targets live in registers (power-of-two
@@ -31,14 +36,98 @@ __device__ inline double rinv_f64(double r2) {
     return y;
 }
 
-template <class R> struct Acc { R v[13]; };
-
 // Kahan-compensated add of a partial sum into (s, c)
 __device__ inline void kahan_add(float& s, float& c, float x) {
     float y = x - c;
     float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+// sources per LDS slot: strengths (pre-scaled by 1/(4 pi)) and, for the Gaussian
+// core, sigma
+template <int K> struct KernelTraits {
+    static constexpr int NS = K == FMMHIP_POINT_SOURCE ? 1 : (K == FMMHIP_VORTEX_GAUSSIAN ? 4 : 3);
+    static constexpr bool HESS = K != FMMHIP_POINT_SOURCE;   // the reference's source kernel has no Hessian term
+    static constexpr bool POT = K == FMMHIP_POINT_SOURCE || K == FMMHIP_POINT_DIPOLE;
+};
+
+template <class R> __device__ inline R erf_(R x);
+template <> __device__ inline double erf_<double>(double x) { return erf(x); }
+template <> __device__ inline float erf_<float>(float x) { return erff(x); }
+template <class R> __device__ inline R exp_(R x);
+template <> __device__ inline double exp_<double>(double x) { return exp(x); }
+template <> __device__ inline float exp_<float>(float x) { return expf(x); }
+
+// one body pair: d = x_target - x_source, ri = 1/|d|, g = staged source values;
+// accumulates [phi, gradient slot (3), Hessian slot (9)] into a
+template <int K, bool PS, bool GS, bool HS, class R>
+__device__ inline void pair_eval(R dx, R dy, R dz, R r2, R ri, const R* g, R* a) {
+    const R ri2 = ri * ri;
+    if (K == FMMHIP_POINT_SOURCE) {
+        const R q = g[0];
+        if (PS) a[0] += q * ri;
+        if (GS) { const R f = q * ri * ri2; a[1] += dx * f; a[2] += dy * f; a[3] += dz * f; }
+    } else if (K == FMMHIP_POINT_DIPOLE) {
+        const R qx = g[0], qy = g[1], qz = g[2];
+        const R qd = qx * dx + qy * dy + qz * dz;
+        const R ri3 = ri * ri2, ri5 = ri3 * ri2;
+        if (PS) a[0] += qd * ri3;
+        const R t = R(3) * qd * ri5;
+        if (GS) { a[1] += t * dx - qx * ri3; a[2] += t * dy - qy * ri3; a[3] += t * dz - qz * ri3; }
+        if (HS) {
+            const R u = R(-15) * qd * ri5 * ri2, f3 = R(3) * ri5;
+            const R d[3] = {dx, dy, dz}, q[3] = {qx, qy, qz};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    a[4 + 3 * i + j] += f3 * (q[j] * d[i] + q[i] * d[j]) + u * d[i] * d[j] + (i == j ? t : R(0));
+        }
+    } else {
+        const R gx = g[0], gy = g[1], gz = g[2];
+        R gf = R(1), gp = R(0), sri = R(0);
+        if (K == FMMHIP_VORTEX_GAUSSIAN) {
+            const R sig = g[3];
+            if (sig > R(0)) {
+                const R r = r2 * ri, rho = r / sig;
+                if (rho < (sizeof(R) == 8 ? R(9.5) : R(7))) {
+                    const R c2pi = R(0.79788456080286535588), e = exp_<R>(R(-0.5) * rho * rho);
+                    gf = erf_<R>(rho * R(0.70710678118654752440)) - c2pi * rho * e;
+                    gp = c2pi * rho * rho * e;
+                    sri = gp / (sig * r);
+                }
+            }
+        }
+        const R d3 = ri * ri2;
+        const R cx = dz * gy - dy * gz, cy = dx * gz - dz * gx, cz = dy * gx - dx * gy;
+        if (GS) { a[1] += gf * cx * d3; a[2] += gf * cy * d3; a[3] += gf * cz * d3; }
+        if (HS) {
+            const R d5 = d3 * ri2, t3 = R(-3) * d5;
+            const R ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
+            const R g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
+            R h[9] = {ax * cx,       ax * cy + g3z, ax * cz - g3y,
+                      ay * cx - g3z, ay * cy,       ay * cz + g3x,
+                      az * cx + g3y, az * cy - g3x, az * cz};
+            if (K == FMMHIP_VORTEX_GAUSSIAN) {
+                const R c[3] = {cx * d3, cy * d3, cz * d3}, d[3] = {dx, dy, dz};
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) h[3 * i + j] = gf * h[3 * i + j] + sri * d[i] * c[j];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a[4 + k] += h[k];
+        }
+    }
+}
+
+// stages source j of the sorted arrays (strength SoA [dim][ns], radius) into slot values
+template <int K, class R>
+__device__ inline void stage_source(const double* __restrict__ sq, const double* __restrict__ srad, int64_t ns, int64_t j, R* g) {
+    g[0] = (R)(sq[j] * INV4PI);
+    if (KernelTraits<K>::NS >= 3) { g[1] = (R)(sq[ns + j] * INV4PI); g[2] = (R)(sq[2 * ns + j] * INV4PI); }
+    if (K == FMMHIP_VORTEX_GAUSSIAN) g[3] = (R)srad[j];
 }
 
 // ---------------------------------------------------------------- FP64 form
@@ -50,8 +139,10 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                                                        const double* __restrict__ ty, const double* __restrict__ tz,
                                                        const double* __restrict__ sx, const double* __restrict__ sy,
                                                        const double* __restrict__ sz, const double* __restrict__ sq,
-                                                       int64_t ns, double* __restrict__ tout, int64_t nt) {
-    __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[3][P2P_BLOCK];
+                                                       const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
+                                                       int64_t nt) {
+    constexpr int NS = KernelTraits<KERNEL>::NS;
+    __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
     const int tid = threadIdx.x;
@@ -64,9 +155,9 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
         const bool active = ti < T.n_bodies;
         const int64_t it = T.first + (active ? ti : 0);
         const double xt = tx[it], yt = ty[it], zt = tz[it];
-        Acc<double> a;
+        double a[13];
 #pragma unroll
-        for (int k = 0; k < 13; ++k) a.v[k] = 0.0;
+        for (int k = 0; k < 13; ++k) a[k] = 0.0;
         for (int64_t li = l0; li < l1; ++li) {
             const Branch S = sbr[dir_s[li]];
             for (int64_t s0 = 0; s0 < S.n_bodies; s0 += P2P_BLOCK) {
@@ -75,48 +166,26 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                 if (tid < cnt) {
                     const int64_t j = S.first + s0 + tid;
                     lx[tid] = sx[j]; ly[tid] = sy[j]; lz[tid] = sz[j];
-                    if (KERNEL == FMMHIP_POINT_SOURCE) lg[0][tid] = sq[j] * INV4PI;
-                    else { lg[0][tid] = sq[j] * INV4PI; lg[1][tid] = sq[ns + j] * INV4PI; lg[2][tid] = sq[2 * ns + j] * INV4PI; }
+                    stage_source<KERNEL, double>(sq, srad, ns, j, lg[tid]);
                 }
                 __syncthreads();
                 if (active)
                     for (int s = lane_k; s < cnt; s += tpt) {
                         const double dx = xt - lx[s], dy = yt - ly[s], dz = zt - lz[s];
                         const double r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 > 0.0) {
-                            const double ri = rinv_f64(r2);
-                            const double ri2 = ri * ri;
-                            if (KERNEL == FMMHIP_POINT_SOURCE) {
-                                const double q = lg[0][s];
-                                if (PS) a.v[0] += q * ri;
-                                if (GS) { const double f = q * ri * ri2; a.v[1] += dx * f; a.v[2] += dy * f; a.v[3] += dz * f; }
-                            } else {
-                                const double gx = lg[0][s], gy = lg[1][s], gz = lg[2][s];
-                                const double d3 = ri * ri2;
-                                const double cx = dz * gy - dy * gz, cy = dx * gz - dz * gx, cz = dy * gx - dx * gy;
-                                if (GS) { a.v[1] += cx * d3; a.v[2] += cy * d3; a.v[3] += cz * d3; }
-                                if (HS) {
-                                    const double d5 = d3 * ri2, t3 = -3.0 * d5;
-                                    const double ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
-                                    const double g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
-                                    a.v[4] += ax * cx;         a.v[5] += ax * cy + g3z;   a.v[6] += ax * cz - g3y;
-                                    a.v[7] += ay * cx - g3z;   a.v[8] += ay * cy;         a.v[9] += ay * cz + g3x;
-                                    a.v[10] += az * cx + g3y;  a.v[11] += az * cy - g3x;  a.v[12] += az * cz;
-                                }
-                            }
-                        }
+                        if (r2 > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dx, dy, dz, r2, rinv_f64(r2), lg[s], a);
                     }
             }
         }
         // fold the tpt partial sums of a target (adjacent lanes)
         for (int w = 1; w < tpt; w <<= 1)
 #pragma unroll
-            for (int k = 0; k < 13; ++k) a.v[k] += __shfl_xor(a.v[k], w, 64);
+            for (int k = 0; k < 13; ++k) a[k] += __shfl_xor(a[k], w, 64);
         if (active && lane_k == 0) {
-            if (PS) tout[it] += a.v[0];
-            if (GS) { tout[nt + it] += a.v[1]; tout[2 * nt + it] += a.v[2]; tout[3 * nt + it] += a.v[3]; }
-            if (HS && KERNEL == FMMHIP_POINT_VORTEX)
-                for (int k = 4; k < 13; ++k) tout[(int64_t)k * nt + it] += a.v[k];
+            if (PS && KernelTraits<KERNEL>::POT) tout[it] += a[0];
+            if (GS) { tout[nt + it] += a[1]; tout[2 * nt + it] += a[2]; tout[3 * nt + it] += a[3]; }
+            if (HS && KernelTraits<KERNEL>::HESS)
+                for (int k = 4; k < 13; ++k) tout[(int64_t)k * nt + it] += a[k];
         }
     }
 }
@@ -130,8 +199,10 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                                                        const double* __restrict__ ty, const double* __restrict__ tz,
                                                        const double* __restrict__ sx, const double* __restrict__ sy,
                                                        const double* __restrict__ sz, const double* __restrict__ sq,
-                                                       int64_t ns, double* __restrict__ tout, int64_t nt) {
-    __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[3][P2P_BLOCK];
+                                                       const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
+                                                       int64_t nt) {
+    constexpr int NS = KernelTraits<KERNEL>::NS;
+    __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
     const int tid = threadIdx.x;
@@ -162,45 +233,19 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                     const float hx = (float)px, hy = (float)py, hz = (float)pz;
                     lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
                     ll[0][tid] = (float)(px - (double)hx); ll[1][tid] = (float)(py - (double)hy); ll[2][tid] = (float)(pz - (double)hz);
-                    if (KERNEL == FMMHIP_POINT_SOURCE) lg[0][tid] = (float)(sq[j] * INV4PI);
-                    else {
-                        lg[0][tid] = (float)(sq[j] * INV4PI);
-                        lg[1][tid] = (float)(sq[ns + j] * INV4PI);
-                        lg[2][tid] = (float)(sq[2 * ns + j] * INV4PI);
-                    }
+                    stage_source<KERNEL, float>(sq, srad, ns, j, lg[tid]);
                 }
                 __syncthreads();
                 if (active) {
-                    float part[NA];
+                    float part[13];
 #pragma unroll
-                    for (int k = 0; k < NA; ++k) part[k] = 0.f;
+                    for (int k = 0; k < 13; ++k) part[k] = 0.f;
                     for (int s = lane_k; s < cnt; s += tpt) {
                         const float dx = (xh - lh[0][s]) + (xl - ll[0][s]);
                         const float dy = (yh - lh[1][s]) + (yl - ll[1][s]);
                         const float dz = (zh - lh[2][s]) + (zl - ll[2][s]);
                         const float r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 > 0.f) {
-                            const float ri = __builtin_amdgcn_rsqf(r2);
-                            const float ri2 = ri * ri;
-                            if (KERNEL == FMMHIP_POINT_SOURCE) {
-                                const float q = lg[0][s];
-                                if (PS) part[0] += q * ri;
-                                if (GS) { const float f = q * ri * ri2; part[1] += dx * f; part[2] += dy * f; part[3] += dz * f; }
-                            } else {
-                                const float gx = lg[0][s], gy = lg[1][s], gz = lg[2][s];
-                                const float d3 = ri * ri2;
-                                const float cx = dz * gy - dy * gz, cy = dx * gz - dz * gx, cz = dy * gx - dx * gy;
-                                if (GS) { part[1] += cx * d3; part[2] += cy * d3; part[3] += cz * d3; }
-                                if (HS) {
-                                    const float d5 = d3 * ri2, t3 = -3.f * d5;
-                                    const float ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
-                                    const float g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
-                                    part[4] += ax * cx;         part[5] += ax * cy + g3z;   part[6] += ax * cz - g3y;
-                                    part[7] += ay * cx - g3z;   part[8] += ay * cy;         part[9] += ay * cz + g3x;
-                                    part[10] += az * cx + g3y;  part[11] += az * cy - g3x;  part[12] += az * cz;
-                                }
-                            }
-                        }
+                        if (r2 > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dx, dy, dz, r2, __builtin_amdgcn_rsqf(r2), lg[s], part);
                     }
 #pragma unroll
                     for (int k = 0; k < NA; ++k) kahan_add(sum[k], cmp[k], part[k]);
@@ -214,9 +259,9 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
 #pragma unroll
             for (int k = 0; k < NA; ++k) acc[k] += __shfl_xor(acc[k], w, 64);
         if (active && lane_k == 0) {
-            if (PS && KERNEL == FMMHIP_POINT_SOURCE) tout[it] += acc[0];
+            if (PS && KernelTraits<KERNEL>::POT) tout[it] += acc[0];
             if (GS) { tout[nt + it] += acc[1]; tout[2 * nt + it] += acc[2]; tout[3 * nt + it] += acc[3]; }
-            if (HS && KERNEL == FMMHIP_POINT_VORTEX)
+            if (HS && KernelTraits<KERNEL>::HESS)
                 for (int k = 4; k < NA; ++k) tout[(int64_t)k * nt + it] += acc[k];
         }
     }
@@ -229,7 +274,8 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
     if (nblk == 0) return;
 #define P2P_ARGS                                                                                                         \
     L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), tt.branches.data(), st.branches.data(), tt.x.data(),    \
-        tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.n, tout, nt
+        tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.radius.data(), st.n,    \
+        tout, nt
 #define P2P_DISPATCH(KFN)                                                                                               \
     do {                                                                                                                \
         int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);                                                          \
@@ -253,8 +299,12 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
 
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
              bool HS, int precision) {
-    if (kernel == FMMHIP_POINT_SOURCE) launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision);
-    else launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision);
+    switch (kernel) {
+        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
+        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
+        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
+        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
+    }
 }
 
 // ------------------------------------------------------ direct!, all pairs
@@ -263,49 +313,33 @@ template <int KERNEL, typename R>
 __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restrict__ txyz, int64_t nt,
                                                           const double* __restrict__ sbuf, int rows, int64_t ns,
                                                           double* __restrict__ out, int PS, int GS, int HS) {
-    __shared__ R lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[3][P2P_BLOCK];
+    constexpr int NS = KernelTraits<KERNEL>::NS;
+    __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK];
+    __shared__ R lg[P2P_BLOCK][NS];
     const int64_t it = blockIdx.x * (int64_t)P2P_BLOCK + threadIdx.x;
     const bool active = it < nt;
     const double xt = active ? txyz[3 * it] : 0.0, yt = active ? txyz[3 * it + 1] : 0.0, zt = active ? txyz[3 * it + 2] : 0.0;
-    double a[13];
-    for (int k = 0; k < 13; ++k) a[k] = 0.0;
+    R a[13];
+    for (int k = 0; k < 13; ++k) a[k] = R(0);
     for (int64_t s0 = 0; s0 < ns; s0 += P2P_BLOCK) {
         const int64_t cnt = std::min<int64_t>(P2P_BLOCK, ns - s0);
         __syncthreads();
         if ((int64_t)threadIdx.x < cnt) {
             const double* b = sbuf + (s0 + threadIdx.x) * rows;
-            lx[threadIdx.x] = (R)b[0]; ly[threadIdx.x] = (R)b[1]; lz[threadIdx.x] = (R)b[2];
-            lg[0][threadIdx.x] = (R)(b[4] * INV4PI);
-            if (KERNEL == FMMHIP_POINT_VORTEX) { lg[1][threadIdx.x] = (R)(b[5] * INV4PI); lg[2][threadIdx.x] = (R)(b[6] * INV4PI); }
+            lx[threadIdx.x] = b[0]; ly[threadIdx.x] = b[1]; lz[threadIdx.x] = b[2];
+            lg[threadIdx.x][0] = (R)(b[4] * INV4PI);
+            if (NS >= 3) { lg[threadIdx.x][1] = (R)(b[5] * INV4PI); lg[threadIdx.x][2] = (R)(b[6] * INV4PI); }
+            if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) lg[threadIdx.x][3] = (R)b[3];
         }
         __syncthreads();
         if (!active) continue;
         for (int s = 0; s < cnt; ++s) {
-            const R dx = (R)(xt - (double)lx[s]), dy = (R)(yt - (double)ly[s]), dz = (R)(zt - (double)lz[s]);
+            const R dx = (R)(xt - lx[s]), dy = (R)(yt - ly[s]), dz = (R)(zt - lz[s]);
             const R r2 = dx * dx + dy * dy + dz * dz;
             if (!(r2 > R(0))) continue;
             R ri;
             if constexpr (sizeof(R) == 8) ri = rinv_f64(r2); else ri = __builtin_amdgcn_rsqf(r2);
-            const R ri2 = ri * ri;
-            if (KERNEL == FMMHIP_POINT_SOURCE) {
-                const R q = lg[0][s];
-                a[0] += q * ri;
-                const R f = q * ri * ri2;
-                a[1] += dx * f; a[2] += dy * f; a[3] += dz * f;
-            } else {
-                const R gx = lg[0][s], gy = lg[1][s], gz = lg[2][s];
-                const R d3 = ri * ri2;
-                const R cx = dz * gy - dy * gz, cy = dx * gz - dz * gx, cz = dy * gx - dx * gy;
-                a[1] += cx * d3; a[2] += cy * d3; a[3] += cz * d3;
-                if (HS) {
-                    const R d5 = d3 * ri2, t3 = R(-3) * d5;
-                    const R ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
-                    const R g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
-                    a[4] += ax * cx;         a[5] += ax * cy + g3z;   a[6] += ax * cz - g3y;
-                    a[7] += ay * cx - g3z;   a[8] += ay * cy;         a[9] += ay * cz + g3x;
-                    a[10] += az * cx + g3y;  a[11] += az * cy - g3x;  a[12] += az * cz;
-                }
-            }
+            pair_eval<KERNEL, true, true, true, R>(dx, dy, dz, r2, ri, lg[s], a);
         }
     }
     if (!active) return;
@@ -314,25 +348,33 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restri
         o[3] += a[0];          // the reference kernel ignores the switches (test/gravitational.jl:92-98)
         o[4] += a[1]; o[5] += a[2]; o[6] += a[3];
     } else {
+        if (PS && KernelTraits<KERNEL>::POT) o[3] += a[0];
         if (GS) { o[4] += a[1]; o[5] += a[2]; o[6] += a[3]; }
         if (HS) for (int k = 4; k < 13; ++k) o[3 + k] += a[k];
     }
-    (void)PS;
+}
+
+template <int K>
+static void launch_direct_all(Context& C, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf, double* tout_aos,
+                              bool PS, bool GS, bool HS, int precision) {
+    const unsigned grid = (unsigned)((nt + P2P_BLOCK - 1) / P2P_BLOCK);
+    const int rows = data_per_body(K);
+    if (precision == FMMHIP_FP64)
+        k_direct_all<K, double><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
+    else
+        k_direct_all<K, float><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
+    LAUNCH_CHECK();
 }
 
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf, double* tout_aos,
                     bool PS, bool GS, bool HS, int precision) {
     if (nt == 0 || ns == 0) return;
-    const unsigned grid = (unsigned)((nt + P2P_BLOCK - 1) / P2P_BLOCK);
-    const int rows = data_per_body(kernel);
-    if (kernel == FMMHIP_POINT_SOURCE) {
-        if (precision == FMMHIP_FP64) k_direct_all<FMMHIP_POINT_SOURCE, double><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
-        else k_direct_all<FMMHIP_POINT_SOURCE, float><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
-    } else {
-        if (precision == FMMHIP_FP64) k_direct_all<FMMHIP_POINT_VORTEX, double><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
-        else k_direct_all<FMMHIP_POINT_VORTEX, float><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
+    switch (kernel) {
+        case FMMHIP_POINT_SOURCE: launch_direct_all<FMMHIP_POINT_SOURCE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
+        case FMMHIP_POINT_VORTEX: launch_direct_all<FMMHIP_POINT_VORTEX>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
+        case FMMHIP_POINT_DIPOLE: launch_direct_all<FMMHIP_POINT_DIPOLE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
+        default: launch_direct_all<FMMHIP_VORTEX_GAUSSIAN>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
     }
-    LAUNCH_CHECK();
 }
 
 }  // namespace fmmhip

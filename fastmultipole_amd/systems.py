@@ -3,7 +3,10 @@
 Gravitational follows test/gravitational.jl (Point{Source}, data_per_body 5,
 buffer rows [x, y, z, radius, strength]); VortexParticles follows
 test/vortex.jl (Point{Vortex}, data_per_body 7, rows [x, y, z, sigma,
-Gamma_x, Gamma_y, Gamma_z]). Each implements the reference's user-system
+Gamma_x, Gamma_y, Gamma_z]); DipolePoints follows the DipolePoints system of
+test/bodytomultipole_test.jl:122-168 (Point{Dipole}, rows [x, y, z, 0, q]);
+GaussianVortexParticles is VortexParticles with the Gaussian-core nearfield
+kernel (sigma = row 4). Each implements the reference's user-system
 interface (src/compatibility.jl): source_system_to_buffer!, data_per_body,
 get_position, strength_dims, get_n_bodies, has_vector_potential,
 buffer_to_target_system! and the body_to_multipole! kernel tag (fmm_kernel).
@@ -105,3 +108,59 @@ class VortexParticles:
     def reset(self):
         self.potential[:] = 0.0
         self.gradient_stretching[:] = 0.0
+
+
+class DipolePoints:
+    """test/bodytomultipole_test.jl:122-168: point dipoles of vector strength q (Point{Dipole})."""
+
+    fmm_kernel = L.POINT_DIPOLE
+
+    def __init__(self, position, strength):
+        self.position = np.ascontiguousarray(position, dtype=np.float64)
+        self.strength = np.ascontiguousarray(strength, dtype=np.float64)
+        n = self.position.shape[0]
+        self.potential = np.zeros(n)
+        self.gradient = np.zeros((n, 3))
+        self.hessian = np.zeros((n, 9))
+
+    def get_n_bodies(self):
+        return self.position.shape[0]
+
+    def data_per_body(self):
+        return 7
+
+    def strength_dims(self):
+        return 3
+
+    def has_vector_potential(self):
+        return False
+
+    def get_position(self, i):
+        return self.position[i]
+
+    def positions(self):
+        return self.position
+
+    def source_buffer(self):
+        return np.column_stack([self.position, np.zeros(self.get_n_bodies()), self.strength])
+
+    def buffer_to_target_system(self, target_buffer, switch):
+        n = self.get_n_bodies()
+        if switch.scalar_potential:
+            self.potential += target_buffer[:n, 3]
+        if switch.gradient:
+            self.gradient += target_buffer[:n, 4:7]
+        if switch.hessian:
+            self.hessian += target_buffer[:n, 7:16]
+
+    def reset(self):
+        self.potential[:] = 0.0
+        self.gradient[:] = 0.0
+        self.hessian[:] = 0.0
+
+
+class GaussianVortexParticles(VortexParticles):
+    """Vortex particles with Gaussian cores of width sigma: the nearfield uses the regularised
+    kernel, the expansions are those of Point{Vortex}."""
+
+    fmm_kernel = L.VORTEX_GAUSSIAN

@@ -73,3 +73,23 @@ def relative_error(v, v_ref):
     den = np.maximum(den, 1e-300 * max(den.max(), 1e-300))
     e = num / den
     return float(e.max()), float(np.percentile(e, 99)), float(np.median(e))
+
+
+def uniform_dipoles(n, seed=1):
+    """Point dipoles: positions U[0,1]^3, moments N(0,1)/n per component. (n, 7) buffer [x, y, z, 0, q]."""
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 7))
+    b[:, :3] = rng.random((n, 3))
+    b[:, 4:7] = rng.normal(size=(n, 3)) / n
+    return b
+
+
+def gaussian_vortex_ring(n, seed=2, radius=1.0, core_sigma=0.1, noise=1e-3, sigma=None):
+    """vortex_ring() with every particle given a Gaussian core of width `sigma` (buffer row 4);
+    the default sigma is 1.5x the mean particle spacing along the ring's core."""
+    b = vortex_ring(n, seed=seed, radius=radius, core_sigma=core_sigma, noise=noise)
+    if sigma is None:
+        vol = 2.0 * math.pi * radius * 2.0 * math.pi * core_sigma ** 2
+        sigma = 1.5 * (vol / n) ** (1.0 / 3.0)
+    b[:, 3] = sigma
+    return b

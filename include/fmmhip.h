@@ -7,14 +7,14 @@
  *   - fmm!(targets, target_tree, sources, source_tree, ..., m2l_list, direct_list, ...)
  *                                                     src/fmm.jl:914-1176
  *   - direct!(targets, sources; ...)                  src/direct.jl:20-75
- * for systems whose body_to_multipole! / direct! are the built-in point kernels
- * Point{Source} (test/gravitational.jl:74-104) and Point{Vortex} (test/vortex.jl:78-128).
+ * for systems whose body_to_multipole! / direct! are one of the built-in point kernels
+ * (fmmhip_kernel below).
  *
  * Data layout across the ABI follows the reference's buffers (src/compatibility.jl:5-20,
  * 155-172): column-major double matrices, i.e. one record per body.
  *   target positions : 3 x N   [x, y, z]
  *   source buffer    : data_per_body x N  [x, y, z, radius, strength...]
- *                      (5 rows for POINT_SOURCE, 7 rows for POINT_VORTEX)
+ *                      (5 rows for POINT_SOURCE, 7 rows for the other kernels)
  *   target results   : 16 x N  [x, y, z, phi, grad(3), hessian(9)] in the ORIGINAL body
  *                      order (the reference's target buffer after unsorting through
  *                      sort_index, src/compatibility.jl:324-338). Hessian entries are in
@@ -43,7 +43,22 @@ typedef enum {
     FMMHIP_EDEPTH = 5    /* octree deeper than 42 levels (coincident bodies) */
 } fmmhip_status;
 
-typedef enum { FMMHIP_POINT_SOURCE = 0, FMMHIP_POINT_VORTEX = 1 } fmmhip_kernel;
+/* Built-in kernels (the system's body_to_multipole! / direct! tag):
+ *   POINT_SOURCE     Point{Source}, test/gravitational.jl:80-104; rows [x,y,z,radius,q]
+ *   POINT_VORTEX     Point{Vortex}, test/vortex.jl:83-128;         rows [x,y,z,sigma,Gx,Gy,Gz]
+ *   POINT_DIPOLE     Point{Dipole} expansion (src/bodytomultipole.jl:431-463) with the direct
+ *                    kernel phi = q.d/(4 pi r^3);                   rows [x,y,z,radius,qx,qy,qz]
+ *   VORTEX_GAUSSIAN  vortex particle with a Gaussian core of width sigma (row 4): the
+ *                    singular kernel times erf(rho/sqrt2) - sqrt(2/pi) rho exp(-rho^2/2),
+ *                    rho = r/sigma; expansions as POINT_VORTEX;     rows [x,y,z,sigma,Gx,Gy,Gz]
+ * Results use the expansion evaluation's convention: slot "gradient" = -grad(phi) for the
+ * scalar kernels, the velocity for the vortex kernels; slot "hessian" = d(gradient slot). */
+typedef enum {
+    FMMHIP_POINT_SOURCE = 0,
+    FMMHIP_POINT_VORTEX = 1,
+    FMMHIP_POINT_DIPOLE = 2,
+    FMMHIP_VORTEX_GAUSSIAN = 3
+} fmmhip_kernel;
 typedef enum { FMMHIP_FP64 = 0, FMMHIP_MIXED = 1 } fmmhip_precision;
 typedef enum { FMMHIP_BARBA = 0, FMMHIP_SELFTUNING = 1, FMMHIP_SELFTUNING_TREESTOP = 2 } fmmhip_list_method;
 typedef enum {

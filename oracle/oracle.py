@@ -19,6 +19,8 @@ _LIB = None
 
 KIND_SOURCE = 0   # Point{Source}, test/gravitational.jl
 KIND_VORTEX = 1   # Point{Vortex}, test/vortex.jl
+KIND_DIPOLE = 2   # Point{Dipole}: P2M src/bodytomultipole.jl:431-463, direct kernel stated in oracle_fmm.cpp
+KIND_VORTEX_GAUSS = 3   # Gaussian-core vortex (sigma = buffer row 4), stated in oracle_fmm.cpp
 METHOD_BARBA, METHOD_SELFTUNING, METHOD_SELFTUNING_TREESTOP = 0, 1, 2
 ERR_NONE, ERR_ROTATED_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
 

@@ -24,7 +24,10 @@ struct OBranch {
 };
 
 enum { METHOD_BARBA = 0, METHOD_SELFTUNING = 1, METHOD_SELFTUNING_TREESTOP = 2 };
-enum { KIND_SOURCE = 0, KIND_VORTEX = 1 };
+// kernel kinds: point source, singular point vortex (the reference's test kernels),
+// point dipole and Gaussian-core vortex (kernels of this library with no
+// reference direct! counterpart; their formulas are stated at the functions)
+enum { KIND_SOURCE = 0, KIND_VORTEX = 1, KIND_DIPOLE = 2, KIND_VORTEX_GAUSS = 3 };
 
 struct OTree {
     std::vector<OBranch> branches;
@@ -411,14 +414,97 @@ static void direct_vortex(double* tbuf, int64_t t0, int64_t nt, const double* sb
     }
 }
 
+// Point dipole of moment q at x_s. The potential is the one the reference's
+// dipole multipole expansion represents (body_to_multipole_point!(Point{Dipole}),
+// src/bodytomultipole.jl:431-463; checked analytically at
+// test/bodytomultipole_test.jl:508-514): phi = q.d / (4 pi r^3), d = x_t - x_s.
+// Outputs follow the expansion evaluation's convention (src/evaluate_expansions.jl):
+// slot "gradient" = -grad phi, slot "hessian" = -grad grad phi.
+static void direct_dipole(double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns, bool PS,
+                          bool GS, bool HS) {
+    for (int64_t js = s0; js < s0 + ns; ++js) {
+        const double* s = sbuf + (size_t)js * sr;
+        const double qx = s[4], qy = s[5], qz = s[6];
+        for (int64_t it = t0; it < t0 + nt; ++it) {
+            double* t = tbuf + (size_t)it * 16;
+            const double d[3] = {t[0] - s[0], t[1] - s[1], t[2] - s[2]};
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(r2 > 0)) continue;
+            const double r = std::sqrt(r2), ri = 1 / r, ri2 = ri * ri, ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+            const double qd = qx * d[0] + qy * d[1] + qz * d[2];
+            const double q[3] = {qx, qy, qz};
+            if (PS) t[3] += qd * ri3 * ONE_OVER_4PI;
+            if (GS)
+                for (int a = 0; a < 3; ++a) t[4 + a] += (3 * qd * d[a] * ri5 - q[a] * ri3) * ONE_OVER_4PI;
+            if (HS)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        t[7 + 3 * a + b] += (3 * (q[b] * d[a] + q[a] * d[b] + (a == b ? qd : 0.0)) * ri5 -
+                                             15 * qd * d[a] * d[b] * ri7) * ONE_OVER_4PI;
+        }
+    }
+}
+
+// Gaussian-core vortex (vortex particle with a Gaussian vorticity blob of
+// width sigma = buffer row 4): the singular vortex kernel of test/vortex.jl
+// scaled by the fraction of the blob inside radius r,
+//   g(rho) = erf(rho/sqrt 2) - sqrt(2/pi) rho exp(-rho^2/2),  rho = r/sigma
+// (the chi distribution's CDF with 3 degrees of freedom), and
+//   grad v = g grad v_singular + g'(rho) (d / (sigma r)) (x) v_singular,
+//   g'(rho) = sqrt(2/pi) rho^2 exp(-rho^2/2).
+// sigma = 0 gives the singular kernel.
+static void direct_vortex_gauss(double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
+                                bool V, bool VG) {
+    const double c2pi = std::sqrt(2.0 / 3.141592653589793);
+    for (int64_t js = s0; js < s0 + ns; ++js) {
+        const double* s = sbuf + (size_t)js * sr;
+        const double sig = s[3], Gx = s[4], Gy = s[5], Gz = s[6];
+        for (int64_t it = t0; it < t0 + nt; ++it) {
+            double* t = tbuf + (size_t)it * 16;
+            const double dx = t[0] - s[0], dy = t[1] - s[1], dz = t[2] - s[2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 > 0)) continue;
+            const double r = std::sqrt(r2), rinv = 1 / r, rinv2 = rinv * rinv;
+            double g = 1.0, gp = 0.0;
+            if (sig > 0) {
+                const double rho = r / sig, e = std::exp(-0.5 * rho * rho);
+                g = std::erf(rho / std::sqrt(2.0)) - c2pi * rho * e;
+                gp = c2pi * rho * rho * e;
+            }
+            const double d3 = ONE_OVER_4PI * rinv * rinv2;
+            const double c[3] = {dz * Gy - dy * Gz, dx * Gz - dz * Gx, dy * Gx - dx * Gy};
+            if (V)
+                for (int a = 0; a < 3; ++a) t[4 + a] += g * c[a] * d3;
+            if (VG) {
+                const double d5 = d3 * rinv2, dd[3] = {dx, dy, dz};
+                double h[9];
+                h[0] = -3 * dx * c[0] * d5;
+                h[1] = (-3 * dx * c[1] + Gz * r2) * d5;
+                h[2] = (-3 * dx * c[2] - Gy * r2) * d5;
+                h[3] = (-3 * dy * c[0] - Gz * r2) * d5;
+                h[4] = -3 * dy * c[1] * d5;
+                h[5] = (-3 * dy * c[2] + Gx * r2) * d5;
+                h[6] = (-3 * dz * c[0] + Gy * r2) * d5;
+                h[7] = (-3 * dz * c[1] - Gx * r2) * d5;
+                h[8] = -3 * dz * c[2] * d5;
+                const double f = sig > 0 ? gp / (sig * r) : 0.0;
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) t[7 + 3 * a + b] += g * h[3 * a + b] + f * dd[a] * c[b] * d3;
+            }
+        }
+    }
+}
+
 static void user_direct(int kind, double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
                         bool PS, bool GS, bool HS) {
-    (void)PS;
     if (kind == KIND_SOURCE) direct_gravitational(tbuf, t0, nt, sbuf, sr, s0, ns);
-    else direct_vortex(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
+    else if (kind == KIND_VORTEX) direct_vortex(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
+    else if (kind == KIND_DIPOLE) direct_dipole(tbuf, t0, nt, sbuf, sr, s0, ns, PS, GS, HS);
+    else direct_vortex_gauss(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
 }
 
 static int data_per_body(int kind) { return kind == KIND_SOURCE ? 5 : 7; }
+static bool has_vector_potential(int kind) { return kind == KIND_VORTEX || kind == KIND_VORTEX_GAUSS; }
 
 // ------------------------------------------------------------ fmm driver
 struct Options {
@@ -487,7 +573,7 @@ static void estimate_influence(Run& R) {
 
 static void fmm_run(Run& R, bool estimate_pass) {
     const Options& o = R.opt;
-    const bool LH = (R.kind == KIND_VORTEX);   // has_vector_potential
+    const bool LH = has_vector_potential(R.kind);
     const int dpb = data_per_body(R.kind);
     build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p);
     build_tree(R.st, R.ns, dpb, R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p);
@@ -522,6 +608,7 @@ static void fmm_run(Run& R, bool estimate_pass) {
             const double* x = &R.st.buffer[(size_t)i * dpb];
             double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
             if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
+            else if (R.kind == KIND_DIPOLE) p2m_point_dipole(M, Exp{harm.data()}, dx, x + 4, p);
             else p2m_point_vortex(M, Exp{harm.data()}, dx, x + 4, p);
         }
     }

@@ -118,6 +118,13 @@ FMM_CASES = [
     # order 20: two back-rotation chunks in the wavefront engine
     ("ring-3k-p20", lambda: W.vortex_ring(3000, seed=28), O.KIND_VORTEX,
      dict(expansion_order=20, leaf_size_source=40, multipole_acceptance=0.5, hessian=True)),
+    # point dipoles (P2M of src/bodytomultipole.jl:431-463) and Gaussian-core vortices
+    ("dipole-4k-p8", lambda: W.uniform_dipoles(4000, seed=29), O.KIND_DIPOLE,
+     dict(expansion_order=8, leaf_size_source=30, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("dipole-3k-p3", lambda: W.uniform_dipoles(3000, seed=30), O.KIND_DIPOLE,
+     dict(expansion_order=3, leaf_size_source=10, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("gauss-ring-6k-p10", lambda: W.gaussian_vortex_ring(6000, seed=31), O.KIND_VORTEX_GAUSS,
+     dict(expansion_order=10, leaf_size_source=64, multipole_acceptance=0.4, hessian=True)),
 ]
 
 
@@ -185,7 +192,9 @@ def test_error_controlled_order_matches_oracle(name, gen, kind, em, ls):
 
 def test_direct_fp64_matches_oracle():
     _say("[gpu] direct fp64")
-    for kind, sbuf in ((O.KIND_SOURCE, W.uniform_sources(3000, seed=41)), (O.KIND_VORTEX, W.vortex_ring(3000, seed=42))):
+    for kind, sbuf in ((O.KIND_SOURCE, W.uniform_sources(3000, seed=41)), (O.KIND_VORTEX, W.vortex_ring(3000, seed=42)),
+                       (O.KIND_DIPOLE, W.uniform_dipoles(3000, seed=43)),
+                       (O.KIND_VORTEX_GAUSS, W.gaussian_vortex_ring(3000, seed=44))):
         ctx = F.default_context()
         txyz = np.ascontiguousarray(sbuf[:, :3])
         out = np.zeros((len(sbuf), 16))
@@ -225,6 +234,25 @@ def test_fmm_api_mirror_and_edge_cases():
     ev, opt = _gpu_eval(O.KIND_SOURCE, W.uniform_sources(100, seed=6), expansion_order=25)
     with pytest.raises(_lib.FmmHipError):
         ev.fmm(opt)
+
+
+@pytest.mark.parametrize("kind,gen", [(O.KIND_DIPOLE, lambda: W.uniform_dipoles(20000, seed=51)),
+                                      (O.KIND_VORTEX_GAUSS, lambda: W.gaussian_vortex_ring(20000, seed=52))],
+                         ids=["dipole", "gauss-ring"])
+def test_mixed_precision_new_kernels(kind, gen):
+    """Mixed form of the dipole and Gaussian-core kernels against the FP64 form of the same
+    evaluation: the FP32 nearfield's error stays below 1e-4 (max) and 1e-6 (median) per target."""
+    _say(f"[gpu] mixed {kind}")
+    sbuf = gen()
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ev, opt = _gpu_eval(kind, sbuf, expansion_order=8, leaf_size_source=32, multipole_acceptance=0.4, hessian=True,
+                            precision=prec)
+        ev.fmm(opt)
+        res[prec] = ev.targets()
+    e = W.relative_error(res["mixed"][:, 4:7], res["fp64"][:, 4:7])
+    _say(f"    mixed vs fp64: max {e[0]:.3e} p99 {e[1]:.3e} median {e[2]:.3e}")
+    assert e[0] < 1e-4 and e[2] < 1e-6, e
 
 
 def test_mixed_precision_ring_error():
