@@ -237,7 +237,7 @@ static TablePtrs ptrs(Context& C) {
 
 static constexpr int RBUF = 1024;      // doubles of per-wavefront item buffer (64 items x 16)
 static constexpr int MAXCH = 2;        // back-rotation chunks held in registers: orders up to 20
-static constexpr int FWD_DEGREES = 4;  // degrees per forward chunk of the error-controlled M2L
+static constexpr int FWD_DEGREES = 64; // degrees per forward chunk of the error-controlled M2L (64 items bound it)
 static constexpr int SMALL_P = 4;      // fixed orders up to this run one M2L per lane (k_m2l_lane)
 static constexpr size_t LDS_LIMIT = 160 * 1024;
 
@@ -284,11 +284,12 @@ struct WaveLds {
     double* cs;    // 2(P+1): cos(v beta), sin(v beta)
     double* fac;   // 6(P+1): running factors of the error tests per degree
     double* pw;    // 2(P+1): multipole power per degree (phi, chi)
+    double* F;     // 2(P+1): k!/t^(k+1), k = 0..2P (M2L z-translation factors)
 };
 
 __host__ __device__ inline int wave_lds_doubles(int P) {
     const int H4 = 4 * (((P + 1) * (P + 2)) >> 1);
-    return H4 + (H4 > RBUF ? H4 : RBUF) + 12 * (P + 1);
+    return H4 + (H4 > RBUF ? H4 : RBUF) + 14 * (P + 1);
 }
 static size_t translate_lds_bytes(int P, int nw) { return sizeof(double) * (size_t)(len_A(P) + nw * wave_lds_doubles(P)); }
 
@@ -319,6 +320,7 @@ __device__ inline WaveLds carve_wave(double* lds, int P) {
     W.cs = W.eim + 2 * (P + 1);
     W.fac = W.cs + 2 * (P + 1);
     W.pw = W.fac + 6 * (P + 1);
+    W.F = W.pw + 2 * (P + 1);
     return W;
 }
 
@@ -330,42 +332,54 @@ __device__ inline const double* load_tables(double* lds, const double* Ad, int P
     return lds;
 }
 
-__device__ inline void c2s(double x, double y, double z, double& r, double& th, double& ph) {   // src/rotate.jl:9-21
-    double x2y2 = x * x + y * y;
-    double r2 = x2y2 + z * z;
-    r = (r2 == 0.0) ? r2 : sqrt(r2);
-    double z_r = z / r;
-    if (r > 0) th = x2y2 > 0 ? acos(z_r) : (z < 0 ? PI_D : 0.0);
-    else th = 0.0;
-    ph = (x2y2 == 0.0) ? 0.0 : atan2(y, x);
-}
-
-// e^{i m phi} by repeated multiplication (src/rotate.jl:25-36) and cos/sin(v beta)
-// by the angle-addition recurrence of update_Ts! (:380-435); lane m / lane 32+v
-__device__ inline void fill_angles(const WaveLds& W, double ph, double th, int P) {
+// Spherical angles of the translation vector (cartesian_to_spherical,
+// src/rotate.jl:5-21) enter only through e^{i phi} and (cos b, sin b); they are
+// taken from the Cartesian components directly (e^{i phi} = (x + i y)/rho,
+// cos b = z/r, sin b = rho/r), equal to the reference's acos/atan2 route up to
+// rounding. Lane k <= P then forms e^{i k phi} by repeated multiplication
+// (update_eimphis!, :25-36) and cos/sin(k b) by the angle-addition recurrence of
+// update_Ts! (:380-435). Returns r.
+__device__ inline double fill_angles(const WaveLds& W, double x, double y, double z, int P) {
+    const double x2y2 = x * x + y * y;
+    const double r = sqrt(x2y2 + z * z);
+    double cp = 1.0, sp = 0.0, cb = z < 0.0 ? -1.0 : 1.0, sb = 0.0;
+    if (x2y2 > 0.0) {
+        const double rho = sqrt(x2y2);
+        cp = x / rho; sp = y / rho;
+        cb = z / r; sb = rho / r;
+    }
     const int l = lane_id();
     if (l <= P) {
-        const double s = sin(ph), c = cos(ph);
-        double er = 1.0, ei = 0.0;
+        double er = 1.0, ei = 0.0, c = 1.0, s = 0.0;
         for (int k = 0; k < l; ++k) {
-            double t = er;
-            er = er * c - ei * s;
-            ei = t * s + ei * c;
-        }
-        W.eim[2 * l] = er;
-        W.eim[2 * l + 1] = ei;
-    } else if (l >= 32 && l - 32 <= P) {
-        const int v = l - 32;
-        const double sb = sin(th), cb = cos(th);
-        double c = 1.0, s = 0.0;
-        for (int k = 0; k < v; ++k) {
-            double cn = cb * c - sb * s;
-            double sn = sb * c + cb * s;
+            const double t = er;
+            er = er * cp - ei * sp;
+            ei = t * sp + ei * cp;
+            const double cn = cb * c - sb * s;
+            const double sn = sb * c + cb * s;
             c = cn;
             s = sn;
         }
-        W.cs[2 * v] = c;
-        W.cs[2 * v + 1] = s;
+        W.eim[2 * l] = er;
+        W.eim[2 * l + 1] = ei;
+        W.cs[2 * l] = c;
+        W.cs[2 * l + 1] = s;
+    }
+    return r;
+}
+
+// F[k] = k!/t^(k+1) for k = 0..2P by the product chain of
+// translate_multipole_to_local_z! (src/translate.jl:181-232); lane k
+__device__ inline void fill_m2l_factors(const WaveLds& W, double t, int P) {
+    const int k = lane_id();
+    if (k <= 2 * P) {
+        const double ot = 1.0 / t;
+        double f = ot, dk = 1.0;
+        for (int j = 0; j < k; ++j) {
+            f *= dk * ot;
+            dk += 1.0;
+        }
+        W.F[k] = f;
     }
 }
 
@@ -595,7 +609,7 @@ enum { OP_M2M = 0, OP_M2L = 1, OP_L2L = 2 };
 
 // z-translation output (n, m) (src/translate.jl:6-50, 181-232, 1851-1891)
 template <int OP, bool LH>
-__device__ inline void translate_one(const double* in, double t, int P, int n, int m, double out[4]) {
+__device__ inline void translate_one(const double* in, const double* F, double t, int P, int n, int m, double out[4]) {
     double v1r = 0, v1i = 0, v2r = 0, v2i = 0;
     if (OP == OP_M2M) {
         const double _t = -t;
@@ -612,20 +626,12 @@ __device__ inline void translate_one(const double* in, double t, int P, int n, i
             iw -= np;
         }
     } else if (OP == OP_M2L) {
-        const double one_over_t = 1.0 / t;
-        double f = one_over_t;   // (n+np)!/t^(n+np+1) by the reference's product chain
-        double dk = 1.0;
-        for (int k = 0; k < n + m; ++k) {
-            f *= dk * one_over_t;
-            dk += 1.0;
-        }
         int iw = nh(m - 1) + m;
-        for (int np = m; np <= P; ++np) {
+        for (int np = m; np <= P; ++np) {   // L_n^m += (n+np)!/t^(n+np+1) M_np^m
+            const double f = F[n + np];
             const double2 a = *(const double2*)(in + 4 * iw);
             v1r = fma(f, a.x, v1r); v1i = fma(f, a.y, v1i);
             if (LH) { const double2 b = *(const double2*)(in + 4 * iw + 2); v2r = fma(f, b.x, v2r); v2i = fma(f, b.y, v2i); }
-            f *= dk * one_over_t;
-            dk += 1.0;
             iw += np + 1;
         }
     } else {
@@ -685,7 +691,7 @@ __device__ inline void translate_and_transform(const WaveLds& W, double r, int P
         int n, m;
         idx_nm(i, n, m);
         double o[4];
-        translate_one<OP, LH>(W.X, r, Pu, n, m, o);
+        translate_one<OP, LH>(W.X, W.F, r, Pu, n, m, o);
         *(double2*)(W.R + 4 * i) = make_double2(o[0], o[1]);
         if (LH) *(double2*)(W.R + 4 * i + 2) = make_double2(o[2], o[3]);
     }
@@ -711,9 +717,8 @@ __device__ inline void translate_and_transform(const WaveLds& W, double r, int P
 template <int OP, bool LH>
 __device__ void translate_fixed(const double* __restrict__ src, double dx, double dy, double dz, const WaveLds& W,
                                 const double* A, int P, const TablePtrs& tb, double (&acc)[MAXCH][8]) {
-    double r, th, ph;
-    c2s(dx, dy, dz, r, th, ph);
-    fill_angles(W, ph, th, P);
+    const double r = fill_angles(W, dx, dy, dz, P);
+    if (OP == OP_M2L) fill_m2l_factors(W, r, P);
     wave_sync();
     const double* fwd_mags = (OP == OP_L2L) ? tb.eta : tb.zeta;
     const double* back_mags = (OP == OP_M2M) ? tb.zeta : tb.eta;
@@ -991,9 +996,8 @@ __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const B
                         double (&acc)[MAXCH][8]) {
     const double dx = T.target_center[0] - S.source_center[0], dy = T.target_center[1] - S.source_center[1],
                  dz = T.target_center[2] - S.source_center[2];
-    double r, th, ph;
-    c2s(dx, dy, dz, r, th, ph);
-    fill_angles(W, ph, th, P);
+    const double r = fill_angles(W, dx, dy, dz, P);
+    fill_m2l_factors(W, r, P);
     int Pu = P;
     success = 1;
     if (E.kind == 0) {

@@ -90,6 +90,7 @@ struct Lists {
     int64_t n_direct_targets = 0;
     DBuf<int32_t> dir_targets;                       // unique target branches in the direct list
     DBuf<int64_t> dir_offsets;                       // n_direct_targets + 1
+    DBuf<int64_t> dir_cum;                           // n_direct + 1: exclusive prefix of source leaf sizes
     int64_t n_m2l_targets = 0;
     DBuf<int32_t> m2l_targets;
     DBuf<int64_t> m2l_offsets;

@@ -158,6 +158,13 @@ __global__ __launch_bounds__(256) void k_count_pairs(const int32_t* __restrict__
     }
 }
 
+// source leaf size of every direct-list entry (input of the flattened source stream of the P2P)
+__global__ void k_dir_counts(const int32_t* __restrict__ s, int64_t n, const Branch* __restrict__ sb, int64_t* __restrict__ cnt) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = sb[s[i]].n_bodies;
+    else if (i == n) cnt[i] = 0;
+}
+
 __global__ void k_init_frontier(int32_t* ft, int32_t* fs, int32_t* fst, unsigned long long* counter) {
     ft[0] = 0; fs[0] = 0; fst[0] = ST_PENDING;
     *counter = 0;
@@ -256,6 +263,13 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
     extract_list(C, S, cur, n, ST_M2L, L.m2l_t, L.m2l_s, L.n_m2l, L.m2l_targets, L.m2l_offsets, L.n_m2l_targets);
     extract_list(C, S, cur, n, ST_DIRECT, L.dir_t, L.dir_s, L.n_direct, L.dir_targets, L.dir_offsets, L.n_direct_targets);
     L.m2l_order.resize(std::max<int64_t>(1, L.n_m2l));
+    if (L.n_direct > 0) {
+        S.pos.resize(L.n_direct + 1);
+        L.dir_cum.resize(L.n_direct + 1);
+        k_dir_counts<<<grid_for(L.n_direct + 1, 256), 256, 0, C.stream>>>(L.dir_s.data(), L.n_direct, st.branches.data(), S.pos.data());
+        LAUNCH_CHECK();
+        excl_scan(C, S.pos.data(), L.dir_cum.data(), L.n_direct + 1);
+    }
     L.n_pairs = 0;
     if (count_pairs && L.n_direct > 0) {
         k_zero_counter<<<1, 1, 0, C.stream>>>(S.counter.data());

@@ -130,11 +130,26 @@ __device__ inline void stage_source(const double* __restrict__ sq, const double*
     if (K == FMMHIP_VORTEX_GAUSSIAN) g[3] = (R)srad[j];
 }
 
+// Source body at position f of the flattened stream of a target leaf's source
+// leaves (direct-list entries l0..l1-1, dir_cum = exclusive prefix of their sizes):
+// tiles span source leaves, so small leaves do not cost a barrier each.
+__device__ inline int64_t stream_body(const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
+                                      const Branch* __restrict__ sbr, int64_t l0, int64_t l1, int64_t f) {
+    int64_t lo = l0, hi = l1 - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (dir_cum[mid] <= f) lo = mid;
+        else hi = mid - 1;
+    }
+    return sbr[dir_s[lo]].first + (f - dir_cum[lo]);
+}
+
 // ---------------------------------------------------------------- FP64 form
 template <int KERNEL, bool PS, bool GS, bool HS>
 __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict__ dir_targets,
                                                        const int64_t* __restrict__ dir_offsets,
-                                                       const int32_t* __restrict__ dir_s, const Branch* __restrict__ tbr,
+                                                       const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
+                                                       const Branch* __restrict__ tbr,
                                                        const Branch* __restrict__ sbr, const double* __restrict__ tx,
                                                        const double* __restrict__ ty, const double* __restrict__ tz,
                                                        const double* __restrict__ sx, const double* __restrict__ sy,
@@ -145,6 +160,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
     __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
+    const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
     const int tid = threadIdx.x;
     int tpt = 1;   // threads per target
     while (tpt < 64 && T.n_bodies * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
@@ -158,24 +174,21 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
         double a[13];
 #pragma unroll
         for (int k = 0; k < 13; ++k) a[k] = 0.0;
-        for (int64_t li = l0; li < l1; ++li) {
-            const Branch S = sbr[dir_s[li]];
-            for (int64_t s0 = 0; s0 < S.n_bodies; s0 += P2P_BLOCK) {
-                const int64_t cnt = std::min<int64_t>(P2P_BLOCK, S.n_bodies - s0);
-                __syncthreads();
-                if (tid < cnt) {
-                    const int64_t j = S.first + s0 + tid;
-                    lx[tid] = sx[j]; ly[tid] = sy[j]; lz[tid] = sz[j];
-                    stage_source<KERNEL, double>(sq, srad, ns, j, lg[tid]);
-                }
-                __syncthreads();
-                if (active)
-                    for (int s = lane_k; s < cnt; s += tpt) {
-                        const double dx = xt - lx[s], dy = yt - ly[s], dz = zt - lz[s];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dx, dy, dz, r2, rinv_f64(r2), lg[s], a);
-                    }
+        for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
+            const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
+            __syncthreads();
+            if (tid < cnt) {
+                const int64_t j = stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid);
+                lx[tid] = sx[j]; ly[tid] = sy[j]; lz[tid] = sz[j];
+                stage_source<KERNEL, double>(sq, srad, ns, j, lg[tid]);
             }
+            __syncthreads();
+            if (active)
+                for (int s = lane_k; s < cnt; s += tpt) {
+                    const double dx = xt - lx[s], dy = yt - ly[s], dz = zt - lz[s];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dx, dy, dz, r2, rinv_f64(r2), lg[s], a);
+                }
         }
         // fold the tpt partial sums of a target (adjacent lanes)
         for (int w = 1; w < tpt; w <<= 1)
@@ -194,7 +207,8 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
 template <int KERNEL, bool PS, bool GS, bool HS>
 __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict__ dir_targets,
                                                        const int64_t* __restrict__ dir_offsets,
-                                                       const int32_t* __restrict__ dir_s, const Branch* __restrict__ tbr,
+                                                       const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
+                                                       const Branch* __restrict__ tbr,
                                                        const Branch* __restrict__ sbr, const double* __restrict__ tx,
                                                        const double* __restrict__ ty, const double* __restrict__ tz,
                                                        const double* __restrict__ sx, const double* __restrict__ sy,
@@ -205,6 +219,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
     __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
+    const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
     const int tid = threadIdx.x;
     const double cx0 = T.target_center[0], cy0 = T.target_center[1], cz0 = T.target_center[2];
     int tpt = 1;
@@ -222,13 +237,12 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
         float sum[NA], cmp[NA];
 #pragma unroll
         for (int k = 0; k < NA; ++k) { sum[k] = 0.f; cmp[k] = 0.f; }
-        for (int64_t li = l0; li < l1; ++li) {
-            const Branch S = sbr[dir_s[li]];
-            for (int64_t s0 = 0; s0 < S.n_bodies; s0 += P2P_BLOCK) {
-                const int64_t cnt = std::min<int64_t>(P2P_BLOCK, S.n_bodies - s0);
+        {
+            for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
+                const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
                 __syncthreads();
                 if (tid < cnt) {
-                    const int64_t j = S.first + s0 + tid;
+                    const int64_t j = stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid);
                     const double px = sx[j] - cx0, py = sy[j] - cy0, pz = sz[j] - cz0;
                     const float hx = (float)px, hy = (float)py, hz = (float)pz;
                     lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
@@ -273,7 +287,8 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
     const int64_t nblk = L.n_direct_targets;
     if (nblk == 0) return;
 #define P2P_ARGS                                                                                                         \
-    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), tt.branches.data(), st.branches.data(), tt.x.data(),    \
+    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), L.dir_cum.data(), tt.branches.data(), st.branches.data(), \
+        tt.x.data(),                                                                                                     \
         tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.radius.data(), st.n,    \
         tout, nt
 #define P2P_DISPATCH(KFN)                                                                                               \
