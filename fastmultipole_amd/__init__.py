@@ -9,10 +9,11 @@ from .core import (Context, Evaluation, DerivativesSwitch, ErrorMethod, Interact
                    PowerAbsolutePotential, PowerRelativeGradient, PowerRelativePotential,
                    RotatedCoefficientsAbsoluteGradient, RotatedCoefficientsRelativeGradient, Barba, SelfTuning,
                    SelfTuningTreeStop, default_context, direct, fmm, make_options)
-from .systems import DipolePoints, GaussianVortexParticles, Gravitational, VortexParticles
+from .systems import (DipolePanels, DipolePoints, GaussianVortexParticles, Gravitational, SourcePanels,
+                      VortexParticles)
 
 __all__ = ["Context", "Evaluation", "DerivativesSwitch", "ErrorMethod", "InteractionListMethod",
            "PowerAbsoluteGradient", "PowerAbsolutePotential", "PowerRelativeGradient", "PowerRelativePotential",
            "RotatedCoefficientsAbsoluteGradient", "RotatedCoefficientsRelativeGradient", "Barba", "SelfTuning",
            "SelfTuningTreeStop", "default_context", "direct", "fmm", "make_options", "Gravitational",
-           "VortexParticles", "DipolePoints", "GaussianVortexParticles"]
+           "VortexParticles", "DipolePoints", "GaussianVortexParticles", "SourcePanels", "DipolePanels"]

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libfmmhip.so")
 
 FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMHIP_EDEPTH = range(6)
 POINT_SOURCE, POINT_VORTEX, POINT_DIPOLE, VORTEX_GAUSSIAN = 0, 1, 2, 3
+PANEL_SOURCE_TRI, PANEL_DIPOLE_TRI, PANEL_SOURCE_QUAD, PANEL_DIPOLE_QUAD = 4, 5, 6, 7
 FP64, MIXED = 0, 1
 BARBA, SELFTUNING, SELFTUNING_TREESTOP = 0, 1, 2
 ERR_NONE, ERR_ROTATED_COEFFICIENTS_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3

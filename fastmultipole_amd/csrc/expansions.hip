@@ -1493,10 +1493,136 @@ __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const
     }
 }
 
+
+// --------------------------------------------------------- panel P2M
+// body_to_multipole! for constant-strength triangles (src/bodytomultipole.jl:643-831):
+// the regular-harmonic recurrences of calculate_q! at x0+xu, calculate_pj! at
+// x0+xv and calculate_ib! at x0 (:112-213), each degree from the previous one
+// (get_nm1 rules, :77-110), then M += |xu x xv| (-1)^(n+m) conj(i_n^m) * (-q)
+// (source) or the dipole conversion source_to_dipole! of i with moment -q n
+// (:251-278). One wavefront per leaf; lane m carries coefficient m of the
+// current degree for all three recurrences at once (one barrier per degree).
+// Quadrilaterals are the triangles (v1, v2, v3) and (v1, v3, v4) with the unit
+// normal of each triangle.
+
+// get_nm1 for lane (n, m) from the degree n-1 values of A ([index][re/im] in LDS)
+__device__ inline void nm1_vals(const double* A, int n, int m, double q[6]) {
+    const int b = nh(n - 2);   // start of degree n-1
+    if (m > 0) { q[0] = A[2 * (b + m - 1)]; q[1] = A[2 * (b + m - 1) + 1]; }
+    else if (n > 1) { q[0] = -A[2 * (b + 1)]; q[1] = A[2 * (b + 1) + 1]; }
+    else { q[0] = 0.0; q[1] = 0.0; }
+    if (m < n) { q[2] = A[2 * (b + m)]; q[3] = A[2 * (b + m) + 1]; } else { q[2] = 0.0; q[3] = 0.0; }
+    if (m + 1 < n) { q[4] = A[2 * (b + m + 1)]; q[5] = A[2 * (b + m + 1) + 1]; } else { q[4] = 0.0; q[5] = 0.0; }
+}
+
+template <bool DIPOLE>
+__device__ void p2m_triangle(const double* v1, const double* v2, const double* v3, const double* center, double strength, int P,
+                             double* Q, double* J, double* I, double* Macc) {
+    const int m = (int)threadIdx.x;
+    const double x0[3] = {v1[0] - center[0], v1[1] - center[1], v1[2] - center[2]};
+    const double xu[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+    const double xv[3] = {v3[0] - v1[0], v3[1] - v1[1], v3[2] - v1[2]};
+    // xyz_to_xi_eta_z (:39-46) of each vector, added componentwise
+    const double x0r = x0[0] * 0.5, x0i = x0[1] * 0.5, xur = xu[0] * 0.5, xui = xu[1] * 0.5, xvr = xv[0] * 0.5, xvi = xv[1] * 0.5;
+    const double qxr = x0r + xur, qxi = x0i + xui, qer = x0r + xur, qei = -x0i + -xui, qz = x0[2] + xu[2];
+    const double jxr = x0r + xvr, jxi = x0i + xvi, jer = x0r + xvr, jei = -x0i + -xvi, jz = x0[2] + xv[2];
+    const double ixr = x0r, ixi = x0i, ier = x0r, iei = -x0i, iz = x0[2];
+    const double c0 = xu[1] * xv[2] - xu[2] * xv[1], c1 = xu[2] * xv[0] - xu[0] * xv[2], c2 = xu[0] * xv[1] - xu[1] * xv[0];
+    const double area2 = sqrt(c0 * c0 + c1 * c1 + c2 * c2);   // |xu x xv|
+    double dqx = 0.0, dqy = 0.0, dqz = 0.0;
+    if (DIPOLE) {   // moment -q n, n the unit normal of this triangle
+        dqx = -c0 / area2 * strength; dqy = -c1 / area2 * strength; dqz = -c2 / area2 * strength;
+    }
+    const double w = DIPOLE ? area2 : area2 * (-strength);
+    for (int n = 0; n <= P; ++n) {
+        const bool act = m <= n;
+        const int i = nh(n - 1) + m;
+        double qr = 0.0, qi = 0.0, jr = 0.0, ji = 0.0, ir = 0.0, ii = 0.0, dr = 0.0, di = 0.0;
+        if (act) {
+            if (n == 0) {
+                qr = 1.0; jr = 1.0; ir = 0.5;
+            } else {
+                double a[6];
+                nm1_vals(Q, n, m, a);   // calculate_q! (:112-139)
+                qr = (-(qxr * a[1] + qxi * a[0]) - (qer * a[5] + qei * a[4]) - qz * a[2]) / (double)n;
+                qi = ((qxr * a[0] - qxi * a[1]) + (qer * a[4] - qei * a[5]) - qz * a[3]) / (double)n;
+                nm1_vals(J, n, m, a);   // calculate_pj! (:146-176)
+                jr = (-(jxr * a[1] + jxi * a[0]) - (jer * a[5] + jei * a[4]) - jz * a[2] + qr) / (double)(n + 1);
+                ji = ((jxr * a[0] - jxi * a[1]) + (jer * a[4] - jei * a[5]) - jz * a[3] + qi) / (double)(n + 1);
+                nm1_vals(I, n, m, a);   // calculate_ib! (:183-213)
+                ir = (-(ixr * a[1] + ixi * a[0]) - (ier * a[5] + iei * a[4]) - iz * a[2] + jr) / (double)(n + 2);
+                ii = ((ixr * a[0] - ixi * a[1]) + (ier * a[4] - iei * a[5]) - iz * a[3] + ji) / (double)(n + 2);
+                if (DIPOLE) {   // source_to_dipole! (:251-278) from i of degree n-1
+                    dr = -dqx * 0.5 * (a[5] + a[1]) + dqy * 0.5 * (a[4] - a[0]) - dqz * a[2];
+                    di = dqx * 0.5 * (a[4] + a[0]) + dqy * 0.5 * (a[5] - a[1]) - dqz * a[3];
+                }
+            }
+            const double sg = ((n + m) & 1) ? -1.0 : 1.0;
+            if (DIPOLE) { Macc[2 * i] += w * sg * dr; Macc[2 * i + 1] -= w * sg * di; }
+            else { Macc[2 * i] += w * sg * ir; Macc[2 * i + 1] -= w * sg * ii; }
+        }
+        __syncthreads();   // every lane has read degree n-1 of the three arrays
+        if (act) {
+            Q[2 * i] = qr; Q[2 * i + 1] = qi;
+            J[2 * i] = jr; J[2 * i + 1] = ji;
+            I[2 * i] = ir; I[2 * i + 1] = ii;
+        }
+        __syncthreads();   // degree n published
+    }
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(64) void k_p2m_panel(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
+                                                  const double* __restrict__ rows, int64_t ns, double* __restrict__ ex, int P) {
+    extern __shared__ double lds[];
+    constexpr int NV = panel_vertices(KERNEL);
+    constexpr bool DIPOLE = panel_dipole(KERNEL);
+    const int H = nh(P);
+    double* Q = lds;
+    double* J = Q + 2 * H;
+    double* I = J + 2 * H;
+    double* Macc = I + 2 * H;
+    const int64_t ib = leaf_index[blockIdx.x];
+    const Branch B = br[ib];
+    for (int k = threadIdx.x; k < 2 * H; k += 64) Macc[k] = 0.0;
+    __syncthreads();
+    for (int64_t j = B.first; j < B.first + B.n_bodies; ++j) {
+        double v[NV][3];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) v[k][a] = rows[(int64_t)(1 + 3 * k + a) * ns + j];
+        const double q = rows[j];
+#pragma unroll
+        for (int t = 0; t < NV - 2; ++t)
+            p2m_triangle<DIPOLE>(v[0], v[t + 1], v[t + 2], B.source_center, q, P, Q, J, I, Macc);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < H; k += 64) {
+        ex[ib * 4 * H + 4 * k] += Macc[2 * k];
+        ex[ib * 4 * H + 4 * k + 1] += Macc[2 * k + 1];
+    }
+}
+
 void run_p2m(Context& C, int kernel, Tree& st, int p) {
     ensure_tables(C, p);
     if (st.n_leaves == 0) return;
     TablePtrs tb = ptrs(C);
+    if (is_panel(kernel)) {
+        const size_t sh = sizeof(double) * 8 * nh(p);
+#define FMMHIP_P2M_PANEL(K)                                                                                              \
+    k_p2m_panel<K><<<(unsigned)st.n_leaves, 64, sh, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.strength.data(), \
+                                                               st.n, st.expansions.data(), p)
+        switch (kernel) {
+            case FMMHIP_PANEL_SOURCE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_TRI); break;
+            case FMMHIP_PANEL_DIPOLE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_TRI); break;
+            case FMMHIP_PANEL_SOURCE_QUAD: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_QUAD); break;
+            default: FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_QUAD); break;
+        }
+#undef FMMHIP_P2M_PANEL
+        LAUNCH_CHECK();
+        return;
+    }
 #define FMMHIP_P2M(K)                                                                                                  \
     k_p2m<K><<<(unsigned)st.n_leaves, 64, 0, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(), \
                                                         st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact)

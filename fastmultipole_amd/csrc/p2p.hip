@@ -312,8 +312,14 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
 #undef P2P_ARGS
 }
 
+void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
+                    bool GS, bool HS);
+void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
+                       double* tout_aos, bool PS, bool GS, bool HS);
+
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
              bool HS, int precision) {
+    if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS); return; }
     switch (kernel) {
         case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
         case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
@@ -384,12 +390,293 @@ static void launch_direct_all(Context& C, int64_t nt, const double* t_xyz, int64
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf, double* tout_aos,
                     bool PS, bool GS, bool HS, int precision) {
     if (nt == 0 || ns == 0) return;
+    if (is_panel(kernel)) { run_direct_panels(C, kernel, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS); return; }
     switch (kernel) {
         case FMMHIP_POINT_SOURCE: launch_direct_all<FMMHIP_POINT_SOURCE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
         case FMMHIP_POINT_VORTEX: launch_direct_all<FMMHIP_POINT_VORTEX>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
         case FMMHIP_POINT_DIPOLE: launch_direct_all<FMMHIP_POINT_DIPOLE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
         default: launch_direct_all<FMMHIP_VORTEX_GAUSSIAN>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
     }
+}
+
+}  // namespace fmmhip
+
+namespace fmmhip {
+
+// ================================================================ panels
+// Constant-strength planar panels, nearfield: induced() of test/panels.jl
+// (rotate_to_panel :78-93, source_dipole_preliminaries :95-133, Panel{Source}
+// :139-216, Panel{Dipole} :224-315) for every (target, panel) pair, FP64 in both
+// precision forms (log/atan edge terms). The per-panel frame and edge geometry
+// is computed once when a panel is staged into LDS.
+
+static constexpr int PANEL_TILE = 64;   // panels per LDS tile
+
+template <int NV> struct PanelStage {
+    double c[3], ex[3], ey[3], ez[3];   // centroid, panel frame (R' rows)
+    double vx[NV + 1], vy[NV + 1];      // vertices in the panel frame (closed polygon)
+    double dx[NV], dy[NV], ds[NV], ms[NV];
+    double strength;
+};
+
+// panel j of the sorted arrays (rows after radius: strength, v1, v2, ...) into a stage
+template <int NV>
+__device__ inline void stage_panel(const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+                                   const double* __restrict__ rows, int64_t n, int64_t j, PanelStage<NV>& P) {
+    double v[NV][3];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[k][a] = rows[(int64_t)(1 + 3 * k + a) * n + j];
+    P.c[0] = px[j]; P.c[1] = py[j]; P.c[2] = pz[j];
+    P.strength = rows[j];
+    // unit normal from the first three vertices (get_normal, src/compatibility.jl:65-72)
+    const double a0 = v[1][0] - v[0][0], a1 = v[1][1] - v[0][1], a2 = v[1][2] - v[0][2];
+    const double b0 = v[2][0] - v[0][0], b1 = v[2][1] - v[0][1], b2 = v[2][2] - v[0][2];
+    double nz[3] = {a1 * b2 - a2 * b1, a2 * b0 - a0 * b2, a0 * b1 - a1 * b0};
+    const double nzn = sqrt(nz[0] * nz[0] + nz[1] * nz[1] + nz[2] * nz[2]);
+    double nx[3] = {b0, b1, b2};
+    const double nxn = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { nz[a] /= nzn; nx[a] /= nxn; }
+    const double ny[3] = {nz[1] * nx[2] - nz[2] * nx[1], nz[2] * nx[0] - nz[0] * nx[2], nz[0] * nx[1] - nz[1] * nx[0]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { P.ex[a] = nx[a]; P.ey[a] = ny[a]; P.ez[a] = nz[a]; }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double d0 = v[k][0] - P.c[0], d1 = v[k][1] - P.c[1], d2 = v[k][2] - P.c[2];
+        P.vx[k] = nx[0] * d0 + nx[1] * d1 + nx[2] * d2;
+        P.vy[k] = ny[0] * d0 + ny[1] * d1 + ny[2] * d2;
+    }
+    P.vx[NV] = P.vx[0];
+    P.vy[NV] = P.vy[0];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        P.dx[k] = P.vx[k + 1] - P.vx[k];
+        P.dy[k] = P.vy[k + 1] - P.vy[k];
+        P.ds[k] = sqrt(P.dx[k] * P.dx[k] + P.dy[k] * P.dy[k]);
+        P.ms[k] = P.dy[k] / P.dx[k];
+    }
+}
+
+// one (target, panel) pair; accumulates [phi, velocity slot (3), Hessian slot (9)] into a
+template <int NV, bool DIPOLE, bool PS, bool GS, bool HS>
+__device__ inline void panel_pair(const PanelStage<NV>& P, double tx, double ty, double tz, double* a) {
+    const double EPS = 2.220446049250313e-16;   // the reference's eps() perturbation
+    const double d0 = tx - P.c[0], d1 = ty - P.c[1], d2 = tz - P.c[2];
+    const double tRx = P.ex[0] * d0 + P.ex[1] * d1 + P.ex[2] * d2;
+    const double tRy = P.ey[0] * d0 + P.ey[1] * d1 + P.ey[2] * d2;
+    double dz = P.ez[0] * d0 + P.ez[1] * d1 + P.ez[2] * d2;
+    if (dz == 0.0) dz = EPS;
+    const double dz2 = dz * dz;
+    double rxs[NV + 1], rys[NV + 1], es[NV + 1], hs[NV + 1], rs[NV + 1];
+#pragma unroll
+    for (int i = 0; i <= NV; ++i) {
+        rxs[i] = tRx - P.vx[i];
+        rys[i] = tRy - P.vy[i];
+        es[i] = rxs[i] * rxs[i] + dz2;
+        hs[i] = rxs[i] * rys[i];
+        const double dy = P.vy[i] - tRy;
+        rs[i] = sqrt(es[i] + dy * dy);
+    }
+    double pot = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // xx, xy, xz, yy, yz, zz
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double ri = rs[i], rip1 = rs[i + 1];
+        const double r_plus = ri + rip1;
+        double num = r_plus - P.ds[i];
+        if (num == 0.0) num += EPS;
+        const double log_term = (!DIPOLE && (PS || GS)) ? log(num / (r_plus + P.ds[i])) : 0.0;
+        if (ri == 0.0) ri += EPS;
+        if (rip1 == 0.0) rip1 += EPS;
+        const double tan_term = (PS || !DIPOLE) ? atan((P.ms[i] * es[i] - hs[i]) / dz / ri) -
+                                                    atan((P.ms[i] * es[i + 1] - hs[i + 1]) / dz / rip1)
+                                              : 0.0;
+        const double dx = P.dx[i], dy = P.dy[i];
+        const double r_times = rs[i] * rs[i + 1];
+        const double rho = r_times + rxs[i] * rxs[i + 1] + rys[i] * rys[i + 1] + dz2;
+        const double lambda = rxs[i] * rys[i + 1] - rxs[i + 1] * rys[i];
+        if (!DIPOLE) {
+            if (PS) { pot += (rxs[i] * dy - rys[i] * dx) / P.ds[i] * log_term; pot += dz * tan_term; }
+            if (GS) { u0 += dy / P.ds[i] * log_term; u1 += -dx / P.ds[i] * log_term; u2 += tan_term; }
+            if (HS) {
+                const double val1 = r_plus * r_plus - P.ds[i] * P.ds[i];
+                const double val2 = rxs[i] / rs[i] + rxs[i + 1] / rs[i + 1];
+                const double val3 = rys[i] / rs[i] + rys[i + 1] / rs[i + 1];
+                const double val4 = r_plus / (r_times * rho);
+                h[0] += 2 * dy / val1 * val2;
+                h[1] += -2 * dx / val1 * val2;
+                h[2] += dz * dy * val4;
+                h[3] += -2 * dx / val1 * val3;
+                h[4] += -dz * dx * val4;
+                h[5] += lambda * val4;
+            }
+        } else {
+            if (PS) pot -= tan_term;
+            if (GS) {
+                const double val4 = r_plus / (r_times * rho);
+                u0 += dz * dy * val4;
+                u1 += -dz * dx * val4;
+                u2 += lambda * val4;
+            }
+            if (HS) {
+                const double rp2 = r_plus * r_plus;
+                const double val1 = (r_times * rp2 + rho * rs[i + 1] * rs[i + 1]) / (rho * rs[i] * r_plus);
+                const double val2 = (r_times * rp2 + rho * rs[i] * rs[i]) / (rho * rs[i + 1] * r_plus);
+                const double val3 = r_plus / (rho * r_times * r_times);
+                h[0] += dz * dy * val3 * (rxs[i] * val1 + rxs[i + 1] * val2);
+                h[1] += dz * dy * val3 * (rys[i] * val1 + rys[i + 1] * val2);
+                h[3] += -dz * dx * val3 * (rys[i] * val1 + rys[i + 1] * val2);
+                const double val6 = dz * (rp2 / rho + (rs[i] * rs[i] - r_times + rs[i + 1] * rs[i + 1]) / r_times);
+                h[5] += lambda * val3 * val6;
+                const double val8 = val3 * (r_times - dz * val6);
+                h[2] += -dy * val8;
+                h[4] += dx * val8;
+            }
+        }
+    }
+    const double s4 = P.strength * INV4PI;
+    if (PS) a[0] += pot * s4;
+    if (GS) {   // velocity = (+-) s/(4 pi) R u, R = [ex ey ez] as columns
+        const double sv = DIPOLE ? s4 : -s4;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[1 + c] += sv * (P.ex[c] * u0 + P.ey[c] * u1 + P.ez[c] * u2);
+    }
+    if (HS) {   // -s/(4 pi) R h R'
+        const double H[3][3] = {{h[0], h[1], h[2]}, {h[1], h[3], h[4]}, {h[2], h[4], h[5]}};
+        const double* E[3] = {P.ex, P.ey, P.ez};
+        double RH[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) RH[r][k] = E[0][r] * H[0][k] + E[1][r] * H[1][k] + E[2][r] * H[2][k];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a[4 + 3 * r + c] += -s4 * (RH[r][0] * E[0][c] + RH[r][1] * E[1][c] + RH[r][2] * E[2][c]);
+    }
+}
+
+template <int NV, bool DIPOLE, bool PS, bool GS, bool HS>
+__global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restrict__ dir_targets,
+                                                         const int64_t* __restrict__ dir_offsets,
+                                                         const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
+                                                         const Branch* __restrict__ tbr, const Branch* __restrict__ sbr,
+                                                         const double* __restrict__ tx, const double* __restrict__ ty,
+                                                         const double* __restrict__ tz, const double* __restrict__ sx,
+                                                         const double* __restrict__ sy, const double* __restrict__ sz,
+                                                         const double* __restrict__ rows, int64_t ns, double* __restrict__ tout,
+                                                         int64_t nt) {
+    __shared__ PanelStage<NV> tile[PANEL_TILE];
+    const Branch T = tbr[dir_targets[blockIdx.x]];
+    const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
+    const int64_t base = dir_cum[l0], end = dir_cum[l1];
+    const int tid = threadIdx.x;
+    int tpt = 1;
+    while (tpt < 64 && T.n_bodies * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
+    const int per_chunk = P2P_BLOCK / tpt;
+    for (int64_t c0 = 0; c0 < T.n_bodies; c0 += per_chunk) {
+        const int64_t ti = c0 + tid / tpt;
+        const int lane_k = tid % tpt;
+        const bool active = ti < T.n_bodies;
+        const int64_t it = T.first + (active ? ti : 0);
+        const double xt = tx[it], yt = ty[it], zt = tz[it];
+        double a[13];
+#pragma unroll
+        for (int k = 0; k < 13; ++k) a[k] = 0.0;
+        for (int64_t f0 = base; f0 < end; f0 += PANEL_TILE) {
+            const int64_t cnt = std::min<int64_t>(PANEL_TILE, end - f0);
+            __syncthreads();
+            if (tid < cnt) stage_panel<NV>(sx, sy, sz, rows, ns, stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid), tile[tid]);
+            __syncthreads();
+            if (active)
+                for (int s = lane_k; s < cnt; s += tpt) panel_pair<NV, DIPOLE, PS, GS, HS>(tile[s], xt, yt, zt, a);
+        }
+        for (int w = 1; w < tpt; w <<= 1)
+#pragma unroll
+            for (int k = 0; k < 13; ++k) a[k] += __shfl_xor(a[k], w, 64);
+        if (active && lane_k == 0) {
+            if (PS) tout[it] += a[0];
+            if (GS) { tout[nt + it] += a[1]; tout[2 * nt + it] += a[2]; tout[3 * nt + it] += a[3]; }
+            if (HS) for (int k = 4; k < 13; ++k) tout[(int64_t)k * nt + it] += a[k];
+        }
+    }
+}
+
+template <int NV, bool DIPOLE>
+static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
+                             bool GS, bool HS) {
+    const int64_t nblk = L.n_direct_targets;
+    if (nblk == 0) return;
+#define PANEL_ARGS                                                                                                       \
+    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), L.dir_cum.data(), tt.branches.data(), st.branches.data(), \
+        tt.x.data(), tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.n, tout, nt
+    const int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);
+    switch (code) {
+        case 0: break;
+        case 1: k_p2p_panel<NV, DIPOLE, false, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 2: k_p2p_panel<NV, DIPOLE, false, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 3: k_p2p_panel<NV, DIPOLE, false, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 4: k_p2p_panel<NV, DIPOLE, true, false, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 5: k_p2p_panel<NV, DIPOLE, true, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 6: k_p2p_panel<NV, DIPOLE, true, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        default: k_p2p_panel<NV, DIPOLE, true, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+    }
+#undef PANEL_ARGS
+    LAUNCH_CHECK();
+}
+
+void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
+                    bool GS, bool HS) {
+    switch (kernel) {
+        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, false>(C, tt, st, L, tout, nt, PS, GS, HS); break;
+        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, true>(C, tt, st, L, tout, nt, PS, GS, HS); break;
+        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, false>(C, tt, st, L, tout, nt, PS, GS, HS); break;
+        default: launch_p2p_panel<4, true>(C, tt, st, L, tout, nt, PS, GS, HS); break;
+    }
+}
+
+// direct! for panels: original-order AoS inputs, 16 x nt AoS output (+=)
+template <int NV, bool DIPOLE>
+__global__ __launch_bounds__(P2P_BLOCK) void k_direct_panels(const double* __restrict__ txyz, int64_t nt,
+                                                             const double* __restrict__ sbuf, int64_t ns,
+                                                             double* __restrict__ out, int PS, int GS, int HS) {
+    __shared__ PanelStage<NV> tile[PANEL_TILE];
+    constexpr int rows = 5 + 3 * NV;
+    const int64_t it = blockIdx.x * (int64_t)P2P_BLOCK + threadIdx.x;
+    const bool active = it < nt;
+    const double xt = active ? txyz[3 * it] : 0.0, yt = active ? txyz[3 * it + 1] : 0.0, zt = active ? txyz[3 * it + 2] : 0.0;
+    double a[13];
+    for (int k = 0; k < 13; ++k) a[k] = 0.0;
+    for (int64_t s0 = 0; s0 < ns; s0 += PANEL_TILE) {
+        const int64_t cnt = std::min<int64_t>(PANEL_TILE, ns - s0);
+        __syncthreads();
+        if ((int64_t)threadIdx.x < cnt) {
+            const double* b = sbuf + (s0 + threadIdx.x) * rows;   // one panel's row as a 1-body SoA
+            stage_panel<NV>(b, b + 1, b + 2, b + 4, 1, 0, tile[threadIdx.x]);
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int s = 0; s < cnt; ++s) panel_pair<NV, DIPOLE, true, true, true>(tile[s], xt, yt, zt, a);
+    }
+    if (!active) return;
+    double* o = out + it * 16;
+    if (PS) o[3] += a[0];
+    if (GS) { o[4] += a[1]; o[5] += a[2]; o[6] += a[3]; }
+    if (HS) for (int k = 4; k < 13; ++k) o[3 + k] += a[k];
+}
+
+void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
+                       double* tout_aos, bool PS, bool GS, bool HS) {
+    const unsigned grid = (unsigned)((nt + P2P_BLOCK - 1) / P2P_BLOCK);
+    switch (kernel) {
+        case FMMHIP_PANEL_SOURCE_TRI: k_direct_panels<3, false><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        case FMMHIP_PANEL_DIPOLE_TRI: k_direct_panels<3, true><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        case FMMHIP_PANEL_SOURCE_QUAD: k_direct_panels<4, false><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        default: k_direct_panels<4, true><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+    }
+    LAUNCH_CHECK();
 }
 
 }  // namespace fmmhip

@@ -164,3 +164,74 @@ class GaussianVortexParticles(VortexParticles):
     kernel, the expansions are those of Point{Vortex}."""
 
     fmm_kernel = L.VORTEX_GAUSSIAN
+
+
+class _Panels:
+    """Constant-strength planar panels (test/bodytomultipole_test.jl:80-120 SourcePanels,
+    :221-272 DipolePanels): vertices (N, 3|4, 3) counter-clockwise about the normal."""
+
+    fmm_kernel = None
+
+    def __init__(self, vertices, strength):
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float64)
+        self.strength = np.ascontiguousarray(strength, dtype=np.float64).reshape(-1)
+        n = self.vertices.shape[0]
+        self.potential = np.zeros(n)
+        self.gradient = np.zeros((n, 3))
+        self.hessian = np.zeros((n, 9))
+
+    @property
+    def nv(self):
+        return self.vertices.shape[1]
+
+    def get_n_bodies(self):
+        return self.vertices.shape[0]
+
+    def data_per_body(self):
+        return 5 + 3 * self.nv
+
+    def strength_dims(self):
+        return 1
+
+    def has_vector_potential(self):
+        return False
+
+    def positions(self):
+        return self.vertices.mean(axis=1)
+
+    def get_position(self, i):
+        return self.vertices[i].mean(axis=0)
+
+    def source_buffer(self):
+        from .workloads import panel_rows
+        return panel_rows(self.vertices, self.strength)
+
+    def buffer_to_target_system(self, target_buffer, switch):
+        n = self.get_n_bodies()
+        if switch.scalar_potential:
+            self.potential += target_buffer[:n, 3]
+        if switch.gradient:
+            self.gradient += target_buffer[:n, 4:7]
+        if switch.hessian:
+            self.hessian += target_buffer[:n, 7:16]
+
+    def reset(self):
+        self.potential[:] = 0.0
+        self.gradient[:] = 0.0
+        self.hessian[:] = 0.0
+
+
+class SourcePanels(_Panels):
+    """Panel{Source}: constant source density on each triangle / planar quadrilateral."""
+
+    @property
+    def fmm_kernel(self):
+        return L.PANEL_SOURCE_QUAD if self.nv == 4 else L.PANEL_SOURCE_TRI
+
+
+class DipolePanels(_Panels):
+    """Panel{Dipole}: constant normal-doublet density on each triangle / planar quadrilateral."""
+
+    @property
+    def fmm_kernel(self):
+        return L.PANEL_DIPOLE_QUAD if self.nv == 4 else L.PANEL_DIPOLE_TRI

@@ -93,3 +93,50 @@ def gaussian_vortex_ring(n, seed=2, radius=1.0, core_sigma=0.1, noise=1e-3, sigm
         sigma = 1.5 * (vol / n) ** (1.0 / 3.0)
     b[:, 3] = sigma
     return b
+
+
+def panel_rows(vertices, strength):
+    """Panel buffer rows [centroid(3), radius, strength, v1(3), v2(3), v3(3), (v4(3))] from an
+    (n, nv, 3) vertex array (counter-clockwise about the normal) and (n,) strengths. The
+    centroid is the vertex mean; the radius is the largest centroid-vertex distance."""
+    v = np.asarray(vertices, dtype=np.float64)
+    cen = v.mean(axis=1)
+    rad = np.linalg.norm(v - cen[:, None, :], axis=2).max(axis=1)
+    return np.column_stack([cen, rad, np.asarray(strength, dtype=np.float64), v.reshape(len(v), -1)])
+
+
+def sphere_panels(m, quads=True, seed=5, radius=1.0):
+    """Panels tiling a sphere: each cube face split into an m x m grid projected onto the sphere.
+    Quadrilaterals are made planar (the 4th vertex is projected onto the plane of the first three);
+    with quads=False each quadrilateral is split into two triangles. Strengths N(0,1)/n_panels.
+    Returns (rows, kind)."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(-1.0, 1.0, m + 1)
+    faces = []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            a, b = [k for k in range(3) if k != axis]
+            for i in range(m):
+                for j in range(m):
+                    quad = []
+                    for (ii, jj) in ((i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1)):
+                        p = np.zeros(3)
+                        p[axis] = sgn
+                        p[a] = t[ii]
+                        p[b] = t[jj]
+                        quad.append(radius * p / np.linalg.norm(p))
+                    quad = np.array(quad)
+                    n = np.cross(quad[1] - quad[0], quad[2] - quad[0])
+                    if np.dot(n, quad.mean(axis=0)) < 0:      # counter-clockwise about the outward normal
+                        quad = quad[::-1].copy()
+                        n = -n
+                    n /= np.linalg.norm(n)
+                    quad[3] = quad[3] - np.dot(quad[3] - quad[0], n) * n
+                    faces.append(quad)
+    faces = np.array(faces)
+    if quads:
+        q = rng.normal(size=len(faces)) / len(faces)
+        return panel_rows(faces, q), 6
+    tris = np.concatenate([faces[:, [0, 1, 2]], faces[:, [0, 2, 3]]], axis=0)
+    q = rng.normal(size=len(tris)) / len(tris)
+    return panel_rows(tris, q), 4

@@ -14,7 +14,7 @@
  * 155-172): column-major double matrices, i.e. one record per body.
  *   target positions : 3 x N   [x, y, z]
  *   source buffer    : data_per_body x N  [x, y, z, radius, strength...]
- *                      (5 rows for POINT_SOURCE, 7 rows for the other kernels)
+ *                      (5 rows for POINT_SOURCE, 7 for the other point kernels, 14 / 17 for panels)
  *   target results   : 16 x N  [x, y, z, phi, grad(3), hessian(9)] in the ORIGINAL body
  *                      order (the reference's target buffer after unsorting through
  *                      sort_index, src/compatibility.jl:324-338). Hessian entries are in
@@ -51,13 +51,22 @@ typedef enum {
  *   VORTEX_GAUSSIAN  vortex particle with a Gaussian core of width sigma (row 4): the
  *                    singular kernel times erf(rho/sqrt2) - sqrt(2/pi) rho exp(-rho^2/2),
  *                    rho = r/sigma; expansions as POINT_VORTEX;     rows [x,y,z,sigma,Gx,Gy,Gz]
+ *   PANEL_*          constant-strength planar panels, Panel{Source} / Panel{Dipole}: nearfield
+ *                    induced() of test/panels.jl, expansions body_to_multipole_panel!
+ *                    (src/bodytomultipole.jl:643-831); triangles: rows [centroid(3), radius,
+ *                    strength, v1(3), v2(3), v3(3)] (14 rows); quadrilaterals add v4(3) (17 rows);
+ *                    vertices counter-clockwise about the normal; FP64 nearfield in both precisions
  * Results use the expansion evaluation's convention: slot "gradient" = -grad(phi) for the
  * scalar kernels, the velocity for the vortex kernels; slot "hessian" = d(gradient slot). */
 typedef enum {
     FMMHIP_POINT_SOURCE = 0,
     FMMHIP_POINT_VORTEX = 1,
     FMMHIP_POINT_DIPOLE = 2,
-    FMMHIP_VORTEX_GAUSSIAN = 3
+    FMMHIP_VORTEX_GAUSSIAN = 3,
+    FMMHIP_PANEL_SOURCE_TRI = 4,
+    FMMHIP_PANEL_DIPOLE_TRI = 5,
+    FMMHIP_PANEL_SOURCE_QUAD = 6,
+    FMMHIP_PANEL_DIPOLE_QUAD = 7
 } fmmhip_kernel;
 typedef enum { FMMHIP_FP64 = 0, FMMHIP_MIXED = 1 } fmmhip_precision;
 typedef enum { FMMHIP_BARBA = 0, FMMHIP_SELFTUNING = 1, FMMHIP_SELFTUNING_TREESTOP = 2 } fmmhip_list_method;

@@ -21,6 +21,9 @@ KIND_SOURCE = 0   # Point{Source}, test/gravitational.jl
 KIND_VORTEX = 1   # Point{Vortex}, test/vortex.jl
 KIND_DIPOLE = 2   # Point{Dipole}: P2M src/bodytomultipole.jl:431-463, direct kernel stated in oracle_fmm.cpp
 KIND_VORTEX_GAUSS = 3   # Gaussian-core vortex (sigma = buffer row 4), stated in oracle_fmm.cpp
+# constant-strength panels (test/panels.jl; P2M src/bodytomultipole.jl:643-831);
+# rows [centroid(3), radius, strength, v1(3), v2(3), v3(3), v4(3) for quadrilaterals]
+KIND_PANEL_SOURCE_TRI, KIND_PANEL_DIPOLE_TRI, KIND_PANEL_SOURCE_QUAD, KIND_PANEL_DIPOLE_QUAD = 4, 5, 6, 7
 METHOD_BARBA, METHOD_SELFTUNING, METHOD_SELFTUNING_TREESTOP = 0, 1, 2
 ERR_NONE, ERR_ROTATED_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
 
@@ -84,6 +87,7 @@ def lib():
         L.orc_m2l.restype = C.c_int
         L.orc_l2p.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_p2m.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp]
+        L.orc_p2m_panel.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp]
         _LIB = L
     return _LIB
 
@@ -102,6 +106,8 @@ def length_Hs(p):
 
 
 def data_per_body(kind):
+    if kind >= KIND_PANEL_SOURCE_TRI:
+        return 5 + 3 * (4 if kind >= KIND_PANEL_SOURCE_QUAD else 3)
     return 5 if kind == KIND_SOURCE else 7
 
 
@@ -376,6 +382,14 @@ def l2p(dx, local, p, LH=False, PS=True, GS=True, HS=True):
     out = np.zeros(13)
     lib().orc_l2p(np.asarray(dx, float), np.ascontiguousarray(local), p, int(LH), int(PS), int(GS), int(HS), out)
     return out[0], out[1:4], out[4:13]
+
+
+def p2m_panel(kind, panel_row, center, p, out=None):
+    """body_to_multipole! of one panel (row in the buffer layout above) about `center`."""
+    out = _exp(p) if out is None else out
+    lib().orc_p2m_panel(kind, np.ascontiguousarray(panel_row, dtype=np.float64),
+                        np.ascontiguousarray(center, dtype=np.float64), p, out)
+    return out
 
 
 def p2m(kind, dx, strength, p, out=None):

@@ -27,7 +27,13 @@ enum { METHOD_BARBA = 0, METHOD_SELFTUNING = 1, METHOD_SELFTUNING_TREESTOP = 2 }
 // kernel kinds: point source, singular point vortex (the reference's test kernels),
 // point dipole and Gaussian-core vortex (kernels of this library with no
 // reference direct! counterpart; their formulas are stated at the functions)
-enum { KIND_SOURCE = 0, KIND_VORTEX = 1, KIND_DIPOLE = 2, KIND_VORTEX_GAUSS = 3 };
+enum { KIND_SOURCE = 0, KIND_VORTEX = 1, KIND_DIPOLE = 2, KIND_VORTEX_GAUSS = 3,
+       // constant-strength planar panels (test/panels.jl), triangles and quadrilaterals;
+       // buffer rows [centroid(3), radius, strength, v1(3), v2(3), v3(3), v4(3) for quads]
+       KIND_PANEL_SOURCE_TRI = 4, KIND_PANEL_DIPOLE_TRI = 5, KIND_PANEL_SOURCE_QUAD = 6, KIND_PANEL_DIPOLE_QUAD = 7 };
+static bool is_panel(int kind) { return kind >= KIND_PANEL_SOURCE_TRI; }
+static int panel_vertices(int kind) { return kind >= KIND_PANEL_SOURCE_QUAD ? 4 : 3; }
+static bool panel_dipole(int kind) { return kind == KIND_PANEL_DIPOLE_TRI || kind == KIND_PANEL_DIPOLE_QUAD; }
 
 struct OTree {
     std::vector<OBranch> branches;
@@ -495,15 +501,203 @@ static void direct_vortex_gauss(double* tbuf, int64_t t0, int64_t nt, const doub
     }
 }
 
+
+// ---------------------------------------------------------- panels
+// Constant-strength planar source / normal-doublet panel with NS vertices:
+// induced() of test/panels.jl (:1-11 entry, :78-93 rotate_to_panel, :95-133
+// source_dipole_preliminaries, :139-216 source, :224-315 dipole). Returns the
+// potential, the velocity slot and the Hessian slot.
+static void panel_induced(const double* tgt, const double (*v)[3], int NS, const double* normal, double strength,
+                          const double* centroid, bool dipole, bool PS, bool GS, bool HS, double& phi, double vel[3],
+                          double H[9]) {
+    const double EPS = 2.220446049250313e-16;   // Julia eps()
+    // rotate_to_panel: R = [new_x new_y new_z], new_z = normal, new_x = (v3 - v1)/|v3 - v1|, new_y = new_z x new_x
+    double nx[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    const double nn = std::sqrt(nx[0] * nx[0] + nx[1] * nx[1] + nx[2] * nx[2]);
+    for (int a = 0; a < 3; ++a) nx[a] /= nn;
+    const double* nz = normal;
+    const double ny[3] = {nz[1] * nx[2] - nz[2] * nx[1], nz[2] * nx[0] - nz[0] * nx[2], nz[0] * nx[1] - nz[1] * nx[0]};
+    const double* Rc[3] = {nx, ny, nz};   // columns of R = rows of R'
+    auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const double dt[3] = {tgt[0] - centroid[0], tgt[1] - centroid[1], tgt[2] - centroid[2]};
+    const double tR[3] = {dot(nx, dt), dot(ny, dt), dot(nz, dt)};
+    double vx[5], vy[5];
+    for (int i = 0; i < NS; ++i) {
+        const double d[3] = {v[i][0] - centroid[0], v[i][1] - centroid[1], v[i][2] - centroid[2]};
+        vx[i] = dot(nx, d);
+        vy[i] = dot(ny, d);
+    }
+    vx[NS] = vx[0];
+    vy[NS] = vy[0];
+    double dxs[4], dys[4], ds[4], ms[4];
+    for (int i = 0; i < NS; ++i) {
+        dxs[i] = vx[i + 1] - vx[i];
+        dys[i] = vy[i + 1] - vy[i];
+        ds[i] = std::sqrt(dxs[i] * dxs[i] + dys[i] * dys[i]);
+        ms[i] = dys[i] / dxs[i];
+    }
+    double dz = tR[2];
+    if (dz == 0.0) dz = EPS;
+    const double dz2 = dz * dz;
+    double rxs[5], rys[5], es[5], hs[5], rs[5], rxr[5], ryr[5];
+    for (int i = 0; i <= NS; ++i) {
+        rxs[i] = tR[0] - vx[i];
+        rys[i] = tR[1] - vy[i];
+        es[i] = rxs[i] * rxs[i] + dz2;
+        hs[i] = rxs[i] * rys[i];
+        const double dy = vy[i] - tR[1];
+        rs[i] = std::sqrt(es[i] + dy * dy);
+        rxr[i] = rxs[i] / rs[i];
+        ryr[i] = rys[i] / rs[i];
+    }
+    double pot = 0.0, u[3] = {0, 0, 0}, h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < NS; ++i) {
+        double num = rs[i] + rs[i + 1] - ds[i];
+        if (num == 0.0) num += EPS;
+        const double log_term = std::log(num / (rs[i] + rs[i + 1] + ds[i]));
+        double ri = rs[i];
+        if (ri == 0.0) ri += EPS;
+        double rip1 = rs[i + 1];
+        if (rip1 == 0.0) rip1 += EPS;
+        const double tan_term = std::atan((ms[i] * es[i] - hs[i]) / dz / ri) - std::atan((ms[i] * es[i + 1] - hs[i + 1]) / dz / rip1);
+        const double dx = dxs[i], dy = dys[i];
+        const double r_plus = rs[i] + rs[i + 1], r_times = rs[i] * rs[i + 1];
+        const double rho = r_times + rxs[i] * rxs[i + 1] + rys[i] * rys[i + 1] + dz2;
+        const double lambda = rxs[i] * rys[i + 1] - rxs[i + 1] * rys[i];
+        if (!dipole) {
+            if (PS) {
+                pot += (rxs[i] * dy - rys[i] * dx) / ds[i] * log_term;
+                pot += dz * tan_term;
+            }
+            if (GS) {
+                u[0] += dy / ds[i] * log_term;
+                u[1] += -dx / ds[i] * log_term;
+                u[2] += tan_term;
+            }
+            if (HS) {
+                const double d2 = ds[i] * ds[i];
+                const double val1 = r_plus * r_plus - d2;
+                const double val2 = rxr[i] + rxr[i + 1], val3 = ryr[i] + ryr[i + 1];
+                const double val4 = r_plus / (r_times * rho);
+                const double xx = 2 * dy / val1 * val2, xy = -2 * dx / val1 * val2, xz = dz * dy * val4;
+                const double yy = -2 * dx / val1 * val3, yz = -dz * dx * val4, zz = lambda * val4;
+                const double m[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+                for (int k = 0; k < 9; ++k) h[k] += m[k];
+            }
+        } else {
+            if (PS) pot -= tan_term;
+            if (GS) {
+                const double val4 = r_plus / (r_times * rho);
+                u[0] += dz * dys[i] * val4;
+                u[1] += -dz * dxs[i] * val4;
+                u[2] += lambda * val4;
+            }
+            if (HS) {
+                const double rp2 = r_plus * r_plus;
+                double val1 = r_times * rp2 + rho * rs[i + 1] * rs[i + 1];
+                val1 /= rho * rs[i] * r_plus;
+                double val2 = r_times * rp2 + rho * rs[i] * rs[i];
+                val2 /= rho * rs[i + 1] * r_plus;
+                const double val3 = r_plus / (rho * r_times * r_times);
+                const double xx = dz * dys[i] * val3 * (rxs[i] * val1 + rxs[i + 1] * val2);
+                const double xy = dz * dys[i] * val3 * (rys[i] * val1 + rys[i + 1] * val2);
+                const double yy = -dz * dxs[i] * val3 * (rys[i] * val1 + rys[i + 1] * val2);
+                const double val4 = rp2 / rho;
+                const double val5 = (rs[i] * rs[i] - r_times + rs[i + 1] * rs[i + 1]) / r_times;
+                const double val6 = dz * (val4 + val5);
+                const double zz = lambda * val3 * val6;
+                const double val7 = r_times - dz * val6;
+                const double val8 = val3 * val7;
+                const double xz = -dys[i] * val8, yz = dxs[i] * val8;
+                const double m[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+                for (int k = 0; k < 9; ++k) h[k] += m[k];
+            }
+        }
+    }
+    const double s4 = strength * ONE_OVER_4PI;
+    phi = PS ? pot * s4 : 0.0;
+    const double sv = dipole ? s4 : -s4, sh = -s4;
+    // R u: sum over columns of R
+    for (int a = 0; a < 3; ++a) vel[a] = GS ? sv * (Rc[0][a] * u[0] + Rc[1][a] * u[1] + Rc[2][a] * u[2]) : 0.0;
+    // R h R': (R h R')[a][b] = sum_jk R[a][j] h[j][k] R[b][k]; h is symmetric
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double acc = 0.0;
+            if (HS)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) acc += Rc[j][a] * h[3 * j + k] * Rc[k][b];
+            H[3 * a + b] = sh * acc;
+        }
+}
+
+// unit normal of a panel from its first three vertices (get_normal, src/compatibility.jl:65-72)
+static void panel_normal(const double (*v)[3], double* n) {
+    const double a[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const double b[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; ++k) n[k] /= nn;
+}
+
+static void direct_panels(int kind, double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
+                          bool PS, bool GS, bool HS) {
+    const int NV = panel_vertices(kind);
+    for (int64_t js = s0; js < s0 + ns; ++js) {
+        const double* s = sbuf + (size_t)js * sr;
+        double v[4][3];
+        for (int k = 0; k < NV; ++k)
+            for (int a = 0; a < 3; ++a) v[k][a] = s[5 + 3 * k + a];
+        double n[3];
+        panel_normal(v, n);
+        for (int64_t it = t0; it < t0 + nt; ++it) {
+            double* t = tbuf + (size_t)it * 16;
+            double phi, vel[3], H[9];
+            panel_induced(t, v, NV, n, s[4], s, panel_dipole(kind), PS, GS, HS, phi, vel, H);
+            if (PS) t[3] += phi;
+            if (GS) for (int a = 0; a < 3; ++a) t[4 + a] += vel[a];
+            if (HS) for (int k = 0; k < 9; ++k) t[7 + k] += H[k];
+        }
+    }
+}
+
+// body_to_multipole! for panels (src/bodytomultipole.jl:643-715): triangles, and quadrilaterals as
+// the triangles (v1, v2, v3) and (v1, v3, v4). The reference's quadrilateral path leaves the second
+// triangle's normal unnormalised (:705 discards the division); here it is normalised, so that the
+// dipole expansion of a quadrilateral matches its induced() field.
+static void p2m_panel(int kind, Exp M, Exp harm, const double* s, const double* center, int p) {
+    const int NV = panel_vertices(kind);
+    double v[4][3];
+    for (int k = 0; k < NV; ++k)
+        for (int a = 0; a < 3; ++a) v[k][a] = s[5 + 3 * k + a];
+    for (int tri = 0; tri < NV - 2; ++tri) {
+        const double* a = v[0];
+        const double* b = v[tri + 1];
+        const double* c = v[tri + 2];
+        const double x0[3] = {a[0] - center[0], a[1] - center[1], a[2] - center[2]};
+        const double xu[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double xv[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        if (!panel_dipole(kind)) p2m_panel_source(M, harm, x0, xu, xv, s[4], p);
+        else {
+            const double vv[3][3] = {{a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}};
+            double n[3];
+            panel_normal(vv, n);
+            p2m_panel_dipole(M, harm, x0, xu, xv, n, s[4], p);
+        }
+    }
+}
+
 static void user_direct(int kind, double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
                         bool PS, bool GS, bool HS) {
+    if (is_panel(kind)) { direct_panels(kind, tbuf, t0, nt, sbuf, sr, s0, ns, PS, GS, HS); return; }
     if (kind == KIND_SOURCE) direct_gravitational(tbuf, t0, nt, sbuf, sr, s0, ns);
     else if (kind == KIND_VORTEX) direct_vortex(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
     else if (kind == KIND_DIPOLE) direct_dipole(tbuf, t0, nt, sbuf, sr, s0, ns, PS, GS, HS);
     else direct_vortex_gauss(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
 }
 
-static int data_per_body(int kind) { return kind == KIND_SOURCE ? 5 : 7; }
+static int data_per_body(int kind) { return kind == KIND_SOURCE ? 5 : is_panel(kind) ? 5 + 3 * panel_vertices(kind) : 7; }
 static bool has_vector_potential(int kind) { return kind == KIND_VORTEX || kind == KIND_VORTEX_GAUSS; }
 
 // ------------------------------------------------------------ fmm driver
@@ -607,7 +801,8 @@ static void fmm_run(Run& R, bool estimate_pass) {
         for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
             const double* x = &R.st.buffer[(size_t)i * dpb];
             double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
-            if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
+            if (is_panel(R.kind)) p2m_panel(R.kind, M, Exp{harm.data()}, x, b.source_center, p);
+            else if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
             else if (R.kind == KIND_DIPOLE) p2m_point_dipole(M, Exp{harm.data()}, dx, x + 4, p);
             else p2m_point_vortex(M, Exp{harm.data()}, dx, x + 4, p);
         }
@@ -876,6 +1071,12 @@ void orc_l2p(const double* dx, const double* local, int p, int LH, int PS, int G
     for (int k = 0; k < 3; ++k) out13[1 + k] = v[k];
     for (int k = 0; k < 9; ++k) out13[4 + k] = h[k];
 }
+// panel row s = [centroid(3), radius, strength, v1, v2, v3, (v4)] of kind 4..7; accumulates into out
+void orc_p2m_panel(int kind, const double* s, const double* center, int p, double* out) {
+    std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
+    p2m_panel(kind, Exp{out}, Exp{harm.data()}, s, center, p);
+}
+
 // kind 0 source (strength[0]), 1 vortex (strength[0..2]), 2 dipole (strength[0..2]); accumulates into out
 void orc_p2m(int kind, const double* dx, const double* strength, int p, double* out) {
     std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);

@@ -1055,6 +1055,102 @@ static void source_to_dipole(Exp h, int i_dipole, int i_source, double qx, doubl
     }
 }
 
+// p_n^m / j_n^m of a triangle (src/bodytomultipole.jl:146-176): index 2, from q (index 1)
+static void calculate_pj(Exp h, double xr, double xi, double er, double ei, double z, int p) {
+    int i = 1;
+    h(1, 2, i) = 1.0;
+    h(2, 2, i) = 0.0;
+    i += 1;
+    for (int n = 1; n <= p; ++n) {
+        int i_nm1_m = i - n;
+        double one_m = 1.0;
+        for (int m = 0; m <= n; ++m) {
+            double q[6];
+            get_nm1(h, 2, n, m, i_nm1_m, one_m, q);
+            const double qr = h(1, 1, i), qi = h(2, 1, i);
+            h(1, 2, i) = (-(xr * q[1] + xi * q[0]) - (er * q[5] + ei * q[4]) - z * q[2] + qr) / (double)(n + 1);
+            h(2, 2, i) = ((xr * q[0] - xi * q[1]) + (er * q[4] - ei * q[5]) - z * q[3] + qi) / (double)(n + 1);
+            i += 1;
+            i_nm1_m += 1;
+            one_m = -one_m;
+        }
+    }
+}
+
+// i_n^m of a triangle (:183-213): index 1, from j (index 2)
+static void calculate_ib(Exp h, double xr, double xi, double er, double ei, double z, int p) {
+    int i = 1;
+    h(1, 1, i) = 0.5;
+    h(2, 1, i) = 0.0;
+    i += 1;
+    for (int n = 1; n <= p; ++n) {
+        int i_nm1_m = i - n;
+        double one_m = 1.0;
+        for (int m = 0; m <= n; ++m) {
+            double q[6];
+            get_nm1(h, 1, n, m, i_nm1_m, one_m, q);
+            const double jr = h(1, 2, i), ji = h(2, 2, i);
+            h(1, 1, i) = (-(xr * q[1] + xi * q[0]) - (er * q[5] + ei * q[4]) - z * q[2] + jr) / (double)(n + 2);
+            h(2, 1, i) = ((xr * q[0] - xi * q[1]) + (er * q[4] - ei * q[5]) - z * q[3] + ji) / (double)(n + 2);
+            i += 1;
+            i_nm1_m += 1;
+            one_m = -one_m;
+        }
+    }
+}
+
+// the three recurrences of a triangle x0 (relative to the expansion centre), x0+xu, x0+xv
+static void panel_harmonics(Exp h, const double* x0, const double* xu, const double* xv, int p) {
+    // xyz_to_xi_eta_z (:39-46) of each point, added componentwise as the reference does
+    const double x0r = x0[0] * 0.5, x0i = x0[1] * 0.5, e0r = x0r, e0i = -x0i;
+    const double xur = xu[0] * 0.5, xui = xu[1] * 0.5, eur = xur, eui = -xui;
+    const double xvr = xv[0] * 0.5, xvi = xv[1] * 0.5, evr = xvr, evi = -xvi;
+    calculate_q(h, x0r + xur, x0i + xui, e0r + eur, e0i + eui, x0[2] + xu[2], p);
+    calculate_pj(h, x0r + xvr, x0i + xvi, e0r + evr, e0i + evi, x0[2] + xv[2], p);
+    calculate_ib(h, x0r, x0i, e0r, e0i, x0[2], p);
+}
+
+static double cross_norm(const double* u, const double* v) {
+    const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+}
+
+void p2m_panel_source(Exp M, Exp h, const double* x0, const double* xu, const double* xv, double strength, int p) {   // :722-774
+    panel_harmonics(h, x0, xu, xv, p);
+    const double Jq = cross_norm(xu, xv) * (-strength);
+    int i = 1;
+    double one_n = 1.0;
+    for (int n = 0; n <= p; ++n) {
+        double one_n_m = one_n;
+        for (int m = 0; m <= n; ++m) {
+            M(1, 1, i) += Jq * one_n_m * h(1, 1, i);
+            M(2, 1, i) -= Jq * one_n_m * h(2, 1, i);
+            i += 1;
+            one_n_m = -one_n_m;
+        }
+        one_n = -one_n;
+    }
+}
+
+void p2m_panel_dipole(Exp M, Exp h, const double* x0, const double* xu, const double* xv, const double* normal,
+                      double strength, int p) {   // :776-831
+    panel_harmonics(h, x0, xu, xv, p);
+    source_to_dipole(h, 2, 1, -normal[0] * strength, -normal[1] * strength, -normal[2] * strength, p);
+    const double J = cross_norm(xu, xv);
+    int i = 1;
+    double one_n = 1.0;
+    for (int n = 0; n <= p; ++n) {
+        double one_n_m = one_n;
+        for (int m = 0; m <= n; ++m) {
+            M(1, 1, i) += J * one_n_m * h(1, 2, i);
+            M(2, 1, i) -= J * one_n_m * h(2, 2, i);
+            i += 1;
+            one_n_m = -one_n_m;
+        }
+        one_n = -one_n;
+    }
+}
+
 void p2m_point_source(Exp M, Exp h, const double* dx, double q, int p) {   // :403-429
     double xr = dx[0] * 0.5, xi = dx[1] * 0.5, er = xr, ei = -xi, z = dx[2];
     double s = -q;

@@ -108,6 +108,11 @@ void evaluate_local(const double* dx, Exp harmonics, std::vector<double>& grad_n
 // body_to_multipole_point! for Point{Source}/Point{Dipole}/Point{Vortex}; src/bodytomultipole.jl:403-496
 void p2m_point_source(Exp M, Exp harm, const double* dx, double q, int p);
 void p2m_point_dipole(Exp M, Exp harm, const double* dx, const double* q, int p);
+// body_to_multipole_panel! for Panel{Source} / Panel{Dipole} triangles (src/bodytomultipole.jl:722-831);
+// x0 = first vertex relative to the expansion centre, xu = v2 - v1, xv = v3 - v1; harm needs 4 H(p) doubles
+void p2m_panel_source(Exp M, Exp harm, const double* x0, const double* xu, const double* xv, double strength, int p);
+void p2m_panel_dipole(Exp M, Exp harm, const double* x0, const double* xu, const double* xv, const double* normal,
+                      double strength, int p);
 void p2m_point_vortex(Exp M, Exp harm, const double* dx, const double* g, int p);
 
 double factorial_to_double(int k);   // Float64(factorial(big(k))), correctly rounded

@@ -125,6 +125,15 @@ FMM_CASES = [
      dict(expansion_order=3, leaf_size_source=10, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
     ("gauss-ring-6k-p10", lambda: W.gaussian_vortex_ring(6000, seed=31), O.KIND_VORTEX_GAUSS,
      dict(expansion_order=10, leaf_size_source=64, multipole_acceptance=0.4, hessian=True)),
+    # constant-strength panels tiling a sphere (quadrilaterals, and triangles from splitting them)
+    ("panel-src-tri-3k-p8", lambda: W.sphere_panels(16, quads=False)[0], O.KIND_PANEL_SOURCE_TRI,
+     dict(expansion_order=8, leaf_size_source=20, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("panel-dip-tri-3k-p8", lambda: W.sphere_panels(16, quads=False)[0], O.KIND_PANEL_DIPOLE_TRI,
+     dict(expansion_order=8, leaf_size_source=20, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("panel-src-quad-1.5k-p6", lambda: W.sphere_panels(16)[0], O.KIND_PANEL_SOURCE_QUAD,
+     dict(expansion_order=6, leaf_size_source=16, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("panel-dip-quad-1.5k-p10", lambda: W.sphere_panels(16)[0], O.KIND_PANEL_DIPOLE_QUAD,
+     dict(expansion_order=10, leaf_size_source=16, multipole_acceptance=0.4, scalar_potential=True, hessian=True)),
 ]
 
 
@@ -194,7 +203,9 @@ def test_direct_fp64_matches_oracle():
     _say("[gpu] direct fp64")
     for kind, sbuf in ((O.KIND_SOURCE, W.uniform_sources(3000, seed=41)), (O.KIND_VORTEX, W.vortex_ring(3000, seed=42)),
                        (O.KIND_DIPOLE, W.uniform_dipoles(3000, seed=43)),
-                       (O.KIND_VORTEX_GAUSS, W.gaussian_vortex_ring(3000, seed=44))):
+                       (O.KIND_VORTEX_GAUSS, W.gaussian_vortex_ring(3000, seed=44)),
+                       (O.KIND_PANEL_SOURCE_TRI, W.sphere_panels(10, quads=False)[0]),
+                       (O.KIND_PANEL_DIPOLE_QUAD, W.sphere_panels(12)[0])):
         ctx = F.default_context()
         txyz = np.ascontiguousarray(sbuf[:, :3])
         out = np.zeros((len(sbuf), 16))

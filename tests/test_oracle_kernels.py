@@ -112,3 +112,102 @@ def test_fmm_converges_to_direct(kind):
     for lo, hi in ((4, 7), (7, 16)):
         err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / np.abs(ref[:, lo:hi]).max()
         assert err < 1e-6, (kind, lo, err)
+
+
+# ------------------------------------------------------------------ panels
+def _panel_row(verts, strength):
+    verts = [np.asarray(v, float) for v in verts]
+    cen = sum(verts[:3]) * 0.33333333333333333 if len(verts) == 3 else sum(verts) / len(verts)
+    rad = max(np.linalg.norm(v - cen) for v in verts)
+    return np.concatenate([cen, [rad, strength], *verts])
+
+
+_TRI = [np.array([0.0, 0.29, 0.0]), np.array([0.0, 0.59, 0.0]), np.array([-0.5, 0.59, 0.0])]   # test/bodytomultipole_test.jl:905-907
+
+
+@pytest.mark.parametrize("kind", [O.KIND_PANEL_SOURCE_TRI, O.KIND_PANEL_DIPOLE_TRI])
+def test_panel_expansion_chain_matches_induced(kind):
+    """test/bodytomultipole_test.jl:901-1095: panel P2M -> M2L -> L2P equals induced() (velocity atol 1e-12)."""
+    row = _panel_row(_TRI, 1.0)
+    xt = np.array([0.2, 0.3, 5.7])
+    p = 10
+    center = row[:3] + np.array([0.1, 0.2, 0.08])
+    M = O.p2m_panel(kind, row, center, p)
+    tc = xt + np.array([0.0001, -0.0002, 0.003])
+    L, _, _ = O.m2l(O.geom(tc), O.geom(center), M, p)
+    phi, v, h = O.l2p(xt - tc, L, p)
+    ref = O.direct(kind, xt[None, :], row[None, :], scalar_potential=True, gradient=True, hessian=True)[0]
+    np.testing.assert_allclose(v, ref[4:7], atol=1e-12)
+    assert phi == pytest.approx(ref[3], rel=1e-10)
+    np.testing.assert_allclose(np.asarray(h).reshape(-1), ref[7:16], atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [O.KIND_PANEL_SOURCE_QUAD, O.KIND_PANEL_DIPOLE_QUAD])
+def test_quad_panel_expansion_chain_matches_induced(kind):
+    quad = [np.array([0.1, 0.2, 0.0]), np.array([0.5, 0.25, 0.02]), np.array([0.45, 0.6, 0.04]), np.array([0.05, 0.55, 0.02])]
+    # make the quadrilateral planar: project the 4th vertex onto the plane of the first three
+    n = np.cross(quad[1] - quad[0], quad[2] - quad[0]); n /= np.linalg.norm(n)
+    quad[3] = quad[3] - np.dot(quad[3] - quad[0], n) * n
+    row = _panel_row(quad, -0.7)
+    xt = np.array([1.9, -2.2, 3.1])
+    p = 14
+    center = row[:3] + np.array([-0.05, 0.03, 0.02])
+    M = O.p2m_panel(kind, row, center, p)
+    tc = xt + np.array([0.01, 0.02, -0.03])
+    L, _, _ = O.m2l(O.geom(tc), O.geom(center), M, p)
+    phi, v, h = O.l2p(xt - tc, L, p)
+    ref = O.direct(kind, xt[None, :], row[None, :], scalar_potential=True, gradient=True, hessian=True)[0]
+    assert phi == pytest.approx(ref[3], rel=1e-9)
+    np.testing.assert_allclose(v, ref[4:7], rtol=1e-8, atol=1e-12 * np.abs(ref[4:7]).max())
+    np.testing.assert_allclose(np.asarray(h).reshape(-1), ref[7:16], rtol=1e-7, atol=1e-10 * np.abs(ref[7:16]).max())
+
+
+@pytest.mark.parametrize("kind", [O.KIND_PANEL_SOURCE_TRI, O.KIND_PANEL_DIPOLE_TRI])
+def test_panel_far_field_and_derivatives(kind):
+    """Far away a panel is a point source of strength q*area (or a dipole q*area*n); near it the
+    velocity slot is -grad(phi) and the Hessian slot is d(velocity slot), by finite differences."""
+    row = _panel_row(_TRI, 0.8)
+    area = 0.5 * np.linalg.norm(np.cross(_TRI[1] - _TRI[0], _TRI[2] - _TRI[0]))
+    n = np.cross(_TRI[1] - _TRI[0], _TRI[2] - _TRI[0]); n /= np.linalg.norm(n)
+    xt = row[:3] + np.array([300.0, -200.0, 500.0])
+    got = O.direct(kind, xt[None, :], row[None, :], scalar_potential=True, gradient=True, hessian=True)[0]
+    if kind == O.KIND_PANEL_SOURCE_TRI:
+        pt = O.direct(O.KIND_SOURCE, xt[None, :], np.array([[*row[:3], 0.0, 0.8 * area]]), scalar_potential=True)[0]
+    else:
+        pt = O.direct(O.KIND_DIPOLE, xt[None, :], np.array([[*row[:3], 0.0, *(0.8 * area * n)]]), scalar_potential=True,
+                      gradient=True)[0]
+    assert got[3] == pytest.approx(pt[3], rel=1e-5)
+    np.testing.assert_allclose(got[4:7], pt[4:7], rtol=1e-4, atol=1e-6 * np.abs(pt[4:7]).max())
+    xt = np.array([0.05, 0.41, 0.13])
+    out = O.direct(kind, xt[None, :], row[None, :], scalar_potential=True, gradient=True, hessian=True)[0]
+    h = 1e-6
+    for a in range(3):
+        e = np.zeros(3); e[a] = h
+        fp = O.direct(kind, (xt + e)[None, :], row[None, :], scalar_potential=True, gradient=True)[0]
+        fm = O.direct(kind, (xt - e)[None, :], row[None, :], scalar_potential=True, gradient=True)[0]
+        assert -(fp[3] - fm[3]) / (2 * h) == pytest.approx(out[4 + a], rel=1e-6, abs=1e-9)
+        np.testing.assert_allclose((fp[4:7] - fm[4:7]) / (2 * h), out[7 + 3 * a:10 + 3 * a], rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("kind", [O.KIND_PANEL_SOURCE_TRI, O.KIND_PANEL_DIPOLE_QUAD])
+def test_panel_fmm_converges_to_direct(kind):
+    rng = np.random.default_rng(8)
+    n = 800
+    nv = 4 if kind >= O.KIND_PANEL_SOURCE_QUAD else 3
+    rows = []
+    for _ in range(n):
+        c = rng.random(3)
+        u = rng.normal(size=3); u /= np.linalg.norm(u)
+        w = np.cross(u, rng.normal(size=3)); w /= np.linalg.norm(w)
+        s = 0.01 + 0.01 * rng.random()
+        ang = np.sort(rng.random(nv)) * 2 * math.pi if nv == 3 else np.array([0.1, 1.6, 3.2, 4.7])
+        verts = [c + s * (math.cos(t) * u + math.sin(t) * w) for t in ang]
+        rows.append(_panel_row(verts, rng.normal() / n))
+    src = np.array(rows)
+    run = O.Run(kind, src[:, :3], src, p=14, leaf_size_source=20, multipole_acceptance=0.4, scalar_potential=True,
+                gradient=True, hessian=True).fmm()
+    got = run.targets()
+    ref = O.direct(kind, src[:, :3], src, scalar_potential=True, gradient=True, hessian=True)
+    for lo, hi in ((3, 4), (4, 7), (7, 16)):
+        err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / np.abs(ref[:, lo:hi]).max()
+        assert err < 1e-6, (kind, lo, err)
