@@ -23,6 +23,8 @@
 // expansion order follow the reference's arithmetic; the rotation and
 // translation products use explicit fma.
 #include "fmmhip_internal.hpp"
+#include <cstdlib>
+#include <cstring>
 #include <algorithm>
 #include <cmath>
 
@@ -50,7 +52,7 @@ __device__ inline void idx_nm(int idx, int& n, int& m) {
 // ============================================================== tables
 struct Tables {
     int p = -1;
-    DBuf<double> Hs, zeta, eta, Mt, Lt, inv_fact;
+    DBuf<double> Hs, zeta, eta, Mt, Lt, inv_fact, fact;
     DBuf<double> Ad;   // dense, even-padded pi/2 tables per degree (rotation kernels)
 };
 
@@ -167,7 +169,7 @@ void ensure_tables(Context& C, int p) {
     Tables& T = *C.tables;
     if (T.p >= PMAX) return;
     const int pm = PMAX;
-    std::vector<double> Hs, z, e, Mt(nh(pm)), Lt(nh(pm)), invf(2 * pm + 4);
+    std::vector<double> Hs, z, e, Mt(nh(pm)), Lt(nh(pm)), invf(2 * pm + 4), fct(2 * pm + 4);
     host_Hs(Hs, pm);
     host_mags(z, e, pm);
     Mt[0] = Lt[0] = 1.0;
@@ -177,7 +179,10 @@ void ensure_tables(Context& C, int p) {
             Mt[i] = std::sqrt(a * b / (double)(2 * n + 1));
             Lt[i] = std::sqrt(1.0 / (a * b * (double)(2 * n + 1)));
         }
-    for (int k = 0; k < (int)invf.size(); ++k) invf[k] = 1.0 / host_factorial(k);
+    for (int k = 0; k < (int)invf.size(); ++k) {
+        fct[k] = host_factorial(k);
+        invf[k] = 1.0 / fct[k];
+    }
     auto up = [&](DBuf<double>& d, const std::vector<double>& h) {
         d.resize(h.size());
         HIP_CHECK(hipMemcpy(d.data(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -193,18 +198,18 @@ void ensure_tables(Context& C, int p) {
                 Ad.push_back(b <= n ? Hn[(hi * (hi + 1)) / 2 + lo] : 0.0);
             }
     }
-    up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf); up(T.Ad, Ad);
+    up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf); up(T.fact, fct); up(T.Ad, Ad);
     T.p = pm;
 }
 
 void free_tables(Tables* t) { delete t; }
 
 struct TablePtrs {
-    const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact, *Ad;
+    const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact, *fact, *Ad;
 };
 static TablePtrs ptrs(Context& C) {
     Tables& T = *C.tables;
-    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data(), T.Ad.data()};
+    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data(), T.fact.data(), T.Ad.data()};
 }
 
 // ====================================================== rotation engine
@@ -1106,6 +1111,597 @@ static void launch_m2l(Context& C, int p, const Lists& Ls, const Tree& st, Tree&
     LAUNCH_CHECK();
 }
 
+// ============================================ M2L on matrix cores (FP64 MFMA)
+// The factorised rotation above applies the same pi/2 table A_n to every
+// interaction, so a batch of interactions turns each of its products into a
+// dense matrix product with a shared left operand: the batch's coefficients
+// form the columns of X (16 columns = BP interactions x NC real components:
+// phi re/im and, with Lamb-Helmholtz terms, chi re/im), and v_mfma_f64_16x16x4f64
+// multiplies 16-row tiles of A_n into them. The z-translation is a shared
+// product too: F(n+k) = (n+k)! t^-(n+1) t^-k, so its Hankel matrix (n+k)! is
+// common and the t powers scale the columns before and the rows after it.
+// Everything else (z-rotations, the P+- / Q+- combination, Lamb-Helmholtz
+// terms, the error tests) is elementwise per column on the vector ALU.
+//
+// Lane layout of v_mfma_f64_16x16x4f64 (D = A B + C, 16 x 4 times 4 x 16): lane l
+// supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds D(l / 16 + 4 r, l % 16)
+// in element r of its result.
+//
+// One workgroup per target box: it takes the box's sources in batches of BP
+// consecutive list entries, so every column of a batch accumulates into the same
+// local expansion, and its wavefronts share each batch (one degree or one column
+// m at a time each); the tables live in LDS. The error-controlled order is found in groups of
+// degrees: the group is rotated, then one lane per (interaction, degree) runs
+// the reference's tests for that degree (the tests of m2l_test_chunk), and each
+// interaction takes its first passing degree. Every interaction is then
+// translated and rotated back at its own order: its columns are masked above
+// that order, so each result is the one of an M2L truncated there.
+
+typedef double mf4 __attribute__((ext_vector_type(4)));
+
+__device__ inline mf4 mfma_f64(double a, double b, mf4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+enum : int { MF_EMPTY = -1, MF_SEARCHING = -2 };
+static constexpr int MF_GROUP = 8;   // degrees rotated between two rounds of error tests
+
+// LDS of one workgroup: the tables every product reads, the batch and the
+// target's accumulator, and one product buffer per wavefront
+struct MfLds {
+    double* Hs;    // len_Hs(P): pi/2 tables (packed lower triangles, read as symmetric A_n)
+    double* zin;   // H: zeta input scale g(m)      (zeta(0, m))
+    double* zout;  // H: zeta output scale 1/g(m)   (zeta(m, 0))
+    double* ein;   // H: eta input scale            (eta(0, m))
+    double* eout;  // H: eta output scale           (eta(m, 0))
+    double* fact;  // 2P+2: k!
+    double* X;     // 16 H: coefficients of the batch, row = (n, m), column = (interaction, component)
+    double* acc;   // 4 H: partial local expansion of the target box
+    double* eim;   // BP x (P+1) x 2: e^{i m phi}
+    double* cs;    // BP x (P+1) x 2: cos / sin (v beta)
+    double* fac;   // BP x (P+1) x 6: running factors of the error tests
+    double* pw;    // BP x (P+1) x 2: multipole power per degree
+    double* rp;    // BP x (P+2): t^-k
+    double* geo;   // BP x 8: t, 1/t, r_mp, 1/r_mp, dx, dy, dz
+    int* ord;      // BP: order of the interaction (MF_EMPTY, MF_SEARCHING)
+    int* js;       // BP: source branch
+    double* T;     // 32 (P+1) for this wavefront: step-1 products R^e / R^o, z-translation rows
+};
+
+__host__ __device__ inline int mf_lds_doubles(int P, int BP, int NW) {
+    const int H = nh(P);
+    return len_Hs(P) + 4 * H + 2 * P + 2 + 16 * H + 4 * H + BP * (P + 1) * 12 + BP * (P + 2) + 8 * BP + BP + NW * 32 * (P + 1);
+}
+static size_t mf_lds_bytes(int P, int BP, int NW) { return sizeof(double) * (size_t)mf_lds_doubles(P, BP, NW); }
+
+__device__ inline MfLds mf_carve(double* b, int P, int BP) {
+    MfLds W;
+    const int H = nh(P);
+    W.Hs = b; b += len_Hs(P);
+    W.zin = b; b += H;
+    W.zout = b; b += H;
+    W.ein = b; b += H;
+    W.eout = b; b += H;
+    W.fact = b; b += 2 * P + 2;
+    W.X = b; b += 16 * H;
+    W.acc = b; b += 4 * H;
+    W.eim = b; b += BP * (P + 1) * 2;
+    W.cs = b; b += BP * (P + 1) * 2;
+    W.fac = b; b += BP * (P + 1) * 6;
+    W.pw = b; b += BP * (P + 1) * 2;
+    W.rp = b; b += BP * (P + 2);
+    W.geo = b; b += 8 * BP;
+    W.ord = (int*)b; b += BP;
+    W.js = W.ord + BP;
+    W.T = b + (size_t)wave_id() * 32 * (P + 1);
+    return W;
+}
+
+// A_n(a, b) from the packed table of degree n
+__device__ inline double a_sym(const double* Hn, int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return Hn[((hi * (hi + 1)) >> 1) + lo];
+}
+
+// per-interaction set-up of a batch: geometry, e^{i k phi}, (cos, sin)(k beta),
+// t^-k and, for the error-controlled forms, the test factors (fill_factors)
+template <int BP>
+__device__ void mf_setup(const MfLds& W, const int32_t* __restrict__ m2l_s, int64_t li0, int nb, const Branch& T,
+                         const Branch* __restrict__ sbr, int P, const ErrParams& E) {
+    const int l = threadIdx.x;
+    if (l < BP) {
+        const bool valid = l < nb;
+        const int32_t js = valid ? m2l_s[li0 + l] : 0;
+        W.js[l] = js;
+        W.ord[l] = valid ? (E.kind == 0 ? P : MF_SEARCHING) : MF_EMPTY;
+        double* g = W.geo + 8 * l;
+        if (valid) {
+            const Branch& S = sbr[js];
+            const double dx = T.target_center[0] - S.source_center[0], dy = T.target_center[1] - S.source_center[1],
+                         dz = T.target_center[2] - S.source_center[2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double d0 = md1(dx, -T.target_box[0], T.target_box[0]);
+            const double d1 = md1(dy, -T.target_box[1], T.target_box[1]);
+            const double d2 = md1(dz, -T.target_box[2], T.target_box[2]);
+            const double r_mp = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            g[0] = r; g[1] = 1.0 / r; g[2] = r_mp; g[3] = 1.0 / r_mp; g[4] = dx; g[5] = dy; g[6] = dz;
+        } else {
+            g[0] = 1.0; g[1] = 1.0; g[2] = 1.0; g[3] = 1.0; g[4] = 0.0; g[5] = 0.0; g[6] = 1.0;
+        }
+    }
+    __syncthreads();
+    for (int it = l; it < BP * (P + 2); it += blockDim.x) {
+        const int q = it / (P + 2), k = it - q * (P + 2);
+        const double* g = W.geo + 8 * q;
+        double f = 1.0;
+        for (int j = 0; j < k; ++j) f *= g[1];
+        W.rp[it] = f;
+        if (k > P) continue;
+        const double x = g[4], y = g[5], z = g[6];
+        const double x2y2 = x * x + y * y;
+        double cp = 1.0, sp = 0.0, cb = z < 0.0 ? -1.0 : 1.0, sb = 0.0;
+        if (x2y2 > 0.0) {
+            const double rho = sqrt(x2y2);
+            cp = x / rho; sp = y / rho;
+            cb = z / g[0]; sb = rho / g[0];
+        }
+        double er = 1.0, ei = 0.0, c = 1.0, s = 0.0;
+        for (int j = 0; j < k; ++j) {
+            const double t = er;
+            er = er * cp - ei * sp;
+            ei = t * sp + ei * cp;
+            const double cn = cb * c - sb * s;
+            const double sn = sb * c + cb * s;
+            c = cn;
+            s = sn;
+        }
+        const int o = 2 * (q * (P + 1) + k);
+        W.eim[o] = er; W.eim[o + 1] = ei;
+        W.cs[o] = c; W.cs[o + 1] = s;
+        if (E.kind != 0 && k >= 1) {
+            const double rinv = g[1], r_l = T.target_radius, r_mp_inv = g[3];
+            double n_f_t_np1 = rinv * rinv, r_l_nm1 = 1.0, r_mp_inv_np2 = r_mp_inv * r_mp_inv * r_mp_inv;
+            double nm1_f_inv = 1.0, np1_f = 2.0;
+            for (int j = 1; j < k; ++j) {
+                n_f_t_np1 *= (double)(j + 1) * rinv;
+                r_l_nm1 *= r_l;
+                r_mp_inv_np2 *= r_mp_inv;
+                nm1_f_inv /= (double)j;
+                np1_f *= (double)(j + 1);
+            }
+            double* f6 = W.fac + 6 * (q * (P + 1) + k);
+            f6[0] = n_f_t_np1; f6[1] = r_l_nm1; f6[2] = r_mp_inv_np2; f6[3] = nm1_f_inv; f6[4] = np1_f;
+        }
+    }
+    __syncthreads();
+}
+
+// the batch's sources, all degrees: z-rotation and input scale into X (zrot_in);
+// with POW the multipole power per degree (chunk_powers, summed in m order).
+// One thread per (interaction, degree).
+template <bool LH, bool POW>
+__device__ void mf_load(const MfLds& W, const double* __restrict__ sex, int H4, const double* __restrict__ Mt, int P) {
+    constexpr int NC = LH ? 4 : 2, BP = 16 / NC;
+    for (int it = threadIdx.x; it < BP * (P + 1); it += blockDim.x) {
+        const int q = it / (P + 1), n = it - q * (P + 1);
+        const int i0 = nh(n - 1);
+        double* x = W.X + 16 * i0 + q * NC;
+        if (W.ord[q] == MF_EMPTY) {
+            for (int m = 0; m <= n; ++m)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) x[16 * m + c] = 0.0;
+            continue;
+        }
+        const double* src = sex + (int64_t)W.js[q] * H4 + 4 * i0;
+        const double* eq = W.eim + 2 * q * (P + 1);
+        double a = 0.0, b = 0.0;
+#pragma unroll 4
+        for (int m = 0; m <= n; ++m) {
+            const double gm = W.zin[i0 + m];
+            const double er = eq[2 * m], ei = eq[2 * m + 1];
+            const double2 s01 = *(const double2*)(src + 4 * m);
+            x[16 * m] = gm * (s01.x * er - s01.y * ei);
+            x[16 * m + 1] = gm * (s01.x * ei + s01.y * er);
+            double2 s23 = make_double2(0.0, 0.0);
+            if (LH) {
+                s23 = *(const double2*)(src + 4 * m + 2);
+                x[16 * m + 2] = gm * (s23.x * er - s23.y * ei);
+                x[16 * m + 3] = gm * (s23.x * ei + s23.y * er);
+            }
+            if (POW) {
+                const double Nt = Mt[i0 + m];
+                a += (s01.x * s01.x + s01.y * s01.y) * Nt * Nt;
+                if (LH) b += (s23.x * s23.x + s23.y * s23.y) * Nt * Nt;
+            }
+        }
+        if (POW) {
+            W.pw[2 * (q * (P + 1) + n)] = sqrt(a);
+            W.pw[2 * (q * (P + 1) + n) + 1] = sqrt(b);
+        }
+    }
+    __syncthreads();
+}
+
+// One y-rotation of degree n for the whole batch, by one wavefront (the
+// factorised form above): step 1 SE / SO = A(v, even / odd m') X(m') on the matrix
+// cores, combination into R^e / R^o (T), step 2 out(m) = A(m, v) R^(parity m)(v),
+// scaled by mout(m) / 2. Forward: out -> X (degree n rows). BACK: out -> back
+// z-rotation -> sum over the interactions of the batch -> acc. The operand loops
+// run to fixed bounds (degree <= 30) so that their loads issue ahead of the products.
+template <bool LH, bool BACK>
+__device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigma, int P, int n) {
+    constexpr int NC = LH ? 4 : 2;
+    const int lane = lane_id(), lr = lane & 15, lg = lane >> 4;
+    const int q = lr / NC, c = lr % NC, isim = c & 1, pot = c >> 1;
+    const double* Hn = W.Hs + len_Hs(n - 1);
+    double* Xn = W.X + 16 * nh(n - 1);
+    const double* mo = mout + nh(n - 1);
+    const int ne = (n >> 1) + 1, no = (n + 1) >> 1;
+    const int ce = (ne + 3) >> 2, co = (no + 3) >> 2;
+    const double* csq = W.cs + 2 * q * (P + 1);
+    double* Te = W.T;
+    double* To = W.T + 16 * (P + 1);
+    const double x0r = Xn[q * NC + 2 * pot], x0i = Xn[q * NC + 2 * pot + 1];
+    for (int t = 0; 16 * t <= n; ++t) {
+        mf4 se = {0.0, 0.0, 0.0, 0.0}, so = {0.0, 0.0, 0.0, 0.0};
+        const int va = 16 * t + lr;
+        double ae[4], be[4], ao[4], bo[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const int me = 2 * (4 * cc + lg), mo2 = me + 1;
+            ae[cc] = (cc < ce && va <= n && me <= n) ? a_sym(Hn, va, me) : 0.0;
+            be[cc] = (cc < ce && me <= n) ? Xn[16 * me + lr] : 0.0;
+            ao[cc] = (cc < co && va <= n && mo2 <= n) ? a_sym(Hn, va, mo2) : 0.0;
+            bo[cc] = (cc < co && mo2 <= n) ? Xn[16 * mo2 + lr] : 0.0;
+        }
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            if (cc < ce) se = mfma_f64(ae[cc], be[cc], se);
+            if (cc < co) so = mfma_f64(ao[cc], bo[cc], so);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int v = 16 * t + lg + 4 * r;
+            const double sx = se[r], ox = so[r];
+            const double sp = __shfl_xor(sx, 1, 64), op = __shfl_xor(ox, 1, 64);
+            if (v > n) continue;
+            const double ser = isim ? sp : sx, sei = isim ? sx : sp, sor = isim ? op : ox, soi = isim ? ox : op;
+            const double a00 = Hn[(v * (v + 1)) >> 1];   // A(0, v)
+            const double dr = ser - a00 * x0r, di = sei - a00 * x0i;
+            const double sg = ((n + v) & 1) ? -1.0 : 1.0;
+            const double w = v ? 2.0 : 1.0;
+            const double cb = w * csq[2 * v], sb = w * sigma * csq[2 * v + 1];
+            const double ppr = dr - sor + sg * (ser + sor), ppi = di - soi - sg * (sei + soi);
+            const double pmr = dr + sor + sg * (ser - sor), pmi = di + soi - sg * (sei - soi);
+            const double qpr = cb * ppr - sb * ppi, qpi = cb * ppi + sb * ppr;
+            const double qmr = cb * pmr + sb * pmi, qmi = cb * pmi - sb * pmr;
+            Te[16 * v + lr] = isim ? (qpi + qmi) : (qpr + qmr);
+            To[16 * v + lr] = isim ? (qpi - qmi) : (qpr - qmr);
+        }
+    }
+    wave_sync();
+    const int kc = (n + 4) >> 2;
+    const double* eq = W.eim + 2 * q * (P + 1);
+    mf4 o[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    double a2[2][8], b2[2][8];
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        const int ma = 2 * lr + par;
+        const double* Tp = par ? To : Te;
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) {
+            const int v = 4 * cc + lg;
+            a2[par][cc] = (cc < kc && ma <= n && v <= n) ? a_sym(Hn, ma, v) : 0.0;
+            b2[par][cc] = (cc < kc && v <= n) ? Tp[16 * v + lr] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc)
+        if (cc < kc) {
+            o[0] = mfma_f64(a2[0][cc], b2[0][cc], o[0]);
+            o[1] = mfma_f64(a2[1][cc], b2[1][cc], o[1]);
+        }
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = 2 * (lg + 4 * r) + par;
+            const bool in = m <= n;
+            const double val = in ? 0.5 * mo[m] * o[par][r] : 0.0;
+            if (!BACK) {
+                if (in) Xn[16 * m + lr] = val;
+            } else {
+                const double vp = __shfl_xor(val, 1, 64);
+                const double er = in ? eq[2 * m] : 1.0, ei = in ? -eq[2 * m + 1] : 0.0;   // back_rotate_z!
+                const double re = isim ? vp : val, im = isim ? val : vp;
+                double y = isim ? (re * ei + im * er) : (re * er - im * ei);
+                for (int k = NC; k < 16; k <<= 1) y += __shfl_xor(y, k, 64);
+                if (q == 0 && in) W.acc[4 * (nh(n - 1) + m) + c] += y;
+            }
+        }
+    }
+    wave_sync();
+}
+
+// Error tests of the degrees n0..n1 (n0 >= 1), one lane per (interaction,
+// degree): the tests of m2l_test_chunk, with the degree-n part of the
+// z-translation formed and consumed in local_power's m order inside the lane.
+// Each searching interaction takes its first passing degree d: order d - 1 + BE.
+template <bool LH>
+__device__ void mf_test_group(const MfLds& W, const ErrParams& E, const TablePtrs& tb, double limit, double r_l, int P, int n0,
+                              int n1) {
+    constexpr int NC = LH ? 4 : 2, BP = 16 / NC;
+    const int G = n1 - n0 + 1;   // <= MF_GROUP
+    const int l = lane_id();
+    const int q = l / MF_GROUP, n = n0 + (l - q * MF_GROUP);
+    bool pass = false;
+    if (q < BP && n <= n1 && W.ord[q] == MF_SEARCHING) {
+        const double* g = W.geo + 8 * q;
+        const double r = g[0], rinv = g[1], r_mp = g[2];
+        const double* Xq = W.X + q * NC;
+        const double* F = W.fac + 6 * (q * (P + 1) + n);
+        const double n_f_t_np1 = F[0], r_l_nm1 = F[1], r_mp_inv_np2 = F[2], nm1_f_inv = F[3], np1_f = F[4];
+        const double* pwq = W.pw + 2 * q * (P + 1);
+        const int in0 = nh(n - 1);
+        double eps_mp;
+        if (E.kind == 1) {
+            const double p0 = Xq[16 * in0], p1r = Xq[16 * (in0 + 1)], p1i = Xq[16 * (in0 + 1) + 1];
+            eps_mp = (sqrt(p1r * p1r + p1i * p1i) + fabs(p0)) * np1_f * r_mp_inv_np2;
+            if (LH) {
+                const double c1r = Xq[16 * (in0 + 1) + 2], c1i = Xq[16 * (in0 + 1) + 3];
+                eps_mp += sqrt(c1r * c1r + c1i * c1i) * np1_f * r_mp_inv_np2 * r_mp;
+            }
+        } else {
+            const double pw_prev = pwq[2 * (n - 1)], pw_n = pwq[2 * n], pc_n = pwq[2 * n + 1];
+            const double Mt_n0 = tb.Mt[nh(n - 2)], Mt_next = tb.Mt[in0];
+            if (E.kind == 3) {
+                eps_mp = SQRT3_D * pw_prev * r_mp_inv_np2 * np1_f / Mt_n0;
+                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * r_mp_inv_np2 * np1_f / Mt_next;
+            } else {
+                eps_mp = SQRT3_D * pw_n * r_mp_inv_np2 * r_mp * np1_f / (Mt_n0 * (double)(n + 1));
+                if (LH) eps_mp += SQRT3_D * (double)n * pc_n * r_mp_inv_np2 * r_mp * np1_f / (Mt_next * (double)(n + 1));
+            }
+        }
+        if (eps_mp <= limit) {
+            double eps_l;
+            if (E.kind == 1) {   // translate_multipole_to_local_z_m01_n (:302-362)
+                double n_m_f = n_f_t_np1;
+                int n_m = n;
+                double p0r = 0.0, n_np_f = n_m_f;
+                int n_np = n_m;
+                for (int np = 0; np <= n; ++np) {
+                    p0r += n_np_f * Xq[16 * ((np * (np + 1)) / 2)];
+                    n_np += 1;
+                    n_np_f *= (double)n_np * rinv;
+                }
+                n_m += 1;
+                n_m_f *= (double)n_m * rinv;
+                double p1r = 0, p1i = 0, c1r = 0, c1i = 0;
+                n_np_f = n_m_f;
+                n_np = n_m;
+                for (int np = 1; np <= n; ++np) {
+                    const int iw = (np * (np + 1)) / 2 + 1;
+                    p1r += n_np_f * Xq[16 * iw];
+                    p1i += n_np_f * Xq[16 * iw + 1];
+                    if (LH) { c1r += n_np_f * Xq[16 * iw + 2]; c1i += n_np_f * Xq[16 * iw + 3]; }
+                    n_np += 1;
+                    n_np_f *= (double)n_np * rinv;
+                }
+                if (LH) {
+                    const double t_over_n = r / (double)n;
+                    p1r -= t_over_n * c1i;
+                    p1i += t_over_n * c1r;
+                }
+                eps_l = (fabs(p1r) + fabs(p1i) + fabs(p0r)) * r_l_nm1 * nm1_f_inv;
+                if (LH) eps_l += sqrt(c1r * c1r + c1i * c1i) * r_l_nm1 * r_l * nm1_f_inv;
+            } else {
+                // degree-n part of the z-translation (translate_multipole_to_local_z_n!, :238-295),
+                // consumed by local_power (:455-502) in its m order
+                double lp = 0, lc = 0, Nt = 0, cnr = 0, cni = 0;
+                for (int m = n; m >= 0; --m) {
+                    double n_m_f = n_f_t_np1;
+                    for (int k = 0; k < m; ++k) n_m_f *= (double)(n + k + 1) * rinv;
+                    double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
+                    int n_np = n + m;
+                    int iw = nh(m - 1) + m;
+                    for (int np = m; np <= n; ++np) {
+                        v1r += f * Xq[16 * iw]; v1i += f * Xq[16 * iw + 1];
+                        if (LH) { v2r += f * Xq[16 * iw + 2]; v2i += f * Xq[16 * iw + 3]; }
+                        n_np += 1;
+                        f *= (double)n_np * rinv;
+                        iw += np + 1;
+                    }
+                    Nt = tb.Lt[in0 + m];
+                    double pr = v1r, pi = v1i;
+                    if (LH) {
+                        const double cor = v2r, coi = v2i;
+                        pr = pr - coi * r * (double)m / (double)n;
+                        pi = pi + cor * r * (double)m / (double)n;
+                        const double cr = cor - cnr * r / (double)(n + 1);
+                        const double ci = coi - cni * r / (double)(n + 1);
+                        cnr = cor; cni = coi;
+                        lc += (cr * cr + ci * ci) * Nt * Nt;
+                    }
+                    lp += (pr * pr + pi * pi) * Nt * Nt;
+                }
+                lp = sqrt(lp);
+                if (LH) lc = sqrt(lc);
+                if (E.kind == 3) {
+                    eps_l = SQRT3_D * lp * r_l_nm1 * nm1_f_inv / Nt;
+                    if (LH) eps_l += SQRT3_D * (double)n * lc * r_l_nm1 * r_l * nm1_f_inv / (Nt * (double)n);
+                } else {
+                    eps_l = SQRT3_D * lp * r_l_nm1 * r_l * nm1_f_inv / (Nt * (double)n);
+                    if (LH) eps_l += SQRT3_D * (double)n * lc * r_l_nm1 * r_l * nm1_f_inv / (Nt * (double)n);
+                }
+            }
+            pass = (eps_mp + eps_l * LOCAL_ERROR_SAFETY <= limit);
+        }
+    }
+    const unsigned long long mask = __ballot(pass);
+    wave_sync();
+    if (l < BP && W.ord[l] == MF_SEARCHING) {
+        const unsigned long long mine = (mask >> (l * MF_GROUP)) & ((1ull << G) - 1ull);
+        if (mine) W.ord[l] = n0 + __ffsll((long long)mine) - 1 - 1 + (E.BE ? 1 : 0);
+    }
+    wave_sync();
+}
+
+// z-translation of column m of the batch at each interaction's own order (rows
+// above it masked), Lamb-Helmholtz transform (lh_one<true>) and the input scale
+// of the back rotation; in place in X, by one wavefront
+template <bool LH>
+__device__ void mf_translate_m(const MfLds& W, int P, int Pb, int m) {
+    constexpr int NC = LH ? 4 : 2;
+    const int lane = lane_id(), lr = lane & 15, lg = lane >> 4, q = lr / NC;
+    const int Pq = W.ord[q];
+    const double* rpq = W.rp + q * (P + 2);
+    const int sz = Pb - m + 1;
+    const int kc = (sz + 3) >> 2;
+    for (int t = 0; 16 * t < sz; ++t) {
+        mf4 o = {0.0, 0.0, 0.0, 0.0};
+        const int na = m + 16 * t + lr;
+        double a[8], b[8];
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) {
+            const int k = m + 4 * cc + lg;
+            a[cc] = (cc < kc && na <= Pb && k <= Pb) ? W.fact[na + k] : 0.0;
+            b[cc] = (cc < kc && k <= Pq) ? W.X[16 * (nh(k - 1) + m) + lr] * rpq[k] : 0.0;
+        }
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc)
+            if (cc < kc) o = mfma_f64(a[cc], b[cc], o);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = m + 16 * t + lg + 4 * r;
+            if (n <= Pb) W.T[16 * (n - m) + lr] = (n <= Pq) ? o[r] * rpq[n + 1] : 0.0;
+        }
+    }
+    wave_sync();
+    for (int it = lane; it < 16 * sz; it += 64) {
+        const int n = m + (it >> 4), col = it & 15, qq = col / NC, cq = col % NC;
+        const double* Tn = W.T + 16 * (n - m) + qq * NC;
+        double out = Tn[cq];
+        if (LH) {
+            const double rq = W.geo[8 * qq];
+            if (cq < 2) {
+                if (n > 0) {
+                    const double r_m_n = rq * (double)m / (double)n;
+                    out = (cq == 0) ? Tn[0] - r_m_n * Tn[3] : Tn[1] + r_m_n * Tn[2];
+                }
+            } else if (n < W.ord[qq]) {
+                out = Tn[cq] - rq / (double)(n + 1) * Tn[16 + cq];
+            }
+        }
+        W.X[16 * (nh(n - 1) + m) + col] = W.ein[nh(n - 1) + m] * out;
+    }
+    wave_sync();
+}
+
+// One workgroup per target box; its wavefronts share each batch of BP sources:
+// degrees (rotations) and columns m (z-translation) are dealt out to them.
+template <bool LH, int NW>
+__global__ __launch_bounds__(64 * NW) void k_m2l_mfma(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
+                                                      const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
+                                                      const Branch* __restrict__ sbr, const double* __restrict__ sex,
+                                                      double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
+                                                      int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
+    constexpr int NC = LH ? 4 : 2, BP = 16 / NC;
+    extern __shared__ double lds[];
+    const MfLds W = mf_carve(lds, P, BP);
+    const int32_t it = m2l_targets[blockIdx.x];
+    const int64_t l0 = m2l_offsets[blockIdx.x], l1 = m2l_offsets[blockIdx.x + 1];
+    const int H = nh(P), H4 = 4 * H;
+    const int tid = threadIdx.x, wv = wave_id();
+    for (int i = tid; i < len_Hs(P); i += 64 * NW) W.Hs[i] = tb.Hs[i];
+    for (int i = tid; i < H; i += 64 * NW) {
+        int n, m;
+        idx_nm(i, n, m);
+        const int b = len_Ts(n - 1);
+        W.zin[i] = tb.zeta[b + m]; W.zout[i] = tb.zeta[b + m * (n + 1)];
+        W.ein[i] = tb.eta[b + m]; W.eout[i] = tb.eta[b + m * (n + 1)];
+    }
+    for (int i = tid; i < 2 * P + 2; i += 64 * NW) W.fact[i] = tb.fact[i];
+    for (int i = tid; i < H4; i += 64 * NW) W.acc[i] = 0.0;
+    const Branch T = tbr[it];
+    const double limit = (E.relative ? T.max_influence * E.eps : E.eps) * 4 * PI_D;
+    int local_max = 0, local_ok = 1;
+    __syncthreads();
+    for (int64_t li0 = l0; li0 < l1; li0 += BP) {
+        const int nb = (int)min((int64_t)BP, l1 - li0);
+        mf_setup<BP>(W, m2l_s, li0, nb, T, sbr, P, E);
+        if (E.kind >= 2) mf_load<LH, true>(W, sex, H4, tb.Mt, P);
+        else mf_load<LH, false>(W, sex, H4, tb.Mt, P);
+        if (E.kind == 0) {
+            for (int n = wv; n <= P; n += NW) mf_rotate_degree<LH, false>(W, W.zout, 1.0, P, n);
+            __syncthreads();
+        } else {
+            for (int n0 = 0; n0 <= P; n0 += MF_GROUP) {
+                const int n1 = min(P, n0 + MF_GROUP - 1);
+                for (int n = n0 + wv; n <= n1; n += NW) mf_rotate_degree<LH, false>(W, W.zout, 1.0, P, n);
+                __syncthreads();
+                if (wv == 0) mf_test_group<LH>(W, E, tb, limit, T.target_radius, P, max(n0, 1), n1);
+                __syncthreads();
+                bool searching = false;
+#pragma unroll
+                for (int q = 0; q < BP; ++q) searching |= (W.ord[q] == MF_SEARCHING);
+                if (!searching) break;
+            }
+        }
+        int Pb = 0;
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < BP; ++q) {
+            int o = W.ord[q];
+            if (o == MF_SEARCHING) { o = P; ok = false; }
+            Pb = max(Pb, o);
+        }
+        __syncthreads();
+        if (tid < BP && W.ord[tid] == MF_SEARCHING) W.ord[tid] = P;
+        __syncthreads();
+        for (int m = wv; m <= Pb; m += NW) mf_translate_m<LH>(W, P, Pb, m);
+        __syncthreads();
+        for (int n = wv; n <= Pb; n += NW) mf_rotate_degree<LH, true>(W, W.eout, -1.0, P, n);
+        if (tid < nb) orders[li0 + tid] = W.ord[tid];
+        local_max = max(local_max, Pb);
+        local_ok = local_ok && ok;
+        __syncthreads();
+    }
+    for (int i = tid; i < H4; i += 64 * NW) tex[(int64_t)it * H4 + i] += W.acc[i];
+    if (tid == 0) {
+        atomicMax(pmax, local_max);
+        if (!local_ok) atomicAnd(all_ok, 0);
+    }
+}
+
+template <bool LH>
+static int mf_waves(int P) {
+    constexpr int BP = LH ? 4 : 8;
+    for (int nw : {8, 4, 2, 1})
+        if (mf_lds_bytes(P, BP, nw) <= LDS_LIMIT) return nw;
+    throw Error(FMMHIP_EINVAL, "expansion order too large for the M2L kernel");
+}
+
+template <bool LH, int NW>
+static void launch_m2l_mfma(Context& C, int p, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E, int32_t* flags,
+                            const TablePtrs& tb) {
+    constexpr int BP = LH ? 4 : 8;
+    const size_t sh = mf_lds_bytes(p, BP, NW);
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_m2l_mfma<LH, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_m2l_mfma<LH, NW><<<(unsigned)Ls.n_m2l_targets, 64 * NW, sh, C.stream>>>(
+        Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
+        st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags, flags + 1, tb);
+    LAUNCH_CHECK();
+}
+
+template <bool LH>
+static void run_m2l_mfma(Context& C, int p, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E, int32_t* flags,
+                         const TablePtrs& tb) {
+    switch (mf_waves<LH>(p)) {
+        case 8: launch_m2l_mfma<LH, 8>(C, p, Ls, st, tt_dst, E, flags, tb); break;
+        case 4: launch_m2l_mfma<LH, 4>(C, p, Ls, st, tt_dst, E, flags, tb); break;
+        case 2: launch_m2l_mfma<LH, 2>(C, p, Ls, st, tt_dst, E, flags, tb); break;
+        default: launch_m2l_mfma<LH, 1>(C, p, Ls, st, tt_dst, E, flags, tb); break;
+    }
+}
+
 // ================================================= small orders: lane per M2L
 // For p <= 4 an expansion (at most 15 coefficients x 2 potentials) fits in
 // registers, so one lane runs one M2L interaction end to end with the same
@@ -1327,6 +1923,15 @@ static void launch_m2l_lane(Context& C, int p, const Lists& Ls, const Tree& st, 
 
 __global__ void k_init_flags(int32_t* pmax, int32_t* ok) { *pmax = 0; *ok = 1; }
 
+// M2L kernel for orders above SMALL_P: the wavefront-per-interaction engine
+// (k_m2l) or the matrix-core batches (k_m2l_mfma). Both compute the same
+// interactions at the same orders; FMMHIP_M2L_KERNEL=mfma selects the second.
+enum M2LEngine { M2L_WAVE = 0, M2L_MFMA = 1 };
+static M2LEngine m2l_engine() {
+    const char* e = std::getenv("FMMHIP_M2L_KERNEL");
+    return (e && std::strcmp(e, "mfma") == 0) ? M2L_MFMA : M2L_WAVE;
+}
+
 void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int p, bool LH, int err_kind, bool err_rel,
              double eps, bool BE, int precision, int32_t* pmax, int32_t* success) {
     (void)precision;
@@ -1342,15 +1947,20 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
         else launch_m2l_lane<false>(C, p, Ls, st, tt_dst, flags.data(), tb);
     } else if (Ls.n_m2l_targets > 0) {
         ErrParams E{err_kind, err_rel ? 1 : 0, eps, BE ? 1 : 0};
-        const int nw = translate_waves(p);
-        if (LH) {
-            if (nw == 8) launch_m2l<true, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-            else if (nw == 6) launch_m2l<true, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-            else launch_m2l<true, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+        if (m2l_engine() == M2L_MFMA) {
+            if (LH) run_m2l_mfma<true>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            else run_m2l_mfma<false>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
         } else {
-            if (nw == 8) launch_m2l<false, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-            else if (nw == 6) launch_m2l<false, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-            else launch_m2l<false, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            const int nw = translate_waves(p);
+            if (LH) {
+                if (nw == 8) launch_m2l<true, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+                else if (nw == 6) launch_m2l<true, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+                else launch_m2l<true, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            } else {
+                if (nw == 8) launch_m2l<false, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+                else if (nw == 6) launch_m2l<false, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+                else launch_m2l<false, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            }
         }
     }
     int32_t h[2];
@@ -1400,99 +2010,168 @@ __device__ inline void solid_pair(double x, double y, double z, int n, int k, co
     if (n - 1 >= k) finish(n - 1, pr, pi, R[2], R[3]);
 }
 
+// Column k of the regular solid harmonics R_j^k, j = k..P, of one position by the
+// Cartesian recurrence above (no trigonometry), into tab[(j(j+1)/2 + k)][re/im].
+__device__ inline void harmonic_column(double x, double y, double z, int k, int P, const double* inv_fact, double* tab) {
+    double qr = 1.0, qi = 0.0;
+    for (int j = 1; j <= k; ++j) {   // Q_k^k
+        const double f = -(double)(2 * j - 1);
+        const double nr = f * (x * qr - y * qi), ni = f * (x * qi + y * qr);
+        qr = nr; qi = ni;
+    }
+    const double rho2 = x * x + y * y + z * z;
+    double pr = 0.0, pi = 0.0, cr = qr, ci = qi;   // Q_{j-1}, Q_j
+    for (int j = k; j <= P; ++j) {
+        if (j > k) {
+            const double a = (double)(2 * j - 1) * z, b = (double)(j + k - 1) * rho2, d = 1.0 / (double)(j - k);
+            const double nr = (a * cr - b * pr) * d, ni = (a * ci - b * pi) * d;
+            pr = cr; pi = ci; cr = nr; ci = ni;
+        }
+        const double s = ((j & 1) ? -1.0 : 1.0) * inv_fact[j + k];   // R = (-1)^j i^k Q / (j+k)!
+        const double ar = cr * s, ai = ci * s;
+        double o0, o1;
+        switch (k & 3) {
+            case 0: o0 = ar; o1 = ai; break;
+            case 1: o0 = -ai; o1 = ar; break;
+            case 2: o0 = -ar; o1 = -ai; break;
+            default: o0 = ai; o1 = -ar; break;
+        }
+        const int i = ((j * (j + 1)) >> 1) + k;
+        tab[2 * i] = o0;
+        tab[2 * i + 1] = o1;
+    }
+}
+
+// R_j^k from a table (zero outside 0 <= k <= j)
+__device__ inline void tab_get(const double* tab, int j, int k, double& r, double& i) {
+    if (j < 0 || k < 0 || k > j) { r = 0.0; i = 0.0; return; }
+    const int idx = ((j * (j + 1)) >> 1) + k;
+    r = tab[2 * idx];
+    i = tab[2 * idx + 1];
+}
+
+// P2M (body_to_multipole!, src/bodytomultipole.jl:387-496): one wavefront per leaf. The
+// bodies go through in groups of 64/(P+1): lane l builds column l % (P+1) of the harmonic
+// table of body l / (P+1) (no barrier inside), then every lane adds the group's
+// contributions to the coefficients it owns (i = lane + 64 r), reading neighbours from
+// the tables. Point{Vortex} uses the mirrored position (mirrored_source_to_vortex!).
 template <int KERNEL>
 __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
                                             const double* __restrict__ x, const double* __restrict__ y,
                                             const double* __restrict__ z, const double* __restrict__ q, int64_t ns,
                                             double* __restrict__ ex, int P, const double* __restrict__ inv_fact) {
-    __shared__ double lb[6][64];
+    extern __shared__ double lds[];
+    constexpr int NR = 4;   // coefficients per lane: H(20) = 231 <= 4 x 64
+    const int H = nh(P);
+    const int G = 64 / (P + 1);
+    double* tabs = lds;                               // G tables of 2H doubles
+    double* qs = lds + (size_t)G * 2 * H;             // G x 3 strengths
     const int64_t ib = leaf_index[blockIdx.x];
     const Branch B = br[ib];
-    const int H = nh(P);
     const double cx = B.source_center[0], cy = B.source_center[1], cz = B.source_center[2];
-    for (int i0 = 0; i0 < H; i0 += 64) {
-        const int i = i0 + threadIdx.x;
-        int n = 0, m = 0;
-        if (i < H) idx_nm(i, n, m);
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        for (int64_t b0 = 0; b0 < B.n_bodies; b0 += 64) {
-            const int cnt = (int)min<int64_t>(64, B.n_bodies - b0);
-            __syncthreads();
-            if ((int)threadIdx.x < cnt) {
-                const int64_t j = B.first + b0 + threadIdx.x;
-                lb[0][threadIdx.x] = x[j] - cx; lb[1][threadIdx.x] = y[j] - cy; lb[2][threadIdx.x] = z[j] - cz;
-                lb[3][threadIdx.x] = q[j];
-                if (KERNEL != FMMHIP_POINT_SOURCE) { lb[4][threadIdx.x] = q[ns + j]; lb[5][threadIdx.x] = q[2 * ns + j]; }
+    const int tid = threadIdx.x;
+    int rn[NR], rm[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int i = tid + 64 * r;
+        if (i < H) idx_nm(i, rn[r], rm[r]);
+        else { rn[r] = -1; rm[r] = 0; }
+    }
+    double acc[NR][4];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0.0;
+    const bool mirror = KERNEL == FMMHIP_POINT_VORTEX;
+    for (int64_t b0 = 0; b0 < B.n_bodies; b0 += G) {
+        const int cnt = (int)min<int64_t>(G, B.n_bodies - b0);
+        const int g = tid / (P + 1), k = tid % (P + 1);
+        if (g < cnt) {
+            const int64_t j = B.first + b0 + g;
+            double dx = x[j] - cx, dy = y[j] - cy, dz = z[j] - cz;
+            if (mirror) { dx = -dx; dy = -dy; dz = -dz; }
+            harmonic_column(dx, dy, dz, k, P, inv_fact, tabs + (size_t)g * 2 * H);
+            if (k == 0) {
+                qs[3 * g] = q[j];
+                if (KERNEL != FMMHIP_POINT_SOURCE) { qs[3 * g + 1] = q[ns + j]; qs[3 * g + 2] = q[2 * ns + j]; }
             }
-            __syncthreads();
-            if (i >= H) continue;
-            for (int s = 0; s < cnt; ++s) {
+        }
+        __syncthreads();
+        for (int gb = 0; gb < cnt; ++gb) {
+            const double* T = tabs + (size_t)gb * 2 * H;
+            const double s0 = qs[3 * gb], s1 = qs[3 * gb + 1], s2 = qs[3 * gb + 2];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int n = rn[r], m = rm[r];
+                if (n < 0) continue;
+                const double om = (m & 1) ? -1.0 : 1.0;
                 if (KERNEL == FMMHIP_POINT_SOURCE) {
                     // M_n^m += (-1)^(n+m) (-q) conj(R_n^m(dx))   (src/bodytomultipole.jl:403-429)
-                    double R[4];
-                    solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m, inv_fact, R);
-                    double f = (((n + m) & 1) ? 1.0 : -1.0) * lb[3][s];
-                    a0 += f * R[0];
-                    a1 -= f * R[1];
+                    double Rr, Ri;
+                    tab_get(T, n, m, Rr, Ri);
+                    const double f = (((n + m) & 1) ? 1.0 : -1.0) * s0;
+                    acc[r][0] += f * Rr;
+                    acc[r][1] -= f * Ri;
                 } else if (KERNEL == FMMHIP_POINT_DIPOLE) {
-                    // source_to_dipole! of -q on the harmonics of dx (src/bodytomultipole.jl:251-278, 431-463):
-                    // h_n^m from q_{n-1}^{m-1}, q_{n-1}^m, q_{n-1}^{m+1} (get_nm1 rules), M += (-1)^(n+m) conj(h)
+                    // source_to_dipole! of -q on q_{n-1}^{m-1,m,m+1} (get_nm1 rules, :77-110, 251-278, 431-463)
                     if (n >= 1) {
-                        const double sxq = -lb[3][s], syq = -lb[4][s], szq = -lb[5][s];
-                        double Rm1[4], R0[4], Rp1[4];
-                        solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m > 0 ? m - 1 : 1, inv_fact, Rm1);
-                        solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m, inv_fact, R0);
-                        solid_pair(lb[0][s], lb[1][s], lb[2][s], n, m + 1, inv_fact, Rp1);
-                        const double q0 = m > 0 ? Rm1[2] : -Rm1[2], q1 = Rm1[3];   // q_{n-1}^{-1} = -conj(q_{n-1}^1)
-                        const double q2 = R0[2], q3 = R0[3], q4 = Rp1[2], q5 = Rp1[3];
+                        double q0, q1, q2, q3, q4, q5;
+                        tab_get(T, n - 1, m > 0 ? m - 1 : 1, q0, q1);
+                        if (m == 0) q0 = -q0;   // q_{n-1}^{-1} = -conj(q_{n-1}^1)
+                        tab_get(T, n - 1, m, q2, q3);
+                        tab_get(T, n - 1, m + 1, q4, q5);
+                        const double sxq = -s0, syq = -s1, szq = -s2;
                         const double hr = -sxq * 0.5 * (q5 + q1) + syq * 0.5 * (q4 - q0) - szq * q2;
                         const double hi = sxq * 0.5 * (q4 + q0) + syq * 0.5 * (q5 - q1) - szq * q3;
                         const double f = ((n + m) & 1) ? -1.0 : 1.0;
-                        a0 += f * hr;
-                        a1 -= f * hi;
+                        acc[r][0] += f * hr;
+                        acc[r][1] -= f * hi;
                     }
                 } else {
                     // mirrored_source_to_vortex! with q = R(-dx) (src/bodytomultipole.jl:283-334, 465-489)
-                    const double vx = lb[3][s], vy = lb[4][s], vz = lb[5][s];
-                    const double ux = -lb[0][s], uy = -lb[1][s], uz = -lb[2][s];
-                    double Rm1[4], R0[4], Rp1[4];
-                    solid_pair(ux, uy, uz, n, m - 1 >= 0 ? m - 1 : 1, inv_fact, Rm1);
-                    solid_pair(ux, uy, uz, n, m, inv_fact, R0);
-                    solid_pair(ux, uy, uz, n, m + 1, inv_fact, Rp1);
-                    const double om = (m & 1) ? -1.0 : 1.0;
+                    const double vx = s0, vy = s1, vz = s2;
+                    double m1r, m1i, m1pr, m1pi, r0r, r0i, r0pr, r0pi, p1r, p1i, p1pr, p1pi;
+                    const int km1 = m - 1 >= 0 ? m - 1 : 1;
+                    tab_get(T, n, km1, m1r, m1i);
+                    tab_get(T, n - 1, km1, m1pr, m1pi);
+                    tab_get(T, n, m, r0r, r0i);
+                    tab_get(T, n - 1, m, r0pr, r0pi);
+                    tab_get(T, n, m + 1, p1r, p1i);
+                    tab_get(T, n - 1, m + 1, p1pr, p1pi);
                     // phi part: q_n^{m-1}, q_n^m, q_n^{m+1}
                     double qm1r, qm1i;
-                    if (m > 0) { qm1r = Rm1[0]; qm1i = Rm1[1]; }
+                    if (m > 0) { qm1r = m1r; qm1i = m1i; }
                     else if (n == 0) { qm1r = 0.0; qm1i = 0.0; }
-                    else { qm1r = -om * Rm1[0]; qm1i = om * Rm1[1]; }   // q_n^{-1} = -conj(q_n^1)
-                    double qmi = R0[1], qmr = R0[0];
-                    double qp1r = (m < n) ? Rp1[0] : 0.0, qp1i = (m < n) ? Rp1[1] : 0.0;
+                    else { qm1r = -om * m1r; qm1i = om * m1i; }   // q_n^{-1} = -conj(q_n^1)
+                    const double qmi = r0i, qmr = r0r;
+                    const double qp1r = (m < n) ? p1r : 0.0, qp1i = (m < n) ? p1i : 0.0;
                     const double A = (double)(n - m + 1) * 0.5, Bc = (double)(n + m + 1) * 0.5, inp1 = 1.0 / (double)(n + 1);
-                    a0 += om * ((-vx * qm1r + vy * qm1i) * A + (vx * qp1r + vy * qp1i) * Bc - vz * (double)m * qmi) * inp1;
-                    a1 += om * ((vx * qm1i + vy * qm1r) * A + (-vx * qp1i + vy * qp1r) * Bc - vz * (double)m * qmr) * inp1;
+                    acc[r][0] += om * ((-vx * qm1r + vy * qm1i) * A + (vx * qp1r + vy * qp1i) * Bc - vz * (double)m * qmi) * inp1;
+                    acc[r][1] += om * ((vx * qm1i + vy * qm1r) * A + (-vx * qp1i + vy * qp1r) * Bc - vz * (double)m * qmr) * inp1;
                     if (n >= 1) {
                         // chi part: q_{n-1}^{m-1}, q_{n-1}^m, q_{n-1}^{m+1} (get_nm1 rules)
                         double pm1r, pm1i;
-                        if (m > 0) { pm1r = Rm1[2]; pm1i = Rm1[3]; }
-                        else if (n > 1) { pm1r = -om * Rm1[2]; pm1i = om * Rm1[3]; }
+                        if (m > 0) { pm1r = m1pr; pm1i = m1pi; }
+                        else if (n > 1) { pm1r = -om * m1pr; pm1i = om * m1pi; }
                         else { pm1r = 0.0; pm1i = 0.0; }
-                        double p0r = (m < n) ? R0[2] : 0.0, p0i = (m < n) ? R0[3] : 0.0;
-                        double pp1r = (m + 1 < n) ? Rp1[2] : 0.0, pp1i = (m + 1 < n) ? Rp1[3] : 0.0;
+                        const double p0r = (m < n) ? r0pr : 0.0, p0i = (m < n) ? r0pi : 0.0;
+                        const double pp1r = (m + 1 < n) ? p1pr : 0.0, pp1i = (m + 1 < n) ? p1pi : 0.0;
                         const double ion = 1.0 / (double)n;
-                        a2 -= om * ion * (0.5 * (-vy * pm1r - vx * pm1i + vy * pp1r - vx * pp1i) - vz * p0r);
-                        a3 -= om * ion * (0.5 * (vy * pm1i - vx * pm1r - vy * pp1i - vx * pp1r) + vz * p0i);
+                        acc[r][2] -= om * ion * (0.5 * (-vy * pm1r - vx * pm1i + vy * pp1r - vx * pp1i) - vz * p0r);
+                        acc[r][3] -= om * ion * (0.5 * (vy * pm1i - vx * pm1r - vy * pp1i - vx * pp1r) + vz * p0i);
                     }
                 }
             }
         }
-        if (i < H) {
-            double* e = ex + ib * 4 * H + 4 * i;
-            e[0] += a0; e[1] += a1;
-            if (KERNEL == FMMHIP_POINT_VORTEX) { e[2] += a2; e[3] += a3; }   // Lamb-Helmholtz chi part
-        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int i = tid + 64 * r;
+        if (i >= H) continue;
+        double* e = ex + ib * 4 * H + 4 * i;
+        e[0] += acc[r][0]; e[1] += acc[r][1];
+        if (KERNEL == FMMHIP_POINT_VORTEX) { e[2] += acc[r][2]; e[3] += acc[r][3]; }   // Lamb-Helmholtz chi part
     }
 }
-
 
 // --------------------------------------------------------- panel P2M
 // body_to_multipole! for constant-strength triangles (src/bodytomultipole.jl:643-831):
@@ -1623,8 +2302,9 @@ void run_p2m(Context& C, int kernel, Tree& st, int p) {
         LAUNCH_CHECK();
         return;
     }
+    const size_t p2m_sh = sizeof(double) * (size_t)(64 / (p + 1)) * (2 * nh(p) + 3);
 #define FMMHIP_P2M(K)                                                                                                  \
-    k_p2m<K><<<(unsigned)st.n_leaves, 64, 0, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(), \
+    k_p2m<K><<<(unsigned)st.n_leaves, 64, p2m_sh, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(), \
                                                         st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact)
     if (kernel == FMMHIP_POINT_SOURCE) FMMHIP_P2M(FMMHIP_POINT_SOURCE);
     else if (kernel == FMMHIP_POINT_DIPOLE) FMMHIP_P2M(FMMHIP_POINT_DIPOLE);

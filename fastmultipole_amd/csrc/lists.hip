@@ -3,21 +3,28 @@
 //
 // The reference recurses depth-first from (root, root); each call either emits
 // one pair (direct or M2L), drops it, or splits the target or the source into
-// its children in branch order. Here the recursion runs breadth-first over a
-// frontier that is kept in depth-first order: every step replaces each pending
-// pair by its outputs in place (exclusive scan of output counts), and emitted
-// pairs are carried along unchanged. When nothing is pending the frontier is
-// the reference's discovery order; a stable radix sort by target then gives
-// sort_by_target's output. Compiled with -ffp-contract=off (the MAC must match).
+// its children in branch order. Here the recursion tree is built breadth-first,
+// one level per step, and only pending pairs are kept: level k + 1 holds the
+// children of the split pairs of level k, in order (exclusive scan of child
+// counts). A backward pass then counts the M2L and direct pairs below every node
+// (two counters packed in one 64-bit word), and a forward pass hands every node
+// the position of its first emitted pair in depth-first order, so each emitted
+// pair is written straight to its slot of the reference's discovery order. A
+// stable radix sort by target then gives sort_by_target's output. Compiled with
+// -ffp-contract=off (the MAC must match).
 #include "fmmhip_internal.hpp"
 #include <cstring>   // host memset for rocprim headers
 #include <rocprim/rocprim.hpp>
 #include <algorithm>
+#include <memory>
+#include <vector>
 
 namespace fmmhip {
 
-enum : int32_t { ST_PENDING = 0, ST_M2L = 1, ST_DIRECT = 2 };
-enum : int32_t { D_CARRY = 0, D_DROP = 1, D_EMIT_M2L = 2, D_EMIT_DIRECT = 3, D_SPLIT_T = 4, D_SPLIT_S = 5 };
+enum : int32_t { D_DROP = 0, D_EMIT_M2L = 1, D_EMIT_DIRECT = 2, D_SPLIT_T = 3, D_SPLIT_S = 4 };
+
+// packed (M2L, direct) pair counts: M2L in the high word
+static constexpr uint64_t ONE_M2L = 1ull << 32, ONE_DIRECT = 1ull;
 
 struct ListParams {
     int64_t ls;
@@ -35,13 +42,15 @@ __device__ inline bool mac_accept(const Branch& t, const Branch& s, double mac) 
     return sep2 * mac * mac > sr * sr;
 }
 
-// one step of build_interaction_lists! for every pending pair (:23-166)
-__global__ void k_list_decide(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, const int32_t* __restrict__ fst,
-                              int64_t n, const Branch* __restrict__ tb, const Branch* __restrict__ sb, ListParams P,
-                              int32_t* __restrict__ dec, int32_t* __restrict__ cnt, unsigned long long* __restrict__ n_pending) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// one step of build_interaction_lists! for every pair of a level (:23-166):
+// decision and number of children (0 unless split); entry n gets count 0 so
+// that the exclusive scan ends with the size of the next level
+__global__ void k_list_decide(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, int64_t n,
+                              const Branch* __restrict__ tb, const Branch* __restrict__ sb, ListParams P,
+                              int8_t* __restrict__ dec, int32_t* __restrict__ cnt) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i == n) cnt[i] = 0;
     if (i >= n) return;
-    if (fst[i] != ST_PENDING) { dec[i] = D_CARRY; cnt[i] = 1; return; }
     const int32_t it = ft[i], js = fs[i];
     const Branch& t = tb[it];
     const Branch& s = sb[js];
@@ -66,47 +75,62 @@ __global__ void k_list_decide(const int32_t* __restrict__ ft, const int32_t* __r
             d = split_t ? D_SPLIT_T : D_SPLIT_S;
         }
     }
-    dec[i] = d;
-    int32_t c = (d == D_DROP) ? 0 : (d == D_SPLIT_T) ? (int32_t)t.n_branches : (d == D_SPLIT_S) ? (int32_t)s.n_branches : 1;
-    cnt[i] = c;
-    if (d == D_SPLIT_T || d == D_SPLIT_S) atomicAdd(n_pending, (unsigned long long)c);
+    dec[i] = (int8_t)d;
+    cnt[i] = (d == D_SPLIT_T) ? (int32_t)t.n_branches : (d == D_SPLIT_S) ? (int32_t)s.n_branches : 0;
 }
 
-__global__ void k_list_write(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, const int32_t* __restrict__ fst,
-                             int64_t n, const Branch* __restrict__ tb, const Branch* __restrict__ sb,
-                             const int32_t* __restrict__ dec, const int64_t* __restrict__ pos, int32_t* __restrict__ ot,
-                             int32_t* __restrict__ os, int32_t* __restrict__ ost) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// children of the split pairs, in branch order, at their scanned positions
+__global__ void k_list_expand(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, int64_t n,
+                              const Branch* __restrict__ tb, const Branch* __restrict__ sb, const int8_t* __restrict__ dec,
+                              const int64_t* __restrict__ pos, int32_t* __restrict__ ot, int32_t* __restrict__ os) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t o = pos[i];
     const int32_t it = ft[i], js = fs[i];
-    switch (dec[i]) {
-        case D_CARRY: ot[o] = it; os[o] = js; ost[o] = fst[i]; break;
-        case D_EMIT_M2L: ot[o] = it; os[o] = js; ost[o] = ST_M2L; break;
-        case D_EMIT_DIRECT: ot[o] = it; os[o] = js; ost[o] = ST_DIRECT; break;
-        case D_SPLIT_T: {
-            const Branch& t = tb[it];
-            for (int64_t k = 0; k < t.n_branches; ++k) { ot[o + k] = (int32_t)(t.child_first + k); os[o + k] = js; ost[o + k] = ST_PENDING; }
-            break;
-        }
-        case D_SPLIT_S: {
-            const Branch& s = sb[js];
-            for (int64_t k = 0; k < s.n_branches; ++k) { ot[o + k] = it; os[o + k] = (int32_t)(s.child_first + k); ost[o + k] = ST_PENDING; }
-            break;
-        }
-        default: break;
+    if (dec[i] == D_SPLIT_T) {
+        const Branch& t = tb[it];
+        for (int64_t k = 0; k < t.n_branches; ++k) { ot[o + k] = (int32_t)(t.child_first + k); os[o + k] = js; }
+    } else if (dec[i] == D_SPLIT_S) {
+        const Branch& s = sb[js];
+        for (int64_t k = 0; k < s.n_branches; ++k) { ot[o + k] = it; os[o + k] = (int32_t)(s.child_first + k); }
     }
 }
 
-__global__ void k_state_flags(const int32_t* __restrict__ st, int64_t n, int32_t want, int32_t* __restrict__ flag) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = st[i] == want ? 1 : 0;
+// packed (M2L, direct) counts of the subtree of every pair of a level
+__global__ void k_list_sizes(int64_t n, const int8_t* __restrict__ dec, const int64_t* __restrict__ pos,
+                             const uint64_t* __restrict__ child_size, uint64_t* __restrict__ size) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = dec[i];
+    uint64_t v = 0;
+    if (d == D_EMIT_M2L) v = ONE_M2L;
+    else if (d == D_EMIT_DIRECT) v = ONE_DIRECT;
+    else if (d == D_SPLIT_T || d == D_SPLIT_S)
+        for (int64_t c = pos[i]; c < pos[i + 1]; ++c) v += child_size[c];
+    size[i] = v;
 }
 
-__global__ void k_compact(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, const int32_t* __restrict__ flag,
-                          const int32_t* __restrict__ rank, int64_t n, uint32_t* __restrict__ kt, int32_t* __restrict__ vs) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) { kt[rank[i]] = (uint32_t)ft[i]; vs[rank[i]] = fs[i]; }
+// depth-first positions: an emitted pair goes to its slot of the M2L or direct
+// list; a split pair passes running offsets to its children
+__global__ void k_list_place(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, int64_t n,
+                             const int8_t* __restrict__ dec, const int64_t* __restrict__ pos, const uint64_t* __restrict__ off,
+                             const uint64_t* __restrict__ child_size, uint64_t* __restrict__ child_off,
+                             uint32_t* __restrict__ m2l_t, int32_t* __restrict__ m2l_s, uint32_t* __restrict__ dir_t,
+                             int32_t* __restrict__ dir_s) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = dec[i];
+    const uint64_t o = off[i];
+    if (d == D_EMIT_M2L) {
+        const uint64_t k = o >> 32;
+        m2l_t[k] = (uint32_t)ft[i]; m2l_s[k] = fs[i];
+    } else if (d == D_EMIT_DIRECT) {
+        const uint64_t k = o & 0xffffffffull;
+        dir_t[k] = (uint32_t)ft[i]; dir_s[k] = fs[i];
+    } else if (d == D_SPLIT_T || d == D_SPLIT_S) {
+        uint64_t r = o;
+        for (int64_t c = pos[i]; c < pos[i + 1]; ++c) { child_off[c] = r; r += child_size[c]; }
+    }
 }
 
 __global__ void k_target_starts(const uint32_t* __restrict__ t, int64_t n, int32_t* __restrict__ flag) {
@@ -165,20 +189,29 @@ __global__ void k_dir_counts(const int32_t* __restrict__ s, int64_t n, const Bra
     else if (i == n) cnt[i] = 0;
 }
 
-__global__ void k_init_frontier(int32_t* ft, int32_t* fs, int32_t* fst, unsigned long long* counter) {
-    ft[0] = 0; fs[0] = 0; fst[0] = ST_PENDING;
-    *counter = 0;
-}
+__global__ void k_init_root(int32_t* ft, int32_t* fs, uint64_t* off) { ft[0] = 0; fs[0] = 0; off[0] = 0; }
 
 __global__ void k_zero_counter(unsigned long long* c) { *c = 0; }
 
+// one level of the recursion tree
+struct ListLevel {
+    DBuf<int32_t> ft, fs;
+    DBuf<int8_t> dec;
+    DBuf<int64_t> pos;
+    DBuf<uint64_t> size, off;
+};
+
 struct ListScratch {
-    DBuf<int32_t> ft[2], fs[2], fst[2];
-    DBuf<int32_t> dec, cnt, flag, rank;
+    std::vector<std::unique_ptr<ListLevel>> levels;
+    DBuf<int32_t> cnt, flag, rank;
     DBuf<int64_t> pos;
     DBuf<unsigned long long> counter;
-    DBuf<uint32_t> kt, kt_sorted;
-    DBuf<int32_t> vs, vs_sorted;
+    DBuf<uint32_t> kt_m2l, kt_dir, kt_sorted;
+    DBuf<int32_t> vs_m2l, vs_dir;
+    ListLevel& level(size_t k) {
+        while (levels.size() <= k) levels.emplace_back(new ListLevel());
+        return *levels[k];
+    }
 };
 
 static ListScratch& lscratch() {
@@ -186,30 +219,22 @@ static ListScratch& lscratch() {
     return *s;
 }
 
-// sorted-by-target list of one kind; fills CSR over unique targets
-static void extract_list(Context& C, ListScratch& S, int cur, int64_t n, int32_t state, DBuf<int32_t>& out_t,
-                         DBuf<int32_t>& out_s, int64_t& n_out, DBuf<int32_t>& targets, DBuf<int64_t>& offsets,
-                         int64_t& n_targets) {
-    S.flag.resize(n + 1);
-    S.rank.resize(n + 1);
-    k_state_flags<<<grid_for(n, 256), 256, 0, C.stream>>>(S.fst[cur].data(), n, state, S.flag.data());
-    LAUNCH_CHECK();
-    excl_scan(C, S.flag.data(), S.rank.data(), n);
-    n_out = (int64_t)read_back(C, S.rank.data() + (n - 1)) + read_back(C, S.flag.data() + (n - 1));
+// stable sort by target (== sort_by_target, src/interaction_list.jl:183-213) of a
+// list in discovery order, then CSR over the unique targets
+static void finish_list(Context& C, ListScratch& S, uint32_t* kt, int32_t* vs, int64_t n_out, int target_bits,
+                        DBuf<int32_t>& out_t, DBuf<int32_t>& out_s, DBuf<int32_t>& targets, DBuf<int64_t>& offsets,
+                        int64_t& n_targets) {
     out_t.resize(std::max<int64_t>(1, n_out));
     out_s.resize(std::max<int64_t>(1, n_out));
     n_targets = 0;
     if (n_out == 0) { offsets.resize(1); targets.resize(1); HIP_CHECK(hipMemsetAsync(offsets.data(), 0, 8, C.stream)); return; }
-    S.kt.resize(n_out); S.vs.resize(n_out); S.kt_sorted.resize(n_out); S.vs_sorted.resize(n_out);
-    k_compact<<<grid_for(n, 256), 256, 0, C.stream>>>(S.ft[cur].data(), S.fs[cur].data(), S.flag.data(), S.rank.data(), n,
-                                                     S.kt.data(), S.vs.data());
-    LAUNCH_CHECK();
-    size_t bytes = 0;   // stable LSD radix sort by target == sort_by_target (src/interaction_list.jl:183-213)
-    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, S.kt.data(), S.kt_sorted.data(), S.vs.data(), out_s.data(), (size_t)n_out,
-                                        0, 32, C.stream));
+    S.kt_sorted.resize(n_out);
+    size_t bytes = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kt, S.kt_sorted.data(), vs, out_s.data(), (size_t)n_out, 0, target_bits,
+                                        C.stream));
     C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, S.kt.data(), S.kt_sorted.data(), S.vs.data(), out_s.data(),
-                                        (size_t)n_out, 0, 32, C.stream));
+    HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, kt, S.kt_sorted.data(), vs, out_s.data(), (size_t)n_out, 0,
+                                        target_bits, C.stream));
     S.flag.resize(n_out + 1);
     S.rank.resize(n_out + 1);
     k_target_starts<<<grid_for(n_out, 256), 256, 0, C.stream>>>(S.kt_sorted.data(), n_out, S.flag.data());
@@ -232,36 +257,71 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
     if (tt.n_branches == 0 || st.n_branches == 0) return;
     ListScratch& S = lscratch();
     ListParams P{ls, mac, method, ff, nf, si};
-    int cur = 0;
-    int64_t n = 1;
-    S.ft[0].resize(1); S.fs[0].resize(1); S.fst[0].resize(1);
-    S.counter.resize(1);
-    k_init_frontier<<<1, 1, 0, C.stream>>>(S.ft[0].data(), S.fs[0].data(), S.fst[0].data(), S.counter.data());
-    LAUNCH_CHECK();
-    for (int step = 0; step < 4096; ++step) {
-        S.dec.resize(n); S.cnt.resize(n); S.pos.resize(n + 1);
-        k_zero_counter<<<1, 1, 0, C.stream>>>(S.counter.data());
-        k_list_decide<<<grid_for(n, 256), 256, 0, C.stream>>>(S.ft[cur].data(), S.fs[cur].data(), S.fst[cur].data(), n,
-                                                              tt.branches.data(), st.branches.data(), P, S.dec.data(),
-                                                              S.cnt.data(), S.counter.data());
+    // forward: levels of pending pairs
+    std::vector<int64_t> sizes;
+    {
+        ListLevel& L0 = S.level(0);
+        L0.ft.resize(1); L0.fs.resize(1); L0.off.resize(1);
+        k_init_root<<<1, 1, 0, C.stream>>>(L0.ft.data(), L0.fs.data(), L0.off.data());
         LAUNCH_CHECK();
-        excl_scan(C, S.cnt.data(), S.pos.data(), n);
-        const int64_t total = read_back(C, S.pos.data() + (n - 1)) + (int64_t)read_back(C, S.cnt.data() + (n - 1));
-        const unsigned long long pending = read_back(C, S.counter.data());
-        const int nxt = cur ^ 1;
-        S.ft[nxt].resize(std::max<int64_t>(1, total)); S.fs[nxt].resize(std::max<int64_t>(1, total));
-        S.fst[nxt].resize(std::max<int64_t>(1, total));
-        k_list_write<<<grid_for(n, 256), 256, 0, C.stream>>>(S.ft[cur].data(), S.fs[cur].data(), S.fst[cur].data(), n,
-                                                            tt.branches.data(), st.branches.data(), S.dec.data(), S.pos.data(),
-                                                            S.ft[nxt].data(), S.fs[nxt].data(), S.fst[nxt].data());
-        LAUNCH_CHECK();
-        cur = nxt;
-        n = total;
-        if (pending == 0 || n == 0) break;
     }
-    if (n == 0) return;
-    extract_list(C, S, cur, n, ST_M2L, L.m2l_t, L.m2l_s, L.n_m2l, L.m2l_targets, L.m2l_offsets, L.n_m2l_targets);
-    extract_list(C, S, cur, n, ST_DIRECT, L.dir_t, L.dir_s, L.n_direct, L.dir_targets, L.dir_offsets, L.n_direct_targets);
+    int64_t n = 1;
+    for (size_t k = 0;; ++k) {
+        if (k >= 4096) throw Error(FMMHIP_EDEPTH, "interaction-list recursion deeper than 4096 levels");
+        ListLevel& A = S.level(k);
+        sizes.push_back(n);
+        A.dec.resize(n);
+        A.pos.resize(n + 1);
+        S.cnt.resize(n + 1);
+        k_list_decide<<<grid_for(n + 1, 256), 256, 0, C.stream>>>(A.ft.data(), A.fs.data(), n, tt.branches.data(),
+                                                                  st.branches.data(), P, A.dec.data(), S.cnt.data());
+        LAUNCH_CHECK();
+        excl_scan(C, S.cnt.data(), A.pos.data(), n + 1);
+        const int64_t next = read_back(C, A.pos.data() + n);
+        if (next == 0) break;
+        ListLevel& B = S.level(k + 1);
+        B.ft.resize(next); B.fs.resize(next);
+        k_list_expand<<<grid_for(n, 256), 256, 0, C.stream>>>(A.ft.data(), A.fs.data(), n, tt.branches.data(), st.branches.data(),
+                                                             A.dec.data(), A.pos.data(), B.ft.data(), B.fs.data());
+        LAUNCH_CHECK();
+        n = next;
+    }
+    const int K = (int)sizes.size();
+    // backward: subtree counts
+    for (int k = K - 1; k >= 0; --k) {
+        ListLevel& A = S.level(k);
+        A.size.resize(sizes[k]);
+        const uint64_t* child = (k + 1 < K) ? S.level(k + 1).size.data() : nullptr;
+        k_list_sizes<<<grid_for(sizes[k], 256), 256, 0, C.stream>>>(sizes[k], A.dec.data(), A.pos.data(), child, A.size.data());
+        LAUNCH_CHECK();
+    }
+    const uint64_t total = read_back(C, S.level(0).size.data());
+    L.n_m2l = (int64_t)(total >> 32);
+    L.n_direct = (int64_t)(total & 0xffffffffull);
+    S.kt_m2l.resize(std::max<int64_t>(1, L.n_m2l)); S.vs_m2l.resize(std::max<int64_t>(1, L.n_m2l));
+    S.kt_dir.resize(std::max<int64_t>(1, L.n_direct)); S.vs_dir.resize(std::max<int64_t>(1, L.n_direct));
+    // forward: depth-first placement
+    for (int k = 0; k < K; ++k) {
+        ListLevel& A = S.level(k);
+        uint64_t* child_off = nullptr;
+        const uint64_t* child_size = nullptr;
+        if (k + 1 < K) {
+            ListLevel& B = S.level(k + 1);
+            B.off.resize(sizes[k + 1]);
+            child_off = B.off.data();
+            child_size = B.size.data();
+        }
+        k_list_place<<<grid_for(sizes[k], 256), 256, 0, C.stream>>>(A.ft.data(), A.fs.data(), sizes[k], A.dec.data(), A.pos.data(),
+                                                                   A.off.data(), child_size, child_off, S.kt_m2l.data(),
+                                                                   S.vs_m2l.data(), S.kt_dir.data(), S.vs_dir.data());
+        LAUNCH_CHECK();
+    }
+    int bits = 1;
+    while (bits < 32 && (int64_t(1) << bits) < tt.n_branches) ++bits;
+    finish_list(C, S, S.kt_m2l.data(), S.vs_m2l.data(), L.n_m2l, bits, L.m2l_t, L.m2l_s, L.m2l_targets, L.m2l_offsets,
+                L.n_m2l_targets);
+    finish_list(C, S, S.kt_dir.data(), S.vs_dir.data(), L.n_direct, bits, L.dir_t, L.dir_s, L.dir_targets, L.dir_offsets,
+                L.n_direct_targets);
     L.m2l_order.resize(std::max<int64_t>(1, L.n_m2l));
     if (L.n_direct > 0) {
         S.pos.resize(L.n_direct + 1);
@@ -272,6 +332,7 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
     }
     L.n_pairs = 0;
     if (count_pairs && L.n_direct > 0) {
+        S.counter.resize(1);
         k_zero_counter<<<1, 1, 0, C.stream>>>(S.counter.data());
         k_count_pairs<<<std::min(1024u, grid_for(L.n_direct, 256)), 256, 0, C.stream>>>(L.dir_t.data(), L.dir_s.data(), L.n_direct,
                                                                        tt.branches.data(), st.branches.data(), S.counter.data());

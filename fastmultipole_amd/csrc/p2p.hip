@@ -15,11 +15,12 @@
 // target are folded with wavefront shuffles.
 //
 // FP64 form: IEEE double arithmetic with 1/r from a refined hardware rsqrt.
-// Mixed form: FP32 arithmetic; positions are stored relative to the target
-// leaf centre as a hi+lo pair of floats (differences of nearby bodies keep
-// ~2^-48 relative accuracy), and each thread's per-source-leaf FP32 partial
-// sum is added to its running total with Kahan compensation. Strengths are
-// pre-scaled by 1/(4 pi).
+// Mixed form: the source leaves whose bounding sphere overlaps the target
+// leaf's (the closest, largest contributions) in FP64; all other pairs in FP32
+// with positions stored relative to the target leaf centre as a hi+lo pair of
+// floats (differences keep ~2^-48 relative accuracy), each thread's per-tile
+// FP32 partial sum added to its running total with Kahan compensation.
+// Strengths are pre-scaled by 1/(4 pi).
 #include "fmmhip_internal.hpp"
 #include <algorithm>
 
@@ -133,18 +134,25 @@ __device__ inline void stage_source(const double* __restrict__ sq, const double*
 // Source body at position f of the flattened stream of a target leaf's source
 // leaves (direct-list entries l0..l1-1, dir_cum = exclusive prefix of their sizes):
 // tiles span source leaves, so small leaves do not cost a barrier each.
-__device__ inline int64_t stream_body(const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
-                                      const Branch* __restrict__ sbr, int64_t l0, int64_t l1, int64_t f) {
+__device__ inline int64_t stream_entry(const int64_t* __restrict__ dir_cum, int64_t l0, int64_t l1, int64_t f) {
     int64_t lo = l0, hi = l1 - 1;
     while (lo < hi) {
         const int64_t mid = (lo + hi + 1) >> 1;
         if (dir_cum[mid] <= f) lo = mid;
         else hi = mid - 1;
     }
-    return sbr[dir_s[lo]].first + (f - dir_cum[lo]);
+    return lo;
+}
+
+__device__ inline int64_t stream_body(const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
+                                      const Branch* __restrict__ sbr, int64_t l0, int64_t l1, int64_t f) {
+    const int64_t li = stream_entry(dir_cum, l0, l1, f);
+    return sbr[dir_s[li]].first + (f - dir_cum[li]);
 }
 
 // ---------------------------------------------------------------- FP64 form
+// Each thread evaluates two adjacent targets of the leaf against its share of
+// the staged sources, so a source read from LDS serves two pair evaluations.
 template <int KERNEL, bool PS, bool GS, bool HS>
 __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict__ dir_targets,
                                                        const int64_t* __restrict__ dir_offsets,
@@ -157,23 +165,26 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                                                        const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
                                                        int64_t nt) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
+    constexpr int NA = KERNEL == FMMHIP_POINT_SOURCE ? 4 : 13;
     __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
     const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
     const int tid = threadIdx.x;
-    int tpt = 1;   // threads per target
-    while (tpt < 64 && T.n_bodies * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
+    const int64_t ngroups = (T.n_bodies + 1) / 2;           // target pairs
+    int tpt = 1;   // threads per target pair
+    while (tpt < 64 && ngroups * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
     const int per_chunk = P2P_BLOCK / tpt;
-    for (int64_t c0 = 0; c0 < T.n_bodies; c0 += per_chunk) {
-        const int64_t ti = c0 + tid / tpt;
+    for (int64_t c0 = 0; c0 < ngroups; c0 += per_chunk) {
+        const int64_t g = c0 + tid / tpt;
         const int lane_k = tid % tpt;
-        const bool active = ti < T.n_bodies;
-        const int64_t it = T.first + (active ? ti : 0);
-        const double xt = tx[it], yt = ty[it], zt = tz[it];
-        double a[13];
+        const bool act0 = 2 * g < T.n_bodies, act1 = 2 * g + 1 < T.n_bodies;
+        const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
+        const double xa = tx[i0], ya = ty[i0], za = tz[i0];
+        const double xb = tx[i1], yb = ty[i1], zb = tz[i1];
+        double a[NA], b[NA];
 #pragma unroll
-        for (int k = 0; k < 13; ++k) a[k] = 0.0;
+        for (int k = 0; k < NA; ++k) { a[k] = 0.0; b[k] = 0.0; }
         for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
             const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
             __syncthreads();
@@ -183,27 +194,41 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                 stage_source<KERNEL, double>(sq, srad, ns, j, lg[tid]);
             }
             __syncthreads();
-            if (active)
+            if (act0)
                 for (int s = lane_k; s < cnt; s += tpt) {
-                    const double dx = xt - lx[s], dy = yt - ly[s], dz = zt - lz[s];
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dx, dy, dz, r2, rinv_f64(r2), lg[s], a);
+                    const double sxs = lx[s], sys = ly[s], szs = lz[s];
+                    double gs[NS];
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) gs[k] = lg[s][k];
+                    const double dxa = xa - sxs, dya = ya - sys, dza = za - szs;
+                    const double r2a = dxa * dxa + dya * dya + dza * dza;
+                    const double dxb = xb - sxs, dyb = yb - sys, dzb = zb - szs;
+                    const double r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                    if (r2a > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, rinv_f64(r2a), gs, a);
+                    if (r2b > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rinv_f64(r2b), gs, b);
                 }
         }
-        // fold the tpt partial sums of a target (adjacent lanes)
+        // fold the tpt partial sums of a target pair (adjacent lanes)
         for (int w = 1; w < tpt; w <<= 1)
 #pragma unroll
-            for (int k = 0; k < 13; ++k) a[k] += __shfl_xor(a[k], w, 64);
-        if (active && lane_k == 0) {
-            if (PS && KernelTraits<KERNEL>::POT) tout[it] += a[0];
-            if (GS) { tout[nt + it] += a[1]; tout[2 * nt + it] += a[2]; tout[3 * nt + it] += a[3]; }
-            if (HS && KernelTraits<KERNEL>::HESS)
-                for (int k = 4; k < 13; ++k) tout[(int64_t)k * nt + it] += a[k];
+            for (int k = 0; k < NA; ++k) { a[k] += __shfl_xor(a[k], w, 64); b[k] += __shfl_xor(b[k], w, 64); }
+        if (lane_k == 0) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!(h ? act1 : act0)) continue;
+                const int64_t it = h ? i1 : i0;
+                const double* r = h ? b : a;
+                if (PS && KernelTraits<KERNEL>::POT) tout[it] += r[0];
+                if (GS) { tout[nt + it] += r[1]; tout[2 * nt + it] += r[2]; tout[3 * nt + it] += r[3]; }
+                if (HS && KernelTraits<KERNEL>::HESS)
+                    for (int k = 4; k < NA; ++k) tout[(int64_t)k * nt + it] += r[k];
+            }
         }
     }
 }
 
 // --------------------------------------------------------------- mixed form
+
 template <int KERNEL, bool PS, bool GS, bool HS>
 __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict__ dir_targets,
                                                        const int64_t* __restrict__ dir_offsets,
@@ -216,67 +241,87 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                                                        const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
                                                        int64_t nt) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
+    constexpr int NA = KERNEL == FMMHIP_POINT_SOURCE ? 4 : 13;
     __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
-    const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
+    const int64_t base = dir_cum[l0], end = dir_cum[l1];
     const int tid = threadIdx.x;
     const double cx0 = T.target_center[0], cy0 = T.target_center[1], cz0 = T.target_center[2];
+    const int64_t ngroups = (T.n_bodies + 1) / 2;
     int tpt = 1;
-    while (tpt < 64 && T.n_bodies * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
+    while (tpt < 64 && ngroups * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
     const int per_chunk = P2P_BLOCK / tpt;
-    constexpr int NA = KERNEL == FMMHIP_POINT_SOURCE ? 4 : 13;
-    for (int64_t c0 = 0; c0 < T.n_bodies; c0 += per_chunk) {
-        const int64_t ti = c0 + tid / tpt;
+    for (int64_t c0 = 0; c0 < ngroups; c0 += per_chunk) {
+        const int64_t g = c0 + tid / tpt;
         const int lane_k = tid % tpt;
-        const bool active = ti < T.n_bodies;
-        const int64_t it = T.first + (active ? ti : 0);
-        const double rx = tx[it] - cx0, ry = ty[it] - cy0, rz = tz[it] - cz0;
-        const float xh = (float)rx, yh = (float)ry, zh = (float)rz;
-        const float xl = (float)(rx - (double)xh), yl = (float)(ry - (double)yh), zl = (float)(rz - (double)zh);
-        float sum[NA], cmp[NA];
+        const bool act0 = 2 * g < T.n_bodies, act1 = 2 * g + 1 < T.n_bodies;
+        const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
+        float th[2][3], tl[2][3];
 #pragma unroll
-        for (int k = 0; k < NA; ++k) { sum[k] = 0.f; cmp[k] = 0.f; }
-        {
-            for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
-                const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
-                __syncthreads();
-                if (tid < cnt) {
-                    const int64_t j = stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid);
-                    const double px = sx[j] - cx0, py = sy[j] - cy0, pz = sz[j] - cz0;
-                    const float hx = (float)px, hy = (float)py, hz = (float)pz;
-                    lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
-                    ll[0][tid] = (float)(px - (double)hx); ll[1][tid] = (float)(py - (double)hy); ll[2][tid] = (float)(pz - (double)hz);
-                    stage_source<KERNEL, float>(sq, srad, ns, j, lg[tid]);
+        for (int h = 0; h < 2; ++h) {
+            const int64_t it = h ? i1 : i0;
+            const double r[3] = {tx[it] - cx0, ty[it] - cy0, tz[it] - cz0};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) { th[h][d] = (float)r[d]; tl[h][d] = (float)(r[d] - (double)th[h][d]); }
+        }
+        float sum[2][NA], cmp[2][NA];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) { sum[0][k] = sum[1][k] = 0.f; cmp[0][k] = cmp[1][k] = 0.f; }
+        for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
+            const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
+            __syncthreads();
+            if (tid < cnt) {
+                const int64_t li = stream_entry(dir_cum, l0, l1, f0 + tid);
+                const Branch& S = sbr[dir_s[li]];
+                const int64_t j = S.first + (f0 + tid - dir_cum[li]);
+                const double px = sx[j] - cx0, py = sy[j] - cy0, pz = sz[j] - cz0;
+                const float hx = (float)px, hy = (float)py, hz = (float)pz;
+                lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
+                ll[0][tid] = (float)(px - (double)hx); ll[1][tid] = (float)(py - (double)hy); ll[2][tid] = (float)(pz - (double)hz);
+                stage_source<KERNEL, float>(sq, srad, ns, j, lg[tid]);
+            }
+            __syncthreads();
+            if (act0) {
+                float pa[13], pb[13];
+#pragma unroll
+                for (int k = 0; k < 13; ++k) { pa[k] = 0.f; pb[k] = 0.f; }
+                for (int s = lane_k; s < cnt; s += tpt) {
+                    const float shx = lh[0][s], shy = lh[1][s], shz = lh[2][s];
+                    const float slx = ll[0][s], sly = ll[1][s], slz = ll[2][s];
+                    float gs[NS];
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) gs[k] = lg[s][k];
+                    const float dxa = (th[0][0] - shx) + (tl[0][0] - slx), dya = (th[0][1] - shy) + (tl[0][1] - sly),
+                                dza = (th[0][2] - shz) + (tl[0][2] - slz);
+                    const float dxb = (th[1][0] - shx) + (tl[1][0] - slx), dyb = (th[1][1] - shy) + (tl[1][1] - sly),
+                                dzb = (th[1][2] - shz) + (tl[1][2] - slz);
+                    const float r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                    if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
+                    if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
                 }
-                __syncthreads();
-                if (active) {
-                    float part[13];
 #pragma unroll
-                    for (int k = 0; k < 13; ++k) part[k] = 0.f;
-                    for (int s = lane_k; s < cnt; s += tpt) {
-                        const float dx = (xh - lh[0][s]) + (xl - ll[0][s]);
-                        const float dy = (yh - lh[1][s]) + (yl - ll[1][s]);
-                        const float dz = (zh - lh[2][s]) + (zl - ll[2][s]);
-                        const float r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dx, dy, dz, r2, __builtin_amdgcn_rsqf(r2), lg[s], part);
-                    }
-#pragma unroll
-                    for (int k = 0; k < NA; ++k) kahan_add(sum[k], cmp[k], part[k]);
-                }
+                for (int k = 0; k < NA; ++k) { kahan_add(sum[0][k], cmp[0][k], pa[k]); kahan_add(sum[1][k], cmp[1][k], pb[k]); }
             }
         }
-        double acc[NA];
+        double acc[2][NA];
 #pragma unroll
-        for (int k = 0; k < NA; ++k) acc[k] = (double)sum[k] - (double)cmp[k];
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int k = 0; k < NA; ++k) acc[h][k] = (double)sum[h][k] - (double)cmp[h][k];
         for (int w = 1; w < tpt; w <<= 1)
 #pragma unroll
-            for (int k = 0; k < NA; ++k) acc[k] += __shfl_xor(acc[k], w, 64);
-        if (active && lane_k == 0) {
-            if (PS && KernelTraits<KERNEL>::POT) tout[it] += acc[0];
-            if (GS) { tout[nt + it] += acc[1]; tout[2 * nt + it] += acc[2]; tout[3 * nt + it] += acc[3]; }
-            if (HS && KernelTraits<KERNEL>::HESS)
-                for (int k = 4; k < NA; ++k) tout[(int64_t)k * nt + it] += acc[k];
+            for (int k = 0; k < NA; ++k) { acc[0][k] += __shfl_xor(acc[0][k], w, 64); acc[1][k] += __shfl_xor(acc[1][k], w, 64); }
+        if (lane_k == 0) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!(h ? act1 : act0)) continue;
+                const int64_t it = h ? i1 : i0;
+                if (PS && KernelTraits<KERNEL>::POT) tout[it] += acc[h][0];
+                if (GS) { tout[nt + it] += acc[h][1]; tout[2 * nt + it] += acc[h][2]; tout[3 * nt + it] += acc[h][3]; }
+                if (HS && KernelTraits<KERNEL>::HESS)
+                    for (int k = 4; k < NA; ++k) tout[(int64_t)k * nt + it] += acc[h][k];
+            }
         }
     }
 }

@@ -199,6 +199,23 @@ def test_error_controlled_order_matches_oracle(name, gen, kind, em, ls):
         assert np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / scale < 1e-11
 
 
+MFMA_FMM = [c for c in FMM_CASES if c[0] in ("laplace-4k-p8", "vortex-4k-p8", "ring-6k-p10", "ring-3k-p20", "panel-src-tri-3k-p8")]
+MFMA_ERR = [c for c in ERR_CASES if c[0] in ("laplace-rotated-abs", "vortex-power-abs", "ring-power-rel")]
+
+
+@pytest.mark.parametrize("name,gen,kind,kw", MFMA_FMM, ids=[c[0] for c in MFMA_FMM])
+def test_fmm_fp64_matrix_core_m2l(name, gen, kind, kw, monkeypatch):
+    """The matrix-core M2L (FMMHIP_M2L_KERNEL=mfma) meets the same bars as the default kernel."""
+    monkeypatch.setenv("FMMHIP_M2L_KERNEL", "mfma")
+    test_fmm_fp64_matches_oracle(name, gen, kind, kw)
+
+
+@pytest.mark.parametrize("name,gen,kind,em,ls", MFMA_ERR, ids=[c[0] for c in MFMA_ERR])
+def test_error_controlled_matrix_core_m2l(name, gen, kind, em, ls, monkeypatch):
+    monkeypatch.setenv("FMMHIP_M2L_KERNEL", "mfma")
+    test_error_controlled_order_matches_oracle(name, gen, kind, em, ls)
+
+
 def test_direct_fp64_matches_oracle():
     _say("[gpu] direct fp64")
     for kind, sbuf in ((O.KIND_SOURCE, W.uniform_sources(3000, seed=41)), (O.KIND_VORTEX, W.vortex_ring(3000, seed=42)),
