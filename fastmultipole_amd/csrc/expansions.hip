@@ -1164,6 +1164,7 @@ struct MfLds {
     int* ord;      // BP: order of the interaction (MF_EMPTY, MF_SEARCHING)
     int* js;       // BP: source branch
     double* T;     // 32 (P+1) for this wavefront: step-1 products R^e / R^o, z-translation rows
+    double* T0;    // all wavefronts' buffers (NW x 32 (P+1)): error-test scratch between rotations
 };
 
 __host__ __device__ inline int mf_lds_doubles(int P, int BP, int NW) {
@@ -1191,6 +1192,7 @@ __device__ inline MfLds mf_carve(double* b, int P, int BP) {
     W.geo = b; b += 8 * BP;
     W.ord = (int*)b; b += BP;
     W.js = W.ord + BP;
+    W.T0 = b;
     W.T = b + (size_t)wave_id() * 32 * (P + 1);
     return W;
 }
@@ -1381,24 +1383,21 @@ __device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigm
     const int kc = (n + 4) >> 2;
     const double* eq = W.eim + 2 * q * (P + 1);
     mf4 o[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-    double a2[2][8], b2[2][8];
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
         const int ma = 2 * lr + par;
         const double* Tp = par ? To : Te;
+        double a2[8], b2[8];
 #pragma unroll
         for (int cc = 0; cc < 8; ++cc) {
             const int v = 4 * cc + lg;
-            a2[par][cc] = (cc < kc && ma <= n && v <= n) ? a_sym(Hn, ma, v) : 0.0;
-            b2[par][cc] = (cc < kc && v <= n) ? Tp[16 * v + lr] : 0.0;
+            a2[cc] = (cc < kc && ma <= n && v <= n) ? a_sym(Hn, ma, v) : 0.0;
+            b2[cc] = (cc < kc && v <= n) ? Tp[16 * v + lr] : 0.0;
         }
-    }
 #pragma unroll
-    for (int cc = 0; cc < 8; ++cc)
-        if (cc < kc) {
-            o[0] = mfma_f64(a2[0][cc], b2[0][cc], o[0]);
-            o[1] = mfma_f64(a2[1][cc], b2[1][cc], o[1]);
-        }
+        for (int cc = 0; cc < 8; ++cc)
+            if (cc < kc) o[par] = mfma_f64(a2[cc], b2[cc], o[par]);
+    }
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
 #pragma unroll
@@ -1421,15 +1420,44 @@ __device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigm
     wave_sync();
 }
 
-// Error tests of the degrees n0..n1 (n0 >= 1), one lane per (interaction,
-// degree): the tests of m2l_test_chunk, with the degree-n part of the
-// z-translation formed and consumed in local_power's m order inside the lane.
-// Each searching interaction takes its first passing degree d: order d - 1 + BE.
+// Error tests of the degrees n0..n1 (n0 >= 1, at most mf_group(NW) of them): the
+// tests of m2l_test_chunk. For the power methods the whole workgroup first forms
+// the degree-n part of the z-translation (translate_multipole_to_local_z_n!,
+// src/translate.jl:238-295) for every (interaction, degree, m) into the
+// wavefronts' product buffers; then wavefront 0 runs one lane per (interaction,
+// degree) and consumes it in local_power's m order. Each searching interaction
+// takes its first passing degree d: order d - 1 + BE. Called by every thread.
 template <bool LH>
 __device__ void mf_test_group(const MfLds& W, const ErrParams& E, const TablePtrs& tb, double limit, double r_l, int P, int n0,
                               int n1) {
     constexpr int NC = LH ? 4 : 2, BP = 16 / NC;
-    const int G = n1 - n0 + 1;   // <= MF_GROUP
+    const int G = n1 - n0 + 1;
+    if (E.kind >= 2) {
+        for (int it = threadIdx.x; it < BP * G * (P + 1); it += blockDim.x) {
+            const int q = it / (G * (P + 1)), rem = it - q * G * (P + 1);
+            const int gi = rem / (P + 1), m = rem - gi * (P + 1), n = n0 + gi;
+            if (m > n || W.ord[q] != MF_SEARCHING) continue;
+            const double rinv = W.geo[8 * q + 1];
+            const double* Xq = W.X + q * NC;
+            double n_m_f = W.fac[6 * (q * (P + 1) + n)];
+            for (int k = 0; k < m; ++k) n_m_f *= (double)(n + k + 1) * rinv;
+            double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
+            int n_np = n + m;
+            int iw = nh(m - 1) + m;
+            for (int np = m; np <= n; ++np) {
+                v1r += f * Xq[16 * iw]; v1i += f * Xq[16 * iw + 1];
+                if (LH) { v2r += f * Xq[16 * iw + 2]; v2i += f * Xq[16 * iw + 3]; }
+                n_np += 1;
+                f *= (double)n_np * rinv;
+                iw += np + 1;
+            }
+            double* t = W.T0 + (size_t)NC * ((q * G + gi) * (P + 1) + m);
+            t[0] = v1r; t[1] = v1i;
+            if (LH) { t[2] = v2r; t[3] = v2i; }
+        }
+        __syncthreads();
+    }
+    if (wave_id() == 0) {
     const int l = lane_id();
     const int q = l / MF_GROUP, n = n0 + (l - q * MF_GROUP);
     bool pass = false;
@@ -1493,26 +1521,14 @@ __device__ void mf_test_group(const MfLds& W, const ErrParams& E, const TablePtr
                 eps_l = (fabs(p1r) + fabs(p1i) + fabs(p0r)) * r_l_nm1 * nm1_f_inv;
                 if (LH) eps_l += sqrt(c1r * c1r + c1i * c1i) * r_l_nm1 * r_l * nm1_f_inv;
             } else {
-                // degree-n part of the z-translation (translate_multipole_to_local_z_n!, :238-295),
-                // consumed by local_power (:455-502) in its m order
+                // local_power (:455-502) of the degree-n translation, in its m order
+                const double* tq = W.T0 + (size_t)NC * ((q * G + (n - n0)) * (P + 1));
                 double lp = 0, lc = 0, Nt = 0, cnr = 0, cni = 0;
                 for (int m = n; m >= 0; --m) {
-                    double n_m_f = n_f_t_np1;
-                    for (int k = 0; k < m; ++k) n_m_f *= (double)(n + k + 1) * rinv;
-                    double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
-                    int n_np = n + m;
-                    int iw = nh(m - 1) + m;
-                    for (int np = m; np <= n; ++np) {
-                        v1r += f * Xq[16 * iw]; v1i += f * Xq[16 * iw + 1];
-                        if (LH) { v2r += f * Xq[16 * iw + 2]; v2i += f * Xq[16 * iw + 3]; }
-                        n_np += 1;
-                        f *= (double)n_np * rinv;
-                        iw += np + 1;
-                    }
                     Nt = tb.Lt[in0 + m];
-                    double pr = v1r, pi = v1i;
+                    double pr = tq[NC * m], pi = tq[NC * m + 1];
                     if (LH) {
-                        const double cor = v2r, coi = v2i;
+                        const double cor = tq[NC * m + 2], coi = tq[NC * m + 3];
                         pr = pr - coi * r * (double)m / (double)n;
                         pi = pi + cor * r * (double)m / (double)n;
                         const double cr = cor - cnr * r / (double)(n + 1);
@@ -1541,7 +1557,8 @@ __device__ void mf_test_group(const MfLds& W, const ErrParams& E, const TablePtr
         const unsigned long long mine = (mask >> (l * MF_GROUP)) & ((1ull << G) - 1ull);
         if (mine) W.ord[l] = n0 + __ffsll((long long)mine) - 1 - 1 + (E.BE ? 1 : 0);
     }
-    wave_sync();
+    }
+    __syncthreads();
 }
 
 // z-translation of column m of the batch at each interaction's own order (rows
@@ -1633,12 +1650,13 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mfma(const int32_t* __restrict_
             for (int n = wv; n <= P; n += NW) mf_rotate_degree<LH, false>(W, W.zout, 1.0, P, n);
             __syncthreads();
         } else {
-            for (int n0 = 0; n0 <= P; n0 += MF_GROUP) {
-                const int n1 = min(P, n0 + MF_GROUP - 1);
+            // the tests' scratch (BP x group x (P+1) x NC) lives in the NW product buffers
+            constexpr int GMAX = (2 * NW < MF_GROUP) ? 2 * NW : MF_GROUP;
+            for (int n0 = 0; n0 <= P; n0 += GMAX) {
+                const int n1 = min(P, n0 + GMAX - 1);
                 for (int n = n0 + wv; n <= n1; n += NW) mf_rotate_degree<LH, false>(W, W.zout, 1.0, P, n);
                 __syncthreads();
-                if (wv == 0) mf_test_group<LH>(W, E, tb, limit, T.target_radius, P, max(n0, 1), n1);
-                __syncthreads();
+                mf_test_group<LH>(W, E, tb, limit, T.target_radius, P, max(n0, 1), n1);
                 bool searching = false;
 #pragma unroll
                 for (int q = 0; q < BP; ++q) searching |= (W.ord[q] == MF_SEARCHING);
@@ -1674,6 +1692,8 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mfma(const int32_t* __restrict_
 template <bool LH>
 static int mf_waves(int P) {
     constexpr int BP = LH ? 4 : 8;
+    // two 4-wavefront workgroups per CU keep two batches in flight when their LDS fits
+    if (2 * mf_lds_bytes(P, BP, 4) <= LDS_LIMIT) return 4;
     for (int nw : {8, 4, 2, 1})
         if (mf_lds_bytes(P, BP, nw) <= LDS_LIMIT) return nw;
     throw Error(FMMHIP_EINVAL, "expansion order too large for the M2L kernel");

@@ -30,11 +30,10 @@ static constexpr int P2P_BLOCK = 256;
 static constexpr double INV4PI = 1.0 / (4.0 * 3.141592653589793);
 
 __device__ inline double rinv_f64(double r2) {
-    double y = __builtin_amdgcn_rsq(r2);            // ~23-bit estimate
-    double h = 0.5 * r2;
-    y = y * (1.5 - h * y * y);                       // Newton: ~46 bits
-    y = y * (1.5 - h * y * y);                       // Newton: full double
-    return y;
+    const double y = __builtin_amdgcn_rsq(r2);      // ~23-bit estimate
+    // one third-order step: y (1 + e/2 + 3e^2/8), e = 1 - r2 y^2 (|e| ~ 2^-23 -> 2^-69)
+    const double e = fma(-r2 * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // Kahan-compensated add of a partial sum into (s, c)
@@ -51,6 +50,13 @@ template <int K> struct KernelTraits {
     static constexpr int NS = K == FMMHIP_POINT_SOURCE ? 1 : (K == FMMHIP_VORTEX_GAUSSIAN ? 4 : 3);
     static constexpr bool HESS = K != FMMHIP_POINT_SOURCE;   // the reference's source kernel has no Hessian term
     static constexpr bool POT = K == FMMHIP_POINT_SOURCE || K == FMMHIP_POINT_DIPOLE;
+    // outputs [phi, gradient (3), Hessian (9)]; in FP64 the point vortex keeps three
+    // more sums, sum_j Gamma_j / r_j^3, whose antisymmetric part of the Hessian is
+    // added once per target (fold_acc) instead of once per pair. FP32 sums keep the
+    // per-pair form: the deferred sums cancel against the rest of the Hessian.
+    static constexpr int NOUT = K == FMMHIP_POINT_SOURCE ? 4 : 13;
+    template <class R> static constexpr bool defer() { return K == FMMHIP_POINT_VORTEX && sizeof(R) == 8; }
+    template <class R> static constexpr int na() { return defer<R>() ? 16 : NOUT; }
 };
 
 template <class R> __device__ inline R erf_(R x);
@@ -106,20 +112,38 @@ __device__ inline void pair_eval(R dx, R dy, R dz, R r2, R ri, const R* g, R* a)
         if (HS) {
             const R d5 = d3 * ri2, t3 = R(-3) * d5;
             const R ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
-            const R g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
-            R h[9] = {ax * cx,       ax * cy + g3z, ax * cz - g3y,
-                      ay * cx - g3z, ay * cy,       ay * cz + g3x,
-                      az * cx + g3y, az * cy - g3x, az * cz};
-            if (K == FMMHIP_VORTEX_GAUSSIAN) {
-                const R c[3] = {cx * d3, cy * d3, cz * d3}, d[3] = {dx, dy, dz};
+            if (KernelTraits<K>::template defer<R>()) {
+                a[4] += ax * cx; a[5] += ax * cy; a[6] += ax * cz;
+                a[7] += ay * cx; a[8] += ay * cy; a[9] += ay * cz;
+                a[10] += az * cx; a[11] += az * cy; a[12] += az * cz;
+                a[13] += gx * d3; a[14] += gy * d3; a[15] += gz * d3;
+            } else {
+                const R g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
+                R h[9] = {ax * cx,       ax * cy + g3z, ax * cz - g3y,
+                          ay * cx - g3z, ay * cy,       ay * cz + g3x,
+                          az * cx + g3y, az * cy - g3x, az * cz};
+                if (K == FMMHIP_VORTEX_GAUSSIAN) {
+                    const R c[3] = {cx * d3, cy * d3, cz * d3}, d[3] = {dx, dy, dz};
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
+                    for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) h[3 * i + j] = gf * h[3 * i + j] + sri * d[i] * c[j];
+                        for (int j = 0; j < 3; ++j) h[3 * i + j] = gf * h[3 * i + j] + sri * d[i] * c[j];
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) a[4 + k] += h[k];
             }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) a[4 + k] += h[k];
         }
+    }
+}
+
+// adds the deferred sums of the point vortex, S = sum_j Gamma_j / r_j^3, to the
+// Hessian slots: the per-pair terms were H(0,1) += S_z, H(0,2) -= S_y, H(1,0) -= S_z,
+// H(1,2) += S_x, H(2,0) += S_y, H(2,1) -= S_x
+template <int K, bool HS, class R>
+__device__ inline void fold_acc(R* a) {
+    if constexpr (KernelTraits<K>::template defer<R>() && HS) {
+        a[5] += a[15]; a[6] -= a[14]; a[7] -= a[15];
+        a[9] += a[13]; a[10] += a[14]; a[11] -= a[13];
     }
 }
 
@@ -165,7 +189,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                                                        const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
                                                        int64_t nt) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
-    constexpr int NA = KERNEL == FMMHIP_POINT_SOURCE ? 4 : 13;
+    constexpr int NA = KernelTraits<KERNEL>::template na<double>(), NOUT = KernelTraits<KERNEL>::NOUT;
     __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
@@ -204,14 +228,22 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                     const double r2a = dxa * dxa + dya * dya + dza * dza;
                     const double dxb = xb - sxs, dyb = yb - sys, dzb = zb - szs;
                     const double r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                    if (r2a > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, rinv_f64(r2a), gs, a);
-                    if (r2b > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rinv_f64(r2b), gs, b);
+                    if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
+                        if (r2a > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, rinv_f64(r2a), gs, a);
+                        if (r2b > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rinv_f64(r2b), gs, b);
+                    } else {   // a coincident pair (r = 0) contributes nothing: 1/r is selected to 0
+                        const double ria = r2a > 0.0 ? rinv_f64(r2a) : 0.0, rib = r2b > 0.0 ? rinv_f64(r2b) : 0.0;
+                        pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, ria, gs, a);
+                        pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rib, gs, b);
+                    }
                 }
         }
         // fold the tpt partial sums of a target pair (adjacent lanes)
         for (int w = 1; w < tpt; w <<= 1)
 #pragma unroll
             for (int k = 0; k < NA; ++k) { a[k] += __shfl_xor(a[k], w, 64); b[k] += __shfl_xor(b[k], w, 64); }
+        fold_acc<KERNEL, HS, double>(a);
+        fold_acc<KERNEL, HS, double>(b);
         if (lane_k == 0) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -221,7 +253,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                 if (PS && KernelTraits<KERNEL>::POT) tout[it] += r[0];
                 if (GS) { tout[nt + it] += r[1]; tout[2 * nt + it] += r[2]; tout[3 * nt + it] += r[3]; }
                 if (HS && KernelTraits<KERNEL>::HESS)
-                    for (int k = 4; k < NA; ++k) tout[(int64_t)k * nt + it] += r[k];
+                    for (int k = 4; k < NOUT; ++k) tout[(int64_t)k * nt + it] += r[k];
             }
         }
     }
@@ -241,7 +273,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                                                        const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
                                                        int64_t nt) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
-    constexpr int NA = KERNEL == FMMHIP_POINT_SOURCE ? 4 : 13;
+    constexpr int NA = KernelTraits<KERNEL>::template na<float>(), NOUT = KernelTraits<KERNEL>::NOUT;
     __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[P2P_BLOCK][NS];
     const Branch T = tbr[dir_targets[blockIdx.x]];
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
@@ -283,9 +315,9 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
             }
             __syncthreads();
             if (act0) {
-                float pa[13], pb[13];
+                float pa[NA], pb[NA];
 #pragma unroll
-                for (int k = 0; k < 13; ++k) { pa[k] = 0.f; pb[k] = 0.f; }
+                for (int k = 0; k < NA; ++k) { pa[k] = 0.f; pb[k] = 0.f; }
                 for (int s = lane_k; s < cnt; s += tpt) {
                     const float shx = lh[0][s], shy = lh[1][s], shz = lh[2][s];
                     const float slx = ll[0][s], sly = ll[1][s], slz = ll[2][s];
@@ -297,8 +329,15 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                     const float dxb = (th[1][0] - shx) + (tl[1][0] - slx), dyb = (th[1][1] - shy) + (tl[1][1] - sly),
                                 dzb = (th[1][2] - shz) + (tl[1][2] - slz);
                     const float r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                    if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
-                    if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
+                    if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
+                        if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
+                        if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
+                    } else {
+                        const float ria = r2a > 0.f ? __builtin_amdgcn_rsqf(r2a) : 0.f;
+                        const float rib = r2b > 0.f ? __builtin_amdgcn_rsqf(r2b) : 0.f;
+                        pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, ria, gs, pa);
+                        pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, rib, gs, pb);
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < NA; ++k) { kahan_add(sum[0][k], cmp[0][k], pa[k]); kahan_add(sum[1][k], cmp[1][k], pb[k]); }
@@ -320,7 +359,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                 if (PS && KernelTraits<KERNEL>::POT) tout[it] += acc[h][0];
                 if (GS) { tout[nt + it] += acc[h][1]; tout[2 * nt + it] += acc[h][2]; tout[3 * nt + it] += acc[h][3]; }
                 if (HS && KernelTraits<KERNEL>::HESS)
-                    for (int k = 4; k < NA; ++k) tout[(int64_t)k * nt + it] += acc[h][k];
+                    for (int k = 4; k < NOUT; ++k) tout[(int64_t)k * nt + it] += acc[h][k];
             }
         }
     }
@@ -385,8 +424,9 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restri
     const int64_t it = blockIdx.x * (int64_t)P2P_BLOCK + threadIdx.x;
     const bool active = it < nt;
     const double xt = active ? txyz[3 * it] : 0.0, yt = active ? txyz[3 * it + 1] : 0.0, zt = active ? txyz[3 * it + 2] : 0.0;
-    R a[13];
-    for (int k = 0; k < 13; ++k) a[k] = R(0);
+    constexpr int NA = KernelTraits<KERNEL>::template na<R>();
+    R a[NA];
+    for (int k = 0; k < NA; ++k) a[k] = R(0);
     for (int64_t s0 = 0; s0 < ns; s0 += P2P_BLOCK) {
         const int64_t cnt = std::min<int64_t>(P2P_BLOCK, ns - s0);
         __syncthreads();
@@ -409,6 +449,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restri
         }
     }
     if (!active) return;
+    fold_acc<KERNEL, true, R>(a);
     double* o = out + it * 16;
     if (KERNEL == FMMHIP_POINT_SOURCE) {
         o[3] += a[0];          // the reference kernel ignores the switches (test/gravitational.jl:92-98)
