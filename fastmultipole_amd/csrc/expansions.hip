@@ -1163,13 +1163,13 @@ struct MfLds {
     double* geo;   // BP x 8: t, 1/t, r_mp, 1/r_mp, dx, dy, dz
     int* ord;      // BP: order of the interaction (MF_EMPTY, MF_SEARCHING)
     int* js;       // BP: source branch
-    double* T;     // 32 (P+1) for this wavefront: step-1 products R^e / R^o, z-translation rows
-    double* T0;    // all wavefronts' buffers (NW x 32 (P+1)): error-test scratch between rotations
+    double* T;     // 16 (P+1) for this wavefront: z-translation rows
+    double* T0;    // all wavefronts' buffers (NW x 16 (P+1)): error-test scratch between rotations
 };
 
 __host__ __device__ inline int mf_lds_doubles(int P, int BP, int NW) {
     const int H = nh(P);
-    return len_Hs(P) + 4 * H + 2 * P + 2 + 16 * H + 4 * H + BP * (P + 1) * 12 + BP * (P + 2) + 8 * BP + BP + NW * 32 * (P + 1);
+    return len_Hs(P) + 4 * H + 2 * P + 2 + 16 * H + 4 * H + BP * (P + 1) * 12 + BP * (P + 2) + 8 * BP + BP + NW * 16 * (P + 1);
 }
 static size_t mf_lds_bytes(int P, int BP, int NW) { return sizeof(double) * (size_t)mf_lds_doubles(P, BP, NW); }
 
@@ -1193,7 +1193,7 @@ __device__ inline MfLds mf_carve(double* b, int P, int BP) {
     W.ord = (int*)b; b += BP;
     W.js = W.ord + BP;
     W.T0 = b;
-    W.T = b + (size_t)wave_id() * 32 * (P + 1);
+    W.T = b + (size_t)wave_id() * 16 * (P + 1);
     return W;
 }
 
@@ -1324,10 +1324,13 @@ __device__ void mf_load(const MfLds& W, const double* __restrict__ sex, int H4, 
 
 // One y-rotation of degree n for the whole batch, by one wavefront (the
 // factorised form above): step 1 SE / SO = A(v, even / odd m') X(m') on the matrix
-// cores, combination into R^e / R^o (T), step 2 out(m) = A(m, v) R^(parity m)(v),
-// scaled by mout(m) / 2. Forward: out -> X (degree n rows). BACK: out -> back
-// z-rotation -> sum over the interactions of the batch -> acc. The operand loops
-// run to fixed bounds (degree <= 30) so that their loads issue ahead of the products.
+// cores, combination into R^e / R^o, step 2 out(m) = A(m, v) R^(parity m)(v),
+// scaled by mout(m) / 2. The result layout of step 1 (row v = 16 t + l/16 + 4 r in
+// element r) is the operand layout step 2 needs for the chunk of rows 4 cc + l/16
+// with cc = 4 t + r, so R^e / R^o pass from step 1 to step 2 in registers.
+// Forward: out -> X (degree n rows). BACK: out -> back z-rotation -> sum over the
+// interactions of the batch -> acc. The operand loops run to fixed bounds
+// (degree <= 30) so that their loads issue ahead of the products.
 template <bool LH, bool BACK>
 __device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigma, int P, int n) {
     constexpr int NC = LH ? 4 : 2;
@@ -1339,10 +1342,13 @@ __device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigm
     const int ne = (n >> 1) + 1, no = (n + 1) >> 1;
     const int ce = (ne + 3) >> 2, co = (no + 3) >> 2;
     const double* csq = W.cs + 2 * q * (P + 1);
-    double* Te = W.T;
-    double* To = W.T + 16 * (P + 1);
     const double x0r = Xn[q * NC + 2 * pot], x0i = Xn[q * NC + 2 * pot + 1];
-    for (int t = 0; 16 * t <= n; ++t) {
+    double re[2][4], ro[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { re[t][r] = 0.0; ro[t][r] = 0.0; }
+        if (16 * t > n) continue;
         mf4 se = {0.0, 0.0, 0.0, 0.0}, so = {0.0, 0.0, 0.0, 0.0};
         const int va = 16 * t + lr;
         double ae[4], be[4], ao[4], bo[4];
@@ -1375,28 +1381,25 @@ __device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigm
             const double pmr = dr + sor + sg * (ser - sor), pmi = di + soi - sg * (sei - soi);
             const double qpr = cb * ppr - sb * ppi, qpi = cb * ppi + sb * ppr;
             const double qmr = cb * pmr + sb * pmi, qmi = cb * pmi - sb * pmr;
-            Te[16 * v + lr] = isim ? (qpi + qmi) : (qpr + qmr);
-            To[16 * v + lr] = isim ? (qpi - qmi) : (qpr - qmr);
+            re[t][r] = isim ? (qpi + qmi) : (qpr + qmr);
+            ro[t][r] = isim ? (qpi - qmi) : (qpr - qmr);
         }
     }
-    wave_sync();
     const int kc = (n + 4) >> 2;
     const double* eq = W.eim + 2 * q * (P + 1);
     mf4 o[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
         const int ma = 2 * lr + par;
-        const double* Tp = par ? To : Te;
-        double a2[8], b2[8];
+        double a2[8];
 #pragma unroll
         for (int cc = 0; cc < 8; ++cc) {
             const int v = 4 * cc + lg;
             a2[cc] = (cc < kc && ma <= n && v <= n) ? a_sym(Hn, ma, v) : 0.0;
-            b2[cc] = (cc < kc && v <= n) ? Tp[16 * v + lr] : 0.0;
         }
 #pragma unroll
         for (int cc = 0; cc < 8; ++cc)
-            if (cc < kc) o[par] = mfma_f64(a2[cc], b2[cc], o[par]);
+            if (cc < kc) o[par] = mfma_f64(a2[cc], par ? ro[cc >> 2][cc & 3] : re[cc >> 2][cc & 3], o[par]);
     }
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
@@ -1410,8 +1413,8 @@ __device__ void mf_rotate_degree(const MfLds& W, const double* mout, double sigm
             } else {
                 const double vp = __shfl_xor(val, 1, 64);
                 const double er = in ? eq[2 * m] : 1.0, ei = in ? -eq[2 * m + 1] : 0.0;   // back_rotate_z!
-                const double re = isim ? vp : val, im = isim ? val : vp;
-                double y = isim ? (re * ei + im * er) : (re * er - im * ei);
+                const double re2 = isim ? vp : val, im2 = isim ? val : vp;
+                double y = isim ? (re2 * ei + im2 * er) : (re2 * er - im2 * ei);
                 for (int k = NC; k < 16; k <<= 1) y += __shfl_xor(y, k, 64);
                 if (q == 0 && in) W.acc[4 * (nh(n - 1) + m) + c] += y;
             }
@@ -1444,6 +1447,7 @@ __device__ void mf_test_group(const MfLds& W, const ErrParams& E, const TablePtr
             double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
             int n_np = n + m;
             int iw = nh(m - 1) + m;
+            #pragma unroll 4
             for (int np = m; np <= n; ++np) {
                 v1r += f * Xq[16 * iw]; v1i += f * Xq[16 * iw + 1];
                 if (LH) { v2r += f * Xq[16 * iw + 2]; v2i += f * Xq[16 * iw + 3]; }
@@ -1524,6 +1528,7 @@ __device__ void mf_test_group(const MfLds& W, const ErrParams& E, const TablePtr
                 // local_power (:455-502) of the degree-n translation, in its m order
                 const double* tq = W.T0 + (size_t)NC * ((q * G + (n - n0)) * (P + 1));
                 double lp = 0, lc = 0, Nt = 0, cnr = 0, cni = 0;
+                #pragma unroll 4
                 for (int m = n; m >= 0; --m) {
                     Nt = tb.Lt[in0 + m];
                     double pr = tq[NC * m], pi = tq[NC * m + 1];
@@ -1615,7 +1620,7 @@ __device__ void mf_translate_m(const MfLds& W, int P, int Pb, int m) {
 // One workgroup per target box; its wavefronts share each batch of BP sources:
 // degrees (rotations) and columns m (z-translation) are dealt out to them.
 template <bool LH, int NW>
-__global__ __launch_bounds__(64 * NW) void k_m2l_mfma(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
+__global__ __launch_bounds__(64 * NW, 2) void k_m2l_mfma(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
                                                       const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
                                                       const Branch* __restrict__ sbr, const double* __restrict__ sex,
                                                       double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
@@ -1651,7 +1656,7 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mfma(const int32_t* __restrict_
             __syncthreads();
         } else {
             // the tests' scratch (BP x group x (P+1) x NC) lives in the NW product buffers
-            constexpr int GMAX = (2 * NW < MF_GROUP) ? 2 * NW : MF_GROUP;
+            constexpr int GMAX = (NW < MF_GROUP) ? NW : MF_GROUP;
             for (int n0 = 0; n0 <= P; n0 += GMAX) {
                 const int n1 = min(P, n0 + GMAX - 1);
                 for (int n = n0 + wv; n <= n1; n += NW) mf_rotate_degree<LH, false>(W, W.zout, 1.0, P, n);
