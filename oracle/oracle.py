@@ -40,6 +40,19 @@ def lib():
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
         L = C.CDLL(path)
+        L.orc_fgs_create.restype = C.c_void_p
+        L.orc_fgs_create.argtypes = [C.c_int, C.c_int64, _dp, _ic, _dp]
+        L.orc_fgs_free.argtypes = [C.c_void_p]
+        L.orc_fgs_free.restype = None
+        L.orc_fgs_n_leaves.argtypes = [C.c_void_p]
+        L.orc_fgs_n_leaves.restype = C.c_int64
+        L.orc_fgs_leaf_info.argtypes = [C.c_void_p, _ip64]
+        for f in ("orc_fgs_self", "orc_fgs_nonself"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int64, _dp]
+        for f in ("orc_fgs_nonself_rows", "orc_fgs_leaf_bodies"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int64, _ip64]
+        L.orc_fgs_solve.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.orc_fgs_solve.restype = C.c_int
         L.orc_run_create.restype = C.c_void_p
         L.orc_run_create.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ic, _dp]
         for f in ("orc_run_free", "orc_run_fmm", "orc_run_trees", "orc_run_lists"):
@@ -424,3 +437,65 @@ def full_to_compressed(v, p):
                 out.append(v[i])
             i += 1
     return np.array(out)
+
+
+INFLUENCE_POTENTIAL, INFLUENCE_NORMAL_GRADIENT = 0, 1
+
+
+class GaussSeidel:
+    """FastGaussSeidel + solve! (src/solve.jl:538-946) for one system that is both
+    target and source (point sources or source panels). influence: the scalar
+    potential (test/solve_test.jl:11-13) or, for panels, the gradient along the
+    panel normal. Matrices are returned column-major as in the reference."""
+
+    def __init__(self, kind, source_data, expansion_order=4, multipole_acceptance=0.5, leaf_size=30,
+                 shrink_recenter=True, influence=INFLUENCE_POTENTIAL):
+        s = np.ascontiguousarray(source_data, dtype=np.float64)
+        assert s.shape[1] == data_per_body(kind)
+        self.kind, self.n = kind, len(s)
+        oi = np.array([expansion_order, leaf_size, int(shrink_recenter), influence], dtype=np.intc)
+        od = np.array([multipole_acceptance], dtype=np.float64)
+        self._h = lib().orc_fgs_create(kind, self.n, np.ascontiguousarray(s.ravel()), oi, od)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_fgs_free(self._h)
+            self._h = None
+
+    def leaves(self):
+        """(n_leaves, 4): [branch, n_bodies, nonself rows, first sorted body] per leaf, leaf_index order."""
+        n = lib().orc_fgs_n_leaves(self._h)
+        out = np.zeros(4 * n, dtype=np.int64)
+        lib().orc_fgs_leaf_info(self._h, out)
+        return out.reshape(n, 4)
+
+    def self_matrix(self, k):
+        nb = int(self.leaves()[k, 1])
+        out = np.zeros(nb * nb)
+        lib().orc_fgs_self(self._h, k, out)
+        return out.reshape(nb, nb).T
+
+    def nonself_matrix(self, k):
+        info = self.leaves()[k]
+        out = np.zeros(int(info[1] * info[2]))
+        lib().orc_fgs_nonself(self._h, k, out)
+        return out.reshape(int(info[1]), int(info[2])).T
+
+    def nonself_rows(self, k):
+        out = np.zeros(int(self.leaves()[k, 2]), dtype=np.int64)
+        lib().orc_fgs_nonself_rows(self._h, k, out)
+        return out
+
+    def leaf_bodies(self, k):
+        out = np.zeros(int(self.leaves()[k, 1]), dtype=np.int64)
+        lib().orc_fgs_leaf_bodies(self._h, k, out)
+        return out
+
+    def solve(self, external, max_iterations=10, tolerance=1e-3):
+        """Returns (strengths, targets (n,16), sweeps, mse history); original body order."""
+        ext = np.ascontiguousarray(external, dtype=np.float64)
+        strengths = np.zeros(self.n)
+        targets = np.zeros(self.n * 16)
+        hist = np.zeros(max(1, max_iterations))
+        sweeps = lib().orc_fgs_solve(self._h, ext, max_iterations, tolerance, strengths, targets, hist)
+        return strengths, targets.reshape(-1, 16), sweeps, hist[:max(1, max_iterations)]

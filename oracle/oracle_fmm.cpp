@@ -850,6 +850,300 @@ static void fmm_run(Run& R, bool estimate_pass) {
     }
 }
 
+
+// ------------------------------------------------------------ FastGaussSeidel
+// src/solve.jl:1-1022 for one system that is both target and source. The
+// influence is the scalar potential (test/solve_test.jl:11-13) or, for panels, the
+// gradient's component along the target panel's unit normal. Differences from
+// the reference, both deliberate: the mean squared residual is taken over every
+// body of every leaf (the reference's residual! of :995-1021 reuses one leaf-sized
+// vector, so its value is that of the last leaves written), and the iteration count
+// returned is the number of Gauss-Seidel sweeps performed.
+struct FGSolver {
+    Run R;
+    int influence = 0;                                    // 0 potential, 1 normal gradient (panels)
+    std::vector<std::array<int32_t, 2>> sorted, full_direct;   // by source; direct + self pairs by target
+    std::vector<int64_t> leaf_of_branch, leaf_start, lpos;     // leaf-order positions (strengths_by_leaf)
+    std::vector<std::vector<double>> self, nonself;       // column-major (rows x cols), per leaf
+    std::vector<int64_t> ns_rows;
+    std::vector<std::array<int64_t, 2>> index_map;        // [first, last) of the leaf's pairs in `sorted`
+    std::vector<int64_t> ns_rhs_off;                      // offset of the leaf's rows in `old_infl`
+    std::vector<double> strengths, rhs, old_infl, extra;
+    std::vector<double> mse_history;
+    int sweeps = 0;
+};
+
+static void panel_unit_normal(int kind, const double* s, double n[3]);
+
+// influence of a unit-strength source body js on target body it (sorted indices)
+static double unit_influence(const FGSolver& F, int64_t it, int64_t js) {
+    const Run& R = F.R;
+    const int dpb = data_per_body(R.kind);
+    double tb[16] = {0};
+    for (int d = 0; d < 3; ++d) tb[d] = R.tt.buffer[(size_t)it * 16 + d];
+    std::vector<double> sb(R.st.buffer.begin() + (size_t)js * dpb, R.st.buffer.begin() + (size_t)(js + 1) * dpb);
+    sb[4] = 1.0;                                          // set_unit_strength! (:62-70)
+    user_direct(R.kind, tb, 0, 1, sb.data(), dpb, 0, 1, true, F.influence == 1, false);
+    if (F.influence == 0) return tb[3];
+    double nrm[3];
+    panel_unit_normal(R.kind, &R.st.buffer[(size_t)it * dpb], nrm);
+    return tb[4] * nrm[0] + tb[5] * nrm[1] + tb[6] * nrm[2];
+}
+
+static double body_influence(const FGSolver& F, const double* t, int64_t it) {   // influence! on a filled target row
+    if (F.influence == 0) return t[3];
+    double nrm[3];
+    panel_unit_normal(F.R.kind, &F.R.st.buffer[(size_t)it * data_per_body(F.R.kind)], nrm);
+    return t[4] * nrm[0] + t[5] * nrm[1] + t[6] * nrm[2];
+}
+
+static void fgs_setup(FGSolver& F) {
+    Run& R = F.R;
+    const Options& o = R.opt;
+    const int dpb = data_per_body(R.kind);
+    build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p);
+    build_tree(R.st, R.ns, dpb, R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p);
+    // lists in discovery order, farfield + nearfield, no self pairs (:569-571)
+    ListBuilder L{R.tt, R.st, o.ls_source, o.theta, true, true, false, o.method, {}, {}};
+    L.run(0, 0);
+    R.m2l = L.m2l;
+    R.direct = L.direct;
+    // sort_by_source (src/interaction_list.jl:215-245): stable counting sort
+    {
+        const size_t nb = R.st.branches.size();
+        std::vector<int64_t> cnt(nb + 1, 0);
+        for (auto& e : R.direct) cnt[e[1] + 1] += 1;
+        for (size_t i = 1; i <= nb; ++i) cnt[i] += cnt[i - 1];
+        F.sorted.assign(R.direct.size(), {0, 0});
+        for (auto& e : R.direct) F.sorted[cnt[e[1]]++] = e;
+    }
+    // add_self_interactions (:515-536)
+    {
+        auto full = R.direct;
+        for (int64_t b : R.st.leaf_index) full.push_back({(int32_t)b, (int32_t)b});
+        F.full_direct = sort_by_target(full, R.tt.branches.size());
+    }
+    const size_t nl = R.st.leaf_index.size();
+    F.leaf_of_branch.assign(R.st.branches.size(), -1);
+    F.leaf_start.assign(nl + 1, 0);
+    F.lpos.assign(R.ns, 0);
+    for (size_t k = 0; k < nl; ++k) {
+        const OBranch& b = R.st.branches[R.st.leaf_index[k]];
+        F.leaf_of_branch[R.st.leaf_index[k]] = (int64_t)k;
+        F.leaf_start[k + 1] = F.leaf_start[k] + b.n_bodies;
+        for (int64_t i = 0; i < b.n_bodies; ++i) F.lpos[b.first + i] = F.leaf_start[k] + i;   // map_by_leaf (:470-495)
+    }
+    // index_by_source (:292-371)
+    F.index_map.assign(nl, {0, 0});
+    for (size_t a = 0; a < F.sorted.size();) {
+        size_t e = a;
+        while (e < F.sorted.size() && F.sorted[e][1] == F.sorted[a][1]) ++e;
+        F.index_map[F.leaf_of_branch[F.sorted[a][1]]] = {(int64_t)a, (int64_t)e};
+        a = e;
+    }
+    // nonself_influence_matrices (:94-290)
+    F.nonself.assign(nl, {});
+    F.ns_rows.assign(nl, 0);
+    F.ns_rhs_off.assign(nl + 1, 0);
+    for (size_t k = 0; k < nl; ++k) {
+        const OBranch& sb = R.st.branches[R.st.leaf_index[k]];
+        int64_t rows = 0;
+        for (int64_t q = F.index_map[k][0]; q < F.index_map[k][1]; ++q) rows += R.tt.branches[F.sorted[q][0]].n_bodies;
+        F.ns_rows[k] = rows;
+        F.ns_rhs_off[k + 1] = F.ns_rhs_off[k] + rows;
+        auto& M = F.nonself[k];
+        M.assign((size_t)rows * sb.n_bodies, 0.0);
+        int64_t r0 = 0;
+        for (int64_t q = F.index_map[k][0]; q < F.index_map[k][1]; ++q) {
+            const OBranch& tb = R.tt.branches[F.sorted[q][0]];
+            for (int64_t c = 0; c < sb.n_bodies; ++c)
+                for (int64_t r = 0; r < tb.n_bodies; ++r)
+                    M[(size_t)c * rows + r0 + r] = unit_influence(F, tb.first + r, sb.first + c);
+            r0 += tb.n_bodies;
+        }
+    }
+    // self_influence_matrices (:378-468)
+    F.self.assign(nl, {});
+    for (size_t k = 0; k < nl; ++k) {
+        const OBranch& b = R.st.branches[R.st.leaf_index[k]];
+        auto& M = F.self[k];
+        M.assign((size_t)b.n_bodies * b.n_bodies, 0.0);
+        for (int64_t c = 0; c < b.n_bodies; ++c)
+            for (int64_t r = 0; r < b.n_bodies; ++r) M[(size_t)c * b.n_bodies + r] = unit_influence(F, b.first + r, b.first + c);
+    }
+    F.strengths.assign(R.ns, 0.0);
+    F.rhs.assign(R.ns, 0.0);
+    F.extra.assign(R.ns, 0.0);
+    F.old_infl.assign(F.ns_rhs_off[nl], 0.0);
+}
+
+// x = A \ b by LU with partial pivoting (Julia's `\` on a square matrix)
+static void lu_solve(std::vector<double> A, int64_t n, const double* b, double* x) {
+    std::vector<int64_t> piv(n);
+    for (int64_t i = 0; i < n; ++i) piv[i] = i;
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t p = k;
+        for (int64_t i = k + 1; i < n; ++i)
+            if (std::fabs(A[(size_t)k * n + i]) > std::fabs(A[(size_t)k * n + p])) p = i;
+        if (p != k) {
+            for (int64_t j = 0; j < n; ++j) std::swap(A[(size_t)j * n + k], A[(size_t)j * n + p]);
+            std::swap(piv[k], piv[p]);
+        }
+        const double d = A[(size_t)k * n + k];
+        for (int64_t i = k + 1; i < n; ++i) A[(size_t)k * n + i] /= d;
+        for (int64_t j = k + 1; j < n; ++j) {
+            const double f = A[(size_t)j * n + k];
+            for (int64_t i = k + 1; i < n; ++i) A[(size_t)j * n + i] -= A[(size_t)k * n + i] * f;
+        }
+    }
+    std::vector<double> y(n);
+    for (int64_t i = 0; i < n; ++i) {
+        double v = b[piv[i]];
+        for (int64_t j = 0; j < i; ++j) v -= A[(size_t)j * n + i] * y[j];
+        y[i] = v;
+    }
+    for (int64_t i = n - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int64_t j = i + 1; j < n; ++j) v -= A[(size_t)j * n + i] * x[j];
+        x[i] = v / A[(size_t)i * n + i];
+    }
+}
+
+// update_nonself_influence! for one source leaf (:740-795)
+static void fgs_nonself_update(FGSolver& F, size_t k) {
+    const Run& R = F.R;
+    const OBranch& sb = R.st.branches[R.st.leaf_index[k]];
+    const int64_t rows = F.ns_rows[k];
+    if (rows == 0) return;
+    const double* s = &F.strengths[F.leaf_start[k]];
+    double* old = &F.old_infl[F.ns_rhs_off[k]];
+    std::vector<double> now(rows, 0.0);
+    for (int64_t c = 0; c < sb.n_bodies; ++c)
+        for (int64_t r = 0; r < rows; ++r) now[r] += F.nonself[k][(size_t)c * rows + r] * s[c];
+    int64_t r0 = 0;
+    for (int64_t q = F.index_map[k][0]; q < F.index_map[k][1]; ++q) {
+        const OBranch& tb = R.tt.branches[F.sorted[q][0]];
+        for (int64_t r = 0; r < tb.n_bodies; ++r) {
+            const int64_t d = F.lpos[tb.first + r];
+            F.rhs[d] += old[r0 + r];
+            F.rhs[d] -= now[r0 + r];
+        }
+        r0 += tb.n_bodies;
+    }
+    for (int64_t r = 0; r < rows; ++r) old[r] = now[r];
+}
+
+// farfield-only fmm! on the solver's trees and M2L list (:839-850): potential and gradient
+static void fgs_farfield(FGSolver& F) {
+    Run& R = F.R;
+    const int p = R.opt.p, Hn = n_harmonics(p), dpb = data_per_body(R.kind);
+    const bool LH = has_vector_potential(R.kind);
+    tables().ensure(p);
+    for (int64_t i = 0; i < R.nt; ++i)
+        for (int k = 3; k < 16; ++k) R.tt.buffer[(size_t)i * 16 + k] = 0.0;
+    std::fill(R.tt.expansions.begin(), R.tt.expansions.end(), 0.0);
+    std::fill(R.st.expansions.begin(), R.st.expansions.end(), 0.0);
+    Scratch S;
+    S.ensure(p);
+    std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
+    for (int64_t il : R.st.leaf_index) {
+        const OBranch& b = R.st.branches[il];
+        Exp M{&R.st.expansions[(size_t)il * 4 * Hn]};
+        for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+            const double* x = &R.st.buffer[(size_t)i * dpb];
+            double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
+            if (is_panel(R.kind)) p2m_panel(R.kind, M, Exp{harm.data()}, x, b.source_center, p);
+            else p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
+        }
+    }
+    for (int64_t ib = (int64_t)R.st.branches.size() - 1; ib >= 0; --ib) {
+        const OBranch& b = R.st.branches[ib];
+        if (b.n_branches != 0)
+            for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
+                multipole_to_multipole(Exp{&R.st.expansions[(size_t)ib * 4 * Hn]}, b.source_center,
+                                       Exp{&R.st.expansions[(size_t)c * 4 * Hn]}, R.st.branches[c].source_center, S, p, LH);
+    }
+    for (auto& pr : R.m2l) {
+        bool ok = true;
+        multipole_to_local(Exp{&R.tt.expansions[(size_t)pr[0] * 4 * Hn]}, geom(R.tt.branches[pr[0]]),
+                           Exp{&R.st.expansions[(size_t)pr[1] * 4 * Hn]}, geom(R.st.branches[pr[1]]), S, p, LH, ErrorMethod(), ok);
+    }
+    for (size_t ib = 0; ib < R.tt.branches.size(); ++ib) {
+        const OBranch& b = R.tt.branches[ib];
+        if (b.n_branches > 0)
+            for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
+                local_to_local(Exp{&R.tt.expansions[(size_t)c * 4 * Hn]}, R.tt.branches[c].target_center,
+                               Exp{&R.tt.expansions[ib * 4 * Hn]}, b.target_center, S, p, LH);
+    }
+    std::vector<double> gnm;
+    for (int64_t il : R.tt.leaf_index) {
+        const OBranch& b = R.tt.branches[il];
+        Exp Lx{&R.tt.expansions[(size_t)il * 4 * Hn]};
+        for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+            double* t = &R.tt.buffer[(size_t)i * 16];
+            double dx[3] = {t[0] - b.target_center[0], t[1] - b.target_center[1], t[2] - b.target_center[2]};
+            double phi, v[3], h[9];
+            evaluate_local(dx, Exp{harm.data()}, gnm, Lx, p, LH, true, true, false, phi, v, h);
+            t[3] += phi; t[4] += v[0]; t[5] += v[1]; t[6] += v[2];
+        }
+    }
+}
+
+// unit normal of a panel body from its buffer row (vertices from row 5 on)
+static void panel_unit_normal(int kind, const double* s, double n[3]) {
+    (void)kind;
+    double v[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) v[k][a] = s[5 + 3 * k + a];
+    panel_normal(v, n);
+}
+
+// solve! (:797-946); ext = the external influence per body (original order)
+static void fgs_solve(FGSolver& F, const double* ext, int max_iterations, double tolerance) {
+    Run& R = F.R;
+    const int dpb = data_per_body(R.kind);
+    const size_t nl = R.st.leaf_index.size();
+    for (int64_t i = 0; i < R.ns; ++i) F.extra[F.lpos[i]] = -ext[R.st.sort_index[i]];   // influence! (:960-993)
+    F.rhs = F.extra;
+    for (int64_t i = 0; i < R.ns; ++i) F.strengths[F.lpos[i]] = R.st.buffer[(size_t)i * dpb + 4];   // update_by_leaf! (:658-688)
+    std::fill(F.old_infl.begin(), F.old_infl.end(), 0.0);
+    for (size_t k = 0; k < nl; ++k) fgs_nonself_update(F, k);
+    F.mse_history.clear();
+    F.sweeps = 0;
+    for (int it = 1; it <= max_iterations; ++it) {
+        fgs_farfield(F);
+        for (int64_t i = 0; i < R.ns; ++i) F.extra[F.lpos[i]] = -body_influence(F, &R.tt.buffer[(size_t)i * 16], i);
+        for (int64_t i = 0; i < R.ns; ++i) F.rhs[i] += F.extra[i];
+        double mse = 0.0;
+        for (size_t k = 0; k < nl; ++k) {
+            const int64_t n = F.leaf_start[k + 1] - F.leaf_start[k];
+            const double* s = &F.strengths[F.leaf_start[k]];
+            for (int64_t r = 0; r < n; ++r) {
+                double v = -F.rhs[F.leaf_start[k] + r];
+                for (int64_t c = 0; c < n; ++c) v += F.self[k][(size_t)c * n + r] * s[c];
+                mse += v * v;
+            }
+        }
+        mse /= (double)std::max<int64_t>(1, R.ns);
+        F.mse_history.push_back(mse);
+        if (mse <= tolerance) break;
+        for (size_t k = 0; k < nl; ++k) {
+            const int64_t n = F.leaf_start[k + 1] - F.leaf_start[k];
+            lu_solve(F.self[k], n, &F.rhs[F.leaf_start[k]], &F.strengths[F.leaf_start[k]]);
+            fgs_nonself_update(F, k);
+        }
+        ++F.sweeps;
+        for (int64_t i = 0; i < R.ns; ++i) R.st.buffer[(size_t)i * dpb + 4] = F.strengths[F.lpos[i]];   // update_by_leaf!
+        for (int64_t i = 0; i < R.ns; ++i) F.rhs[i] -= F.extra[i];
+    }
+    // final nearfield with the full direct list on top of the last farfield (:917-931)
+    for (auto& pr : F.full_direct) {
+        const OBranch& s = R.st.branches[pr[1]];
+        const OBranch& t = R.tt.branches[pr[0]];
+        user_direct(R.kind, R.tt.buffer.data(), t.first, t.n_bodies, R.st.buffer.data(), dpb, s.first, s.n_bodies, true, true, false);
+    }
+}
+
 }  // namespace orc
 
 // ======================================================================
@@ -1085,4 +1379,61 @@ void orc_p2m(int kind, const double* dx, const double* strength, int p, double* 
     else p2m_point_dipole(Exp{out}, Exp{harm.data()}, dx, strength, p);
 }
 
+
+// FastGaussSeidel: oi = [p, leaf_size, shrink_recenter, influence], od = [multipole_acceptance]
+void* orc_fgs_create(int kind, int64_t n, const double* sfull, const int* oi, const double* od) {
+    FGSolver* F = new FGSolver();
+    Run& R = F->R;
+    R.kind = kind; R.nt = n; R.ns = n;
+    const int dpb = data_per_body(kind);
+    R.sfull.assign(sfull, sfull + (size_t)dpb * n);
+    R.tfull.assign((size_t)16 * n, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) R.tfull[(size_t)i * 16 + d] = sfull[(size_t)i * dpb + d];
+    R.opt.p = oi[0]; R.opt.ls_source = R.opt.ls_target = oi[1]; R.opt.shrink = oi[2] != 0;
+    R.opt.method = METHOD_BARBA; R.opt.theta = od[0];
+    R.opt.PS = true; R.opt.GS = true; R.opt.HS = false;
+    F->influence = oi[3];
+    fgs_setup(*F);
+    return F;
+}
+void orc_fgs_free(void* h) { delete (FGSolver*)h; }
+int64_t orc_fgs_n_leaves(void* h) { return (int64_t)((FGSolver*)h)->R.st.leaf_index.size(); }
+// per leaf: [branch, n_bodies, nonself rows, first sorted body]
+void orc_fgs_leaf_info(void* h, int64_t* out) {
+    FGSolver& F = *(FGSolver*)h;
+    for (size_t k = 0; k < F.R.st.leaf_index.size(); ++k) {
+        const OBranch& b = F.R.st.branches[F.R.st.leaf_index[k]];
+        out[4 * k] = F.R.st.leaf_index[k]; out[4 * k + 1] = b.n_bodies; out[4 * k + 2] = F.ns_rows[k]; out[4 * k + 3] = b.first;
+    }
+}
+void orc_fgs_self(void* h, int64_t k, double* out) { auto& M = ((FGSolver*)h)->self[k]; std::memcpy(out, M.data(), M.size() * 8); }
+void orc_fgs_nonself(void* h, int64_t k, double* out) { auto& M = ((FGSolver*)h)->nonself[k]; std::memcpy(out, M.data(), M.size() * 8); }
+// original index of the body of every nonself row of leaf k, and of the leaf's own bodies
+void orc_fgs_nonself_rows(void* h, int64_t k, int64_t* out) {
+    FGSolver& F = *(FGSolver*)h;
+    int64_t r = 0;
+    for (int64_t q = F.index_map[k][0]; q < F.index_map[k][1]; ++q) {
+        const OBranch& tb = F.R.tt.branches[F.sorted[q][0]];
+        for (int64_t i = 0; i < tb.n_bodies; ++i) out[r++] = F.R.tt.sort_index[tb.first + i];
+    }
+}
+void orc_fgs_leaf_bodies(void* h, int64_t k, int64_t* out) {
+    FGSolver& F = *(FGSolver*)h;
+    const OBranch& b = F.R.st.branches[F.R.st.leaf_index[k]];
+    for (int64_t i = 0; i < b.n_bodies; ++i) out[i] = F.R.st.sort_index[b.first + i];
+}
+// solve!: strengths and the 16 x n target buffer back in original order
+int orc_fgs_solve(void* h, const double* ext, int max_iterations, double tolerance, double* strengths, double* targets,
+                  double* mse_history) {
+    FGSolver& F = *(FGSolver*)h;
+    fgs_solve(F, ext, max_iterations, tolerance);
+    const int dpb = data_per_body(F.R.kind);
+    for (int64_t i = 0; i < F.R.ns; ++i) {
+        strengths[F.R.st.sort_index[i]] = F.R.st.buffer[(size_t)i * dpb + 4];
+        std::memcpy(targets + (size_t)F.R.tt.sort_index[i] * 16, &F.R.tt.buffer[(size_t)i * 16], 16 * sizeof(double));
+    }
+    for (size_t k = 0; k < F.mse_history.size(); ++k) mse_history[k] = F.mse_history[k];
+    return F.sweeps;
+}
 }  // extern "C"
