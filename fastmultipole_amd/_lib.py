@@ -14,6 +14,7 @@ FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMHIP_EDE
 POINT_SOURCE, POINT_VORTEX, POINT_DIPOLE, VORTEX_GAUSSIAN = 0, 1, 2, 3
 PANEL_SOURCE_TRI, PANEL_DIPOLE_TRI, PANEL_SOURCE_QUAD, PANEL_DIPOLE_QUAD = 4, 5, 6, 7
 FP64, MIXED = 0, 1
+INFLUENCE_POTENTIAL, INFLUENCE_NORMAL_GRADIENT = 0, 1
 BARBA, SELFTUNING, SELFTUNING_TREESTOP = 0, 1, 2
 ERR_NONE, ERR_ROTATED_COEFFICIENTS_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
 
@@ -24,6 +25,8 @@ EXPORTED = [
     "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_get_branches", "fmmhip_eval_get_levels",
     "fmmhip_eval_get_leaf_index", "fmmhip_eval_get_sort_index", "fmmhip_eval_n_list", "fmmhip_eval_get_list",
     "fmmhip_eval_get_m2l_orders", "fmmhip_eval_get_expansions", "fmmhip_direct",
+    "fmmhip_fgs_setup", "fmmhip_fgs_solve", "fmmhip_fgs_n_leaves", "fmmhip_fgs_leaf_info", "fmmhip_fgs_get_matrix",
+    "fmmhip_fgs_nonself_rows",
 ]
 
 
@@ -97,6 +100,13 @@ def load():
     L.fmmhip_eval_get_m2l_orders.argtypes = [vp, dp]
     L.fmmhip_eval_get_expansions.argtypes = [vp, i32, dp]
     L.fmmhip_direct.argtypes = [vp, i32, i64, dp, i64, dp, i32, i32, i32, i32, dp, i32]
+    L.fmmhip_fgs_setup.argtypes = [vp, C.POINTER(Options), i32]
+    L.fmmhip_fgs_solve.argtypes = [vp, dp, i32, i32, C.c_double, dp, C.POINTER(i32), dp]
+    L.fmmhip_fgs_n_leaves.argtypes = [vp]
+    L.fmmhip_fgs_n_leaves.restype = i64
+    L.fmmhip_fgs_leaf_info.argtypes = [vp, dp]
+    L.fmmhip_fgs_get_matrix.argtypes = [vp, i32, i64, dp]
+    L.fmmhip_fgs_nonself_rows.argtypes = [vp, i64, dp]
     _LIB = L
     return L
 

@@ -276,6 +276,75 @@ class Evaluation:
         return out[:n]
 
 
+# ------------------------------------------------------------ FastGaussSeidel
+class FastGaussSeidel:
+    """FastGaussSeidel(system; expansion_order=4, multipole_acceptance=0.5, leaf_size=30,
+    shrink_recenter=true) of src/solve.jl:538-645 for one system that is both target and source
+    (point sources or source panels). `influence` selects the value matched per body: the scalar
+    potential (the reference test's influence!) or, for panels, the normal component of the gradient.
+    Influence matrices and solver state stay in HBM."""
+
+    def __init__(self, system, expansion_order=4, multipole_acceptance=0.5, leaf_size=30, shrink_recenter=True,
+                 influence=L.INFLUENCE_POTENTIAL, ctx=None):
+        kernel = _kernel_of(system)
+        self.system = system
+        self.ev = Evaluation(ctx)
+        self.ev.set_systems(kernel, None, system.source_buffer())
+        self.opt = make_options(expansion_order=expansion_order, leaf_size_source=leaf_size,
+                                multipole_acceptance=multipole_acceptance, shrink_recenter=shrink_recenter,
+                                interaction_list_method=Barba(), scalar_potential=True)
+        self.ev.ctx.check(self.ev.ctx.lib.fmmhip_fgs_setup(self.ev._h, C.byref(self.opt), int(influence)))
+
+    def leaves(self):
+        n = self.ev.ctx.lib.fmmhip_fgs_n_leaves(self.ev._h)
+        out = np.zeros((max(n, 1), 4), dtype=np.int64)
+        if n:
+            self.ev.ctx.check(self.ev.ctx.lib.fmmhip_fgs_leaf_info(self.ev._h, out.ctypes.data))
+        return out[:n]
+
+    def matrix(self, which, leaf):
+        """which 0: self matrix (n x n), 1: non-self matrix (rows x n), as numpy arrays."""
+        info = self.leaves()[leaf]
+        rows = int(info[1] if which == 0 else info[2])
+        buf = np.zeros(max(1, rows * int(info[1])))
+        self.ev.ctx.check(self.ev.ctx.lib.fmmhip_fgs_get_matrix(self.ev._h, which, leaf, buf.ctypes.data))
+        return buf[:rows * int(info[1])].reshape(int(info[1]), rows).T
+
+    def nonself_rows(self, leaf):
+        rows = int(self.leaves()[leaf, 2])
+        out = np.zeros(max(1, rows), dtype=np.int64)
+        self.ev.ctx.check(self.ev.ctx.lib.fmmhip_fgs_nonself_rows(self.ev._h, leaf, out.ctypes.data))
+        return out[:rows]
+
+    def solve(self, external, max_iterations=10, tolerance=1e-3):
+        """solve! (src/solve.jl:797-946). external: per-body external influence (original order).
+        Returns (strengths, targets (n,16), sweeps, mse history) and writes the strengths and the
+        potential/gradient back into the system."""
+        ext = np.ascontiguousarray(external, dtype=np.float64)
+        n = self.ev.ns
+        strengths = np.zeros(n)
+        hist = np.zeros(max(1, max_iterations))
+        sweeps = C.c_int32(0)
+        self.ev.ctx.check(self.ev.ctx.lib.fmmhip_fgs_solve(self.ev._h, ext.ctypes.data, 0, int(max_iterations),
+                                                           float(tolerance), strengths.ctypes.data, C.byref(sweeps),
+                                                           hist.ctypes.data))
+        targets = self.ev.targets()
+        if hasattr(self.system, "set_strengths"):
+            self.system.set_strengths(strengths)
+        if hasattr(self.system, "buffer_to_target_system"):
+            self.system.buffer_to_target_system(targets, DerivativesSwitch(True, True, False))
+        return strengths, targets, int(sweeps.value), hist[:max(1, max_iterations)]
+
+
+def solve(system, solver, max_iterations=10, tolerance=1e-3, external=None):
+    """solve!(system, solver; max_iterations, tolerance): the external influence defaults to the
+    system's current potential (target_influence_to_buffer! of test/solve_test.jl:15-17)."""
+    if external is None:
+        pot = np.asarray(system.potential, dtype=np.float64)
+        external = pot[:, 0] if pot.ndim == 2 else pot
+    return solver.solve(external, max_iterations=max_iterations, tolerance=tolerance)
+
+
 # ------------------------------------------------------------------ fmm!
 def _as_tuple(x):
     return x if isinstance(x, tuple) else (x,)

@@ -112,6 +112,11 @@ static void eval_build_lists(Eval& E, const fmmhip_options& o, bool count_pairs 
 
 static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, bool is_estimate);
 
+Eval::~Eval() {
+    delete estimate;
+    free_fgs(fgs);
+}
+
 // estimate_influence! for relative tolerances (src/fmm.jl:1193-1276): a full
 // low-order evaluation (p = 3, leaf 5, MAC 0.6, potential and gradient) on its
 // own trees; max |grad| (or potential) per main-tree branch
@@ -523,6 +528,93 @@ fmmhip_status fmmhip_direct(fmmhip_ctx* h, int32_t kernel, int64_t nt, const dou
             if (nt) HIP_CHECK(hipMemcpyAsync(out, dout.data(), sizeof(double) * 16 * nt, hipMemcpyDeviceToHost, C->stream));
         }
         HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
+
+// --------------------------------------------------------- FastGaussSeidel
+fmmhip_status fmmhip_fgs_setup(fmmhip_eval* ev, const fmmhip_options* opt, int32_t influence) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        check_options(opt);
+        if (E.kernel != FMMHIP_POINT_SOURCE && E.kernel != FMMHIP_PANEL_SOURCE_TRI && E.kernel != FMMHIP_PANEL_SOURCE_QUAD)
+            throw Error(FMMHIP_EINVAL, "FastGaussSeidel supports point sources and source panels");
+        if (influence != FMMHIP_INFLUENCE_POTENTIAL && influence != FMMHIP_INFLUENCE_NORMAL_GRADIENT)
+            throw Error(FMMHIP_EINVAL, "unknown influence");
+        if (influence == FMMHIP_INFLUENCE_NORMAL_GRADIENT && !is_panel(E.kernel))
+            throw Error(FMMHIP_EINVAL, "the normal-gradient influence needs panels");
+        if (E.t_xyz || E.nt != E.ns) throw Error(FMMHIP_EINVAL, "FastGaussSeidel needs one system as target and source");
+        // FastGaussSeidel's fixed choices (src/solve.jl:538-571): Barba lists, farfield and
+        // nearfield without self pairs, one leaf size, potential and gradient
+        fmmhip_options o = *opt;
+        o.list_method = FMMHIP_BARBA;
+        o.leaf_size_target = o.leaf_size_source;
+        o.farfield = 1; o.nearfield = 1; o.self_induced = 0;
+        o.error_kind = FMMHIP_ERR_NONE;
+        o.scalar_potential = 1; o.gradient = 1; o.hessian = 0;
+        o.precision = FMMHIP_FP64;
+        eval_build_trees(E, o);
+        E.lists.keep_discovery = true;
+        try {
+            eval_build_lists(E, o, false);
+        } catch (...) {
+            E.lists.keep_discovery = false;
+            throw;
+        }
+        E.lists.keep_discovery = false;
+        fgs_setup(E, o, influence);
+    });
+}
+
+fmmhip_status fmmhip_fgs_solve(fmmhip_eval* ev, const double* external, int32_t on_device, int32_t max_iterations, double tolerance,
+                               double* strengths_out, int32_t* sweeps, double* mse_history) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (!E.fgs) throw Error(FMMHIP_EINVAL, "fmmhip_fgs_setup has not run");
+        if (E.ns && !external) throw Error(FMMHIP_EINVAL, "external influence is NULL");
+        if (max_iterations < 0) throw Error(FMMHIP_EINVAL, "max_iterations must be >= 0");
+        const double* ext = external;
+        DBuf<double> tmp;
+        if (!on_device) {
+            tmp.resize(std::max<int64_t>(1, E.ns));
+            if (E.ns) HIP_CHECK(hipMemcpyAsync(tmp.data(), external, sizeof(double) * E.ns, hipMemcpyHostToDevice, C->stream));
+            ext = tmp.data();
+        }
+        fgs_solve(E, ext, max_iterations, tolerance, strengths_out, sweeps, mse_history);
+    });
+}
+
+int64_t fmmhip_fgs_n_leaves(fmmhip_eval* ev) { return ev ? fgs_n_leaves(ev->e) : 0; }
+
+fmmhip_status fmmhip_fgs_leaf_info(fmmhip_eval* ev, int64_t* out) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        if (!ev->e.fgs) throw Error(FMMHIP_EINVAL, "fmmhip_fgs_setup has not run");
+        fgs_leaf_info(ev->e, out);
+    });
+}
+
+fmmhip_status fmmhip_fgs_get_matrix(fmmhip_eval* ev, int32_t which, int64_t leaf, double* out) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        if (!ev->e.fgs) throw Error(FMMHIP_EINVAL, "fmmhip_fgs_setup has not run");
+        if (which != 0 && which != 1) throw Error(FMMHIP_EINVAL, "which must be 0 (self) or 1 (non-self)");
+        fgs_matrix(ev->e, which, leaf, out);
+    });
+}
+
+fmmhip_status fmmhip_fgs_nonself_rows(fmmhip_eval* ev, int64_t leaf, int64_t* out) {
+    if (!ev) return FMMHIP_EINVAL;
+    Context* C = ev->e.ctx;
+    ABI_TRY(C, {
+        if (!ev->e.fgs) throw Error(FMMHIP_EINVAL, "fmmhip_fgs_setup has not run");
+        fgs_nonself_rows(ev->e, leaf, out);
     });
 }
 

@@ -94,6 +94,9 @@ struct Lists {
     int64_t n_m2l_targets = 0;
     DBuf<int32_t> m2l_targets;
     DBuf<int64_t> m2l_offsets;
+    bool keep_discovery = false;                     // also keep the direct list in discovery order
+    DBuf<uint32_t> dfs_t;                            // (build_interaction_lists' order, before sort_by_target)
+    DBuf<int32_t> dfs_s;
 };
 
 struct Tables;   // device constant tables (rotation normalisations), expansions.hip
@@ -122,7 +125,8 @@ struct Eval {
     int p_built = -1;
     Eval* estimate = nullptr;                 // persistent low-order evaluation of estimate_influence
     DBuf<double> influence;                   // per-target max |grad| (or |phi|) of the estimate
-    ~Eval() { delete estimate; }
+    struct FgsState* fgs = nullptr;           // FastGaussSeidel solver state (fgs.hip)
+    ~Eval();
 };
 
 __host__ __device__ constexpr bool is_panel(int kernel) { return kernel >= FMMHIP_PANEL_SOURCE_TRI; }
@@ -163,5 +167,14 @@ void run_l2l(Context& C, Tree& tt, int p, bool LH);
 void run_l2p(Context& C, Tree& tt, double* tout, int64_t nt, int p, bool LH, bool PS, bool GS, bool HS);
 // api.hip helpers
 void gather_outputs(Context& C, const Tree& tt, const double* tout, double* aos, int64_t nt);
+// fgs.hip: FastGaussSeidel (src/solve.jl)
+void fgs_setup(Eval& E, const fmmhip_options& o, int influence);
+void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolerance, double* strengths_host, int32_t* sweeps,
+               double* mse_host);
+int64_t fgs_n_leaves(const Eval& E);
+void fgs_leaf_info(const Eval& E, int64_t* out_host);
+void fgs_matrix(const Eval& E, int which, int64_t leaf, double* out_host);
+void fgs_nonself_rows(const Eval& E, int64_t leaf, int64_t* out_host);
+void free_fgs(struct FgsState* f);
 
 }  // namespace fmmhip

@@ -316,6 +316,14 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
                                                                    S.vs_m2l.data(), S.kt_dir.data(), S.vs_dir.data());
         LAUNCH_CHECK();
     }
+    if (L.keep_discovery) {
+        L.dfs_t.resize(std::max<int64_t>(1, L.n_direct));
+        L.dfs_s.resize(std::max<int64_t>(1, L.n_direct));
+        if (L.n_direct) {
+            HIP_CHECK(hipMemcpyAsync(L.dfs_t.data(), S.kt_dir.data(), sizeof(uint32_t) * L.n_direct, hipMemcpyDeviceToDevice, C.stream));
+            HIP_CHECK(hipMemcpyAsync(L.dfs_s.data(), S.vs_dir.data(), sizeof(int32_t) * L.n_direct, hipMemcpyDeviceToDevice, C.stream));
+        }
+    }
     int bits = 1;
     while (bits < 32 && (int64_t(1) << bits) < tt.n_branches) ++bits;
     finish_list(C, S, S.kt_m2l.data(), S.vs_m2l.data(), L.n_m2l, bits, L.m2l_t, L.m2l_s, L.m2l_targets, L.m2l_offsets,

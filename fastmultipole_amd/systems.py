@@ -51,6 +51,10 @@ class Gravitational:
         """All bodies through source_system_to_buffer! (:50-54), as an (N, 5) array."""
         return np.column_stack([self.position, self.radius, self.strength])
 
+    def set_strengths(self, strengths):
+        """buffer_to_system_strength! (test/solve_test.jl:27-32)."""
+        self.strength = np.ascontiguousarray(strengths, dtype=np.float64)
+
     def buffer_to_target_system(self, target_buffer, switch):
         """buffer_to_target_system! (:106-119): assigns phi, gradient and hessian (zero when switched off)."""
         n = self.get_n_bodies()
@@ -205,6 +209,9 @@ class _Panels:
     def source_buffer(self):
         from .workloads import panel_rows
         return panel_rows(self.vertices, self.strength)
+
+    def set_strengths(self, strengths):
+        self.strength = np.ascontiguousarray(strengths, dtype=np.float64).reshape(-1)
 
     def buffer_to_target_system(self, target_buffer, switch):
         n = self.get_n_bodies()

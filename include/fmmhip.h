@@ -167,6 +167,36 @@ fmmhip_status fmmhip_direct(fmmhip_ctx* ctx, int32_t kernel, int64_t n_targets, 
                             int32_t gradient, int32_t hessian, int32_t precision, double* target_out,
                             int32_t on_device);
 
+/* ---- FastGaussSeidel (src/solve.jl) for one system that is both target and source
+ * (fmmhip_eval_set_systems with target_xyz = NULL): point sources or source panels.
+ * influence: the value matched per body, FMMHIP_INFLUENCE_POTENTIAL (the scalar potential,
+ * the reference's test influence!, test/solve_test.jl:11-13) or FMMHIP_INFLUENCE_NORMAL_GRADIENT
+ * (panels: the gradient along the panel's unit normal). */
+enum { FMMHIP_INFLUENCE_POTENTIAL = 0, FMMHIP_INFLUENCE_NORMAL_GRADIENT = 1 };
+
+/* FastGaussSeidel(system; expansion_order, multipole_acceptance, leaf_size, shrink_recenter)
+ * (src/solve.jl:538-645): trees, Barba lists, per-leaf self and non-self influence matrices in HBM.
+ * Uses opt->expansion_order, leaf_size_source, multipole_acceptance, shrink_recenter; leaves hold at
+ * most 101 bodies. */
+fmmhip_status fmmhip_fgs_setup(fmmhip_eval* ev, const fmmhip_options* opt, int32_t influence);
+/* solve!(system, solver; max_iterations, tolerance) (src/solve.jl:797-946). external: the external
+ * influence per body (the reference's target_influence_to_buffer! values), original order, host or
+ * device memory. Strengths start from the source buffer given to fmmhip_eval_set_systems;
+ * strengths_out (host, original order, may be NULL) receives the solved strengths; the target
+ * buffer (potential, gradient: farfield of the last iteration plus the nearfield of the solved
+ * strengths) is read with fmmhip_eval_get_targets. sweeps: Gauss-Seidel sweeps performed;
+ * mse_history (host, max_iterations entries, may be NULL): mean squared residual over all bodies
+ * at each iteration. */
+fmmhip_status fmmhip_fgs_solve(fmmhip_eval* ev, const double* external, int32_t on_device, int32_t max_iterations,
+                               double tolerance, double* strengths_out, int32_t* sweeps, double* mse_history);
+/* inspection: per leaf (leaf_index order) [branch, n_bodies, non-self rows, first sorted body];
+ * matrices column-major (which = 0 self n x n, 1 non-self rows x n, src/solve.jl:3-51);
+ * original index of the body of every non-self row */
+int64_t fmmhip_fgs_n_leaves(fmmhip_eval* ev);
+fmmhip_status fmmhip_fgs_leaf_info(fmmhip_eval* ev, int64_t* out);
+fmmhip_status fmmhip_fgs_get_matrix(fmmhip_eval* ev, int32_t which, int64_t leaf, double* out);
+fmmhip_status fmmhip_fgs_nonself_rows(fmmhip_eval* ev, int64_t leaf, int64_t* out);
+
 #ifdef __cplusplus
 }
 #endif
