@@ -19,10 +19,11 @@
 // current strengths, moves its influence into the right-hand side, measures the
 // mean squared residual of the self blocks over all bodies, and, above the
 // tolerance, sweeps the leaves in order: block solve, then the leaf's non-self
-// influence on its neighbours (update_nonself_influence!, :740-795). The sweep is
-// sequential as in the reference (leaf k sees the updates of leaves < k); one
-// workgroup runs it. After the iterations the nearfield of the new strengths,
-// self pairs included, is added to the last farfield (:917-931).
+// influence on its neighbours (update_nonself_influence!, :740-795). The sweep keeps
+// the reference's order (leaf k sees the updates of exactly the leaves < k) while
+// solving independent leaves together: leaves are grouped into levels of the
+// dependency order and each level is one launch. After the iterations the nearfield
+// of the new strengths, self pairs included, is added to the last farfield (:917-931).
 #include "fmmhip_internal.hpp"
 #include "panels.hpp"
 #include <cstring>   // host memset for rocprim headers
@@ -54,6 +55,12 @@ struct FgsState {
     DBuf<double> ns_M, old_infl;
     DBuf<double> strengths, rhs, extra, normals, leaf_sq;
     std::vector<int64_t> leaf_start_h;
+    // sweep schedule: leaf k's level is one above the highest level of the leaves before it
+    // that it exchanges influence with, so one level's leaves are solved concurrently and
+    // every leaf still sees exactly the updates of the leaves before it
+    std::vector<int64_t> level_ptr;   // host: levels + 1
+    DBuf<int32_t> level_leaves;       // leaves grouped by level, in order within a level
+    DBuf<int32_t> all_leaves;         // 0 .. nl-1
 };
 
 void free_fgs(FgsState* f) { delete f; }
@@ -245,10 +252,6 @@ __global__ __launch_bounds__(256) void k_fgs_invert(const int64_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------- solve
-// right-hand side vectors live in leaf order; loads that another wavefront of
-// the sweep may have just stored go through the coherent path
-__device__ inline double ld_coherent(const double* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
 __global__ void k_fgs_from_bodies(const int64_t* __restrict__ lpos, const double* __restrict__ v, const int64_t* __restrict__ sort_index,
                                   int64_t n, double scale, int original_order, double* __restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -296,47 +299,47 @@ __global__ __launch_bounds__(256) void k_fgs_residual(const int64_t* __restrict_
     if (threadIdx.x == 0) leaf_sq[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// One Gauss-Seidel sweep over the leaves in order, by one workgroup. SOLVE = 0
-// only moves every leaf's non-self influence into the right-hand side (the
-// update before the iterations, :872-874).
+// One level of a Gauss-Seidel sweep: one workgroup per leaf of the level. SOLVE:
+// s_k = A_k^-1 rhs_k; then the leaf's non-self influence replaces its previous one
+// in the right-hand side of its neighbours (update_nonself_influence!, :740-795).
+// Leaves of one level never update each other's right-hand side; two of them may
+// update the same neighbour, hence the atomic adds. SOLVE = 0 only moves every
+// leaf's influence into the right-hand side (the update before the iterations,
+// :872-874).
 template <bool SOLVE>
-__global__ __launch_bounds__(256) void k_fgs_sweep(int64_t nl, const int64_t* __restrict__ leaf_start,
+__global__ __launch_bounds__(256) void k_fgs_level(const int32_t* __restrict__ leaves, const int64_t* __restrict__ leaf_start,
                                                    const int64_t* __restrict__ self_off, const double* __restrict__ Ainv,
                                                    const int64_t* __restrict__ leaf_row0, const int64_t* __restrict__ ns_off,
                                                    const double* __restrict__ M, const int64_t* __restrict__ row_body,
                                                    const int64_t* __restrict__ lpos, double* __restrict__ s, double* rhs,
                                                    double* __restrict__ old_infl) {
-    __shared__ double xs[FGS_MAX_LEAF];
-    for (int64_t k = 0; k < nl; ++k) {
-        const int64_t s0 = leaf_start[k];
-        const int n = (int)(leaf_start[k + 1] - s0);
-        if (SOLVE) {   // s_k = A_k^-1 rhs_k
-            const double* B = Ainv + self_off[k];
-            for (int r = threadIdx.x; r < n; r += blockDim.x) {
-                double v = 0.0;
-                for (int c = 0; c < n; ++c) v += B[c * n + r] * ld_coherent(rhs + s0 + c);
-                xs[r] = v;
-            }
-            __syncthreads();
-            for (int r = threadIdx.x; r < n; r += blockDim.x) s[s0 + r] = xs[r];
-        } else {
-            for (int r = threadIdx.x; r < n; r += blockDim.x) xs[r] = s[s0 + r];
-            __syncthreads();
-        }
-        const int64_t r0 = leaf_row0[k], rows = leaf_row0[k + 1] - r0;
-        const double* Mk = M + ns_off[k];
-        for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
-            double now = 0.0;
-            for (int c = 0; c < n; ++c) now += Mk[c * rows + r] * xs[c];
-            double* d = rhs + lpos[row_body[r0 + r]];
-            double v = ld_coherent(d);
-            v += old_infl[r0 + r];
-            v -= now;
-            __hip_atomic_store(d, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            old_infl[r0 + r] = now;
-        }
-        __threadfence();
+    __shared__ double xs[FGS_MAX_LEAF], rk[FGS_MAX_LEAF];
+    const int64_t k = leaves[blockIdx.x];
+    const int64_t s0 = leaf_start[k];
+    const int n = (int)(leaf_start[k + 1] - s0);
+    if (SOLVE) {
+        const double* B = Ainv + self_off[k];
+        for (int c = threadIdx.x; c < n; c += blockDim.x) rk[c] = rhs[s0 + c];
         __syncthreads();
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            double v = 0.0;
+#pragma unroll 8
+            for (int c = 0; c < n; ++c) v += B[c * n + r] * rk[c];
+            xs[r] = v;
+            s[s0 + r] = v;
+        }
+    } else {
+        for (int r = threadIdx.x; r < n; r += blockDim.x) xs[r] = s[s0 + r];
+    }
+    __syncthreads();
+    const int64_t r0 = leaf_row0[k], rows = leaf_row0[k + 1] - r0;
+    const double* Mk = M + ns_off[k];
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
+        double now = 0.0;
+#pragma unroll 8
+        for (int c = 0; c < n; ++c) now += Mk[c * rows + r] * xs[c];
+        atomicAdd(rhs + lpos[row_body[r0 + r]], old_infl[r0 + r] - now);
+        old_infl[r0 + r] = now;
     }
 }
 
@@ -435,6 +438,41 @@ void fgs_setup(Eval& E, const fmmhip_options& o, int influence) {
     if (np)
         k_fgs_row_body<<<grid_for(np, 256), 256, 0, C.stream>>>(F.by_src_t.data(), np, E.tt.branches.data(), F.pair_rows.data(),
                                                                F.row_body.data());
+    // sweep schedule (host, once): neighbours from the direct list, in both directions
+    {
+        std::vector<uint32_t> ps(std::max<int64_t>(1, np));
+        std::vector<int32_t> pt(std::max<int64_t>(1, np));
+        std::vector<Branch> br(st.n_branches);
+        if (np) {
+            HIP_CHECK(hipMemcpy(ps.data(), F.by_src_s.data(), sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(pt.data(), F.by_src_t.data(), sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+        }
+        HIP_CHECK(hipMemcpy(br.data(), st.branches.data(), sizeof(Branch) * br.size(), hipMemcpyDeviceToHost));
+        std::vector<std::vector<int32_t>> nb(nl);
+        for (int64_t q = 0; q < np; ++q) {
+            const int32_t a = (int32_t)br[ps[q]].leaf, b = (int32_t)br[pt[q]].leaf;   // trees are identical
+            nb[a].push_back(b);
+            nb[b].push_back(a);
+        }
+        std::vector<int64_t> level(nl, 0);
+        int64_t nlev = 0;
+        for (int64_t k = 0; k < nl; ++k) {
+            int64_t l = 0;
+            for (int32_t j : nb[k])
+                if (j < k) l = std::max(l, level[j] + 1);
+            level[k] = l;
+            nlev = std::max(nlev, l + 1);
+        }
+        F.level_ptr.assign(nlev + 1, 0);
+        for (int64_t k = 0; k < nl; ++k) F.level_ptr[level[k] + 1] += 1;
+        for (int64_t l = 0; l < nlev; ++l) F.level_ptr[l + 1] += F.level_ptr[l];
+        std::vector<int32_t> order(std::max<int64_t>(1, nl)), all(std::max<int64_t>(1, nl)), fill(F.level_ptr.begin(), F.level_ptr.end() - 1);
+        for (int64_t k = 0; k < nl; ++k) { order[fill[level[k]]++] = (int32_t)k; all[k] = (int32_t)k; }
+        F.level_leaves.resize(std::max<int64_t>(1, nl));
+        F.all_leaves.resize(std::max<int64_t>(1, nl));
+        HIP_CHECK(hipMemcpy(F.level_leaves.data(), order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(F.all_leaves.data(), all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
+    }
     F.normals.resize(std::max<int64_t>(3, 3 * ns));
     if (influence == 1) {
         k_fgs_normals<<<grid_for(ns, 256), 256, 0, C.stream>>>(st.strength.data(), ns, F.normals.data());
@@ -488,9 +526,10 @@ void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolera
     LAUNCH_CHECK();
     HIP_CHECK(hipMemsetAsync(F.old_infl.data(), 0, F.old_infl.size() * sizeof(double), C.stream));
     if (nl)
-        k_fgs_sweep<false><<<1, 256, 0, C.stream>>>(nl, F.leaf_start.data(), F.self_off.data(), F.self_inv.data(), F.leaf_row0.data(),
-                                                   F.ns_off.data(), F.ns_M.data(), F.row_body.data(), F.lpos.data(),
-                                                   F.strengths.data(), F.rhs.data(), F.old_infl.data());
+        k_fgs_level<false><<<(unsigned)nl, 256, 0, C.stream>>>(F.all_leaves.data(), F.leaf_start.data(), F.self_off.data(),
+                                                              F.self_inv.data(), F.leaf_row0.data(), F.ns_off.data(), F.ns_M.data(),
+                                                              F.row_body.data(), F.lpos.data(), F.strengths.data(), F.rhs.data(),
+                                                              F.old_infl.data());
     LAUNCH_CHECK();
     std::vector<double> sq(std::max<int64_t>(1, nl));
     int done = 0;
@@ -511,10 +550,13 @@ void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolera
         mse /= (double)std::max<int64_t>(1, n);
         if (mse_host) mse_host[it - 1] = mse;
         if (mse <= tolerance) break;
-        if (nl)
-            k_fgs_sweep<true><<<1, 256, 0, C.stream>>>(nl, F.leaf_start.data(), F.self_off.data(), F.self_inv.data(),
-                                                      F.leaf_row0.data(), F.ns_off.data(), F.ns_M.data(), F.row_body.data(),
-                                                      F.lpos.data(), F.strengths.data(), F.rhs.data(), F.old_infl.data());
+        for (size_t l = 0; l + 1 < F.level_ptr.size(); ++l) {
+            const int64_t a = F.level_ptr[l], cnt = F.level_ptr[l + 1] - a;
+            k_fgs_level<true><<<(unsigned)cnt, 256, 0, C.stream>>>(F.level_leaves.data() + a, F.leaf_start.data(), F.self_off.data(),
+                                                                   F.self_inv.data(), F.leaf_row0.data(), F.ns_off.data(),
+                                                                   F.ns_M.data(), F.row_body.data(), F.lpos.data(),
+                                                                   F.strengths.data(), F.rhs.data(), F.old_infl.data());
+        }
         LAUNCH_CHECK();
         ++done;
         k_fgs_to_sorted<<<grid_for(n, 256), 256, 0, C.stream>>>(F.lpos.data(), F.strengths.data(), n, st.strength.data());
