@@ -27,7 +27,10 @@ EXPORTED = [
     "fmmhip_eval_get_m2l_orders", "fmmhip_eval_get_expansions", "fmmhip_direct",
     "fmmhip_fgs_setup", "fmmhip_fgs_solve", "fmmhip_fgs_n_leaves", "fmmhip_fgs_leaf_info", "fmmhip_fgs_get_matrix",
     "fmmhip_fgs_nonself_rows",
+    "fmmhip_server_create", "fmmhip_server_destroy", "fmmhip_server_submit", "fmmhip_server_wait",
+    "fmmhip_server_stats",
 ]
+REQUEST_FMM, REQUEST_SOLVE = 0, 1
 
 
 class Options(C.Structure):
@@ -47,6 +50,35 @@ class Stats(C.Structure):
         ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_p2p", C.c_double), ("ms_upward", C.c_double),
         ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_total", C.c_double),
         ("ms_estimate", C.c_double), ("ms_gather", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("kernel", C.c_int32), ("n", C.c_int64), ("source_buffer", C.c_void_p),
+        ("options", Options), ("influence", C.c_int32), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
+        ("external", C.c_void_p),
+    ]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("sweeps", C.c_int32), ("pmax", C.c_int32), ("queue_ms", C.c_double),
+        ("run_ms", C.c_double), ("latency_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double),
+        ("footprint_bytes", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("completed", C.c_int64), ("queued", C.c_int64), ("running", C.c_int32), ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double), ("mean_queue_ms", C.c_double), ("span_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -107,6 +139,12 @@ def load():
     L.fmmhip_fgs_leaf_info.argtypes = [vp, dp]
     L.fmmhip_fgs_get_matrix.argtypes = [vp, i32, i64, dp]
     L.fmmhip_fgs_nonself_rows.argtypes = [vp, i64, dp]
+    L.fmmhip_server_create.argtypes = [C.c_int, i32, C.c_double, C.POINTER(vp)]
+    L.fmmhip_server_destroy.argtypes = [vp]
+    L.fmmhip_server_destroy.restype = None
+    L.fmmhip_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]
+    L.fmmhip_server_wait.argtypes = [vp, i64, dp, dp, C.POINTER(RequestStats)]
+    L.fmmhip_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
     _LIB = L
     return L
 

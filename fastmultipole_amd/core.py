@@ -110,11 +110,14 @@ class Context:
 _DEFAULT = None
 _LIVE_EVALS = weakref.WeakSet()
 _LIVE_CTXS = weakref.WeakSet()
+_LIVE_SERVERS = weakref.WeakSet()
 
 
 @atexit.register
 def _release_all():
-    """Release device state before interpreter teardown: evaluations first, then contexts."""
+    """Release device state before interpreter teardown: servers, evaluations, then contexts."""
+    for sv in list(_LIVE_SERVERS):
+        sv.close()
     for ev in list(_LIVE_EVALS):
         ev.close()
     for ctx in list(_LIVE_CTXS):
@@ -343,6 +346,103 @@ def solve(system, solver, max_iterations=10, tolerance=1e-3, external=None):
         pot = np.asarray(system.potential, dtype=np.float64)
         external = pot[:, 0] if pot.ndim == 2 else pot
     return solver.solve(external, max_iterations=max_iterations, tolerance=tolerance)
+
+
+# ------------------------------------------------------------------ request server
+class Server:
+    """Concurrent request loop on one device (fmmhip_server): fmm!(system) evaluations and
+    FastGaussSeidel solves of uneven sizes, each the same computation as the single call.
+
+    n_slots workers each own a stream and device-resident buffers reused between requests; a
+    request is admitted when its estimated HBM footprint fits within hbm_budget_bytes beside the
+    running ones. submit_* copy the inputs and return a ticket at once; wait(ticket) blocks."""
+
+    def __init__(self, n_slots=4, hbm_budget_bytes=64 * 2**30, device=-1):
+        self.lib = L.load()
+        h = C.c_void_p()
+        st = self.lib.fmmhip_server_create(int(device), int(n_slots), float(hbm_budget_bytes), C.byref(h))
+        if st != L.FMMHIP_OK:
+            raise L.FmmHipError(st, "fmmhip_server_create failed (bad arguments or no usable GPU)")
+        self._h = h
+        self._sizes = {}
+        _LIVE_SERVERS.add(self)
+
+    def _submit(self, req, n, kind):
+        t = C.c_int64()
+        st = self.lib.fmmhip_server_submit(self._h, C.byref(req), C.byref(t))
+        if st != L.FMMHIP_OK:
+            raise L.FmmHipError(st, "fmmhip_server_submit rejected the request")
+        self._sizes[t.value] = (n, kind)
+        return t.value
+
+    def submit_fmm(self, system, *, leaf_size=20, expansion_order=5, multipole_acceptance=0.4, error_tolerance=None,
+                   shrink_recenter=True, interaction_list_method=None, scalar_potential=False, gradient=True,
+                   hessian=False, farfield=True, nearfield=True, self_induced=True, precision="fp64"):
+        """Queue fmm!(system; kwargs...) (src/fmm.jl:807-884) for one system as target and source."""
+        sbuf = np.ascontiguousarray(system.source_buffer(), dtype=np.float64)
+        req = L.Request()
+        req.kind, req.kernel, req.n = L.REQUEST_FMM, _kernel_of(system), sbuf.shape[0]
+        req.source_buffer = sbuf.ctypes.data
+        req.options = make_options(expansion_order, leaf_size, leaf_size, multipole_acceptance,
+                                   interaction_list_method, shrink_recenter, error_tolerance, scalar_potential,
+                                   gradient, hessian, farfield, nearfield, self_induced, precision)
+        return self._submit(req, sbuf.shape[0], L.REQUEST_FMM)
+
+    def submit_solve(self, system, external, *, expansion_order=4, multipole_acceptance=0.5, leaf_size=30,
+                     shrink_recenter=True, influence=L.INFLUENCE_POTENTIAL, max_iterations=10, tolerance=1e-3):
+        """Queue FastGaussSeidel(system; ...) followed by solve!(system, solver; max_iterations, tolerance)
+        (src/solve.jl:538-645, 797-946)."""
+        sbuf = np.ascontiguousarray(system.source_buffer(), dtype=np.float64)
+        ext = np.ascontiguousarray(external, dtype=np.float64)
+        if ext.shape != (sbuf.shape[0],):
+            raise ValueError("external must hold one value per body")
+        req = L.Request()
+        req.kind, req.kernel, req.n = L.REQUEST_SOLVE, _kernel_of(system), sbuf.shape[0]
+        req.source_buffer = sbuf.ctypes.data
+        req.options = make_options(expansion_order=expansion_order, leaf_size_source=leaf_size,
+                                   multipole_acceptance=multipole_acceptance, shrink_recenter=shrink_recenter,
+                                   interaction_list_method=Barba(), scalar_potential=True)
+        req.influence, req.max_iterations, req.tolerance = int(influence), int(max_iterations), float(tolerance)
+        req.external = ext.ctypes.data
+        return self._submit(req, sbuf.shape[0], L.REQUEST_SOLVE)
+
+    def wait(self, ticket):
+        """Block until the request ends. Returns (targets (n,16) in body order, strengths or None,
+        stats dict). A request whose tolerance was missed still returns results, with
+        stats["status"] == FMMHIP_ETOL."""
+        n, kind = self._sizes.pop(ticket)
+        out = np.zeros((max(n, 1), 16))
+        strengths = np.zeros(max(n, 1)) if kind == L.REQUEST_SOLVE else None
+        rs = L.RequestStats()
+        st = self.lib.fmmhip_server_wait(self._h, int(ticket), out.ctypes.data,
+                                         None if strengths is None else strengths.ctypes.data, C.byref(rs))
+        if st not in (L.FMMHIP_OK, L.FMMHIP_ETOL):
+            raise L.FmmHipError(st, f"request {ticket} failed")
+        return out[:n], (None if strengths is None else strengths[:n]), rs.as_dict()
+
+    def stats(self):
+        s = L.ServerStats()
+        st = self.lib.fmmhip_server_stats(self._h, C.byref(s))
+        if st != L.FMMHIP_OK:
+            raise L.FmmHipError(st, "fmmhip_server_stats failed")
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.fmmhip_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ fmm!

@@ -287,6 +287,8 @@ void fmmhip_ctx_destroy(fmmhip_ctx* h) {
     if (!h) return;
     (void)hipStreamSynchronize(h->c.stream);
     free_tables(h->c.tables);
+    free_tree_scratch(h->c.tree_scratch);
+    free_list_scratch(h->c.list_scratch);
     (void)hipStreamDestroy(h->c.stream);
     delete h;
 }

@@ -1963,7 +1963,7 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
     (void)tt;
     ensure_tables(C, p);
     TablePtrs tb = ptrs(C);
-    static DBuf<int32_t>& flags = *new DBuf<int32_t>();   // process lifetime
+    DBuf<int32_t>& flags = C.m2l_flags;
     flags.resize(2);
     k_init_flags<<<1, 1, 0, C.stream>>>(flags.data(), flags.data() + 1);
     LAUNCH_CHECK();
@@ -2484,7 +2484,7 @@ void run_l2p(Context& C, Tree& tt, double* tout, int64_t nt, int p, bool LH, boo
     ensure_tables(C, p);
     if (tt.n_leaves == 0) return;
     TablePtrs tb = ptrs(C);
-    static DBuf<double>& coef = *new DBuf<double>();      // process lifetime
+    DBuf<double>& coef = C.l2p_coef;
     coef.resize((size_t)tt.n_leaves * 26 * nh(p));
     if (LH)
         k_l2p_coef<true><<<(unsigned)tt.n_leaves, 64, 0, C.stream>>>(tt.leaf_index.data(), tt.expansions.data(), p, PS, GS, HS, tb.inv_fact, coef.data());

@@ -99,15 +99,25 @@ struct Lists {
     DBuf<int32_t> dfs_s;
 };
 
-struct Tables;   // device constant tables (rotation normalisations), expansions.hip
+struct Tables;        // device constant tables (rotation normalisations), expansions.hip
+struct TreeScratch;   // tree build work buffers, tree.hip
+struct ListScratch;   // interaction-list work buffers, lists.hip
 
+// One stream and everything a build or an evaluation on it needs besides the
+// evaluation's own state: contexts on different threads share nothing.
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
     Tables* tables = nullptr;
+    TreeScratch* tree_scratch = nullptr;
+    ListScratch* list_scratch = nullptr;
+    DBuf<int32_t> m2l_flags;    // pmax and success of the last M2L pass
+    DBuf<double> l2p_coef;      // per-leaf local-to-body coefficients of the last L2P pass
     DBuf<unsigned char> temp;   // scratch for sorts/scans
 };
+void free_tree_scratch(TreeScratch* s);
+void free_list_scratch(ListScratch* s);
 
 struct Eval {
     Context* ctx = nullptr;

@@ -214,10 +214,12 @@ struct ListScratch {
     }
 };
 
-static ListScratch& lscratch() {
-    static ListScratch* s = new ListScratch();   // process lifetime, see tree.hip
-    return *s;
+static ListScratch& lscratch(Context& C) {   // per context, like the tree build's
+    if (!C.list_scratch) C.list_scratch = new ListScratch();
+    return *C.list_scratch;
 }
+
+void free_list_scratch(ListScratch* s) { delete s; }
 
 // stable sort by target (== sort_by_target, src/interaction_list.jl:183-213) of a
 // list in discovery order, then CSR over the unique targets
@@ -255,7 +257,7 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
     L.n_m2l = L.n_direct = L.n_pairs = 0;
     L.n_direct_targets = L.n_m2l_targets = 0;
     if (tt.n_branches == 0 || st.n_branches == 0) return;
-    ListScratch& S = lscratch();
+    ListScratch& S = lscratch(C);
     ListParams P{ls, mac, method, ff, nf, si};
     // forward: levels of pending pairs
     std::vector<int64_t> sizes;

@@ -361,12 +361,12 @@ struct TreeScratch {
     DBuf<double> part;
 };
 
-static TreeScratch& scratch() {
-    // one build at a time per process; intentionally never freed so that no device
-    // memory is released after the HIP runtime has shut down at process exit
-    static TreeScratch* s = new TreeScratch();
-    return *s;
+static TreeScratch& scratch(Context& C) {   // per context: builds on different streams share nothing
+    if (!C.tree_scratch) C.tree_scratch = new TreeScratch();
+    return *C.tree_scratch;
 }
+
+void free_tree_scratch(TreeScratch* s) { delete s; }
 
 // Level-by-level structure from sorted keys; returns false if deeper levels than
 // max_level would be needed.
@@ -455,7 +455,7 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     if (n == 0) return;
     if (n >= (int64_t)1 << 32) throw Error(FMMHIP_EINVAL, "more than 2^32 bodies in one system");
     if (ls < 1) throw Error(FMMHIP_EINVAL, "leaf size must be positive");
-    TreeScratch& S = scratch();
+    TreeScratch& S = scratch(C);
     // root cube (center_box, src/tree.jl:894-907; :20-26)
     const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
     S.part.resize(6 * nblk);

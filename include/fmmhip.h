@@ -197,6 +197,55 @@ fmmhip_status fmmhip_fgs_leaf_info(fmmhip_eval* ev, int64_t* out);
 fmmhip_status fmmhip_fgs_get_matrix(fmmhip_eval* ev, int32_t which, int64_t leaf, double* out);
 fmmhip_status fmmhip_fgs_nonself_rows(fmmhip_eval* ev, int64_t leaf, int64_t* out);
 
+/* ---- Request server: many fmm!(system) evaluations and FastGaussSeidel solves of uneven
+ * sizes, submitted from any host thread, run concurrently on one device. The reference has
+ * no counterpart (a Julia caller loops over fmm!(system) / solve!(system, solver),
+ * src/fmm.jl:862-884, src/solve.jl:797-946); each request does exactly what that call does
+ * for one system that is both target and source.
+ * n_slots worker threads (1..16) each own a context (stream) and an evaluation whose device
+ * buffers persist between requests. A free worker takes the oldest waiting request whose
+ * estimated HBM footprint fits beside the running ones within hbm_budget_bytes (or the oldest
+ * one when nothing runs). */
+enum { FMMHIP_REQUEST_FMM = 0, FMMHIP_REQUEST_SOLVE = 1 };
+typedef struct {
+    int32_t kind;                 /* FMMHIP_REQUEST_FMM or FMMHIP_REQUEST_SOLVE */
+    int32_t kernel;               /* fmmhip_kernel; SOLVE: point sources or source panels */
+    int64_t n;                    /* bodies */
+    const double* source_buffer;  /* host, data_per_body x n; copied by submit */
+    fmmhip_options options;       /* fmm! keywords; SOLVE uses the ones fmmhip_fgs_setup reads */
+    int32_t influence;            /* SOLVE: FMMHIP_INFLUENCE_* */
+    int32_t max_iterations;       /* SOLVE */
+    double tolerance;             /* SOLVE */
+    const double* external;       /* SOLVE: host, n external influences; copied by submit */
+} fmmhip_request;
+typedef struct {
+    int32_t status;               /* the request's fmmhip_status (FMMHIP_ETOL: results delivered, tolerance missed) */
+    int32_t sweeps;               /* SOLVE: Gauss-Seidel sweeps */
+    int32_t pmax;                 /* FMM: fmmhip_stats.pmax */
+    double queue_ms, run_ms, latency_ms;  /* submit -> start, start -> end, submit -> end */
+    double start_ms, end_ms;      /* start and end, in ms since the server was created */
+    double footprint_bytes;       /* the admission estimate */
+} fmmhip_request_stats;
+typedef struct {
+    int64_t completed, queued;
+    int32_t running;
+    double p50_ms, p99_ms;        /* latency percentiles over completed requests */
+    double mean_queue_ms;
+    double span_ms;               /* first submit -> last completion */
+} fmmhip_server_stats_t;
+typedef struct fmmhip_server fmmhip_server;
+
+fmmhip_status fmmhip_server_create(int device, int32_t n_slots, double hbm_budget_bytes, fmmhip_server** out);
+/* waits for running requests; requests not yet taken are dropped */
+void fmmhip_server_destroy(fmmhip_server* server);
+fmmhip_status fmmhip_server_submit(fmmhip_server* server, const fmmhip_request* request, int64_t* ticket);
+/* blocks until the request ends and returns its status; target_out (16 x n, original order) and
+ * strengths_out (SOLVE, n) are host buffers or NULL, filled when the status is FMMHIP_OK or
+ * FMMHIP_ETOL. A ticket is waited on once. */
+fmmhip_status fmmhip_server_wait(fmmhip_server* server, int64_t ticket, double* target_out, double* strengths_out,
+                                 fmmhip_request_stats* stats);
+fmmhip_status fmmhip_server_stats(fmmhip_server* server, fmmhip_server_stats_t* out);
+
 #ifdef __cplusplus
 }
 #endif

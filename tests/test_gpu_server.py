@@ -1,0 +1,140 @@
+"""Request server (fmmhip_server_*): every request in a concurrent mix returns what the single
+call returns.
+
+Bars: fmm! requests bit-identical to fmm!(system) through one evaluation (every FMM pass is
+deterministic: integer atomics only); FastGaussSeidel solves within 1e-10 relative of the single
+solve with the same sweep count (a sweep level adds right-hand-side updates with floating-point
+atomics, whose order varies run to run). The single calls are themselves pinned against the
+oracle by the parity and FastGaussSeidel tests."""
+import numpy as np
+import pytest
+
+import fastmultipole_amd as F
+from fastmultipole_amd import _lib, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+class _Rows:
+    """A system given by its source rows and built-in kernel (duck-typed like the host systems)."""
+
+    def __init__(self, kernel, rows):
+        self.fmm_kernel = kernel
+        self.rows = np.ascontiguousarray(rows, dtype=np.float64)
+
+    def source_buffer(self):
+        return self.rows
+
+
+FMM_CASES = [
+    ("sources-20k", _lib.POINT_SOURCE, lambda: W.uniform_sources(20000, seed=11), dict(expansion_order=5)),
+    ("ring-8k-p10", _lib.POINT_VORTEX, lambda: W.vortex_ring(8000, seed=12), dict(expansion_order=10, leaf_size=40)),
+    ("plummer-30k-hess", _lib.POINT_SOURCE, lambda: W.plummer_sources(30000, seed=13),
+     dict(expansion_order=6, scalar_potential=True, hessian=True)),
+    ("sources-500", _lib.POINT_SOURCE, lambda: W.uniform_sources(500, seed=14), dict(expansion_order=4, leaf_size=10)),
+    ("dipoles-4k", _lib.POINT_DIPOLE, lambda: W.uniform_dipoles(4000, seed=15), dict(expansion_order=7)),
+]
+SOLVE_CASES = [
+    ("quads-sphere", lambda: W.sphere_panels(10, quads=True), dict(leaf_size=30, expansion_order=6), -1.0),
+    ("tris-sphere-normal", lambda: W.sphere_panels(8, quads=False),
+     dict(leaf_size=20, expansion_order=6, influence=_lib.INFLUENCE_NORMAL_GRADIENT), None),
+]
+
+
+def _single_fmm(kernel, rows, kw):
+    ev = F.Evaluation()
+    ev.set_systems(kernel, None, rows)
+    ls = kw.get("leaf_size", 20)
+    opt = F.make_options(kw.get("expansion_order", 5), ls, ls, 0.4, None, True, None,
+                         kw.get("scalar_potential", False), True, kw.get("hessian", False))
+    ev.fmm(opt)
+    out = ev.targets()
+    ev.close()
+    return out
+
+
+def _solve_inputs(gen, ext_value):
+    rows, kind = gen()
+    n = len(rows)
+    ext = np.full(n, ext_value) if ext_value is not None else -np.random.default_rng(3).standard_normal(n)
+    return kind, rows, ext
+
+
+def _single_solve(kind, rows, ext, kw):
+    s = _Rows(kind, rows)
+    fgs = F.FastGaussSeidel(s, expansion_order=kw.get("expansion_order", 4), leaf_size=kw.get("leaf_size", 30),
+                            influence=kw.get("influence", _lib.INFLUENCE_POTENTIAL))
+    strengths, targets, sweeps, _ = fgs.solve(ext, max_iterations=6, tolerance=1e-12)
+    fgs.ev.close()
+    return strengths, targets, sweeps
+
+
+def _submit_all(server, fmm_inputs, solve_inputs):
+    tickets = []
+    for (name, kernel, rows, kw) in fmm_inputs:
+        tickets.append(("fmm", name, server.submit_fmm(_Rows(kernel, rows), **kw)))
+    for (name, kind, rows, ext, kw) in solve_inputs:
+        skw = {k: v for k, v in kw.items()}
+        tickets.append(("solve", name, server.submit_solve(_Rows(kind, rows), ext, max_iterations=6,
+                                                           tolerance=1e-12, **skw)))
+    return tickets
+
+
+def test_server_mix_matches_single_calls():
+    print("[gpu] server mix: single calls", flush=True)
+    fmm_inputs = [(name, k, gen(), kw) for name, k, gen, kw in FMM_CASES]
+    solve_inputs = []
+    for name, gen, kw, ev in SOLVE_CASES:
+        kind, rows, ext = _solve_inputs(gen, ev)
+        solve_inputs.append((name, kind, rows, ext, kw))
+    want_fmm = {name: _single_fmm(k, rows, kw) for name, k, rows, kw in fmm_inputs}
+    want_solve = {name: _single_solve(kind, rows, ext, kw) for name, kind, rows, ext, kw in solve_inputs}
+    print("[gpu] server mix: 4 slots, two rounds of every request", flush=True)
+    with F.Server(n_slots=4, hbm_budget_bytes=32 * 2**30) as server:
+        tickets = _submit_all(server, fmm_inputs, solve_inputs)
+        tickets += _submit_all(server, fmm_inputs[::-1], solve_inputs[::-1])
+        for kind, name, t in tickets:
+            targets, strengths, st = server.wait(t)
+            assert st["status"] == _lib.FMMHIP_OK, (name, st)
+            assert st["latency_ms"] >= st["run_ms"] > 0.0
+            if kind == "fmm":
+                np.testing.assert_array_equal(targets, want_fmm[name], err_msg=name)
+            else:
+                s_ref, t_ref, sw_ref = want_solve[name]
+                assert st["sweeps"] == sw_ref, name
+                assert np.abs(strengths - s_ref).max() <= 1e-10 * np.abs(s_ref).max(), name
+                for lo, hi in ((3, 4), (4, 7)):
+                    sc = np.abs(t_ref[:, lo:hi]).max()
+                    assert np.abs(targets[:, lo:hi] - t_ref[:, lo:hi]).max() <= 1e-10 * sc, (name, lo)
+        stats = server.stats()
+        assert stats["completed"] == len(tickets) and stats["queued"] == 0 and stats["running"] == 0
+        assert 0.0 < stats["p50_ms"] <= stats["p99_ms"] <= stats["span_ms"]
+
+
+def test_server_budget_serialises_requests():
+    """A budget below every request's footprint runs one request at a time (the oldest first)
+    and still completes all of them."""
+    print("[gpu] server tiny budget", flush=True)
+    rows = [W.uniform_sources(n, seed=21 + i) for i, n in enumerate((3000, 6000, 1500))]
+    want = [_single_fmm(_lib.POINT_SOURCE, r, dict(expansion_order=4)) for r in rows]
+    with F.Server(n_slots=3, hbm_budget_bytes=1.0) as server:
+        ts = [server.submit_fmm(_Rows(_lib.POINT_SOURCE, r), expansion_order=4) for r in rows]
+        spans = []
+        for t, w in zip(ts, want):
+            targets, _, st = server.wait(t)
+            np.testing.assert_array_equal(targets, w)
+            spans.append((st["start_ms"], st["end_ms"]))
+    spans.sort()
+    for (s0, e0), (s1, _) in zip(spans, spans[1:]):
+        assert s1 >= e0, spans     # no overlap: each starts after the previous ended
+
+
+def test_server_rejects_bad_requests():
+    with F.Server(n_slots=1) as server:
+        with pytest.raises(ValueError):
+            server.submit_solve(_Rows(_lib.POINT_SOURCE, W.uniform_sources(100, seed=1)), np.zeros(5))
+        t = server.submit_solve(_Rows(_lib.POINT_VORTEX, W.vortex_ring(200, seed=1)), np.zeros(200))
+        with pytest.raises(_lib.FmmHipError):
+            server.wait(t)
+    with pytest.raises(_lib.FmmHipError):
+        F.Server(n_slots=0)
