@@ -58,6 +58,89 @@ def pmc_traffic(kernel_key):
     return None
 
 
+def serving_sample(F, W, _lib, args):
+    """Serving stream: Poisson arrivals mixing fmm! evaluations (N log-uniform in [1e3, 2e6], vortex
+    rings and uniform-cube sources alternating, the headline's accuracy settings) with
+    FastGaussSeidel solves of quad source panels tiling a unit sphere (N in [20k, 200k], random
+    normal-velocity right-hand side, residual tolerance 1e-8), through the request server (HBM-budget
+    admission, requests overlapping on the device). Every request's inputs are generated before the
+    clock starts; latency is submit -> completion as the server measures it."""
+    rng = np.random.default_rng(7)
+    em = F.PowerRelativeGradient(args.tol)
+    pool = []
+    for i in range(args.serve_requests):
+        if i % 5 == 4:
+            m = int(round(math.sqrt(math.exp(rng.uniform(math.log(2e4), math.log(2e5))) / 6)))
+            rows, kind = W.sphere_panels(m, quads=True, seed=100 + i)
+            ext = rng.standard_normal(len(rows))
+            pool.append(("solve", _RowSystem(kind, rows), ext))
+        else:
+            n = int(math.exp(rng.uniform(math.log(1e3), math.log(2e6))))
+            if i % 2:
+                pool.append(("fmm", _RowSystem(_lib.POINT_VORTEX, W.vortex_ring(n, seed=100 + i)), None))
+            else:
+                pool.append(("fmm", _RowSystem(_lib.POINT_SOURCE, W.uniform_sources(n, seed=100 + i)), None))
+
+    def submit(server, item):
+        kind, sysm, ext = item
+        if kind == "fmm":
+            return server.submit_fmm(sysm, leaf_size=args.leaf, expansion_order=args.pmax,
+                                     multipole_acceptance=args.mac, error_tolerance=em, gradient=True,
+                                     hessian=sysm.fmm_kernel == _lib.POINT_VORTEX)
+        return server.submit_solve(sysm, ext, expansion_order=6, leaf_size=30, multipole_acceptance=0.5,
+                                   influence=_lib.INFLUENCE_NORMAL_GRADIENT, max_iterations=30, tolerance=1e-8)
+
+    def run(server, rate):
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(pool))) if math.isfinite(rate) else np.zeros(len(pool))
+        t0 = time.perf_counter()
+        tickets = []
+        for (item, ta) in zip(pool, arrivals):
+            delay = t0 + ta - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append((item[0], len(item[1].rows), submit(server, item)))
+        lat = {"fmm": [], "solve": []}
+        queue, first, last, bodies = [], math.inf, 0.0, 0
+        for kind, n, t in tickets:
+            _, _, st = server.wait(t)
+            lat[kind].append(st["latency_ms"])
+            queue.append(st["queue_ms"])
+            first = min(first, st["start_ms"] - st["queue_ms"])
+            last = max(last, st["end_ms"])
+            bodies += n
+        wall = (last - first) * 1e-3
+        allv = lat["fmm"] + lat["solve"]
+        pct = lambda v, q: round(float(np.percentile(v, q)), 3)
+        return {"requests": len(pool), "requests_per_s": round(len(pool) / wall, 3),
+                "bodies_per_s": round(bodies / wall, 1), "span_s": round(wall, 3),
+                "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99), "mean_queue_ms": round(float(np.mean(queue)), 3),
+                "fmm_p50_ms": pct(lat["fmm"], 50), "fmm_p99_ms": pct(lat["fmm"], 99),
+                "solve_p50_ms": pct(lat["solve"], 50), "solve_p99_ms": pct(lat["solve"], 99)}
+
+    out = {}
+    with F.Server(n_slots=args.serve_slots, hbm_budget_bytes=args.serve_budget_gib * 2**30) as server:
+        warm = run(server, math.inf)          # grows every worker's device buffers; not reported
+        log(f"serving warm-up burst: {warm}")
+        for rate in [float(r) for r in args.serve_rates.split(",")]:
+            key = "burst" if not math.isfinite(rate) else f"{rate:g}/s"
+            out[key] = run(server, rate)
+            log(f"serving at {key}: {out[key]}")
+    return {"workload": f"{len(pool)} requests: fmm! N log-uniform [1e3, 2e6] (rings / uniform cubes) and "
+                        "FastGaussSeidel quad-panel sphere solves N in [2e4, 2e5] (1 in 5), Poisson arrivals",
+            "slots": args.serve_slots, "hbm_budget_gib": args.serve_budget_gib, "rates": out}
+
+
+class _RowSystem:
+    """A system handed to the request server as source rows plus a built-in kernel."""
+
+    def __init__(self, kernel, rows):
+        self.fmm_kernel = kernel
+        self.rows = np.ascontiguousarray(rows, dtype=np.float64)
+
+    def source_buffer(self):
+        return self.rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +155,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=30)
+    ap.add_argument("--serve-rates", default="6,inf")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=float, default=96.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)"}))
@@ -153,6 +241,8 @@ def main():
     roof["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, profiles/ring1m_traffic_pmc.md)"
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
 
+    serving = None if args.no_serve else serving_sample(F, W, _lib, args)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -200,7 +290,8 @@ def main():
         "meets_error_bar": bool(M["err_v"][0] <= 1e-6),
         "latency_ms": {"p50": round(float(np.percentile(M["lat"], 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(M["lat"], 99)) * 1e3, 3),
-                       "note": "one request = one full evaluation issued back to back; no serving loop"},
+                       "note": "the headline workload's evaluations issued back to back, one at a time"},
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -112,28 +112,26 @@ def sphere_panels(m, quads=True, seed=5, radius=1.0):
     Returns (rows, kind)."""
     rng = np.random.default_rng(seed)
     t = np.linspace(-1.0, 1.0, m + 1)
-    faces = []
+    ii, jj = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    corners = [(ii, jj), (ii + 1, jj), (ii + 1, jj + 1), (ii, jj + 1)]
+    blocks = []
     for axis in range(3):
         for sgn in (-1.0, 1.0):
             a, b = [k for k in range(3) if k != axis]
-            for i in range(m):
-                for j in range(m):
-                    quad = []
-                    for (ii, jj) in ((i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1)):
-                        p = np.zeros(3)
-                        p[axis] = sgn
-                        p[a] = t[ii]
-                        p[b] = t[jj]
-                        quad.append(radius * p / np.linalg.norm(p))
-                    quad = np.array(quad)
-                    n = np.cross(quad[1] - quad[0], quad[2] - quad[0])
-                    if np.dot(n, quad.mean(axis=0)) < 0:      # counter-clockwise about the outward normal
-                        quad = quad[::-1].copy()
-                        n = -n
-                    n /= np.linalg.norm(n)
-                    quad[3] = quad[3] - np.dot(quad[3] - quad[0], n) * n
-                    faces.append(quad)
-    faces = np.array(faces)
+            quad = np.zeros((m * m, 4, 3))
+            for c, (ci, cj) in enumerate(corners):
+                quad[:, c, axis] = sgn
+                quad[:, c, a] = t[ci].ravel()
+                quad[:, c, b] = t[cj].ravel()
+            blocks.append(quad)
+    faces = np.concatenate(blocks)
+    faces = radius * faces / np.linalg.norm(faces, axis=2, keepdims=True)
+    n = np.cross(faces[:, 1] - faces[:, 0], faces[:, 2] - faces[:, 0])
+    flip = np.einsum("ij,ij->i", n, faces.mean(axis=1)) < 0      # counter-clockwise about the outward normal
+    faces[flip] = faces[flip, ::-1]
+    n[flip] = -n[flip]
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    faces[:, 3] -= np.einsum("ij,ij->i", faces[:, 3] - faces[:, 0], n)[:, None] * n
     if quads:
         q = rng.normal(size=len(faces)) / len(faces)
         return panel_rows(faces, q), 6
