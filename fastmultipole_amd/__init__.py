@@ -8,7 +8,8 @@ from . import _lib
 from .core import (Context, Evaluation, DerivativesSwitch, ErrorMethod, InteractionListMethod, PowerAbsoluteGradient,
                    PowerAbsolutePotential, PowerRelativeGradient, PowerRelativePotential,
                    RotatedCoefficientsAbsoluteGradient, RotatedCoefficientsRelativeGradient, Barba, SelfTuning,
-                   SelfTuningTreeStop, default_context, direct, fmm, make_options, FastGaussSeidel, Server, solve)
+                   SelfTuningTreeStop, default_context, direct, fmm, make_options, FastGaussSeidel, Server, solve,
+                   tune_fmm)
 from .systems import (DipolePanels, DipolePoints, GaussianVortexParticles, Gravitational, SourcePanels,
                       VortexParticles)
 
@@ -17,4 +18,4 @@ __all__ = ["Context", "Evaluation", "DerivativesSwitch", "ErrorMethod", "Interac
            "RotatedCoefficientsAbsoluteGradient", "RotatedCoefficientsRelativeGradient", "Barba", "SelfTuning",
            "SelfTuningTreeStop", "default_context", "direct", "fmm", "make_options", "Gravitational",
            "VortexParticles", "DipolePoints", "GaussianVortexParticles", "SourcePanels", "DipolePanels",
-           "FastGaussSeidel", "Server", "solve"]
+           "FastGaussSeidel", "Server", "solve", "tune_fmm"]

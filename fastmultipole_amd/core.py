@@ -10,6 +10,8 @@ buffer_to_target_system!, has_vector_potential).
 All evaluation runs on the GPU through libfmmhip; there is no CPU path.
 """
 import atexit
+import math
+import time
 import ctypes as C
 import weakref
 from dataclasses import dataclass
@@ -461,7 +463,8 @@ def _kernel_of(system):
 def fmm(target_systems, source_systems=None, *, leaf_size=None, leaf_size_source=None, leaf_size_target=None,
         expansion_order=5, multipole_acceptance=None, error_tolerance=None, shrink_recenter=True,
         interaction_list_method=None, scalar_potential=False, gradient=True, hessian=False, farfield=True,
-        nearfield=True, self_induced=True, update_target_systems=True, precision="fp64", evaluation=None):
+        nearfield=True, self_induced=True, update_target_systems=True, precision="fp64", evaluation=None,
+        tune=False):
     """fmm!(systems) / fmm!(targets, sources) of src/fmm.jl:807-915.
 
     Returns the reference's 8-tuple (optimized_args, cache, target_tree, source_tree,
@@ -495,10 +498,93 @@ def fmm(target_systems, source_systems=None, *, leaf_size=None, leaf_size_source
     switch = DerivativesSwitch(bool(scalar_potential), bool(gradient), bool(hessian))
     if update_target_systems:
         tsys.buffer_to_target_system(out, switch)
-    optimized_args = {"leaf_size_source": opt.leaf_size_source, "expansion_order": max(opt.expansion_order, 1),
+    leaf, order = opt.leaf_size_source, opt.expansion_order
+    if tune:
+        leaf, order = _tuned_leaf_size(stats, leaf), stats.pmax
+    optimized_args = {"leaf_size_source": leaf, "expansion_order": max(order, 1),
                       "multipole_acceptance": opt.multipole_acceptance}
     ev.last_stats = stats
     return (optimized_args, ev, ev.tree(0), ev.tree(1), ev.list(0), ev.list(1), (switch,), bool(stats.error_success))
+
+
+def _tuned_leaf_size(stats, leaf_size_source):
+    """fmm!(...; tune=true)'s leaf-size update (src/fmm.jl:1073-1089): with per-M2L time t_m2l and
+    per-body-pair nearfield time t_direct, the leaf size balancing both is ceil(sqrt(t_m2l / t_direct));
+    with an empty M2L list the leaf size halves (at least 1). Times are the device phase times of the
+    evaluation just run. When no body pair was evaluated the leaf size is kept (the reference's ratio
+    is undefined there)."""
+    if stats.n_m2l == 0:
+        return max(leaf_size_source >> 1, 1)
+    if stats.n_p2p_pairs == 0 or stats.ms_p2p <= 0.0:
+        return leaf_size_source
+    t_m2l = stats.ms_m2l / stats.n_m2l
+    t_direct = stats.ms_p2p / stats.n_p2p_pairs
+    return max(int(math.ceil(math.sqrt(t_m2l / t_direct))), 1)
+
+
+def tune_fmm(target_systems, source_systems=None, *, error_tolerance=None, expansion_order=4, leaf_size_source=20,
+             max_expansion_order=20, multipole_acceptances=None, verbose=True, **kwargs):
+    """tune_fmm(target_systems, source_systems; ...) of src/autotune.jl:53-232.
+
+    For each multipole acceptance (default 0.3, 0.4, ..., 0.8): one fmm!(tune=true) at the maximum
+    expansion order gives a leaf size; a second at that leaf size with the maximum lowered to the
+    order used + 2 (rerun at the full maximum if the tolerance is then missed) gives the expansion
+    order; a final timed fmm! prices the combination. Tolerances missed at the maximum order skip the
+    acceptance. Returns (tuned_params, cache): the fastest combination's leaf_size_source,
+    expansion_order and multipole_acceptance, and the device-resident Evaluation for later calls."""
+    if source_systems is None:
+        source_systems = target_systems
+    if multipole_acceptances is None:
+        multipole_acceptances = [round(0.3 + 0.1 * k, 10) for k in range(6)]
+    if verbose:
+        print("\n#======= Begin FastMultipole.tune_fmm() =======#")
+    kw = dict(kwargs)
+    kw.pop("tune", None)
+    kw.pop("update_target_systems", None)
+    kw.pop("evaluation", None)
+    single = source_systems is target_systems
+    cache = Evaluation()
+
+    def call(order, leaf, mac):
+        t0 = time.perf_counter()
+        res = fmm(target_systems, None if single else source_systems, leaf_size_source=leaf,
+                  leaf_size_target=kw.get("leaf_size_target", leaf), expansion_order=order, multipole_acceptance=mac,
+                  error_tolerance=error_tolerance, tune=True, update_target_systems=False, evaluation=cache,
+                  **{k: v for k, v in kw.items() if k != "leaf_size_target"})
+        return time.perf_counter() - t0, res[0], res[7]
+
+    fixed = error_tolerance is None
+    best = (math.inf, max_expansion_order, leaf_size_source, multipole_acceptances[0])
+    call(1, leaf_size_source, multipole_acceptances[0])        # first call: allocations and code paths warm
+    original_max = max_expansion_order
+    for mac in multipole_acceptances:
+        if verbose:
+            print(f"\nmultipole_acceptance = {mac}...")
+        t, optargs, ok = call(expansion_order if fixed else max_expansion_order, leaf_size_source, mac)
+        if not ok:
+            print(f"\terror tolerance not satisfied for max expansion order P={max_expansion_order};")
+            print("\tskipping this multipole_acceptance...")
+            continue
+        leaf_size_source = optargs["leaf_size_source"]
+        this_max = optargs["expansion_order"] + 2
+        t, optargs, ok = call(expansion_order if fixed else this_max, leaf_size_source, mac)
+        if not ok:
+            max_expansion_order = original_max
+            t, optargs, ok = call(expansion_order if fixed else max_expansion_order, leaf_size_source, mac)
+        expansion_order = optargs["expansion_order"]
+        t, optargs, ok = call(expansion_order, leaf_size_source, mac)
+        if t < best[0]:
+            best = (t, expansion_order, leaf_size_source, mac)
+        if verbose:
+            print(f"\n\tBest Parameters: \n\t\tleaf_size_source:    {leaf_size_source}\n"
+                  f"\t\texpansion_order:     {expansion_order}\n\t\tmultipole_acceptance: {mac}\n"
+                  f"\t\tcost:                {t} seconds")
+    tuned = {"leaf_size_source": best[2], "expansion_order": best[1], "multipole_acceptance": best[3]}
+    if verbose:
+        print(f"\nFinished autotune!\n\nParameters: \n\tleaf_size_source:    {best[2]}\n"
+              f"\texpansion_order:     {best[1]}\n\tmultipole_acceptance: {best[3]}\n\tcost:                {best[0]} seconds"
+              "\n\n#===============================================#\n")
+    return tuned, cache
 
 
 def direct(target_systems, source_systems=None, *, scalar_potential=False, gradient=True, hessian=False, precision="fp64",
