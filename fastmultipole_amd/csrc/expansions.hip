@@ -1748,14 +1748,18 @@ __device__ inline void lane_rotate(double (&x)[nh(P)][4], const double* __restri
         const int i0 = nh(n - 1);
         const double* Hn = Hs + len_Hs(n - 1);
         const double* mg = mags + len_Ts(n - 1);
-        double U[P + 1][4];
+        // inputs scaled in place by g(m'); outputs accumulated over v per m
 #pragma unroll
         for (int mp = 0; mp <= n; ++mp) {
             const double g = mg[mp];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) U[mp][q] = g * x[i0 + mp][q];
+            for (int q = 0; q < NQ; ++q) x[i0 + mp][q] *= g;
         }
-        double Re[P + 1][4], Ro[P + 1][4];
+        double a4[P + 1][4];
+#pragma unroll
+        for (int m = 0; m <= n; ++m)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a4[m][q] = 0.0;
 #pragma unroll
         for (int v = 0; v <= n; ++v) {
             double se[4] = {0.0, 0.0, 0.0, 0.0}, so[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1764,39 +1768,149 @@ __device__ inline void lane_rotate(double (&x)[nh(P)][4], const double* __restri
                 const double a = Hn[hidx(mp, v)];
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
-                    if (mp & 1) so[q] = fma(a, U[mp][q], so[q]);
-                    else se[q] = fma(a, U[mp][q], se[q]);
+                    if (mp & 1) so[q] = fma(a, x[i0 + mp][q], so[q]);
+                    else se[q] = fma(a, x[i0 + mp][q], se[q]);
                 }
             }
             const double a00 = Hn[hidx(0, v)];
             const double sg = ((n + v) & 1) ? -1.0 : 1.0;
             const double w = v ? 2.0 : 1.0;
             const double c = w * cv[v], sn = w * sigma * sv[v];
+            double re[4], ro[4];
 #pragma unroll
             for (int pot = 0; pot < NQ / 2; ++pot) {
                 const double ser = se[2 * pot], sei = se[2 * pot + 1], sor = so[2 * pot], soi = so[2 * pot + 1];
-                const double dr = ser - a00 * U[0][2 * pot], di = sei - a00 * U[0][2 * pot + 1];
+                const double dr = ser - a00 * x[i0][2 * pot], di = sei - a00 * x[i0][2 * pot + 1];
                 const double ppr = dr - sor + sg * (ser + sor), ppi = di - soi - sg * (sei + soi);
                 const double pmr = dr + sor + sg * (ser - sor), pmi = di + soi - sg * (sei - soi);
                 const double qpr = c * ppr - sn * ppi, qpi = c * ppi + sn * ppr;
                 const double qmr = c * pmr + sn * pmi, qmi = c * pmi - sn * pmr;
-                Re[v][2 * pot] = qpr + qmr; Re[v][2 * pot + 1] = qpi + qmi;
-                Ro[v][2 * pot] = qpr - qmr; Ro[v][2 * pot + 1] = qpi - qmi;
+                re[2 * pot] = qpr + qmr; re[2 * pot + 1] = qpi + qmi;
+                ro[2 * pot] = qpr - qmr; ro[2 * pot + 1] = qpi - qmi;
+            }
+#pragma unroll
+            for (int m = 0; m <= n; ++m) {
+                const double a = Hn[hidx(m, v)];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) a4[m][q] = fma(a, (m & 1) ? ro[q] : re[q], a4[m][q]);
             }
         }
 #pragma unroll
         for (int m = 0; m <= n; ++m) {
-            double a4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int v = 0; v <= n; ++v) {
-                const double a = Hn[hidx(m, v)];
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) a4[q] = fma(a, (m & 1) ? Ro[v][q] : Re[v][q], a4[q]);
-            }
             const double g = 0.5 * mg[m * (n + 1)];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) x[i0 + m][q] = g * a4[q];
+            for (int q = 0; q < NQ; ++q) x[i0 + m][q] = g * a4[m][q];
         }
+    }
+}
+
+template <int P, bool LH>
+__device__ inline void lane_m2l(double tx, double ty, double tz, const Branch& S, const double* __restrict__ sex, int32_t js,
+                                const TablePtrs& tb, double (&x)[nh(P)][4]) {
+    constexpr int H = nh(P);
+    constexpr int NQ = LH ? 4 : 2;
+    const double dx = tx - S.source_center[0], dy = ty - S.source_center[1], dz = tz - S.source_center[2];
+    const double x2y2 = dx * dx + dy * dy;
+    const double r = sqrt(x2y2 + dz * dz);
+    double cb, sb, ce, se;
+    if (x2y2 > 0.0) {
+        const double rho = sqrt(x2y2);
+        cb = dz / r; sb = rho / r; ce = dx / rho; se = dy / rho;
+    } else {
+        cb = dz < 0.0 ? -1.0 : 1.0; sb = 0.0; ce = 1.0; se = 0.0;
+    }
+    double cv[P + 1], sv[P + 1], er[P + 1], ei[P + 1];
+    cv[0] = 1.0; sv[0] = 0.0; er[0] = 1.0; ei[0] = 0.0;
+#pragma unroll
+    for (int k = 1; k <= P; ++k) {
+        cv[k] = cb * cv[k - 1] - sb * sv[k - 1];
+        sv[k] = sb * cv[k - 1] + cb * sv[k - 1];
+        er[k] = er[k - 1] * ce - ei[k - 1] * se;
+        ei[k] = er[k - 1] * se + ei[k - 1] * ce;
+    }
+    const double* src = sex + (int64_t)js * 4 * H;
+#pragma unroll
+    for (int n = 0; n <= P; ++n)
+#pragma unroll
+        for (int m = 0; m <= n; ++m) {   // rotate_z! (src/rotate.jl:448-483)
+            const int i = nh(n - 1) + m;
+#pragma unroll
+            for (int pot = 0; pot < NQ / 2; ++pot) {
+                const double a = src[4 * i + 2 * pot], b = src[4 * i + 2 * pot + 1];
+                x[i][2 * pot] = a * er[m] - b * ei[m];
+                x[i][2 * pot + 1] = a * ei[m] + b * er[m];
+            }
+        }
+    lane_rotate<P, LH>(x, tb.Hs, tb.zeta, 1.0, cv, sv);
+    // translate_multipole_to_local_z! (src/translate.jl:181-232): F[k] = k!/t^(k+1),
+    // one order m at a time (the translation couples the degrees of one m only), then the
+    // Lamb-Helmholtz local transform (:98-142) of that column, written back in place
+    double F[2 * P + 1];
+    const double ot = 1.0 / r;
+    F[0] = ot;
+#pragma unroll
+    for (int k = 1; k <= 2 * P; ++k) F[k] = F[k - 1] * ((double)k * ot);
+#pragma unroll
+    for (int m = 0; m <= P; ++m) {
+        double y[P + 1][4];
+#pragma unroll
+        for (int n = m; n <= P; ++n) {
+            double v4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int np = m; np <= P; ++np)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) v4[q] = fma(F[n + np], x[nh(np - 1) + m][q], v4[q]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y[n][q] = v4[q];
+        }
+#pragma unroll
+        for (int n = m; n <= P; ++n) {
+            const int i = nh(n - 1) + m;
+            double o0 = y[n][0], o1 = y[n][1], o2 = 0.0, o3 = 0.0;
+            if (LH) {
+                const double cr = y[n][2], ci = y[n][3];
+                if (n > 0) {
+                    const double r_m_n = r * (double)m / (double)n;
+                    o0 = y[n][0] - r_m_n * ci;
+                    o1 = y[n][1] + r_m_n * cr;
+                }
+                o2 = cr; o3 = ci;
+                if (n < P) {
+                    const double r_np1 = r / (double)(n + 1);
+                    o2 = cr - r_np1 * y[n + 1][2];
+                    o3 = ci - r_np1 * y[n + 1][3];
+                }
+            }
+            x[i][0] = o0; x[i][1] = o1;
+            if (LH) { x[i][2] = o2; x[i][3] = o3; }
+        }
+    }
+    lane_rotate<P, LH>(x, tb.Hs, tb.eta, -1.0, cv, sv);
+#pragma unroll
+    for (int n = 0; n <= P; ++n)
+#pragma unroll
+        for (int m = 0; m <= n; ++m) {   // back_rotate_z! (src/rotate.jl:584-616)
+            const int i = nh(n - 1) + m;
+#pragma unroll
+            for (int pot = 0; pot < NQ / 2; ++pot) {
+                const double a = x[i][2 * pot], b = x[i][2 * pot + 1];
+                x[i][2 * pot] = a * er[m] + b * ei[m];
+                x[i][2 * pot + 1] = b * er[m] - a * ei[m];
+            }
+        }
+}
+
+// Sum over the 64 lanes of v[k] (k < 64), delivered to lane k: recursive halving, each
+// step keeping the half of the array selected by one lane bit and adding the partner's
+// other half (63 shuffles for 64 values).
+template <int L>
+__device__ inline void rs_step(double (&v)[64], int lane) {
+    const bool up = (lane & L) != 0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const double mine = up ? v[j + L] : v[j];
+        const double other = up ? v[j] : v[j + L];
+        v[j] = mine + __shfl_xor(other, L, 64);
     }
 }
 
@@ -1808,123 +1922,38 @@ __global__ __launch_bounds__(256) void k_m2l_lane(const int32_t* __restrict__ m2
                                                   int32_t* __restrict__ pmax, TablePtrs tb) {
     constexpr int H = nh(P);
     constexpr int NQ = LH ? 4 : 2;
+    static_assert(H * NQ <= 64, "one coefficient per lane");
     const int64_t wt = (int64_t)blockIdx.x * 4 + wave_id();
     if (wt >= n_targets) return;   // uniform per wavefront
     const int lane = lane_id();
     const int32_t it = m2l_targets[wt];
     const int64_t l0 = m2l_offsets[wt], l1 = m2l_offsets[wt + 1];
     const double tx = tbr[it].target_center[0], ty = tbr[it].target_center[1], tz = tbr[it].target_center[2];
-    double acc[H][4];
-#pragma unroll
-    for (int i = 0; i < H; ++i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
-    for (int64_t li = l0 + lane; li < l1; li += 64) {
-        const int32_t js = m2l_s[li];
-        const Branch& S = sbr[js];
-        const double dx = tx - S.source_center[0], dy = ty - S.source_center[1], dz = tz - S.source_center[2];
-        const double x2y2 = dx * dx + dy * dy;
-        const double r = sqrt(x2y2 + dz * dz);
-        double cb, sb, ce, se;
-        if (x2y2 > 0.0) {
-            const double rho = sqrt(x2y2);
-            cb = dz / r; sb = rho / r; ce = dx / rho; se = dy / rho;
-        } else {
-            cb = dz < 0.0 ? -1.0 : 1.0; sb = 0.0; ce = 1.0; se = 0.0;
-        }
-        double cv[P + 1], sv[P + 1], er[P + 1], ei[P + 1];
-        cv[0] = 1.0; sv[0] = 0.0; er[0] = 1.0; ei[0] = 0.0;
-#pragma unroll
-        for (int k = 1; k <= P; ++k) {
-            cv[k] = cb * cv[k - 1] - sb * sv[k - 1];
-            sv[k] = sb * cv[k - 1] + cb * sv[k - 1];
-            er[k] = er[k - 1] * ce - ei[k - 1] * se;
-            ei[k] = er[k - 1] * se + ei[k - 1] * ce;
-        }
+    double acc = 0.0;   // lane k: coefficient k = NQ i + q of the target's local expansion
+    for (int64_t base = l0; base < l1; base += 64) {
+        const int64_t li = base + lane;
         double x[H][4];
-        const double* src = sex + (int64_t)js * 4 * H;
 #pragma unroll
-        for (int n = 0; n <= P; ++n)
+        for (int i = 0; i < H; ++i)
 #pragma unroll
-            for (int m = 0; m <= n; ++m) {   // rotate_z! (src/rotate.jl:448-483)
-                const int i = nh(n - 1) + m;
-#pragma unroll
-                for (int pot = 0; pot < NQ / 2; ++pot) {
-                    const double a = src[4 * i + 2 * pot], b = src[4 * i + 2 * pot + 1];
-                    x[i][2 * pot] = a * er[m] - b * ei[m];
-                    x[i][2 * pot + 1] = a * ei[m] + b * er[m];
-                }
-            }
-        lane_rotate<P, LH>(x, tb.Hs, tb.zeta, 1.0, cv, sv);
-        // translate_multipole_to_local_z! (src/translate.jl:181-232): F[k] = k!/t^(k+1)
-        double F[2 * P + 1];
-        const double ot = 1.0 / r;
-        F[0] = ot;
-#pragma unroll
-        for (int k = 1; k <= 2 * P; ++k) F[k] = F[k - 1] * ((double)k * ot);
-        double y[H][4];
-#pragma unroll
-        for (int n = 0; n <= P; ++n)
-#pragma unroll
-            for (int m = 0; m <= n; ++m) {
-                double v4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int np = m; np <= P; ++np)
-#pragma unroll
-                    for (int q = 0; q < NQ; ++q) v4[q] = fma(F[n + np], x[nh(np - 1) + m][q], v4[q]);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) y[nh(n - 1) + m][q] = v4[q];
-            }
-        // Lamb-Helmholtz local transform (:98-142) and the back rotation's input scale
-#pragma unroll
-        for (int n = 0; n <= P; ++n)
-#pragma unroll
-            for (int m = 0; m <= n; ++m) {
-                const int i = nh(n - 1) + m;
-                double o0 = y[i][0], o1 = y[i][1], o2 = 0.0, o3 = 0.0;
-                if (LH) {
-                    const double cr = y[i][2], ci = y[i][3];
-                    if (n > 0) {
-                        const double r_m_n = r * (double)m / (double)n;
-                        o0 = y[i][0] - r_m_n * ci;
-                        o1 = y[i][1] + r_m_n * cr;
-                    }
-                    o2 = cr; o3 = ci;
-                    if (n < P) {
-                        const double r_np1 = r / (double)(n + 1);
-                        o2 = cr - r_np1 * y[i + n + 1][2];
-                        o3 = ci - r_np1 * y[i + n + 1][3];
-                    }
-                }
-                x[i][0] = o0; x[i][1] = o1;
-                if (LH) { x[i][2] = o2; x[i][3] = o3; }
-            }
-        lane_rotate<P, LH>(x, tb.Hs, tb.eta, -1.0, cv, sv);
-#pragma unroll
-        for (int n = 0; n <= P; ++n)
-#pragma unroll
-            for (int m = 0; m <= n; ++m) {   // back_rotate_z! (src/rotate.jl:584-616)
-                const int i = nh(n - 1) + m;
-#pragma unroll
-                for (int pot = 0; pot < NQ / 2; ++pot) {
-                    const double a = x[i][2 * pot], b = x[i][2 * pot + 1];
-                    acc[i][2 * pot] += a * er[m] + b * ei[m];
-                    acc[i][2 * pot + 1] += b * er[m] - a * ei[m];
-                }
-            }
-        orders[li] = P;
-    }
-    // sum the lanes' partial locals and add them to the target box's expansion
-    double* dst = tex + (int64_t)it * 4 * H;
-#pragma unroll
-    for (int i = 0; i < H; ++i)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double v = acc[i][q];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == ((i * NQ + q) & 63)) dst[4 * i + q] += v;
+            for (int q = 0; q < 4; ++q) x[i][q] = 0.0;
+        if (li < l1) {
+            const int32_t js = m2l_s[li];
+            lane_m2l<P, LH>(tx, ty, tz, sbr[js], sex, js, tb, x);
+            orders[li] = P;
         }
+        double v[64];
+#pragma unroll
+        for (int k = 0; k < 64; ++k) v[k] = k < H * NQ ? x[k / NQ][k % NQ] : 0.0;
+        rs_step<32>(v, lane);
+        rs_step<16>(v, lane);
+        rs_step<8>(v, lane);
+        rs_step<4>(v, lane);
+        rs_step<2>(v, lane);
+        rs_step<1>(v, lane);
+        acc += v[0];
+    }
+    if (lane < H * NQ) tex[(int64_t)it * 4 * H + 4 * (lane / NQ) + lane % NQ] += acc;
     if (lane == 0 && l1 > l0) atomicMax(pmax, P);
 }
 
