@@ -53,7 +53,8 @@ __device__ inline void idx_nm(int idx, int& n, int& m) {
 struct Tables {
     int p = -1;
     DBuf<double> Hs, zeta, eta, Mt, Lt, inv_fact, fact;
-    DBuf<double> Ad;   // dense, even-padded pi/2 tables per degree (rotation kernels)
+    DBuf<double> Ad;     // dense, even-padded pi/2 tables per degree (rotation kernels)
+    DBuf<double> tfac;   // per degree n: 1/(n-1)! and (n+1)! by the error tests' running loops
 };
 
 // Host computation of the pi/2 Wigner tables, zeta/eta magnitudes and power
@@ -198,18 +199,32 @@ void ensure_tables(Context& C, int p) {
                 Ad.push_back(b <= n ? Hn[(hi * (hi + 1)) / 2 + lo] : 0.0);
             }
     }
+    // 1/(n-1)! and (n+1)! as the error tests' running loops form them (same operations, so the
+    // same doubles): start 1 and 2, then divide by k and multiply by k + 1 for k = 1..n-1
+    std::vector<double> tf(2 * (pm + 1), 0.0);
+    for (int n = 1; n <= pm; ++n) {
+        double a = 1.0, b = 2.0;
+        for (int k = 1; k < n; ++k) {
+            a /= (double)k;
+            b *= (double)(k + 1);
+        }
+        tf[2 * n] = a;
+        tf[2 * n + 1] = b;
+    }
     up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf); up(T.fact, fct); up(T.Ad, Ad);
+    up(T.tfac, tf);
     T.p = pm;
 }
 
 void free_tables(Tables* t) { delete t; }
 
 struct TablePtrs {
-    const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact, *fact, *Ad;
+    const double *Hs, *zeta, *eta, *Mt, *Lt, *inv_fact, *fact, *Ad, *tfac;
 };
 static TablePtrs ptrs(Context& C) {
     Tables& T = *C.tables;
-    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data(), T.fact.data(), T.Ad.data()};
+    return {T.Hs.data(), T.zeta.data(), T.eta.data(), T.Mt.data(), T.Lt.data(), T.inv_fact.data(), T.fact.data(), T.Ad.data(),
+            T.tfac.data()};
 }
 
 // ====================================================== rotation engine
@@ -243,6 +258,10 @@ static TablePtrs ptrs(Context& C) {
 static constexpr int RBUF = 1024;      // doubles of per-wavefront item buffer (64 items x 16)
 static constexpr int MAXCH = 2;        // back-rotation chunks held in registers: orders up to 20
 static constexpr int FWD_DEGREES = 64; // degrees per forward chunk of the error-controlled M2L (64 items bound it)
+// error-controlled M2L passes: orders up to the first cap in one pass with more wavefronts per
+// workgroup, the interactions needing more at the full order in a second
+enum { M2L_PASS_ALL = 0, M2L_PASS_FIRST = 1, M2L_PASS_DEFERRED = 2 };
+static constexpr int M2L_FIRST_CAP = 14;
 static constexpr int SMALL_P = 4;      // fixed orders up to this run one M2L per lane (k_m2l_lane)
 static constexpr size_t LDS_LIMIT = 160 * 1024;
 
@@ -290,18 +309,23 @@ struct WaveLds {
     double* fac;   // 6(P+1): running factors of the error tests per degree
     double* pw;    // 2(P+1): multipole power per degree (phi, chi)
     double* F;     // 2(P+1): k!/t^(k+1), k = 0..2P (M2L z-translation factors)
+    const double* Mt;   // shared: multipole power normalisations, nh(P)
+    const double* Lt;   // shared: local power normalisations, nh(P)
+    const double* tf;   // shared: 1/(n-1)!, (n+1)! per degree, 2(P+1)
 };
 
 __host__ __device__ inline int wave_lds_doubles(int P) {
     const int H4 = 4 * (((P + 1) * (P + 2)) >> 1);
     return H4 + (H4 > RBUF ? H4 : RBUF) + 14 * (P + 1);
 }
-static size_t translate_lds_bytes(int P, int nw) { return sizeof(double) * (size_t)(len_A(P) + nw * wave_lds_doubles(P)); }
+// shared part: the pi/2 tables, then Mt, Lt and the test factors
+__host__ __device__ inline int tables_len(int P) { return len_A(P) + 2 * nh(P) + 2 * (P + 1); }
+static size_t translate_lds_bytes(int P, int nw) { return sizeof(double) * (size_t)(tables_len(P) + nw * wave_lds_doubles(P)); }
 
 // wavefronts per workgroup: as many (8, 6 or 4) as the LDS holds
-static int translate_waves(int P) {
-    for (int nw : {8, 6, 4})
-        if (translate_lds_bytes(P, nw) <= LDS_LIMIT) return nw;
+static int translate_waves(int P, int max_nw = 8) {
+    for (int nw : {12, 10, 8, 6, 4})
+        if (nw <= max_nw && translate_lds_bytes(P, nw) <= LDS_LIMIT) return nw;
     throw Error(FMMHIP_EINVAL, "expansion order too large for the translation kernels");
 }
 
@@ -318,7 +342,7 @@ static size_t translate_setup(K kernel, int P, int nw) {
 
 __device__ inline WaveLds carve_wave(double* lds, int P) {
     WaveLds W;
-    W.X = lds + len_A(P) + (size_t)wave_id() * wave_lds_doubles(P);
+    W.X = lds + tables_len(P) + (size_t)wave_id() * wave_lds_doubles(P);
     const int H4 = 4 * nh(P);
     W.R = W.X + H4;
     W.eim = W.R + (H4 > RBUF ? H4 : RBUF);
@@ -326,13 +350,22 @@ __device__ inline WaveLds carve_wave(double* lds, int P) {
     W.fac = W.cs + 2 * (P + 1);
     W.pw = W.fac + 6 * (P + 1);
     W.F = W.pw + 2 * (P + 1);
+    W.Mt = lds + len_A(P);
+    W.Lt = W.Mt + nh(P);
+    W.tf = W.Lt + nh(P);
     return W;
 }
 
-// copies the dense pi/2 tables into LDS (whole workgroup, ends with a barrier)
-__device__ inline const double* load_tables(double* lds, const double* Ad, int P) {
-    const int n = len_A(P);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = Ad[i];
+// copies the dense pi/2 tables, Mt, Lt and the test factors into LDS (whole
+// workgroup, ends with a barrier)
+__device__ inline const double* load_tables(double* lds, const TablePtrs& tb, int P) {
+    const int n = len_A(P), h = nh(P);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = tb.Ad[i];
+    for (int i = threadIdx.x; i < h; i += blockDim.x) {
+        lds[n + i] = tb.Mt[i];
+        lds[n + h + i] = tb.Lt[i];
+    }
+    for (int i = threadIdx.x; i < 2 * (P + 1); i += blockDim.x) lds[n + 2 * h + i] = tb.tfac[i];
     __syncthreads();
     return lds;
 }
@@ -744,7 +777,7 @@ template <bool LH, int NW>
 __global__ __launch_bounds__(64 * NW) void k_m2m_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
                                                        int64_t b1, int P, TablePtrs tb) {
     extern __shared__ double lds[];
-    const double* A = load_tables(lds, tb.Ad, P);
+    const double* A = load_tables(lds, tb, P);
     const WaveLds W = carve_wave(lds, P);
     const int64_t ib = b0 + (int64_t)blockIdx.x * NW + wave_id();
     if (ib > b1) return;
@@ -785,7 +818,7 @@ template <bool LH, int NW>
 __global__ __launch_bounds__(64 * NW) void k_l2l_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
                                                        int64_t b1, int P, TablePtrs tb) {
     extern __shared__ double lds[];
-    const double* A = load_tables(lds, tb.Ad, P);
+    const double* A = load_tables(lds, tb, P);
     const WaveLds W = carve_wave(lds, P);
     const int64_t ib = b0 + (int64_t)blockIdx.x * NW + wave_id();
     if (ib > b1) return;
@@ -848,16 +881,13 @@ __device__ inline void fill_factors(const WaveLds& W, const ErrGeo& G, int P) {
     const int n = lane_id();
     if (n < 1 || n > P) return;
     double n_f_t_np1 = G.rinv * G.rinv, r_l_nm1 = 1.0, r_mp_inv_np2 = G.r_mp_inv * G.r_mp_inv * G.r_mp_inv;
-    double nm1_f_inv = 1.0, np1_f = 2.0;
     for (int k = 1; k < n; ++k) {
         n_f_t_np1 *= (double)(k + 1) * G.rinv;
         r_l_nm1 *= G.r_l;
         r_mp_inv_np2 *= G.r_mp_inv;
-        nm1_f_inv /= (double)k;
-        np1_f *= (double)(k + 1);
     }
     double* f = W.fac + 6 * n;
-    f[0] = n_f_t_np1; f[1] = r_l_nm1; f[2] = r_mp_inv_np2; f[3] = nm1_f_inv; f[4] = np1_f;
+    f[0] = n_f_t_np1; f[1] = r_l_nm1; f[2] = r_mp_inv_np2; f[3] = W.tf[2 * n]; f[4] = W.tf[2 * n + 1];
 }
 
 // degree powers of the chunk from the terms zrot_in left in R (summed in m order)
@@ -880,21 +910,35 @@ __device__ inline void chunk_powers(const WaveLds& W, int n0, int n1) {
 template <bool LH>
 __device__ int m2l_test_chunk(const WaveLds& W, const ErrGeo& G, const ErrParams& E, const TablePtrs& tb, int nA, int nB) {
     if (E.kind >= 2) {
-        // degree-n part of the z-translation, all m (translate_multipole_to_local_z_n!, :238-295) -> R
+        // degree-n part of the z-translation, all m (translate_multipole_to_local_z_n!, :238-295) -> R.
+        // The weight of X[np][m] in degree n is the running product n!/t^(n+1) * prod_{k=n+1}^{n+np} (k/t),
+        // the same chain whatever m is, so each degree's chain is formed once (lane n) and kept in R
+        // after the results.
         const int i0 = nh(nA - 1), cnt = nh(nB) - i0;
+        double* chain = W.R + 4 * cnt;
+        {
+            const int n = nA + lane_id();
+            if (n <= nB) {
+                double f = W.fac[6 * n];
+                double* cf = chain + (nh(n - 1) - i0);
+                cf[0] = f;
+                for (int np = 1; np <= n; ++np) {
+                    f *= (double)(n + np) * G.rinv;
+                    cf[np] = f;
+                }
+            }
+        }
+        wave_sync();
         for (int k = lane_id(); k < cnt; k += 64) {
             int n, m;
             idx_nm(i0 + k, n, m);
-            double n_m_f = W.fac[6 * n];
-            for (int q = 0; q < m; ++q) n_m_f *= (double)(n + q + 1) * G.rinv;
-            double v1r = 0, v1i = 0, v2r = 0, v2i = 0, f = n_m_f;
-            int n_np = n + m;
+            const double* cf = chain + (nh(n - 1) - i0);
+            double v1r = 0, v1i = 0, v2r = 0, v2i = 0;
             int iw = nh(m - 1) + m;
             for (int np = m; np <= n; ++np) {
+                const double f = cf[np];
                 v1r += f * W.X[4 * iw]; v1i += f * W.X[4 * iw + 1];
                 if (LH) { v2r += f * W.X[4 * iw + 2]; v2i += f * W.X[4 * iw + 3]; }
-                n_np += 1;
-                f *= (double)n_np * G.rinv;
                 iw += np + 1;
             }
             double* t = W.R + 4 * k;
@@ -918,7 +962,7 @@ __device__ int m2l_test_chunk(const WaveLds& W, const ErrGeo& G, const ErrParams
             }
         } else {
             const double pw_prev = W.pw[2 * (n - 1)], pw_n = W.pw[2 * n], pc_n = W.pw[2 * n + 1];
-            const double Mt_n0 = tb.Mt[nh(n - 2)], Mt_next = tb.Mt[in0];
+            const double Mt_n0 = W.Mt[nh(n - 2)], Mt_next = W.Mt[in0];
             if (E.kind == 3) {
                 eps_mp = SQRT3_D * pw_prev * r_mp_inv_np2 * np1_f / Mt_n0;
                 if (LH) eps_mp += SQRT3_D * (double)n * pc_n * r_mp_inv_np2 * np1_f / Mt_next;
@@ -962,15 +1006,19 @@ __device__ int m2l_test_chunk(const WaveLds& W, const ErrGeo& G, const ErrParams
             } else {   // local_power (:455-502) of the degree-n translation
                 const double* tmp = W.R + 4 * (in0 - nh(nA - 1));
                 double lp = 0, lc = 0, Nt = 0, cnr = 0, cni = 0;
+                // the Lamb-Helmholtz corrections multiply by r/n and r/(n+1) formed once
+                // (the reference divides per term: rounding-level differences)
+                const double r_n = G.r / (double)n, r_np1 = G.r / (double)(n + 1);
                 for (int m = n; m >= 0; --m) {
-                    Nt = tb.Lt[in0 + m];
+                    Nt = W.Lt[in0 + m];
                     double pr = tmp[4 * m], pi = tmp[4 * m + 1];
                     if (LH) {
                         const double cor = tmp[4 * m + 2], coi = tmp[4 * m + 3];
-                        pr = pr - coi * G.r * (double)m / (double)n;
-                        pi = pi + cor * G.r * (double)m / (double)n;
-                        const double cr = cor - cnr * G.r / (double)(n + 1);
-                        const double ci = coi - cni * G.r / (double)(n + 1);
+                        const double rm = r_n * (double)m;
+                        pr = pr - coi * rm;
+                        pi = pi + cor * rm;
+                        const double cr = cor - cnr * r_np1;
+                        const double ci = coi - cni * r_np1;
                         cnr = cor; cni = coi;
                         lc += (cr * cr + ci * ci) * Nt * Nt;
                     }
@@ -996,11 +1044,10 @@ __device__ int m2l_test_chunk(const WaveLds& W, const ErrGeo& G, const ErrParams
 
 // one M2L interaction (fixed order when E.kind == 0) accumulated into acc; returns the order used
 template <bool LH>
-__device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const Branch& S, const WaveLds& W,
+__device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const double (&sc)[3], const WaveLds& W,
                         const double* A, int P, const TablePtrs& tb, const ErrParams& E, int& success,
-                        double (&acc)[MAXCH][8]) {
-    const double dx = T.target_center[0] - S.source_center[0], dy = T.target_center[1] - S.source_center[1],
-                 dz = T.target_center[2] - S.source_center[2];
+                        double (&acc)[MAXCH][8], bool defer) {
+    const double dx = T.target_center[0] - sc[0], dy = T.target_center[1] - sc[1], dz = T.target_center[2] - sc[2];
     const double r = fill_angles(W, dx, dy, dz, P);
     fill_m2l_factors(W, r, P);
     int Pu = P;
@@ -1033,7 +1080,7 @@ __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const B
         for (int n0 = 0; n0 <= P;) {
             const int n1 = chunk_end(n0, P, FWD_DEGREES);
             if (power) {
-                zrot_in<LH, true>(src, W, tb.zeta, tb.Mt, n0, n1);
+                zrot_in<LH, true>(src, W, tb.zeta, W.Mt, n0, n1);
                 wave_sync();
                 chunk_powers<LH>(W, n0, n1);
             } else {
@@ -1052,6 +1099,7 @@ __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const B
             }
             n0 = n1 + 1;
         }
+        if (!success && defer) return -1;   // left for the pass at the full order
     }
     translate_and_transform<OP_M2L, LH>(W, r, Pu, tb.eta);
     rot_back_acc<LH>(W, A, tb.eta, -1.0, P, Pu, acc);
@@ -1065,35 +1113,55 @@ template <bool LH, int NW>
 __global__ __launch_bounds__(64 * NW) void k_m2l(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
                                                  const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
                                                  const Branch* __restrict__ sbr, const double* __restrict__ sex,
-                                                 double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
-                                                 int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
+                                                 double* __restrict__ tex, int P, int PE, int mode, ErrParams E,
+                                                 int32_t* __restrict__ orders, int32_t* __restrict__ pmax,
+                                                 int32_t* __restrict__ all_ok, TablePtrs tb) {
     extern __shared__ double lds[];
-    const double* A = load_tables(lds, tb.Ad, P);
+    const int64_t l0 = m2l_offsets[blockIdx.x], l1 = m2l_offsets[blockIdx.x + 1];
+    if (mode == M2L_PASS_DEFERRED) {   // only the interactions the first pass left (order -1)
+        int pending = 0;
+        for (int64_t li = l0 + threadIdx.x; li < l1; li += 64 * NW) pending |= orders[li] < 0;
+        if (!__syncthreads_or(pending)) return;
+    }
+    const double* A = load_tables(lds, tb, P);
     const WaveLds W = carve_wave(lds, P);
     const int32_t it = m2l_targets[blockIdx.x];
-    const int64_t l0 = m2l_offsets[blockIdx.x], l1 = m2l_offsets[blockIdx.x + 1];
-    const int H4 = 4 * nh(P);
+    const int H4 = 4 * nh(P), HE4 = 4 * nh(PE);   // coefficients worked on / stored per expansion
     const Branch T = tbr[it];
     double acc[MAXCH][8] = {};
     int local_max = 0, local_ok = 1;
-    for (int64_t li = l0 + wave_id(); li < l1; li += NW) {
-        const int32_t js = m2l_s[li];
-        const Branch S = sbr[js];
+    // the next interaction's source index and centre are loaded one iteration ahead
+    int64_t li = l0 + wave_id();
+    int32_t js_next = 0;
+    double sc_next[3] = {0.0, 0.0, 0.0};
+    if (li < l1) {
+        js_next = m2l_s[li];
+        for (int d = 0; d < 3; ++d) sc_next[d] = sbr[js_next].source_center[d];
+    }
+    for (; li < l1; li += NW) {
+        const int32_t js = js_next;
+        const double sc[3] = {sc_next[0], sc_next[1], sc_next[2]};
+        if (li + NW < l1) {
+            js_next = m2l_s[li + NW];
+            for (int d = 0; d < 3; ++d) sc_next[d] = sbr[js_next].source_center[d];
+        }
+        if (mode == M2L_PASS_DEFERRED && orders[li] >= 0) continue;
         int ok = 1;
-        const int Pu = m2l_pair<LH>(sex + (int64_t)js * H4, T, S, W, A, P, tb, E, ok, acc);
+        const int Pu = m2l_pair<LH>(sex + (int64_t)js * HE4, T, sc, W, A, P, tb, E, ok, acc, mode == M2L_PASS_FIRST);
         if (lane_id() == 0) orders[li] = Pu;
+        if (Pu < 0) continue;
         local_max = max(local_max, Pu);
         local_ok = local_ok && ok;
     }
     acc_store<false>(W.X, P, acc);
     __syncthreads();
-    const double* base = lds + len_A(P);
+    const double* base = lds + tables_len(P);
     const int stride = wave_lds_doubles(P);
     for (int i = threadIdx.x; i < H4; i += 64 * NW) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) s += base[(size_t)w * stride + i];
-        tex[(int64_t)it * H4 + i] += s;
+        tex[(int64_t)it * HE4 + i] += s;
     }
     if (lane_id() == 0) {
         atomicMax(pmax, local_max);
@@ -1102,13 +1170,26 @@ __global__ __launch_bounds__(64 * NW) void k_m2l(const int32_t* __restrict__ m2l
 }
 
 template <bool LH, int NW>
-static void launch_m2l(Context& C, int p, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E, int32_t* flags,
-                       const TablePtrs& tb) {
+static void launch_m2l(Context& C, int p, int pe, int mode, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
+                       int32_t* flags, const TablePtrs& tb) {
     const size_t sh = translate_setup(k_m2l<LH, NW>, p, NW);
     k_m2l<LH, NW><<<(unsigned)Ls.n_m2l_targets, 64 * NW, sh, C.stream>>>(
         Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
-        st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags, flags + 1, tb);
+        st.expansions.data(), tt_dst.expansions.data(), p, pe, mode, E, Ls.m2l_order.data(), flags, flags + 1, tb);
     LAUNCH_CHECK();
+}
+
+// the wave M2L at order cap p over arrays of order pe, in the given pass mode
+template <bool LH>
+static void run_m2l_wave(Context& C, int p, int pe, int mode, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
+                         int32_t* flags, const TablePtrs& tb) {
+    switch (translate_waves(p, 12)) {
+        case 12: launch_m2l<LH, 12>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        case 10: launch_m2l<LH, 10>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        case 8: launch_m2l<LH, 8>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        case 6: launch_m2l<LH, 6>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        default: launch_m2l<LH, 4>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+    }
 }
 
 // ============================================ M2L on matrix cores (FP64 MFMA)
@@ -2004,17 +2085,15 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
         if (m2l_engine() == M2L_MFMA) {
             if (LH) run_m2l_mfma<true>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
             else run_m2l_mfma<false>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+        } else if (err_kind != FMMHIP_ERR_NONE && p > M2L_FIRST_CAP) {
+            // error-controlled orders above the cap: a first pass at the cap (more wavefronts fit
+            // in LDS) settles every interaction whose tolerance is met by then; the rest run at p
+            auto run = LH ? run_m2l_wave<true> : run_m2l_wave<false>;
+            run(C, M2L_FIRST_CAP, p, M2L_PASS_FIRST, Ls, st, tt_dst, E, flags.data(), tb);
+            run(C, p, p, M2L_PASS_DEFERRED, Ls, st, tt_dst, E, flags.data(), tb);
         } else {
-            const int nw = translate_waves(p);
-            if (LH) {
-                if (nw == 8) launch_m2l<true, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-                else if (nw == 6) launch_m2l<true, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-                else launch_m2l<true, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-            } else {
-                if (nw == 8) launch_m2l<false, 8>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-                else if (nw == 6) launch_m2l<false, 6>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-                else launch_m2l<false, 4>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
-            }
+            auto run = LH ? run_m2l_wave<true> : run_m2l_wave<false>;
+            run(C, p, p, M2L_PASS_ALL, Ls, st, tt_dst, E, flags.data(), tb);
         }
     }
     int32_t h[2];
