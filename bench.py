@@ -156,11 +156,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 256 MiB device to device three times (the FETCH_SIZE / WRITE_SIZE calibration)")
     ap.add_argument("--serve-requests", type=int, default=30)
     ap.add_argument("--serve-rates", default="6,inf")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=float, default=96.0)
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": 3 * a.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)"}))
         sys.exit(2)
@@ -234,7 +245,7 @@ def main():
                 "peak": PEAK_FP32_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}
     else:
-        key = "k_m2l<true, 6>"
+        key = "k_m2l<true, 10>"
         roof = {"kernel": key + " (rotation-based M2L with Lamb-Helmholtz terms, FP64; F(p) flop/interaction)",
                 "bound": "vector-alu", "achieved": round(m2l_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}

@@ -79,8 +79,14 @@ struct Timer {
         HIP_CHECK(hipEventCreate(&b));
         HIP_CHECK(hipEventRecord(a, C.stream));
     }
-    double stop() {
+    bool marked = false;
+    // records the end without waiting for it (read later by stop)
+    void mark() {
         HIP_CHECK(hipEventRecord(b, C.stream));
+        marked = true;
+    }
+    double stop() {
+        if (!marked) HIP_CHECK(hipEventRecord(b, C.stream));
         HIP_CHECK(hipEventSynchronize(b));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, a, b));
@@ -112,19 +118,59 @@ static void eval_build_lists(Eval& E, const fmmhip_options& o, bool count_pairs 
 
 static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, bool is_estimate);
 
+// releases everything a context holds besides its handle
+static void release_context(Context& c) {
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    free_tables(c.tables);
+    free_tree_scratch(c.tree_scratch);
+    free_list_scratch(c.list_scratch);
+    c.tables = nullptr;
+    c.tree_scratch = nullptr;
+    c.list_scratch = nullptr;
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+}
+
 Eval::~Eval() {
     delete estimate;
     free_fgs(fgs);
+    if (side) {
+        release_context(*side);
+        delete side;
+    }
+}
+
+// the evaluation's second stream, with its own scratch and tables
+static Context& side_context(Eval& E) {
+    if (!E.side) {
+        Context* c = new Context();
+        c->device = E.ctx->device;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            throw Error(FMMHIP_EDEVICE, "hipStreamCreateWithFlags failed");
+        }
+        E.side = c;
+    }
+    return *E.side;
+}
+
+// makes `waiter` wait for the work enqueued on `signaller` so far
+static void stream_wait(Context& waiter, Context& signaller) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const hipError_t a = hipEventRecord(e, signaller.stream);
+    const hipError_t b = a == hipSuccess ? hipStreamWaitEvent(waiter.stream, e, 0) : a;
+    (void)hipEventDestroy(e);
+    HIP_CHECK(b);
 }
 
 // estimate_influence! for relative tolerances (src/fmm.jl:1193-1276): a full
 // low-order evaluation (p = 3, leaf 5, MAC 0.6, potential and gradient) on its
 // own trees; max |grad| (or potential) per main-tree branch
-static void estimate_influence(Eval& E, const fmmhip_options& o) {
-    Context& C = *E.ctx;
+static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
     if (!E.estimate) E.estimate = new Eval();
     Eval& X = *E.estimate;
-    X.ctx = E.ctx;
+    X.ctx = &C;
     X.kernel = E.kernel;
     X.nt = E.nt;
     X.ns = E.ns;
@@ -160,11 +206,6 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
     const bool PS = o.scalar_potential != 0, GS = o.gradient != 0, HS = o.hessian != 0;
     Timer total(C);
     double t_est = 0, t_gather = 0;
-    if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) {
-        Timer t(C);
-        estimate_influence(E, o);
-        t_est = t.stop();
-    }
     E.tout.resize((size_t)13 * std::max<int64_t>(1, E.nt));
     HIP_CHECK(hipMemsetAsync(E.tout.data(), 0, E.tout.size() * sizeof(double), C.stream));
     if (E.tt.n_branches) HIP_CHECK(hipMemsetAsync(E.tt.expansions.data(), 0, E.tt.expansions.size() * sizeof(double), C.stream));
@@ -173,22 +214,37 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
     E.error_success = 1;
     double t_p2p = 0, t_up = 0, t_m2l = 0, t_down = 0;
     if (E.nt > 0 && E.ns > 0) {
+        // The nearfield (P2P) runs on the evaluation's stream while the far field
+        // (estimate_influence, upward pass, M2L, L2L) runs on a second stream; L2P adds
+        // into the nearfield's results once both are done. The nearfield's launch fills the
+        // device, so far-field phase times include waiting for it (the estimate's small
+        // launches mostly run after it); the overlap takes the host synchronisations of the
+        // estimate's tree and list builds off the critical path. The estimate's own
+        // evaluation runs on one stream.
+        Context& F = is_estimate ? C : side_context(E);
         ensure_tables(C, p);
-        {
-            Timer t(C);
-            run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision);
-            t_p2p = t.stop();
+        if (&F != &C) {
+            ensure_tables(F, p);
+            stream_wait(F, C);
+        }
+        Timer tp(C);
+        run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision);
+        tp.mark();
+        if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) {
+            Timer t(F);
+            estimate_influence(E, o, F);
+            t_est = t.stop();
         }
         {
-            Timer t(C);
-            run_p2m(C, E.kernel, E.st, p);
-            run_m2m(C, E.st, p, LH);
+            Timer t(F);
+            run_p2m(F, E.kernel, E.st, p);
+            run_m2m(F, E.st, p, LH);
             t_up = t.stop();
         }
         {
-            Timer t(C);
+            Timer t(F);
             int32_t pmax = 0, ok = 1;
-            run_m2l(C, E.tt, E.st, E.tt, E.lists, p, LH, o.error_kind, o.error_relative != 0, o.error_tolerance,
+            run_m2l(F, E.tt, E.st, E.tt, E.lists, p, LH, o.error_kind, o.error_relative != 0, o.error_tolerance,
                     o.error_bonus != 0, o.precision, &pmax, &ok);
             E.pmax = pmax;
             E.error_success = ok;
@@ -196,11 +252,17 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
             t_m2l = t.stop();
         }
         {
-            Timer t(C);
-            run_l2l(C, E.tt, p, LH);
-            run_l2p(C, E.tt, E.tout.data(), E.nt, p, LH, PS, GS, HS);
+            Timer t(F);
+            run_l2l(F, E.tt, p, LH);
             t_down = t.stop();
         }
+        if (&F != &C) stream_wait(C, F);
+        {
+            Timer t(C);
+            run_l2p(C, E.tt, E.tout.data(), E.nt, p, LH, PS, GS, HS);
+            t_down += t.stop();
+        }
+        t_p2p = tp.stop();
     }
     if (!is_estimate) {
         Timer t(C);
@@ -285,11 +347,7 @@ fmmhip_status fmmhip_ctx_create(int device, fmmhip_ctx** out) {
 
 void fmmhip_ctx_destroy(fmmhip_ctx* h) {
     if (!h) return;
-    (void)hipStreamSynchronize(h->c.stream);
-    free_tables(h->c.tables);
-    free_tree_scratch(h->c.tree_scratch);
-    free_list_scratch(h->c.list_scratch);
-    (void)hipStreamDestroy(h->c.stream);
+    release_context(h->c);
     delete h;
 }
 

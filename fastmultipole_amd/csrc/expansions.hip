@@ -454,6 +454,11 @@ __device__ inline void zrot_in(const double* __restrict__ src, const WaveLds& W,
     }
 }
 
+// Item buffer of a rotation chunk in R: slot-major, 8 double2 slots per item (v half h,
+// potential, m parity: slot 4h + 2 pot + parity) of the chunk's (at most 64) items, so the
+// lanes of a wavefront write consecutive 16-byte words.
+__device__ inline double2* rslot(const WaveLds& W, int slot, int item) { return (double2*)W.R + slot * 64 + item; }
+
 // first product of the chunk [n0, n1] (degrees <= nlim): X -> R
 template <bool LH>
 __device__ inline void rot_step1(const WaveLds& W, const double* A, double sigma, int n0, int n1, int nlim) {
@@ -496,7 +501,6 @@ __device__ inline void rot_step1(const WaveLds& W, const double* A, double sigma
             se1[2] = fma(ae.y, xe1.x, se1[2]); se1[3] = fma(ae.y, xe1.y, se1[3]);
         }
     }
-    double* out = W.R + 16 * k;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int v = 2 * j + h;
@@ -517,9 +521,8 @@ __device__ inline void rot_step1(const WaveLds& W, const double* A, double sigma
             // Q+ = (c + i sn) P+,  Q- = (c - i sn) P-
             const double qpr = c * ppr - sn * ppi, qpi = c * ppi + sn * ppr;
             const double qmr = c * pmr + sn * pmi, qmi = c * pmi - sn * pmr;
-            double* o = out + 8 * h + 4 * pot;
-            *(double2*)o = make_double2(qpr + qmr, qpi + qmi);         // even m
-            *(double2*)(o + 2) = make_double2(qpr - qmr, qpi - qmi);   // odd m
+            *rslot(W, 4 * h + 2 * pot, k) = make_double2(qpr + qmr, qpi + qmi);       // even m
+            *rslot(W, 4 * h + 2 * pot + 1, k) = make_double2(qpr - qmr, qpi - qmi);   // odd m
         }
     }
 }
@@ -529,16 +532,16 @@ template <bool LH>
 __device__ inline void rot_step2(const WaveLds& W, const double* A, const double* mags, int n0, int n, int j, double o[8]) {
     const int s = a_stride(n);
     const double* An = A + a_base(n) + 2 * j;   // row v: (A(v, 2j), A(v, 2j+1)) = (A(2j, v), A(2j+1, v))
-    const double* Rn = W.R + 16 * (items_below(n) - items_below(n0));
+    const int kn = items_below(n) - items_below(n0);   // the degree's first item in the chunk
     double e[4] = {0.0, 0.0, 0.0, 0.0}, d[4] = {0.0, 0.0, 0.0, 0.0};
     for (int v = 0; v <= n; ++v) {
         const double2 a = *(const double2*)(An + v * s);
-        const double* rv = Rn + 8 * v;   // item v/2, half v&1
-        const double2 re0 = *(const double2*)rv, ro0 = *(const double2*)(rv + 2);
+        const int kk = kn + (v >> 1), h4 = 4 * (v & 1);   // item v/2, half v&1
+        const double2 re0 = *rslot(W, h4, kk), ro0 = *rslot(W, h4 + 1, kk);
         e[0] = fma(a.x, re0.x, e[0]); e[1] = fma(a.x, re0.y, e[1]);
         d[0] = fma(a.y, ro0.x, d[0]); d[1] = fma(a.y, ro0.y, d[1]);
         if (LH) {
-            const double2 re1 = *(const double2*)(rv + 4), ro1 = *(const double2*)(rv + 6);
+            const double2 re1 = *rslot(W, h4 + 2, kk), ro1 = *rslot(W, h4 + 3, kk);
             e[2] = fma(a.x, re1.x, e[2]); e[3] = fma(a.x, re1.y, e[3]);
             d[2] = fma(a.y, ro1.x, d[2]); d[3] = fma(a.y, ro1.y, d[3]);
         }

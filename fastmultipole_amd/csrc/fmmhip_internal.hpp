@@ -134,6 +134,7 @@ struct Eval {
     int32_t pmax = 0, error_success = 1;
     int p_built = -1;
     Eval* estimate = nullptr;                 // persistent low-order evaluation of estimate_influence
+    Context* side = nullptr;                  // second stream: the far field runs beside the nearfield
     DBuf<double> influence;                   // per-target max |grad| (or |phi|) of the estimate
     struct FgsState* fgs = nullptr;           // FastGaussSeidel solver state (fgs.hip)
     ~Eval();
