@@ -174,6 +174,47 @@ __device__ inline int64_t stream_body(const int32_t* __restrict__ dir_s, const i
     return sbr[dir_s[li]].first + (f - dir_cum[li]);
 }
 
+// Lanes of a workgroup over a target leaf's work units (target pairs, or targets): each unit is
+// taken by a group of t adjacent lanes of one wavefront, 64/t groups per wavefront (the lanes
+// left over idle), with t the largest group size that covers the leaf in one pass. Any t, not only
+// powers of two, so leaves of every size keep most lanes busy; a group's partial sums are folded
+// by shuffle-down steps that stay inside the group.
+struct LeafLanes {
+    int t;         // lanes per unit
+    int gpw;       // units per wavefront
+    int per_pass;  // units per workgroup pass
+    int gl;        // this lane's unit within its wavefront (>= gpw: idle lane)
+    int lane_k;    // this lane's rank within its unit's group
+};
+__device__ inline LeafLanes leaf_lanes(int64_t units) {
+    constexpr int NW = P2P_BLOCK / 64;
+    LeafLanes L;
+    L.t = 1;
+    for (int c = 64; c > 1; --c)
+        if ((int64_t)NW * (64 / c) >= units) {
+            L.t = c;
+            break;
+        }
+    L.gpw = 64 / L.t;
+    L.per_pass = NW * L.gpw;
+    const int ln = (int)(threadIdx.x & 63);
+    L.gl = ln / L.t;
+    L.lane_k = ln % L.t;
+    return L;
+}
+// this lane's unit in the pass starting at c0
+__device__ inline int64_t leaf_unit(const LeafLanes& L, int64_t c0) {
+    return c0 + (int64_t)(threadIdx.x >> 6) * L.gpw + L.gl;
+}
+// sum of a group's values, left in its first lane
+__device__ inline double group_fold(double v, const LeafLanes& L) {
+    for (int w = 1; w < L.t; w <<= 1) {
+        const double o = __shfl_down(v, w, 64);
+        if (L.lane_k + w < L.t) v += o;
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------- FP64 form
 // Each thread evaluates two adjacent targets of the leaf against its share of
 // the staged sources, so a source read from LDS serves two pair evaluations.
@@ -196,13 +237,12 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
     const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
     const int tid = threadIdx.x;
     const int64_t ngroups = (T.n_bodies + 1) / 2;           // target pairs
-    int tpt = 1;   // threads per target pair
-    while (tpt < 64 && ngroups * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
-    const int per_chunk = P2P_BLOCK / tpt;
-    for (int64_t c0 = 0; c0 < ngroups; c0 += per_chunk) {
-        const int64_t g = c0 + tid / tpt;
-        const int lane_k = tid % tpt;
-        const bool act0 = 2 * g < T.n_bodies, act1 = 2 * g + 1 < T.n_bodies;
+    const LeafLanes LL = leaf_lanes(ngroups);
+    const int tpt = LL.t, lane_k = LL.lane_k;
+    for (int64_t c0 = 0; c0 < ngroups; c0 += LL.per_pass) {
+        const int64_t g = leaf_unit(LL, c0);
+        const bool own = LL.gl < LL.gpw;
+        const bool act0 = own && 2 * g < T.n_bodies, act1 = own && 2 * g + 1 < T.n_bodies;
         const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
         const double xa = tx[i0], ya = ty[i0], za = tz[i0];
         const double xb = tx[i1], yb = ty[i1], zb = tz[i1];
@@ -238,10 +278,9 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                     }
                 }
         }
-        // fold the tpt partial sums of a target pair (adjacent lanes)
-        for (int w = 1; w < tpt; w <<= 1)
+        // fold the partial sums of a target pair's lanes
 #pragma unroll
-            for (int k = 0; k < NA; ++k) { a[k] += __shfl_xor(a[k], w, 64); b[k] += __shfl_xor(b[k], w, 64); }
+        for (int k = 0; k < NA; ++k) { a[k] = group_fold(a[k], LL); b[k] = group_fold(b[k], LL); }
         fold_acc<KERNEL, HS, double>(a);
         fold_acc<KERNEL, HS, double>(b);
         if (lane_k == 0) {
@@ -281,13 +320,12 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
     const int tid = threadIdx.x;
     const double cx0 = T.target_center[0], cy0 = T.target_center[1], cz0 = T.target_center[2];
     const int64_t ngroups = (T.n_bodies + 1) / 2;
-    int tpt = 1;
-    while (tpt < 64 && ngroups * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
-    const int per_chunk = P2P_BLOCK / tpt;
-    for (int64_t c0 = 0; c0 < ngroups; c0 += per_chunk) {
-        const int64_t g = c0 + tid / tpt;
-        const int lane_k = tid % tpt;
-        const bool act0 = 2 * g < T.n_bodies, act1 = 2 * g + 1 < T.n_bodies;
+    const LeafLanes LL = leaf_lanes(ngroups);
+    const int tpt = LL.t, lane_k = LL.lane_k;
+    for (int64_t c0 = 0; c0 < ngroups; c0 += LL.per_pass) {
+        const int64_t g = leaf_unit(LL, c0);
+        const bool own = LL.gl < LL.gpw;
+        const bool act0 = own && 2 * g < T.n_bodies, act1 = own && 2 * g + 1 < T.n_bodies;
         const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
         float th[2][3], tl[2][3];
 #pragma unroll
@@ -348,9 +386,8 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int k = 0; k < NA; ++k) acc[h][k] = (double)sum[h][k] - (double)cmp[h][k];
-        for (int w = 1; w < tpt; w <<= 1)
 #pragma unroll
-            for (int k = 0; k < NA; ++k) { acc[0][k] += __shfl_xor(acc[0][k], w, 64); acc[1][k] += __shfl_xor(acc[1][k], w, 64); }
+        for (int k = 0; k < NA; ++k) { acc[0][k] = group_fold(acc[0][k], LL); acc[1][k] = group_fold(acc[1][k], LL); }
         if (lane_k == 0) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -507,13 +544,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
     const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
     const int64_t base = dir_cum[l0], end = dir_cum[l1];
     const int tid = threadIdx.x;
-    int tpt = 1;
-    while (tpt < 64 && T.n_bodies * (tpt * 2) <= P2P_BLOCK) tpt *= 2;
-    const int per_chunk = P2P_BLOCK / tpt;
-    for (int64_t c0 = 0; c0 < T.n_bodies; c0 += per_chunk) {
-        const int64_t ti = c0 + tid / tpt;
-        const int lane_k = tid % tpt;
-        const bool active = ti < T.n_bodies;
+    const LeafLanes LL = leaf_lanes(T.n_bodies);
+    const int tpt = LL.t, lane_k = LL.lane_k;
+    for (int64_t c0 = 0; c0 < T.n_bodies; c0 += LL.per_pass) {
+        const int64_t ti = leaf_unit(LL, c0);
+        const bool active = LL.gl < LL.gpw && ti < T.n_bodies;
         const int64_t it = T.first + (active ? ti : 0);
         const double xt = tx[it], yt = ty[it], zt = tz[it];
         double a[13];
@@ -527,9 +562,8 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
             if (active)
                 for (int s = lane_k; s < cnt; s += tpt) panel_pair<NV, DIPOLE, PS, GS, HS>(tile[s], xt, yt, zt, a);
         }
-        for (int w = 1; w < tpt; w <<= 1)
 #pragma unroll
-            for (int k = 0; k < 13; ++k) a[k] += __shfl_xor(a[k], w, 64);
+        for (int k = 0; k < 13; ++k) a[k] = group_fold(a[k], LL);
         if (active && lane_k == 0) {
             if (PS) tout[it] += a[0];
             if (GS) { tout[nt + it] += a[1]; tout[2 * nt + it] += a[2]; tout[3 * nt + it] += a[3]; }
