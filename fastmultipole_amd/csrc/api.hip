@@ -127,6 +127,8 @@ static void release_context(Context& c) {
     c.tables = nullptr;
     c.tree_scratch = nullptr;
     c.list_scratch = nullptr;
+    if (c.pinned) (void)hipHostFree(c.pinned);
+    c.pinned = nullptr;
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
 }

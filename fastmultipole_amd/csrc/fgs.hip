@@ -75,14 +75,6 @@ static void excl_scan(Context& C, const T* in, O* out, int64_t n) {
     HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, O(0), (size_t)n, rocprim::plus<O>(), C.stream));
 }
 
-template <class T>
-static T read_back(Context& C, const T* dptr) {
-    T v;
-    HIP_CHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, C.stream));
-    HIP_CHECK(hipStreamSynchronize(C.stream));
-    return v;
-}
-
 // ---------------------------------------------------------------- set-up
 __global__ void k_fgs_leaf_sizes(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index, int64_t nl,
                                  int64_t* __restrict__ n, int64_t* __restrict__ n2) {

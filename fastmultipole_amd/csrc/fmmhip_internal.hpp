@@ -115,7 +115,30 @@ struct Context {
     DBuf<int32_t> m2l_flags;    // pmax and success of the last M2L pass
     DBuf<double> l2p_coef;      // per-leaf local-to-body coefficients of the last L2P pass
     DBuf<unsigned char> temp;   // scratch for sorts/scans
+    void* pinned = nullptr;     // 64 bytes of pinned host memory for scalar read-backs
 };
+
+// Device scalars read back through the context's pinned buffer, one stream synchronisation per call
+inline void* pinned_scratch(Context& C) {
+    if (!C.pinned) HIP_CHECK(hipHostMalloc(&C.pinned, 64, hipHostMallocDefault));
+    return C.pinned;
+}
+template <class T>
+inline T read_back(Context& C, const T* d) {
+    T* h = (T*)pinned_scratch(C);
+    HIP_CHECK(hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, C.stream));
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+    return *h;
+}
+// a + b of two device scalars (an exclusive scan's last entry plus its last flag)
+template <class T>
+inline int64_t read_back_sum(Context& C, const T* a, const T* b) {
+    T* h = (T*)pinned_scratch(C);
+    HIP_CHECK(hipMemcpyAsync(h, a, sizeof(T), hipMemcpyDeviceToHost, C.stream));
+    HIP_CHECK(hipMemcpyAsync(h + 1, b, sizeof(T), hipMemcpyDeviceToHost, C.stream));
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+    return (int64_t)h[0] + (int64_t)h[1];
+}
 void free_tree_scratch(TreeScratch* s);
 void free_list_scratch(ListScratch* s);
 

@@ -156,14 +156,6 @@ static void excl_scan(Context& C, const T* in, O* out, int64_t n) {
     HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, O(0), (size_t)n, rocprim::plus<O>(), C.stream));
 }
 
-template <class T>
-static T read_back(Context& C, const T* dptr) {
-    T v;
-    HIP_CHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, C.stream));
-    HIP_CHECK(hipStreamSynchronize(C.stream));
-    return v;
-}
-
 // number of body-body pairs of the direct list (nearfield work): grid-stride
 // partial sums, one atomic per workgroup
 __global__ __launch_bounds__(256) void k_count_pairs(const int32_t* __restrict__ t, const int32_t* __restrict__ s, int64_t n,
@@ -242,7 +234,7 @@ static void finish_list(Context& C, ListScratch& S, uint32_t* kt, int32_t* vs, i
     k_target_starts<<<grid_for(n_out, 256), 256, 0, C.stream>>>(S.kt_sorted.data(), n_out, S.flag.data());
     LAUNCH_CHECK();
     excl_scan(C, S.flag.data(), S.rank.data(), n_out);
-    n_targets = (int64_t)read_back(C, S.rank.data() + (n_out - 1)) + read_back(C, S.flag.data() + (n_out - 1));
+    n_targets = read_back_sum(C, S.rank.data() + (n_out - 1), S.flag.data() + (n_out - 1));
     targets.resize(n_targets);
     offsets.resize(n_targets + 1);
     k_csr<<<grid_for(n_out, 256), 256, 0, C.stream>>>(S.kt_sorted.data(), S.flag.data(), S.rank.data(), n_out, targets.data(),

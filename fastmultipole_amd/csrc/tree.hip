@@ -345,14 +345,6 @@ static void sort_keys_u64(Context& C, const uint64_t* kin, uint64_t* kout, int64
     HIP_CHECK(rocprim::radix_sort_keys(C.temp.data(), bytes, kin, kout, (size_t)n, 0, 64, C.stream));
 }
 
-template <class T>
-static T read_back(Context& C, const T* dptr) {
-    T v;
-    HIP_CHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, C.stream));
-    HIP_CHECK(hipStreamSynchronize(C.stream));
-    return v;
-}
-
 struct TreeScratch {
     DBuf<uint64_t> khi, klo, khi_s, klo_s, key2, key2_s;
     DBuf<uint32_t> idx, idx_s;
@@ -400,9 +392,7 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, double center[3
                                                         S.act_count.data(), S.sub_first.data(), S.sub_count.data(), S.nonempty.data());
         LAUNCH_CHECK();
         exclusive_scan_i32(C, S.nonempty.data(), S.rank.data(), m);
-        int32_t last_rank = read_back(C, S.rank.data() + (m - 1));
-        int32_t last_ne = read_back(C, S.nonempty.data() + (m - 1));
-        const int64_t n_children = (int64_t)last_rank + last_ne;
+        const int64_t n_children = read_back_sum(C, S.rank.data() + (m - 1), S.nonempty.data() + (m - 1));
         const int64_t child_base = n_branches;
         if ((size_t)(n_branches + n_children) > T.branches.cap) {
             DBuf<Branch> bigger;
@@ -420,9 +410,7 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, double center[3
                                                                       S.rank.data(), child_base, T.branches.data());
         LAUNCH_CHECK();
         exclusive_scan_i32(C, S.split.data(), S.srank.data(), n_children);
-        int32_t ls_rank = read_back(C, S.srank.data() + (n_children - 1));
-        int32_t ls_flag = read_back(C, S.split.data() + (n_children - 1));
-        const int64_t next_active = (int64_t)ls_rank + ls_flag;
+        const int64_t next_active = read_back_sum(C, S.srank.data() + (n_children - 1), S.split.data() + (n_children - 1));
         S.act_branch.resize(std::max<int64_t>(1, next_active));
         S.act_first.resize(std::max<int64_t>(1, next_active));
         S.act_count.resize(std::max<int64_t>(1, next_active));
@@ -503,7 +491,7 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     k_leaf_flags<<<grid_for(nb, 256), 256, 0, C.stream>>>(T.branches.data(), nb, S.flag.data());
     LAUNCH_CHECK();
     exclusive_scan_i32(C, S.flag.data(), S.frank.data(), nb);
-    T.n_leaves = (int64_t)read_back(C, S.frank.data() + (nb - 1)) + read_back(C, S.flag.data() + (nb - 1));
+    T.n_leaves = read_back_sum(C, S.frank.data() + (nb - 1), S.flag.data() + (nb - 1));
     T.leaf_index.resize(T.n_leaves);
     k_leaf_number<<<grid_for(nb, 256), 256, 0, C.stream>>>(T.branches.data(), nb, S.flag.data(), S.frank.data(), T.leaf_index.data());
     LAUNCH_CHECK();
