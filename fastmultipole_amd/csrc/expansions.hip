@@ -329,6 +329,30 @@ static int translate_waves(int P, int max_nw = 8) {
     throw Error(FMMHIP_EINVAL, "expansion order too large for the translation kernels");
 }
 
+// wavefronts per workgroup of the wave M2L: as many wavefronts per CU as the LDS holds (at most 12,
+// the register limit), where two or more resident workgroups count 1.3 times, since a workgroup that
+// waits for its slowest wavefront then leaves the CU to another (measured on the 1M ring at an order
+// cap of 14: 2 x 4 wavefronts 55.0 ms, 1 x 10 wavefronts 58.9 ms; at 20: 1 x 6 beats 2 x 2).
+// Even wavefront counts only.
+static int m2l_waves(int P) {
+    int best = 0;
+    double best_eff = 0.0;
+    for (int nw : {12, 10, 8, 6, 4, 2}) {
+        const size_t bytes = translate_lds_bytes(P, nw);
+        if (bytes > LDS_LIMIT) continue;
+        // resident workgroups, with a 4 KB margin (two 5-wavefront workgroups of 79 KB at an order
+        // cap of 14 measured as one)
+        const int wg = std::max(1, (int)((LDS_LIMIT - 4096) / bytes));
+        const double eff = std::min(12, nw * wg) * (wg >= 2 ? 1.3 : 1.0);
+        if (eff > best_eff) {
+            best_eff = eff;
+            best = nw;
+        }
+    }
+    if (!best) throw Error(FMMHIP_EINVAL, "expansion order too large for the translation kernels");
+    return best;
+}
+
 // launch set-up shared by the translation kernels: order check and the LDS limit
 template <class K>
 static size_t translate_setup(K kernel, int P, int nw) {
@@ -1186,9 +1210,14 @@ static void launch_m2l(Context& C, int p, int pe, int mode, const Lists& Ls, con
 template <bool LH>
 static void run_m2l_wave(Context& C, int p, int pe, int mode, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
                          int32_t* flags, const TablePtrs& tb) {
-    switch (translate_waves(p, 12)) {
+    const int nw = m2l_waves(p);
+    switch (nw) {
+        case 2: launch_m2l<LH, 2>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        case 3: launch_m2l<LH, 3>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
         case 12: launch_m2l<LH, 12>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
         case 10: launch_m2l<LH, 10>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        case 5: launch_m2l<LH, 5>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
+        case 4: launch_m2l<LH, 4>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
         case 8: launch_m2l<LH, 8>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
         case 6: launch_m2l<LH, 6>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
         default: launch_m2l<LH, 4>(C, p, pe, mode, Ls, st, tt_dst, E, flags, tb); break;
