@@ -57,6 +57,10 @@ template <int K> struct KernelTraits {
     static constexpr int NOUT = K == FMMHIP_POINT_SOURCE ? 4 : 13;
     template <class R> static constexpr bool defer() { return K == FMMHIP_POINT_VORTEX && sizeof(R) == 8; }
     template <class R> static constexpr int na() { return defer<R>() ? 16 : NOUT; }
+    // accumulators a launch with these outputs writes (the others stay zero and are not folded)
+    template <class R, bool PS, bool GS, bool HS> static constexpr bool used(int k) {
+        return k == 0 ? (PS && POT) : k < 4 ? GS : k < 13 ? (HS && HESS) : (HS && defer<R>());
+    }
 };
 
 template <class R> __device__ inline R erf_(R x);
@@ -280,7 +284,8 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
         }
         // fold the partial sums of a target pair's lanes
 #pragma unroll
-        for (int k = 0; k < NA; ++k) { a[k] = group_fold(a[k], LL); b[k] = group_fold(b[k], LL); }
+        for (int k = 0; k < NA; ++k)
+            if (KernelTraits<KERNEL>::template used<double, PS, GS, HS>(k)) { a[k] = group_fold(a[k], LL); b[k] = group_fold(b[k], LL); }
         fold_acc<KERNEL, HS, double>(a);
         fold_acc<KERNEL, HS, double>(b);
         if (lane_k == 0) {
@@ -387,7 +392,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
 #pragma unroll
             for (int k = 0; k < NA; ++k) acc[h][k] = (double)sum[h][k] - (double)cmp[h][k];
 #pragma unroll
-        for (int k = 0; k < NA; ++k) { acc[0][k] = group_fold(acc[0][k], LL); acc[1][k] = group_fold(acc[1][k], LL); }
+        for (int k = 0; k < NA; ++k)
+            if (KernelTraits<KERNEL>::template used<float, PS, GS, HS>(k)) {
+                acc[0][k] = group_fold(acc[0][k], LL);
+                acc[1][k] = group_fold(acc[1][k], LL);
+            }
         if (lane_k == 0) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
