@@ -114,14 +114,17 @@ __device__ inline void pair_eval(R dx, R dy, R dz, R r2, R ri, const R* g, R* a)
         const R cx = dz * gy - dy * gz, cy = dx * gz - dz * gx, cz = dy * gx - dx * gy;
         if (GS) { a[1] += gf * cx * d3; a[2] += gf * cy * d3; a[3] += gf * cz * d3; }
         if (HS) {
-            const R d5 = d3 * ri2, t3 = R(-3) * d5;
-            const R ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
+            const R d5 = d3 * ri2;
             if (KernelTraits<K>::template defer<R>()) {
+                // sum_j d5 d_i c_j; the factor -3 is applied once per target (fold_acc)
+                const R ax = d5 * dx, ay = d5 * dy, az = d5 * dz;
                 a[4] += ax * cx; a[5] += ax * cy; a[6] += ax * cz;
                 a[7] += ay * cx; a[8] += ay * cy; a[9] += ay * cz;
                 a[10] += az * cx; a[11] += az * cy; a[12] += az * cz;
                 a[13] += gx * d3; a[14] += gy * d3; a[15] += gz * d3;
             } else {
+                const R t3 = R(-3) * d5;
+                const R ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
                 const R g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
                 R h[9] = {ax * cx,       ax * cy + g3z, ax * cz - g3y,
                           ay * cx - g3z, ay * cy,       ay * cz + g3x,
@@ -140,12 +143,15 @@ __device__ inline void pair_eval(R dx, R dy, R dz, R r2, R ri, const R* g, R* a)
     }
 }
 
-// adds the deferred sums of the point vortex, S = sum_j Gamma_j / r_j^3, to the
-// Hessian slots: the per-pair terms were H(0,1) += S_z, H(0,2) -= S_y, H(1,0) -= S_z,
-// H(1,2) += S_x, H(2,0) += S_y, H(2,1) -= S_x
+// completes the deferred form of the point vortex: the symmetric sums were kept
+// without their factor -3, and S = sum_j Gamma_j / r_j^3 is added to the Hessian
+// slots as the per-pair terms H(0,1) += S_z, H(0,2) -= S_y, H(1,0) -= S_z,
+// H(1,2) += S_x, H(2,0) += S_y, H(2,1) -= S_x would have
 template <int K, bool HS, class R>
 __device__ inline void fold_acc(R* a) {
     if constexpr (KernelTraits<K>::template defer<R>() && HS) {
+#pragma unroll
+        for (int k = 4; k < 13; ++k) a[k] *= R(-3);
         a[5] += a[15]; a[6] -= a[14]; a[7] -= a[15];
         a[9] += a[13]; a[10] += a[14]; a[11] -= a[13];
     }
