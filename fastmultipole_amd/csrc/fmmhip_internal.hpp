@@ -114,6 +114,8 @@ struct Context {
     ListScratch* list_scratch = nullptr;
     DBuf<int32_t> m2l_flags;    // pmax and success of the last M2L pass
     DBuf<double> l2p_coef;      // per-leaf local-to-body coefficients of the last L2P pass
+    DBuf<unsigned long long> p2p_queue;   // work counter of the persistent nearfield launch
+    int n_cu = 0;               // compute units of the device (0 until first needed)
     DBuf<unsigned char> temp;   // scratch for sorts/scans
     void* pinned = nullptr;     // 64 bytes of pinned host memory for scalar read-backs
 };
@@ -186,8 +188,10 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
 void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t leaf_size_source, double mac,
                  int method, bool ff, bool nf, bool si, bool count_pairs);
 // p2p.hip
+// nearfield of the target leaves b0 .. b1 - 1 of the direct list (all by default)
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
-             bool PS, bool GS, bool HS, int precision);
+             bool PS, bool GS, bool HS, int precision,
+             int64_t b0 = 0, int64_t b1 = -1);
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
                     double* tout_aos, bool PS, bool GS, bool HS, int precision);
 // expansions.hip

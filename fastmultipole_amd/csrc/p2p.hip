@@ -22,6 +22,7 @@
 // FP32 partial sum added to its running total with Kahan compensation.
 // Strengths are pre-scaled by 1/(4 pi).
 #include "fmmhip_internal.hpp"
+#include <cstdlib>
 #include <algorithm>
 
 namespace fmmhip {
@@ -158,11 +159,13 @@ __device__ inline void fold_acc(R* a) {
 }
 
 // stages source j of the sorted arrays (strength SoA [dim][ns], radius) into slot values
+// g[0], g[stride], ...
 template <int K, class R>
-__device__ inline void stage_source(const double* __restrict__ sq, const double* __restrict__ srad, int64_t ns, int64_t j, R* g) {
+__device__ inline void stage_source(const double* __restrict__ sq, const double* __restrict__ srad, int64_t ns, int64_t j, R* g,
+                                    int stride = 1) {
     g[0] = (R)(sq[j] * INV4PI);
-    if (KernelTraits<K>::NS >= 3) { g[1] = (R)(sq[ns + j] * INV4PI); g[2] = (R)(sq[2 * ns + j] * INV4PI); }
-    if (K == FMMHIP_VORTEX_GAUSSIAN) g[3] = (R)srad[j];
+    if (KernelTraits<K>::NS >= 3) { g[stride] = (R)(sq[ns + j] * INV4PI); g[2 * stride] = (R)(sq[2 * ns + j] * INV4PI); }
+    if (K == FMMHIP_VORTEX_GAUSSIAN) g[3 * stride] = (R)srad[j];
 }
 
 // Source body at position f of the flattened stream of a target leaf's source
@@ -226,6 +229,43 @@ __device__ inline double group_fold(double v, const LeafLanes& L) {
 }
 
 // ---------------------------------------------------------------- FP64 form
+// 1/r of a pair for the FP64 kernels, 0 for a coincident pair (r = 0 contributes nothing).
+// Every term of the vortex and dipole kernels and of the source kernel's gradient carries at
+// least 1/r^2, so there clearing the high word is enough: what is left is at most a denormal
+// whose square is 0, one select instead of two. The source potential (q/r) clears both words.
+template <bool FULL>
+__device__ inline double rinv_guarded(double r2) {
+    const double ri = rinv_f64(r2);
+    if (FULL) return r2 > 0.0 ? ri : 0.0;
+    const uint64_t bits = (uint64_t)__double_as_longlong(ri);
+    const uint32_t hi = r2 > 0.0 ? (uint32_t)(bits >> 32) : 0u;
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | (bits & 0xffffffffull)));
+}
+
+// One source of an LDS tile against a lane's two targets. A tile holds rows [x, y, z, g...]
+// of P2P_BLOCK doubles, so the reads of a source are one address with constant offsets.
+template <int KERNEL, bool PS, bool GS, bool HS>
+__device__ inline void p2p_source_f64(const double* __restrict__ q, double xa, double ya, double za, double xb, double yb,
+                                      double zb, double* a, double* b) {
+    constexpr int NS = KernelTraits<KERNEL>::NS;
+    const double sxs = q[0], sys = q[P2P_BLOCK], szs = q[2 * P2P_BLOCK];
+    double gs[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) gs[k] = q[(3 + k) * P2P_BLOCK];
+    const double dxa = xa - sxs, dya = ya - sys, dza = za - szs;
+    const double r2a = dxa * dxa + dya * dya + dza * dza;
+    const double dxb = xb - sxs, dyb = yb - sys, dzb = zb - szs;
+    const double r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+    if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
+        if (r2a > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, rinv_f64(r2a), gs, a);
+        if (r2b > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rinv_f64(r2b), gs, b);
+    } else {
+        constexpr bool FULL = KERNEL == FMMHIP_POINT_SOURCE && PS;
+        pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, rinv_guarded<FULL>(r2a), gs, a);
+        pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rinv_guarded<FULL>(r2b), gs, b);
+    }
+}
+
 // Each thread evaluates two adjacent targets of the leaf against its share of
 // the staged sources, so a source read from LDS serves two pair evaluations.
 template <int KERNEL, bool PS, bool GS, bool HS>
@@ -238,74 +278,81 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
                                                        const double* __restrict__ sx, const double* __restrict__ sy,
                                                        const double* __restrict__ sz, const double* __restrict__ sq,
                                                        const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
-                                                       int64_t nt) {
+                                                       int64_t nt, int64_t blk0,
+                                                       unsigned long long* __restrict__ queue, int64_t n_queue) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
     constexpr int NA = KernelTraits<KERNEL>::template na<double>(), NOUT = KernelTraits<KERNEL>::NOUT;
-    __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK], lg[P2P_BLOCK][NS];
-    const Branch T = tbr[dir_targets[blockIdx.x]];
-    const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
-    const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
+    __shared__ double tile[3 + NS][P2P_BLOCK];   // rows x, y, z, staged source values
+    // one target leaf per workgroup; with a queue (persistent launch) each workgroup takes leaves
+    // from it until none are left
+    __shared__ int64_t s_next;
     const int tid = threadIdx.x;
-    const int64_t ngroups = (T.n_bodies + 1) / 2;           // target pairs
-    const LeafLanes LL = leaf_lanes(ngroups);
-    const int tpt = LL.t, lane_k = LL.lane_k;
-    for (int64_t c0 = 0; c0 < ngroups; c0 += LL.per_pass) {
-        const int64_t g = leaf_unit(LL, c0);
-        const bool own = LL.gl < LL.gpw;
-        const bool act0 = own && 2 * g < T.n_bodies, act1 = own && 2 * g + 1 < T.n_bodies;
-        const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
-        const double xa = tx[i0], ya = ty[i0], za = tz[i0];
-        const double xb = tx[i1], yb = ty[i1], zb = tz[i1];
-        double a[NA], b[NA];
-#pragma unroll
-        for (int k = 0; k < NA; ++k) { a[k] = 0.0; b[k] = 0.0; }
-        for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
-            const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
+    int64_t bi = blk0 + blockIdx.x;
+    for (;;) {
+        if (queue) {
+            __syncthreads();   // the previous leaf is done with the LDS tiles and s_next
+            if (tid == 0) s_next = (int64_t)atomicAdd(queue, 1ull);
             __syncthreads();
-            if (tid < cnt) {
-                const int64_t j = stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid);
-                lx[tid] = sx[j]; ly[tid] = sy[j]; lz[tid] = sz[j];
-                stage_source<KERNEL, double>(sq, srad, ns, j, lg[tid]);
-            }
-            __syncthreads();
-            if (act0)
-                for (int s = lane_k; s < cnt; s += tpt) {
-                    const double sxs = lx[s], sys = ly[s], szs = lz[s];
-                    double gs[NS];
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) gs[k] = lg[s][k];
-                    const double dxa = xa - sxs, dya = ya - sys, dza = za - szs;
-                    const double r2a = dxa * dxa + dya * dya + dza * dza;
-                    const double dxb = xb - sxs, dyb = yb - sys, dzb = zb - szs;
-                    const double r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                    if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
-                        if (r2a > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, rinv_f64(r2a), gs, a);
-                        if (r2b > 0.0) pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rinv_f64(r2b), gs, b);
-                    } else {   // a coincident pair (r = 0) contributes nothing: 1/r is selected to 0
-                        const double ria = r2a > 0.0 ? rinv_f64(r2a) : 0.0, rib = r2b > 0.0 ? rinv_f64(r2b) : 0.0;
-                        pair_eval<KERNEL, PS, GS, HS, double>(dxa, dya, dza, r2a, ria, gs, a);
-                        pair_eval<KERNEL, PS, GS, HS, double>(dxb, dyb, dzb, r2b, rib, gs, b);
-                    }
+            const int64_t nx = s_next;
+            const int64_t next = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(nx >> 32)) << 32) |
+                                           (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)nx));
+            if (next >= n_queue) break;
+            bi = blk0 + next;
+        }
+        const Branch T = tbr[dir_targets[bi]];
+        const int64_t l0 = dir_offsets[bi], l1 = dir_offsets[bi + 1];
+        const int64_t base = dir_cum[l0], end = dir_cum[l1];   // the target leaf's source bodies as one stream
+        const int64_t ngroups = (T.n_bodies + 1) / 2;           // target pairs
+        const LeafLanes LL = leaf_lanes(ngroups);
+        const int tpt = LL.t, lane_k = LL.lane_k;
+        for (int64_t c0 = 0; c0 < ngroups; c0 += LL.per_pass) {
+            const int64_t g = leaf_unit(LL, c0);
+            const bool own = LL.gl < LL.gpw;
+            const bool act0 = own && 2 * g < T.n_bodies, act1 = own && 2 * g + 1 < T.n_bodies;
+            const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
+            const double xa = tx[i0], ya = ty[i0], za = tz[i0];
+            const double xb = tx[i1], yb = ty[i1], zb = tz[i1];
+            double a[NA], b[NA];
+    #pragma unroll
+            for (int k = 0; k < NA; ++k) { a[k] = 0.0; b[k] = 0.0; }
+            for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
+                const int cnt = (int)std::min<int64_t>(P2P_BLOCK, end - f0);
+                __syncthreads();
+                if (tid < cnt) {
+                    const int64_t j = stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid);
+                    tile[0][tid] = sx[j]; tile[1][tid] = sy[j]; tile[2][tid] = sz[j];
+                    stage_source<KERNEL, double>(sq, srad, ns, j, &tile[3][tid], P2P_BLOCK);
                 }
-        }
-        // fold the partial sums of a target pair's lanes
-#pragma unroll
-        for (int k = 0; k < NA; ++k)
-            if (KernelTraits<KERNEL>::template used<double, PS, GS, HS>(k)) { a[k] = group_fold(a[k], LL); b[k] = group_fold(b[k], LL); }
-        fold_acc<KERNEL, HS, double>(a);
-        fold_acc<KERNEL, HS, double>(b);
-        if (lane_k == 0) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (!(h ? act1 : act0)) continue;
-                const int64_t it = h ? i1 : i0;
-                const double* r = h ? b : a;
-                if (PS && KernelTraits<KERNEL>::POT) tout[it] += r[0];
-                if (GS) { tout[nt + it] += r[1]; tout[2 * nt + it] += r[2]; tout[3 * nt + it] += r[3]; }
-                if (HS && KernelTraits<KERNEL>::HESS)
-                    for (int k = 4; k < NOUT; ++k) tout[(int64_t)k * nt + it] += r[k];
+                __syncthreads();
+                if (act0) {
+                    // the lane takes sources lane_k, lane_k + t, ... (t lanes per target pair) in
+                    // uniform rounds, so only the LDS address is per-lane loop work
+                    const int rounds = cnt / tpt;
+                    const double* p = &tile[0][lane_k];
+                    for (int r = 0; r < rounds; ++r, p += tpt) p2p_source_f64<KERNEL, PS, GS, HS>(p, xa, ya, za, xb, yb, zb, a, b);
+                    if (lane_k < cnt - rounds * tpt) p2p_source_f64<KERNEL, PS, GS, HS>(p, xa, ya, za, xb, yb, zb, a, b);
+                }
+            }
+            // fold the partial sums of a target pair's lanes
+    #pragma unroll
+            for (int k = 0; k < NA; ++k)
+                if (KernelTraits<KERNEL>::template used<double, PS, GS, HS>(k)) { a[k] = group_fold(a[k], LL); b[k] = group_fold(b[k], LL); }
+            fold_acc<KERNEL, HS, double>(a);
+            fold_acc<KERNEL, HS, double>(b);
+            if (lane_k == 0) {
+    #pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!(h ? act1 : act0)) continue;
+                    const int64_t it = h ? i1 : i0;
+                    const double* r = h ? b : a;
+                    if (PS && KernelTraits<KERNEL>::POT) tout[it] += r[0];
+                    if (GS) { tout[nt + it] += r[1]; tout[2 * nt + it] += r[2]; tout[3 * nt + it] += r[3]; }
+                    if (HS && KernelTraits<KERNEL>::HESS)
+                        for (int k = 4; k < NOUT; ++k) tout[(int64_t)k * nt + it] += r[k];
+                }
             }
         }
+        if (!queue) break;
     }
 }
 
@@ -321,124 +368,158 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                                                        const double* __restrict__ sx, const double* __restrict__ sy,
                                                        const double* __restrict__ sz, const double* __restrict__ sq,
                                                        const double* __restrict__ srad, int64_t ns, double* __restrict__ tout,
-                                                       int64_t nt) {
+                                                       int64_t nt, int64_t blk0,
+                                                       unsigned long long* __restrict__ queue, int64_t n_queue) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
     constexpr int NA = KernelTraits<KERNEL>::template na<float>(), NOUT = KernelTraits<KERNEL>::NOUT;
     __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[P2P_BLOCK][NS];
-    const Branch T = tbr[dir_targets[blockIdx.x]];
-    const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
-    const int64_t base = dir_cum[l0], end = dir_cum[l1];
-    const int tid = threadIdx.x;
-    const double cx0 = T.target_center[0], cy0 = T.target_center[1], cz0 = T.target_center[2];
-    const int64_t ngroups = (T.n_bodies + 1) / 2;
-    const LeafLanes LL = leaf_lanes(ngroups);
-    const int tpt = LL.t, lane_k = LL.lane_k;
-    for (int64_t c0 = 0; c0 < ngroups; c0 += LL.per_pass) {
-        const int64_t g = leaf_unit(LL, c0);
-        const bool own = LL.gl < LL.gpw;
-        const bool act0 = own && 2 * g < T.n_bodies, act1 = own && 2 * g + 1 < T.n_bodies;
-        const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
-        float th[2][3], tl[2][3];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int64_t it = h ? i1 : i0;
-            const double r[3] = {tx[it] - cx0, ty[it] - cy0, tz[it] - cz0};
-#pragma unroll
-            for (int d = 0; d < 3; ++d) { th[h][d] = (float)r[d]; tl[h][d] = (float)(r[d] - (double)th[h][d]); }
-        }
-        float sum[2][NA], cmp[2][NA];
-#pragma unroll
-        for (int k = 0; k < NA; ++k) { sum[0][k] = sum[1][k] = 0.f; cmp[0][k] = cmp[1][k] = 0.f; }
-        for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
-            const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
+    // one target leaf per workgroup; with a queue (persistent launch) each workgroup takes leaves
+    // from it until none are left
+    __shared__ int64_t s_next;
+    int64_t bi = blk0 + blockIdx.x;
+    for (;;) {
+        if (queue) {
+            __syncthreads();   // the previous leaf is done with the LDS tiles and s_next
+            if (threadIdx.x == 0) s_next = (int64_t)atomicAdd(queue, 1ull);
             __syncthreads();
-            if (tid < cnt) {
-                const int64_t li = stream_entry(dir_cum, l0, l1, f0 + tid);
-                const Branch& S = sbr[dir_s[li]];
-                const int64_t j = S.first + (f0 + tid - dir_cum[li]);
-                const double px = sx[j] - cx0, py = sy[j] - cy0, pz = sz[j] - cz0;
-                const float hx = (float)px, hy = (float)py, hz = (float)pz;
-                lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
-                ll[0][tid] = (float)(px - (double)hx); ll[1][tid] = (float)(py - (double)hy); ll[2][tid] = (float)(pz - (double)hz);
-                stage_source<KERNEL, float>(sq, srad, ns, j, lg[tid]);
-            }
-            __syncthreads();
-            if (act0) {
-                float pa[NA], pb[NA];
-#pragma unroll
-                for (int k = 0; k < NA; ++k) { pa[k] = 0.f; pb[k] = 0.f; }
-                for (int s = lane_k; s < cnt; s += tpt) {
-                    const float shx = lh[0][s], shy = lh[1][s], shz = lh[2][s];
-                    const float slx = ll[0][s], sly = ll[1][s], slz = ll[2][s];
-                    float gs[NS];
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) gs[k] = lg[s][k];
-                    const float dxa = (th[0][0] - shx) + (tl[0][0] - slx), dya = (th[0][1] - shy) + (tl[0][1] - sly),
-                                dza = (th[0][2] - shz) + (tl[0][2] - slz);
-                    const float dxb = (th[1][0] - shx) + (tl[1][0] - slx), dyb = (th[1][1] - shy) + (tl[1][1] - sly),
-                                dzb = (th[1][2] - shz) + (tl[1][2] - slz);
-                    const float r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                    if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
-                        if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
-                        if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
-                    } else {
-                        const float ria = r2a > 0.f ? __builtin_amdgcn_rsqf(r2a) : 0.f;
-                        const float rib = r2b > 0.f ? __builtin_amdgcn_rsqf(r2b) : 0.f;
-                        pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, ria, gs, pa);
-                        pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, rib, gs, pb);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < NA; ++k) { kahan_add(sum[0][k], cmp[0][k], pa[k]); kahan_add(sum[1][k], cmp[1][k], pb[k]); }
-            }
+            if (s_next >= n_queue) break;
+            bi = blk0 + s_next;
         }
-        double acc[2][NA];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int k = 0; k < NA; ++k) acc[h][k] = (double)sum[h][k] - (double)cmp[h][k];
-#pragma unroll
-        for (int k = 0; k < NA; ++k)
-            if (KernelTraits<KERNEL>::template used<float, PS, GS, HS>(k)) {
-                acc[0][k] = group_fold(acc[0][k], LL);
-                acc[1][k] = group_fold(acc[1][k], LL);
-            }
-        if (lane_k == 0) {
-#pragma unroll
+        const Branch T = tbr[dir_targets[bi]];
+        const int64_t l0 = dir_offsets[bi], l1 = dir_offsets[bi + 1];
+        const int64_t base = dir_cum[l0], end = dir_cum[l1];
+        const int tid = threadIdx.x;
+        const double cx0 = T.target_center[0], cy0 = T.target_center[1], cz0 = T.target_center[2];
+        const int64_t ngroups = (T.n_bodies + 1) / 2;
+        const LeafLanes LL = leaf_lanes(ngroups);
+        const int tpt = LL.t, lane_k = LL.lane_k;
+        for (int64_t c0 = 0; c0 < ngroups; c0 += LL.per_pass) {
+            const int64_t g = leaf_unit(LL, c0);
+            const bool own = LL.gl < LL.gpw;
+            const bool act0 = own && 2 * g < T.n_bodies, act1 = own && 2 * g + 1 < T.n_bodies;
+            const int64_t i0 = T.first + (act0 ? 2 * g : 0), i1 = act1 ? i0 + 1 : i0;
+            float th[2][3], tl[2][3];
+    #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                if (!(h ? act1 : act0)) continue;
                 const int64_t it = h ? i1 : i0;
-                if (PS && KernelTraits<KERNEL>::POT) tout[it] += acc[h][0];
-                if (GS) { tout[nt + it] += acc[h][1]; tout[2 * nt + it] += acc[h][2]; tout[3 * nt + it] += acc[h][3]; }
-                if (HS && KernelTraits<KERNEL>::HESS)
-                    for (int k = 4; k < NOUT; ++k) tout[(int64_t)k * nt + it] += acc[h][k];
+                const double r[3] = {tx[it] - cx0, ty[it] - cy0, tz[it] - cz0};
+    #pragma unroll
+                for (int d = 0; d < 3; ++d) { th[h][d] = (float)r[d]; tl[h][d] = (float)(r[d] - (double)th[h][d]); }
+            }
+            float sum[2][NA], cmp[2][NA];
+    #pragma unroll
+            for (int k = 0; k < NA; ++k) { sum[0][k] = sum[1][k] = 0.f; cmp[0][k] = cmp[1][k] = 0.f; }
+            for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
+                const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
+                __syncthreads();
+                if (tid < cnt) {
+                    const int64_t li = stream_entry(dir_cum, l0, l1, f0 + tid);
+                    const Branch& S = sbr[dir_s[li]];
+                    const int64_t j = S.first + (f0 + tid - dir_cum[li]);
+                    const double px = sx[j] - cx0, py = sy[j] - cy0, pz = sz[j] - cz0;
+                    const float hx = (float)px, hy = (float)py, hz = (float)pz;
+                    lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
+                    ll[0][tid] = (float)(px - (double)hx); ll[1][tid] = (float)(py - (double)hy); ll[2][tid] = (float)(pz - (double)hz);
+                    stage_source<KERNEL, float>(sq, srad, ns, j, lg[tid]);
+                }
+                __syncthreads();
+                if (act0) {
+                    float pa[NA], pb[NA];
+    #pragma unroll
+                    for (int k = 0; k < NA; ++k) { pa[k] = 0.f; pb[k] = 0.f; }
+                    for (int s = lane_k; s < cnt; s += tpt) {
+                        const float shx = lh[0][s], shy = lh[1][s], shz = lh[2][s];
+                        const float slx = ll[0][s], sly = ll[1][s], slz = ll[2][s];
+                        float gs[NS];
+    #pragma unroll
+                        for (int k = 0; k < NS; ++k) gs[k] = lg[s][k];
+                        const float dxa = (th[0][0] - shx) + (tl[0][0] - slx), dya = (th[0][1] - shy) + (tl[0][1] - sly),
+                                    dza = (th[0][2] - shz) + (tl[0][2] - slz);
+                        const float dxb = (th[1][0] - shx) + (tl[1][0] - slx), dyb = (th[1][1] - shy) + (tl[1][1] - sly),
+                                    dzb = (th[1][2] - shz) + (tl[1][2] - slz);
+                        const float r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                        if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
+                            if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
+                            if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
+                        } else {
+                            const float ria = r2a > 0.f ? __builtin_amdgcn_rsqf(r2a) : 0.f;
+                            const float rib = r2b > 0.f ? __builtin_amdgcn_rsqf(r2b) : 0.f;
+                            pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, ria, gs, pa);
+                            pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, rib, gs, pb);
+                        }
+                    }
+    #pragma unroll
+                    for (int k = 0; k < NA; ++k) { kahan_add(sum[0][k], cmp[0][k], pa[k]); kahan_add(sum[1][k], cmp[1][k], pb[k]); }
+                }
+            }
+            double acc[2][NA];
+    #pragma unroll
+            for (int h = 0; h < 2; ++h)
+    #pragma unroll
+                for (int k = 0; k < NA; ++k) acc[h][k] = (double)sum[h][k] - (double)cmp[h][k];
+    #pragma unroll
+            for (int k = 0; k < NA; ++k)
+                if (KernelTraits<KERNEL>::template used<float, PS, GS, HS>(k)) {
+                    acc[0][k] = group_fold(acc[0][k], LL);
+                    acc[1][k] = group_fold(acc[1][k], LL);
+                }
+            if (lane_k == 0) {
+    #pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!(h ? act1 : act0)) continue;
+                    const int64_t it = h ? i1 : i0;
+                    if (PS && KernelTraits<KERNEL>::POT) tout[it] += acc[h][0];
+                    if (GS) { tout[nt + it] += acc[h][1]; tout[2 * nt + it] += acc[h][2]; tout[3 * nt + it] += acc[h][3]; }
+                    if (HS && KernelTraits<KERNEL>::HESS)
+                        for (int k = 4; k < NOUT; ++k) tout[(int64_t)k * nt + it] += acc[h][k];
+                }
             }
         }
+        if (!queue) break;
     }
+}
+
+__global__ void k_zero_queue(unsigned long long* q) { *q = 0ull; }
+
+// Persistent launch: as many workgroups as the device holds at once (from the kernel's own
+// register and LDS use), each taking target leaves from the queue in list order until it is
+// empty, so no workgroup waits for a slot behind the last ones and no slot idles at the end.
+// Every workgroup exits once the queue index passes the leaf count.
+template <class... P, class... A>
+static void launch_queued(Context& C, void (*kfn)(P...), int64_t nblk, A... args) {
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kfn), P2P_BLOCK, 0));
+    if (!C.n_cu) HIP_CHECK(hipDeviceGetAttribute(&C.n_cu, hipDeviceAttributeMultiprocessorCount, C.device));
+    const int64_t grid = std::min<int64_t>(nblk, (int64_t)std::max(1, per_cu) * C.n_cu);
+    kfn<<<(unsigned)grid, P2P_BLOCK, 0, C.stream>>>(args...);
 }
 
 template <int K>
 static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
-                       bool HS, int precision) {
-    const int64_t nblk = L.n_direct_targets;
-    if (nblk == 0) return;
+                       bool HS, int precision, int64_t b0, int64_t b1) {
+    const int64_t nblk = b1 - b0;
+    if (nblk <= 0) return;
+    C.p2p_queue.resize(1);
+    unsigned long long* queue = C.p2p_queue.data();
+    k_zero_queue<<<1, 1, 0, C.stream>>>(queue);
+    LAUNCH_CHECK();
 #define P2P_ARGS                                                                                                         \
     L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), L.dir_cum.data(), tt.branches.data(), st.branches.data(), \
         tt.x.data(),                                                                                                     \
         tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.radius.data(), st.n,    \
-        tout, nt
+        tout, nt, b0, queue, nblk
 #define P2P_DISPATCH(KFN)                                                                                               \
     do {                                                                                                                \
         int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);                                                          \
         switch (code) {                                                                                                 \
             case 0: break;                                                                                              \
-            case 1: KFN<K, false, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;             \
-            case 2: KFN<K, false, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;             \
-            case 3: KFN<K, false, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;              \
-            case 4: KFN<K, true, false, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;             \
-            case 5: KFN<K, true, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;              \
-            case 6: KFN<K, true, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;              \
-            default: KFN<K, true, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(P2P_ARGS); break;              \
+            case 1: launch_queued(C, KFN<K, false, false, true>, nblk, P2P_ARGS); break;                               \
+            case 2: launch_queued(C, KFN<K, false, true, false>, nblk, P2P_ARGS); break;                               \
+            case 3: launch_queued(C, KFN<K, false, true, true>, nblk, P2P_ARGS); break;                                \
+            case 4: launch_queued(C, KFN<K, true, false, false>, nblk, P2P_ARGS); break;                               \
+            case 5: launch_queued(C, KFN<K, true, false, true>, nblk, P2P_ARGS); break;                                \
+            case 6: launch_queued(C, KFN<K, true, true, false>, nblk, P2P_ARGS); break;                                \
+            default: launch_queued(C, KFN<K, true, true, true>, nblk, P2P_ARGS); break;                                \
         }                                                                                                               \
     } while (0)
     if (precision == FMMHIP_FP64) P2P_DISPATCH(k_p2p_f64);
@@ -449,18 +530,19 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
 }
 
 void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
-                    bool GS, bool HS);
+                    bool GS, bool HS, int64_t b0, int64_t b1);
 void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
                        double* tout_aos, bool PS, bool GS, bool HS);
 
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
-             bool HS, int precision) {
-    if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS); return; }
+             bool HS, int precision, int64_t b0, int64_t b1) {
+    if (b1 < 0) b1 = L.n_direct_targets;
+    if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS, b0, b1); return; }
     switch (kernel) {
-        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
-        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
-        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
-        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision); break;
+        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
+        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
+        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
+        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
     }
 }
 
@@ -553,10 +635,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
                                                          const double* __restrict__ tz, const double* __restrict__ sx,
                                                          const double* __restrict__ sy, const double* __restrict__ sz,
                                                          const double* __restrict__ rows, int64_t ns, double* __restrict__ tout,
-                                                         int64_t nt) {
+                                                         int64_t nt, int64_t blk0) {
     __shared__ PanelStage<NV> tile[PANEL_TILE];
-    const Branch T = tbr[dir_targets[blockIdx.x]];
-    const int64_t l0 = dir_offsets[blockIdx.x], l1 = dir_offsets[blockIdx.x + 1];
+    const int64_t bi = blk0 + blockIdx.x;   // target leaf of this workgroup
+    const Branch T = tbr[dir_targets[bi]];
+    const int64_t l0 = dir_offsets[bi], l1 = dir_offsets[bi + 1];
     const int64_t base = dir_cum[l0], end = dir_cum[l1];
     const int tid = threadIdx.x;
     const LeafLanes LL = leaf_lanes(T.n_bodies);
@@ -589,12 +672,12 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
 
 template <int NV, bool DIPOLE>
 static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
-                             bool GS, bool HS) {
-    const int64_t nblk = L.n_direct_targets;
-    if (nblk == 0) return;
+                             bool GS, bool HS, int64_t b0, int64_t b1) {
+    const int64_t nblk = b1 - b0;
+    if (nblk <= 0) return;
 #define PANEL_ARGS                                                                                                       \
     L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), L.dir_cum.data(), tt.branches.data(), st.branches.data(), \
-        tt.x.data(), tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.n, tout, nt
+        tt.x.data(), tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.n, tout, nt, b0
     const int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);
     switch (code) {
         case 0: break;
@@ -611,12 +694,12 @@ static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const L
 }
 
 void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
-                    bool GS, bool HS) {
+                    bool GS, bool HS, int64_t b0, int64_t b1) {
     switch (kernel) {
-        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, false>(C, tt, st, L, tout, nt, PS, GS, HS); break;
-        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, true>(C, tt, st, L, tout, nt, PS, GS, HS); break;
-        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, false>(C, tt, st, L, tout, nt, PS, GS, HS); break;
-        default: launch_p2p_panel<4, true>(C, tt, st, L, tout, nt, PS, GS, HS); break;
+        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
+        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
+        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
+        default: launch_p2p_panel<4, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
     }
 }
 
