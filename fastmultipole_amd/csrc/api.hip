@@ -186,7 +186,15 @@ static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
     eo.shrink_recenter = 1;
     eo.list_method = FMMHIP_SELFTUNING_TREESTOP;
     eo.farfield = eo.nearfield = eo.self_induced = 1;
-    eval_build_trees(X, eo);
+    if (X.t_xyz) eval_build_trees(X, eo);
+    else {   // the targets are the sources: one build gives both trees
+        build_tree(C, X.st, X.ns, X.s_buf, data_per_body(X.kernel), false, eo.leaf_size_source, eo.list_method, true,
+                   eo.expansion_order, strength_dims(X.kernel));
+        clone_target_tree(C, X.st, X.tt);
+        X.trees_built = true;
+        X.lists_built = false;
+        X.p_built = eo.expansion_order;
+    }
     eval_build_lists(X, eo, false);
     eval_core(X, eo, nullptr, true);
     DBuf<double>& infl = E.influence;
@@ -230,7 +238,12 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
             stream_wait(F, C);
         }
         Timer tp(C);
-        run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision);
+        // Beside the far-field stream the FP64 nearfield leaves one workgroup slot per CU free:
+        // its rate is the same at 3 and 4 workgroups per CU and the far field's kernels (the
+        // estimate's tree, list and sort launches) get room to run. The FP32 nearfield is
+        // slower with fewer workgroups and takes all.
+        const int slots = &F != &C && o.precision == FMMHIP_FP64 ? -1 : 0;
+        run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision, 0, -1, slots);
         tp.mark();
         if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) {
             Timer t(F);

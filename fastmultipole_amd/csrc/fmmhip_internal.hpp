@@ -184,14 +184,18 @@ inline int n_harmonics(int p) { return ((p + 1) * (p + 2)) / 2; }
 // tree.hip
 void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, bool target, int64_t leaf_size,
                 int method, bool shrink, int p, int strength_dims);
+// the target tree of the bodies of source tree s (same leaf size and options), by device copies
+void clone_target_tree(Context& C, const Tree& s, Tree& t);
 // lists.hip
 void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t leaf_size_source, double mac,
                  int method, bool ff, bool nf, bool si, bool count_pairs);
 // p2p.hip
-// nearfield of the target leaves b0 .. b1 - 1 of the direct list (all by default)
+// nearfield of the target leaves b0 .. b1 - 1 of the direct list (all by default). The point
+// kernels run as workgroups taking leaves from the context's queue: at most `slots` per CU
+// (0: as many as fit, < 0: that many fewer than fit).
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
              bool PS, bool GS, bool HS, int precision,
-             int64_t b0 = 0, int64_t b1 = -1);
+             int64_t b0 = 0, int64_t b1 = -1, int slots = 0);
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
                     double* tout_aos, bool PS, bool GS, bool HS, int precision);
 // expansions.hip

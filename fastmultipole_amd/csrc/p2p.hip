@@ -485,18 +485,21 @@ __global__ void k_zero_queue(unsigned long long* q) { *q = 0ull; }
 // register and LDS use), each taking target leaves from the queue in list order until it is
 // empty, so no workgroup waits for a slot behind the last ones and no slot idles at the end.
 // Every workgroup exits once the queue index passes the leaf count.
+// slots > 0: at most that many workgroups per CU; slots < 0: that many fewer than fit (at least
+// one); the rest of the CU stays free for kernels of other streams.
 template <class... P, class... A>
-static void launch_queued(Context& C, void (*kfn)(P...), int64_t nblk, A... args) {
+static void launch_queued(Context& C, int slots, void (*kfn)(P...), int64_t nblk, A... args) {
     int per_cu = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kfn), P2P_BLOCK, 0));
     if (!C.n_cu) HIP_CHECK(hipDeviceGetAttribute(&C.n_cu, hipDeviceAttributeMultiprocessorCount, C.device));
-    const int64_t grid = std::min<int64_t>(nblk, (int64_t)std::max(1, per_cu) * C.n_cu);
+    per_cu = std::max(1, slots > 0 ? std::min(per_cu, slots) : per_cu + slots);
+    const int64_t grid = std::min<int64_t>(nblk, (int64_t)per_cu * C.n_cu);
     kfn<<<(unsigned)grid, P2P_BLOCK, 0, C.stream>>>(args...);
 }
 
 template <int K>
 static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
-                       bool HS, int precision, int64_t b0, int64_t b1) {
+                       bool HS, int precision, int64_t b0, int64_t b1, int slots) {
     const int64_t nblk = b1 - b0;
     if (nblk <= 0) return;
     C.p2p_queue.resize(1);
@@ -513,13 +516,13 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
         int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);                                                          \
         switch (code) {                                                                                                 \
             case 0: break;                                                                                              \
-            case 1: launch_queued(C, KFN<K, false, false, true>, nblk, P2P_ARGS); break;                               \
-            case 2: launch_queued(C, KFN<K, false, true, false>, nblk, P2P_ARGS); break;                               \
-            case 3: launch_queued(C, KFN<K, false, true, true>, nblk, P2P_ARGS); break;                                \
-            case 4: launch_queued(C, KFN<K, true, false, false>, nblk, P2P_ARGS); break;                               \
-            case 5: launch_queued(C, KFN<K, true, false, true>, nblk, P2P_ARGS); break;                                \
-            case 6: launch_queued(C, KFN<K, true, true, false>, nblk, P2P_ARGS); break;                                \
-            default: launch_queued(C, KFN<K, true, true, true>, nblk, P2P_ARGS); break;                                \
+            case 1: launch_queued(C, slots, KFN<K, false, false, true>, nblk, P2P_ARGS); break;                               \
+            case 2: launch_queued(C, slots, KFN<K, false, true, false>, nblk, P2P_ARGS); break;                               \
+            case 3: launch_queued(C, slots, KFN<K, false, true, true>, nblk, P2P_ARGS); break;                                \
+            case 4: launch_queued(C, slots, KFN<K, true, false, false>, nblk, P2P_ARGS); break;                               \
+            case 5: launch_queued(C, slots, KFN<K, true, false, true>, nblk, P2P_ARGS); break;                                \
+            case 6: launch_queued(C, slots, KFN<K, true, true, false>, nblk, P2P_ARGS); break;                                \
+            default: launch_queued(C, slots, KFN<K, true, true, true>, nblk, P2P_ARGS); break;                                \
         }                                                                                                               \
     } while (0)
     if (precision == FMMHIP_FP64) P2P_DISPATCH(k_p2p_f64);
@@ -535,14 +538,14 @@ void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, 
                        double* tout_aos, bool PS, bool GS, bool HS);
 
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
-             bool HS, int precision, int64_t b0, int64_t b1) {
+             bool HS, int precision, int64_t b0, int64_t b1, int slots) {
     if (b1 < 0) b1 = L.n_direct_targets;
     if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS, b0, b1); return; }
     switch (kernel) {
-        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
-        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
-        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
-        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1); break;
+        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
+        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
+        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
+        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
     }
 }
 

@@ -522,4 +522,35 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     T.expansions.resize((size_t)nb * 4 * n_harmonics(p));
 }
 
+// Tree structure, body order, positions and the branches' target centres, boxes and radii
+// depend only on the positions, leaf size and options, not on the tree's role, so a target tree
+// of the same bodies is the source tree's copy (its source_* fields keep the source tree's
+// values, which a target tree does not read).
+void clone_target_tree(Context& C, const Tree& s, Tree& t) {
+    t.n = s.n;
+    t.rows = s.rows;
+    t.p = s.p;
+    t.strength_dims = 0;
+    t.n_branches = s.n_branches;
+    t.depth = s.depth;
+    t.level_first = s.level_first;
+    t.level_last = s.level_last;
+    t.leaf_index_h.clear();
+    t.n_leaves = s.n_leaves;
+    if (s.n == 0) return;
+    auto copy = [&](auto& dst, const auto& src, size_t count) {
+        dst.resize(count);
+        if (count)
+            HIP_CHECK(hipMemcpyAsync(dst.data(), src.data(), count * sizeof(*src.data()), hipMemcpyDeviceToDevice, C.stream));
+    };
+    copy(t.branches, s.branches, (size_t)s.n_branches);
+    copy(t.sort_index, s.sort_index, (size_t)s.n);
+    copy(t.inverse_sort, s.inverse_sort, (size_t)s.n);
+    copy(t.leaf_index, s.leaf_index, (size_t)s.n_leaves);
+    copy(t.x, s.x, (size_t)s.n);
+    copy(t.y, s.y, (size_t)s.n);
+    copy(t.z, s.z, (size_t)s.n);
+    t.expansions.resize(s.expansions.size());
+}
+
 }  // namespace fmmhip
