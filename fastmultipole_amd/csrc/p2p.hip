@@ -8,18 +8,19 @@
 //                  g(r/sigma) = erf(rho/sqrt 2) - sqrt(2/pi) rho exp(-rho^2/2)
 // driven by the direct list sorted by target (src/fmm.jl:18-40).
 //
-// One workgroup per target leaf. Targets live in registers (power-of-two
-// threads per target when the leaf is small, each thread taking a strided share
-// of the sources); each source leaf of the list is staged through LDS and
-// broadcast to the whole workgroup. The partial sums of the threads of one
-// target are folded with wavefront shuffles.
+// A workgroup evaluates one target leaf at a time, taking leaves from a queue
+// (persistent launch of as many workgroups as the device holds). Targets live in
+// registers, two per thread, with a group of t lanes per target pair when the
+// leaf is small, each lane taking a strided share of the sources; the leaf's
+// source leaves form one stream staged through LDS in 256-body tiles and
+// broadcast to the whole workgroup. The partial sums of a group are folded with
+// wavefront shuffles.
 //
 // FP64 form: IEEE double arithmetic with 1/r from a refined hardware rsqrt.
-// Mixed form: the source leaves whose bounding sphere overlaps the target
-// leaf's (the closest, largest contributions) in FP64; all other pairs in FP32
-// with positions stored relative to the target leaf centre as a hi+lo pair of
-// floats (differences keep ~2^-48 relative accuracy), each thread's per-tile
-// FP32 partial sum added to its running total with Kahan compensation.
+// Mixed form: every pair in FP32 with positions stored relative to the target
+// leaf centre as a hi+lo pair of floats (differences keep ~2^-48 relative
+// accuracy), each thread's per-tile FP32 partial sum added to its running total
+// with Kahan compensation.
 // Strengths are pre-scaled by 1/(4 pi).
 #include "fmmhip_internal.hpp"
 #include <cstdlib>
@@ -228,6 +229,12 @@ __device__ inline double group_fold(double v, const LeafLanes& L) {
     return v;
 }
 
+// a 64-bit value that is the same in every lane (read from LDS), as a wave-uniform value
+__device__ inline int64_t uniform_i64(int64_t v) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                     (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
 // ---------------------------------------------------------------- FP64 form
 // 1/r of a pair for the FP64 kernels, 0 for a coincident pair (r = 0 contributes nothing).
 // Every term of the vortex and dipole kernels and of the source kernel's gradient carries at
@@ -293,9 +300,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
             __syncthreads();   // the previous leaf is done with the LDS tiles and s_next
             if (tid == 0) s_next = (int64_t)atomicAdd(queue, 1ull);
             __syncthreads();
-            const int64_t nx = s_next;
-            const int64_t next = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(nx >> 32)) << 32) |
-                                           (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)nx));
+            const int64_t next = uniform_i64(s_next);
             if (next >= n_queue) break;
             bi = blk0 + next;
         }
@@ -357,6 +362,33 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f64(const int32_t* __restrict
 }
 
 // --------------------------------------------------------------- mixed form
+// One source of an LDS tile (rows hi x, y, z, lo x, y, z, staged values) against a lane's two
+// targets (hi th, lo tl relative to the leaf centre); the pair's partial sums go to pa, pb.
+template <int KERNEL, bool PS, bool GS, bool HS>
+__device__ inline void p2p_source_f32(const float* __restrict__ q, const float (&th)[2][3], const float (&tl)[2][3], float* pa,
+                                      float* pb) {
+    constexpr int NS = KernelTraits<KERNEL>::NS;
+    const float shx = q[0], shy = q[P2P_BLOCK], shz = q[2 * P2P_BLOCK];
+    const float slx = q[3 * P2P_BLOCK], sly = q[4 * P2P_BLOCK], slz = q[5 * P2P_BLOCK];
+    float gs[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) gs[k] = q[(6 + k) * P2P_BLOCK];
+    const float dxa = (th[0][0] - shx) + (tl[0][0] - slx), dya = (th[0][1] - shy) + (tl[0][1] - sly),
+                dza = (th[0][2] - shz) + (tl[0][2] - slz);
+    const float dxb = (th[1][0] - shx) + (tl[1][0] - slx), dyb = (th[1][1] - shy) + (tl[1][1] - sly),
+                dzb = (th[1][2] - shz) + (tl[1][2] - slz);
+    const float r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+    if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
+        if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
+        if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
+    } else {   // a coincident pair (r = 0) contributes nothing: 1/r is selected to 0
+        const float ria = r2a > 0.f ? __builtin_amdgcn_rsqf(r2a) : 0.f;
+        const float rib = r2b > 0.f ? __builtin_amdgcn_rsqf(r2b) : 0.f;
+        pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, ria, gs, pa);
+        pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, rib, gs, pb);
+    }
+}
+
 
 template <int KERNEL, bool PS, bool GS, bool HS>
 __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict__ dir_targets,
@@ -372,23 +404,25 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
                                                        unsigned long long* __restrict__ queue, int64_t n_queue) {
     constexpr int NS = KernelTraits<KERNEL>::NS;
     constexpr int NA = KernelTraits<KERNEL>::template na<float>(), NOUT = KernelTraits<KERNEL>::NOUT;
-    __shared__ float lh[3][P2P_BLOCK], ll[3][P2P_BLOCK], lg[P2P_BLOCK][NS];
+    // rows: source position hi (3) and lo (3) parts relative to the target leaf centre, staged values
+    __shared__ float tile[6 + NS][P2P_BLOCK];
     // one target leaf per workgroup; with a queue (persistent launch) each workgroup takes leaves
     // from it until none are left
     __shared__ int64_t s_next;
+    const int tid = threadIdx.x;
     int64_t bi = blk0 + blockIdx.x;
     for (;;) {
         if (queue) {
             __syncthreads();   // the previous leaf is done with the LDS tiles and s_next
-            if (threadIdx.x == 0) s_next = (int64_t)atomicAdd(queue, 1ull);
+            if (tid == 0) s_next = (int64_t)atomicAdd(queue, 1ull);
             __syncthreads();
-            if (s_next >= n_queue) break;
-            bi = blk0 + s_next;
+            const int64_t next = uniform_i64(s_next);
+            if (next >= n_queue) break;
+            bi = blk0 + next;
         }
         const Branch T = tbr[dir_targets[bi]];
         const int64_t l0 = dir_offsets[bi], l1 = dir_offsets[bi + 1];
         const int64_t base = dir_cum[l0], end = dir_cum[l1];
-        const int tid = threadIdx.x;
         const double cx0 = T.target_center[0], cy0 = T.target_center[1], cz0 = T.target_center[2];
         const int64_t ngroups = (T.n_bodies + 1) / 2;
         const LeafLanes LL = leaf_lanes(ngroups);
@@ -410,44 +444,27 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_f32(const int32_t* __restrict
     #pragma unroll
             for (int k = 0; k < NA; ++k) { sum[0][k] = sum[1][k] = 0.f; cmp[0][k] = cmp[1][k] = 0.f; }
             for (int64_t f0 = base; f0 < end; f0 += P2P_BLOCK) {
-                const int64_t cnt = std::min<int64_t>(P2P_BLOCK, end - f0);
+                const int cnt = (int)std::min<int64_t>(P2P_BLOCK, end - f0);
                 __syncthreads();
                 if (tid < cnt) {
-                    const int64_t li = stream_entry(dir_cum, l0, l1, f0 + tid);
-                    const Branch& S = sbr[dir_s[li]];
-                    const int64_t j = S.first + (f0 + tid - dir_cum[li]);
+                    const int64_t j = stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid);
                     const double px = sx[j] - cx0, py = sy[j] - cy0, pz = sz[j] - cz0;
                     const float hx = (float)px, hy = (float)py, hz = (float)pz;
-                    lh[0][tid] = hx; lh[1][tid] = hy; lh[2][tid] = hz;
-                    ll[0][tid] = (float)(px - (double)hx); ll[1][tid] = (float)(py - (double)hy); ll[2][tid] = (float)(pz - (double)hz);
-                    stage_source<KERNEL, float>(sq, srad, ns, j, lg[tid]);
+                    tile[0][tid] = hx; tile[1][tid] = hy; tile[2][tid] = hz;
+                    tile[3][tid] = (float)(px - (double)hx); tile[4][tid] = (float)(py - (double)hy);
+                    tile[5][tid] = (float)(pz - (double)hz);
+                    stage_source<KERNEL, float>(sq, srad, ns, j, &tile[6][tid], P2P_BLOCK);
                 }
                 __syncthreads();
                 if (act0) {
                     float pa[NA], pb[NA];
     #pragma unroll
                     for (int k = 0; k < NA; ++k) { pa[k] = 0.f; pb[k] = 0.f; }
-                    for (int s = lane_k; s < cnt; s += tpt) {
-                        const float shx = lh[0][s], shy = lh[1][s], shz = lh[2][s];
-                        const float slx = ll[0][s], sly = ll[1][s], slz = ll[2][s];
-                        float gs[NS];
-    #pragma unroll
-                        for (int k = 0; k < NS; ++k) gs[k] = lg[s][k];
-                        const float dxa = (th[0][0] - shx) + (tl[0][0] - slx), dya = (th[0][1] - shy) + (tl[0][1] - sly),
-                                    dza = (th[0][2] - shz) + (tl[0][2] - slz);
-                        const float dxb = (th[1][0] - shx) + (tl[1][0] - slx), dyb = (th[1][1] - shy) + (tl[1][1] - sly),
-                                    dzb = (th[1][2] - shz) + (tl[1][2] - slz);
-                        const float r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                        if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) {
-                            if (r2a > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, __builtin_amdgcn_rsqf(r2a), gs, pa);
-                            if (r2b > 0.f) pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, __builtin_amdgcn_rsqf(r2b), gs, pb);
-                        } else {
-                            const float ria = r2a > 0.f ? __builtin_amdgcn_rsqf(r2a) : 0.f;
-                            const float rib = r2b > 0.f ? __builtin_amdgcn_rsqf(r2b) : 0.f;
-                            pair_eval<KERNEL, PS, GS, HS, float>(dxa, dya, dza, r2a, ria, gs, pa);
-                            pair_eval<KERNEL, PS, GS, HS, float>(dxb, dyb, dzb, r2b, rib, gs, pb);
-                        }
-                    }
+                    // sources lane_k, lane_k + t, ... in uniform rounds (only the LDS address is per lane)
+                    const int rounds = cnt / tpt;
+                    const float* p = &tile[0][lane_k];
+                    for (int r = 0; r < rounds; ++r, p += tpt) p2p_source_f32<KERNEL, PS, GS, HS>(p, th, tl, pa, pb);
+                    if (lane_k < cnt - rounds * tpt) p2p_source_f32<KERNEL, PS, GS, HS>(p, th, tl, pa, pb);
     #pragma unroll
                     for (int k = 0; k < NA; ++k) { kahan_add(sum[0][k], cmp[0][k], pa[k]); kahan_add(sum[1][k], cmp[1][k], pb[k]); }
                 }
