@@ -309,7 +309,7 @@ def main():
                 "peak": PEAK_FP32_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}
     else:
-        key = "k_m2l<true, 10>"
+        key = "k_m2l<true, 4>"   # the first (order <= 16) pass: four wavefronts per workgroup
         roof = {"kernel": key + " (rotation-based M2L with Lamb-Helmholtz terms, FP64; F(p) flop/interaction)",
                 "bound": "vector-alu", "achieved": round(m2l_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}

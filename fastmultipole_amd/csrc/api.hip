@@ -238,11 +238,11 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
             stream_wait(F, C);
         }
         Timer tp(C);
-        // Beside the far-field stream the FP64 nearfield leaves one workgroup slot per CU free:
-        // its rate is the same at 3 and 4 workgroups per CU and the far field's kernels (the
-        // estimate's tree, list and sort launches) get room to run. The FP32 nearfield is
-        // slower with fewer workgroups and takes all.
-        const int slots = &F != &C && o.precision == FMMHIP_FP64 ? -1 : 0;
+        // Beside the far-field stream the nearfield leaves one workgroup slot per CU free: its
+        // rate is about the same at 3 and 4 workgroups per CU, and the far field's kernels (the
+        // estimate's tree, list and sort launches) are placed at once instead of waiting for
+        // the nearfield's last workgroups.
+        const int slots = &F != &C ? -1 : 0;
         run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision, 0, -1, slots);
         tp.mark();
         if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) {
