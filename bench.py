@@ -4,17 +4,20 @@ Workload (the configuration the headline metric is quoted on): N = 1,000,000
 point vortices on a ring of radius 1 with a Gaussian core (sigma 0.1),
 tangential strengths with 1e-3 relative noise; velocity + velocity gradient;
 error-controlled expansion order PowerRelativeGradient with p <= 20;
-leaf size 64, MAC 0.4, shrink_recenter. The reference's tolerance is a
-per-interaction heuristic, so the default tolerance (1e-9) is the one whose
-measured error against the FP64 direct sum stays <= 1e-6 on every sampled
-target for the FP64 form; the line reports each form's measured max / p99 error. One step = one complete fmm!
+leaf size 64, MAC 0.55, shrink_recenter. The reference's tolerance is a
+per-interaction heuristic, so the default tolerance (1e-9) and multipole
+acceptance (0.55; not fixed by the workload) are the fastest pair measured
+whose velocity and velocity-gradient errors against the FP64 direct sum both
+stay <= 1e-6 on every sampled target for the FP64 form; the line reports each
+form's measured max / p99 errors. One step = one complete fmm!
 evaluation (both trees, interaction lists, estimate_influence pre-pass, P2P,
 upward, M2L, downward) on inputs already resident in HBM.
 
 Prints one JSON line. Both forms (mixed: FP32 nearfield; fp64) are timed in
 the same run, each with its error against the FP64 O(N^2) direct sum on
 10,000 sampled targets. `value` is the evaluations/s of the fastest form whose
-max error meets the 1e-6 bar (the mixed form if none does, flagged).
+max velocity and gradient errors meet the 1e-6 bar (the mixed form if none
+does, flagged).
 """
 import argparse
 import json
@@ -133,7 +136,7 @@ def serving_sample(F, W, _lib, args):
 def batch_sample(F, W, _lib, args):
     """Batched uneven instances: 512 independent systems, N = floor(exp(U[ln 1e3, ln 2e5])) (seed 4),
     even instances uniform-cube sources (potential + gradient, p 8, MAC 0.5), odd instances vortex rings
-    (velocity + gradient, PowerRelativeGradient at the headline's tolerance, p <= 20, MAC 0.4); FP32
+    (velocity + gradient, PowerRelativeGradient at the headline's tolerance and MAC, p <= 20); FP32
     nearfield with compensated sums (the mixed form). Each kind's leaf size is chosen by tune_fmm on a
     median-size instance. All systems are submitted at once to the request server; the timed region is
     the second of two identical rounds (the first grows the workers' buffers)."""
@@ -210,7 +213,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--leaf", type=int, default=64)
-    ap.add_argument("--mac", type=float, default=0.4)
+    ap.add_argument("--mac", type=float, default=0.55)
     ap.add_argument("--pmax", type=int, default=20)
     ap.add_argument("--samples", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000)
@@ -289,7 +292,10 @@ def main():
             f"vel err max {e_v[0]:.2e} p99 {e_v[1]:.2e}; grad err max {e_g[0]:.2e} p99 {e_g[1]:.2e}; "
             f"pmax {stats[-1]['pmax']} m2l {stats[-1]['n_m2l']} pairs {stats[-1]['n_p2p_pairs']}")
 
-    ok_forms = [k for k, v in forms.items() if v["err_v"][0] <= 1e-6]
+    def meets_bar(v):   # velocity and its gradient, the workload's two outputs
+        return v["err_v"][0] <= 1e-6 and v["err_grad"][0] <= 1e-6
+
+    ok_forms = [k for k, v in forms.items() if meets_bar(v)]
     if ok_forms:
         main_form = max(ok_forms, key=lambda k: forms[k]["evals"])
     else:
@@ -302,19 +308,22 @@ def main():
     hist = np.bincount(M["orders"], minlength=args.pmax + 1)
     m2l_total = float(sum(c * m2l_flops(p) for p, c in enumerate(hist)))
     m2l_tflops = m2l_total / (ph["ms_m2l"] * 1e-3) / 1e12
-    if ph["ms_p2p"] >= ph["ms_m2l"]:
-        key = "k_p2p_f32<1, false, true, true>" if main_form == "mixed" else "k_p2p_f64<1, false, true, true>"
-        roof = {"kernel": key + " (nearfield P2P, vortex v + grad v, 63 flop/pair)",
+    key = "k_p2p_f32<1, false, true, true>" if main_form == "mixed" else "k_p2p_f64<1, false, true, true>"
+    roof_p2p = {"kernel": key + " (nearfield P2P, vortex v + grad v, 63 flop/pair)",
                 "bound": "vector-alu", "achieved": round(p2p_tflops, 3),
                 "peak": PEAK_FP32_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}
-    else:
-        key = "k_m2l<true, 4>"   # the first (order <= 16) pass: four wavefronts per workgroup
-        roof = {"kernel": key + " (rotation-based M2L with Lamb-Helmholtz terms, FP64; F(p) flop/interaction)",
+    # the error-controlled M2L: a first launch at the order cap 18, then the deferred interactions
+    # at the full order; achieved = F(p) flops summed over the interactions / the M2L phase time
+    roof_m2l = {"kernel": "k_m2l<true, NW> (rotation-based M2L with Lamb-Helmholtz terms, FP64, both launches; "
+                          "F(p) flop/interaction)",
                 "bound": "vector-alu", "achieved": round(m2l_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "traffic": pmc_traffic(key)}
-    roof["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, profiles/ring1m_traffic_pmc.md)"
-    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+                "traffic": None}
+    roofs = [roof_p2p, roof_m2l] if ph["ms_p2p"] >= ph["ms_m2l"] else [roof_m2l, roof_p2p]
+    for r in roofs:
+        r["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, profiles/ring1m_traffic_pmc.md)"
+        r["frac"] = round(r["achieved"] / r["peak"], 4)
+    roof, roof_second = roofs
 
     serving = None if args.no_serve else serving_sample(F, W, _lib, args)
     batch = None if args.no_batch else batch_sample(F, W, _lib, args)
@@ -350,11 +359,12 @@ def main():
                                f"PowerRelativeGradient({args.tol}), p<={args.pmax}, leaf {args.leaf}, MAC {args.mac}, "
                                "shrink_recenter, SelfTuningTreeStop"},
         "roofline": roof,
+        "roofline_second": roof_second,
         "cpu_baseline": cpu,
         "forms": {k: {"evals_per_s": round(v["evals"], 1), "ms_per_step": round(v["ms"], 3),
                       "rel_err_velocity": {"max": v["err_v"][0], "p99": v["err_v"][1], "median": v["err_v"][2]},
                       "rel_err_gradient": {"max": v["err_grad"][0], "p99": v["err_grad"][1]},
-                      "meets_error_bar": bool(v["err_v"][0] <= 1e-6),
+                      "meets_error_bar": bool(meets_bar(v)),
                       "phases_ms": {k2: round(x, 3) for k2, x in v["phases"].items()},
                       "pmax": v["stats"]["pmax"], "n_m2l": v["stats"]["n_m2l"],
                       "p2p_pairs_per_s": v["stats"]["n_p2p_pairs"] / (v["phases"]["ms_p2p"] * 1e-3),
@@ -363,7 +373,7 @@ def main():
         "mixed_over_fp64": (round(forms["mixed"]["evals"] / forms["fp64"]["evals"], 3)
                             if "mixed" in forms and "fp64" in forms else None),
         "form": main_form,
-        "meets_error_bar": bool(M["err_v"][0] <= 1e-6),
+        "meets_error_bar": bool(meets_bar(M)),
         "latency_ms": {"p50": round(float(np.percentile(M["lat"], 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(M["lat"], 99)) * 1e3, 3),
                        "note": "the headline workload's evaluations issued back to back, one at a time"},

@@ -261,7 +261,7 @@ static constexpr int FWD_DEGREES = 64; // degrees per forward chunk of the error
 // error-controlled M2L passes: orders up to the first cap in one pass with more wavefronts per
 // workgroup, the interactions needing more at the full order in a second
 enum { M2L_PASS_ALL = 0, M2L_PASS_FIRST = 1, M2L_PASS_DEFERRED = 2 };
-static constexpr int M2L_FIRST_CAP = 16;
+static constexpr int M2L_FIRST_CAP = 18;
 static constexpr int SMALL_P = 4;      // fixed orders up to this run one M2L per lane (k_m2l_lane)
 static constexpr size_t LDS_LIMIT = 160 * 1024;
 
