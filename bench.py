@@ -313,12 +313,15 @@ def main():
                 "bound": "vector-alu", "achieved": round(p2p_tflops, 3),
                 "peak": PEAK_FP32_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}
-    # the error-controlled M2L: a first launch at the order cap 18, then the deferred interactions
-    # at the full order; achieved = F(p) flops summed over the interactions / the M2L phase time
-    roof_m2l = {"kernel": "k_m2l<true, NW> (rotation-based M2L with Lamb-Helmholtz terms, FP64, both launches; "
-                          "F(p) flop/interaction)",
+    # the error-controlled M2L: a first launch at the order cap 18 (eight wavefronts per workgroup), then
+    # the deferred interactions at the full order (six); achieved = F(p) flops summed over the
+    # interactions / the M2L phase time (the two launches back to back)
+    m2l_keys = ("k_m2l<true, 8>", "k_m2l<true, 6>")
+    m2l_traffic = [pmc_traffic(k) for k in m2l_keys]
+    roof_m2l = {"kernel": " + ".join(m2l_keys) + " (rotation-based M2L with Lamb-Helmholtz terms, FP64, both launches; "
+                                                 "F(p) flop/interaction)",
                 "bound": "vector-alu", "achieved": round(m2l_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "traffic": None}
+                "traffic": None if None in m2l_traffic else sum(m2l_traffic)}
     roofs = [roof_p2p, roof_m2l] if ph["ms_p2p"] >= ph["ms_m2l"] else [roof_m2l, roof_p2p]
     for r in roofs:
         r["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, profiles/ring1m_traffic_pmc.md)"
