@@ -187,9 +187,10 @@ static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
     eo.list_method = FMMHIP_SELFTUNING_TREESTOP;
     eo.farfield = eo.nearfield = eo.self_induced = 1;
     if (X.t_xyz) eval_build_trees(X, eo);
-    else {   // the targets are the sources: one build gives both trees
+    else {   // the targets are the sources: one build gives both trees; the Morton keys and their
+             // order are the main source tree's (same buffer), built on the evaluation's stream
         build_tree(C, X.st, X.ns, X.s_buf, data_per_body(X.kernel), false, eo.leaf_size_source, eo.list_method, true,
-                   eo.expansion_order, strength_dims(X.kernel));
+                   eo.expansion_order, strength_dims(X.kernel), tree_keys(*E.ctx, E.st));
         clone_target_tree(C, X.st, X.tt);
         X.trees_built = true;
         X.lists_built = false;

@@ -80,6 +80,7 @@ struct Tree {
     int strength_dims = 0;
     DBuf<double> expansions;         // [branch][H][2][2] (reference layout per branch)
     int p = 0;
+    uint64_t build_id = 0;           // identifies the build whose keys a context's scratch may still hold
 };
 
 struct Lists {
@@ -182,8 +183,12 @@ inline int n_harmonics(int p) { return ((p + 1) * (p + 2)) / 2; }
 
 // ---------------------------------------------------------- module entry
 // tree.hip
+// keys: the scratch of another context whose last build sorted the same body buffer (its Morton
+// keys and order are reused instead of computed again), or null
 void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, bool target, int64_t leaf_size,
-                int method, bool shrink, int p, int strength_dims);
+                int method, bool shrink, int p, int strength_dims, const TreeScratch* keys = nullptr);
+// the scratch of context C if it still holds the sorted keys of T's current build, else null
+const TreeScratch* tree_keys(const Context& C, const Tree& T);
 // the target tree of the bodies of source tree s (same leaf size and options), by device copies
 void clone_target_tree(Context& C, const Tree& s, Tree& t);
 // lists.hip

@@ -18,6 +18,7 @@
 #include <cstring>   // host memset for rocprim headers
 #include <rocprim/rocprim.hpp>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 namespace fmmhip {
@@ -351,6 +352,13 @@ struct TreeScratch {
     DBuf<int64_t> act_branch, act_first, act_count, sub_first, sub_count;
     DBuf<int32_t> nonempty, rank, split, srank, flag, frank;
     DBuf<double> part;
+    // the build whose keys (khi, klo, idx unsorted and khi_s, klo_s, idx_s sorted) the arrays hold
+    const Tree* last = nullptr;
+    uint64_t last_id = 0;
+    const double* buf = nullptr;
+    int rows = 0;
+    int64_t n = 0;
+    double center[3] = {0, 0, 0}, bmax = 0, radius = 0;
 };
 
 static TreeScratch& scratch(Context& C) {   // per context: builds on different streams share nothing
@@ -359,6 +367,11 @@ static TreeScratch& scratch(Context& C) {   // per context: builds on different 
 }
 
 void free_tree_scratch(TreeScratch* s) { delete s; }
+
+const TreeScratch* tree_keys(const Context& C, const Tree& T) {
+    const TreeScratch* S = C.tree_scratch;
+    return S && S->last == &T && T.build_id != 0 && S->last_id == T.build_id ? S : nullptr;
+}
 
 // Level-by-level structure from sorted keys; returns false if deeper levels than
 // max_level would be needed.
@@ -431,7 +444,7 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, double center[3
 }
 
 void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, bool target, int64_t ls, int method,
-                bool shrink, int p, int sdims) {
+                bool shrink, int p, int sdims, const TreeScratch* keys) {
     T.n = n;
     T.rows = rows;
     T.p = p;
@@ -444,29 +457,52 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     if (n >= (int64_t)1 << 32) throw Error(FMMHIP_EINVAL, "more than 2^32 bodies in one system");
     if (ls < 1) throw Error(FMMHIP_EINVAL, "leaf size must be positive");
     TreeScratch& S = scratch(C);
-    // root cube (center_box, src/tree.jl:894-907; :20-26)
-    const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
-    S.part.resize(6 * nblk);
-    k_bbox_partial<<<nblk, 256, 0, C.stream>>>(d_buf, rows, n, S.part.data());
-    LAUNCH_CHECK();
-    std::vector<double> part(6 * nblk);
-    HIP_CHECK(hipMemcpyAsync(part.data(), S.part.data(), part.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
-    HIP_CHECK(hipStreamSynchronize(C.stream));
-    double mn[3] = {part[0], part[1], part[2]}, mx[3] = {part[3], part[4], part[5]};
-    for (int b = 1; b < nblk; ++b)
-        for (int d = 0; d < 3; ++d) { mn[d] = jmin(mn[d], part[6 * b + d]); mx[d] = jmax(mx[d], part[6 * b + 3 + d]); }
-    double center[3], box[3];
-    for (int d = 0; d < 3; ++d) { center[d] = (mx[d] + mn[d]) * 0.5; box[d] = mx[d] - center[d]; }
-    const double bmax = std::max(std::max(box[0], box[1]), box[2]);
-    const double radius = std::sqrt(bmax * bmax * 3.0);
-    // full octant paths, then a stable sort by the first 21 levels
+    double center[3], bmax, radius;
     S.khi.resize(n); S.klo.resize(n); S.idx.resize(n); S.khi_s.resize(n); S.klo_s.resize(n); S.idx_s.resize(n);
-    k_path_keys<<<grid_for(n, 256), 256, 0, C.stream>>>(d_buf, rows, n, center[0], center[1], center[2], bmax, S.khi.data(),
-                                                       S.klo.data(), S.idx.data());
-    LAUNCH_CHECK();
-    sort_pairs_u64(C, S.khi.data(), S.khi_s.data(), S.idx.data(), S.idx_s.data(), n);
-    k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.klo.data(), S.idx_s.data(), n, S.klo_s.data());
-    LAUNCH_CHECK();
+    if (keys && keys != &S && keys->n == n && keys->buf == d_buf && keys->rows == rows) {
+        // the root cube, the octant paths and their order depend only on the positions: take them
+        // from the other context's build of the same buffer (its stream has been waited for)
+        for (int d = 0; d < 3; ++d) center[d] = keys->center[d];
+        bmax = keys->bmax;
+        radius = keys->radius;
+        auto copy = [&](auto& dst, const auto& src) {
+            HIP_CHECK(hipMemcpyAsync(dst.data(), src.data(), (size_t)n * sizeof(*src.data()), hipMemcpyDeviceToDevice, C.stream));
+        };
+        copy(S.khi, keys->khi); copy(S.klo, keys->klo); copy(S.idx, keys->idx);
+        copy(S.khi_s, keys->khi_s); copy(S.klo_s, keys->klo_s); copy(S.idx_s, keys->idx_s);
+    } else {
+        // root cube (center_box, src/tree.jl:894-907; :20-26)
+        const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+        S.part.resize(6 * nblk);
+        k_bbox_partial<<<nblk, 256, 0, C.stream>>>(d_buf, rows, n, S.part.data());
+        LAUNCH_CHECK();
+        std::vector<double> part(6 * nblk);
+        HIP_CHECK(hipMemcpyAsync(part.data(), S.part.data(), part.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+        HIP_CHECK(hipStreamSynchronize(C.stream));
+        double mn[3] = {part[0], part[1], part[2]}, mx[3] = {part[3], part[4], part[5]};
+        for (int b = 1; b < nblk; ++b)
+            for (int d = 0; d < 3; ++d) { mn[d] = jmin(mn[d], part[6 * b + d]); mx[d] = jmax(mx[d], part[6 * b + 3 + d]); }
+        double box[3];
+        for (int d = 0; d < 3; ++d) { center[d] = (mx[d] + mn[d]) * 0.5; box[d] = mx[d] - center[d]; }
+        bmax = std::max(std::max(box[0], box[1]), box[2]);
+        radius = std::sqrt(bmax * bmax * 3.0);
+        // full octant paths, then a stable sort by the first 21 levels
+        k_path_keys<<<grid_for(n, 256), 256, 0, C.stream>>>(d_buf, rows, n, center[0], center[1], center[2], bmax, S.khi.data(),
+                                                           S.klo.data(), S.idx.data());
+        LAUNCH_CHECK();
+        sort_pairs_u64(C, S.khi.data(), S.khi_s.data(), S.idx.data(), S.idx_s.data(), n);
+        k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.klo.data(), S.idx_s.data(), n, S.klo_s.data());
+        LAUNCH_CHECK();
+    }
+    static std::atomic<uint64_t> builds{0};
+    S.last = &T;
+    S.last_id = T.build_id = ++builds;
+    S.buf = d_buf;
+    S.rows = rows;
+    S.n = n;
+    for (int d = 0; d < 3; ++d) S.center[d] = center[d];
+    S.bmax = bmax;
+    S.radius = radius;
     if (!build_structure(C, T, S, center, bmax, radius, ls, method, LEVELS_PER_WORD)) {
         // deeper than 21 levels: order by the second word first, then stably by the first
         DBuf<uint64_t> tmpk;
