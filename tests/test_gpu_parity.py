@@ -199,6 +199,23 @@ def test_error_controlled_order_matches_oracle(name, gen, kind, em, ls):
         assert np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / scale < 1e-11
 
 
+def test_relative_tolerance_separate_targets_matches_oracle():
+    """estimate_influence! with a target system of its own (two tree builds for the estimate, the
+    main target tree of other points): max_influence per target branch and the per-interaction
+    orders as in the oracle."""
+    _say("[gpu] error-controlled ring-power-rel with separate targets")
+    sbuf = W.vortex_ring(6000, seed=36)
+    txyz = np.ascontiguousarray(W.vortex_ring(3000, seed=37)[:, :3] * 1.02)
+    kw = dict(expansion_order=20, leaf_size_source=48, multipole_acceptance=0.5,
+              error_tolerance=F.PowerRelativeGradient(1e-6))
+    ev, opt = _gpu_eval(O.KIND_VORTEX, sbuf, txyz=txyz, **kw)
+    ev.fmm(opt)
+    run = _oracle_run(O.KIND_VORTEX, sbuf, txyz=txyz, p=20, **{k: v for k, v in kw.items() if k != "expansion_order"}).fmm()
+    np.testing.assert_allclose(ev.tree(0).reals[:, 14], run.branches(0)[1][:, 14], rtol=1e-11, atol=0)
+    Pg, Po = ev.m2l_orders(), run.m2l_orders()
+    assert len(Pg) == len(Po) and float(np.mean(Pg == Po)) >= 0.999
+
+
 MFMA_FMM = [c for c in FMM_CASES if c[0] in ("laplace-4k-p8", "vortex-4k-p8", "ring-6k-p10", "ring-3k-p20", "panel-src-tri-3k-p8")]
 MFMA_ERR = [c for c in ERR_CASES if c[0] in ("laplace-rotated-abs", "vortex-power-abs", "ring-power-rel")]
 
