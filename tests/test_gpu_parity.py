@@ -97,7 +97,22 @@ def test_tree_and_lists_bit_exact(name, gen, kind, ls, method, shrink):
         np.testing.assert_array_equal(ev.list(which), run.list(which))
 
 
+def _with_duplicates(b, k, seed):
+    """b with k more bodies at the positions of k of its bodies (other radii and strengths): the near
+    field then holds coincident pairs of distinct bodies, which contribute nothing."""
+    rng = np.random.default_rng(seed)
+    extra = b[rng.choice(len(b), size=k, replace=False)].copy()
+    extra[:, 3:] = b[rng.choice(len(b), size=k, replace=False), 3:]
+    return np.ascontiguousarray(np.vstack([b, extra]))
+
+
 FMM_CASES = [
+    ("laplace-dup-3k-p6", lambda: _with_duplicates(W.uniform_sources(3000, seed=26), 300, 1), O.KIND_SOURCE,
+     dict(expansion_order=6, leaf_size_source=30, multipole_acceptance=0.5, scalar_potential=True, hessian=False)),
+    ("vortex-dup-3k-p8", lambda: _with_duplicates(W.uniform_vortices(3000, seed=27), 300, 2), O.KIND_VORTEX,
+     dict(expansion_order=8, leaf_size_source=30, multipole_acceptance=0.5)),
+    ("dipole-dup-3k-p6", lambda: _with_duplicates(W.uniform_dipoles(3000, seed=28), 300, 3), O.KIND_DIPOLE,
+     dict(expansion_order=6, leaf_size_source=30, multipole_acceptance=0.5, scalar_potential=True)),
     ("laplace-4k-p8", lambda: W.uniform_sources(4000, seed=21), O.KIND_SOURCE, dict(expansion_order=8, leaf_size_source=50,
                                                                                     multipole_acceptance=0.5,
                                                                                     scalar_potential=True, hessian=False)),
