@@ -1073,7 +1073,7 @@ __device__ int m2l_test_chunk(const WaveLds& W, const ErrGeo& G, const ErrParams
 template <bool LH>
 __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const double (&sc)[3], const WaveLds& W,
                         const double* A, int P, const TablePtrs& tb, const ErrParams& E, int& success,
-                        double (&acc)[MAXCH][8], bool defer) {
+                        double (&acc)[MAXCH][8], bool defer, int n_test0) {
     const double dx = T.target_center[0] - sc[0], dy = T.target_center[1] - sc[1], dz = T.target_center[2] - sc[2];
     const double r = fill_angles(W, dx, dy, dz, P);
     fill_m2l_factors(W, r, P);
@@ -1115,7 +1115,8 @@ __device__ int m2l_pair(const double* __restrict__ src, const Branch& T, const d
             }
             wave_sync();
             rot_chunk_store<LH>(W, A, tb.zeta, 1.0, n0, n1);
-            const int nA = n0 < 1 ? 1 : n0;
+            // degrees below n_test0 are known to fail (tested by the first pass)
+            const int nA = max(n0 < 1 ? 1 : n0, n_test0);
             if (nA <= n1) {
                 const int hit = m2l_test_chunk<LH>(W, G, E, tb, nA, n1);
                 if (hit >= 0) {
@@ -1174,7 +1175,10 @@ __global__ __launch_bounds__(64 * NW) void k_m2l(const int32_t* __restrict__ m2l
         }
         if (mode == M2L_PASS_DEFERRED && orders[li] >= 0) continue;
         int ok = 1;
-        const int Pu = m2l_pair<LH>(sex + (int64_t)js * HE4, T, sc, W, A, P, tb, E, ok, acc, mode == M2L_PASS_FIRST);
+        // the deferred pass redoes interactions whose tests of degrees 1 .. cap all failed in the first
+        // pass: each degree's test is independent, so it resumes testing at cap + 1
+        const int Pu = m2l_pair<LH>(sex + (int64_t)js * HE4, T, sc, W, A, P, tb, E, ok, acc, mode == M2L_PASS_FIRST,
+                                    mode == M2L_PASS_DEFERRED ? M2L_FIRST_CAP + 1 : 1);
         if (lane_id() == 0) orders[li] = Pu;
         if (Pu < 0) continue;
         local_max = max(local_max, Pu);
