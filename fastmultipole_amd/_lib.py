@@ -66,7 +66,8 @@ class Request(C.Structure):
 
 class RequestStats(C.Structure):
     _fields_ = [
-        ("status", C.c_int32), ("sweeps", C.c_int32), ("pmax", C.c_int32), ("queue_ms", C.c_double),
+        ("status", C.c_int32), ("sweeps", C.c_int32), ("pmax", C.c_int32), ("error_success", C.c_int32),
+        ("converged", C.c_int32), ("queue_ms", C.c_double),
         ("run_ms", C.c_double), ("latency_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double),
         ("footprint_bytes", C.c_double),
     ]

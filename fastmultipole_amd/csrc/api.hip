@@ -107,6 +107,7 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
     E.trees_built = true;
     E.lists_built = false;
     E.p_built = p;
+    E.keys_fresh = true;
 }
 
 static void eval_build_lists(Eval& E, const fmmhip_options& o, bool count_pairs = true) {
@@ -190,7 +191,7 @@ static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
     else {   // the targets are the sources: one build gives both trees; the Morton keys and their
              // order are the main source tree's (same buffer), built on the evaluation's stream
         build_tree(C, X.st, X.ns, X.s_buf, data_per_body(X.kernel), false, eo.leaf_size_source, eo.list_method, true,
-                   eo.expansion_order, strength_dims(X.kernel), tree_keys(*E.ctx, E.st));
+                   eo.expansion_order, strength_dims(X.kernel), E.keys_fresh ? tree_keys(*E.ctx, E.st) : nullptr);
         clone_target_tree(C, X.st, X.tt);
         X.trees_built = true;
         X.lists_built = false;
@@ -288,6 +289,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
     }
     const double t_tot = total.stop();
     E.ran = true;
+    if (!is_estimate) E.keys_fresh = false;
     if (stats) {
         stats->pmax = E.pmax;
         stats->error_success = E.error_success;

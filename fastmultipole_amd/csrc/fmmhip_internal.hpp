@@ -157,6 +157,10 @@ struct Eval {
     DBuf<double> tout;                        // sorted target outputs, SoA [13][nt]: phi, v(3), h(9)
     DBuf<double> tout_aos;                    // 16 x nt original order
     bool trees_built = false, lists_built = false, ran = false;
+    // the context's tree scratch holds the sorted keys of the positions st was built from, and no
+    // evaluation has run since: estimate_influence may reuse them (positions handed over in place
+    // can change between an evaluation and the next one)
+    bool keys_fresh = false;
     int32_t pmax = 0, error_success = 1;
     int p_built = -1;
     Eval* estimate = nullptr;                 // persistent low-order evaluation of estimate_influence

@@ -41,6 +41,7 @@ struct Job {
     double footprint = 0.0;
     std::vector<double> targets, strengths;
     int32_t status = FMMHIP_OK, sweeps = 0, pmax = 0;
+    int32_t error_success = 1, converged = 1;
     std::string error;
     double t_submit = 0, t_start = 0, t_end = 0;
     bool done = false;
@@ -93,14 +94,22 @@ void run_job(fmmhip_ctx* ctx, fmmhip_eval* ev, Job& J) {
         fmmhip_stats s{};
         st = fmmhip_fmm(ev, &r.options, &s);
         J.pmax = s.pmax;
+        J.error_success = s.error_success;
     } else if (st == FMMHIP_OK) {
         st = fmmhip_fgs_setup(ev, &r.options, r.influence);
         if (st == FMMHIP_OK) {
             J.strengths.resize((size_t)r.n);
-            st = fmmhip_fgs_solve(ev, J.ext.data(), 0, r.max_iterations, r.tolerance, J.strengths.data(), &J.sweeps, nullptr);
+            st = fmmhip_fgs_solve(ev, J.ext.data(), 0, r.max_iterations, r.tolerance, J.strengths.data(), &J.sweeps,
+                                  nullptr);
+            // solve! stops before the sweep of the first iteration whose residual meets the tolerance;
+            // a solve that sweeps max_iterations times never met it
+            if (st == FMMHIP_OK && r.n > 0) J.converged = J.sweeps < r.max_iterations;
         }
     }
     if (st != FMMHIP_OK) J.error = fmmhip_last_error(ctx);
+    // the call succeeded but the tolerance was not met (reference: error_success == false for fmm!, or
+    // solve! ending at max_iterations): results are still delivered
+    if (st == FMMHIP_OK && (!J.error_success || !J.converged)) st = FMMHIP_ETOL;
     if (st == FMMHIP_OK || st == FMMHIP_ETOL) {   // a missed tolerance still leaves results, as in the reference
         J.targets.resize((size_t)16 * r.n);
         const int g = fmmhip_eval_get_targets(ev, J.targets.data(), 0);
@@ -264,6 +273,8 @@ fmmhip_status fmmhip_server_wait(fmmhip_server* S, int64_t ticket, double* targe
         stats->status = st;
         stats->sweeps = J->sweeps;
         stats->pmax = J->pmax;
+        stats->error_success = J->error_success;
+        stats->converged = J->converged;
         stats->queue_ms = J->t_start - J->t_submit;
         stats->run_ms = J->t_end - J->t_start;
         stats->latency_ms = J->t_end - J->t_submit;

@@ -39,7 +39,9 @@ typedef enum {
     FMMHIP_EINVAL = 1,   /* bad argument (reference: @assert / ArgumentError) */
     FMMHIP_ENOMEM = 2,   /* device allocation failed */
     FMMHIP_EDEVICE = 3,  /* HIP runtime or kernel launch error, or no GPU */
-    FMMHIP_ETOL = 4,     /* error tolerance not met (reference: error_success == false) */
+    FMMHIP_ETOL = 4,     /* error tolerance not met (reference: error_success == false), results delivered;
+                            returned by the request server (fmmhip_server_wait); the direct calls return
+                            FMMHIP_OK and report fmmhip_stats.error_success, as fmm! returns error_success */
     FMMHIP_EDEPTH = 5    /* octree deeper than 42 levels (coincident bodies) */
 } fmmhip_status;
 
@@ -219,9 +221,13 @@ typedef struct {
     const double* external;       /* SOLVE: host, n external influences; copied by submit */
 } fmmhip_request;
 typedef struct {
-    int32_t status;               /* the request's fmmhip_status (FMMHIP_ETOL: results delivered, tolerance missed) */
+    int32_t status;               /* the request's fmmhip_status; FMMHIP_ETOL: results delivered, but the FMM's
+                                     error tolerance was missed (error_success == 0) or the solve stopped at
+                                     max_iterations above its residual tolerance (converged == 0) */
     int32_t sweeps;               /* SOLVE: Gauss-Seidel sweeps */
     int32_t pmax;                 /* FMM: fmmhip_stats.pmax */
+    int32_t error_success;        /* FMM: fmmhip_stats.error_success (1 for a solve) */
+    int32_t converged;            /* SOLVE: residual tolerance met (1 for an FMM) */
     double queue_ms, run_ms, latency_ms;  /* submit -> start, start -> end, submit -> end */
     double start_ms, end_ms;      /* start and end, in ms since the server was created */
     double footprint_bytes;       /* the admission estimate */

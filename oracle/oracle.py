@@ -69,6 +69,7 @@ def lib():
         L.orc_get_expansions.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_get_list.argtypes = [C.c_void_p, C.c_int, _ip32]
         L.orc_get_m2l_orders.argtypes = [C.c_void_p, _ip32]
+        L.orc_get_m2l_margins.argtypes = [C.c_void_p, _dp]
         L.orc_get_targets.argtypes = [C.c_void_p, _dp]
         L.orc_pmax.argtypes = [C.c_void_p]
         L.orc_pmax.restype = C.c_int
@@ -219,6 +220,14 @@ class Run:
         n = lib().orc_n_list(self._h, 0)
         out = np.zeros(n, dtype=np.int32)
         lib().orc_get_m2l_orders(self._h, out)
+        return out
+
+    def m2l_margins(self):
+        """Per M2L interaction: how close its order decision came to the tolerance, the smallest
+        |estimate/limit - 1| over the comparisons made (inf at fixed order)."""
+        n = lib().orc_n_list(self._h, 0)
+        out = np.zeros(n)
+        lib().orc_get_m2l_margins(self._h, out)
         return out
 
     def targets(self):

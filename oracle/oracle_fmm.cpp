@@ -720,6 +720,7 @@ struct Run {
     int Pmax = 0;
     bool error_success = true;
     std::vector<int32_t> m2l_P;   // expansion order chosen per M2L interaction
+    std::vector<double> m2l_margin;   // last_m2l_margin of each M2L interaction
 };
 
 static BranchGeom geom(const OBranch& b) {
@@ -816,12 +817,14 @@ static void fmm_run(Run& R, bool estimate_pass) {
     }
     // horizontal pass (src/fmm.jl:455-490)
     R.m2l_P.assign(R.m2l.size(), 0);
+    R.m2l_margin.assign(R.m2l.size(), INFINITY);
     for (size_t k = 0; k < R.m2l.size(); ++k) {
         int32_t it = R.m2l[k][0], js = R.m2l[k][1];
         bool ok = true;
         int P = multipole_to_local(Exp{&R.tt.expansions[(size_t)it * 4 * Hn]}, geom(R.tt.branches[it]),
                                    Exp{&R.st.expansions[(size_t)js * 4 * Hn]}, geom(R.st.branches[js]), S, p, LH, o.em, ok);
         R.m2l_P[k] = P;
+        R.m2l_margin[k] = last_m2l_margin;
         R.Pmax = std::max(P, R.Pmax);
         R.error_success = R.error_success && ok;
     }
@@ -1244,6 +1247,10 @@ void orc_get_list(void* h, int which, int32_t* out) {
 void orc_get_m2l_orders(void* h, int32_t* out) {
     Run& R = *(Run*)h;
     std::memcpy(out, R.m2l_P.data(), R.m2l_P.size() * sizeof(int32_t));
+}
+void orc_get_m2l_margins(void* h, double* out) {
+    Run& R = *(Run*)h;
+    std::memcpy(out, R.m2l_margin.data(), R.m2l_margin.size() * sizeof(double));
 }
 // target results in the ORIGINAL body order (16 x nt)
 void orc_get_targets(void* h, double* out) {

@@ -729,8 +729,16 @@ void minimum_distance(const double* c1, const double* c2, const double* box2, do
     out[2] = md1(c2[2] - c1[2], -box2[2], box2[2]);
 }
 
+thread_local double last_m2l_margin = INFINITY;
+
+static void note_margin(double lhs, double limit) {
+    double m = std::fabs(lhs / limit - 1.0);
+    if (!(m >= last_m2l_margin)) last_m2l_margin = m;   // NaN ratios count as zero margin
+}
+
 int multipole_to_local(Exp target, const BranchGeom& tb, Exp source, const BranchGeom& sb, Scratch& s, int p, bool LH,
                        const ErrorMethod& em, bool& error_success) {
+    last_m2l_margin = INFINITY;
     s.ensure(p);
     Tables& TB = tables();
     TB.ensure(p);
@@ -823,6 +831,7 @@ int multipole_to_local(Exp target, const BranchGeom& tb, Exp source, const Branc
             eps_mp = SQRT3 * pow_next * r_mp_inv_np2 * r_mp * np1_f / (Mt_n0 * (double)(n + 1));
             if (LH) eps_mp += SQRT3 * (double)n * pow_c_next * r_mp_inv_np2 * r_mp * np1_f / (Mt_next * (double)(n + 1));
         }
+        note_margin(eps_mp, limit);
         if (eps_mp <= limit) {
             if (em.kind == 1) {
                 double q0, q1r, q1i, c1r, c1i;
@@ -841,6 +850,7 @@ int multipole_to_local(Exp target, const BranchGeom& tb, Exp source, const Branc
                     if (LH) eps_l += SQRT3 * (double)n * lc * r_l_nm1 * r_l * nm1_f_inv / (Lt_n0 * (double)n);
                 }
             }
+            note_margin(eps_mp + eps_l * LOCAL_ERROR_SAFETY, limit);
             if (eps_mp + eps_l * LOCAL_ERROR_SAFETY <= limit) {
                 P = n - 1 + (em.BE ? 1 : 0);
                 error_success = true;

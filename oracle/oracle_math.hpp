@@ -96,6 +96,11 @@ struct BranchGeom {
 // returns expansion order used; error_success via out parameter (src/translate.jl:519-1412)
 int multipole_to_local(Exp target, const BranchGeom& tb, Exp source, const BranchGeom& sb,
                        Scratch& s, int p, bool LH, const ErrorMethod& em, bool& error_success);
+// closeness of the last error-controlled multipole_to_local's order decision to its tolerance:
+// min over every comparison it made (eps_mp <= limit, eps_mp + 2 eps_l <= limit) of |lhs/limit - 1|
+// (+inf for a fixed-order call). Test infrastructure: an interaction whose margin is well above
+// rounding cannot change order under last-bit differences in the rotated coefficients.
+extern thread_local double last_m2l_margin;
 
 void minimum_distance(const double* c1, const double* c2, const double* box2, double* out); // src/error.jl:5-20
 

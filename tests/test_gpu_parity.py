@@ -5,8 +5,11 @@ Bars:
 * FP64 evaluation: potential/gradient/hessian within 1e-11 of the oracle's FMM
   relative to the largest target value (same expansion order, leaf size, MAC);
 * error-controlled expansion order: the order chosen per M2L interaction equals
-  the oracle's for at least 99.9% of interactions (1-ulp differences in acos/atan
-  can flip a threshold) and the results still meet 1e-11 relative;
+  the oracle's for every interaction whose error estimate is not within 1e-9
+  (relative) of the tolerance (only such a decision can flip under last-bit
+  differences of the rotated coefficients); pmax and error_success equal the
+  oracle's; targets under no flipped interaction match the oracle to 1e-11
+  relative, the others within twice the oracle's own error against direct!;
 * mixed precision: the stated error bound against the FP64 O(N^2) direct sum.
 """
 import math
@@ -178,6 +181,7 @@ def test_fmm_fp64_matches_oracle(name, gen, kind, kw):
 
 
 ERR_CASES = [
+    # (name, generator, kernel, error method, leaf size[, multipole acceptance])
     ("laplace-rotated-abs", lambda: W.uniform_sources(5000, seed=31, strength_scale=1 / 5000), O.KIND_SOURCE,
      F.RotatedCoefficientsAbsoluteGradient(1e-5, False), 10),
     ("vortex-power-abs", lambda: W.uniform_vortices(5000, seed=32), O.KIND_VORTEX, F.PowerAbsoluteGradient(1e-5, False), 10),
@@ -186,15 +190,42 @@ ERR_CASES = [
     ("ring-power-rel", lambda: W.vortex_ring(8000, seed=34), O.KIND_VORTEX, F.PowerRelativeGradient(1e-6), 64),
     ("laplace-rotated-rel", lambda: W.uniform_sources(5000, seed=35), O.KIND_SOURCE,
      F.RotatedCoefficientsRelativeGradient(1e-6), 20),
+    # PowerRelativePotential: estimate_influence! keeps the potential per target (src/fmm.jl:1185-1187)
+    ("laplace-power-potential-rel", lambda: W.uniform_sources(5000, seed=40), O.KIND_SOURCE,
+     F.PowerRelativePotential(1e-7), 20),
+    # tolerances the order cap of 20 cannot meet everywhere (error_success false, pmax 21): orders 19-20
+    # come from the deferred second M2L launch, and capped interactions keep the full order
+    ("ring-power-rel-1e-10-mac0.55", lambda: W.vortex_ring(8000, seed=38), O.KIND_VORTEX,
+     F.PowerRelativeGradient(1e-10), 64, 0.55),
+    ("ring-power-rel-1e-12-mac0.55", lambda: W.vortex_ring(6000, seed=39), O.KIND_VORTEX,
+     F.PowerRelativeGradient(1e-12), 48, 0.55),
 ]
 
+# an order decision whose estimate lies this close (relative) to the tolerance may flip under
+# last-bit differences of the rotated coefficients; every other interaction must pick the oracle's order
+MARGIN = 1e-9
 
-@pytest.mark.parametrize("name,gen,kind,em,ls", ERR_CASES, ids=[c[0] for c in ERR_CASES])
-def test_error_controlled_order_matches_oracle(name, gen, kind, em, ls):
+
+def _bodies_under(run, branches):
+    """Original indices of the target bodies inside the given target branches (and so under every
+    L2L from them)."""
+    ints, _ = run.branches(0)
+    si = run.sort_index(0)
+    out = np.zeros(len(si), dtype=bool)
+    for b in set(int(x) for x in branches):
+        out[si[ints[b, 1]:ints[b, 1] + ints[b, 0]]] = True
+    return out
+
+
+@pytest.mark.parametrize("case", ERR_CASES, ids=[c[0] for c in ERR_CASES])
+def test_error_controlled_order_matches_oracle(case):
+    name, gen, kind, em, ls = case[:5]
+    mac = case[5] if len(case) > 5 else 0.5
     _say(f"[gpu] error-controlled {name}")
     sbuf = gen()
-    kw = dict(expansion_order=20, leaf_size_source=ls, multipole_acceptance=0.5, error_tolerance=em,
-              scalar_potential=(em.kind == _lib.ERR_POWER_POTENTIAL))
+    potential = em.kind == _lib.ERR_POWER_POTENTIAL
+    kw = dict(expansion_order=20, leaf_size_source=ls, multipole_acceptance=mac, error_tolerance=em,
+              scalar_potential=potential)
     ev, opt = _gpu_eval(kind, sbuf, **kw)
     st = ev.fmm(opt)
     run = _oracle_run(kind, sbuf, p=20, **{k: v for k, v in kw.items() if k != "expansion_order"}).fmm()
@@ -202,16 +233,32 @@ def test_error_controlled_order_matches_oracle(name, gen, kind, em, ls):
         mi_g = ev.tree(0).reals[:, 14]
         mi_o = run.branches(0)[1][:, 14]
         np.testing.assert_allclose(mi_g, mi_o, rtol=1e-11, atol=0)
-    Pg, Po = ev.m2l_orders(), run.m2l_orders()
+    Pg, Po, margin = ev.m2l_orders(), run.m2l_orders(), run.m2l_margins()
     assert len(Pg) == len(Po)
-    match = float(np.mean(Pg == Po))
-    _say(f"    same order for {match * 100:.4f}% of {len(Po)} M2L interactions; pmax gpu {st.pmax} oracle {run.pmax}")
+    flipped = Pg != Po
+    match = 1.0 - float(np.mean(flipped)) if len(Po) else 1.0
+    _say(f"    same order for {match * 100:.4f}% of {len(Po)} M2L interactions ({int(flipped.sum())} differ, "
+         f"{int((margin < MARGIN).sum())} within {MARGIN:g} of the tolerance); orders >= 19: {int((Po >= 19).sum())}; "
+         f"pmax gpu {st.pmax} oracle {run.pmax}; error_success gpu {st.error_success} oracle {int(run.error_success)}")
+    assert not np.any(flipped & (margin >= MARGIN)), "an order decision away from the tolerance differs"
     assert match >= 0.999
+    assert st.pmax == run.pmax and bool(st.error_success) == run.error_success
+    # outputs: bodies under no flipped interaction's target branch match the oracle to 1e-11; the others
+    # differ by at most the truncation error, bounded by twice the oracle's own error against direct!
     got, ref = ev.targets(), run.targets()
-    lo, hi = (3, 4) if em.kind == _lib.ERR_POWER_POTENTIAL else (4, 7)
+    lo, hi = (3, 4) if potential else (4, 7)
     scale = np.abs(ref[:, lo:hi]).max()
-    if match == 1.0:
-        assert np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / scale < 1e-11
+    moved = _bodies_under(run, run.list(0)[flipped, 0])
+    if np.any(~moved):
+        err = np.abs(got[~moved, lo:hi] - ref[~moved, lo:hi]).max() / scale
+        _say(f"    {int((~moved).sum())} targets under unflipped interactions: max rel diff {err:.3e}")
+        assert err < 1e-11, err
+    if np.any(moved):
+        d = O.direct(kind, sbuf[:, :3], sbuf, scalar_potential=potential)
+        bound = 2 * np.abs(ref[:, lo:hi] - d[:, lo:hi]).max() + 1e-11 * scale
+        dev = np.abs(got[moved, lo:hi] - ref[moved, lo:hi]).max()
+        _say(f"    {int(moved.sum())} targets under flipped interactions: max diff {dev:.3e} (bound {bound:.3e})")
+        assert dev <= bound
 
 
 def test_relative_tolerance_separate_targets_matches_oracle():
@@ -227,8 +274,13 @@ def test_relative_tolerance_separate_targets_matches_oracle():
     ev.fmm(opt)
     run = _oracle_run(O.KIND_VORTEX, sbuf, txyz=txyz, p=20, **{k: v for k, v in kw.items() if k != "expansion_order"}).fmm()
     np.testing.assert_allclose(ev.tree(0).reals[:, 14], run.branches(0)[1][:, 14], rtol=1e-11, atol=0)
-    Pg, Po = ev.m2l_orders(), run.m2l_orders()
-    assert len(Pg) == len(Po) and float(np.mean(Pg == Po)) >= 0.999
+    Pg, Po, margin = ev.m2l_orders(), run.m2l_orders(), run.m2l_margins()
+    assert len(Pg) == len(Po) and not np.any((Pg != Po) & (margin >= MARGIN))
+    _say(f"    same order for {float(np.mean(Pg == Po)) * 100:.4f}% of {len(Po)} M2L interactions")
+    ref = run.targets()
+    err = np.abs(ev.targets()[:, 4:7] - ref[:, 4:7]).max() / np.abs(ref[:, 4:7]).max()
+    if np.all(Pg == Po):
+        assert err < 1e-11, err
 
 
 MFMA_FMM = [c for c in FMM_CASES if c[0] in ("laplace-4k-p8", "vortex-4k-p8", "ring-6k-p10", "ring-3k-p20", "panel-src-tri-3k-p8")]
@@ -242,10 +294,10 @@ def test_fmm_fp64_matrix_core_m2l(name, gen, kind, kw, monkeypatch):
     test_fmm_fp64_matches_oracle(name, gen, kind, kw)
 
 
-@pytest.mark.parametrize("name,gen,kind,em,ls", MFMA_ERR, ids=[c[0] for c in MFMA_ERR])
-def test_error_controlled_matrix_core_m2l(name, gen, kind, em, ls, monkeypatch):
+@pytest.mark.parametrize("case", MFMA_ERR, ids=[c[0] for c in MFMA_ERR])
+def test_error_controlled_matrix_core_m2l(case, monkeypatch):
     monkeypatch.setenv("FMMHIP_M2L_KERNEL", "mfma")
-    test_error_controlled_order_matches_oracle(name, gen, kind, em, ls)
+    test_error_controlled_order_matches_oracle(case)
 
 
 def test_direct_fp64_matches_oracle():
@@ -279,7 +331,7 @@ def test_fmm_api_mirror_and_edge_cases():
     F.direct(sysB, scalar_potential=True)
     rel = np.abs(sysA.potential[:, 0] - sysB.potential[:, 0]).max() / np.abs(sysB.potential[:, 0]).max()
     assert rel < 1e-8
-    # a single body, two coincident bodies, a root leaf, and an empty target set
+    # a single body, two coincident bodies, a root leaf
     for n in (1, 2, 5):
         b = np.zeros((n, 5))
         b[:, 4] = 1.0
@@ -290,6 +342,17 @@ def test_fmm_api_mirror_and_edge_cases():
         got = ev.targets()
         ref = O.direct(O.KIND_SOURCE, b[:, :3], b, scalar_potential=True)
         np.testing.assert_allclose(got[:, 3:7], ref[:, 3:7], rtol=1e-12, atol=1e-14)
+    # an empty target set and an empty source set: nothing to evaluate, no error (src/fmm.jl:1154-1158)
+    b = W.uniform_sources(50, seed=7)
+    ev, opt = _gpu_eval(O.KIND_SOURCE, b, txyz=np.zeros((0, 3)), expansion_order=4, scalar_potential=True)
+    st = ev.fmm(opt)
+    assert ev.targets().shape == (0, 16) and st.n_m2l == 0 and st.n_direct == 0 and st.error_success == 1
+    tx = np.random.default_rng(8).random((40, 3))
+    ev, opt = _gpu_eval(O.KIND_SOURCE, np.zeros((0, 5)), txyz=tx, expansion_order=4, scalar_potential=True)
+    st = ev.fmm(opt)
+    got = ev.targets()
+    np.testing.assert_array_equal(got[:, :3], tx)
+    assert not np.any(got[:, 3:]) and st.n_m2l == 0 and st.n_direct == 0
     # unsupported expansion order is an error, not a fallback
     ev, opt = _gpu_eval(O.KIND_SOURCE, W.uniform_sources(100, seed=6), expansion_order=25)
     with pytest.raises(_lib.FmmHipError):
