@@ -180,17 +180,108 @@ def batch_sample(F, W, _lib, args):
         for rnd in range(2):
             t0 = time.perf_counter()
             tickets = [submit(server, kind, sysm) for kind, sysm in systems]
-            lat = [server.wait(t)[2]["latency_ms"] for t in tickets]
+            sts = [server.wait(t)[2] for t in tickets]
             dt = time.perf_counter() - t0
+            lat = [x["latency_ms"] for x in sts]
+            status = {"ok": sum(x["status"] == _lib.FMMHIP_OK for x in sts),
+                      "tolerance_missed": sum(x["status"] == _lib.FMMHIP_ETOL for x in sts)}
             out = {"systems": len(systems), "bodies": int(sizes.sum()), "seconds": round(dt, 3),
                    "systems_per_s": round(len(systems) / dt, 1), "bodies_per_s": round(float(sizes.sum()) / dt, 1),
-                   "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+                   "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "status": status}
             log(f"batch round {rnd}: {out}")
     out.update({"workload": f"{len(systems)} systems, N log-uniform [1e3, 2e5] (seed 4), even: uniform-cube sources "
                             "(potential + gradient, p 8, MAC 0.5), odd: vortex rings (velocity + gradient, "
                             f"PowerRelativeGradient({args.tol}), p <= {args.pmax}, MAC {args.mac}); mixed form",
                 "slots": args.batch_slots, "tuned_leaf_size": {k: v["leaf_size_source"] for k, v in tuned.items()},
+                "tune_found": {k: bool(v["tuned"]) for k, v in tuned.items()},
                 "tune_seconds": round(t_tune, 2)})
+    return out
+
+
+def precision_sweep(F, W, _lib, args):
+    """Precision sweep: N = 4,194,304 point sources from a Plummer sphere (scale 1, truncated at r = 20),
+    strengths N(0,1); potential + gradient at fixed expansion orders 4 / 8 / 16 (leaf args.sweep_leaf,
+    MAC args.sweep_mac) in each precision form; error of the gradient (and potential) against the FP64
+    O(N^2) direct sum on 10,000 seeded sampled targets; phase times per evaluation (inputs resident in
+    HBM, warm-up evaluation first)."""
+    n = args.sweep_n
+    sbuf = W.plummer_sources(n, seed=3)
+    ctx = F.default_context()
+    ev = F.Evaluation(ctx)
+    ev.set_systems(_lib.POINT_SOURCE, None, sbuf)
+    rng = np.random.default_rng(3)
+    idx = np.sort(rng.choice(n, size=min(args.samples, n), replace=False))
+    txyz = np.ascontiguousarray(sbuf[idx, :3])
+    ref = np.zeros((len(idx), 16))
+    ref[:, :3] = txyz
+    ctx.check(ctx.lib.fmmhip_direct(ctx._h, _lib.POINT_SOURCE, len(idx), txyz.ctypes.data, n,
+                                    np.ascontiguousarray(sbuf).ctypes.data, 1, 1, 0, _lib.FP64, ref.ctypes.data, 0))
+    out = {}
+    for form in args.sweep_forms.split(","):
+        for p in (4, 8, 16):
+            opt = F.make_options(expansion_order=p, leaf_size_source=args.sweep_leaf, multipole_acceptance=args.sweep_mac,
+                                 scalar_potential=True, gradient=True, precision=form)
+            ev.fmm(opt)
+            t0 = time.perf_counter()
+            st = [ev.fmm(opt).as_dict() for _ in range(args.sweep_steps)]
+            dt = (time.perf_counter() - t0) / args.sweep_steps
+            got = ev.targets()[idx]
+            e_g = W.relative_error(got[:, 4:7], ref[:, 4:7])
+            e_p = W.relative_error(got[:, 3:4], ref[:, 3:4])
+            ph = {k: round(float(np.mean([x[k] for x in st])), 3) for k in
+                  ("ms_tree", "ms_lists", "ms_p2p", "ms_upward", "ms_m2l", "ms_downward", "ms_total")}
+            s = st[-1]
+            rec = {"ms_per_eval": round(dt * 1e3, 3), "evals_per_s": round(n / dt, 1),
+                   "rel_err_gradient": {"max": e_g[0], "p99": e_g[1], "median": e_g[2]},
+                   "rel_err_potential": {"max": e_p[0], "p99": e_p[1]},
+                   "phases_ms": ph, "n_m2l": s["n_m2l"], "p2p_pairs": s["n_p2p_pairs"],
+                   "m2l_per_s": round(s["n_m2l"] / (ph["ms_m2l"] * 1e-3), 1) if ph["ms_m2l"] > 0 else None,
+                   "m2l_tflops": round(s["n_m2l"] * m2l_flops(p, lh=False) / (ph["ms_m2l"] * 1e-3) / 1e12, 3)
+                   if ph["ms_m2l"] > 0 else None}
+            out[f"{form}/p{p}"] = rec
+            log(f"sweep {form} p={p}: {rec}")
+    ev.close()
+    return {"workload": f"Plummer sphere N={n} (scale 1, r <= 20), N(0,1) strengths, potential + gradient, "
+                        f"leaf {args.sweep_leaf}, MAC {args.sweep_mac}, fixed p; error vs FP64 direct on "
+                        f"{len(idx)} sampled targets", "results": out}
+
+
+def cpu_baseline(W, args):
+    """The oracle (C++ FP64 restatement of the reference's fmm! path, OpenMP over targets as the
+    reference's multithreaded drivers split the work) on the host cores, on bounded samples of the
+    headline workload (same distribution and settings): all cores on an N = args.cpu_sample ring and one
+    core on an N = args.cpu_sample_1 ring. Protocol as the reference's benchmark script
+    (scripts/benchmark_singlethread_20250402.jl:20-30): one untimed warm-up run, then the median (and
+    min) of args.cpu_runs timed runs."""
+    from oracle import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+
+    def leg(m, threads, runs):
+        cb = W.vortex_ring(m, seed=2)
+        O.threads(threads)
+        ts = []
+        for k in range(runs + 1):
+            t0 = time.perf_counter()
+            O.Run(O.KIND_VORTEX, cb[:, :3], cb, p=args.pmax, leaf_size_source=args.leaf,
+                  multipole_acceptance=args.mac, error_kind=O.ERR_POWER_GRADIENT, relative=True, eps=args.tol,
+                  gradient=True, hessian=True).fmm()
+            if k:
+                ts.append(time.perf_counter() - t0)
+        med = float(np.median(ts))
+        log(f"cpu baseline: {m} bodies on {threads} thread(s): median {med:.2f} s, min {min(ts):.2f} s over {runs}")
+        return {"value": round(m / med, 1), "unit": "evals/s", "cores": threads, "n": m,
+                "median_s": round(med, 3), "min_s": round(min(ts), 3), "runs": runs}
+
+    full = leg(args.cpu_sample, cores, args.cpu_runs)
+    one = leg(args.cpu_sample_1, 1, max(3, args.cpu_runs - 2)) if args.cpu_sample_1 > 0 else None
+    O.threads(cores)
+    out = dict(full)
+    out.update({"kind": "port",
+                "sample": f"oracle (C++ FP64 restatement, OpenMP) full fmm! on a {args.cpu_sample}-particle ring of the "
+                          f"headline distribution and settings on {cores} host threads; warm-up, then the median of "
+                          f"{args.cpu_runs} runs",
+                "single_core": one})
     return out
 
 
@@ -216,7 +307,9 @@ def main():
     ap.add_argument("--mac", type=float, default=0.55)
     ap.add_argument("--pmax", type=int, default=20)
     ap.add_argument("--samples", type=int, default=10000)
-    ap.add_argument("--cpu-sample", type=int, default=100000)
+    ap.add_argument("--cpu-sample", type=int, default=100000, help="ring size of the all-cores CPU baseline")
+    ap.add_argument("--cpu-sample-1", type=int, default=30000, help="ring size of the one-core CPU baseline (0: skip)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU-baseline runs after one warm-up")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--no-serve", action="store_true")
@@ -227,6 +320,12 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=float, default=96.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the 4M Plummer precision sweep")
+    ap.add_argument("--sweep-n", type=int, default=4194304)
+    ap.add_argument("--sweep-leaf", type=int, default=50)
+    ap.add_argument("--sweep-mac", type=float, default=0.5)
+    ap.add_argument("--sweep-steps", type=int, default=2)
+    ap.add_argument("--sweep-forms", default="fp64,mixed")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-slots", type=int, default=8)
     args = ap.parse_args()
@@ -328,22 +427,13 @@ def main():
         r["frac"] = round(r["achieved"] / r["peak"], 4)
     roof, roof_second = roofs
 
+    sweep = None if args.no_sweep else precision_sweep(F, W, _lib, args)
     serving = None if args.no_serve else serving_sample(F, W, _lib, args)
     batch = None if args.no_batch else batch_sample(F, W, _lib, args)
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        m = args.cpu_sample
-        cb = W.vortex_ring(m, seed=2)
-        t0 = time.perf_counter()
-        O.Run(O.KIND_VORTEX, cb[:, :3], cb, p=args.pmax, leaf_size_source=args.leaf, multipole_acceptance=args.mac,
-              error_kind=O.ERR_POWER_GRADIENT, relative=True, eps=args.tol, gradient=True, hessian=True).fmm()
-        tc = time.perf_counter() - t0
-        cpu = {"value": round(m / tc, 1), "unit": "evals/s", "cores": 1, "kind": "port",
-               "sample": f"oracle (C++ FP64 restatement) full fmm! on a {m}-particle ring of the same distribution "
-                         f"and settings, {tc:.1f} s on one host core"}
-        log(f"cpu baseline: {m} bodies in {tc:.2f} s")
+        cpu = cpu_baseline(W, args)
 
     line = {
         "metric": "FMM evals/s at rel.err<=1e-6 (mixed vs FP64 GPU) + p50/p99 request latency",
@@ -380,6 +470,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(M["lat"], 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(M["lat"], 99)) * 1e3, 3),
                        "note": "the headline workload's evaluations issued back to back, one at a time"},
+        "precision_sweep": sweep,
         "serving": serving,
         "batch": batch,
     }

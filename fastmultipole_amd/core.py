@@ -579,7 +579,10 @@ def tune_fmm(target_systems, source_systems=None, *, error_tolerance=None, expan
             print(f"\n\tBest Parameters: \n\t\tleaf_size_source:    {leaf_size_source}\n"
                   f"\t\texpansion_order:     {expansion_order}\n\t\tmultipole_acceptance: {mac}\n"
                   f"\t\tcost:                {t} seconds")
-    tuned = {"leaf_size_source": best[2], "expansion_order": best[1], "multipole_acceptance": best[3]}
+    # tuned is False when every acceptance was skipped (tolerance missed at the maximum order): the
+    # parameters are then the ones passed in, not a measured choice
+    tuned = {"leaf_size_source": best[2], "expansion_order": best[1], "multipole_acceptance": best[3],
+             "tuned": math.isfinite(best[0])}
     if verbose:
         print(f"\nFinished autotune!\n\nParameters: \n\tleaf_size_source:    {best[2]}\n"
               f"\texpansion_order:     {best[1]}\n\tmultipole_acceptance: {best[3]}\n\tcost:                {best[0]} seconds"

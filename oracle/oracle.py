@@ -53,6 +53,8 @@ def lib():
             getattr(L, f).argtypes = [C.c_void_p, C.c_int64, _ip64]
         L.orc_fgs_solve.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, _dp, _dp, _dp]
         L.orc_fgs_solve.restype = C.c_int
+        L.orc_threads.argtypes = [C.c_int]
+        L.orc_threads.restype = C.c_int
         L.orc_run_create.restype = C.c_void_p
         L.orc_run_create.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ic, _dp]
         for f in ("orc_run_free", "orc_run_fmm", "orc_run_trees", "orc_run_lists"):
@@ -104,6 +106,11 @@ def lib():
         L.orc_p2m_panel.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp]
         _LIB = L
     return _LIB
+
+
+def threads(n=0):
+    """Set the oracle's OpenMP thread count (n > 0) and return the count in effect."""
+    return lib().orc_threads(int(n))
 
 
 def nh(p):

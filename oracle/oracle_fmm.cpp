@@ -7,6 +7,7 @@
 // Index convention: branches, bodies and leaves are 0-based here; the
 // reference's 1-based values are these plus one (root parent 0 <-> -1).
 #include "oracle_math.hpp"
+#include <omp.h>
 #include <cstring>
 #include <memory>
 #include <array>
@@ -786,69 +787,136 @@ static void fmm_run(Run& R, bool estimate_pass) {
     tables().ensure(o.p);
     int p = o.p;
     int Hn = n_harmonics(p);
+    // The passes are split over OpenMP threads the way the reference's multithreaded drivers split
+    // them (src/fmm.jl:57-93, 348-438, 492-598, 642-757): by target leaf (nearfield, L2P), by source
+    // leaf (P2M), by parent branch within a level (M2M), by target branch (M2L) and by child branch
+    // within a level (L2L). Every expansion and target value is written by one thread and accumulates
+    // its terms in the serial order, so results are bit-identical for any thread count.
+    // target ranges of the target-sorted lists
+    auto runs_of = [](const std::vector<std::array<int32_t, 2>>& L) {
+        std::vector<size_t> r;
+        for (size_t k = 0; k < L.size(); ++k)
+            if (k == 0 || L[k][0] != L[k - 1][0]) r.push_back(k);
+        r.push_back(L.size());
+        return r;
+    };
     // nearfield (src/fmm.jl:3-40)
-    for (auto& pr : R.direct) {
-        const OBranch& s = R.st.branches[pr[1]];
-        const OBranch& t = R.tt.branches[pr[0]];
-        user_direct(R.kind, R.tt.buffer.data(), t.first, t.n_bodies, R.st.buffer.data(), dpb, s.first, s.n_bodies, o.PS, o.GS, o.HS);
+    {
+        const std::vector<size_t> rd = runs_of(R.direct);
+        const int64_t nr = (int64_t)rd.size() - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t g = 0; g < nr; ++g)
+            for (size_t k = rd[g]; k < rd[g + 1]; ++k) {
+                const auto& pr = R.direct[k];
+                const OBranch& s = R.st.branches[pr[1]];
+                const OBranch& t = R.tt.branches[pr[0]];
+                user_direct(R.kind, R.tt.buffer.data(), t.first, t.n_bodies, R.st.buffer.data(), dpb, s.first, s.n_bodies,
+                            o.PS, o.GS, o.HS);
+            }
     }
-    Scratch S;
-    S.ensure(p);
-    std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
     // upward pass: P2M (src/bodytomultipole.jl:19-26) then M2M (src/fmm.jl:262-284)
-    for (int64_t il : R.st.leaf_index) {
-        const OBranch& b = R.st.branches[il];
-        Exp M{&R.st.expansions[(size_t)il * 4 * Hn]};
-        for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
-            const double* x = &R.st.buffer[(size_t)i * dpb];
-            double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
-            if (is_panel(R.kind)) p2m_panel(R.kind, M, Exp{harm.data()}, x, b.source_center, p);
-            else if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
-            else if (R.kind == KIND_DIPOLE) p2m_point_dipole(M, Exp{harm.data()}, dx, x + 4, p);
-            else p2m_point_vortex(M, Exp{harm.data()}, dx, x + 4, p);
+    const int64_t nsl = (int64_t)R.st.leaf_index.size();
+#pragma omp parallel
+    {
+        std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t k = 0; k < nsl; ++k) {
+            const int64_t il = R.st.leaf_index[k];
+            const OBranch& b = R.st.branches[il];
+            Exp M{&R.st.expansions[(size_t)il * 4 * Hn]};
+            for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+                const double* x = &R.st.buffer[(size_t)i * dpb];
+                double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
+                if (is_panel(R.kind)) p2m_panel(R.kind, M, Exp{harm.data()}, x, b.source_center, p);
+                else if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
+                else if (R.kind == KIND_DIPOLE) p2m_point_dipole(M, Exp{harm.data()}, dx, x + 4, p);
+                else p2m_point_vortex(M, Exp{harm.data()}, dx, x + 4, p);
+            }
         }
     }
-    for (int64_t ib = (int64_t)R.st.branches.size() - 1; ib >= 0; --ib) {
-        const OBranch& b = R.st.branches[ib];
-        if (b.n_branches != 0)
-            for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
-                multipole_to_multipole(Exp{&R.st.expansions[(size_t)ib * 4 * Hn]}, b.source_center,
-                                       Exp{&R.st.expansions[(size_t)c * 4 * Hn]}, R.st.branches[c].source_center, S, p, LH);
+    for (int64_t lv = (int64_t)R.st.levels.size() - 1; lv >= 0; --lv) {
+        const int64_t b0 = R.st.levels[lv][0], b1 = R.st.levels[lv][1];
+#pragma omp parallel
+        {
+            Scratch S;
+            S.ensure(p);
+#pragma omp for schedule(dynamic, 4)
+            for (int64_t ib = b0; ib <= b1; ++ib) {
+                const OBranch& b = R.st.branches[ib];
+                if (b.n_branches != 0)
+                    for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
+                        multipole_to_multipole(Exp{&R.st.expansions[(size_t)ib * 4 * Hn]}, b.source_center,
+                                               Exp{&R.st.expansions[(size_t)c * 4 * Hn]}, R.st.branches[c].source_center, S,
+                                               p, LH);
+            }
+        }
     }
     // horizontal pass (src/fmm.jl:455-490)
     R.m2l_P.assign(R.m2l.size(), 0);
     R.m2l_margin.assign(R.m2l.size(), INFINITY);
+    std::vector<char> ok_k(R.m2l.size(), 1);
+    {
+        const std::vector<size_t> rm = runs_of(R.m2l);
+        const int64_t nr = (int64_t)rm.size() - 1;
+#pragma omp parallel
+        {
+            Scratch S;
+            S.ensure(p);
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t g = 0; g < nr; ++g)
+                for (size_t k = rm[g]; k < rm[g + 1]; ++k) {
+                    int32_t it = R.m2l[k][0], js = R.m2l[k][1];
+                    bool ok = true;
+                    int P = multipole_to_local(Exp{&R.tt.expansions[(size_t)it * 4 * Hn]}, geom(R.tt.branches[it]),
+                                               Exp{&R.st.expansions[(size_t)js * 4 * Hn]}, geom(R.st.branches[js]), S, p, LH,
+                                               o.em, ok);
+                    R.m2l_P[k] = P;
+                    R.m2l_margin[k] = last_m2l_margin;
+                    ok_k[k] = ok ? 1 : 0;
+                }
+        }
+    }
+    R.Pmax = 0;
     for (size_t k = 0; k < R.m2l.size(); ++k) {
-        int32_t it = R.m2l[k][0], js = R.m2l[k][1];
-        bool ok = true;
-        int P = multipole_to_local(Exp{&R.tt.expansions[(size_t)it * 4 * Hn]}, geom(R.tt.branches[it]),
-                                   Exp{&R.st.expansions[(size_t)js * 4 * Hn]}, geom(R.st.branches[js]), S, p, LH, o.em, ok);
-        R.m2l_P[k] = P;
-        R.m2l_margin[k] = last_m2l_margin;
-        R.Pmax = std::max(P, R.Pmax);
-        R.error_success = R.error_success && ok;
+        R.Pmax = std::max(R.m2l_P[k], R.Pmax);
+        R.error_success = R.error_success && ok_k[k];
     }
     if (!R.error_success) R.Pmax += 1;
     // downward pass: L2L (src/fmm.jl:602-622) then L2P (src/evaluate_expansions.jl:1-43)
-    for (size_t ib = 0; ib < R.tt.branches.size(); ++ib) {
-        const OBranch& b = R.tt.branches[ib];
-        if (b.n_branches > 0)
-            for (int64_t c = b.child_first; c < b.child_first + b.n_branches; ++c)
-                local_to_local(Exp{&R.tt.expansions[(size_t)c * 4 * Hn]}, R.tt.branches[c].target_center,
-                               Exp{&R.tt.expansions[ib * 4 * Hn]}, b.target_center, S, p, LH);
+    for (size_t lv = 1; lv < R.tt.levels.size(); ++lv) {
+        const int64_t c0 = R.tt.levels[lv][0], c1 = R.tt.levels[lv][1];
+#pragma omp parallel
+        {
+            Scratch S;
+            S.ensure(p);
+#pragma omp for schedule(dynamic, 4)
+            for (int64_t c = c0; c <= c1; ++c) {
+                const OBranch& ch = R.tt.branches[c];
+                const int64_t ib = ch.i_parent;
+                local_to_local(Exp{&R.tt.expansions[(size_t)c * 4 * Hn]}, ch.target_center,
+                               Exp{&R.tt.expansions[(size_t)ib * 4 * Hn]}, R.tt.branches[ib].target_center, S, p, LH);
+            }
+        }
     }
-    std::vector<double> gnm;
-    for (int64_t il : R.tt.leaf_index) {
-        const OBranch& b = R.tt.branches[il];
-        Exp L{&R.tt.expansions[(size_t)il * 4 * Hn]};
-        for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
-            double* t = &R.tt.buffer[(size_t)i * 16];
-            double dx[3] = {t[0] - b.target_center[0], t[1] - b.target_center[1], t[2] - b.target_center[2]};
-            double phi, v[3], h[9];
-            evaluate_local(dx, Exp{harm.data()}, gnm, L, p, LH, o.PS, o.GS, o.HS, phi, v, h);
-            if (o.PS) t[3] += phi;
-            if (o.GS) { t[4] += v[0]; t[5] += v[1]; t[6] += v[2]; }
-            if (o.HS) for (int k = 0; k < 9; ++k) t[7 + k] += h[k];
+    const int64_t ntl = (int64_t)R.tt.leaf_index.size();
+#pragma omp parallel
+    {
+        std::vector<double> harm(4 * n_harmonics(p + 2), 0.0);
+        std::vector<double> gnm;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t k = 0; k < ntl; ++k) {
+            const int64_t il = R.tt.leaf_index[k];
+            const OBranch& b = R.tt.branches[il];
+            Exp L{&R.tt.expansions[(size_t)il * 4 * Hn]};
+            for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
+                double* t = &R.tt.buffer[(size_t)i * 16];
+                double dx[3] = {t[0] - b.target_center[0], t[1] - b.target_center[1], t[2] - b.target_center[2]};
+                double phi, v[3], h[9];
+                evaluate_local(dx, Exp{harm.data()}, gnm, L, p, LH, o.PS, o.GS, o.HS, phi, v, h);
+                if (o.PS) t[3] += phi;
+                if (o.GS) { t[4] += v[0]; t[5] += v[1]; t[6] += v[2]; }
+                if (o.HS) for (int k2 = 0; k2 < 9; ++k2) t[7 + k2] += h[k2];
+            }
         }
     }
 }
@@ -1174,6 +1242,11 @@ void* orc_run_create(int kind, int64_t nt, const double* tfull, int64_t ns, cons
 }
 
 void orc_run_free(void* h) { delete (Run*)h; }
+// OpenMP threads of the following calls (0: leave as is); returns the count in effect
+int orc_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 void orc_run_fmm(void* h) { fmm_run(*(Run*)h, false); }
 
 // tree only (no lists, no evaluation): which = 0 target tree, 1 source tree

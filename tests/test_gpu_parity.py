@@ -143,6 +143,14 @@ FMM_CASES = [
      dict(expansion_order=3, leaf_size_source=10, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
     ("gauss-ring-6k-p10", lambda: W.gaussian_vortex_ring(6000, seed=31), O.KIND_VORTEX_GAUSS,
      dict(expansion_order=10, leaf_size_source=64, multipole_acceptance=0.4, hessian=True)),
+    # the precision-sweep distribution: Plummer sphere (scale 1, truncated at r = 20), N(0,1) strengths, at
+    # the sweep's orders 4 / 8 / 16 (deep, unbalanced trees: the core sits many levels below the halo)
+    ("plummer-30k-p4", lambda: W.plummer_sources(30000, seed=81), O.KIND_SOURCE,
+     dict(expansion_order=4, leaf_size_source=40, multipole_acceptance=0.5, scalar_potential=True)),
+    ("plummer-30k-p8", lambda: W.plummer_sources(30000, seed=82), O.KIND_SOURCE,
+     dict(expansion_order=8, leaf_size_source=40, multipole_acceptance=0.5, scalar_potential=True)),
+    ("plummer-30k-p16", lambda: W.plummer_sources(30000, seed=83), O.KIND_SOURCE,
+     dict(expansion_order=16, leaf_size_source=40, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
     # constant-strength panels tiling a sphere (quadrilaterals, and triangles from splitting them)
     ("panel-src-tri-3k-p8", lambda: W.sphere_panels(16, quads=False)[0], O.KIND_PANEL_SOURCE_TRI,
      dict(expansion_order=8, leaf_size_source=20, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
@@ -165,6 +173,9 @@ def test_fmm_fp64_matches_oracle(name, gen, kind, kw):
     run = _oracle_run(kind, sbuf, p=kw["expansion_order"], **{k: v for k, v in kw.items() if k != "expansion_order"}).fmm()
     ref = run.targets()
     assert st.pmax == run.pmax and bool(st.error_success) == run.error_success
+    if name.startswith("plummer"):
+        _compare_trees(ev, run, 0)
+        _compare_trees(ev, run, 1)
     for lo, hi in ((3, 4), (4, 7), (7, 16)):
         scale = np.abs(ref[:, lo:hi]).max()
         if scale == 0:

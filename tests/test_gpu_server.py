@@ -171,3 +171,43 @@ def test_server_reports_missed_tolerance():
         _, strengths, st = server.wait(t_solve)
         assert st["status"] == _lib.FMMHIP_ETOL and st["converged"] == 0 and st["sweeps"] == 1, st
         assert np.all(np.isfinite(strengths))
+
+
+def test_server_batched_uneven_instances_match_oracle():
+    """The batched-instances workload at test size: 32 independent systems with N log-uniform in
+    [1e3, 5e3], even ones uniform-cube sources (potential + gradient, p 8, MAC 0.5), odd ones vortex rings
+    (velocity + gradient, p 10, MAC 0.4), all submitted at once to 8 slots in both precision forms.
+    Every request returns FMMHIP_OK; every FP64 result matches the oracle's fmm! to 1e-11 relative and
+    every mixed-form result (FP32 nearfield) stays within 1e-5 of it."""
+    import math
+    from oracle import oracle as O
+    print("[gpu] server batched uneven instances (32 systems)", flush=True)
+    rng = np.random.default_rng(4)
+    sizes = np.floor(np.exp(rng.uniform(math.log(1e3), math.log(5e3), 32))).astype(int)
+    systems = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            systems.append((_lib.POINT_SOURCE, W.uniform_sources(int(n), seed=500 + i),
+                            dict(expansion_order=8, leaf_size=30, multipole_acceptance=0.5, scalar_potential=True)))
+        else:
+            systems.append((_lib.POINT_VORTEX, W.vortex_ring(int(n), seed=500 + i),
+                            dict(expansion_order=10, leaf_size=40, multipole_acceptance=0.4)))
+    with F.Server(n_slots=8) as server:
+        tickets = [(i, prec, server.submit_fmm(_Rows(k, rows), precision=prec, **kw))
+                   for prec in ("fp64", "mixed") for i, (k, rows, kw) in enumerate(systems)]
+        results = {}
+        for i, prec, t in tickets:
+            targets, _, st = server.wait(t)
+            assert st["status"] == _lib.FMMHIP_OK and st["error_success"] == 1, (i, prec, st)
+            results[(i, prec)] = targets
+    for i, (k, rows, kw) in enumerate(systems):
+        okind = O.KIND_SOURCE if k == _lib.POINT_SOURCE else O.KIND_VORTEX
+        ref = O.Run(okind, rows[:, :3], rows, p=kw["expansion_order"], leaf_size_source=kw["leaf_size"],
+                    multipole_acceptance=kw["multipole_acceptance"],
+                    scalar_potential=kw.get("scalar_potential", False)).fmm().targets()
+        lo, hi = 4, 7
+        sc = np.abs(ref[:, lo:hi]).max()
+        e64 = np.abs(results[(i, "fp64")][:, lo:hi] - ref[:, lo:hi]).max() / sc
+        e32 = np.abs(results[(i, "mixed")][:, lo:hi] - ref[:, lo:hi]).max() / sc
+        assert e64 < 1e-11, (i, e64)
+        assert e32 < 1e-5, (i, e32)
