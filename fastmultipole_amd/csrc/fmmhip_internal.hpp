@@ -116,6 +116,7 @@ struct Context {
     DBuf<int32_t> m2l_flags;    // pmax and success of the last M2L pass
     DBuf<double> l2p_coef;      // per-leaf local-to-body coefficients of the last L2P pass
     DBuf<unsigned long long> p2p_queue;   // work counter of the persistent nearfield launch
+    DBuf<double> direct_part;   // chunk sums of the split direct! form
     int n_cu = 0;               // compute units of the device (0 until first needed)
     DBuf<unsigned char> temp;   // scratch for sorts/scans
     void* pinned = nullptr;     // 64 bytes of pinned host memory for scalar read-backs
@@ -205,8 +206,12 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
              bool PS, bool GS, bool HS, int precision,
              int64_t b0 = 0, int64_t b1 = -1, int slots = 0);
+// part: workspace of direct_part_bytes() for the chunk sums of the split form (null: the context's own)
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
-                    double* tout_aos, bool PS, bool GS, bool HS, int precision);
+                    double* tout_aos, bool PS, bool GS, bool HS, int precision, double* part = nullptr);
+// source chunks of direct! over nt targets and ns sources, and the bytes of their partial sums (0: no workspace)
+int direct_split(int64_t nt, int64_t ns);
+size_t direct_part_bytes(int kernel, int64_t nt, int64_t ns);
 // expansions.hip
 void ensure_tables(Context& C, int p);
 void free_tables(Tables* t);

@@ -567,6 +567,19 @@ void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists
 }
 
 // ------------------------------------------------------ direct!, all pairs
+// folded sums of one target -> its 16-entry AoS record (accumulated)
+template <int KERNEL, class T>
+__device__ inline void direct_store(double* o, const T* a, int PS, int GS, int HS) {
+    if (KERNEL == FMMHIP_POINT_SOURCE) {
+        o[3] += a[0];          // the reference kernel ignores the switches (test/gravitational.jl:92-98)
+        o[4] += a[1]; o[5] += a[2]; o[6] += a[3];
+    } else {
+        if (PS && KernelTraits<KERNEL>::POT) o[3] += a[0];
+        if (GS) { o[4] += a[1]; o[5] += a[2]; o[6] += a[3]; }
+        if (HS) for (int k = 4; k < 13; ++k) o[3 + k] += a[k];
+    }
+}
+
 // original-order AoS inputs; output accumulated into a 16 x nt AoS buffer
 template <int KERNEL, typename R>
 __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restrict__ txyz, int64_t nt,
@@ -604,23 +617,111 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restri
     }
     if (!active) return;
     fold_acc<KERNEL, true, R>(a);
-    double* o = out + it * 16;
-    if (KERNEL == FMMHIP_POINT_SOURCE) {
-        o[3] += a[0];          // the reference kernel ignores the switches (test/gravitational.jl:92-98)
-        o[4] += a[1]; o[5] += a[2]; o[6] += a[3];
-    } else {
-        if (PS && KernelTraits<KERNEL>::POT) o[3] += a[0];
-        if (GS) { o[4] += a[1]; o[5] += a[2]; o[6] += a[3]; }
-        if (HS) for (int k = 4; k < 13; ++k) o[3 + k] += a[k];
+    direct_store<KERNEL>(out + it * 16, a, PS, GS, HS);
+}
+
+// Few targets against many sources: the sources are split into S chunks and workgroup (b, c)
+// evaluates target block b against chunk c, writing its folded sums to part[(c * NOUT + k) * nt + it]
+// (every entry for it < nt written, nothing accumulated); k_direct_reduce then adds the chunks'
+// sums per target in chunk order. One workgroup per target block would leave most of the device
+// idle at, e.g., 10,000 sampled targets.
+template <int KERNEL, typename R>
+__global__ __launch_bounds__(P2P_BLOCK) void k_direct_part(const double* __restrict__ txyz, int64_t nt,
+                                                           const double* __restrict__ sbuf, int rows, int64_t ns,
+                                                           int64_t chunk, double* __restrict__ part) {
+    constexpr int NS = KernelTraits<KERNEL>::NS;
+    constexpr int NOUT = KernelTraits<KERNEL>::NOUT;
+    __shared__ double lx[P2P_BLOCK], ly[P2P_BLOCK], lz[P2P_BLOCK];
+    __shared__ R lg[P2P_BLOCK][NS];
+    const int64_t it = blockIdx.x * (int64_t)P2P_BLOCK + threadIdx.x;
+    const bool active = it < nt;
+    const double xt = active ? txyz[3 * it] : 0.0, yt = active ? txyz[3 * it + 1] : 0.0, zt = active ? txyz[3 * it + 2] : 0.0;
+    constexpr int NA = KernelTraits<KERNEL>::template na<R>();
+    R a[NA];
+    for (int k = 0; k < NA; ++k) a[k] = R(0);
+    const int64_t c0 = (int64_t)blockIdx.y * chunk, c1 = std::min<int64_t>(ns, c0 + chunk);
+    for (int64_t s0 = c0; s0 < c1; s0 += P2P_BLOCK) {
+        const int64_t cnt = std::min<int64_t>(P2P_BLOCK, c1 - s0);
+        __syncthreads();
+        if ((int64_t)threadIdx.x < cnt) {
+            const double* b = sbuf + (s0 + threadIdx.x) * rows;
+            lx[threadIdx.x] = b[0]; ly[threadIdx.x] = b[1]; lz[threadIdx.x] = b[2];
+            lg[threadIdx.x][0] = (R)(b[4] * INV4PI);
+            if (NS >= 3) { lg[threadIdx.x][1] = (R)(b[5] * INV4PI); lg[threadIdx.x][2] = (R)(b[6] * INV4PI); }
+            if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) lg[threadIdx.x][3] = (R)b[3];
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int s = 0; s < cnt; ++s) {
+            const R dx = (R)(xt - lx[s]), dy = (R)(yt - ly[s]), dz = (R)(zt - lz[s]);
+            const R r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 > R(0))) continue;
+            R ri;
+            if constexpr (sizeof(R) == 8) ri = rinv_f64(r2); else ri = __builtin_amdgcn_rsqf(r2);
+            pair_eval<KERNEL, true, true, true, R>(dx, dy, dz, r2, ri, lg[s], a);
+        }
     }
+    if (!active) return;
+    fold_acc<KERNEL, true, R>(a);
+    double* pc = part + (int64_t)blockIdx.y * NOUT * nt + it;
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) pc[(int64_t)k * nt] = (double)a[k];
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(P2P_BLOCK) void k_direct_reduce(const double* __restrict__ part, int S, int64_t nt,
+                                                             double* __restrict__ out, int PS, int GS, int HS) {
+    constexpr int NOUT = KernelTraits<KERNEL>::NOUT;
+    const int64_t it = blockIdx.x * (int64_t)P2P_BLOCK + threadIdx.x;
+    if (it >= nt) return;
+    double a[NOUT];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) a[k] = 0.0;
+    for (int c = 0; c < S; ++c) {
+        const double* pc = part + (int64_t)c * NOUT * nt + it;
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) a[k] += pc[(int64_t)k * nt];
+    }
+    direct_store<KERNEL>(out + it * 16, a, PS, GS, HS);
+}
+
+// source chunks of the split direct! sum: 1 when the target blocks alone fill the device
+// (>= 1024 workgroups); otherwise about 2048 workgroups in all, each chunk at least four source
+// tiles. S * nt <= 2048 * P2P_BLOCK, so the partial sums need at most 13 * 8 * 524288 bytes.
+int direct_split(int64_t nt, int64_t ns) {
+    const int64_t tb = (nt + P2P_BLOCK - 1) / P2P_BLOCK;
+    if (tb >= 1024 || nt == 0) return 1;
+    int64_t S = 2048 / tb;
+    S = std::min<int64_t>(S, std::max<int64_t>(1, ns / (4 * P2P_BLOCK)));
+    return (int)std::max<int64_t>(1, S);
+}
+
+size_t direct_part_bytes(int kernel, int64_t nt, int64_t ns) {
+    if (is_panel(kernel)) return 0;
+    const int S = direct_split(nt, ns);
+    if (S <= 1) return 0;
+    const int nout = kernel == FMMHIP_POINT_SOURCE ? KernelTraits<FMMHIP_POINT_SOURCE>::NOUT : 13;
+    return sizeof(double) * (size_t)S * (size_t)nout * (size_t)nt;
 }
 
 template <int K>
 static void launch_direct_all(Context& C, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf, double* tout_aos,
-                              bool PS, bool GS, bool HS, int precision) {
+                              bool PS, bool GS, bool HS, int precision, double* part) {
     const unsigned grid = (unsigned)((nt + P2P_BLOCK - 1) / P2P_BLOCK);
     const int rows = data_per_body(K);
-    if (precision == FMMHIP_FP64)
+    const int S = direct_split(nt, ns);
+    if (S > 1) {
+        int64_t chunk = (ns + S - 1) / S;
+        chunk = (chunk + P2P_BLOCK - 1) / P2P_BLOCK * P2P_BLOCK;
+        const int Sr = (int)((ns + chunk - 1) / chunk);   // chunks that hold sources
+        const dim3 g2(grid, (unsigned)Sr);
+        if (precision == FMMHIP_FP64)
+            k_direct_part<K, double><<<g2, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, chunk, part);
+        else
+            k_direct_part<K, float><<<g2, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, chunk, part);
+        LAUNCH_CHECK();
+        k_direct_reduce<K><<<grid, P2P_BLOCK, 0, C.stream>>>(part, Sr, nt, tout_aos, PS, GS, HS);
+    } else if (precision == FMMHIP_FP64)
         k_direct_all<K, double><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
     else
         k_direct_all<K, float><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, rows, ns, tout_aos, PS, GS, HS);
@@ -628,14 +729,27 @@ static void launch_direct_all(Context& C, int64_t nt, const double* t_xyz, int64
 }
 
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf, double* tout_aos,
-                    bool PS, bool GS, bool HS, int precision) {
+                    bool PS, bool GS, bool HS, int precision, double* part) {
     if (nt == 0 || ns == 0) return;
     if (is_panel(kernel)) { run_direct_panels(C, kernel, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS); return; }
+    const size_t need = direct_part_bytes(kernel, nt, ns);
+    if (need && !part) {   // the context's workspace for the chunk sums
+        C.direct_part.resize(need / sizeof(double));
+        part = C.direct_part.data();
+    }
     switch (kernel) {
-        case FMMHIP_POINT_SOURCE: launch_direct_all<FMMHIP_POINT_SOURCE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
-        case FMMHIP_POINT_VORTEX: launch_direct_all<FMMHIP_POINT_VORTEX>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
-        case FMMHIP_POINT_DIPOLE: launch_direct_all<FMMHIP_POINT_DIPOLE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
-        default: launch_direct_all<FMMHIP_VORTEX_GAUSSIAN>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision); break;
+        case FMMHIP_POINT_SOURCE:
+            launch_direct_all<FMMHIP_POINT_SOURCE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);
+            break;
+        case FMMHIP_POINT_VORTEX:
+            launch_direct_all<FMMHIP_POINT_VORTEX>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);
+            break;
+        case FMMHIP_POINT_DIPOLE:
+            launch_direct_all<FMMHIP_POINT_DIPOLE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);
+            break;
+        default:
+            launch_direct_all<FMMHIP_VORTEX_GAUSSIAN>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);
+            break;
     }
 }
 
