@@ -24,7 +24,7 @@ EXPORTED = [
     "fmmhip_eval_build_lists", "fmmhip_eval_run", "fmmhip_fmm", "fmmhip_eval_get_targets", "fmmhip_eval_n_branches",
     "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_get_branches", "fmmhip_eval_get_levels",
     "fmmhip_eval_get_leaf_index", "fmmhip_eval_get_sort_index", "fmmhip_eval_n_list", "fmmhip_eval_get_list",
-    "fmmhip_eval_get_m2l_orders", "fmmhip_eval_get_expansions", "fmmhip_direct",
+    "fmmhip_eval_get_m2l_orders", "fmmhip_eval_get_expansions", "fmmhip_direct", "fmmhip_direct_noalloc",
     "fmmhip_fgs_setup", "fmmhip_fgs_solve", "fmmhip_fgs_n_leaves", "fmmhip_fgs_leaf_info", "fmmhip_fgs_get_matrix",
     "fmmhip_fgs_nonself_rows",
     "fmmhip_server_create", "fmmhip_server_destroy", "fmmhip_server_submit", "fmmhip_server_wait",
@@ -133,6 +133,7 @@ def load():
     L.fmmhip_eval_get_m2l_orders.argtypes = [vp, dp]
     L.fmmhip_eval_get_expansions.argtypes = [vp, i32, dp]
     L.fmmhip_direct.argtypes = [vp, i32, i64, dp, i64, dp, i32, i32, i32, i32, dp, i32]
+    L.fmmhip_direct_noalloc.argtypes = [vp, i32, i64, dp, i64, dp, i32, i32, i32, i32, dp]
     L.fmmhip_fgs_setup.argtypes = [vp, C.POINTER(Options), i32]
     L.fmmhip_fgs_solve.argtypes = [vp, dp, i32, i32, C.c_double, dp, C.POINTER(i32), dp]
     L.fmmhip_fgs_n_leaves.argtypes = [vp]

@@ -610,6 +610,28 @@ fmmhip_status fmmhip_direct(fmmhip_ctx* h, int32_t kernel, int64_t nt, const dou
 }
 
 
+// The allocation-free form of fmmhip_direct: device pointers only, and the chunk sums go to the
+// library's static per-device workspace instead of memory allocated here. No hipMalloc / hipFree.
+// Calls that overlap on one device share that workspace and race.
+fmmhip_status fmmhip_direct_noalloc(fmmhip_ctx* h, int32_t kernel, int64_t nt, const double* t_xyz, int64_t ns,
+                                    const double* s_buf, int32_t PS, int32_t GS, int32_t HS, int32_t precision,
+                                    double* out) {
+    if (!h) return FMMHIP_EINVAL;
+    Context* C = &h->c;
+    ABI_TRY(C, {
+        if (!valid_kernel(kernel)) throw Error(FMMHIP_EINVAL, "unknown kernel");
+        if (nt < 0 || ns < 0 || (nt && (!t_xyz || !out)) || (ns && !s_buf)) throw Error(FMMHIP_EINVAL, "bad direct arguments");
+        if (precision != FMMHIP_FP64 && precision != FMMHIP_MIXED) throw Error(FMMHIP_EINVAL, "unknown precision");
+        const size_t need = direct_part_bytes(kernel, nt, ns);
+        if (need > direct_scratch_bytes())
+            throw Error(FMMHIP_EINVAL, "direct_noalloc needs " + std::to_string(need) + " bytes of workspace; the static "
+                                       "workspace holds " + std::to_string(direct_scratch_bytes()));
+        double* part = need ? direct_scratch(C->device) : nullptr;
+        run_direct_all(*C, kernel, nt, t_xyz, ns, s_buf, out, PS, GS, HS, precision, part);
+        HIP_CHECK(hipStreamSynchronize(C->stream));
+    });
+}
+
 // --------------------------------------------------------- FastGaussSeidel
 fmmhip_status fmmhip_fgs_setup(fmmhip_eval* ev, const fmmhip_options* opt, int32_t influence) {
     if (!ev) return FMMHIP_EINVAL;

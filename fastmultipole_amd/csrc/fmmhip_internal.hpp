@@ -212,6 +212,10 @@ void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int
 // source chunks of direct! over nt targets and ns sources, and the bytes of their partial sums (0: no workspace)
 int direct_split(int64_t nt, int64_t ns);
 size_t direct_part_bytes(int kernel, int64_t nt, int64_t ns);
+// device address of the library's static direct! workspace on the current device (looked up once per
+// device) and its size in bytes
+double* direct_scratch(int device);
+size_t direct_scratch_bytes();
 // expansions.hip
 void ensure_tables(Context& C, int p);
 void free_tables(Tables* t);

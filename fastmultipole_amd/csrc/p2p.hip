@@ -23,6 +23,8 @@
 // with Kahan compensation.
 // Strengths are pre-scaled by 1/(4 pi).
 #include "fmmhip_internal.hpp"
+#include <map>
+#include <mutex>
 #include <cstdlib>
 #include <algorithm>
 
@@ -702,6 +704,34 @@ size_t direct_part_bytes(int kernel, int64_t nt, int64_t ns) {
     if (S <= 1) return 0;
     const int nout = kernel == FMMHIP_POINT_SOURCE ? KernelTraits<FMMHIP_POINT_SOURCE>::NOUT : 13;
     return sizeof(double) * (size_t)S * (size_t)nout * (size_t)nt;
+}
+
+// Static workspace of the allocation-free direct! entry point (fmmhip_direct_noalloc): the largest
+// chunk-sum buffer direct_split can ask for, 13 sums x 2048 x P2P_BLOCK targets. Every GPU the
+// library runs on holds one copy until the process ends. In host code `scratch` names a host-side
+// stand-in, not device memory: kernels and copies take the address hipGetSymbolAddress returns for
+// the current device, which direct_scratch looks up once per device and keeps.
+static constexpr size_t DIRECT_SCRATCH_DOUBLES = (size_t)13 * 2048 * P2P_BLOCK;
+static __device__ __align__(256) double scratch[DIRECT_SCRATCH_DOUBLES];
+
+size_t direct_scratch_bytes() { return sizeof(double) * DIRECT_SCRATCH_DOUBLES; }
+
+double* direct_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, double*> ptrs;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(device);
+    if (it != ptrs.end()) return it->second;
+    int cur = -1;
+    HIP_CHECK(hipGetDevice(&cur));
+    if (cur != device) HIP_CHECK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HIP_CHECK(hipSetDevice(cur));
+    HIP_CHECK(e);
+    if (!p) throw Error(FMMHIP_EDEVICE, "hipGetSymbolAddress returned no address for the direct! workspace");
+    ptrs[device] = (double*)p;
+    return (double*)p;
 }
 
 template <int K>

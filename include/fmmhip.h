@@ -169,6 +169,18 @@ fmmhip_status fmmhip_direct(fmmhip_ctx* ctx, int32_t kernel, int64_t n_targets, 
                             int32_t gradient, int32_t hessian, int32_t precision, double* target_out,
                             int32_t on_device);
 
+/* Allocation-free direct!: no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as fmmhip_direct with on_device = 1 (target_xyz, source_buffer and
+ * target_out are device pointers); it calls no hipMalloc, hipFree or allocating routine. Few
+ * targets against many sources are split into source chunks whose sums go to a static workspace of
+ * the library (52 MiB per device, held until the process ends); the call fails with FMMHIP_EINVAL if
+ * a size would need more (it cannot with the library's chunk choice). Calls that overlap on one
+ * device share that workspace and race: serialise them. */
+fmmhip_status fmmhip_direct_noalloc(fmmhip_ctx* ctx, int32_t kernel, int64_t n_targets, const double* target_xyz,
+                                    int64_t n_sources, const double* source_buffer, int32_t scalar_potential,
+                                    int32_t gradient, int32_t hessian, int32_t precision, double* target_out);
+
 /* ---- FastGaussSeidel (src/solve.jl) for one system that is both target and source
  * (fmmhip_eval_set_systems with target_xyz = NULL): point sources or source panels.
  * influence: the value matched per body, FMMHIP_INFLUENCE_POTENTIAL (the scalar potential,
