@@ -23,6 +23,10 @@
 // expansion order follow the reference's arithmetic; the rotation and
 // translation products use explicit fma.
 #include "fmmhip_internal.hpp"
+#include "tables_host.hpp"
+#include "m2l_mx.hpp"
+#include <cstddef>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
@@ -57,129 +61,17 @@ struct Tables {
     DBuf<double> tfac;   // per degree n: 1/(n-1)! and (n+1)! by the error tests' running loops
 };
 
-// Host computation of the pi/2 Wigner tables, zeta/eta magnitudes and power
-// normalisations (src/rotate.jl:95-223, 625-1004; src/translate.jl:366-448),
-// in the reference's operation order.
-static std::vector<double> host_pnms(int p, double s, double c) {
-    std::vector<double> P(nh(p), 0.0);
-    double Pn0 = 1.0, fact = 1.0;
-    for (int m = 0; m <= p; ++m) {
-        double Pnm = Pn0;
-        P[(m * (m + 1)) / 2 + m] = Pnm;
-        double t1 = Pnm;
-        Pnm = c * (double)(2 * m + 1) * t1;
-        for (int n = m + 1; n <= p; ++n) {
-            P[(n * (n + 1)) / 2 + m] = Pnm;
-            double t2 = t1;
-            t1 = Pnm;
-            Pnm = (c * (double)(2 * n + 1) * t1 - (double)(n + m) * t2) / (double)(n - m + 1);
-        }
-        Pn0 = -Pn0 * fact * s;
-        fact += 2;
-    }
-    return P;
-}
-
-static void host_Hs(std::vector<double>& Hs, int pmax) {
-    const double beta = PI_D / 2;
-    const double sb = std::sin(beta), cb = std::cos(beta);
-    std::vector<double> P = host_pnms(pmax + 1, sb, cb);
-    auto Pn = [&](int n, int m) { return P[(n * (n + 1)) / 2 + m]; };
-    std::vector<double> Hn1(pmax + 3), b(2 * (pmax + 1) + 2), d(pmax + 2);
-    Hs.assign(len_Hs(pmax), 0.0);
-    Hs[0] = 1.0;
-    for (int n = 1; n <= pmax; ++n) {
-        double* H = &Hs[len_Hs(n - 1)];
-        auto Hat = [&](int mp, int m) -> double& { return H[(m * (m + 1)) / 2 + mp]; };
-        double s = 1.0, f = 1.0;
-        for (int m = 0; m <= n; ++m) { Hat(0, m) = s * f * Pn(n, m); s = -s; f /= std::sqrt((double)((n + m + 1) * (n - m))); }
-        s = 1.0; f = 1.0;
-        for (int m = 0; m <= n + 1; ++m) { Hn1[m] = s * f * Pn(n + 1, m); s = -s; f /= std::sqrt((double)((n + 1 - m) * (n + m + 2))); }
-        const int bn = n + 1;
-        for (int m = -bn; m <= -1; ++m)
-            b[bn + m] = -std::sqrt((double)(bn - m - 1) / (double)(2 * bn - 1) * (double)(bn - m) / (double)(2 * bn + 1));
-        for (int m = 0; m <= bn; ++m)
-            b[bn + m] = std::sqrt((double)(bn - m - 1) / (double)(2 * bn - 1) * (double)(bn - m) / (double)(2 * bn + 1));
-        const double omc = (1.0 - cb) / 2, opc = (1.0 + cb) / 2;
-        for (int m = 1; m <= n; ++m) {   // b at 1-based index k is b[k-1]
-            double bmm1 = b[bn - m - 1], bm1 = b[bn + m - 1], b0 = b[bn];
-            double anm = std::sqrt((double)(n + 1 + m) / (double)(2 * n + 1) * (double)(n + 1 - m) / (double)(2 * n + 3));
-            Hat(1, m) = (bmm1 * omc * Hn1[m + 1] - bm1 * opc * Hn1[m - 1] - anm * sb * Hn1[m]) / b0;
-        }
-        for (int m = 0; m <= n; ++m) d[m] = std::sqrt((double)((n - m) * (n + m + 1))) / 2;
-        auto D = [&](int k) { return d[k - 1]; };
-        for (int mp = 1; mp <= n - 1; ++mp) {
-            for (int m = mp + 1; m <= n - 1; ++m)
-                Hat(mp + 1, m) = (D(mp) * Hat(mp - 1, m) - D(m) * Hat(mp, m - 1) + D(m + 1) * Hat(mp, m + 1)) / D(mp + 1);
-            int m = n;
-            Hat(mp + 1, m) = (D(mp) * Hat(mp - 1, m) - D(m) * Hat(mp, m - 1)) / D(mp + 1);
-        }
-    }
-}
-
-static void host_mags(std::vector<double>& z, std::vector<double>& e, int pmax) {
-    z.assign(len_Ts(pmax), 0.0);
-    e.assign(len_Ts(pmax), 0.0);
-    z[0] = e[0] = 1.0;
-    for (int n = 1; n <= pmax; ++n) {
-        double* zz = &z[len_Ts(n - 1)];
-        double* ee = &e[len_Ts(n - 1)];
-        double zm = 1.0, em = 1.0;
-        int i = 0;
-        for (int m = 0; m <= n; ++m) {
-            double zmp = zm, emp = em;
-            for (int mp = 0; mp <= n; ++mp) {
-                zz[i] = zmp;
-                ee[i] = emp;
-                int a = (mp + 1 < n) ? n - mp : 1;
-                zmp *= std::sqrt((double)(n + mp + 1) / (double)a);
-                emp *= std::sqrt((double)a / (double)(n + mp + 1));
-                ++i;
-            }
-            int a = (m < n - 1) ? n - m : 1;
-            zm *= std::sqrt((double)a / (double)(n + m + 1));
-            em *= std::sqrt((double)(n + m + 1) / (double)a);
-        }
-    }
-}
-
-// k! correctly rounded to double (exact integer product, one rounding)
-static double host_factorial(int k) {
-    std::vector<uint32_t> limbs(1, 1u);
-    for (int f = 2; f <= k; ++f) {
-        uint64_t carry = 0;
-        for (auto& l : limbs) { uint64_t v = (uint64_t)l * (uint64_t)f + carry; l = (uint32_t)v; carry = v >> 32; }
-        if (carry) limbs.push_back((uint32_t)carry);
-    }
-    int top = (int)limbs.size() - 1;
-    int nbits = 32 * top + (32 - __builtin_clz(limbs[top]));
-    if (nbits <= 53) { double r = 0; for (int i = top; i >= 0; --i) r = r * 4294967296.0 + limbs[i]; return r; }
-    auto bit = [&](int b) { return (int)((limbs[b / 32] >> (b % 32)) & 1u); };
-    uint64_t lead = 0;
-    for (int b2 = nbits - 1; b2 >= nbits - 64; --b2) lead = (lead << 1) | (uint64_t)bit(b2);
-    bool sticky = false;
-    for (int b2 = nbits - 65; b2 >= 0 && !sticky; --b2) sticky = bit(b2);
-    uint64_t mant = lead >> 11, rem = lead & 0x7FF;
-    if (rem > 0x400 || (rem == 0x400 && (sticky || (mant & 1)))) mant += 1;
-    return std::ldexp((double)mant, nbits - 53);
-}
-
 void ensure_tables(Context& C, int p) {
+    using namespace tabh;
     if (p > PMAX) throw Error(FMMHIP_EINVAL, "expansion order above 30 is not supported");
     if (!C.tables) C.tables = new Tables();
     Tables& T = *C.tables;
     if (T.p >= PMAX) return;
     const int pm = PMAX;
-    std::vector<double> Hs, z, e, Mt(nh(pm)), Lt(nh(pm)), invf(2 * pm + 4), fct(2 * pm + 4);
+    std::vector<double> Hs, z, e, Mt, Lt, invf(2 * pm + 4), fct(2 * pm + 4);
     host_Hs(Hs, pm);
     host_mags(z, e, pm);
-    Mt[0] = Lt[0] = 1.0;
-    for (int n = 1, i = 1; n <= pm; ++n)
-        for (int m = 0; m <= n; ++m, ++i) {
-            double a = host_factorial(n + m), b = host_factorial(n - m);
-            Mt[i] = std::sqrt(a * b / (double)(2 * n + 1));
-            Lt[i] = std::sqrt(1.0 / (a * b * (double)(2 * n + 1)));
-        }
+    host_power_norms(Mt, Lt, pm);
     for (int k = 0; k < (int)invf.size(); ++k) {
         fct[k] = host_factorial(k);
         invf[k] = 1.0 / fct[k];
@@ -188,29 +80,7 @@ void ensure_tables(Context& C, int p) {
         d.resize(h.size());
         HIP_CHECK(hipMemcpy(d.data(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     };
-    // dense symmetric copy: A_n(a, b) = Hs_n(min(a, b), max(a, b)), rows padded to an even length
-    std::vector<double> Ad;
-    for (int n = 0; n <= pm; ++n) {
-        const int s = (n + 2) & ~1;
-        const double* Hn = &Hs[((n) * (n + 1) * (n + 2)) / 6];
-        for (int a = 0; a <= n; ++a)
-            for (int b = 0; b < s; ++b) {
-                const int lo = std::min(a, b), hi = std::max(a, b);
-                Ad.push_back(b <= n ? Hn[(hi * (hi + 1)) / 2 + lo] : 0.0);
-            }
-    }
-    // 1/(n-1)! and (n+1)! as the error tests' running loops form them (same operations, so the
-    // same doubles): start 1 and 2, then divide by k and multiply by k + 1 for k = 1..n-1
-    std::vector<double> tf(2 * (pm + 1), 0.0);
-    for (int n = 1; n <= pm; ++n) {
-        double a = 1.0, b = 2.0;
-        for (int k = 1; k < n; ++k) {
-            a /= (double)k;
-            b *= (double)(k + 1);
-        }
-        tf[2 * n] = a;
-        tf[2 * n + 1] = b;
-    }
+    const std::vector<double> Ad = host_dense_A(Hs, pm), tf = host_test_factorials(pm);
     up(T.Hs, Hs); up(T.zeta, z); up(T.eta, e); up(T.Mt, Mt); up(T.Lt, Lt); up(T.inv_fact, invf); up(T.fact, fct); up(T.Ad, Ad);
     up(T.tfac, tf);
     T.p = pm;
@@ -2092,15 +1962,235 @@ static void launch_m2l_lane(Context& C, int p, const Lists& Ls, const Tree& st, 
     LAUNCH_CHECK();
 }
 
-__global__ void k_init_flags(int32_t* pmax, int32_t* ok) { *pmax = 0; *ok = 1; }
+
+// ================================ M2L on matrix cores, register-resident (m2l_mx.hpp)
+// Backend of the engine in m2l_mx.hpp for one gfx950 wavefront: a per-lane value is a register,
+// neighbouring lanes are read with DPP (quad permutations, row rotations), other lanes with
+// ds_bpermute (__shfl), and the product is v_mfma_f64_16x16x4f64.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp((int)u, (int)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(u >> 32), (int)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+static constexpr int DPP_QUAD(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+static constexpr int DPP_ROW_ROR(int n) { return 0x120 + n; }
+
+struct MxGpu {
+    using V = double;
+    using I = int;
+    using M = bool;
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    using PV = const double*;
+    static __device__ __forceinline__ I lane() { return (int)(threadIdx.x & 63); }
+    static __device__ __forceinline__ V sel(M m, V a, V b) { return m ? a : b; }
+    static __device__ __forceinline__ I seli(M m, I a, I b) { return m ? a : b; }
+    static __device__ __forceinline__ I mini(I a, int b) { return min(a, b); }
+    static __device__ __forceinline__ I opaque(I a) {   // a value the compiler cannot move out of its scope
+        asm volatile("" : "+v"(a));
+        return a;
+    }
+    static __device__ __forceinline__ V fabs(V a) { return ::fabs(a); }
+    static __device__ __forceinline__ V sqrt(V a) { return ::sqrt(a); }
+    static __device__ __forceinline__ V fma(V a, V b, V c) { return ::fma(a, b, c); }
+    static __device__ __forceinline__ V tod(I a) { return (double)a; }
+    static __device__ __forceinline__ V perm1(V a) { return dpp_d<DPP_QUAD(1, 0, 3, 2)>(a); }
+    static __device__ __forceinline__ V perm2(V a) { return dpp_d<DPP_QUAD(2, 3, 0, 1)>(a); }
+    static __device__ __forceinline__ V perm3(V a) { return dpp_d<DPP_QUAD(3, 2, 1, 0)>(a); }
+    static __device__ __forceinline__ V bperm(V a, I src) { return __shfl(a, src, 64); }
+    static __device__ __forceinline__ V quadsum(V a) {
+        const V b = a + __shfl_xor(a, 16, 64);
+        return b + __shfl_xor(b, 32, 64);
+    }
+    template <int NC>
+    static __device__ __forceinline__ V slotsum(V a) {   // sum over lanes of the row with the same lane % NC
+        V b = a + dpp_d<DPP_ROW_ROR(8)>(a);
+        b = b + dpp_d<DPP_ROW_ROR(4)>(b);
+        if constexpr (NC == 2) b = b + dpp_d<DPP_ROW_ROR(2)>(b);
+        return b;
+    }
+    template <int NC, int j>
+    static __device__ __forceinline__ V qbcast(V a) {
+        if constexpr (NC == 4) return dpp_d<DPP_QUAD(j, j, j, j)>(a);
+        else return dpp_d<DPP_QUAD(j, j, 2 + j, 2 + j)>(a);
+    }
+    static __device__ __forceinline__ V4 zero4() { return V4{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ V get4(const V4& a, int r) { return a[r]; }
+    static __device__ __forceinline__ void set4(V4& a, int r, V x) { a[r] = x; }
+    static __device__ __forceinline__ void mfma(V a, V b, V4& d) { d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0); }
+    static __device__ __forceinline__ bool any(M m) { return __ballot(m) != 0ull; }
+    static __device__ __forceinline__ bool all(M m) { return __ballot(!m) == 0ull; }
+    static __device__ __forceinline__ int maxi(I a, M m) {
+        int v = m ? a : INT_MIN;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+        return v;
+    }
+    static __device__ __forceinline__ V gld(const double* p, I i, M ok) {
+        V r = 0.0;
+        if (ok) r = p[i];
+        return r;
+    }
+    static __device__ __forceinline__ I gldi(const int32_t* p, I i, M ok) {
+        I r = 0;
+        if (ok) r = p[i];
+        return r;
+    }
+    static __device__ __forceinline__ PV ptr(const double* base, I js, int64_t stride) { return base + (int64_t)js * stride; }
+    static __device__ __forceinline__ V gldp(PV p, I i, M ok) {
+        V r = 0.0;
+        if (ok) r = p[i];
+        return r;
+    }
+    static __device__ __forceinline__ V lds(const double* p, I i) { return p[i]; }
+    static __device__ __forceinline__ void lds_st(double* p, I i, V x, M ok) {
+        if (ok) p[i] = x;
+    }
+    static __device__ __forceinline__ void lds_add(double* p, I i, V x, M ok) {
+        if (ok) p[i] += x;
+    }
+    static __device__ __forceinline__ void sti(int32_t* p, I i, I x, M ok) {
+        if (ok) p[i] = x;
+    }
+    static __device__ __forceinline__ void wsync() { wave_sync(); }
+};
+
+template <int CAP, bool LH>
+__host__ __device__ constexpr int mx_wave_doubles() {
+    return (LH ? 4 : 2) * nh(CAP) + 4 * (LH ? 4 : 8) * (CAP + 1) + (LH ? 4 : 8) * (2 * CAP + 1);
+}
+template <int CAP>
+__host__ __device__ constexpr int mx_shared_doubles() {
+    return mx::ALEN(CAP) + 6 * nh(CAP) + 4 * (CAP + 1);
+}
+template <int CAP, bool LH>
+static size_t mx_lds_bytes(int nw) {
+    return sizeof(double) * (size_t)(mx_shared_doubles<CAP>() + nw * mx_wave_doubles<CAP, LH>());
+}
+
+// Persistent: every workgroup copies the tables into LDS once, then each wavefront takes whole target
+// boxes from a counter (flags[2]) and runs their interaction lists BP at a time; a box's local
+// expansion is summed in the wavefront's LDS and added to the target's expansion once, by the one
+// wavefront that owns the box (deterministic: the same sums in the same order on every run).
+template <int CAP, bool LH, int NW>
+__global__ __launch_bounds__(64 * NW) void k_m2l_mx(const int32_t* __restrict__ m2l_targets,
+                                                    const int64_t* __restrict__ m2l_offsets,
+                                                    const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
+                                                    const Branch* __restrict__ sbr, const double* __restrict__ sex,
+                                                    double* __restrict__ tex, int n_boxes, int P, int PE, ErrParams E,
+                                                    int32_t* __restrict__ orders, int32_t* __restrict__ flags, TablePtrs tb) {
+    using Eng = mx::Engine<MxGpu, CAP, LH>;
+    constexpr int H = nh(CAP), NC = Eng::NC, BP = Eng::BP, NS = CAP + 1;
+    extern __shared__ double lds[];
+    double* A = lds;
+    double* zin = A + mx::ALEN(CAP);
+    double* zoh = zin + H;
+    double* ein = zoh + H;
+    double* eoh = ein + H;
+    double* Mt = eoh + H;
+    double* Lt = Mt + H;
+    double* tf = Lt + H;
+    for (int i = threadIdx.x; i < mx::ALEN(CAP); i += blockDim.x) A[i] = tb.Ad[i];
+    for (int i = threadIdx.x; i < H; i += blockDim.x) {
+        int n, m;
+        idx_nm(i, n, m);
+        const int b = len_Ts(n - 1);
+        zin[i] = tb.zeta[b + m];
+        zoh[i] = 0.5 * tb.zeta[b + m * (n + 1)];
+        ein[i] = tb.eta[b + m];
+        eoh[i] = 0.5 * tb.eta[b + m * (n + 1)];
+        Mt[i] = tb.Mt[i];
+        Lt[i] = tb.Lt[i];
+    }
+    double* i0 = tf + 2 * NS;
+    for (int i = threadIdx.x; i < 2 * NS; i += blockDim.x) tf[i] = tb.tfac[i];
+    for (int n = threadIdx.x; n < NS; n += blockDim.x) {
+        i0[2 * n] = 1.0 / tb.Mt[nh(n - 1)];
+        i0[2 * n + 1] = 1.0 / tb.Lt[nh(n - 1)];
+    }
+    __syncthreads();
+    double* wl = lds + mx_shared_doubles<CAP>() + (size_t)wave_id() * mx_wave_doubles<CAP, LH>();
+    const mx::Wave W{wl + NC * H, wl + NC * H + 2 * BP * NS, wl, wl + NC * H + 4 * BP * NS};
+    const mx::Tabs T{A, zin, zoh, ein, eoh, Mt, Lt, tf, i0};
+    const int64_t HE4 = 4 * (int64_t)nh(PE);
+    const int lane = lane_id();
+    int pmax = 0, okf = 1;
+    for (;;) {
+        int box = 0;
+        if (lane == 0) box = atomicAdd(flags + 2, 1);
+        box = __shfl(box, 0, 64);
+        if (box >= n_boxes) break;
+        for (int i = lane; i < NC * H; i += 64) W.acc[i] = 0.0;
+        wave_sync();
+        const int32_t it = m2l_targets[box];
+        const Branch& Tb = tbr[it];
+        mx::Box X;
+        for (int d = 0; d < 3; ++d) {
+            X.tc[d] = Tb.target_center[d];
+            X.tbox[d] = Tb.target_box[d];
+        }
+        X.r_l = Tb.target_radius;
+        const double eps = E.relative ? Tb.max_influence * E.eps : E.eps;
+        X.limit = eps * 4 * PI_D;
+        X.P = P;
+        X.kind = E.kind;
+        X.BE = E.BE;
+        const int64_t l0 = m2l_offsets[box], l1 = m2l_offsets[box + 1];
+        for (int64_t b0 = l0; b0 < l1; b0 += BP)
+            Eng::batch(T, W, X, m2l_s + b0, (int)min((int64_t)BP, l1 - b0), (const double*)sbr, (int)(sizeof(Branch) / 8),
+                       (int)(offsetof(Branch, source_center) / 8), sex, HE4, orders + b0, pmax, okf);
+        double* dst = tex + (int64_t)it * HE4;
+        for (int i = lane; i < NC * nh(P); i += 64) dst[4 * (i / NC) + i % NC] += W.acc[i];
+        wave_sync();
+    }
+    if (lane == 0) {
+        atomicMax(flags, pmax);
+        if (!okf) atomicAnd(flags + 1, 0);
+    }
+}
+
+template <int CAP, bool LH, int NW>
+static void launch_m2l_mx(Context& C, int p, int pe, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
+                          int32_t* flags, const TablePtrs& tb) {
+    auto kfn = k_m2l_mx<CAP, LH, NW>;
+    const size_t sh = mx_lds_bytes<CAP, LH>(NW);
+    HIP_CHECK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kfn), 64 * NW, sh));
+    if (!C.n_cu) HIP_CHECK(hipDeviceGetAttribute(&C.n_cu, hipDeviceAttributeMultiprocessorCount, C.device));
+    const int64_t want = (Ls.n_m2l_targets + NW - 1) / NW;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)std::max(per_cu, 1) * C.n_cu));
+    kfn<<<grid, 64 * NW, sh, C.stream>>>(Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(),
+                                         st.branches.data(), st.expansions.data(), tt_dst.expansions.data(),
+                                         (int)Ls.n_m2l_targets, p, pe, E, Ls.m2l_order.data(), flags, tb);
+    LAUNCH_CHECK();
+}
+
+// order caps of the instantiated register engines
+static constexpr int MX_CAPS[] = {8, 12, 16, 20};
+static constexpr int MX_NW = 4;
+
+template <bool LH>
+static void run_m2l_mx(Context& C, int p, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E, int32_t* flags,
+                       const TablePtrs& tb) {
+    if (p <= 8) launch_m2l_mx<8, LH, MX_NW>(C, p, p, Ls, st, tt_dst, E, flags, tb);
+    else if (p <= 12) launch_m2l_mx<12, LH, MX_NW>(C, p, p, Ls, st, tt_dst, E, flags, tb);
+    else if (p <= 16) launch_m2l_mx<16, LH, MX_NW>(C, p, p, Ls, st, tt_dst, E, flags, tb);
+    else launch_m2l_mx<20, LH, MX_NW>(C, p, p, Ls, st, tt_dst, E, flags, tb);
+}
+
+__global__ void k_init_flags(int32_t* pmax, int32_t* ok) { *pmax = 0; *ok = 1; ok[1] = 0; }
 
 // M2L kernel for orders above SMALL_P: the wavefront-per-interaction engine
 // (k_m2l) or the matrix-core batches (k_m2l_mfma). Both compute the same
 // interactions at the same orders; FMMHIP_M2L_KERNEL=mfma selects the second.
-enum M2LEngine { M2L_WAVE = 0, M2L_MFMA = 1 };
+enum M2LEngine { M2L_WAVE = 0, M2L_MFMA = 1, M2L_MX = 2 };
 static M2LEngine m2l_engine() {
     const char* e = std::getenv("FMMHIP_M2L_KERNEL");
-    return (e && std::strcmp(e, "mfma") == 0) ? M2L_MFMA : M2L_WAVE;
+    if (e && std::strcmp(e, "mfma") == 0) return M2L_MFMA;
+    if (e && std::strcmp(e, "wave") == 0) return M2L_WAVE;
+    return M2L_MX;
 }
 
 void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int p, bool LH, int err_kind, bool err_rel,
@@ -2110,7 +2200,7 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
     ensure_tables(C, p);
     TablePtrs tb = ptrs(C);
     DBuf<int32_t>& flags = C.m2l_flags;
-    flags.resize(2);
+    flags.resize(3);
     k_init_flags<<<1, 1, 0, C.stream>>>(flags.data(), flags.data() + 1);
     LAUNCH_CHECK();
     if (Ls.n_m2l_targets > 0 && err_kind == FMMHIP_ERR_NONE && p <= SMALL_P) {
@@ -2118,7 +2208,11 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
         else launch_m2l_lane<false>(C, p, Ls, st, tt_dst, flags.data(), tb);
     } else if (Ls.n_m2l_targets > 0) {
         ErrParams E{err_kind, err_rel ? 1 : 0, eps, BE ? 1 : 0};
-        if (m2l_engine() == M2L_MFMA) {
+        const M2LEngine eng = m2l_engine();
+        if (eng == M2L_MX && p <= 20) {
+            if (LH) run_m2l_mx<true>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+            else run_m2l_mx<false>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
+        } else if (eng == M2L_MFMA) {
             if (LH) run_m2l_mfma<true>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
             else run_m2l_mfma<false>(C, p, Ls, st, tt_dst, E, flags.data(), tb);
         } else if (err_kind != FMMHIP_ERR_NONE && p > M2L_FIRST_CAP) {
