@@ -112,7 +112,7 @@ struct Box {
     int BE;         // the order settles at the passing degree (1) or one below (0)
 };
 
-template <class B, int CAP, bool LH>
+template <class B, int CAP, bool LH, bool FULL = false>
 struct Engine {
     using V = typename B::V;
     using I = typename B::I;
@@ -124,6 +124,7 @@ struct Engine {
     static constexpr int NL = NLREG(CAP);
     static constexpr int NS = CAP + 1;   // slot-table entries
     static constexpr int NF = 2 * CAP + 1;   // translation factors per slot
+    static constexpr int NX = KE(CAP) + KO(CAP);   // registers of one degree
     static constexpr double SQ3 = 1.7320508075688772;
 
     // src/error.jl:5-10
@@ -322,6 +323,22 @@ struct Engine {
         sfor<0, ko>([&](auto OO) { out(std::integral_constant<int, 1>(), OO, PO, oO[decltype(OO)::value]); });
     }
 
+    // the source coefficients of degree n for this lane's rows (E registers, then O), zero for rows m > n
+    // and for empty slots
+    template <int n>
+    MX_FN static void fetch(const PV& src, I q, I comp, M valid, V (&xn)[NX]) {
+        auto one = [&](auto par, auto RI, V& x) {
+            constexpr int pr = decltype(par)::value, ri = decltype(RI)::value;
+            constexpr int m0 = 8 * ri + pr;
+            const I m = q * 2 + m0;
+            const M ok = (m0 + 6 <= n) ? valid : (valid && (m <= n));
+            const I mi = (m0 + 6 <= n) ? m : B::mini(m, n);
+            x = B::gldp(src, (mi + mnh(n - 1)) * 4 + comp, ok);
+        };
+        sfor<0, KE(n)>([&](auto EE) { one(std::integral_constant<int, 0>(), EE, xn[decltype(EE)::value]); });
+        sfor<0, KO(n)>([&](auto OO) { one(std::integral_constant<int, 1>(), OO, xn[KE(n) + decltype(OO)::value]); });
+    }
+
     // value of component j of this lane's slot, from the same lane quad
     template <int j>
     MX_FN static V comp_of(V x) { return B::template qbcast<NC, j>(x); }
@@ -337,7 +354,7 @@ struct Engine {
         const I c = c0, q = q0, s = c / NC, comp = c % NC, pot = comp >> 1;
         const M isIm = (comp & 1) != 0;
         const M valid = s < nb;
-        const int P = X.P;
+        const int P = FULL ? CAP : X.P;   // FULL: the launch's order is the engine's cap
         const bool err = X.kind != 0;
 
         V dx, dy, dz;
@@ -349,6 +366,8 @@ struct Engine {
         const I js = B::gldi(ms, B::seli(valid, s, I(0)), valid);
         const PV src = B::ptr(sex, js, HE4);
         const V cj = B::sel(isIm, V(-1.0), V(1.0));
+        const V phi_sign = B::sel(comp == 0, V(-1.0), B::sel(comp == 1, V(1.0), V(0.0)));   // Lamb-Helmholtz phi term
+        const M chi_lane = pot == 1;
         RotLane RL0;
         RL0.c = c;
         RL0.q = q;
@@ -359,6 +378,9 @@ struct Engine {
 
         V L[NL];
         sfor<0, NL>([&](auto II) { L[decltype(II)::value] = V(0.0); });
+        // degree k's source coefficients are loaded during degree k - 1 (their HBM latency hidden)
+        V xn[NX];
+        fetch<0>(src, q, comp, valid, xn);
         M active = valid;
         M succ = valid && !err;
         I Pl = I(P);
@@ -371,6 +393,7 @@ struct Engine {
         sfor<0, CAP + 1>([&](auto KK) {
             constexpr int k = decltype(KK)::value;
             if (!running || k > P) return;
+            B::sched_fence();   // keep each degree's instructions together (limits register pressure)
             kdone = k;
             constexpr int ke = KE(k), ko = KO(k), ke1 = imax(ke, 1), ko1 = imax(ko, 1);
             // lane indices made opaque per degree: their derived addresses and masks are recomputed here
@@ -383,16 +406,19 @@ struct Engine {
             RL.s = s;
             // ---- degree k of the source multipoles: power terms, z-rotation, input scale
             V uE[ke1], uO[ko1], ME[ke1], MO[ko1];
+            V xc[NX];
+            sfor<0, ke + ko>([&](auto II) { xc[decltype(II)::value] = xn[decltype(II)::value]; });
+            if constexpr (k + 1 <= CAP) {
+                if (FULL || k + 1 <= P) fetch<k + 1>(src, q, comp, valid, xn);
+            }
             V pw_t = V(0.0);
             const bool power = X.kind >= 2;
             auto load = [&](auto par, auto RI, V& u) {
                 constexpr int pr = decltype(par)::value, ri = decltype(RI)::value;
                 constexpr int m0 = 8 * ri + pr;
                 const I m = q * 2 + m0;
-                // rows m > k load zero (every quad's row is inside the degree when m0 + 6 <= k)
-                const M ok = (m0 + 6 <= k) ? valid : (valid && (m <= k));
                 const I mi = (m0 + 6 <= k) ? m : B::mini(m, k);
-                const V x = B::gldp(src, (mi + mnh(k - 1)) * 4 + comp, ok);
+                const V x = xc[(pr ? ke : 0) + ri];
                 if (power) {
                     const V nt = B::lds(T.Mt, mi + mnh(k - 1));
                     const V x2 = x * x;
@@ -542,7 +568,6 @@ struct Engine {
             sfor<0, ko>([&](auto OO) { MO[decltype(OO)::value] = B::sel(addk, MO[decltype(OO)::value], V(0.0)); });
             sfor<0, CAP + 1>([&](auto JJ) {
                 constexpr int j = decltype(JJ)::value;
-                if (j > P) return;
                 if constexpr (j != k) {
                     const V fac = B::lds(W.F, fb + (k + j));
                     sfor<0, imin(KE(j), ke)>([&](auto EE) {
@@ -570,10 +595,13 @@ struct Engine {
                 constexpr int n = decltype(NN)::value;
                 if (n > kdone) return;
                 constexpr int ke = KE(n), ko = KO(n);
-                const I c = B::opaque(c0), q = B::opaque(q0), comp = c % NC, pot = comp >> 1;
-                const M has_next = Pl > n;
+                const I q = B::opaque(q0);
+                // phi += (r m / n) (-/+) chi on the phi lanes; chi -= r / (n + 1) chi_(n+1) on the chi lanes below
+                // the interaction's order. Per-lane coefficients, zero where a term does not apply, so each
+                // register is two multiply-adds with the reference's rounding (x + 0 y = x).
                 const V r_np1 = r * (1.0 / (n + 1));
                 const V r_n = r * (1.0 / (n > 0 ? n : 1));
+                const V bco = B::sel(chi_lane && (Pl > n), -r_np1, V(0.0));
                 auto lh = [&](auto par, auto RI) {
                     constexpr int pr = decltype(par)::value, ri = decltype(RI)::value;
                     constexpr int at = LOFF(n) + (pr ? ke : 0) + ri;
@@ -581,15 +609,10 @@ struct Engine {
                     constexpr int at1 = LOFF(nn) + (pr ? KE(nn) : 0) + ri;
                     const I m = q * 2 + (8 * ri + pr);
                     const V x = L[at];
-                    const V y = B::perm3(x);
-                    V phi = x;
-                    if constexpr (n > 0) {
-                        const V rmn = r_n * B::tod(m);
-                        phi = B::sel(comp == 0, x - rmn * y, x + rmn * y);
-                    }
-                    V chi = x;
-                    if constexpr (n < CAP) chi = B::sel(has_next, x - r_np1 * L[at1], x);
-                    L[at] = B::sel(pot == 0, phi, chi);
+                    V nv = x;
+                    if constexpr (n > 0) nv = x + (phi_sign * (r_n * B::tod(m))) * B::perm3(x);
+                    if constexpr (n < CAP) nv = nv + bco * L[at1];
+                    L[at] = nv;
                 };
                 sfor<0, ke>([&](auto EE) { lh(std::integral_constant<int, 0>(), EE); });
                 sfor<0, ko>([&](auto OO) { lh(std::integral_constant<int, 1>(), OO); });
@@ -600,6 +623,7 @@ struct Engine {
         sfor<0, CAP + 1>([&](auto NN) {
             constexpr int n = decltype(NN)::value;
             if (n > kdone) return;
+            B::sched_fence();
             constexpr int ke = KE(n), ko = KO(n), ke1 = imax(ke, 1), ko1 = imax(ko, 1);
             const I c = B::opaque(c0), q = B::opaque(q0), s = c / NC, comp = c % NC;
             const I sb2 = s * (NS * 2);

@@ -1,0 +1,11 @@
+// Matrix-core M2L kernels at order cap 8 (m2l_mx_gpu.hpp).
+#include "m2l_mx_gpu.hpp"
+
+namespace fmmhip {
+
+void run_m2l_mx_c8(Context& C, int p, bool LH, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
+                      int32_t* flags, const TablePtrs& tb) {
+    run_m2l_mx_cap<8>(C, p, LH, Ls, st, tt_dst, E, flags, tb);
+}
+
+}  // namespace fmmhip

@@ -56,6 +56,7 @@ struct CpuB {
         return r;
     }
     static I opaque(const I& a) { return a; }
+    static void sched_fence() {}
     static I mini(const I& a, int b) {
         I r;
         for (int i = 0; i < W; ++i) r.v[i] = a.v[i] < b ? a.v[i] : b;
