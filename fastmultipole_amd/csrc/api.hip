@@ -1,6 +1,7 @@
 // C ABI of libfmmhip (include/fmmhip.h): contexts, evaluations and the fmm!
 // pipeline (src/fmm.jl:862-1176) on one MI355X.
 #include "fmmhip_internal.hpp"
+#include <memory>
 #include <cstring>
 #include <algorithm>
 #include <cmath>
@@ -145,16 +146,17 @@ Eval::~Eval() {
 
 // the evaluation's second stream, with its own scratch and tables
 static Context& side_context(Eval& E) {
-    if (!E.side) {
+    Context*& slot = E.side;
+    if (!slot) {
         Context* c = new Context();
         c->device = E.ctx->device;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
             delete c;
             throw Error(FMMHIP_EDEVICE, "hipStreamCreateWithFlags failed");
         }
-        E.side = c;
+        slot = c;
     }
-    return *E.side;
+    return *slot;
 }
 
 // makes `waiter` wait for the work enqueued on `signaller` so far
@@ -169,16 +171,9 @@ static void stream_wait(Context& waiter, Context& signaller) {
 
 // estimate_influence! for relative tolerances (src/fmm.jl:1193-1276): a full
 // low-order evaluation (p = 3, leaf 5, MAC 0.6, potential and gradient) on its
-// own trees; max |grad| (or potential) per main-tree branch
-static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
-    if (!E.estimate) E.estimate = new Eval();
-    Eval& X = *E.estimate;
-    X.ctx = &C;
-    X.kernel = E.kernel;
-    X.nt = E.nt;
-    X.ns = E.ns;
-    X.t_xyz = E.t_xyz;
-    X.s_buf = E.s_buf;
+// own trees; max |grad| (or potential) per main-tree branch. In two parts: the
+// trees and lists (many small, host-synchronised launches), then the evaluation.
+static fmmhip_options estimate_options(const fmmhip_options& o) {
     fmmhip_options eo = o;
     eo.scalar_potential = 1; eo.gradient = 1; eo.hessian = 0;
     eo.leaf_size_source = 5; eo.leaf_size_target = 5;
@@ -187,6 +182,19 @@ static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
     eo.shrink_recenter = 1;
     eo.list_method = FMMHIP_SELFTUNING_TREESTOP;
     eo.farfield = eo.nearfield = eo.self_induced = 1;
+    return eo;
+}
+
+static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
+    if (!E.estimate) E.estimate = new Eval();
+    Eval& X = *E.estimate;
+    X.ctx = &C;
+    X.kernel = E.kernel;
+    X.nt = E.nt;
+    X.ns = E.ns;
+    X.t_xyz = E.t_xyz;
+    X.s_buf = E.s_buf;
+    const fmmhip_options eo = estimate_options(o);
     if (X.t_xyz) eval_build_trees(X, eo);
     else {   // the targets are the sources: one build gives both trees; the Morton keys and their
              // order are the main source tree's (same buffer), built on the evaluation's stream
@@ -198,7 +206,11 @@ static void estimate_influence(Eval& E, const fmmhip_options& o, Context& C) {
         X.p_built = eo.expansion_order;
     }
     eval_build_lists(X, eo, false);
-    eval_core(X, eo, nullptr, true);
+}
+
+static void estimate_eval(Eval& E, const fmmhip_options& o, Context& C) {
+    Eval& X = *E.estimate;
+    eval_core(X, estimate_options(o), nullptr, true);
     DBuf<double>& infl = E.influence;
     infl.resize(std::max<int64_t>(1, E.nt));
     k_map_influence<<<grid_for(E.nt, 256), 256, 0, C.stream>>>(X.tt.sort_index.data(), E.tt.inverse_sort.data(), X.tout.data(),
@@ -239,6 +251,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
             ensure_tables(F, p);
             stream_wait(F, C);
         }
+        const bool estimate = !is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative;
         Timer tp(C);
         // Beside the far-field stream the nearfield leaves one workgroup slot per CU free: its
         // rate is about the same at 3 and 4 workgroups per CU, and the far field's kernels (the
@@ -247,9 +260,14 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         const int slots = &F != &C ? -1 : 0;
         run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision, 0, -1, slots);
         tp.mark();
-        if (!is_estimate && o.error_kind != FMMHIP_ERR_NONE && o.error_relative) {
+        if (estimate) {
+            // estimate_influence (trees, lists, evaluation) on the far-field stream beside the nearfield.
+            // Measured alternatives: its trees and lists first, alone on the device (110.1 against 107.5 ms
+            // per evaluation of the 1M ring: its evaluation then waits for the nearfield), and the upward
+            // pass on a third stream beside it (109.0 ms: both then share the one free slot per CU).
             Timer t(F);
-            estimate_influence(E, o, F);
+            estimate_build(E, o, F);
+            estimate_eval(E, o, F);
             t_est = t.stop();
         }
         {
