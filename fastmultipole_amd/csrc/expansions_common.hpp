@@ -44,16 +44,18 @@ struct ErrParams {
     int BE;
 };
 
+// processing order of the error-controlled M2L list within each target box (m2l_mx_order.hip)
+const int32_t* m2l_batch_order(Context& C, Lists& Ls, const Tree& tt, const Tree& st);
 // matrix-core M2L (m2l_mx_c*.hip): one launch over the M2L list at order (cap) p <= 20
 #define FMMHIP_MX_DECL(name)                                                                                           \
     void name(Context& C, int p, bool LH, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,         \
-              int32_t* flags, const TablePtrs& tb);
+              const int32_t* perm, int32_t* flags, const TablePtrs& tb);
 FMMHIP_MX_DECL(run_m2l_mx_c8)
 FMMHIP_MX_DECL(run_m2l_mx_c12)
 FMMHIP_MX_DECL(run_m2l_mx_c16)
 FMMHIP_MX_DECL(run_m2l_mx_c20)
 #undef FMMHIP_MX_DECL
-void run_m2l_mx(Context& C, int p, bool LH, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
-                int32_t* flags, const TablePtrs& tb);
+void run_m2l_mx(Context& C, int p, bool LH, Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E, int32_t* flags,
+                const TablePtrs& tb);
 
 }  // namespace fmmhip

@@ -87,6 +87,8 @@ struct Lists {
     int64_t n_m2l = 0, n_direct = 0, n_pairs = 0;
     DBuf<int32_t> m2l_t, m2l_s, dir_t, dir_s;        // sorted by target, DFS order within target
     DBuf<int32_t> m2l_order;                         // expansion order used per M2L
+    DBuf<int32_t> m2l_perm, m2l_iota;                // processing order of the M2L list within each target box
+    DBuf<float> m2l_key, m2l_key_sorted;
     // CSR over target branches for the direct list: unique targets and offsets
     int64_t n_direct_targets = 0;
     DBuf<int32_t> dir_targets;                       // unique target branches in the direct list

@@ -102,6 +102,15 @@ struct Wave {
     double* F;     // BP x (2 CAP + 1): z-translation factors j! / t^(j+1) of each slot
 };
 
+// BP consecutive entries b0 .. b0 + nb - 1 of the target box's part of the M2L list, taken in the
+// order perm gives (list indices; null: list order)
+struct Batch {
+    const int32_t* m2l_s;   // source branch per list entry
+    const int32_t* perm;    // processing order of the list entries, or null
+    int b0, nb;
+    int32_t* orders;        // expansion order per list entry (written)
+};
+
 // the target box and the method (uniform over the wavefront)
 struct Box {
     double tc[3], tbox[3];
@@ -135,10 +144,14 @@ struct Engine {
         return B::sel(same, mn, V(0.0));
     }
 
-    // translation vector target centre - source centre of slot `slot` (benign values for empty slots)
-    MX_FN static void geometry(const Box& X, const int32_t* ms, I slot, M ok, const double* sbr, int br_stride,
-                               int sc_off, V& dx, V& dy, V& dz) {
-        const I js = B::gldi(ms, B::seli(ok, slot, I(0)), ok);
+    // list index of slot `slot` of the batch
+    MX_FN static I entry(const Batch& Bt, I slot, M ok) {
+        const I k = B::seli(ok, slot, I(0)) + Bt.b0;
+        return Bt.perm ? B::gldi(Bt.perm, k, ok) : k;
+    }
+
+    // translation vector target centre - source centre of source branch js (benign values for empty slots)
+    MX_FN static void geometry(const Box& X, I js, M ok, const double* sbr, int br_stride, int sc_off, V& dx, V& dy, V& dz) {
         const I base = js * br_stride + sc_off;
         const V sx = B::gld(sbr, base, ok), sy = B::gld(sbr, base + 1, ok), sz = B::gld(sbr, base + 2, ok);
         dx = B::sel(ok, X.tc[0] - sx, V(0.0));
@@ -168,15 +181,14 @@ struct Engine {
     // angle-addition recurrence of update_Ts!, :380-435; the factor chain of
     // translate_multipole_to_local_z!, src/translate.jl:181-232), so every entry is the double the
     // reference's chain produces.
-    MX_FN static void slot_tables(const Box& X, const Wave& W, const int32_t* ms, int nb, const double* sbr, int br_stride,
-                                  int sc_off) {
+    MX_FN static void slot_tables(const Box& X, const Wave& W, const Batch& Bt, const double* sbr, int br_stride, int sc_off) {
         constexpr int LPS = 64 / BP;
         B::wsync();   // the previous batch's reads of the tables are done
         const I l = B::lane();
         const I sT = l / LPS, kT = l % LPS;
-        const M vT = sT < nb;
+        const M vT = sT < Bt.nb;
         V dx, dy, dz;
-        geometry(X, ms, sT, vT, sbr, br_stride, sc_off, dx, dy, dz);
+        geometry(X, B::gldi(Bt.m2l_s, entry(Bt, sT, vT), vT), vT, sbr, br_stride, sc_off, dx, dy, dz);
         V cp, sp, cb, sb;
         const V r = angles(dx, dy, dz, cp, sp, cb, sb);
         const V ot = 1.0 / r;
@@ -343,27 +355,27 @@ struct Engine {
     template <int j>
     MX_FN static V comp_of(V x) { return B::template qbcast<NC, j>(x); }
 
-    // BP consecutive interactions ms[0..nb) of one target box: their local expansions are added to
-    // W.acc and their orders written to orders[0..nb). pmax / ok collect the largest order and whether
-    // every tolerance was met.
-    MX_FN static void batch(const Tabs& T, const Wave& W, const Box& X, const int32_t* ms, int nb, const double* sbr,
-                            int br_stride, int sc_off, const double* sex, int64_t HE4, int32_t* orders, int& pmax, int& okflag) {
-        slot_tables(X, W, ms, nb, sbr, br_stride, sc_off);
+    // One batch of interactions of one target box: their local expansions are added to W.acc and their
+    // orders written to Bt.orders. pmax / ok collect the largest order and whether every tolerance was met.
+    MX_FN static void batch(const Tabs& T, const Wave& W, const Box& X, const Batch& Bt, const double* sbr, int br_stride,
+                            int sc_off, const double* sex, int64_t HE4, int& pmax, int& okflag) {
+        slot_tables(X, W, Bt, sbr, br_stride, sc_off);
         const I l0 = B::lane();
         const I c0 = l0 & 15, q0 = l0 >> 4;
         const I c = c0, q = q0, s = c / NC, comp = c % NC, pot = comp >> 1;
         const M isIm = (comp & 1) != 0;
-        const M valid = s < nb;
+        const M valid = s < Bt.nb;
         const int P = FULL ? CAP : X.P;   // FULL: the launch's order is the engine's cap
         const bool err = X.kind != 0;
 
+        const I li = entry(Bt, s, valid);
+        const I js = B::gldi(Bt.m2l_s, li, valid);
         V dx, dy, dz;
-        geometry(X, ms, s, valid, sbr, br_stride, sc_off, dx, dy, dz);
+        geometry(X, js, valid, sbr, br_stride, sc_off, dx, dy, dz);
         const V r = B::sqrt(dx * dx + dy * dy + dz * dz);
         const V d0 = md1(dx, -X.tbox[0], X.tbox[0]), d1 = md1(dy, -X.tbox[1], X.tbox[1]), d2 = md1(dz, -X.tbox[2], X.tbox[2]);
         const V r_mp = B::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
         const V rmi = 1.0 / r_mp;
-        const I js = B::gldi(ms, B::seli(valid, s, I(0)), valid);
         const PV src = B::ptr(sex, js, HE4);
         const V cj = B::sel(isIm, V(-1.0), V(1.0));
         const V phi_sign = B::sel(comp == 0, V(-1.0), B::sel(comp == 1, V(1.0), V(0.0)));   // Lamb-Helmholtz phi term
@@ -662,7 +674,7 @@ struct Engine {
 
         // ---- orders and flags
         const M wr = valid && (q == 0) && (comp == 0);
-        B::sti(orders, B::seli(wr, s, I(0)), Pl, wr);
+        B::sti(Bt.orders, B::seli(wr, li, I(0)), Pl, wr);
         const int mx = B::maxi(Pl, valid);
         if (mx > pmax) pmax = mx;
         if (!B::all(succ || !valid)) okflag = 0;

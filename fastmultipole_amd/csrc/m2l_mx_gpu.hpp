@@ -128,7 +128,8 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mx(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
                                                     const Branch* __restrict__ sbr, const double* __restrict__ sex,
                                                     double* __restrict__ tex, int n_boxes, int P, int PE, ErrParams E,
-                                                    int32_t* __restrict__ orders, int32_t* __restrict__ flags, TablePtrs tb) {
+                                                    const int32_t* __restrict__ perm, int32_t* __restrict__ orders,
+                                                    int32_t* __restrict__ flags, TablePtrs tb) {
     using Eng = mx::Engine<MxGpu, CAP, LH, FULL>;
     constexpr int H = nh(CAP), NC = Eng::NC, BP = Eng::BP, NS = CAP + 1;
     extern __shared__ double lds[];
@@ -186,9 +187,11 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mx(const int32_t* __restrict__ 
         X.kind = E.kind;
         X.BE = E.BE;
         const int64_t l0 = m2l_offsets[box], l1 = m2l_offsets[box + 1];
-        for (int64_t b0 = l0; b0 < l1; b0 += BP)
-            Eng::batch(T, W, X, m2l_s + b0, (int)min((int64_t)BP, l1 - b0), (const double*)sbr, (int)(sizeof(Branch) / 8),
-                       (int)(offsetof(Branch, source_center) / 8), sex, HE4, orders + b0, pmax, okf);
+        for (int64_t b0 = l0; b0 < l1; b0 += BP) {
+            const mx::Batch Bt{m2l_s, perm, (int)b0, (int)min((int64_t)BP, l1 - b0), orders};
+            Eng::batch(T, W, X, Bt, (const double*)sbr, (int)(sizeof(Branch) / 8), (int)(offsetof(Branch, source_center) / 8),
+                       sex, HE4, pmax, okf);
+        }
         double* dst = tex + (int64_t)it * HE4;
         for (int i = lane; i < NC * nh(P); i += 64) dst[4 * (i / NC) + i % NC] += W.acc[i];
         wave_sync();
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mx(const int32_t* __restrict__ 
 
 template <int CAP, bool LH, int NW, bool FULL>
 static void launch_m2l_mx(Context& C, int p, int pe, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
-                          int32_t* flags, const TablePtrs& tb) {
+                          const int32_t* perm, int32_t* flags, const TablePtrs& tb) {
     auto kfn = k_m2l_mx<CAP, LH, NW, FULL>;
     const size_t sh = mx_lds_bytes<CAP, LH>(NW);
     HIP_CHECK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -212,7 +215,7 @@ static void launch_m2l_mx(Context& C, int p, int pe, const Lists& Ls, const Tree
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)std::max(per_cu, 1) * C.n_cu));
     kfn<<<grid, 64 * NW, sh, C.stream>>>(Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(),
                                          st.branches.data(), st.expansions.data(), tt_dst.expansions.data(),
-                                         (int)Ls.n_m2l_targets, p, pe, E, Ls.m2l_order.data(), flags, tb);
+                                         (int)Ls.n_m2l_targets, p, pe, E, perm, Ls.m2l_order.data(), flags, tb);
     LAUNCH_CHECK();
 }
 
@@ -222,14 +225,14 @@ static constexpr int MX_NW = 4;   // wavefronts per workgroup (one per SIMD at t
 // the launch at order p <= CAP (CAP the smallest instantiated cap >= p)
 template <int CAP>
 void run_m2l_mx_cap(Context& C, int p, bool LH, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
-                    int32_t* flags, const TablePtrs& tb) {
+                    const int32_t* perm, int32_t* flags, const TablePtrs& tb) {
     static const bool full_ok = std::getenv("FMMHIP_MX_NOFULL") == nullptr;   // measurement switch
     if (p == CAP && full_ok) {
-        if (LH) launch_m2l_mx<CAP, true, MX_NW, true>(C, p, p, Ls, st, tt_dst, E, flags, tb);
-        else launch_m2l_mx<CAP, false, MX_NW, true>(C, p, p, Ls, st, tt_dst, E, flags, tb);
+        if (LH) launch_m2l_mx<CAP, true, MX_NW, true>(C, p, p, Ls, st, tt_dst, E, perm, flags, tb);
+        else launch_m2l_mx<CAP, false, MX_NW, true>(C, p, p, Ls, st, tt_dst, E, perm, flags, tb);
     } else {
-        if (LH) launch_m2l_mx<CAP, true, MX_NW, false>(C, p, p, Ls, st, tt_dst, E, flags, tb);
-        else launch_m2l_mx<CAP, false, MX_NW, false>(C, p, p, Ls, st, tt_dst, E, flags, tb);
+        if (LH) launch_m2l_mx<CAP, true, MX_NW, false>(C, p, p, Ls, st, tt_dst, E, perm, flags, tb);
+        else launch_m2l_mx<CAP, false, MX_NW, false>(C, p, p, Ls, st, tt_dst, E, perm, flags, tb);
     }
 }
 

@@ -281,7 +281,8 @@ int run(int P, int kind, int relative, double eps, int BE, const double* tg, int
     for (int i = 0; i < nsrc; ++i) ms[i] = i;
     int pm = 0, okf = 1;
     for (int b0 = 0; b0 < nsrc; b0 += E::BP)
-        E::batch(T, Wv, X, ms.data() + b0, std::min(E::BP, nsrc - b0), sg, 15, 0, sex, he4, orders + b0, pm, okf);
+        E::batch(T, Wv, X, fmmhip::mx::Batch{ms.data(), nullptr, b0, std::min(E::BP, nsrc - b0), orders}, sg, 15, 0, sex, he4,
+                 pm, okf);
     const int H = fmmhip::mx::mnh(P);
     for (int i = 0; i < H; ++i)
         for (int c = 0; c < 4; ++c) out[4 * i + c] = c < E::NC ? acc[i * E::NC + c] : 0.0;
