@@ -15,6 +15,10 @@ namespace fmmhip {
 
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)std::max<int64_t>(1, (n + block - 1) / block); }
 
+void ctx_disable_graphs(fmmhip_ctx* ctx) {
+    if (ctx) ctx->c.graphs = false;
+}
+
 __global__ void k_gather_outputs(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                                  const int64_t* __restrict__ sort_index, const double* __restrict__ tout, int64_t nt,
                                  double* __restrict__ aos) {

@@ -1976,11 +1976,23 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
             run(C, p, p, M2L_PASS_ALL, Ls, st, tt_dst, E, flags.data(), tb);
         }
     }
+    if (!pmax) return;   // no read-back: the launch sequence stays free of host synchronisation (graph capture)
     int32_t h[2];
     HIP_CHECK(hipMemcpyAsync(h, flags.data(), sizeof(h), hipMemcpyDeviceToHost, C.stream));
     HIP_CHECK(hipStreamSynchronize(C.stream));
     *pmax = h[0];
     *success = h[1];
+}
+
+__global__ void k_fill_zero(double* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+void fill_zero(Context& C, double* p, size_t n) {
+    if (!n) return;
+    const unsigned grid = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+    k_fill_zero<<<grid, 256, 0, C.stream>>>(p, (int64_t)n);
+    LAUNCH_CHECK();
 }
 
 // ================================================================= P2M

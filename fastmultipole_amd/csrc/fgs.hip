@@ -25,6 +25,7 @@
 // dependency order and each level is one launch. After the iterations the nearfield
 // of the new strengths, self pairs included, is added to the last farfield (:917-931).
 #include "fmmhip_internal.hpp"
+#include <cstdlib>
 #include "panels.hpp"
 #include <cstring>   // host memset for rocprim headers
 #include <rocprim/rocprim.hpp>
@@ -489,17 +490,18 @@ void fgs_setup(Eval& E, const fmmhip_options& o, int influence) {
 
 // farfield-only evaluation on the current strengths (fmm! with an empty direct list, :839-850):
 // potential and gradient into E.tout (sorted)
+// farfield-only evaluation of the current strengths (fixed order: no read-back, no host
+// synchronisation, zeroing by kernels, so the sequence can be captured in a graph)
 static void fgs_farfield(Eval& E, FgsState& F) {
     Context& C = *E.ctx;
     const int p = F.p;
-    HIP_CHECK(hipMemsetAsync(E.tout.data(), 0, E.tout.size() * sizeof(double), C.stream));
-    HIP_CHECK(hipMemsetAsync(E.tt.expansions.data(), 0, E.tt.expansions.size() * sizeof(double), C.stream));
-    HIP_CHECK(hipMemsetAsync(E.st.expansions.data(), 0, E.st.expansions.size() * sizeof(double), C.stream));
+    fill_zero(C, E.tout.data(), E.tout.size());
+    fill_zero(C, E.tt.expansions.data(), E.tt.expansions.size());
+    fill_zero(C, E.st.expansions.data(), E.st.expansions.size());
     ensure_tables(C, p);
     run_p2m(C, E.kernel, E.st, p);
     run_m2m(C, E.st, p, false);
-    int32_t pmax = 0, ok = 1;
-    run_m2l(C, E.tt, E.st, E.tt, E.lists, p, false, FMMHIP_ERR_NONE, false, 0.0, true, FMMHIP_FP64, &pmax, &ok);
+    run_m2l(C, E.tt, E.st, E.tt, E.lists, p, false, FMMHIP_ERR_NONE, false, 0.0, true, FMMHIP_FP64, nullptr, nullptr);
     run_l2l(C, E.tt, p, false);
     run_l2p(C, E.tt, E.tout.data(), E.nt, p, false, true, true, false);
 }
@@ -524,24 +526,20 @@ void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolera
                                                               F.old_infl.data());
     LAUNCH_CHECK();
     std::vector<double> sq(std::max<int64_t>(1, nl));
-    int done = 0;
-    for (int it = 1; it <= max_iterations; ++it) {
+    // one iteration: farfield of the current strengths, the right-hand side's farfield term, the residual
+    auto part_a = [&]() {
         fgs_farfield(E, F);
         k_fgs_extra<<<grid_for(n, 256), 256, 0, C.stream>>>(F.lpos.data(), E.tout.data(), n, F.influence, F.normals.data(),
                                                            F.extra.data(), F.rhs.data());
         LAUNCH_CHECK();
-        double mse = 0.0;
         if (nl) {
             k_fgs_residual<<<(unsigned)nl, 256, 0, C.stream>>>(F.leaf_start.data(), F.self_off.data(), F.self_A.data(),
                                                                F.strengths.data(), F.rhs.data(), F.leaf_sq.data());
             LAUNCH_CHECK();
-            HIP_CHECK(hipMemcpyAsync(sq.data(), F.leaf_sq.data(), sizeof(double) * nl, hipMemcpyDeviceToHost, C.stream));
-            HIP_CHECK(hipStreamSynchronize(C.stream));
-            for (int64_t k = 0; k < nl; ++k) mse += sq[k];
         }
-        mse /= (double)std::max<int64_t>(1, n);
-        if (mse_host) mse_host[it - 1] = mse;
-        if (mse <= tolerance) break;
+    };
+    // the sweep in the reference's leaf order, then the strengths back into the tree's order
+    auto part_b = [&]() {
         for (size_t l = 0; l + 1 < F.level_ptr.size(); ++l) {
             const int64_t a = F.level_ptr[l], cnt = F.level_ptr[l + 1] - a;
             k_fgs_level<true><<<(unsigned)cnt, 256, 0, C.stream>>>(F.level_leaves.data() + a, F.leaf_start.data(), F.self_off.data(),
@@ -550,11 +548,71 @@ void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolera
                                                                    F.strengths.data(), F.rhs.data(), F.old_infl.data());
         }
         LAUNCH_CHECK();
-        ++done;
         k_fgs_to_sorted<<<grid_for(n, 256), 256, 0, C.stream>>>(F.lpos.data(), F.strengths.data(), n, st.strength.data());
         k_fgs_axpy<<<grid_for(n, 256), 256, 0, C.stream>>>(F.extra.data(), n, -1.0, F.rhs.data());
         LAUNCH_CHECK();
+    };
+    auto residual = [&]() {
+        double mse = 0.0;
+        if (nl) {
+            HIP_CHECK(hipMemcpyAsync(sq.data(), F.leaf_sq.data(), sizeof(double) * nl, hipMemcpyDeviceToHost, C.stream));
+            HIP_CHECK(hipStreamSynchronize(C.stream));
+            for (int64_t k = 0; k < nl; ++k) mse += sq[k];
+        }
+        return mse / (double)std::max<int64_t>(1, n);
+    };
+    // Iterations after the first replay one captured graph (sweep, then the next farfield and residual):
+    // its buffers are all sized by the first, eager iteration, and nothing in it synchronises with the
+    // host. FMMHIP_GRAPH=0 launches every kernel directly instead.
+    const char* genv = std::getenv("FMMHIP_GRAPH");
+    bool use_graph = C.graphs && !(genv && genv[0] == '0');
+    hipGraphExec_t gexec = nullptr;
+    // captures part_b + part_a; any failure ends the capture (the stream must not stay in capture mode)
+    // and the solve continues with direct launches
+    auto capture = [&]() -> bool {
+        if (hipStreamBeginCapture(C.stream, hipStreamCaptureModeRelaxed) != hipSuccess) return false;
+        bool ok = true;
+        try {
+            part_b();
+            part_a();
+        } catch (const Error&) {
+            ok = false;
+        }
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(C.stream, &g) != hipSuccess) ok = false;
+        if (ok && hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            gexec = nullptr;
+            ok = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return ok;
+    };
+    int done = 0;
+    if (max_iterations >= 1) part_a();
+    for (int it = 1; it <= max_iterations; ++it) {
+        const double mse = residual();
+        if (mse_host) mse_host[it - 1] = mse;
+        if (mse <= tolerance) break;
+        ++done;
+        if (it == max_iterations) {
+            part_b();
+            break;
+        }
+        if (!use_graph) {
+            part_b();
+            part_a();
+            continue;
+        }
+        if (!gexec && !capture()) {
+            use_graph = false;   // nothing of the failed capture ran: this iteration's work directly
+            part_b();
+            part_a();
+            continue;
+        }
+        HIP_CHECK(hipGraphLaunch(gexec, C.stream));
     }
+    if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
     if (max_iterations < 1) HIP_CHECK(hipMemsetAsync(E.tout.data(), 0, E.tout.size() * sizeof(double), C.stream));
     // final nearfield with the self pairs, on top of the last farfield (:917-931)
     run_p2p(C, E.kernel, E.tt, st, F.full, E.tout.data(), E.nt, true, true, false, FMMHIP_FP64);

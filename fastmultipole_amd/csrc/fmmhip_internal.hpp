@@ -122,6 +122,10 @@ struct Context {
     int n_cu = 0;               // compute units of the device (0 until first needed)
     DBuf<unsigned char> temp;   // scratch for sorts/scans
     void* pinned = nullptr;     // 64 bytes of pinned host memory for scalar read-backs
+    // Launch sequences without host synchronisation may be captured in graphs and replayed. Off for
+    // contexts that share the device with other threads (the request server's workers): a
+    // device-synchronising call there (hipFree) while a stream is being captured invalidates the capture.
+    bool graphs = true;
 };
 
 // Device scalars read back through the context's pinned buffer, one stream synchronisation per call
@@ -146,6 +150,8 @@ inline int64_t read_back_sum(Context& C, const T* a, const T* b) {
     return (int64_t)h[0] + (int64_t)h[1];
 }
 void free_tree_scratch(TreeScratch* s);
+// the request server's worker contexts capture no graphs (api.hip)
+void ctx_disable_graphs(fmmhip_ctx* ctx);
 void free_list_scratch(ListScratch* s);
 
 struct Eval {
@@ -223,9 +229,12 @@ void ensure_tables(Context& C, int p);
 void free_tables(Tables* t);
 void run_p2m(Context& C, int kernel, Tree& st, int p);
 void run_m2m(Context& C, Tree& st, int p, bool LH);
+// pmax / success null: no read-back of the launch's largest order and tolerance flag (no host synchronisation)
 void run_m2l(Context& C, const Tree& tt, Tree& st_src, Tree& tt_dst, Lists& L, int p, bool LH, int err_kind,
              bool err_rel, double eps, bool BE, int precision, int32_t* pmax, int32_t* success);
 void run_l2l(Context& C, Tree& tt, int p, bool LH);
+// zeroes n doubles with a kernel (graph-capturable, unlike hipMemsetAsync)
+void fill_zero(Context& C, double* p, size_t n);
 void run_l2p(Context& C, Tree& tt, double* tout, int64_t nt, int p, bool LH, bool PS, bool GS, bool HS);
 // api.hip helpers
 void gather_outputs(Context& C, const Tree& tt, const double* tout, double* aos, int64_t nt);

@@ -125,6 +125,7 @@ void worker(fmmhip_server* S) {
     fmmhip_ctx* ctx = nullptr;
     fmmhip_eval* ev = nullptr;
     int st = fmmhip_ctx_create(S->device, &ctx);
+    if (st == FMMHIP_OK) fmmhip::ctx_disable_graphs(ctx);   // workers run beside each other: no stream capture
     if (st == FMMHIP_OK) st = fmmhip_eval_create(ctx, &ev);
     {
         std::lock_guard<std::mutex> lk(S->mu);
