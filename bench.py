@@ -412,15 +412,16 @@ def main():
                 "bound": "vector-alu", "achieved": round(p2p_tflops, 3),
                 "peak": PEAK_FP32_TFLOPS if main_form == "mixed" else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "traffic": pmc_traffic(key)}
-    # the error-controlled M2L: a first launch at the order cap 18 (eight wavefronts per workgroup), then
-    # the deferred interactions at the full order (six); achieved = F(p) flops summed over the
-    # interactions / the M2L phase time (the two launches back to back)
-    m2l_keys = ("k_m2l<true, 8>", "k_m2l<true, 6>")
-    m2l_traffic = [pmc_traffic(k) for k in m2l_keys]
-    roof_m2l = {"kernel": " + ".join(m2l_keys) + " (rotation-based M2L with Lamb-Helmholtz terms, FP64, both launches; "
-                                                 "F(p) flop/interaction)",
-                "bound": "vector-alu", "achieved": round(m2l_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "traffic": None if None in m2l_traffic else sum(m2l_traffic)}
+    # the error-controlled M2L: one persistent launch of the matrix-core engine at the order cap (the
+    # smallest instantiated cap >= pmax, specialised for order == cap); achieved = F(p) flops summed over
+    # the interactions at the orders they used / the M2L phase time (the launch and the per-box ordering
+    # sort before it)
+    cap = next(c for c in (8, 12, 16, 20) if c >= args.pmax)
+    m2l_key = f"k_m2l_mx<{cap}, true, 4, {'true' if cap == args.pmax else 'false'}>"
+    roof_m2l = {"kernel": m2l_key + " (rotation-based M2L with Lamb-Helmholtz terms on v_mfma_f64_16x16x4f64, FP64; "
+                                    "F(p) flop/interaction)",
+                "bound": "mfma", "achieved": round(m2l_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "traffic": pmc_traffic(m2l_key)}
     roofs = [roof_p2p, roof_m2l] if ph["ms_p2p"] >= ph["ms_m2l"] else [roof_m2l, roof_p2p]
     for r in roofs:
         r["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, profiles/ring1m_traffic_pmc.md)"

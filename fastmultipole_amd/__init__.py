@@ -9,7 +9,7 @@ from .core import (Context, Evaluation, DerivativesSwitch, ErrorMethod, Interact
                    PowerAbsolutePotential, PowerRelativeGradient, PowerRelativePotential,
                    RotatedCoefficientsAbsoluteGradient, RotatedCoefficientsRelativeGradient, Barba, SelfTuning,
                    SelfTuningTreeStop, default_context, direct, fmm, make_options, FastGaussSeidel, Server, solve,
-                   tune_fmm)
+                   tune_fmm, autotune)
 from .systems import (DipolePanels, DipolePoints, GaussianVortexParticles, Gravitational, SourcePanels,
                       VortexParticles)
 
@@ -18,4 +18,4 @@ __all__ = ["Context", "Evaluation", "DerivativesSwitch", "ErrorMethod", "Interac
            "RotatedCoefficientsAbsoluteGradient", "RotatedCoefficientsRelativeGradient", "Barba", "SelfTuning",
            "SelfTuningTreeStop", "default_context", "direct", "fmm", "make_options", "Gravitational",
            "VortexParticles", "DipolePoints", "GaussianVortexParticles", "SourcePanels", "DipolePanels",
-           "FastGaussSeidel", "Server", "solve", "tune_fmm"]
+           "FastGaussSeidel", "Server", "solve", "tune_fmm", "autotune"]

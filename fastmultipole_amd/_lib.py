@@ -20,6 +20,7 @@ ERR_NONE, ERR_ROTATED_COEFFICIENTS_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRAD
 
 EXPORTED = [
     "fmmhip_options_default", "fmmhip_ctx_create", "fmmhip_ctx_destroy", "fmmhip_last_error", "fmmhip_sync",
+    "fmmhip_ctx_set_p2p_free_slots",
     "fmmhip_eval_create", "fmmhip_eval_destroy", "fmmhip_eval_set_systems", "fmmhip_eval_build_trees",
     "fmmhip_eval_build_lists", "fmmhip_eval_run", "fmmhip_fmm", "fmmhip_eval_get_targets", "fmmhip_eval_n_branches",
     "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_get_branches", "fmmhip_eval_get_levels",
@@ -113,6 +114,7 @@ def load():
     L.fmmhip_last_error.argtypes = [vp]
     L.fmmhip_last_error.restype = C.c_char_p
     L.fmmhip_sync.argtypes = [vp]
+    L.fmmhip_ctx_set_p2p_free_slots.argtypes = [vp, i32]
     L.fmmhip_eval_create.argtypes = [vp, C.POINTER(vp)]
     L.fmmhip_eval_destroy.argtypes = [vp]
     L.fmmhip_eval_destroy.restype = None

@@ -97,6 +97,10 @@ class Context:
         if st != L.FMMHIP_OK:
             raise L.FmmHipError(st, self.lib.fmmhip_last_error(self._h).decode())
 
+    def set_p2p_free_slots(self, slots):
+        """Workgroup slots per compute unit the nearfield leaves to the far-field stream (0..3)."""
+        self.check(self.lib.fmmhip_ctx_set_p2p_free_slots(self._h, int(slots)))
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.fmmhip_ctx_destroy(self._h)
@@ -588,6 +592,68 @@ def tune_fmm(target_systems, source_systems=None, *, error_tolerance=None, expan
               f"\texpansion_order:     {best[1]}\n\tmultipole_acceptance: {best[3]}\n\tcost:                {best[0]} seconds"
               "\n\n#===============================================#\n")
     return tuned, cache
+
+
+def autotune(system, *, error_tolerance=None, expansion_order=20, leaf_sizes=(16, 24, 32, 48, 64, 96, 128),
+             multipole_acceptances=(0.4, 0.45, 0.5, 0.55, 0.6), free_slots=(0, 1, 2), start=None, rounds=2,
+             evaluation=None, **kwargs):
+    """The project's own tuner for this GPU (no counterpart in the reference; tune_fmm above is the
+    reference's heuristic). Every candidate is a complete evaluation of `system` (trees, lists,
+    estimate_influence, all passes), priced by the library's device phase times (ms_tree + ms_lists +
+    ms_total of the second of two runs). Coordinate descent over the leaf size, the multipole acceptance
+    (error-controlled orders only: with a fixed order the acceptance sets the accuracy and is kept) and
+    the nearfield's launch shape (workgroup slots per compute unit left to the far-field stream), for
+    `rounds` rounds from `start` (default: the middle candidates). Settings whose evaluation misses the
+    error tolerance (error_success false) are never chosen.
+
+    Returns (params, evaluation, history): params has leaf_size, multipole_acceptance, p2p_free_slots,
+    ms (device time) and expansion_order (the largest order used); evaluation is left at the chosen
+    setting; history lists every measured candidate."""
+    kernel = _kernel_of(system)
+    ev = evaluation or Evaluation()
+    ev.set_systems(kernel, None, system.source_buffer())
+    fixed = error_tolerance is None
+    macs = (kwargs.pop("multipole_acceptance", 0.5),) if fixed else tuple(multipole_acceptances)
+    cur = dict(start or {})
+    cur.setdefault("leaf_size", leaf_sizes[len(leaf_sizes) // 2])
+    cur.setdefault("multipole_acceptance", macs[len(macs) // 2])
+    cur.setdefault("p2p_free_slots", 1)
+    seen, history = {}, []
+
+    def price(leaf, mac, slots):
+        key = (leaf, mac, slots)
+        if key in seen:
+            return seen[key]
+        ev.ctx.set_p2p_free_slots(slots)
+        opt = make_options(expansion_order=expansion_order, leaf_size_source=leaf, multipole_acceptance=mac,
+                           error_tolerance=error_tolerance, **kwargs)
+        ev.fmm(opt)
+        st = ev.fmm(opt)
+        ms = st.ms_tree + st.ms_lists + st.ms_total
+        ok = bool(st.error_success)
+        seen[key] = (ms, ok, int(st.pmax))
+        history.append({"leaf_size": leaf, "multipole_acceptance": mac, "p2p_free_slots": slots, "ms": ms,
+                        "error_success": ok, "pmax": int(st.pmax)})
+        return seen[key]
+
+    def better(a, b):   # a, b: (ms, ok, pmax); feasible beats infeasible, then faster
+        return (a[1] and not b[1]) or (a[1] == b[1] and a[0] < b[0])
+
+    best = price(cur["leaf_size"], cur["multipole_acceptance"], cur["p2p_free_slots"])
+    for _ in range(rounds):
+        for name, cands in (("leaf_size", leaf_sizes), ("multipole_acceptance", macs), ("p2p_free_slots", free_slots)):
+            for v in cands:
+                trial = dict(cur, **{name: v})
+                r = price(trial["leaf_size"], trial["multipole_acceptance"], trial["p2p_free_slots"])
+                if better(r, best):
+                    best, cur = r, trial
+    # leave the evaluation and its context at the chosen setting
+    price_key = (cur["leaf_size"], cur["multipole_acceptance"], cur["p2p_free_slots"])
+    ev.ctx.set_p2p_free_slots(cur["p2p_free_slots"])
+    ev.fmm(make_options(expansion_order=expansion_order, leaf_size_source=cur["leaf_size"],
+                        multipole_acceptance=cur["multipole_acceptance"], error_tolerance=error_tolerance, **kwargs))
+    params = dict(cur, ms=seen[price_key][0], error_success=seen[price_key][1], expansion_order=seen[price_key][2])
+    return params, ev, history
 
 
 def direct(target_systems, source_systems=None, *, scalar_potential=False, gradient=True, hessian=False, precision="fp64",

@@ -261,7 +261,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         // rate is about the same at 3 and 4 workgroups per CU, and the far field's kernels (the
         // estimate's tree, list and sort launches) are placed at once instead of waiting for
         // the nearfield's last workgroups.
-        const int slots = &F != &C ? -1 : 0;
+        const int slots = &F != &C ? -C.p2p_free_slots : 0;
         run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision, 0, -1, slots);
         tp.mark();
         if (estimate) {
@@ -397,6 +397,15 @@ fmmhip_status fmmhip_sync(fmmhip_ctx* h) {
     if (!h) return FMMHIP_EINVAL;
     Context* C = &h->c;
     ABI_TRY(C, HIP_CHECK(hipStreamSynchronize(C->stream)));
+}
+
+fmmhip_status fmmhip_ctx_set_p2p_free_slots(fmmhip_ctx* h, int32_t slots) {
+    if (!h) return FMMHIP_EINVAL;
+    Context* C = &h->c;
+    ABI_TRY(C, {
+        if (slots < 0 || slots > 3) throw Error(FMMHIP_EINVAL, "free slots per compute unit must be 0..3");
+        C->p2p_free_slots = slots;
+    });
 }
 
 fmmhip_status fmmhip_eval_create(fmmhip_ctx* h, fmmhip_eval** out) {

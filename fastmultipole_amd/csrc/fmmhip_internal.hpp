@@ -126,6 +126,7 @@ struct Context {
     // contexts that share the device with other threads (the request server's workers): a
     // device-synchronising call there (hipFree) while a stream is being captured invalidates the capture.
     bool graphs = true;
+    int p2p_free_slots = 1;     // workgroup slots per CU the nearfield leaves to the far-field stream
 };
 
 // Device scalars read back through the context's pinned buffer, one stream synchronisation per call

@@ -226,8 +226,7 @@ static constexpr int MX_NW = 4;   // wavefronts per workgroup (one per SIMD at t
 template <int CAP>
 void run_m2l_mx_cap(Context& C, int p, bool LH, const Lists& Ls, const Tree& st, Tree& tt_dst, const ErrParams& E,
                     const int32_t* perm, int32_t* flags, const TablePtrs& tb) {
-    static const bool full_ok = std::getenv("FMMHIP_MX_NOFULL") == nullptr;   // measurement switch
-    if (p == CAP && full_ok) {
+    if (p == CAP) {   // degree loops of fixed length (34.1 against 34.4 ms on the 1M ring at cap 20)
         if (LH) launch_m2l_mx<CAP, true, MX_NW, true>(C, p, p, Ls, st, tt_dst, E, perm, flags, tb);
         else launch_m2l_mx<CAP, false, MX_NW, true>(C, p, p, Ls, st, tt_dst, E, perm, flags, tb);
     } else {

@@ -119,6 +119,10 @@ fmmhip_status fmmhip_ctx_create(int device, fmmhip_ctx** out);
 void fmmhip_ctx_destroy(fmmhip_ctx* ctx);
 const char* fmmhip_last_error(const fmmhip_ctx* ctx);
 fmmhip_status fmmhip_sync(fmmhip_ctx* ctx);
+/* Launch shape of later evaluations on this context (no counterpart in the reference; set by the
+ * project's autotuner, core.autotune): workgroup slots per compute unit the persistent nearfield leaves
+ * free for the far-field stream (0..3, default 1). */
+fmmhip_status fmmhip_ctx_set_p2p_free_slots(fmmhip_ctx* ctx, int32_t slots);
 
 /* An evaluation keeps trees, lists, expansions and buffers resident in HBM between
  * calls (the reference's Cache + Tree objects, src/containers.jl:264-292). */

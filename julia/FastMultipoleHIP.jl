@@ -206,6 +206,11 @@ mutable struct HipEval
     end
 end
 
+# launch shape of later evaluations (set by the GPU-side autotuner): workgroup slots per compute unit
+# the nearfield leaves to the far-field stream, 0..3
+set_p2p_free_slots!(h::HipEval, slots::Integer) =
+    check(ccall((:fmmhip_ctx_set_p2p_free_slots, libfmmhip), Cint, (Ptr{Cvoid}, Cint), h.ctx, slots), h.ctx)
+
 function close!(h::HipEval)
     if h.ev != C_NULL
         ccall((:fmmhip_eval_destroy, libfmmhip), Cvoid, (Ptr{Cvoid},), h.ev)

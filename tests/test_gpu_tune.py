@@ -59,3 +59,27 @@ def test_tune_fmm_meets_tolerance():
     assert ok
     err = np.abs(system.potential[:, 0] - ref.potential[:, 0]).max()
     assert err <= 1e-4, (params, err)
+
+
+def test_autotune_measured_search_meets_tolerance():
+    """core.autotune (the project's own tuner): every candidate is measured on the device, the chosen
+    setting is the fastest measured one among those meeting the tolerance, its launch shape is applied
+    to the context, and fmm! at the chosen setting meets the tolerance against the FP64 direct sum."""
+    print("[gpu] autotune", flush=True)
+    buf = W.vortex_ring(30000, seed=2)
+    sys_ = F.VortexParticles(buf[:, 0:3], buf[:, 4:7], buf[:, 3])
+    em = F.PowerRelativeGradient(1e-7)
+    params, ev, hist = F.autotune(sys_, error_tolerance=em, expansion_order=20, leaf_sizes=(16, 32, 64),
+                                  multipole_acceptances=(0.45, 0.55), free_slots=(0, 1), rounds=1, gradient=True)
+    print(f"[gpu] autotune chose {params} from {len(hist)} candidates", flush=True)
+    assert len(hist) >= 5 and all(h["ms"] > 0 for h in hist)
+    feasible = [h for h in hist if h["error_success"]]
+    assert feasible and params["error_success"]
+    assert params["ms"] == min(h["ms"] for h in feasible)
+    assert params["leaf_size"] in (16, 32, 64) and params["multipole_acceptance"] in (0.45, 0.55)
+    _, _, *_, ok = F.fmm(sys_, expansion_order=20, leaf_size=params["leaf_size"],
+                         multipole_acceptance=params["multipole_acceptance"], error_tolerance=em, gradient=True,
+                         update_target_systems=False, evaluation=ev)
+    assert ok
+    with pytest.raises(Exception):
+        ev.ctx.set_p2p_free_slots(4)
