@@ -8,14 +8,14 @@ from . import _lib
 from .core import (Context, Evaluation, DerivativesSwitch, ErrorMethod, InteractionListMethod, PowerAbsoluteGradient,
                    PowerAbsolutePotential, PowerRelativeGradient, PowerRelativePotential,
                    RotatedCoefficientsAbsoluteGradient, RotatedCoefficientsRelativeGradient, Barba, SelfTuning,
-                   SelfTuningTreeStop, default_context, direct, fmm, make_options, FastGaussSeidel, Server, solve,
-                   tune_fmm, autotune)
+                   SelfTuningTreeStop, default_context, direct, fmm, fmm_batch, make_options, FastGaussSeidel, Server,
+                   solve, tune_fmm, autotune)
 from .systems import (DipolePanels, DipolePoints, GaussianVortexParticles, Gravitational, SourcePanels,
                       VortexParticles)
 
 __all__ = ["Context", "Evaluation", "DerivativesSwitch", "ErrorMethod", "InteractionListMethod",
            "PowerAbsoluteGradient", "PowerAbsolutePotential", "PowerRelativeGradient", "PowerRelativePotential",
            "RotatedCoefficientsAbsoluteGradient", "RotatedCoefficientsRelativeGradient", "Barba", "SelfTuning",
-           "SelfTuningTreeStop", "default_context", "direct", "fmm", "make_options", "Gravitational",
+           "SelfTuningTreeStop", "default_context", "direct", "fmm", "fmm_batch", "make_options", "Gravitational",
            "VortexParticles", "DipolePoints", "GaussianVortexParticles", "SourcePanels", "DipolePanels",
            "FastGaussSeidel", "Server", "solve", "tune_fmm", "autotune"]

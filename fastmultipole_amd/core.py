@@ -240,6 +240,27 @@ class Evaluation:
         self.kernel, self.nt, self.ns = kernel, nt, ns
         self.ctx.check(self.ctx.lib.fmmhip_eval_set_systems(self._h, kernel, nt, tp or None, ns, sp or None, int(on_device)))
 
+    def set_batch(self, kernel, source_buffers, on_device=False):
+        """Several independent systems, each its own target and source (fmmhip_eval_set_batch).
+
+        source_buffers: a list of (n_i, data_per_body) arrays, or with on_device=True one device buffer of the
+        concatenated columns (an object with data_ptr() and shape) followed by the list of counts."""
+        if on_device:
+            buf, counts = source_buffers
+            counts = np.ascontiguousarray(counts, dtype=np.int64)
+            sp = buf.data_ptr() if hasattr(buf, "data_ptr") else int(buf)
+            self._keep = (buf, counts)
+        else:
+            counts = np.array([b.shape[0] for b in source_buffers], dtype=np.int64)
+            sb = np.ascontiguousarray(np.concatenate([np.asarray(b, dtype=np.float64) for b in source_buffers]))
+            self._keep = (sb, counts)
+            sp = sb.ctypes.data
+        n = int(counts.sum())
+        self.kernel, self.nt, self.ns = kernel, n, n
+        self.counts = counts
+        self.ctx.check(self.ctx.lib.fmmhip_eval_set_batch(self._h, kernel, len(counts), counts.ctypes.data, sp,
+                                                          int(on_device)))
+
     def build_trees(self, opt):
         self.ctx.check(self.ctx.lib.fmmhip_eval_build_trees(self._h, C.byref(opt)))
 
@@ -509,6 +530,35 @@ def fmm(target_systems, source_systems=None, *, leaf_size=None, leaf_size_source
                       "multipole_acceptance": opt.multipole_acceptance}
     ev.last_stats = stats
     return (optimized_args, ev, ev.tree(0), ev.tree(1), ev.list(0), ev.list(1), (switch,), bool(stats.error_success))
+
+
+def fmm_batch(systems, *, leaf_size=20, expansion_order=5, multipole_acceptance=0.4, error_tolerance=None,
+              shrink_recenter=True, interaction_list_method=None, scalar_potential=False, gradient=True, hessian=False,
+              update_target_systems=True, precision="fp64", evaluation=None):
+    """fmm!(system) (src/fmm.jl:807-915) for each of several independent systems of one kernel, in one
+    evaluation: one tree per system, lists within each system, every pass of all systems in the same
+    launches. Each system's results are those of its own fmm! call. Returns (outputs, evaluation): outputs[i]
+    is system i's (n_i, 16) target buffer; evaluation.last_stats covers the whole batch (error_success is
+    false if any system missed the tolerance)."""
+    systems = list(systems)
+    if not systems:
+        return [], None
+    kernel = _kernel_of(systems[0])
+    if any(_kernel_of(s) != kernel for s in systems):
+        raise ValueError("the systems of one batch share one kernel")
+    ev = evaluation or Evaluation()
+    ev.set_batch(kernel, [s.source_buffer() for s in systems])
+    opt = make_options(expansion_order, leaf_size, leaf_size, multipole_acceptance, interaction_list_method,
+                       shrink_recenter, error_tolerance, scalar_potential, gradient, hessian, True, True, True, precision)
+    ev.last_stats = ev.fmm(opt)
+    out = ev.targets()
+    switch = DerivativesSwitch(bool(scalar_potential), bool(gradient), bool(hessian))
+    offs = np.concatenate([[0], np.cumsum(ev.counts)])
+    outputs = [out[offs[i]:offs[i + 1]] for i in range(len(systems))]
+    if update_target_systems:
+        for s, o in zip(systems, outputs):
+            s.buffer_to_target_system(o, switch)
+    return outputs, ev
 
 
 def _tuned_leaf_size(stats, leaf_size_source):

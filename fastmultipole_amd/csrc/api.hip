@@ -105,6 +105,8 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
     Context& C = *E.ctx;
     const int dpb = data_per_body(E.kernel);
     const int p = o.expansion_order;
+    E.tt.seg = E.seg;
+    E.st.seg = E.seg;
     if (E.t_xyz) build_tree(C, E.tt, E.nt, E.t_xyz, 3, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
     else build_tree(C, E.tt, E.nt, E.s_buf, dpb, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
     build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
@@ -198,6 +200,8 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
     X.ns = E.ns;
     X.t_xyz = E.t_xyz;
     X.s_buf = E.s_buf;
+    X.seg = E.seg;
+    X.st.seg = E.seg;
     const fmmhip_options eo = estimate_options(o);
     if (X.t_xyz) eval_build_trees(X, eo);
     else {   // the targets are the sources: one build gives both trees; the Morton keys and their
@@ -264,6 +268,19 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         const int slots = &F != &C ? -C.p2p_free_slots : 0;
         run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision, 0, -1, slots);
         tp.mark();
+        // With estimate_influence on the far-field stream, the upward pass follows the nearfield on its
+        // stream: it needs nothing of the estimate, so it runs beside the estimate's evaluation once the
+        // nearfield is done (103.0 against 104.9 ms per evaluation of the 1M ring with it on the far-field
+        // stream after the estimate; the estimate's own nearfield and far field on two streams gained
+        // nothing). Without the estimate it stays on the far-field stream, beside the nearfield.
+        const bool up_on_c = estimate && &F != &C;
+        std::unique_ptr<Timer> tu;
+        if (up_on_c) {
+            tu.reset(new Timer(C));
+            run_p2m(C, E.kernel, E.st, p);
+            run_m2m(C, E.st, p, LH);
+            tu->mark();
+        }
         if (estimate) {
             // estimate_influence (trees, lists, evaluation) on the far-field stream beside the nearfield.
             // Measured alternatives: its trees and lists first, alone on the device (110.1 against 107.5 ms
@@ -274,11 +291,13 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
             estimate_eval(E, o, F);
             t_est = t.stop();
         }
-        {
+        if (!up_on_c) {
             Timer t(F);
             run_p2m(F, E.kernel, E.st, p);
             run_m2m(F, E.st, p, LH);
             t_up = t.stop();
+        } else {
+            stream_wait(F, C);
         }
         {
             Timer t(F);
@@ -295,6 +314,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
             run_l2l(F, E.tt, p, LH);
             t_down = t.stop();
         }
+        if (tu) t_up = tu->stop();
         if (&F != &C) stream_wait(C, F);
         {
             Timer t(C);
@@ -434,6 +454,7 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
         E.kernel = kernel;
         E.nt = nt;
         E.ns = ns;
+        E.seg.clear();
         const int dpb = data_per_body(kernel);
         if (on_device) {
             E.t_xyz = t_xyz;
@@ -449,6 +470,38 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
             } else E.t_xyz = nullptr;
             HIP_CHECK(hipStreamSynchronize(C->stream));
         }
+        E.trees_built = E.lists_built = E.ran = false;
+    });
+}
+
+fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_systems, const int64_t* counts,
+                                     const double* s_buf, int32_t on_device) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (!valid_kernel(kernel)) throw Error(FMMHIP_EINVAL, "unknown kernel");
+        if (n_systems < 1 || !counts) throw Error(FMMHIP_EINVAL, "a batch needs at least one system");
+        std::vector<int64_t> seg(n_systems + 1, 0);
+        for (int64_t i = 0; i < n_systems; ++i) {
+            if (counts[i] < 1) throw Error(FMMHIP_EINVAL, "every system of a batch needs at least one body");
+            seg[i + 1] = seg[i] + counts[i];
+        }
+        const int64_t n = seg[n_systems];
+        if (!s_buf) throw Error(FMMHIP_EINVAL, "missing system buffer");
+        if (n >= (int64_t)1 << 31) throw Error(FMMHIP_EINVAL, "more than 2^31 bodies in one batch");
+        E.kernel = kernel;
+        E.nt = E.ns = n;
+        E.t_xyz = nullptr;
+        const int dpb = data_per_body(kernel);
+        if (on_device) E.s_buf = s_buf;
+        else {
+            E.s_buf_in.resize((int64_t)dpb * n);
+            HIP_CHECK(hipMemcpyAsync(E.s_buf_in.data(), s_buf, sizeof(double) * dpb * n, hipMemcpyHostToDevice, C->stream));
+            HIP_CHECK(hipStreamSynchronize(C->stream));
+            E.s_buf = E.s_buf_in.data();
+        }
+        E.seg = n_systems > 1 ? seg : std::vector<int64_t>();
         E.trees_built = E.lists_built = E.ran = false;
     });
 }
@@ -676,7 +729,8 @@ fmmhip_status fmmhip_fgs_setup(fmmhip_eval* ev, const fmmhip_options* opt, int32
             throw Error(FMMHIP_EINVAL, "unknown influence");
         if (influence == FMMHIP_INFLUENCE_NORMAL_GRADIENT && !is_panel(E.kernel))
             throw Error(FMMHIP_EINVAL, "the normal-gradient influence needs panels");
-        if (E.t_xyz || E.nt != E.ns) throw Error(FMMHIP_EINVAL, "FastGaussSeidel needs one system as target and source");
+        if (E.t_xyz || E.nt != E.ns || !E.seg.empty())
+            throw Error(FMMHIP_EINVAL, "FastGaussSeidel needs one system as target and source");
         // FastGaussSeidel's fixed choices (src/solve.jl:538-571): Barba lists, farfield and
         // nearfield without self pairs, one leaf size, potential and gradient
         fmmhip_options o = *opt;

@@ -81,6 +81,7 @@ struct Tree {
     DBuf<double> expansions;         // [branch][H][2][2] (reference layout per branch)
     int p = 0;
     uint64_t build_id = 0;           // identifies the build whose keys a context's scratch may still hold
+    std::vector<int64_t> seg;        // forest of independent systems: their body offsets (n_systems + 1); else empty
 };
 
 struct Lists {
@@ -94,6 +95,9 @@ struct Lists {
     DBuf<int32_t> dir_targets;                       // unique target branches in the direct list
     DBuf<int64_t> dir_offsets;                       // n_direct_targets + 1
     DBuf<int64_t> dir_cum;                           // n_direct + 1: exclusive prefix of source leaf sizes
+    // direct targets can nest (SelfTuning pairs branches that are not leaves): the positions in
+    // dir_targets where each level of the target tree starts, plus n_direct_targets; else empty
+    std::vector<int64_t> dir_level_split;
     int64_t n_m2l_targets = 0;
     DBuf<int32_t> m2l_targets;
     DBuf<int64_t> m2l_offsets;
@@ -162,6 +166,8 @@ struct Eval {
     DBuf<double> t_xyz_in, s_buf_in;          // host-copied inputs (when on_device == 0)
     const double* t_xyz = nullptr;            // device pointer to 3 x nt
     const double* s_buf = nullptr;            // device pointer to dpb x ns
+    std::vector<int64_t> seg;                 // several independent systems (each its own target and source):
+                                              // their body offsets; empty for one target and one source system
     Tree tt, st;
     Lists lists;
     DBuf<double> tout;                        // sorted target outputs, SoA [13][nt]: phi, v(3), h(9)

@@ -181,7 +181,11 @@ __global__ void k_dir_counts(const int32_t* __restrict__ s, int64_t n, const Bra
     else if (i == n) cnt[i] = 0;
 }
 
-__global__ void k_init_root(int32_t* ft, int32_t* fs, uint64_t* off) { ft[0] = 0; fs[0] = 0; off[0] = 0; }
+// the recursion starts from (root, root) of every system; roots are branches 0 .. n - 1 of both trees
+__global__ void k_init_roots(int32_t* ft, int32_t* fs, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) { ft[i] = (int32_t)i; fs[i] = (int32_t)i; }
+}
 
 __global__ void k_zero_counter(unsigned long long* c) { *c = 0; }
 
@@ -251,15 +255,19 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
     if (tt.n_branches == 0 || st.n_branches == 0) return;
     ListScratch& S = lscratch(C);
     ListParams P{ls, mac, method, ff, nf, si};
+    // a forest of independent systems (one root per system) pairs each system with itself only
+    const int64_t n_roots = tt.seg.size() > 2 ? (int64_t)tt.seg.size() - 1 : 1;
+    if ((st.seg.size() > 2 ? (int64_t)st.seg.size() - 1 : 1) != n_roots)
+        throw Error(FMMHIP_EINVAL, "target and source forests hold different numbers of systems");
     // forward: levels of pending pairs
     std::vector<int64_t> sizes;
     {
         ListLevel& L0 = S.level(0);
-        L0.ft.resize(1); L0.fs.resize(1); L0.off.resize(1);
-        k_init_root<<<1, 1, 0, C.stream>>>(L0.ft.data(), L0.fs.data(), L0.off.data());
+        L0.ft.resize(n_roots); L0.fs.resize(n_roots); L0.off.resize(n_roots);
+        k_init_roots<<<grid_for(n_roots, 256), 256, 0, C.stream>>>(L0.ft.data(), L0.fs.data(), n_roots);
         LAUNCH_CHECK();
     }
-    int64_t n = 1;
+    int64_t n = n_roots;
     for (size_t k = 0;; ++k) {
         if (k >= 4096) throw Error(FMMHIP_EDEPTH, "interaction-list recursion deeper than 4096 levels");
         ListLevel& A = S.level(k);
@@ -289,7 +297,10 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
         k_list_sizes<<<grid_for(sizes[k], 256), 256, 0, C.stream>>>(sizes[k], A.dec.data(), A.pos.data(), child, A.size.data());
         LAUNCH_CHECK();
     }
-    const uint64_t total = read_back(C, S.level(0).size.data());
+    // depth-first positions of the roots' subtrees: exclusive scan of their packed counts
+    ListLevel& L0 = S.level(0);
+    excl_scan(C, L0.size.data(), L0.off.data(), n_roots);
+    const uint64_t total = (uint64_t)read_back_sum(C, L0.off.data() + (n_roots - 1), L0.size.data() + (n_roots - 1));
     L.n_m2l = (int64_t)(total >> 32);
     L.n_direct = (int64_t)(total & 0xffffffffull);
     S.kt_m2l.resize(std::max<int64_t>(1, L.n_m2l)); S.vs_m2l.resize(std::max<int64_t>(1, L.n_m2l));
@@ -324,6 +335,19 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
                 L.n_m2l_targets);
     finish_list(C, S, S.kt_dir.data(), S.vs_dir.data(), L.n_direct, bits, L.dir_t, L.dir_s, L.dir_targets, L.dir_offsets,
                 L.n_direct_targets);
+    L.dir_level_split.clear();
+    if (method == FMMHIP_SELFTUNING && L.n_direct_targets > 0) {
+        // SelfTuning emits direct pairs between branches that are not leaves (src/interaction_list.jl:84-88),
+        // so one direct target's bodies can lie inside another's. The nearfield then runs one launch per
+        // level of targets (targets of one level are disjoint), ancestors first: the order of the
+        // reference's target-sorted list.
+        std::vector<int32_t> t(L.n_direct_targets);
+        HIP_CHECK(hipMemcpyAsync(t.data(), L.dir_targets.data(), t.size() * sizeof(int32_t), hipMemcpyDeviceToHost, C.stream));
+        HIP_CHECK(hipStreamSynchronize(C.stream));
+        for (size_t l = 0; l < tt.level_first.size(); ++l)
+            L.dir_level_split.push_back(std::lower_bound(t.begin(), t.end(), (int32_t)tt.level_first[l]) - t.begin());
+        L.dir_level_split.push_back(L.n_direct_targets);
+    }
     L.m2l_order.resize(std::max<int64_t>(1, L.n_m2l));
     if (L.n_direct > 0) {
         S.pos.resize(L.n_direct + 1);

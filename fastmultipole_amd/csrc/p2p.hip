@@ -553,12 +553,28 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
 
 void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
                     bool GS, bool HS, int64_t b0, int64_t b1);
+// targets b0 .. b1 - 1 of the direct list's CSR, none inside another
+static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
+                          bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots);
 void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
                        double* tout_aos, bool PS, bool GS, bool HS);
 
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
              bool HS, int precision, int64_t b0, int64_t b1, int slots) {
     if (b1 < 0) b1 = L.n_direct_targets;
+    if (!L.dir_level_split.empty()) {   // nested targets: one launch per level, ancestors first
+        const std::vector<int64_t>& s = L.dir_level_split;
+        for (size_t l = 0; l + 1 < s.size(); ++l) {
+            const int64_t a = std::max(b0, s[l]), b = std::min(b1, s[l + 1]);
+            if (a < b) run_p2p_range(C, kernel, tt, st, L, tout, nt, PS, GS, HS, precision, a, b, slots);
+        }
+        return;
+    }
+    run_p2p_range(C, kernel, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots);
+}
+
+static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
+                          bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots) {
     if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS, b0, b1); return; }
     switch (kernel) {
         case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;

@@ -11,6 +11,12 @@
 // searches, using the reference's leaf test on the node counts. A second sort
 // by (leaf, original index) restores the reference's in-leaf order.
 //
+// A forest (several independent systems, Tree.seg holding their body offsets) is
+// built in the same launches: each system gets its own root cube from its own
+// bodies, the keys are sorted within each system's range, and the roots are
+// branches 0 .. n_systems - 1 of level 0, so every system's subtree is the tree
+// the reference builds for that system alone.
+//
 // This file is compiled with -ffp-contract=off: every floating-point result
 // here (centres, boxes, radii, the leaf test, the MAC) must match the
 // reference's IEEE double arithmetic bit for bit.
@@ -46,11 +52,14 @@ __device__ __host__ inline bool exceeds(int64_t n, int64_t ls, int method) {
 }
 
 // ------------------------------------------------------------- bounding box
-__global__ void k_bbox_partial(const double* __restrict__ buf, int rows, int64_t n, double* __restrict__ part) {
+// block (x, y) reduces its share of system y's bodies (seg null: one system, bodies 0 .. n - 1)
+__global__ void k_bbox_partial(const double* __restrict__ buf, int rows, int64_t n, const int64_t* __restrict__ seg,
+                               double* __restrict__ part) {
     __shared__ double s[6][256];
+    const int64_t f = seg ? seg[blockIdx.y] : 0, e = seg ? seg[blockIdx.y + 1] : n;
     double mn[3], mx[3];
-    for (int d = 0; d < 3; ++d) { mn[d] = buf[d]; mx[d] = buf[d]; }
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    for (int d = 0; d < 3; ++d) { mn[d] = buf[f * rows + d]; mx[d] = buf[f * rows + d]; }
+    for (int64_t i = f + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e; i += (int64_t)gridDim.x * blockDim.x)
         for (int d = 0; d < 3; ++d) {
             double v = buf[i * rows + d];
             mn[d] = jmin(mn[d], v);
@@ -67,17 +76,28 @@ __global__ void k_bbox_partial(const double* __restrict__ buf, int rows, int64_t
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int d = 0; d < 6; ++d) part[blockIdx.x * 6 + d] = s[d][0];
+        for (int d = 0; d < 6; ++d) part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + d] = s[d][0];
 }
 
 // ---------------------------------------------------------------- path keys
-__global__ void k_path_keys(const double* __restrict__ buf, int rows, int64_t n, double c0, double c1, double c2,
-                            double bmax, uint64_t* __restrict__ khi, uint64_t* __restrict__ klo,
-                            uint32_t* __restrict__ idx) {
+// roots: (centre x, y, z, half side) of the root cube, one per system; seg: the systems' body offsets
+// (n_seg + 1 entries), or null for one system
+__global__ void k_path_keys(const double* __restrict__ buf, int rows, int64_t n, const double* __restrict__ roots,
+                            const int64_t* __restrict__ seg, int n_seg, uint64_t* __restrict__ khi,
+                            uint64_t* __restrict__ klo, uint32_t* __restrict__ idx) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    int r = 0;
+    if (seg) {   // last system whose first body is at or before i
+        int lo = 0, hi = n_seg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (seg[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        r = lo;
+    }
     const double x = buf[i * rows + 0], y = buf[i * rows + 1], z = buf[i * rows + 2];
-    double cx = c0, cy = c1, cz = c2, box = bmax;
+    double cx = roots[4 * r], cy = roots[4 * r + 1], cz = roots[4 * r + 2], box = roots[4 * r + 3];
     uint64_t w[2] = {0, 0};
     for (int word = 0; word < 2; ++word)
         for (int l = 0; l < LEVELS_PER_WORD; ++l) {
@@ -313,12 +333,10 @@ __global__ void k_shrink_level(Branch* __restrict__ br, int64_t b0, int64_t b1, 
     b.source_radius = sr;
 }
 
-__global__ void k_init_root(Branch r, Branch* __restrict__ br, int64_t* __restrict__ act_branch, int64_t* __restrict__ act_first,
-                            int64_t* __restrict__ act_count, int64_t n) {
-    br[0] = r;
-    act_branch[0] = 0;
-    act_first[0] = 0;
-    act_count[0] = n;
+// the roots, one per system
+__global__ void k_init_roots(const Branch* __restrict__ roots, int n_roots, Branch* __restrict__ br) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_roots) br[i] = roots[i];
 }
 
 // ------------------------------------------------------------ host helpers
@@ -332,8 +350,18 @@ static void exclusive_scan_i32(Context& C, const T* in, T* out, int64_t n) {
     HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), C.stream));
 }
 
-static void sort_pairs_u64(Context& C, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, int64_t n) {
+// stable sort by a 63-bit key, within each system's range when seg (device, n_seg + 1 offsets) is given
+static void sort_pairs_u64(Context& C, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, int64_t n,
+                           const int64_t* seg = nullptr, int n_seg = 1) {
     size_t bytes = 0;
+    if (seg && n_seg > 1) {
+        HIP_CHECK(rocprim::segmented_radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (unsigned)n, (unsigned)n_seg, seg,
+                                                      seg + 1, 0, 63, C.stream));
+        C.temp.resize(bytes + 16);
+        HIP_CHECK(rocprim::segmented_radix_sort_pairs(C.temp.data(), bytes, kin, kout, vin, vout, (unsigned)n, (unsigned)n_seg,
+                                                      seg, seg + 1, 0, 63, C.stream));
+        return;
+    }
     HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, 63, C.stream));
     C.temp.resize(bytes + 16);
     HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, kin, kout, vin, vout, (size_t)n, 0, 63, C.stream));
@@ -351,14 +379,17 @@ struct TreeScratch {
     DBuf<uint32_t> idx, idx_s;
     DBuf<int64_t> act_branch, act_first, act_count, sub_first, sub_count;
     DBuf<int32_t> nonempty, rank, split, srank, flag, frank;
-    DBuf<double> part;
+    DBuf<double> part, roots;
+    DBuf<int64_t> seg;
+    DBuf<Branch> root_branches;
     // the build whose keys (khi, klo, idx unsorted and khi_s, klo_s, idx_s sorted) the arrays hold
     const Tree* last = nullptr;
     uint64_t last_id = 0;
     const double* buf = nullptr;
     int rows = 0;
     int64_t n = 0;
-    double center[3] = {0, 0, 0}, bmax = 0, radius = 0;
+    std::vector<int64_t> seg_h;       // the systems' body offsets of that build (empty: one system)
+    std::vector<double> roots_h;      // per system: root centre (3), half side, radius
 };
 
 static TreeScratch& scratch(Context& C) {   // per context: builds on different streams share nothing
@@ -374,28 +405,51 @@ const TreeScratch* tree_keys(const Context& C, const Tree& T) {
 }
 
 // Level-by-level structure from sorted keys; returns false if deeper levels than
-// max_level would be needed.
-static bool build_structure(Context& C, Tree& T, TreeScratch& S, double center[3], double bmax, double radius,
-                            int64_t ls, int method, int max_level) {
+// max_level would be needed. roots_h: per system the root centre (3), half side and radius.
+static bool build_structure(Context& C, Tree& T, TreeScratch& S, const std::vector<int64_t>& seg,
+                            const std::vector<double>& roots_h, int64_t ls, int method, int max_level) {
     const int64_t n = T.n;
-    Branch r;
-    r.n_bodies = n; r.first = 0; r.n_branches = 0; r.child_first = -1; r.parent = -1; r.leaf = -1;
-    for (int d = 0; d < 3; ++d) {
-        r.source_center[d] = r.target_center[d] = center[d];
-        r.source_box[d] = r.target_box[d] = bmax;
+    const int n_roots = (int)seg.size() - 1;
+    std::vector<Branch> rb(n_roots);
+    std::vector<int64_t> act(3 * (size_t)n_roots);   // active roots: branch, first body, count
+    int64_t n_active = 0;
+    for (int i = 0; i < n_roots; ++i) {
+        Branch& r = rb[i];
+        r.n_bodies = seg[i + 1] - seg[i]; r.first = seg[i]; r.n_branches = 0; r.child_first = -1; r.parent = -1; r.leaf = -1;
+        for (int d = 0; d < 3; ++d) {
+            r.source_center[d] = r.target_center[d] = roots_h[5 * i + d];
+            r.source_box[d] = r.target_box[d] = roots_h[5 * i + 3];
+        }
+        r.source_radius = r.target_radius = roots_h[5 * i + 4];
+        r.max_influence = 0.0; r.pad = 0.0;
+        if (exceeds(r.n_bodies, ls, method)) {
+            act[n_active] = i;
+            act[n_roots + n_active] = r.first;
+            act[2 * (size_t)n_roots + n_active] = r.n_bodies;
+            ++n_active;
+        }
     }
-    r.source_radius = r.target_radius = radius;
-    r.max_influence = 0.0; r.pad = 0.0;
     // capacity: every split node adds at most 8 children; grow on demand
-    size_t cap = std::max<size_t>(1024, (size_t)(2 * n / std::max<int64_t>(1, ls) + 64));
+    size_t cap = std::max<size_t>(1024, (size_t)(2 * n / std::max<int64_t>(1, ls) + 64 + n_roots));
     T.branches.resize(cap);
     T.level_first.assign(1, 0);
-    T.level_last.assign(1, 0);
-    int64_t n_branches = 1;
-    int64_t n_active = exceeds(n, ls, method) ? 1 : 0;
-    S.act_branch.resize(1); S.act_first.resize(1); S.act_count.resize(1);
-    k_init_root<<<1, 1, 0, C.stream>>>(r, T.branches.data(), S.act_branch.data(), S.act_first.data(), S.act_count.data(), n);
+    T.level_last.assign(1, n_roots - 1);
+    int64_t n_branches = n_roots;
+    S.root_branches.resize(n_roots);
+    HIP_CHECK(hipMemcpyAsync(S.root_branches.data(), rb.data(), n_roots * sizeof(Branch), hipMemcpyHostToDevice, C.stream));
+    k_init_roots<<<grid_for(n_roots, 256), 256, 0, C.stream>>>(S.root_branches.data(), n_roots, T.branches.data());
     LAUNCH_CHECK();
+    S.act_branch.resize(std::max<int64_t>(1, n_active));
+    S.act_first.resize(std::max<int64_t>(1, n_active));
+    S.act_count.resize(std::max<int64_t>(1, n_active));
+    if (n_active) {
+        HIP_CHECK(hipMemcpyAsync(S.act_branch.data(), act.data(), n_active * sizeof(int64_t), hipMemcpyHostToDevice, C.stream));
+        HIP_CHECK(hipMemcpyAsync(S.act_first.data(), act.data() + n_roots, n_active * sizeof(int64_t), hipMemcpyHostToDevice,
+                                 C.stream));
+        HIP_CHECK(hipMemcpyAsync(S.act_count.data(), act.data() + 2 * (size_t)n_roots, n_active * sizeof(int64_t),
+                                 hipMemcpyHostToDevice, C.stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(C.stream));   // rb and act are read by the copies
     int level = 0;
     while (n_active > 0) {
         if (level >= max_level) return false;
@@ -456,43 +510,64 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     if (n == 0) return;
     if (n >= (int64_t)1 << 32) throw Error(FMMHIP_EINVAL, "more than 2^32 bodies in one system");
     if (ls < 1) throw Error(FMMHIP_EINVAL, "leaf size must be positive");
+    // the systems' body offsets: T.seg for a forest, else the one range [0, n)
+    std::vector<int64_t> seg = T.seg.size() > 2 ? T.seg : std::vector<int64_t>{0, n};
+    const int n_seg = (int)seg.size() - 1;
+    if (seg.front() != 0 || seg.back() != n) throw Error(FMMHIP_EINVAL, "system offsets do not cover the bodies");
+    for (int i = 0; i < n_seg; ++i)
+        if (seg[i + 1] <= seg[i]) throw Error(FMMHIP_EINVAL, "every system of a forest needs at least one body");
     TreeScratch& S = scratch(C);
-    double center[3], bmax, radius;
     S.khi.resize(n); S.klo.resize(n); S.idx.resize(n); S.khi_s.resize(n); S.klo_s.resize(n); S.idx_s.resize(n);
-    if (keys && keys != &S && keys->n == n && keys->buf == d_buf && keys->rows == rows) {
-        // the root cube, the octant paths and their order depend only on the positions: take them
+    S.seg.resize(n_seg + 1);
+    HIP_CHECK(hipMemcpyAsync(S.seg.data(), seg.data(), (n_seg + 1) * sizeof(int64_t), hipMemcpyHostToDevice, C.stream));
+    const int64_t* dseg = n_seg > 1 ? S.seg.data() : nullptr;
+    std::vector<double> roots_h(5 * n_seg);
+    if (keys && keys != &S && keys->n == n && keys->buf == d_buf && keys->rows == rows &&
+        (keys->seg_h.empty() ? n_seg == 1 : keys->seg_h == seg)) {
+        // the root cubes, the octant paths and their order depend only on the positions: take them
         // from the other context's build of the same buffer (its stream has been waited for)
-        for (int d = 0; d < 3; ++d) center[d] = keys->center[d];
-        bmax = keys->bmax;
-        radius = keys->radius;
+        roots_h = keys->roots_h;
         auto copy = [&](auto& dst, const auto& src) {
             HIP_CHECK(hipMemcpyAsync(dst.data(), src.data(), (size_t)n * sizeof(*src.data()), hipMemcpyDeviceToDevice, C.stream));
         };
         copy(S.khi, keys->khi); copy(S.klo, keys->klo); copy(S.idx, keys->idx);
         copy(S.khi_s, keys->khi_s); copy(S.klo_s, keys->klo_s); copy(S.idx_s, keys->idx_s);
     } else {
-        // root cube (center_box, src/tree.jl:894-907; :20-26)
-        const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
-        S.part.resize(6 * nblk);
-        k_bbox_partial<<<nblk, 256, 0, C.stream>>>(d_buf, rows, n, S.part.data());
+        // root cube of each system (center_box, src/tree.jl:894-907; :20-26)
+        const int64_t longest = [&] { int64_t m = 0; for (int i = 0; i < n_seg; ++i) m = std::max(m, seg[i + 1] - seg[i]); return m; }();
+        const int nblk = (int)std::min<int64_t>(n_seg > 1 ? 64 : 1024, (longest + 255) / 256);
+        S.part.resize((size_t)6 * nblk * n_seg);
+        k_bbox_partial<<<dim3(nblk, n_seg), 256, 0, C.stream>>>(d_buf, rows, n, dseg, S.part.data());
         LAUNCH_CHECK();
-        std::vector<double> part(6 * nblk);
+        std::vector<double> part((size_t)6 * nblk * n_seg);
         HIP_CHECK(hipMemcpyAsync(part.data(), S.part.data(), part.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
         HIP_CHECK(hipStreamSynchronize(C.stream));
-        double mn[3] = {part[0], part[1], part[2]}, mx[3] = {part[3], part[4], part[5]};
-        for (int b = 1; b < nblk; ++b)
-            for (int d = 0; d < 3; ++d) { mn[d] = jmin(mn[d], part[6 * b + d]); mx[d] = jmax(mx[d], part[6 * b + 3 + d]); }
-        double box[3];
-        for (int d = 0; d < 3; ++d) { center[d] = (mx[d] + mn[d]) * 0.5; box[d] = mx[d] - center[d]; }
-        bmax = std::max(std::max(box[0], box[1]), box[2]);
-        radius = std::sqrt(bmax * bmax * 3.0);
-        // full octant paths, then a stable sort by the first 21 levels
-        k_path_keys<<<grid_for(n, 256), 256, 0, C.stream>>>(d_buf, rows, n, center[0], center[1], center[2], bmax, S.khi.data(),
+        std::vector<double> cube(4 * n_seg);
+        for (int i = 0; i < n_seg; ++i) {
+            const double* q = part.data() + (size_t)6 * nblk * i;
+            double mn[3] = {q[0], q[1], q[2]}, mx[3] = {q[3], q[4], q[5]};
+            for (int b = 1; b < nblk; ++b)
+                for (int d = 0; d < 3; ++d) { mn[d] = jmin(mn[d], q[6 * b + d]); mx[d] = jmax(mx[d], q[6 * b + 3 + d]); }
+            double box[3];
+            for (int d = 0; d < 3; ++d) {
+                roots_h[5 * i + d] = (mx[d] + mn[d]) * 0.5;
+                box[d] = mx[d] - roots_h[5 * i + d];
+            }
+            const double bmax = std::max(std::max(box[0], box[1]), box[2]);
+            roots_h[5 * i + 3] = bmax;
+            roots_h[5 * i + 4] = std::sqrt(bmax * bmax * 3.0);
+            for (int d = 0; d < 4; ++d) cube[4 * i + d] = roots_h[5 * i + d];
+        }
+        S.roots.resize(4 * n_seg);
+        HIP_CHECK(hipMemcpyAsync(S.roots.data(), cube.data(), cube.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
+        // full octant paths, then a stable sort (within each system) by the first 21 levels
+        k_path_keys<<<grid_for(n, 256), 256, 0, C.stream>>>(d_buf, rows, n, S.roots.data(), dseg, n_seg, S.khi.data(),
                                                            S.klo.data(), S.idx.data());
         LAUNCH_CHECK();
-        sort_pairs_u64(C, S.khi.data(), S.khi_s.data(), S.idx.data(), S.idx_s.data(), n);
+        sort_pairs_u64(C, S.khi.data(), S.khi_s.data(), S.idx.data(), S.idx_s.data(), n, dseg, n_seg);
         k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.klo.data(), S.idx_s.data(), n, S.klo_s.data());
         LAUNCH_CHECK();
+        HIP_CHECK(hipStreamSynchronize(C.stream));   // cube is read by the copy
     }
     static std::atomic<uint64_t> builds{0};
     S.last = &T;
@@ -500,25 +575,24 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     S.buf = d_buf;
     S.rows = rows;
     S.n = n;
-    for (int d = 0; d < 3; ++d) S.center[d] = center[d];
-    S.bmax = bmax;
-    S.radius = radius;
-    if (!build_structure(C, T, S, center, bmax, radius, ls, method, LEVELS_PER_WORD)) {
+    S.seg_h = n_seg > 1 ? seg : std::vector<int64_t>();
+    S.roots_h = roots_h;
+    if (!build_structure(C, T, S, seg, roots_h, ls, method, LEVELS_PER_WORD)) {
         // deeper than 21 levels: order by the second word first, then stably by the first
         DBuf<uint64_t> tmpk;
         DBuf<uint32_t> tmpi;
         tmpk.resize(n); tmpi.resize(n);
-        sort_pairs_u64(C, S.klo.data(), tmpk.data(), S.idx.data(), tmpi.data(), n);
+        sort_pairs_u64(C, S.klo.data(), tmpk.data(), S.idx.data(), tmpi.data(), n, dseg, n_seg);
         k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.khi.data(), tmpi.data(), n, S.khi_s.data());
         LAUNCH_CHECK();
         // khi_s now holds hi keys in lo-sorted order; stable sort by hi carrying the index
         DBuf<uint64_t> hi2;
         hi2.resize(n);
-        sort_pairs_u64(C, S.khi_s.data(), hi2.data(), tmpi.data(), S.idx_s.data(), n);
+        sort_pairs_u64(C, S.khi_s.data(), hi2.data(), tmpi.data(), S.idx_s.data(), n, dseg, n_seg);
         HIP_CHECK(hipMemcpyAsync(S.khi_s.data(), hi2.data(), n * sizeof(uint64_t), hipMemcpyDeviceToDevice, C.stream));
         k_gather_u64<<<grid_for(n, 256), 256, 0, C.stream>>>(S.klo.data(), S.idx_s.data(), n, S.klo_s.data());
         LAUNCH_CHECK();
-        if (!build_structure(C, T, S, center, bmax, radius, ls, method, 2 * LEVELS_PER_WORD))
+        if (!build_structure(C, T, S, seg, roots_h, ls, method, 2 * LEVELS_PER_WORD))
             throw Error(FMMHIP_EDEPTH, "octree deeper than 42 levels (coincident bodies?)");
     }
     const int64_t nb = T.n_branches;
@@ -564,6 +638,7 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
 // values, which a target tree does not read).
 void clone_target_tree(Context& C, const Tree& s, Tree& t) {
     t.n = s.n;
+    t.seg = s.seg;
     t.rows = s.rows;
     t.p = s.p;
     t.strength_dims = 0;

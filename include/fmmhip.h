@@ -135,6 +135,17 @@ void fmmhip_eval_destroy(fmmhip_eval* ev);
 fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n_targets, const double* target_xyz,
                                       int64_t n_sources, const double* source_buffer, int32_t on_device);
 
+/* Several independent systems of one kernel, each its own target and source (fmm!(system) for each of
+ * them), evaluated together: one tree per system (its own root cube, the tree the reference builds for
+ * that system alone), interaction lists within each system only, and every pass of all systems in the
+ * same launches. No counterpart in the reference, whose callers loop over fmm!; for batches of many
+ * small systems. counts: n_systems sizes, each >= 1; source_buffer: the systems' columns one after
+ * another (data_per_body x sum(counts), column-major). The stages below, fmmhip_fmm and
+ * fmmhip_eval_get_targets then work on the batch; results follow the same concatenated order, and the
+ * stats (pmax, error_success, counts, times) cover the whole batch. FastGaussSeidel takes one system. */
+fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_systems, const int64_t* counts,
+                                     const double* source_buffer, int32_t on_device);
+
 /* Stages of fmm!: Tree() for both systems (src/tree.jl:3-122) */
 fmmhip_status fmmhip_eval_build_trees(fmmhip_eval* ev, const fmmhip_options* opt);
 /* build_interaction_lists + sort_by_target (src/fmm.jl:904-907) */

@@ -126,6 +126,16 @@ FMM_CASES = [
      dict(expansion_order=8, leaf_size_source=30, multipole_acceptance=0.5, hessian=True)),
     ("ring-6k-p10", lambda: W.vortex_ring(6000, seed=24), O.KIND_VORTEX,
      dict(expansion_order=10, leaf_size_source=64, multipole_acceptance=0.4, hessian=True)),
+    # SelfTuning pairs branches that are not leaves in the direct list, so direct targets nest (an ancestor's
+    # bodies include its descendants'); Barba's lists (leaf pairs only) for comparison
+    ("laplace-5k-p6-selftuning", lambda: W.uniform_sources(5000, seed=700), O.KIND_SOURCE,
+     dict(expansion_order=6, leaf_size_source=24, multipole_acceptance=0.45, scalar_potential=True, hessian=True,
+          interaction_list_method=F.SelfTuning())),
+    ("vortex-4k-p8-selftuning-ls8", lambda: W.uniform_vortices(4000, seed=701), O.KIND_VORTEX,
+     dict(expansion_order=8, leaf_size_source=8, multipole_acceptance=0.5, hessian=True,
+          interaction_list_method=F.SelfTuning())),
+    ("vortex-4k-p8-barba", lambda: W.uniform_vortices(4000, seed=702), O.KIND_VORTEX,
+     dict(expansion_order=8, leaf_size_source=30, multipole_acceptance=0.5, interaction_list_method=F.Barba())),
     # orders <= 4 take the lane-per-interaction M2L kernel
     ("vortex-5k-p3-leaf5", lambda: W.uniform_vortices(5000, seed=25), O.KIND_VORTEX,
      dict(expansion_order=3, leaf_size_source=5, multipole_acceptance=0.6, scalar_potential=True, hessian=True)),
