@@ -24,6 +24,7 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -137,26 +138,28 @@ def batch_sample(F, W, _lib, args):
     """Batched uneven instances: 512 independent systems, N = floor(exp(U[ln 1e3, ln 2e5])) (seed 4),
     even instances uniform-cube sources (potential + gradient, p 8, MAC 0.5), odd instances vortex rings
     (velocity + gradient, PowerRelativeGradient at the headline's tolerance and MAC, p <= 20); FP32
-    nearfield with compensated sums (the mixed form). Each kind's leaf size is chosen by tune_fmm on a
-    median-size instance. All systems are submitted at once to the request server; the timed region is
-    the second of two identical rounds (the first grows the workers' buffers)."""
+    nearfield with compensated sums (the mixed form). Each kind's leaf size is chosen by autotune (measured
+    device time) on one instance of the body-weighted median size. Each kind runs as one batch evaluation (fmm_batch); for comparison all systems
+    are also submitted at once to the request server. Timed regions are the second of two identical rounds
+    (the first grows the buffers)."""
     rng = np.random.default_rng(4)
     sizes = np.floor(np.exp(rng.uniform(math.log(1e3), math.log(2e5), args.batch_systems))).astype(int)
     em = F.PowerRelativeGradient(args.tol)
-    med = int(np.median(sizes))
+    # tuned on one instance of the body-weighted median size (where half the bodies sit in smaller systems)
+    srt = np.sort(sizes)
+    med = int(srt[np.searchsorted(np.cumsum(srt), srt.sum() / 2)])
     t0 = time.perf_counter()
     tuned = {}
     for kind in ("cube", "ring"):
         if kind == "cube":
             s = _RowSystem(_lib.POINT_SOURCE, W.uniform_sources(med, seed=5))
-            params, cache = F.tune_fmm(s, expansion_order=8, multipole_acceptances=[0.5], verbose=False,
+            params, ev, _ = F.autotune(s, expansion_order=8, multipole_acceptance=0.5, free_slots=(1,), rounds=1,
                                        scalar_potential=True, gradient=True, precision="mixed")
         else:
             s = _RowSystem(_lib.POINT_VORTEX, W.vortex_ring(med, seed=6))
-            params, cache = F.tune_fmm(s, error_tolerance=em, max_expansion_order=args.pmax,
-                                       multipole_acceptances=[args.mac], verbose=False, gradient=True, hessian=True,
-                                       precision="mixed")
-        cache.close()
+            params, ev, _ = F.autotune(s, error_tolerance=em, expansion_order=args.pmax, multipole_acceptances=(args.mac,),
+                                       free_slots=(1,), rounds=1, gradient=True, hessian=True, precision="mixed")
+        ev.close()
         tuned[kind] = params
     t_tune = time.perf_counter() - t0
     systems = []
@@ -166,35 +169,94 @@ def batch_sample(F, W, _lib, args):
         else:
             systems.append(("ring", _RowSystem(_lib.POINT_VORTEX, W.vortex_ring(int(n), seed=1000 + i))))
 
-    def submit(server, kind, sysm):
+    def opts(kind):
         t = tuned[kind]
         if kind == "cube":
-            return server.submit_fmm(sysm, leaf_size=t["leaf_size_source"], expansion_order=8, multipole_acceptance=0.5,
-                                     scalar_potential=True, gradient=True, precision="mixed")
-        return server.submit_fmm(sysm, leaf_size=t["leaf_size_source"], expansion_order=args.pmax,
-                                 multipole_acceptance=args.mac, error_tolerance=em, gradient=True, hessian=True,
-                                 precision="mixed")
+            return dict(leaf_size=t["leaf_size"], expansion_order=8, multipole_acceptance=0.5,
+                        scalar_potential=True, gradient=True, precision="mixed")
+        return dict(leaf_size=t["leaf_size"], expansion_order=args.pmax, multipole_acceptance=args.mac,
+                    error_tolerance=em, gradient=True, hessian=True, precision="mixed")
 
-    out = None
-    with F.Server(n_slots=args.batch_slots, hbm_budget_bytes=args.serve_budget_gib * 2**30) as server:
+    def summary(dt, lat, status):
+        return {"systems": len(systems), "bodies": int(sizes.sum()), "seconds": round(dt, 3),
+                "systems_per_s": round(len(systems) / dt, 1), "bodies_per_s": round(float(sizes.sum()) / dt, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "status": status}
+
+    # one batch evaluation per kind (fmm_batch: a forest of the kind's systems, every pass in shared launches);
+    # a system's latency is the end of its kind's batch
+    # each kind on a context (stream) of its own, the two batches run concurrently from two host threads
+    evs = {k: F.Evaluation(F.Context()) for k in ("cube", "ring")}
+    staged = {}   # each kind's systems as one host buffer of concatenated columns (staged before the clock)
+    for kind in ("cube", "ring"):
+        members = [sysm for k, sysm in systems if k == kind]
+        staged[kind] = (members[0].fmm_kernel, np.ascontiguousarray(np.concatenate([m.rows for m in members])),
+                        np.array([len(m.rows) for m in members], dtype=np.int64))
+
+    def run_kind(kind, t0, res):
+        kernel, buf, counts = staged[kind]
+        ev, kw = evs[kind], opts(kind)
+        ctx = ev.ctx
+        ctx.check(ctx.lib.fmmhip_eval_set_batch(ev._h, kernel, len(counts), counts.ctypes.data, buf.ctypes.data, 0))
+        ev.nt = ev.ns = int(counts.sum())
+        ls = kw.pop("leaf_size")
+        t1 = time.perf_counter()
+        st = ev.fmm(F.make_options(leaf_size_source=ls, leaf_size_target=ls, **kw))
+        t2 = time.perf_counter()
+        ev.targets()    # results back to the host, as the server's wait delivers them
+        t3 = time.perf_counter()
+        res[kind] = (len(counts), bool(st.error_success), (t3 - t0) * 1e3,
+                     {"copy_in_ms": round((t1 - t0) * 1e3, 1), "fmm_ms": round((t2 - t1) * 1e3, 1),
+                      "copy_out_ms": round((t3 - t2) * 1e3, 1),
+                      **{k: round(getattr(st, k), 1) for k in ("ms_tree", "ms_lists", "ms_estimate", "ms_p2p",
+                                                              "ms_upward", "ms_m2l", "ms_downward")}})
+
+    fused = None
+    for rnd in range(2):
+        res = {}
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=run_kind, args=(k, t0, res)) for k in ("cube", "ring")]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        dt = time.perf_counter() - t0
+        if len(res) != 2:
+            raise RuntimeError("a batch evaluation failed")
+        lat, status = [], {"ok": 0, "tolerance_missed": 0}
+        for kind, (n, ok, t_end, _) in res.items():
+            lat += [t_end] * n
+            status["ok" if ok else "tolerance_missed"] += n
+        fused = summary(dt, lat, status)
+        fused["phases"] = {k: v[3] for k, v in res.items()}
+        log(f"batch (fused) round {rnd}: {fused}")
+    for ev in evs.values():
+        ctx = ev.ctx
+        ev.close()
+        ctx.close()
+
+    server = None
+    with F.Server(n_slots=args.batch_slots, hbm_budget_bytes=args.serve_budget_gib * 2**30) as srv:
         for rnd in range(2):
             t0 = time.perf_counter()
-            tickets = [submit(server, kind, sysm) for kind, sysm in systems]
-            sts = [server.wait(t)[2] for t in tickets]
+            tickets = [srv.submit_fmm(sysm, **opts(kind)) for kind, sysm in systems]
+            sts = [srv.wait(t)[2] for t in tickets]
             dt = time.perf_counter() - t0
-            lat = [x["latency_ms"] for x in sts]
             status = {"ok": sum(x["status"] == _lib.FMMHIP_OK for x in sts),
                       "tolerance_missed": sum(x["status"] == _lib.FMMHIP_ETOL for x in sts)}
-            out = {"systems": len(systems), "bodies": int(sizes.sum()), "seconds": round(dt, 3),
-                   "systems_per_s": round(len(systems) / dt, 1), "bodies_per_s": round(float(sizes.sum()) / dt, 1),
-                   "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                   "status": status}
-            log(f"batch round {rnd}: {out}")
+            server = summary(dt, [x["latency_ms"] for x in sts], status)
+            log(f"batch (server) round {rnd}: {server}")
+    out = dict(fused)
     out.update({"workload": f"{len(systems)} systems, N log-uniform [1e3, 2e5] (seed 4), even: uniform-cube sources "
                             "(potential + gradient, p 8, MAC 0.5), odd: vortex rings (velocity + gradient, "
                             f"PowerRelativeGradient({args.tol}), p <= {args.pmax}, MAC {args.mac}); mixed form",
-                "slots": args.batch_slots, "tuned_leaf_size": {k: v["leaf_size_source"] for k, v in tuned.items()},
-                "tune_found": {k: bool(v["tuned"]) for k, v in tuned.items()},
+                "mode": "two batch evaluations (one per kind: a forest of its systems, every pass in shared "
+                        "launches), concurrently on two streams; a system's latency is the end of its kind's batch",
+                "server": dict(server, slots=args.batch_slots,
+                               mode="512 requests to the request server at once, overlapping on the device"),
+                "tuned_leaf_size": {k: v["leaf_size"] for k, v in tuned.items()},
+                "tuned_on": f"autotune on one instance of each kind at N = {med} (body-weighted median)",
+                "tuned_error_success": {k: bool(v["error_success"]) for k, v in tuned.items()},
                 "tune_seconds": round(t_tune, 2)})
     return out
 
@@ -306,6 +368,8 @@ def main():
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--mac", type=float, default=0.55)
     ap.add_argument("--pmax", type=int, default=20)
+    ap.add_argument("--p2p-free-slots", type=int, default=None,
+                    help="workgroup slots per CU the nearfield leaves to the far-field stream (library default 1)")
     ap.add_argument("--samples", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000, help="ring size of the all-cores CPU baseline")
     ap.add_argument("--cpu-sample-1", type=int, default=30000, help="ring size of the one-core CPU baseline (0: skip)")
@@ -348,6 +412,8 @@ def main():
     n = args.n
     sbuf = W.vortex_ring(n, seed=2)
     ctx = F.default_context()
+    if args.p2p_free_slots is not None:
+        ctx.set_p2p_free_slots(args.p2p_free_slots)
     ev = F.Evaluation(ctx)
     ev.set_systems(_lib.POINT_VORTEX, None, sbuf)          # host -> HBM once, outside the timed region
     em = F.PowerRelativeGradient(args.tol)

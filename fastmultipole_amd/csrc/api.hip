@@ -49,21 +49,27 @@ __global__ void k_map_influence(const int64_t* __restrict__ est_sort, const int6
     }
 }
 
-// max_influence of every branch over its bodies (src/fmm.jl:1231-1262)
-__global__ void k_branch_max(Branch* __restrict__ br, int64_t nb, const double* __restrict__ infl) {
-    __shared__ double s[256];
-    const int64_t ib = blockIdx.x;
-    if (ib >= nb) return;
-    Branch& B = br[ib];
-    double m = B.max_influence;
-    for (int64_t i = B.first + threadIdx.x; i < B.first + B.n_bodies; i += blockDim.x) m = fmax(m, infl[i]);
-    s[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
-        __syncthreads();
+// max_influence of every branch over its bodies (src/fmm.jl:1231-1262): one wavefront per leaf takes
+// the max over the leaf's bodies and raises the leaf and its ancestors to it. Values are >= +0.0 (the
+// reference's max starts from 0.0, and Julia's max(0.0, -0.0) is 0.0), and non-negative doubles order
+// like their bit patterns, so the raise is an integer atomic max; the max of a set is exact in any order.
+// A walk stops at an ancestor that already holds as much: whichever leaf raised it raises the rest.
+__global__ __launch_bounds__(64) void k_branch_max(Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
+                                                   int64_t n_leaves, const double* __restrict__ infl) {
+    const int64_t l = blockIdx.x;
+    if (l >= n_leaves) return;
+    const int64_t ib = leaf_index[l];
+    const int64_t f = br[ib].first, e = f + br[ib].n_bodies;
+    double m = 0.0;
+    for (int64_t i = f + threadIdx.x; i < e; i += 64) m = fmax(m, infl[i]);
+    for (int w = 32; w > 0; w >>= 1) m = fmax(m, __shfl_xor(m, w, 64));
+    if (threadIdx.x != 0) return;
+    const unsigned long long v = (unsigned long long)__double_as_longlong(m > 0.0 ? m : 0.0);
+    for (int64_t b = ib; b >= 0; b = br[b].parent) {
+        unsigned long long* slot = (unsigned long long*)&br[b].max_influence;
+        if (__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= v) break;
+        atomicMax(slot, v);
     }
-    if (threadIdx.x == 0) B.max_influence = s[0];
 }
 
 static void check_options(const fmmhip_options* o) {
@@ -107,10 +113,17 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
     const int p = o.expansion_order;
     E.tt.seg = E.seg;
     E.st.seg = E.seg;
-    if (E.t_xyz) build_tree(C, E.tt, E.nt, E.t_xyz, 3, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
-    else build_tree(C, E.tt, E.nt, E.s_buf, dpb, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
-    build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
-               strength_dims(E.kernel));
+    if (!E.t_xyz && o.leaf_size_target == o.leaf_size_source) {
+        // the targets are the sources with one leaf size: the target tree is the source tree's copy
+        build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
+                   strength_dims(E.kernel));
+        clone_target_tree(C, E.st, E.tt);
+    } else {
+        if (E.t_xyz) build_tree(C, E.tt, E.nt, E.t_xyz, 3, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
+        else build_tree(C, E.tt, E.nt, E.s_buf, dpb, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
+        build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
+                   strength_dims(E.kernel));
+    }
     E.trees_built = true;
     E.lists_built = false;
     E.p_built = p;
@@ -225,7 +238,8 @@ static void estimate_eval(Eval& E, const fmmhip_options& o, Context& C) {
                                                               E.nt, o.error_kind == FMMHIP_ERR_POWER_POTENTIAL ? 1 : 0,
                                                               infl.data());
     LAUNCH_CHECK();
-    k_branch_max<<<(unsigned)E.tt.n_branches, 256, 0, C.stream>>>(E.tt.branches.data(), E.tt.n_branches, infl.data());
+    k_branch_max<<<(unsigned)std::max<int64_t>(1, E.tt.n_leaves), 64, 0, C.stream>>>(E.tt.branches.data(), E.tt.leaf_index.data(),
+                                                                                   E.tt.n_leaves, infl.data());
     LAUNCH_CHECK();
 }
 
@@ -272,7 +286,8 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         // stream: it needs nothing of the estimate, so it runs beside the estimate's evaluation once the
         // nearfield is done (103.0 against 104.9 ms per evaluation of the 1M ring with it on the far-field
         // stream after the estimate; the estimate's own nearfield and far field on two streams gained
-        // nothing). Without the estimate it stays on the far-field stream, beside the nearfield.
+        // nothing; 102.3 against 100.9 ms with it before the nearfield, beside the estimate's trees).
+        // Without the estimate it stays on the far-field stream, beside the nearfield.
         const bool up_on_c = estimate && &F != &C;
         std::unique_ptr<Timer> tu;
         if (up_on_c) {

@@ -632,10 +632,23 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     T.expansions.resize((size_t)nb * 4 * n_harmonics(p));
 }
 
+// a target tree's source fields are its target fields: shrink_recenter! adds no body radius to
+// either in a target tree, so both come out the same
+__global__ void k_source_from_target(Branch* __restrict__ br, int64_t nb) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    Branch& b = br[i];
+    for (int d = 0; d < 3; ++d) {
+        b.source_center[d] = b.target_center[d];
+        b.source_box[d] = b.target_box[d];
+    }
+    b.source_radius = b.target_radius;
+}
+
 // Tree structure, body order, positions and the branches' target centres, boxes and radii
 // depend only on the positions, leaf size and options, not on the tree's role, so a target tree
-// of the same bodies is the source tree's copy (its source_* fields keep the source tree's
-// values, which a target tree does not read).
+// of the same bodies is the source tree's copy with its source fields replaced by its target
+// fields: the tree a target build of those bodies gives, bit for bit.
 void clone_target_tree(Context& C, const Tree& s, Tree& t) {
     t.n = s.n;
     t.seg = s.seg;
@@ -662,6 +675,8 @@ void clone_target_tree(Context& C, const Tree& s, Tree& t) {
     copy(t.y, s.y, (size_t)s.n);
     copy(t.z, s.z, (size_t)s.n);
     t.expansions.resize(s.expansions.size());
+    k_source_from_target<<<grid_for(s.n_branches, 256), 256, 0, C.stream>>>(t.branches.data(), s.n_branches);
+    LAUNCH_CHECK();
 }
 
 }  // namespace fmmhip

@@ -376,6 +376,44 @@ function hip_fmm!(hip::HipEval, t, s=t; tune=false, update_target_systems=true, 
             HipListView(hip, 1), (sw,), error_success)
 end
 
+"""
+    hip_fmm_batch!(hip, systems; update_target_systems=true, kwargs...)
+
+`fmm!(system; kwargs...)` for each of several independent systems of one kernel (no counterpart in the
+reference, whose callers loop over `fmm!`): one tree per system, lists within each system, every pass of
+all systems in the same launches. Each system's results equal its own `fmm!` call. Returns the batch's
+`FmmHipStats` (`error_success == 0` if any system missed the tolerance).
+"""
+function hip_fmm_batch!(hip::HipEval, systems::AbstractVector; update_target_systems=true, kwargs...)
+    isempty(systems) && return FmmHipStats()
+    k = hip_kernel(systems[1])
+    k === nothing && throw(ArgumentError("$(typeof(systems[1])) declares no hip_kernel"))
+    all(s -> hip_kernel(s) == k, systems) || throw(ArgumentError("the systems of one batch share one kernel"))
+    bufs = [source_buffer(s) for s in systems]
+    counts = Int64[size(b, 2) for b in bufs]
+    sbuf = reduce(hcat, bufs)
+    opt = FmmHipOptions(; kwargs...)
+    stats = FmmHipStats()
+    GC.@preserve sbuf counts begin
+        check(ccall((:fmmhip_eval_set_batch, libfmmhip), Cint,
+                    (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Float64}, Cint),
+                    hip.ev, k, length(counts), counts, sbuf, 0), hip.ctx)
+        check(ccall((:fmmhip_fmm, libfmmhip), Cint, (Ptr{Cvoid}, Ref{FmmHipOptions}, Ref{FmmHipStats}),
+                    hip.ev, opt, stats), hip.ctx)
+    end
+    if update_target_systems
+        tbuf = Matrix{Float64}(undef, 16, sum(counts))
+        check(ccall((:fmmhip_eval_get_targets, libfmmhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), hip.ev, tbuf, 0), hip.ctx)
+        sw = DerivativesSwitch(opt.scalar_potential != 0, opt.gradient != 0, opt.hessian != 0)
+        first = 1
+        for (s, n) in zip(systems, counts)
+            scatter_targets!(s, tbuf[:, first:first+n-1], sw)
+            first += n
+        end
+    end
+    return stats
+end
+
 # fmm! with an optional `hip` keyword: GPU when routable, the reference's path otherwise
 function fmm!(target_systems::Tuple, source_systems::Tuple, hip::HipEval; kwargs...)
     if length(target_systems) == 1 && length(source_systems) == 1 && hip_kernel(source_systems[1]) !== nothing
