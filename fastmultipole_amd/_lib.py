@@ -22,14 +22,14 @@ EXPORTED = [
     "fmmhip_options_default", "fmmhip_ctx_create", "fmmhip_ctx_destroy", "fmmhip_last_error", "fmmhip_sync",
     "fmmhip_ctx_set_p2p_free_slots",
     "fmmhip_eval_create", "fmmhip_eval_destroy", "fmmhip_eval_set_systems", "fmmhip_eval_set_batch",
-    "fmmhip_eval_build_trees",
+    "fmmhip_eval_get_batch_stats", "fmmhip_eval_build_trees",
     "fmmhip_eval_build_lists", "fmmhip_eval_run", "fmmhip_fmm", "fmmhip_eval_get_targets", "fmmhip_eval_n_branches",
     "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_get_branches", "fmmhip_eval_get_levels",
     "fmmhip_eval_get_leaf_index", "fmmhip_eval_get_sort_index", "fmmhip_eval_n_list", "fmmhip_eval_get_list",
     "fmmhip_eval_get_m2l_orders", "fmmhip_eval_get_expansions", "fmmhip_direct", "fmmhip_direct_noalloc",
     "fmmhip_fgs_setup", "fmmhip_fgs_solve", "fmmhip_fgs_n_leaves", "fmmhip_fgs_leaf_info", "fmmhip_fgs_get_matrix",
     "fmmhip_fgs_nonself_rows",
-    "fmmhip_server_create", "fmmhip_server_destroy", "fmmhip_server_submit", "fmmhip_server_wait",
+    "fmmhip_server_create", "fmmhip_server_destroy", "fmmhip_server_set_batching", "fmmhip_server_submit", "fmmhip_server_wait",
     "fmmhip_server_stats",
 ]
 REQUEST_FMM, REQUEST_SOLVE = 0, 1
@@ -121,6 +121,7 @@ def load():
     L.fmmhip_eval_destroy.restype = None
     L.fmmhip_eval_set_systems.argtypes = [vp, i32, i64, dp, i64, dp, i32]
     L.fmmhip_eval_set_batch.argtypes = [vp, i32, i64, dp, dp, i32]
+    L.fmmhip_eval_get_batch_stats.argtypes = [vp, dp, dp]
     for f in ("fmmhip_eval_build_trees", "fmmhip_eval_build_lists"):
         getattr(L, f).argtypes = [vp, C.POINTER(Options)]
     for f in ("fmmhip_eval_run", "fmmhip_fmm"):
@@ -148,6 +149,7 @@ def load():
     L.fmmhip_server_create.argtypes = [C.c_int, i32, C.c_double, C.POINTER(vp)]
     L.fmmhip_server_destroy.argtypes = [vp]
     L.fmmhip_server_destroy.restype = None
+    L.fmmhip_server_set_batching.argtypes = [vp, i32, i64]
     L.fmmhip_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]
     L.fmmhip_server_wait.argtypes = [vp, i64, dp, dp, C.POINTER(RequestStats)]
     L.fmmhip_server_stats.argtypes = [vp, C.POINTER(ServerStats)]

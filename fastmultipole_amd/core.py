@@ -238,6 +238,7 @@ class Evaluation:
             tp = 0 if tx is None else tx.ctypes.data
             sp = sb.ctypes.data
         self.kernel, self.nt, self.ns = kernel, nt, ns
+        self.counts = None
         self.ctx.check(self.ctx.lib.fmmhip_eval_set_systems(self._h, kernel, nt, tp or None, ns, sp or None, int(on_device)))
 
     def set_batch(self, kernel, source_buffers, on_device=False):
@@ -260,6 +261,15 @@ class Evaluation:
         self.counts = counts
         self.ctx.check(self.ctx.lib.fmmhip_eval_set_batch(self._h, kernel, len(counts), counts.ctypes.data, sp,
                                                           int(on_device)))
+
+    def batch_stats(self):
+        """Per system of the last batch evaluation (one entry for a single system): (pmax, error_success)
+        arrays, each system's values as its own fmm! call reports them."""
+        k = len(getattr(self, "counts", [1])) if getattr(self, "counts", None) is not None else 1
+        pm = np.zeros(k, dtype=np.int32)
+        ok = np.zeros(k, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.fmmhip_eval_get_batch_stats(self._h, pm.ctypes.data, ok.ctypes.data))
+        return pm, ok.astype(bool)
 
     def build_trees(self, opt):
         self.ctx.check(self.ctx.lib.fmmhip_eval_build_trees(self._h, C.byref(opt)))
@@ -382,9 +392,11 @@ class Server:
 
     n_slots workers each own a stream and device-resident buffers reused between requests; a
     request is admitted when its estimated HBM footprint fits within hbm_budget_bytes beside the
-    running ones. submit_* copy the inputs and return a ticket at once; wait(ticket) blocks."""
+    running ones. FMM requests with the same kernel and options that wait together run as one batch
+    evaluation (up to batch_requests requests and batch_bodies bodies; batch_requests=1: each alone).
+    submit_* copy the inputs and return a ticket at once; wait(ticket) blocks."""
 
-    def __init__(self, n_slots=4, hbm_budget_bytes=64 * 2**30, device=-1):
+    def __init__(self, n_slots=4, hbm_budget_bytes=64 * 2**30, device=-1, batch_requests=64, batch_bodies=4 << 20):
         self.lib = L.load()
         h = C.c_void_p()
         st = self.lib.fmmhip_server_create(int(device), int(n_slots), float(hbm_budget_bytes), C.byref(h))
@@ -393,6 +405,10 @@ class Server:
         self._h = h
         self._sizes = {}
         _LIVE_SERVERS.add(self)
+        st = self.lib.fmmhip_server_set_batching(h, int(batch_requests), int(batch_bodies))
+        if st != L.FMMHIP_OK:
+            self.close()
+            raise L.FmmHipError(st, "fmmhip_server_set_batching rejected its arguments")
 
     def _submit(self, req, n, kind):
         t = C.c_int64()
@@ -539,7 +555,8 @@ def fmm_batch(systems, *, leaf_size=20, expansion_order=5, multipole_acceptance=
     evaluation: one tree per system, lists within each system, every pass of all systems in the same
     launches. Each system's results are those of its own fmm! call. Returns (outputs, evaluation): outputs[i]
     is system i's (n_i, 16) target buffer; evaluation.last_stats covers the whole batch (error_success is
-    false if any system missed the tolerance)."""
+    false if any system missed the tolerance), evaluation.batch_stats() gives each system's pmax and
+    error_success as its own fmm! call reports them."""
     systems = list(systems)
     if not systems:
         return [], None

@@ -72,6 +72,31 @@ __global__ __launch_bounds__(64) void k_branch_max(Branch* __restrict__ br, cons
     }
 }
 
+// per system of a batch: the largest expansion order its M2L interactions used, and whether each of its
+// target boxes met the tolerance; a box belongs to the system holding its first body
+__global__ void k_init_sys(int32_t* __restrict__ sys, int n_seg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_seg) { sys[2 * i] = 0; sys[2 * i + 1] = 1; }
+}
+
+__global__ void k_system_status(const int32_t* __restrict__ m2l_targets, const int64_t* __restrict__ m2l_offsets,
+                                int64_t n_boxes, const int32_t* __restrict__ orders, const int32_t* __restrict__ box_ok,
+                                const Branch* __restrict__ tbr, const int64_t* __restrict__ seg, int n_seg,
+                                int32_t* __restrict__ sys) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= n_boxes) return;
+    const int64_t first = tbr[m2l_targets[b]].first;
+    int lo = 0, hi = n_seg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= first) lo = mid; else hi = mid - 1;
+    }
+    int mx = 0;
+    for (int64_t li = m2l_offsets[b]; li < m2l_offsets[b + 1]; ++li) mx = max(mx, orders[li]);
+    atomicMax(sys + 2 * lo, mx);
+    if (!box_ok[b]) atomicAnd(sys + 2 * lo + 1, 0);
+}
+
 static void check_options(const fmmhip_options* o) {
     if (!o) throw Error(FMMHIP_EINVAL, "options is NULL");
     if (o->expansion_order < 1 || o->expansion_order > 20) throw Error(FMMHIP_EINVAL, "expansion_order must be in 1..20");
@@ -317,8 +342,38 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         {
             Timer t(F);
             int32_t pmax = 0, ok = 1;
+            const bool batch = !is_estimate && !E.seg.empty();
+            E.lists.track_box_ok = batch;
+            if (batch) {
+                const int n_seg = (int)E.seg.size() - 1;
+                E.sys_status.resize(2 * n_seg);
+                k_init_sys<<<grid_for(n_seg, 256), 256, 0, F.stream>>>(E.sys_status.data(), n_seg);
+                LAUNCH_CHECK();
+            }
             run_m2l(F, E.tt, E.st, E.tt, E.lists, p, LH, o.error_kind, o.error_relative != 0, o.error_tolerance,
-                    o.error_bonus != 0, o.precision, &pmax, &ok);
+                    o.error_bonus != 0, o.precision, batch ? nullptr : &pmax, &ok);
+            if (batch) {
+                if (E.lists.n_m2l_targets > 0) {
+                    k_system_status<<<grid_for(E.lists.n_m2l_targets, 256), 256, 0, F.stream>>>(
+                        E.lists.m2l_targets.data(), E.lists.m2l_offsets.data(), E.lists.n_m2l_targets, E.lists.m2l_order.data(),
+                        E.lists.m2l_box_ok.data(), E.tt.branches.data(), E.seg_dev.data(), (int)E.seg.size() - 1,
+                        E.sys_status.data());
+                    LAUNCH_CHECK();
+                }
+                const int n_seg = (int)E.seg.size() - 1;
+                std::vector<int32_t> h(2 * n_seg);
+                HIP_CHECK(hipMemcpyAsync(h.data(), E.sys_status.data(), h.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                         F.stream));
+                HIP_CHECK(hipStreamSynchronize(F.stream));
+                E.sys_pmax.assign(n_seg, 0);
+                E.sys_ok.assign(n_seg, 1);
+                for (int i = 0; i < n_seg; ++i) {
+                    E.sys_ok[i] = h[2 * i + 1];
+                    E.sys_pmax[i] = h[2 * i] + (h[2 * i + 1] ? 0 : 1);   // as the evaluation's pmax below
+                    pmax = std::max(pmax, h[2 * i]);
+                    ok = ok && h[2 * i + 1];
+                }
+            }
             E.pmax = pmax;
             E.error_success = ok;
             if (!ok) E.pmax += 1;
@@ -470,6 +525,8 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
         E.nt = nt;
         E.ns = ns;
         E.seg.clear();
+        E.sys_pmax.clear();
+        E.sys_ok.clear();
         const int dpb = data_per_body(kernel);
         if (on_device) {
             E.t_xyz = t_xyz;
@@ -517,7 +574,32 @@ fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_s
             E.s_buf = E.s_buf_in.data();
         }
         E.seg = n_systems > 1 ? seg : std::vector<int64_t>();
+        if (n_systems > 1) {
+            E.seg_dev.resize(n_systems + 1);
+            HIP_CHECK(hipMemcpyAsync(E.seg_dev.data(), seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                                     C->stream));
+            HIP_CHECK(hipStreamSynchronize(C->stream));
+        }
+        E.sys_pmax.clear();
+        E.sys_ok.clear();
         E.trees_built = E.lists_built = E.ran = false;
+    });
+}
+
+fmmhip_status fmmhip_eval_get_batch_stats(fmmhip_eval* ev, int32_t* pmax, int32_t* error_success) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (!E.ran) throw Error(FMMHIP_EINVAL, "the evaluation has not run");
+        if (E.seg.empty()) {
+            if (pmax) pmax[0] = E.pmax;
+            if (error_success) error_success[0] = E.error_success;
+        } else {
+            if (E.sys_pmax.size() + 1 != E.seg.size()) throw Error(FMMHIP_EINVAL, "no per-system results");
+            if (pmax) std::copy(E.sys_pmax.begin(), E.sys_pmax.end(), pmax);
+            if (error_success) std::copy(E.sys_ok.begin(), E.sys_ok.end(), error_success);
+        }
     });
 }
 

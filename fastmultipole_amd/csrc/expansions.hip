@@ -981,7 +981,7 @@ __global__ __launch_bounds__(64 * NW) void k_m2l(const int32_t* __restrict__ m2l
                                                  const Branch* __restrict__ sbr, const double* __restrict__ sex,
                                                  double* __restrict__ tex, int P, int PE, int mode, ErrParams E,
                                                  int32_t* __restrict__ orders, int32_t* __restrict__ pmax,
-                                                 int32_t* __restrict__ all_ok, TablePtrs tb) {
+                                                 int32_t* __restrict__ all_ok, int32_t* __restrict__ box_ok, TablePtrs tb) {
     extern __shared__ double lds[];
     const int64_t l0 = m2l_offsets[blockIdx.x], l1 = m2l_offsets[blockIdx.x + 1];
     if (mode == M2L_PASS_DEFERRED) {   // only the interactions the first pass left (order -1)
@@ -1034,7 +1034,10 @@ __global__ __launch_bounds__(64 * NW) void k_m2l(const int32_t* __restrict__ m2l
     }
     if (lane_id() == 0) {
         atomicMax(pmax, local_max);
-        if (!local_ok) atomicAnd(all_ok, 0);
+        if (!local_ok) {
+            atomicAnd(all_ok, 0);
+            if (box_ok) box_ok[blockIdx.x] = 0;
+        }
     }
 }
 
@@ -1044,7 +1047,8 @@ static void launch_m2l(Context& C, int p, int pe, int mode, const Lists& Ls, con
     const size_t sh = translate_setup(k_m2l<LH, NW>, p, NW);
     k_m2l<LH, NW><<<(unsigned)Ls.n_m2l_targets, 64 * NW, sh, C.stream>>>(
         Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
-        st.expansions.data(), tt_dst.expansions.data(), p, pe, mode, E, Ls.m2l_order.data(), flags, flags + 1, tb);
+        st.expansions.data(), tt_dst.expansions.data(), p, pe, mode, E, Ls.m2l_order.data(), flags, flags + 1,
+        Ls.track_box_ok ? Ls.m2l_box_ok.data() : nullptr, tb);
     LAUNCH_CHECK();
 }
 
@@ -1579,7 +1583,8 @@ __global__ __launch_bounds__(64 * NW, 2) void k_m2l_mfma(const int32_t* __restri
                                                       const int32_t* __restrict__ m2l_s, const Branch* __restrict__ tbr,
                                                       const Branch* __restrict__ sbr, const double* __restrict__ sex,
                                                       double* __restrict__ tex, int P, ErrParams E, int32_t* __restrict__ orders,
-                                                      int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok, TablePtrs tb) {
+                                                      int32_t* __restrict__ pmax, int32_t* __restrict__ all_ok,
+                                                      int32_t* __restrict__ box_ok, TablePtrs tb) {
     constexpr int NC = LH ? 4 : 2, BP = 16 / NC;
     extern __shared__ double lds[];
     const MfLds W = mf_carve(lds, P, BP);
@@ -1645,7 +1650,10 @@ __global__ __launch_bounds__(64 * NW, 2) void k_m2l_mfma(const int32_t* __restri
     for (int i = tid; i < H4; i += 64 * NW) tex[(int64_t)it * H4 + i] += W.acc[i];
     if (tid == 0) {
         atomicMax(pmax, local_max);
-        if (!local_ok) atomicAnd(all_ok, 0);
+        if (!local_ok) {
+            atomicAnd(all_ok, 0);
+            if (box_ok) box_ok[blockIdx.x] = 0;
+        }
     }
 }
 
@@ -1667,7 +1675,8 @@ static void launch_m2l_mfma(Context& C, int p, const Lists& Ls, const Tree& st, 
     HIP_CHECK(hipFuncSetAttribute((const void*)k_m2l_mfma<LH, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     k_m2l_mfma<LH, NW><<<(unsigned)Ls.n_m2l_targets, 64 * NW, sh, C.stream>>>(
         Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(), st.branches.data(),
-        st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags, flags + 1, tb);
+        st.expansions.data(), tt_dst.expansions.data(), p, E, Ls.m2l_order.data(), flags, flags + 1,
+        Ls.track_box_ok ? Ls.m2l_box_ok.data() : nullptr, tb);
     LAUNCH_CHECK();
 }
 
@@ -1933,6 +1942,10 @@ static void launch_m2l_lane(Context& C, int p, const Lists& Ls, const Tree& st, 
 
 __global__ void k_init_flags(int32_t* pmax, int32_t* ok) { *pmax = 0; *ok = 1; ok[1] = 0; }
 
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 // M2L kernel for orders above SMALL_P: the wavefront-per-interaction engine
 // (k_m2l) or the matrix-core batches (k_m2l_mfma). Both compute the same
 // interactions at the same orders; FMMHIP_M2L_KERNEL=mfma selects the second.
@@ -1954,6 +1967,12 @@ void run_m2l(Context& C, const Tree& tt, Tree& st, Tree& tt_dst, Lists& Ls, int 
     flags.resize(3);
     k_init_flags<<<1, 1, 0, C.stream>>>(flags.data(), flags.data() + 1);
     LAUNCH_CHECK();
+    if (Ls.track_box_ok) {
+        Ls.m2l_box_ok.resize(std::max<int64_t>(1, Ls.n_m2l_targets));
+        k_fill_i32<<<(unsigned)std::min<int64_t>(1024, (Ls.n_m2l_targets + 255) / 256 + 1), 256, 0, C.stream>>>(
+            Ls.m2l_box_ok.data(), Ls.n_m2l_targets, 1);
+        LAUNCH_CHECK();
+    }
     if (Ls.n_m2l_targets > 0 && err_kind == FMMHIP_ERR_NONE && p <= SMALL_P) {
         if (LH) launch_m2l_lane<true>(C, p, Ls, st, tt_dst, flags.data(), tb);
         else launch_m2l_lane<false>(C, p, Ls, st, tt_dst, flags.data(), tb);

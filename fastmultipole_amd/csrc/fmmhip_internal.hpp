@@ -88,6 +88,10 @@ struct Lists {
     int64_t n_m2l = 0, n_direct = 0, n_pairs = 0;
     DBuf<int32_t> m2l_t, m2l_s, dir_t, dir_s;        // sorted by target, DFS order within target
     DBuf<int32_t> m2l_order;                         // expansion order used per M2L
+    // per M2L target box: 1 while every interaction of the box met its tolerance (written by the
+    // error-controlled M2L when track_box_ok is set; batches derive each system's error_success from it)
+    bool track_box_ok = false;
+    DBuf<int32_t> m2l_box_ok;
     DBuf<int32_t> m2l_perm, m2l_iota;                // processing order of the M2L list within each target box
     DBuf<float> m2l_key, m2l_key_sorted;
     // CSR over target branches for the direct list: unique targets and offsets
@@ -168,6 +172,9 @@ struct Eval {
     const double* s_buf = nullptr;            // device pointer to dpb x ns
     std::vector<int64_t> seg;                 // several independent systems (each its own target and source):
                                               // their body offsets; empty for one target and one source system
+    DBuf<int64_t> seg_dev;                    // seg on the device
+    DBuf<int32_t> sys_status;                 // per system of a batch: largest order used, tolerance met
+    std::vector<int32_t> sys_pmax, sys_ok;    // host copies after the evaluation (pmax + 1 where not met)
     Tree tt, st;
     Lists lists;
     DBuf<double> tout;                        // sorted target outputs, SoA [13][nt]: phi, v(3), h(9)

@@ -129,7 +129,8 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mx(const int32_t* __restrict__ 
                                                     const Branch* __restrict__ sbr, const double* __restrict__ sex,
                                                     double* __restrict__ tex, int n_boxes, int P, int PE, ErrParams E,
                                                     const int32_t* __restrict__ perm, int32_t* __restrict__ orders,
-                                                    int32_t* __restrict__ flags, TablePtrs tb) {
+                                                    int32_t* __restrict__ flags, int32_t* __restrict__ box_ok,
+                                                    TablePtrs tb) {
     using Eng = mx::Engine<MxGpu, CAP, LH, FULL>;
     constexpr int H = nh(CAP), NC = Eng::NC, BP = Eng::BP, NS = CAP + 1;
     extern __shared__ double lds[];
@@ -187,10 +188,15 @@ __global__ __launch_bounds__(64 * NW) void k_m2l_mx(const int32_t* __restrict__ 
         X.kind = E.kind;
         X.BE = E.BE;
         const int64_t l0 = m2l_offsets[box], l1 = m2l_offsets[box + 1];
+        int okb = 1;
         for (int64_t b0 = l0; b0 < l1; b0 += BP) {
             const mx::Batch Bt{m2l_s, perm, (int)b0, (int)min((int64_t)BP, l1 - b0), orders};
             Eng::batch(T, W, X, Bt, (const double*)sbr, (int)(sizeof(Branch) / 8), (int)(offsetof(Branch, source_center) / 8),
-                       sex, HE4, pmax, okf);
+                       sex, HE4, pmax, okb);
+        }
+        if (!okb) {
+            okf = 0;
+            if (box_ok && lane == 0) box_ok[box] = 0;
         }
         double* dst = tex + (int64_t)it * HE4;
         for (int i = lane; i < NC * nh(P); i += 64) dst[4 * (i / NC) + i % NC] += W.acc[i];
@@ -215,7 +221,8 @@ static void launch_m2l_mx(Context& C, int p, int pe, const Lists& Ls, const Tree
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)std::max(per_cu, 1) * C.n_cu));
     kfn<<<grid, 64 * NW, sh, C.stream>>>(Ls.m2l_targets.data(), Ls.m2l_offsets.data(), Ls.m2l_s.data(), tt_dst.branches.data(),
                                          st.branches.data(), st.expansions.data(), tt_dst.expansions.data(),
-                                         (int)Ls.n_m2l_targets, p, pe, E, perm, Ls.m2l_order.data(), flags, tb);
+                                         (int)Ls.n_m2l_targets, p, pe, E, perm, Ls.m2l_order.data(), flags,
+                                         Ls.track_box_ok ? const_cast<int32_t*>(Ls.m2l_box_ok.data()) : nullptr, tb);
     LAUNCH_CHECK();
 }
 

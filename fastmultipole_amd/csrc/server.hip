@@ -8,11 +8,15 @@
 // (the budget), or the first waiting request when nothing runs, so a request
 // larger than the budget still completes, alone. Requests that arrive while
 // others run are admitted as slots and memory free up, and their kernels overlap
-// on the device through the workers' streams. Each request's work is the same
-// library call it would be alone (the FMM passes are deterministic, so results
-// are identical alone and in the mix; a solve's sweep adds right-hand-side
-// updates atomically and matches to rounding). Latency is measured from submit
-// to completion.
+// on the device through the workers' streams. A worker that takes an FMM request
+// also takes the FMM requests queued behind it with the same kernel and options
+// (up to a request and body cap, within the budget) and runs them as one batch
+// evaluation (fmmhip_eval_set_batch: one tree per request, every pass of all of
+// them in the same launches); each request's results and status are those of its
+// own fmm! call. The FMM passes are deterministic, so results are identical
+// alone, batched and in the mix; a solve's sweep adds right-hand-side updates
+// atomically and matches to rounding. Latency is measured from submit to
+// completion.
 //
 // This layer uses the library's own C ABI, like any other client.
 #include "fmmhip.h"
@@ -83,6 +87,8 @@ struct fmmhip_server {
     std::string init_error;
     int init_status = FMMHIP_OK;
     int ready = 0;
+    int32_t batch_requests = 64;        // FMM requests run together at most (1: none)
+    int64_t batch_bodies = 4 << 20;     // bodies of one such batch at most
 };
 
 namespace {
@@ -121,6 +127,54 @@ void run_job(fmmhip_ctx* ctx, fmmhip_eval* ev, Job& J) {
     J.status = st;
 }
 
+bool same_options(const fmmhip_options& a, const fmmhip_options& b) {
+    return a.expansion_order == b.expansion_order && a.leaf_size_source == b.leaf_size_source &&
+           a.leaf_size_target == b.leaf_size_target && a.multipole_acceptance == b.multipole_acceptance &&
+           a.list_method == b.list_method && a.shrink_recenter == b.shrink_recenter && a.error_kind == b.error_kind &&
+           a.error_relative == b.error_relative && a.error_tolerance == b.error_tolerance &&
+           a.error_bonus == b.error_bonus && a.scalar_potential == b.scalar_potential && a.gradient == b.gradient &&
+           a.hessian == b.hessian && a.farfield == b.farfield && a.nearfield == b.nearfield &&
+           a.self_induced == b.self_induced && a.precision == b.precision;
+}
+
+// FMM requests of one kernel and options as one batch evaluation; each gets its own results and status
+void run_batch(fmmhip_ctx* ctx, fmmhip_eval* ev, std::vector<Job*>& G) {
+    const fmmhip_request& r = G[0]->req;
+    const size_t dpb = (size_t)fmmhip::data_per_body(r.kernel);
+    std::vector<int64_t> counts;
+    int64_t n = 0;
+    for (Job* J : G) { counts.push_back(J->req.n); n += J->req.n; }
+    std::vector<double> src(dpb * (size_t)n), out;
+    size_t o = 0;
+    for (Job* J : G) { std::memcpy(src.data() + o, J->src.data(), J->src.size() * sizeof(double)); o += J->src.size(); }
+    std::vector<int32_t> pmax(G.size(), 0), ok(G.size(), 1);
+    int st = fmmhip_eval_set_batch(ev, r.kernel, (int64_t)G.size(), counts.data(), src.data(), 0);
+    if (st == FMMHIP_OK) {
+        fmmhip_stats s{};
+        st = fmmhip_fmm(ev, &r.options, &s);
+    }
+    if (st == FMMHIP_OK) st = fmmhip_eval_get_batch_stats(ev, pmax.data(), ok.data());
+    if (st == FMMHIP_OK) {
+        out.resize((size_t)16 * n);
+        st = fmmhip_eval_get_targets(ev, out.data(), 0);
+    }
+    const std::string err = st == FMMHIP_OK ? std::string() : std::string(fmmhip_last_error(ctx));
+    o = 0;
+    for (size_t i = 0; i < G.size(); ++i) {
+        Job& J = *G[i];
+        if (st == FMMHIP_OK) {
+            J.pmax = pmax[i];
+            J.error_success = ok[i];
+            J.targets.assign(out.begin() + o, out.begin() + o + (size_t)16 * J.req.n);
+            J.status = ok[i] ? FMMHIP_OK : FMMHIP_ETOL;
+        } else {
+            J.status = st;
+            J.error = err;
+        }
+        o += (size_t)16 * J.req.n;
+    }
+}
+
 void worker(fmmhip_server* S) {
     fmmhip_ctx* ctx = nullptr;
     fmmhip_eval* ev = nullptr;
@@ -137,7 +191,7 @@ void worker(fmmhip_server* S) {
     }
     S->cv_done.notify_all();
     while (st == FMMHIP_OK) {
-        Job* J = nullptr;
+        std::vector<Job*> G;   // the request taken, then the FMM requests that join it
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
@@ -146,28 +200,50 @@ void worker(fmmhip_server* S) {
                 for (; it != S->queue.end(); ++it)
                     if (S->running == 0 || S->in_flight + (*it)->footprint <= S->budget) break;
                 if (it != S->queue.end()) {
-                    J = *it;
+                    Job* J = *it;
                     S->queue.erase(it);
                     S->in_flight += J->footprint;
                     S->running += 1;
+                    G.push_back(J);
+                    if (J->req.kind == FMMHIP_REQUEST_FMM && J->req.n > 0) {
+                        int64_t bodies = J->req.n;
+                        for (auto k = S->queue.begin(); k != S->queue.end() && (int32_t)G.size() < S->batch_requests;) {
+                            const fmmhip_request& q = (*k)->req;
+                            if (q.kind == FMMHIP_REQUEST_FMM && q.kernel == J->req.kernel && q.n > 0 &&
+                                bodies + q.n <= S->batch_bodies && same_options(q.options, J->req.options) &&
+                                S->in_flight + (*k)->footprint <= S->budget) {
+                                S->in_flight += (*k)->footprint;
+                                bodies += q.n;
+                                G.push_back(*k);
+                                k = S->queue.erase(k);
+                            } else {
+                                ++k;
+                            }
+                        }
+                    }
                     break;
                 }
                 S->cv_work.wait(lk);
             }
-            if (!J) break;
+            if (G.empty()) break;
         }
-        J->t_start = now_ms();
-        run_job(ctx, ev, *J);
-        J->t_end = now_ms();
+        const double t0 = now_ms();
+        if (G.size() == 1) run_job(ctx, ev, *G[0]);
+        else run_batch(ctx, ev, G);
+        const double t1 = now_ms();
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->in_flight -= J->footprint;
             S->running -= 1;
-            J->done = true;
-            S->latencies.push_back(J->t_end - J->t_submit);
-            S->queue_times.push_back(J->t_start - J->t_submit);
-            S->last_done = std::max(S->last_done, J->t_end);
-            S->completed += 1;
+            for (Job* J : G) {
+                J->t_start = t0;
+                J->t_end = t1;
+                S->in_flight -= J->footprint;
+                J->done = true;
+                S->latencies.push_back(J->t_end - J->t_submit);
+                S->queue_times.push_back(J->t_start - J->t_submit);
+                S->completed += 1;
+            }
+            S->last_done = std::max(S->last_done, t1);
         }
         S->cv_done.notify_all();
         S->cv_work.notify_all();
@@ -224,6 +300,14 @@ void fmmhip_server_destroy(fmmhip_server* S) {
     for (Job* J : S->queue)
         if (!S->jobs.count(J->ticket)) delete J;
     delete S;
+}
+
+fmmhip_status fmmhip_server_set_batching(fmmhip_server* S, int32_t max_requests, int64_t max_bodies) {
+    if (!S || max_requests < 1 || max_bodies < 1) return FMMHIP_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->batch_requests = max_requests;
+    S->batch_bodies = max_bodies;
+    return FMMHIP_OK;
 }
 
 fmmhip_status fmmhip_server_submit(fmmhip_server* S, const fmmhip_request* r, int64_t* ticket) {

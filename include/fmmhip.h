@@ -145,6 +145,11 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
  * stats (pmax, error_success, counts, times) cover the whole batch. FastGaussSeidel takes one system. */
 fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_systems, const int64_t* counts,
                                      const double* source_buffer, int32_t on_device);
+/* After an evaluation: per system of the batch, the expansion order and error_success its own fmm! call
+ * reports (pmax: the largest order its M2L interactions used, plus one where the tolerance was not met).
+ * Arrays of n_systems entries (one entry for an evaluation of a single system); either may be null.
+ * No counterpart in the reference (it belongs to fmmhip_eval_set_batch). */
+fmmhip_status fmmhip_eval_get_batch_stats(fmmhip_eval* ev, int32_t* pmax, int32_t* error_success);
 
 /* Stages of fmm!: Tree() for both systems (src/tree.jl:3-122) */
 fmmhip_status fmmhip_eval_build_trees(fmmhip_eval* ev, const fmmhip_options* opt);
@@ -271,6 +276,11 @@ typedef struct fmmhip_server fmmhip_server;
 fmmhip_status fmmhip_server_create(int device, int32_t n_slots, double hbm_budget_bytes, fmmhip_server** out);
 /* waits for running requests; requests not yet taken are dropped */
 void fmmhip_server_destroy(fmmhip_server* server);
+/* FMM requests with the same kernel and options that wait together run as one batch evaluation
+ * (fmmhip_eval_set_batch), up to max_requests requests and max_bodies bodies (defaults 64 and 4 Mi);
+ * max_requests = 1 runs every request alone. Results and statuses are those of the requests' own
+ * calls. No counterpart in the reference. */
+fmmhip_status fmmhip_server_set_batching(fmmhip_server* server, int32_t max_requests, int64_t max_bodies);
 fmmhip_status fmmhip_server_submit(fmmhip_server* server, const fmmhip_request* request, int64_t* ticket);
 /* blocks until the request ends and returns its status; target_out (16 x n, original order) and
  * strengths_out (SOLVE, n) are host buffers or NULL, filled when the status is FMMHIP_OK or

@@ -81,6 +81,9 @@ def test_batch_matches_single_calls(name, kernel, sizes, kw):
     assert st.n_direct == sum(c[3] for _, c in singles)
     assert st.pmax == max(c[4] for _, c in singles)
     assert st.error_success == min(c[5] for _, c in singles)
+    pm, ok = ev.batch_stats()
+    np.testing.assert_array_equal(pm, [c[4] for _, c in singles])
+    np.testing.assert_array_equal(ok, [bool(c[5]) for _, c in singles])
     ev.close()
 
 

@@ -211,3 +211,50 @@ def test_server_batched_uneven_instances_match_oracle():
         e32 = np.abs(results[(i, "mixed")][:, lo:hi] - ref[:, lo:hi]).max() / sc
         assert e64 < 1e-11, (i, e64)
         assert e32 < 1e-5, (i, e32)
+
+
+def test_server_batches_waiting_requests_like_single_calls():
+    """FMM requests with the same kernel and options that wait together run as one batch evaluation. Two
+    option sets (a relative tolerance some systems miss at p <= 10, and a fixed order) plus a solve are
+    submitted at once to one worker: each request's targets are bit-identical to its own fmm! call, and its
+    status, pmax and error_success are that call's. Requests that ran together share start and end times:
+    with one worker and everything queued, at least one batch holds several requests, and no batch mixes
+    the two option sets. With batch_requests=1 every request runs alone."""
+    print("[gpu] server batching of waiting requests", flush=True)
+    rel = dict(leaf_size=30, expansion_order=10, multipole_acceptance=0.5,
+               error_tolerance=F.PowerRelativeGradient(1e-9))
+    fixed = dict(leaf_size=20, expansion_order=6, multipole_acceptance=0.4)
+    sizes = [3000, 1, 9000, 500, 20000, 2, 4000, 12000, 800, 6000]
+    reqs = []
+    for i, n in enumerate(sizes):
+        rows = W.vortex_ring(n, seed=600 + i) if i % 2 == 0 else W.uniform_vortices(n, seed=600 + i)
+        reqs.append((_Rows(_lib.POINT_VORTEX, rows), rel if i % 3 != 2 else fixed))
+
+    def single(sysm, kw):
+        ev = F.Evaluation()
+        _, ev, *_ = F.fmm(sysm, evaluation=ev, update_target_systems=False, **kw)
+        out, st = ev.targets(), ev.last_stats
+        ev.close()
+        return out, st.pmax, st.error_success
+
+    singles = [single(s, kw) for s, kw in reqs]
+    kind, srows, ext = _solve_inputs(SOLVE_CASES[0][1], SOLVE_CASES[0][3])
+    for batch_requests in (64, 1):
+        with F.Server(n_slots=1, batch_requests=batch_requests) as server:
+            tickets = [server.submit_fmm(s, **kw) for s, kw in reqs]
+            tsolve = server.submit_solve(_Rows(kind, srows), ext, **SOLVE_CASES[0][2])
+            got = [server.wait(t) for t in tickets]
+            server.wait(tsolve)
+        groups = {}
+        for i, ((out, _, st), (ref, pmax, ok)) in enumerate(zip(got, singles)):
+            np.testing.assert_array_equal(out, ref, err_msg=f"request {i} (n={sizes[i]})")
+            assert st["pmax"] == pmax and st["error_success"] == ok, (i, st, pmax, ok)
+            assert st["status"] == (_lib.FMMHIP_OK if ok else _lib.FMMHIP_ETOL), (i, st)
+            groups.setdefault((st["start_ms"], st["end_ms"]), []).append(i)
+        sets = [{id(reqs[i][1]) for i in g} for g in groups.values()]
+        assert all(len(s) == 1 for s in sets), groups
+        if batch_requests == 1:
+            assert all(len(g) == 1 for g in groups.values()), groups
+        else:
+            assert max(len(g) for g in groups.values()) > 1, groups
+        print(f"[gpu] batch_requests={batch_requests}: groups {sorted(groups.values())}", flush=True)
