@@ -396,7 +396,7 @@ class Server:
     evaluation (up to batch_requests requests and batch_bodies bodies; batch_requests=1: each alone).
     submit_* copy the inputs and return a ticket at once; wait(ticket) blocks."""
 
-    def __init__(self, n_slots=4, hbm_budget_bytes=64 * 2**30, device=-1, batch_requests=64, batch_bodies=4 << 20):
+    def __init__(self, n_slots=4, hbm_budget_bytes=64 * 2**30, device=-1, batch_requests=64, batch_bodies=2 << 20):
         self.lib = L.load()
         h = C.c_void_p()
         st = self.lib.fmmhip_server_create(int(device), int(n_slots), float(hbm_budget_bytes), C.byref(h))

@@ -9,8 +9,9 @@
 // larger than the budget still completes, alone. Requests that arrive while
 // others run are admitted as slots and memory free up, and their kernels overlap
 // on the device through the workers' streams. A worker that takes an FMM request
-// also takes the FMM requests queued behind it with the same kernel and options
-// (up to a request and body cap, within the budget) and runs them as one batch
+// also takes FMM requests queued behind it with the same kernel and options (its
+// share of the queue spread over it and the idle workers, up to a request and body
+// cap, within the budget) and runs them as one batch
 // evaluation (fmmhip_eval_set_batch: one tree per request, every pass of all of
 // them in the same launches); each request's results and status are those of its
 // own fmm! call. The FMM passes are deterministic, so results are identical
@@ -88,7 +89,7 @@ struct fmmhip_server {
     int init_status = FMMHIP_OK;
     int ready = 0;
     int32_t batch_requests = 64;        // FMM requests run together at most (1: none)
-    int64_t batch_bodies = 4 << 20;     // bodies of one such batch at most
+    int64_t batch_bodies = 2 << 20;     // bodies of one such batch at most
 };
 
 namespace {
@@ -206,8 +207,13 @@ void worker(fmmhip_server* S) {
                     S->running += 1;
                     G.push_back(J);
                     if (J->req.kind == FMMHIP_REQUEST_FMM && J->req.n > 0) {
+                        // this worker's share of what waits: the queue spread over it and the idle workers,
+                        // so a burst still runs on every worker
+                        const int64_t idle = std::max<int64_t>(0, (int64_t)S->workers.size() - S->running);
+                        const int64_t share = ((int64_t)S->queue.size() + 1 + idle) / (idle + 1);
+                        const int64_t most = std::min<int64_t>(S->batch_requests, share);
                         int64_t bodies = J->req.n;
-                        for (auto k = S->queue.begin(); k != S->queue.end() && (int32_t)G.size() < S->batch_requests;) {
+                        for (auto k = S->queue.begin(); k != S->queue.end() && (int64_t)G.size() < most;) {
                             const fmmhip_request& q = (*k)->req;
                             if (q.kind == FMMHIP_REQUEST_FMM && q.kernel == J->req.kernel && q.n > 0 &&
                                 bodies + q.n <= S->batch_bodies && same_options(q.options, J->req.options) &&

@@ -277,7 +277,8 @@ fmmhip_status fmmhip_server_create(int device, int32_t n_slots, double hbm_budge
 /* waits for running requests; requests not yet taken are dropped */
 void fmmhip_server_destroy(fmmhip_server* server);
 /* FMM requests with the same kernel and options that wait together run as one batch evaluation
- * (fmmhip_eval_set_batch), up to max_requests requests and max_bodies bodies (defaults 64 and 4 Mi);
+ * (fmmhip_eval_set_batch): a worker takes its share of the waiting requests (the queue spread over it
+ * and the idle workers), up to max_requests requests and max_bodies bodies (defaults 64 and 2 Mi);
  * max_requests = 1 runs every request alone. Results and statuses are those of the requests' own
  * calls. No counterpart in the reference. */
 fmmhip_status fmmhip_server_set_batching(fmmhip_server* server, int32_t max_requests, int64_t max_bodies);

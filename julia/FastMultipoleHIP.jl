@@ -382,7 +382,7 @@ end
 `fmm!(system; kwargs...)` for each of several independent systems of one kernel (no counterpart in the
 reference, whose callers loop over `fmm!`): one tree per system, lists within each system, every pass of
 all systems in the same launches. Each system's results equal its own `fmm!` call. Returns the batch's
-`FmmHipStats` (`error_success == 0` if any system missed the tolerance).
+`FmmHipStats` and per system the `(pmax, error_success)` its own `fmm!` call reports.
 """
 function hip_fmm_batch!(hip::HipEval, systems::AbstractVector; update_target_systems=true, kwargs...)
     isempty(systems) && return FmmHipStats()
@@ -411,7 +411,10 @@ function hip_fmm_batch!(hip::HipEval, systems::AbstractVector; update_target_sys
             first += n
         end
     end
-    return stats
+    pmax = zeros(Int32, length(counts)); ok = zeros(Int32, length(counts))
+    check(ccall((:fmmhip_eval_get_batch_stats, libfmmhip), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}), hip.ev, pmax, ok),
+          hip.ctx)
+    return stats, [(Int(pmax[i]), ok[i] != 0) for i in eachindex(counts)]
 end
 
 # fmm! with an optional `hip` keyword: GPU when routable, the reference's path otherwise
@@ -500,21 +503,26 @@ end
 #--- request server ---#
 
 """
-    HipServer(; slots=4, hbm_budget=64 * 2^30, device=-1)
+    HipServer(; slots=4, hbm_budget=64 * 2^30, device=-1, batch_requests=64, batch_bodies=2 << 20)
 
 Many `fmm!(system)` evaluations and FastGaussSeidel solves of uneven sizes, submitted from any task,
-run concurrently on one device (fmmhip_server_*). `wait!` blocks in C: call it from a task on a
-thread of its own so the Julia scheduler keeps running.
+run concurrently on one device (fmmhip_server_*). FMM requests with the same kernel and options that
+wait together run as one batch evaluation (up to `batch_requests` requests and `batch_bodies` bodies;
+`batch_requests=1`: each alone), each with its own call's results and status. `wait!` blocks in C:
+call it from a task on a thread of its own so the Julia scheduler keeps running.
 """
 mutable struct HipServer
     h::Ptr{Cvoid}
-    function HipServer(; slots::Integer=4, hbm_budget::Real=64 * 2^30, device::Integer=-1)
+    function HipServer(; slots::Integer=4, hbm_budget::Real=64 * 2^30, device::Integer=-1,
+                       batch_requests::Integer=64, batch_bodies::Integer=2 << 20)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         st = ccall((:fmmhip_server_create, libfmmhip), Cint, (Cint, Cint, Cdouble, Ref{Ptr{Cvoid}}), device, slots,
                    hbm_budget, h)
         check(st, C_NULL)
         s = new(h[])
         finalizer(close!, s)
+        check(ccall((:fmmhip_server_set_batching, libfmmhip), Cint, (Ptr{Cvoid}, Int32, Int64), s.h, batch_requests,
+                    batch_bodies), C_NULL)
         return s
     end
 end
