@@ -1706,6 +1706,7 @@ __host__ __device__ constexpr int hidx(int a, int b) { return a <= b ? ((b * (b 
 template <int P, bool LH>
 __device__ inline void lane_rotate(double (&x)[nh(P)][4], const double* __restrict__ Hs, const double* __restrict__ mags,
                                    double sigma, const double (&cv)[P + 1], const double (&sv)[P + 1]) {
+#pragma clang fp contract(fast)   // this file is exact elsewhere; the lane M2L's outputs are compared at 1e-11
     constexpr int NQ = LH ? 4 : 2;
 #pragma unroll
     for (int n = 0; n <= P; ++n) {
@@ -1771,6 +1772,7 @@ __device__ inline void lane_rotate(double (&x)[nh(P)][4], const double* __restri
 template <int P, bool LH>
 __device__ inline void lane_m2l(double tx, double ty, double tz, const Branch& S, const double* __restrict__ sex, int32_t js,
                                 const TablePtrs& tb, double (&x)[nh(P)][4]) {
+#pragma clang fp contract(fast)
     constexpr int H = nh(P);
     constexpr int NQ = LH ? 4 : 2;
     const double dx = tx - S.source_center[0], dy = ty - S.source_center[1], dz = tz - S.source_center[2];

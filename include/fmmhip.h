@@ -131,7 +131,11 @@ void fmmhip_eval_destroy(fmmhip_eval* ev);
 
 /* Hand over one target system and one source system (fmm!(targets, sources)).
  * on_device = 0: host pointers, copied in. on_device = 1: device pointers (HBM),
- * read in place during the following calls; they must stay valid. */
+ * read in place during the following calls; they must stay valid, and positions read in
+ * place must not change between fmmhip_eval_build_trees and the evaluation that follows
+ * (the trees are built from them, and the influence estimate of a relative tolerance
+ * reuses the source tree's octant keys within that evaluation; fmmhip_fmm builds and
+ * evaluates in one call). */
 fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n_targets, const double* target_xyz,
                                       int64_t n_sources, const double* source_buffer, int32_t on_device);
 

@@ -420,3 +420,28 @@ def test_mixed_precision_ring_error():
     _say(f"    fp64 form rel err max {e64[0]:.3e} p99 {e64[1]:.3e};  mixed max {e32[0]:.3e} p99 {e32[1]:.3e}")
     assert e64[1] < 1e-6
     assert e32[1] < 1e-5
+
+
+@pytest.mark.xfail(strict=False, reason="the mixed form's FP32 nearfield sets an error floor above 1e-6: the "
+                                        "1M-ring bench measures max 1.44e-6 (velocity) and 1.81e-6 (gradient)")
+def test_mixed_form_meets_bench_bar():
+    """The bench's accuracy bar for the mixed form at its settings (PowerRelativeGradient(1e-9), p <= 20,
+    leaf 64, MAC 0.55) on a 100k ring: max relative error of velocity and gradient <= 1e-6 against the FP64
+    direct sum on 4,000 sampled targets. Expected to fail until the FP32 pair terms change."""
+    _say("[gpu] mixed form at the bench bar")
+    sbuf = W.vortex_ring(100000, seed=62)
+    ev, opt = _gpu_eval(O.KIND_VORTEX, sbuf, expansion_order=20, leaf_size_source=64, multipole_acceptance=0.55,
+                        error_tolerance=F.PowerRelativeGradient(1e-9), hessian=True, precision="mixed")
+    ev.fmm(opt)
+    got = ev.targets()
+    idx = np.sort(np.random.default_rng(5).choice(len(sbuf), 4000, replace=False))
+    txyz = np.ascontiguousarray(sbuf[idx, :3])
+    ref = np.zeros((len(idx), 16))
+    ref[:, :3] = txyz
+    ctx = F.default_context()
+    ctx.check(ctx.lib.fmmhip_direct(ctx._h, O.KIND_VORTEX, len(idx), txyz.ctypes.data, len(sbuf),
+                                    np.ascontiguousarray(sbuf).ctypes.data, 0, 1, 1, _lib.FP64, ref.ctypes.data, 0))
+    ev_v = W.relative_error(got[idx, 4:7], ref[:, 4:7])
+    ev_g = W.relative_error(got[idx, 7:16], ref[:, 7:16])
+    _say(f"    mixed form: velocity max {ev_v[0]:.3e}, gradient max {ev_g[0]:.3e}")
+    assert ev_v[0] <= 1e-6 and ev_g[0] <= 1e-6, (ev_v[0], ev_g[0])
