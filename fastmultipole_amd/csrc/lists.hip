@@ -13,6 +13,7 @@
 // stable radix sort by target then gives sort_by_target's output. Compiled with
 // -ffp-contract=off (the MAC must match).
 #include "fmmhip_internal.hpp"
+#include "scan3.hpp"
 #include <cstring>   // host memset for rocprim headers
 #include <rocprim/rocprim.hpp>
 #include <algorithm>
@@ -150,10 +151,7 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)std::ma
 
 template <class T, class O>
 static void excl_scan(Context& C, const T* in, O* out, int64_t n) {
-    size_t bytes = 0;
-    HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, O(0), (size_t)n, rocprim::plus<O>(), C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, O(0), (size_t)n, rocprim::plus<O>(), C.stream));
+    excl_scan3<T, O>(C, in, out, n);
 }
 
 // number of body-body pairs of the direct list (nearfield work): grid-stride

@@ -21,6 +21,7 @@
 // here (centres, boxes, radii, the leaf test, the MAC) must match the
 // reference's IEEE double arithmetic bit for bit.
 #include "fmmhip_internal.hpp"
+#include "scan3.hpp"
 #include <cstring>   // host memset for rocprim headers
 #include <rocprim/rocprim.hpp>
 #include <algorithm>
@@ -344,10 +345,7 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)std::ma
 
 template <class T>
 static void exclusive_scan_i32(Context& C, const T* in, T* out, int64_t n) {
-    size_t bytes = 0;
-    HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), C.stream));
+    excl_scan3<T, T>(C, in, out, n);
 }
 
 // stable sort by a 63-bit key, within each system's range when seg (device, n_seg + 1 offsets) is given
