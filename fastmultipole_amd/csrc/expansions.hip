@@ -2460,17 +2460,17 @@ __global__ __launch_bounds__(64) void k_l2p_coef(const int64_t* __restrict__ lea
     }
 }
 
-__global__ __launch_bounds__(128) void k_l2p_eval(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
-                                                  const double* __restrict__ x, const double* __restrict__ y,
-                                                  const double* __restrict__ z, int P, const double* __restrict__ coef,
-                                                  double* __restrict__ tout, int64_t nt, int PS, int GS, int HS) {
-    extern __shared__ double lc[];
+// One wavefront per leaf. Every lane reads the same coefficients at the same time, so they come from
+// memory through the scalar cache (wave-uniform addresses) rather than being staged in LDS, whose 48 KB
+// per leaf at order 20 held a CU to three leaves.
+__global__ __launch_bounds__(64) void k_l2p_eval(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
+                                                 const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ z, int P, const double* __restrict__ coef,
+                                                 double* __restrict__ tout, int64_t nt, int PS, int GS, int HS) {
     const int64_t ib = leaf_index[blockIdx.x];
     const Branch B = br[ib];
     const int H = nh(P);
-    const double* cg = coef + (int64_t)blockIdx.x * 26 * H;
-    for (int k = threadIdx.x; k < 26 * H; k += blockDim.x) lc[k] = cg[k];
-    __syncthreads();
+    const double* __restrict__ lc = coef + (int64_t)blockIdx.x * 26 * H;
     for (int64_t t = threadIdx.x; t < B.n_bodies; t += blockDim.x) {
         const int64_t it = B.first + t;
         const double dx = x[it] - B.target_center[0], dy = y[it] - B.target_center[1], dz = z[it] - B.target_center[2];
@@ -2512,7 +2512,7 @@ void run_l2p(Context& C, Tree& tt, double* tout, int64_t nt, int p, bool LH, boo
     else
         k_l2p_coef<false><<<(unsigned)tt.n_leaves, 64, 0, C.stream>>>(tt.leaf_index.data(), tt.expansions.data(), p, PS, GS, HS, tb.inv_fact, coef.data());
     LAUNCH_CHECK();
-    k_l2p_eval<<<(unsigned)tt.n_leaves, 128, sizeof(double) * 26 * nh(p), C.stream>>>(
+    k_l2p_eval<<<(unsigned)tt.n_leaves, 64, 0, C.stream>>>(
         tt.branches.data(), tt.leaf_index.data(), tt.x.data(), tt.y.data(), tt.z.data(), p, coef.data(), tout, nt, PS, GS, HS);
     LAUNCH_CHECK();
 }
