@@ -644,34 +644,58 @@ __device__ void translate_fixed(const double* __restrict__ src, double dx, doubl
 }
 
 // ================================================================= M2M
+// Per level, one wavefront per child translates the child's multipole to its parent's centre into a
+// per-child slot; a second launch adds each parent's children's slots, in child order, into the
+// parent's expansion. A wavefront per parent translating its up to 8 children in turn left the top
+// levels (1, 8, 64 ... parents) to a handful of wavefronts.
 template <bool LH, int NW>
-__global__ __launch_bounds__(64 * NW) void k_m2m_level(Branch* __restrict__ br, double* __restrict__ ex, int64_t b0,
-                                                       int64_t b1, int P, TablePtrs tb) {
+__global__ __launch_bounds__(64 * NW) void k_m2m_child(const Branch* __restrict__ br, const double* __restrict__ ex,
+                                                       int64_t c0, int64_t c1, int P, TablePtrs tb, double* __restrict__ slot) {
     extern __shared__ double lds[];
     const double* A = load_tables(lds, tb, P);
     const WaveLds W = carve_wave(lds, P);
-    const int64_t ib = b0 + (int64_t)blockIdx.x * NW + wave_id();
-    if (ib > b1) return;
-    const Branch B = br[ib];
-    if (B.n_branches == 0) return;
+    const int64_t ic = c0 + (int64_t)blockIdx.x * NW + wave_id();
+    if (ic > c1) return;
+    const Branch& Ch = br[ic];
+    const Branch& B = br[Ch.parent];
     const int H4 = 4 * nh(P);
     double acc[MAXCH][8] = {};
-    for (int64_t c = B.child_first; c < B.child_first + B.n_branches; ++c) {
-        const Branch& Ch = br[c];
-        double dx = B.source_center[0] - Ch.source_center[0], dy = B.source_center[1] - Ch.source_center[1],
-               dz = B.source_center[2] - Ch.source_center[2];
-        translate_fixed<OP_M2M, LH>(ex + c * H4, dx, dy, dz, W, A, P, tb, acc);
-    }
-    acc_store<true>(ex + ib * H4, P, acc);
+    const double dx = B.source_center[0] - Ch.source_center[0], dy = B.source_center[1] - Ch.source_center[1],
+                 dz = B.source_center[2] - Ch.source_center[2];
+    translate_fixed<OP_M2M, LH>(ex + ic * H4, dx, dy, dz, W, A, P, tb, acc);
+    acc_store<false>(slot + (ic - c0) * H4, P, acc);
+}
+
+// every coefficient of every parent of a level: the sum of its children's slots, in child order
+__global__ void k_m2m_sum(const Branch* __restrict__ br, double* __restrict__ ex, int64_t b0, int64_t b1, int64_t c0,
+                          int H4, const double* __restrict__ slot) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (b1 - b0 + 1) * H4) return;
+    const int64_t ib = b0 + t / H4;
+    const int q = (int)(t % H4);
+    const Branch& B = br[ib];
+    if (B.n_branches == 0) return;
+    double s = 0.0;
+    for (int64_t c = B.child_first; c < B.child_first + B.n_branches; ++c) s += slot[(c - c0) * H4 + q];
+    ex[ib * H4 + q] += s;
 }
 
 template <bool LH, int NW>
 static void launch_m2m(Context& C, Tree& st, int p, const TablePtrs& tb) {
-    const size_t sh = translate_setup(k_m2m_level<LH, NW>, p, NW);
+    const size_t sh = translate_setup(k_m2m_child<LH, NW>, p, NW);
+    const int H4 = 4 * nh(p);
+    int64_t widest = 1;
+    for (size_t l = 1; l < st.level_first.size(); ++l) widest = std::max<int64_t>(widest, st.level_last[l] - st.level_first[l] + 1);
+    C.m2m_slot.resize((size_t)widest * H4);
     for (int64_t l = (int64_t)st.level_first.size() - 2; l >= 0; --l) {
-        int64_t b0 = st.level_first[l], b1 = st.level_last[l];
-        const unsigned grid = (unsigned)((b1 - b0 + NW) / NW);
-        k_m2m_level<LH, NW><<<grid, 64 * NW, sh, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, p, tb);
+        const int64_t b0 = st.level_first[l], b1 = st.level_last[l];
+        const int64_t c0 = st.level_first[l + 1], c1 = st.level_last[l + 1];
+        k_m2m_child<LH, NW><<<(unsigned)((c1 - c0 + NW) / NW), 64 * NW, sh, C.stream>>>(st.branches.data(), st.expansions.data(),
+                                                                                       c0, c1, p, tb, C.m2m_slot.data());
+        LAUNCH_CHECK();
+        const int64_t items = (b1 - b0 + 1) * H4;
+        k_m2m_sum<<<(unsigned)((items + 255) / 256), 256, 0, C.stream>>>(st.branches.data(), st.expansions.data(), b0, b1, c0,
+                                                                        H4, C.m2m_slot.data());
         LAUNCH_CHECK();
     }
 }

@@ -125,6 +125,7 @@ struct Context {
     ListScratch* list_scratch = nullptr;
     DBuf<int32_t> m2l_flags;    // pmax and success of the last M2L pass
     DBuf<double> l2p_coef;      // per-leaf local-to-body coefficients of the last L2P pass
+    DBuf<double> m2m_slot;      // per-child translated multipoles of one level of the upward pass
     DBuf<unsigned long long> p2p_queue;   // work counter of the persistent nearfield launch
     DBuf<double> direct_part;   // chunk sums of the split direct! form
     int n_cu = 0;               // compute units of the device (0 until first needed)
