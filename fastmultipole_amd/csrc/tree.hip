@@ -605,7 +605,7 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     LAUNCH_CHECK();
     // in-leaf order = original index order
     S.key2.resize(n); S.key2_s.resize(n);
-    k_leaf_keys<<<(unsigned)T.n_leaves, 256, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves, S.idx_s.data(),
+    k_leaf_keys<<<(unsigned)T.n_leaves, 64, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves, S.idx_s.data(),
                                                            S.key2.data());
     LAUNCH_CHECK();
     sort_keys_u64(C, S.key2.data(), S.key2_s.data(), n);
