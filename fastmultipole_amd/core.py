@@ -262,6 +262,36 @@ class Evaluation:
         self.ctx.check(self.ctx.lib.fmmhip_eval_set_batch(self._h, kernel, len(counts), counts.ctypes.data, sp,
                                                           int(on_device)))
 
+    def set_tuple(self, kernels, source_buffers, on_device=False):
+        """Several systems in one tree, each a target and a source of all of them (fmmhip_eval_set_tuple).
+
+        kernels: one point kernel per system; source_buffers: one (n_i, data_per_body) array per system, or
+        with on_device=True one device buffer of 7 x sum(n_i) rows [x, y, z, radius, s0, s1, s2] followed by
+        the list of counts."""
+        kernels = np.ascontiguousarray(kernels, dtype=np.int32)
+        if on_device:
+            buf, counts = source_buffers
+            counts = np.ascontiguousarray(counts, dtype=np.int64)
+            sp = buf.data_ptr() if hasattr(buf, "data_ptr") else int(buf)
+            self._keep = (buf, counts, kernels)
+        else:
+            counts = np.array([b.shape[0] for b in source_buffers], dtype=np.int64)
+            rows = []
+            for b in source_buffers:
+                b = np.asarray(b, dtype=np.float64)
+                r = np.zeros((b.shape[0], 7))
+                r[:, :min(7, b.shape[1])] = b[:, :7]
+                rows.append(r)
+            sb = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros((0, 7)))
+            self._keep = (sb, counts, kernels)
+            sp = sb.ctypes.data
+        n = int(counts.sum())
+        self.kernel, self.nt, self.ns = None, n, n
+        self.counts = None
+        self.tuple_counts = counts
+        self.ctx.check(self.ctx.lib.fmmhip_eval_set_tuple(self._h, len(counts), kernels.ctypes.data, counts.ctypes.data,
+                                                          sp, int(on_device)))
+
     def batch_stats(self):
         """Per system of the last batch evaluation (one entry for a single system): (pmax, error_success)
         arrays, each system's values as its own fmm! call reports them."""
@@ -524,7 +554,11 @@ def fmm(target_systems, source_systems=None, *, leaf_size=None, leaf_size_source
         multipole_acceptance = 0.4
     targets, sources = _as_tuple(target_systems), _as_tuple(source_systems)
     if len(targets) != 1 or len(sources) != 1:
-        raise NotImplementedError("one target system and one source system per call")
+        if len(targets) != len(sources) or any(t is not s for t, s in zip(targets, sources)):
+            raise NotImplementedError("several systems run as one tuple that is both the targets and the sources")
+        return _fmm_tuple(sources, leaf_size_source, leaf_size_target, expansion_order, multipole_acceptance,
+                          error_tolerance, shrink_recenter, interaction_list_method, scalar_potential, gradient, hessian,
+                          farfield, nearfield, self_induced, update_target_systems, precision, evaluation)
     tsys, ssys = targets[0], sources[0]
     kernel = _kernel_of(ssys)
     ev = evaluation or Evaluation()
@@ -546,6 +580,37 @@ def fmm(target_systems, source_systems=None, *, leaf_size=None, leaf_size_source
                       "multipole_acceptance": opt.multipole_acceptance}
     ev.last_stats = stats
     return (optimized_args, ev, ev.tree(0), ev.tree(1), ev.list(0), ev.list(1), (switch,), bool(stats.error_success))
+
+
+def _fmm_tuple(systems, leaf_size_source, leaf_size_target, expansion_order, multipole_acceptance, error_tolerance,
+               shrink_recenter, interaction_list_method, scalar_potential, gradient, hessian, farfield, nearfield,
+               self_induced, update_target_systems, precision, evaluation):
+    """fmm!((sys1, sys2, ...)) (src/fmm.jl:807-884): one tree over the bodies of every system (src/tree.jl:3-122),
+    each system a target of all and a source with its own kernel. Point kernels, one leaf size, and
+    SelfTuningTreeStop lists (the default)."""
+    if leaf_size_target != leaf_size_source:
+        raise ValueError("a tuple of systems takes one leaf size")
+    if interaction_list_method is None:
+        interaction_list_method = SelfTuningTreeStop()
+    ev = evaluation or Evaluation()
+    ev.set_tuple([_kernel_of(s) for s in systems], [s.source_buffer() for s in systems])
+    opt = make_options(expansion_order, leaf_size_source, leaf_size_target, multipole_acceptance,
+                       interaction_list_method, shrink_recenter, error_tolerance, scalar_potential, gradient, hessian,
+                       farfield, nearfield, self_induced, precision)
+    stats = ev.fmm(opt)
+    out = ev.targets()
+    switch = DerivativesSwitch(bool(scalar_potential), bool(gradient), bool(hessian))
+    offs = np.concatenate([[0], np.cumsum(ev.tuple_counts)])
+    outputs = [out[offs[i]:offs[i + 1]] for i in range(len(systems))]
+    if update_target_systems:
+        for s, o in zip(systems, outputs):
+            s.buffer_to_target_system(o, switch)
+    ev.last_stats = stats
+    ev.tuple_outputs = outputs
+    optimized_args = {"leaf_size_source": opt.leaf_size_source, "expansion_order": max(opt.expansion_order, 1),
+                      "multipole_acceptance": opt.multipole_acceptance}
+    return (optimized_args, ev, ev.tree(0), ev.tree(1), ev.list(0), ev.list(1), (switch,) * len(systems),
+            bool(stats.error_success))
 
 
 def fmm_batch(systems, *, leaf_size=20, expansion_order=5, multipole_acceptance=0.4, error_tolerance=None,

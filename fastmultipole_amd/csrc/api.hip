@@ -1,6 +1,7 @@
 // C ABI of libfmmhip (include/fmmhip.h): contexts, evaluations and the fmm!
 // pipeline (src/fmm.jl:862-1176) on one MI355X.
 #include "fmmhip_internal.hpp"
+#include "scan3.hpp"
 #include <memory>
 #include <cstring>
 #include <algorithm>
@@ -97,6 +98,94 @@ __global__ void k_system_status(const int32_t* __restrict__ m2l_targets, const i
     if (!box_ok[b]) atomicAnd(sys + 2 * lo + 1, 0);
 }
 
+// Several systems in one tree: within a leaf the bodies are in original order, so each system's bodies
+// (original indices off[s] .. off[s + 1] - 1) are one sub-range of the leaf. Per system, a copy of the
+// branches whose leaves hold that sub-range (other branches keep their full range).
+__global__ void k_sys_branches(const Branch* __restrict__ br, int64_t nb, const int64_t* __restrict__ sort_index,
+                               const int64_t* __restrict__ off, int n_sys, Branch* __restrict__ out) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const Branch B = br[b];
+    auto lower = [&](int64_t v) {   // first position of the leaf whose original index is >= v
+        int64_t lo = B.first, hi = B.first + B.n_bodies;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (sort_index[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    for (int s = 0; s < n_sys; ++s) {
+        Branch Bs = B;
+        if (B.n_branches == 0) {
+            const int64_t f = lower(off[s]);
+            Bs.first = f;
+            Bs.n_bodies = lower(off[s + 1]) - f;
+        }
+        out[(int64_t)s * nb + b] = Bs;
+    }
+}
+
+// one system's body count of every direct-list source leaf; entry n is 0 (the scan then ends with the total)
+__global__ void k_sys_dir_counts(const int32_t* __restrict__ dir_s, int64_t n, const Branch* __restrict__ sbr,
+                                 int64_t* __restrict__ cnt) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = sbr[dir_s[i]].n_bodies;
+    else if (i == n) cnt[i] = 0;
+}
+
+static bool multi(const Eval& E) { return !E.sys_kind.empty(); }
+
+static void sys_branches(Eval& E) {
+    if (!multi(E) || E.st.n_branches == 0) return;
+    Context& C = *E.ctx;
+    const int S = (int)E.sys_kind.size();
+    const int64_t nb = E.st.n_branches;
+    E.sys_branches.resize((size_t)S * nb);
+    k_sys_branches<<<grid_for(nb, 256), 256, 0, C.stream>>>(E.st.branches.data(), nb, E.st.sort_index.data(),
+                                                           E.sys_off_dev.data(), S, E.sys_branches.data());
+    LAUNCH_CHECK();
+}
+
+static void sys_dir_cum(Eval& E) {
+    if (!multi(E)) return;
+    Context& C = *E.ctx;
+    const int S = (int)E.sys_kind.size();
+    const int64_t nd = E.lists.n_direct, nb = E.st.n_branches;
+    E.sys_dir_cum.resize((size_t)S * (nd + 1));
+    if (nd == 0) return;
+    DBuf<int64_t> cnt;
+    cnt.resize(nd + 1);
+    for (int s = 0; s < S; ++s) {
+        k_sys_dir_counts<<<grid_for(nd + 1, 256), 256, 0, C.stream>>>(E.lists.dir_s.data(), nd,
+                                                                     E.sys_branches.data() + (size_t)s * nb, cnt.data());
+        LAUNCH_CHECK();
+        excl_scan3<int64_t, int64_t>(C, cnt.data(), E.sys_dir_cum.data() + (size_t)s * (nd + 1), nd + 1);
+    }
+    HIP_CHECK(hipStreamSynchronize(C.stream));   // cnt is freed on return
+}
+
+// P2M of the source tree: once per system of a tuple, with that system's kernel and leaf ranges
+static void eval_p2m(Context& F, Eval& E, int p) {
+    if (!multi(E)) { run_p2m(F, E.kernel, E.st, p); return; }
+    for (size_t s = 0; s < E.sys_kind.size(); ++s)
+        run_p2m(F, E.sys_kind[s], E.st, p, E.sys_branches.data() + s * E.st.n_branches);
+}
+
+// nearfield: once per source system of a tuple (the reference's order, src/fmm.jl:3-40), each with its kernel
+static void eval_p2p(Context& C, Eval& E, bool PS, bool GS, bool HS, int precision, int slots) {
+    if (!multi(E)) {
+        run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, precision, 0, -1, slots);
+        return;
+    }
+    const int64_t nb = E.st.n_branches, nd = E.lists.n_direct;
+    for (size_t s = 0; s < E.sys_kind.size(); ++s) {
+        SourceView sv;
+        sv.branches = E.sys_branches.data() + s * nb;
+        sv.dir_cum = E.sys_dir_cum.data() + s * (nd + 1);
+        run_p2p(C, E.sys_kind[s], E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, precision, 0, -1, slots, sv);
+    }
+}
+
 static void check_options(const fmmhip_options* o) {
     if (!o) throw Error(FMMHIP_EINVAL, "options is NULL");
     if (o->expansion_order < 1 || o->expansion_order > 20) throw Error(FMMHIP_EINVAL, "expansion_order must be in 1..20");
@@ -149,6 +238,7 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
         build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
                    strength_dims(E.kernel));
     }
+    sys_branches(E);
     E.trees_built = true;
     E.lists_built = false;
     E.p_built = p;
@@ -157,8 +247,11 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
 
 static void eval_build_lists(Eval& E, const fmmhip_options& o, bool count_pairs = true) {
     if (!E.trees_built) throw Error(FMMHIP_EINVAL, "trees are not built");
+    if (multi(E) && o.list_method != FMMHIP_SELFTUNING_TREESTOP)
+        throw Error(FMMHIP_EINVAL, "several systems in one tree take SelfTuningTreeStop lists");
     build_lists(*E.ctx, E.tt, E.st, E.lists, o.leaf_size_source, o.multipole_acceptance, o.list_method, o.farfield != 0,
                 o.nearfield != 0, o.self_induced != 0, count_pairs);
+    sys_dir_cum(E);
     E.lists_built = true;
 }
 
@@ -240,6 +333,13 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
     X.s_buf = E.s_buf;
     X.seg = E.seg;
     X.st.seg = E.seg;
+    X.sys_kind = E.sys_kind;
+    X.sys_off = E.sys_off;
+    if (multi(E)) {
+        X.sys_off_dev.resize(E.sys_off.size());
+        HIP_CHECK(hipMemcpyAsync(X.sys_off_dev.data(), E.sys_off_dev.data(), E.sys_off.size() * sizeof(int64_t),
+                                 hipMemcpyDeviceToDevice, C.stream));
+    }
     const fmmhip_options eo = estimate_options(o);
     if (X.t_xyz) eval_build_trees(X, eo);
     else {   // the targets are the sources: one build gives both trees; the Morton keys and their
@@ -247,6 +347,7 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
         build_tree(C, X.st, X.ns, X.s_buf, data_per_body(X.kernel), false, eo.leaf_size_source, eo.list_method, true,
                    eo.expansion_order, strength_dims(X.kernel), E.keys_fresh ? tree_keys(*E.ctx, E.st) : nullptr);
         clone_target_tree(C, X.st, X.tt);
+        sys_branches(X);
         X.trees_built = true;
         X.lists_built = false;
         X.p_built = eo.expansion_order;
@@ -305,7 +406,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         // estimate's tree, list and sort launches) are placed at once instead of waiting for
         // the nearfield's last workgroups.
         const int slots = &F != &C ? -C.p2p_free_slots : 0;
-        run_p2p(C, E.kernel, E.tt, E.st, E.lists, E.tout.data(), E.nt, PS, GS, HS, o.precision, 0, -1, slots);
+        eval_p2p(C, E, PS, GS, HS, o.precision, slots);
         tp.mark();
         // With estimate_influence on the far-field stream, the upward pass follows the nearfield on its
         // stream: it needs nothing of the estimate, so it runs beside the estimate's evaluation once the
@@ -317,7 +418,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         std::unique_ptr<Timer> tu;
         if (up_on_c) {
             tu.reset(new Timer(C));
-            run_p2m(C, E.kernel, E.st, p);
+            eval_p2m(C, E, p);
             run_m2m(C, E.st, p, LH);
             tu->mark();
         }
@@ -333,7 +434,7 @@ static void eval_core(Eval& E, const fmmhip_options& o, fmmhip_stats* stats, boo
         }
         if (!up_on_c) {
             Timer t(F);
-            run_p2m(F, E.kernel, E.st, p);
+            eval_p2m(F, E, p);
             run_m2m(F, E.st, p, LH);
             t_up = t.stop();
         } else {
@@ -527,6 +628,8 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
         E.seg.clear();
         E.sys_pmax.clear();
         E.sys_ok.clear();
+        E.sys_kind.clear();
+        E.sys_off.clear();
         const int dpb = data_per_body(kernel);
         if (on_device) {
             E.t_xyz = t_xyz;
@@ -565,6 +668,8 @@ fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_s
         E.kernel = kernel;
         E.nt = E.ns = n;
         E.t_xyz = nullptr;
+        E.sys_kind.clear();
+        E.sys_off.clear();
         const int dpb = data_per_body(kernel);
         if (on_device) E.s_buf = s_buf;
         else {
@@ -582,6 +687,49 @@ fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_s
         }
         E.sys_pmax.clear();
         E.sys_ok.clear();
+        E.trees_built = E.lists_built = E.ran = false;
+    });
+}
+
+fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const int32_t* kernels, const int64_t* counts,
+                                     const double* buffer, int32_t on_device) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (n_systems < 1 || !kernels || !counts) throw Error(FMMHIP_EINVAL, "a tuple needs at least one system");
+        std::vector<int64_t> off(n_systems + 1, 0);
+        bool lh = false;
+        for (int64_t i = 0; i < n_systems; ++i) {
+            const int k = kernels[i];
+            if (k != FMMHIP_POINT_SOURCE && k != FMMHIP_POINT_VORTEX && k != FMMHIP_POINT_DIPOLE && k != FMMHIP_VORTEX_GAUSSIAN)
+                throw Error(FMMHIP_EINVAL, "systems of a tuple take point kernels");
+            if (counts[i] < 0) throw Error(FMMHIP_EINVAL, "negative system size");
+            lh = lh || has_vector_potential(k);
+            off[i + 1] = off[i] + counts[i];
+        }
+        const int64_t n = off[n_systems];
+        if (n > 0 && !buffer) throw Error(FMMHIP_EINVAL, "missing system buffer");
+        if (n >= (int64_t)1 << 31) throw Error(FMMHIP_EINVAL, "more than 2^31 bodies in one tuple");
+        // the tree's bodies carry 7 rows [x, y, z, radius, s0, s1, s2]; the kernel tag that sizes them, and the
+        // Lamb-Helmholtz terms if any system is a vortex system
+        E.kernel = lh ? FMMHIP_POINT_VORTEX : FMMHIP_POINT_DIPOLE;
+        E.nt = E.ns = n;
+        E.t_xyz = nullptr;
+        E.seg.clear();
+        E.sys_pmax.clear();
+        E.sys_ok.clear();
+        if (on_device) E.s_buf = buffer;
+        else {
+            E.s_buf_in.resize(std::max<int64_t>(1, 7 * n));
+            if (n) HIP_CHECK(hipMemcpyAsync(E.s_buf_in.data(), buffer, sizeof(double) * 7 * n, hipMemcpyHostToDevice, C->stream));
+            E.s_buf = E.s_buf_in.data();
+        }
+        E.sys_kind.assign(kernels, kernels + n_systems);
+        E.sys_off = off;
+        E.sys_off_dev.resize(off.size());
+        HIP_CHECK(hipMemcpyAsync(E.sys_off_dev.data(), off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, C->stream));
+        HIP_CHECK(hipStreamSynchronize(C->stream));
         E.trees_built = E.lists_built = E.ran = false;
     });
 }
@@ -826,7 +974,7 @@ fmmhip_status fmmhip_fgs_setup(fmmhip_eval* ev, const fmmhip_options* opt, int32
             throw Error(FMMHIP_EINVAL, "unknown influence");
         if (influence == FMMHIP_INFLUENCE_NORMAL_GRADIENT && !is_panel(E.kernel))
             throw Error(FMMHIP_EINVAL, "the normal-gradient influence needs panels");
-        if (E.t_xyz || E.nt != E.ns || !E.seg.empty())
+        if (E.t_xyz || E.nt != E.ns || !E.seg.empty() || !E.sys_kind.empty())
             throw Error(FMMHIP_EINVAL, "FastGaussSeidel needs one system as target and source");
         // FastGaussSeidel's fixed choices (src/solve.jl:538-571): Barba lists, farfield and
         // nearfield without self pairs, one leaf size, potential and gradient

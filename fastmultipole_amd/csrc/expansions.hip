@@ -2353,7 +2353,8 @@ __global__ __launch_bounds__(64) void k_p2m_panel(const Branch* __restrict__ br,
     }
 }
 
-void run_p2m(Context& C, int kernel, Tree& st, int p) {
+void run_p2m(Context& C, int kernel, Tree& st, int p, const Branch* sbr) {
+    if (!sbr) sbr = st.branches.data();
     ensure_tables(C, p);
     if (st.n_leaves == 0) return;
     TablePtrs tb = ptrs(C);
@@ -2374,7 +2375,7 @@ void run_p2m(Context& C, int kernel, Tree& st, int p) {
     }
     const size_t p2m_sh = sizeof(double) * (size_t)(64 / (p + 1)) * (2 * nh(p) + 3);
 #define FMMHIP_P2M(K)                                                                                                  \
-    k_p2m<K><<<(unsigned)st.n_leaves, 64, p2m_sh, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.x.data(), st.y.data(), \
+    k_p2m<K><<<(unsigned)st.n_leaves, 64, p2m_sh, C.stream>>>(sbr, st.leaf_index.data(), st.x.data(), st.y.data(),             \
                                                         st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact)
     if (kernel == FMMHIP_POINT_SOURCE) FMMHIP_P2M(FMMHIP_POINT_SOURCE);
     else if (kernel == FMMHIP_POINT_DIPOLE) FMMHIP_P2M(FMMHIP_POINT_DIPOLE);

@@ -174,6 +174,13 @@ struct Eval {
     std::vector<int64_t> seg;                 // several independent systems (each its own target and source):
                                               // their body offsets; empty for one target and one source system
     DBuf<int64_t> seg_dev;                    // seg on the device
+    // several systems in one tree, each a target and a source of all (fmm!((s1, s2, ...))): their point
+    // kernels and body offsets (empty: one system of `kernel`)
+    std::vector<int32_t> sys_kind;
+    std::vector<int64_t> sys_off;
+    DBuf<int64_t> sys_off_dev;
+    DBuf<Branch> sys_branches;                // per system: the source tree's branches with its leaf ranges
+    DBuf<int64_t> sys_dir_cum;                // per system: prefix of its counts over the direct list
     DBuf<int32_t> sys_status;                 // per system of a batch: largest order used, tolerance met
     std::vector<int32_t> sys_pmax, sys_ok;    // host copies after the evaluation (pmax + 1 where not met)
     Tree tt, st;
@@ -226,9 +233,15 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
 // nearfield of the target leaves b0 .. b1 - 1 of the direct list (all by default). The point
 // kernels run as workgroups taking leaves from the context's queue: at most `slots` per CU
 // (0: as many as fit, < 0: that many fewer than fit).
+// One system of several sharing the source tree: its bodies' range in each leaf (a copy of the tree's
+// branches with the leaves' first / n_bodies replaced) and the direct list's prefix of its counts.
+struct SourceView {
+    const Branch* branches = nullptr;
+    const int64_t* dir_cum = nullptr;
+};
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
              bool PS, bool GS, bool HS, int precision,
-             int64_t b0 = 0, int64_t b1 = -1, int slots = 0);
+             int64_t b0 = 0, int64_t b1 = -1, int slots = 0, SourceView sv = SourceView());
 // part: workspace of direct_part_bytes() for the chunk sums of the split form (null: the context's own)
 void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
                     double* tout_aos, bool PS, bool GS, bool HS, int precision, double* part = nullptr);
@@ -242,7 +255,8 @@ size_t direct_scratch_bytes();
 // expansions.hip
 void ensure_tables(Context& C, int p);
 void free_tables(Tables* t);
-void run_p2m(Context& C, int kernel, Tree& st, int p);
+// sbr: the source tree's branches with one system's leaf ranges (several systems in one tree), or null
+void run_p2m(Context& C, int kernel, Tree& st, int p, const Branch* sbr = nullptr);
 void run_m2m(Context& C, Tree& st, int p, bool LH);
 // pmax / success null: no read-back of the launch's largest order and tolerance flag (no host synchronisation)
 void run_m2l(Context& C, const Tree& tt, Tree& st_src, Tree& tt_dst, Lists& L, int p, bool LH, int err_kind,

@@ -518,15 +518,17 @@ static void launch_queued(Context& C, int slots, void (*kfn)(P...), int64_t nblk
 
 template <int K>
 static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
-                       bool HS, int precision, int64_t b0, int64_t b1, int slots) {
+                       bool HS, int precision, int64_t b0, int64_t b1, int slots, SourceView sv) {
     const int64_t nblk = b1 - b0;
     if (nblk <= 0) return;
+    const Branch* sbr = sv.branches ? sv.branches : st.branches.data();
+    const int64_t* cum = sv.dir_cum ? sv.dir_cum : L.dir_cum.data();
     C.p2p_queue.resize(1);
     unsigned long long* queue = C.p2p_queue.data();
     k_zero_queue<<<1, 1, 0, C.stream>>>(queue);
     LAUNCH_CHECK();
 #define P2P_ARGS                                                                                                         \
-    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), L.dir_cum.data(), tt.branches.data(), st.branches.data(), \
+    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), cum, tt.branches.data(), sbr,                            \
         tt.x.data(),                                                                                                     \
         tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.radius.data(), st.n,    \
         tout, nt, b0, queue, nblk
@@ -555,32 +557,32 @@ void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, cons
                     bool GS, bool HS, int64_t b0, int64_t b1);
 // targets b0 .. b1 - 1 of the direct list's CSR, none inside another
 static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
-                          bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots);
+                          bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots, SourceView sv);
 void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, int64_t ns, const double* s_buf,
                        double* tout_aos, bool PS, bool GS, bool HS);
 
 void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS, bool GS,
-             bool HS, int precision, int64_t b0, int64_t b1, int slots) {
+             bool HS, int precision, int64_t b0, int64_t b1, int slots, SourceView sv) {
     if (b1 < 0) b1 = L.n_direct_targets;
     if (!L.dir_level_split.empty()) {   // nested targets: one launch per level, ancestors first
         const std::vector<int64_t>& s = L.dir_level_split;
         for (size_t l = 0; l + 1 < s.size(); ++l) {
             const int64_t a = std::max(b0, s[l]), b = std::min(b1, s[l + 1]);
-            if (a < b) run_p2p_range(C, kernel, tt, st, L, tout, nt, PS, GS, HS, precision, a, b, slots);
+            if (a < b) run_p2p_range(C, kernel, tt, st, L, tout, nt, PS, GS, HS, precision, a, b, slots, sv);
         }
         return;
     }
-    run_p2p_range(C, kernel, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots);
+    run_p2p_range(C, kernel, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv);
 }
 
 static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
-                          bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots) {
+                          bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots, SourceView sv) {
     if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS, b0, b1); return; }
     switch (kernel) {
-        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
-        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
-        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
-        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots); break;
+        case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
+        case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
+        case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
+        default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
     }
 }
 

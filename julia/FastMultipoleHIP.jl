@@ -417,10 +417,60 @@ function hip_fmm_batch!(hip::HipEval, systems::AbstractVector; update_target_sys
     return stats, [(Int(pmax[i]), ok[i] != 0) for i in eachindex(counts)]
 end
 
+const POINT_KERNELS = (FMMHIP_POINT_SOURCE, FMMHIP_POINT_VORTEX, FMMHIP_POINT_DIPOLE, FMMHIP_VORTEX_GAUSSIAN)
+
+"""
+    hip_fmm_tuple!(hip, systems::Tuple; update_target_systems=true, kwargs...)
+
+`fmm!(systems)` for a tuple of point systems that share one tree (src/fmm.jl:807-884, src/tree.jl:3-122):
+every system is a target of all of them and a source with its own kernel. One leaf size and
+`SelfTuningTreeStop` lists. Returns the reference's 8-tuple as `hip_fmm!` does, with one
+`DerivativesSwitch` per system.
+"""
+function hip_fmm_tuple!(hip::HipEval, systems::Tuple; update_target_systems=true, kwargs...)
+    ks = [hip_kernel(s) for s in systems]
+    all(k -> k in POINT_KERNELS, ks) || throw(ArgumentError("a tuple of systems takes point kernels"))
+    kernels = Int32.(ks)
+    bufs = [source_buffer(s) for s in systems]
+    counts = Int64[size(b, 2) for b in bufs]
+    sbuf = zeros(Float64, 7, sum(counts))              # rows x, y, z, radius, s0, s1, s2
+    first = 1
+    for b in bufs
+        sbuf[1:size(b, 1), first:first+size(b, 2)-1] .= b
+        first += size(b, 2)
+    end
+    opt = FmmHipOptions(; kwargs...)
+    stats = FmmHipStats()
+    st = GC.@preserve sbuf counts kernels begin
+        check(ccall((:fmmhip_eval_set_tuple, libfmmhip), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}, Cint),
+                    hip.ev, length(counts), kernels, counts, sbuf, 0), hip.ctx)
+        check(ccall((:fmmhip_fmm, libfmmhip), Cint, (Ptr{Cvoid}, Ref{FmmHipOptions}, Ref{FmmHipStats}),
+                    hip.ev, opt, stats), hip.ctx)
+    end
+    sw = DerivativesSwitch(opt.scalar_potential != 0, opt.gradient != 0, opt.hessian != 0)
+    if update_target_systems
+        tbuf = Matrix{Float64}(undef, 16, sum(counts))
+        check(ccall((:fmmhip_eval_get_targets, libfmmhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), hip.ev, tbuf, 0), hip.ctx)
+        first = 1
+        for (s, n) in zip(systems, counts)
+            scatter_targets!(s, tbuf[:, first:first+n-1], sw)
+            first += n
+        end
+    end
+    error_success = st !== :tolerance_not_met && stats.error_success != 0
+    return (optargs_of(opt, stats; tune=false), hip, HipTreeView(hip, 0), HipTreeView(hip, 1), HipListView(hip, 0),
+            HipListView(hip, 1), ntuple(_ -> sw, length(systems)), error_success)
+end
+
 # fmm! with an optional `hip` keyword: GPU when routable, the reference's path otherwise
 function fmm!(target_systems::Tuple, source_systems::Tuple, hip::HipEval; kwargs...)
     if length(target_systems) == 1 && length(source_systems) == 1 && hip_kernel(source_systems[1]) !== nothing
         return hip_fmm!(hip, target_systems[1], source_systems[1]; kwargs...)
+    end
+    if target_systems === source_systems && all(s -> hip_kernel(s) in POINT_KERNELS, source_systems) &&
+       get(kwargs, :interaction_list_method, SelfTuningTreeStop()) isa SelfTuningTreeStop
+        return hip_fmm_tuple!(hip, source_systems; kwargs...)
     end
     return fmm!(target_systems, source_systems; kwargs...)
 end

@@ -57,6 +57,8 @@ def lib():
         L.orc_threads.restype = C.c_int
         L.orc_run_create.restype = C.c_void_p
         L.orc_run_create.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ic, _dp]
+        L.orc_run_create_tuple.restype = C.c_void_p
+        L.orc_run_create_tuple.argtypes = [C.c_int, _ic, _ip64, _dp, _dp, _ic, _dp]
         for f in ("orc_run_free", "orc_run_fmm", "orc_run_trees", "orc_run_lists"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = None
@@ -132,6 +134,30 @@ def data_per_body(kind):
     return 5 if kind == KIND_SOURCE else 7
 
 
+def _options(p, leaf_size_source, leaf_size_target, method, shrink_recenter, error_kind, relative, eps, BE,
+             scalar_potential, gradient, hessian, farfield, nearfield, self_induced, multipole_acceptance):
+    if leaf_size_target is None:
+        leaf_size_target = leaf_size_source
+    oi = np.array([p, leaf_size_source, leaf_size_target, method, int(shrink_recenter), error_kind, int(relative),
+                   int(BE), int(scalar_potential), int(gradient), int(hessian), int(farfield), int(nearfield),
+                   int(self_induced)], dtype=np.intc)
+    od = np.array([multipole_acceptance, eps], dtype=np.float64)
+    return oi, od
+
+
+def pack_tuple(kinds, datas):
+    """Concatenate point systems system-major into one 7-row buffer [x, y, z, radius, s0, s1, s2]
+    (a source system's strength in s0, zeros after it)."""
+    rows = []
+    for k, d in zip(kinds, datas):
+        d = np.asarray(d, dtype=np.float64)
+        assert k in (KIND_SOURCE, KIND_VORTEX, KIND_DIPOLE, KIND_VORTEX_GAUSS) and d.shape[1] == data_per_body(k)
+        r = np.zeros((len(d), 7))
+        r[:, :d.shape[1]] = d
+        rows.append(r)
+    return np.ascontiguousarray(np.concatenate(rows, axis=0))
+
+
 class Run:
     """One oracle FMM evaluation (trees, lists, passes) on one target and one source system.
 
@@ -152,13 +178,33 @@ class Run:
         self.nt, self.ns = len(t), len(s)
         if leaf_size_target is None:
             leaf_size_target = leaf_size_source
-        oi = np.array([p, leaf_size_source, leaf_size_target, method, int(shrink_recenter), error_kind, int(relative),
-                       int(BE), int(scalar_potential), int(gradient), int(hessian), int(farfield), int(nearfield),
-                       int(self_induced)], dtype=np.intc)
-        od = np.array([multipole_acceptance, eps], dtype=np.float64)
+        oi, od = _options(p, leaf_size_source, leaf_size_target, method, shrink_recenter, error_kind, relative, eps, BE,
+                          scalar_potential, gradient, hessian, farfield, nearfield, self_induced, multipole_acceptance)
         self._h = L.orc_run_create(kind, self.nt, np.ascontiguousarray(t.ravel()), self.ns,
                                    np.ascontiguousarray(s.ravel()), oi, od)
         self.p = p
+
+    @classmethod
+    def tuple(cls, kinds, datas, p=5, leaf_size_source=20, leaf_size_target=None, method=METHOD_SELFTUNING_TREESTOP,
+              shrink_recenter=True, error_kind=ERR_NONE, relative=False, eps=0.0, BE=True, scalar_potential=False,
+              gradient=True, hessian=False, farfield=True, nearfield=True, self_induced=True, multipole_acceptance=0.4):
+        """fmm! on a tuple of point systems (each system both target and source): one tree over all bodies,
+        each source system's nearfield and P2M with its own kernel. Targets come back concatenated in the
+        order of `datas`."""
+        L = lib()
+        self = cls.__new__(cls)
+        s = pack_tuple(kinds, datas)
+        counts = np.array([len(d) for d in datas], dtype=np.int64)
+        t = np.zeros((len(s), 16))
+        t[:, :3] = s[:, :3]
+        self.kind = KIND_VORTEX if any(k in (KIND_VORTEX, KIND_VORTEX_GAUSS) for k in kinds) else KIND_DIPOLE
+        self.nt = self.ns = len(s)
+        oi, od = _options(p, leaf_size_source, leaf_size_target, method, shrink_recenter, error_kind, relative, eps, BE,
+                          scalar_potential, gradient, hessian, farfield, nearfield, self_induced, multipole_acceptance)
+        self._h = L.orc_run_create_tuple(len(kinds), np.ascontiguousarray(kinds, dtype=np.intc), counts,
+                                         np.ascontiguousarray(t.ravel()), np.ascontiguousarray(s.ravel()), oi, od)
+        self.p = p
+        return self
 
     def __del__(self):
         if getattr(self, "_h", None):

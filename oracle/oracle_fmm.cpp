@@ -713,6 +713,11 @@ struct Options {
 
 struct Run {
     int kind;
+    // several systems in one tree (fmm! on a tuple, src/fmm.jl:3-40 and src/fmm.jl:331-345): the kernel of
+    // each system and its original index range off[s] .. off[s + 1] - 1 of the concatenated 7-row buffer;
+    // empty for one system
+    std::vector<int> kinds;
+    std::vector<int64_t> off;
     int64_t nt, ns;
     std::vector<double> tfull, sfull;   // original order: 16 x nt, dpb x ns
     Options opt;
@@ -740,6 +745,7 @@ static void fmm_run(Run& R, bool estimate_pass);
 static void estimate_influence(Run& R) {
     Run E;
     E.kind = R.kind; E.nt = R.nt; E.ns = R.ns; E.tfull = R.tfull; E.sfull = R.sfull;
+    E.kinds = R.kinds; E.off = R.off;
     E.opt = R.opt;
     E.opt.PS = true; E.opt.GS = true; E.opt.HS = false;
     E.opt.ls_source = 5; E.opt.ls_target = 5;
@@ -765,6 +771,25 @@ static void estimate_influence(Run& R) {
         for (int64_t j = b.first; j < b.first + b.n_bodies; ++j) mi = std::max(mi, infl[j]);
         b.max_influence = mi;
     }
+}
+
+// bodies of system s in branch b of a source tree: the partition is stable, so within a leaf the bodies
+// keep their original order and each system's bodies are one sub-range (bodies_index[i_system])
+static void system_range(const Run& R, const OBranch& b, size_t s, int64_t& first, int64_t& n) {
+    const int64_t* si = R.st.sort_index.data();
+    const int64_t* f = std::lower_bound(si + b.first, si + b.first + b.n_bodies, R.off[s]);
+    const int64_t* l = std::lower_bound(si + b.first, si + b.first + b.n_bodies, R.off[s + 1]);
+    first = f - si;
+    n = l - f;
+}
+
+// P2M of one body (src/bodytomultipole.jl:19-26)
+static void p2m_body(int kind, Exp M, Exp harm, const double* x, const double* center, int p) {
+    double dx[3] = {x[0] - center[0], x[1] - center[1], x[2] - center[2]};
+    if (is_panel(kind)) p2m_panel(kind, M, harm, x, center, p);
+    else if (kind == KIND_SOURCE) p2m_point_source(M, harm, dx, x[4], p);
+    else if (kind == KIND_DIPOLE) p2m_point_dipole(M, harm, dx, x + 4, p);
+    else p2m_point_vortex(M, harm, dx, x + 4, p);
 }
 
 static void fmm_run(Run& R, bool estimate_pass) {
@@ -801,7 +826,7 @@ static void fmm_run(Run& R, bool estimate_pass) {
         return r;
     };
     // nearfield (src/fmm.jl:3-40)
-    {
+    if (R.kinds.empty()) {
         const std::vector<size_t> rd = runs_of(R.direct);
         const int64_t nr = (int64_t)rd.size() - 1;
 #pragma omp parallel for schedule(dynamic, 1)
@@ -811,6 +836,21 @@ static void fmm_run(Run& R, bool estimate_pass) {
                 const OBranch& s = R.st.branches[pr[1]];
                 const OBranch& t = R.tt.branches[pr[0]];
                 user_direct(R.kind, R.tt.buffer.data(), t.first, t.n_bodies, R.st.buffer.data(), dpb, s.first, s.n_bodies,
+                            o.PS, o.GS, o.HS);
+            }
+    }
+    // several systems: each source system in turn over the whole direct list, with its own kernel
+    for (size_t is = 0; is < R.kinds.size(); ++is) {
+        const std::vector<size_t> rd = runs_of(R.direct);
+        const int64_t nr = (int64_t)rd.size() - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t g = 0; g < nr; ++g)
+            for (size_t k = rd[g]; k < rd[g + 1]; ++k) {
+                const auto& pr = R.direct[k];
+                const OBranch& t = R.tt.branches[pr[0]];
+                int64_t s0, sn;
+                system_range(R, R.st.branches[pr[1]], is, s0, sn);
+                user_direct(R.kinds[is], R.tt.buffer.data(), t.first, t.n_bodies, R.st.buffer.data(), dpb, s0, sn,
                             o.PS, o.GS, o.HS);
             }
     }
@@ -824,13 +864,14 @@ static void fmm_run(Run& R, bool estimate_pass) {
             const int64_t il = R.st.leaf_index[k];
             const OBranch& b = R.st.branches[il];
             Exp M{&R.st.expansions[(size_t)il * 4 * Hn]};
-            for (int64_t i = b.first; i < b.first + b.n_bodies; ++i) {
-                const double* x = &R.st.buffer[(size_t)i * dpb];
-                double dx[3] = {x[0] - b.source_center[0], x[1] - b.source_center[1], x[2] - b.source_center[2]};
-                if (is_panel(R.kind)) p2m_panel(R.kind, M, Exp{harm.data()}, x, b.source_center, p);
-                else if (R.kind == KIND_SOURCE) p2m_point_source(M, Exp{harm.data()}, dx, x[4], p);
-                else if (R.kind == KIND_DIPOLE) p2m_point_dipole(M, Exp{harm.data()}, dx, x + 4, p);
-                else p2m_point_vortex(M, Exp{harm.data()}, dx, x + 4, p);
+            if (R.kinds.empty())
+                for (int64_t i = b.first; i < b.first + b.n_bodies; ++i)
+                    p2m_body(R.kind, M, Exp{harm.data()}, &R.st.buffer[(size_t)i * dpb], b.source_center, p);
+            for (size_t is = 0; is < R.kinds.size(); ++is) {   // systems in order (src/fmm.jl:331-345)
+                int64_t s0, sn;
+                system_range(R, b, is, s0, sn);
+                for (int64_t i = s0; i < s0 + sn; ++i)
+                    p2m_body(R.kinds[is], M, Exp{harm.data()}, &R.st.buffer[(size_t)i * dpb], b.source_center, p);
             }
         }
     }
@@ -1238,6 +1279,24 @@ void* orc_run_create(int kind, int64_t nt, const double* tfull, int64_t ns, cons
     o.PS = oi[8] != 0; o.GS = oi[9] != 0; o.HS = oi[10] != 0;
     o.ff = oi[11] != 0; o.nf = oi[12] != 0; o.si = oi[13] != 0;
     o.theta = od[0]; o.em.eps = od[1];
+    return R;
+}
+
+// several systems in one tree: n_sys point kernels (0 source, 1 vortex, 2 dipole, 3 Gaussian vortex) with
+// counts[s] bodies each, concatenated system-major in sfull (7 x sum(counts): x, y, z, radius, s0, s1, s2);
+// the targets are the same bodies (tfull 16 x sum(counts)). The tree's kernel tag is a vortex kernel when
+// any system has a vector potential (src/compatibility.jl:292-298), else a dipole kernel (7 rows).
+void* orc_run_create_tuple(int n_sys, const int* kinds, const int64_t* counts, const double* tfull, const double* sfull,
+                           const int* oi, const double* od) {
+    std::vector<int64_t> off(n_sys + 1, 0);
+    bool lh = false;
+    for (int s = 0; s < n_sys; ++s) {
+        off[s + 1] = off[s] + counts[s];
+        lh = lh || has_vector_potential(kinds[s]);
+    }
+    Run* R = (Run*)orc_run_create(lh ? KIND_VORTEX : KIND_DIPOLE, off[n_sys], tfull, off[n_sys], sfull, oi, od);
+    R->kinds.assign(kinds, kinds + n_sys);
+    R->off = off;
     return R;
 }
 
