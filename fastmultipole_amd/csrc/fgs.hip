@@ -34,13 +34,16 @@
 
 namespace fmmhip {
 
-static constexpr int FGS_MAX_LEAF = 101;   // self blocks are inverted in LDS: 2 n^2 doubles <= 160 KiB
+static constexpr int FGS_LDS_LEAF = 101;    // self blocks of up to this many bodies are inverted in LDS (2 n^2 doubles <= 160 KiB)
+static constexpr int FGS_MAX_LEAF = 4096;   // a sweep keeps a leaf's right-hand side and strengths in LDS (2 n doubles <= 64 KiB)
+static constexpr int FGS_STAGE = 101;       // panels staged in LDS at a time by the influence-block kernel
 static constexpr double FGS_INV4PI = 1.0 / (4.0 * 3.141592653589793);
 
 struct FgsState {
     int influence = 0;
     int p = 0;
     int64_t nl = 0, rows = 0;
+    int max_leaf = 0;              // bodies of the largest leaf
     Lists full;                    // direct list with self pairs, for the final nearfield
     DBuf<int64_t> leaf_start;      // nl + 1: leaf-order offsets (strengths_by_leaf)
     DBuf<int64_t> lpos;            // sorted body -> leaf-order position
@@ -150,47 +153,56 @@ __global__ __launch_bounds__(256) void k_fgs_blocks(const Branch* __restrict__ b
                                                     const int64_t* __restrict__ off, const int64_t* __restrict__ leaf_row0,
                                                     const int64_t* __restrict__ row_body, const double* __restrict__ nrm,
                                                     int influence, double* __restrict__ out) {
-    __shared__ PanelStage<(NV > 0 ? NV : 3)> tile[FGS_MAX_LEAF];
+    __shared__ PanelStage<(NV > 0 ? NV : 3)> tile[FGS_STAGE];
     const Branch& b = br[leaf_index[blockIdx.x]];
     const int64_t n = b.n_bodies;
     const int64_t rows = SELF ? n : leaf_row0[blockIdx.x + 1] - leaf_row0[blockIdx.x];
     if (rows == 0) return;
-    if (NV > 0) {
-        for (int64_t c = threadIdx.x; c < n; c += blockDim.x) {
-            stage_panel<(NV > 0 ? NV : 3)>(x, y, z, srows, ns, b.first + c, tile[c]);
-            tile[c].strength = 1.0;
-        }
-        __syncthreads();
-    }
     double* M = out + off[blockIdx.x];
-    for (int64_t e = threadIdx.x; e < rows * n; e += blockDim.x) {
-        const int64_t r = e % rows, c = e / rows;
-        const int64_t it = SELF ? b.first + r : row_body[leaf_row0[blockIdx.x] + r];
-        const int64_t js = b.first + c;
-        double v = 0.0;
-        if (NV == 0) {   // point source at unit strength (test/gravitational.jl:80-104)
-            const double dx = x[it] - x[js], dy = y[it] - y[js], dz = z[it] - z[js];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 > 0) v = 1.0 / sqrt(r2) * FGS_INV4PI;
-        } else {
-            double a[13] = {0.0};
-            if (influence == 0) panel_pair<(NV > 0 ? NV : 3), false, true, false, false>(tile[c], x[it], y[it], z[it], a);
-            else panel_pair<(NV > 0 ? NV : 3), false, false, true, false>(tile[c], x[it], y[it], z[it], a);
-            v = influence == 0 ? a[0] : a[1] * nrm[3 * it] + a[2] * nrm[3 * it + 1] + a[3] * nrm[3 * it + 2];
+    // columns in chunks of FGS_STAGE (panels staged once per chunk); every entry is computed on its own
+    for (int64_t c0 = 0; c0 < n; c0 += FGS_STAGE) {
+        const int64_t nc = n - c0 < FGS_STAGE ? n - c0 : FGS_STAGE;
+        if (NV > 0) {
+            for (int64_t c = threadIdx.x; c < nc; c += blockDim.x) {
+                stage_panel<(NV > 0 ? NV : 3)>(x, y, z, srows, ns, b.first + c0 + c, tile[c]);
+                tile[c].strength = 1.0;
+            }
+            __syncthreads();
         }
-        M[c * rows + r] = v;
+        for (int64_t e = threadIdx.x; e < rows * nc; e += blockDim.x) {
+            const int64_t r = e % rows, cl = e / rows, c = c0 + cl;
+            const int64_t it = SELF ? b.first + r : row_body[leaf_row0[blockIdx.x] + r];
+            const int64_t js = b.first + c;
+            double v = 0.0;
+            if (NV == 0) {   // point source at unit strength (test/gravitational.jl:80-104)
+                const double dx = x[it] - x[js], dy = y[it] - y[js], dz = z[it] - z[js];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > 0) v = 1.0 / sqrt(r2) * FGS_INV4PI;
+            } else {
+                double a[13] = {0.0};
+                if (influence == 0) panel_pair<(NV > 0 ? NV : 3), false, true, false, false>(tile[cl], x[it], y[it], z[it], a);
+                else panel_pair<(NV > 0 ? NV : 3), false, false, true, false>(tile[cl], x[it], y[it], z[it], a);
+                v = influence == 0 ? a[0] : a[1] * nrm[3 * it] + a[2] * nrm[3 * it + 1] + a[3] * nrm[3 * it + 2];
+            }
+            M[c * rows + r] = v;
+        }
+        if (NV > 0) __syncthreads();   // the next chunk overwrites the stage
     }
 }
 
-// inverse of every self block: Gauss-Jordan with partial pivoting on [A | I] in LDS
+// inverse of every self block: Gauss-Jordan with partial pivoting on [A | I]. HBM = false: blocks of up to
+// FGS_LDS_LEAF bodies, worked in LDS; HBM = true: the larger blocks, worked in device memory (work holds
+// 2 n^2 doubles per block at twice the block's offset). Each launch skips the other's blocks; both run
+// the same steps in the same order.
+template <bool HBM>
 __global__ __launch_bounds__(256) void k_fgs_invert(const int64_t* __restrict__ leaf_start, const int64_t* __restrict__ off,
-                                                    const double* __restrict__ A, double* __restrict__ Ainv) {
+                                                    const double* __restrict__ A, double* __restrict__ Ainv, double* work) {
     extern __shared__ double sh[];
     __shared__ int piv;
     const int n = (int)(leaf_start[blockIdx.x + 1] - leaf_start[blockIdx.x]);
-    if (n == 0) return;
-    double* a = sh;           // row-major n x n
-    double* b = sh + n * n;   // row-major n x n
+    if (n == 0 || (n > FGS_LDS_LEAF) != HBM) return;
+    double* a = HBM ? work + 2 * off[blockIdx.x] : sh;   // row-major n x n
+    double* b = a + (int64_t)n * n;                      // row-major n x n
     const double* src = A + off[blockIdx.x];
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int r = e % n, c = e / n;   // column-major source
@@ -305,8 +317,10 @@ __global__ __launch_bounds__(256) void k_fgs_level(const int32_t* __restrict__ l
                                                    const int64_t* __restrict__ leaf_row0, const int64_t* __restrict__ ns_off,
                                                    const double* __restrict__ M, const int64_t* __restrict__ row_body,
                                                    const int64_t* __restrict__ lpos, double* __restrict__ s, double* rhs,
-                                                   double* __restrict__ old_infl) {
-    __shared__ double xs[FGS_MAX_LEAF], rk[FGS_MAX_LEAF];
+                                                   double* __restrict__ old_infl, int cap) {
+    extern __shared__ double xr[];   // 2 cap doubles (cap = the largest leaf): strengths, then the right-hand side
+    double* xs = xr;
+    double* rk = xr + cap;
     const int64_t k = leaves[blockIdx.x];
     const int64_t s0 = leaf_start[k];
     const int n = (int)(leaf_start[k + 1] - s0);
@@ -389,7 +403,13 @@ void fgs_setup(Eval& E, const fmmhip_options& o, int influence) {
     HIP_CHECK(hipStreamSynchronize(C.stream));
     int64_t max_n = 0;
     for (int64_t k = 0; k < nl; ++k) max_n = std::max(max_n, F.leaf_start_h[k + 1] - F.leaf_start_h[k]);
-    if (max_n > FGS_MAX_LEAF) throw Error(FMMHIP_EINVAL, "FastGaussSeidel supports leaves of at most 101 bodies");
+    if (max_n > FGS_MAX_LEAF) throw Error(FMMHIP_EINVAL, "FastGaussSeidel supports leaves of at most 4096 bodies");
+    F.max_leaf = (int)max_n;
+    {
+        const int lsh = (int)(sizeof(double) * 2 * (size_t)std::max<int64_t>(1, max_n));
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_fgs_level<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lsh));
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_fgs_level<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lsh));
+    }
     k_fgs_lpos<<<(unsigned)std::max<int64_t>(1, nl), 256, 0, C.stream>>>(st.branches.data(), st.leaf_index.data(), nl,
                                                                         F.leaf_start.data(), F.lpos.data());
     LAUNCH_CHECK();
@@ -477,10 +497,20 @@ void fgs_setup(Eval& E, const fmmhip_options& o, int influence) {
     if (nl) {
         build_blocks<true>(C, E.kernel, st, F, F.self_off.data(), F.self_A.data());
         build_blocks<false>(C, E.kernel, st, F, F.ns_off.data(), F.ns_M.data());
-        const size_t sh = sizeof(double) * 2 * (size_t)max_n * (size_t)max_n;
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_fgs_invert, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-        k_fgs_invert<<<(unsigned)nl, 256, sh, C.stream>>>(F.leaf_start.data(), F.self_off.data(), F.self_A.data(), F.self_inv.data());
+        const int64_t m = std::min<int64_t>(max_n, FGS_LDS_LEAF);
+        const size_t sh = sizeof(double) * 2 * (size_t)m * (size_t)m;
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_fgs_invert<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        k_fgs_invert<false><<<(unsigned)nl, 256, sh, C.stream>>>(F.leaf_start.data(), F.self_off.data(), F.self_A.data(),
+                                                                 F.self_inv.data(), nullptr);
         LAUNCH_CHECK();
+        if (max_n > FGS_LDS_LEAF) {   // larger blocks: the same elimination in device memory
+            DBuf<double> work;
+            work.resize(2 * (size_t)n_self);
+            k_fgs_invert<true><<<(unsigned)nl, 256, 0, C.stream>>>(F.leaf_start.data(), F.self_off.data(), F.self_A.data(),
+                                                                   F.self_inv.data(), work.data());
+            LAUNCH_CHECK();
+            HIP_CHECK(hipStreamSynchronize(C.stream));   // work is freed on return
+        }
     }
     F.old_infl.resize(std::max<int64_t>(1, F.rows));
     F.strengths.resize(std::max<int64_t>(1, ns)); F.rhs.resize(std::max<int64_t>(1, ns)); F.extra.resize(std::max<int64_t>(1, ns));
@@ -519,11 +549,13 @@ void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolera
                                                              F.strengths.data());
     LAUNCH_CHECK();
     HIP_CHECK(hipMemsetAsync(F.old_infl.data(), 0, F.old_infl.size() * sizeof(double), C.stream));
+    const int cap = std::max(1, F.max_leaf);
+    const size_t level_sh = sizeof(double) * 2 * (size_t)cap;
     if (nl)
-        k_fgs_level<false><<<(unsigned)nl, 256, 0, C.stream>>>(F.all_leaves.data(), F.leaf_start.data(), F.self_off.data(),
-                                                              F.self_inv.data(), F.leaf_row0.data(), F.ns_off.data(), F.ns_M.data(),
-                                                              F.row_body.data(), F.lpos.data(), F.strengths.data(), F.rhs.data(),
-                                                              F.old_infl.data());
+        k_fgs_level<false><<<(unsigned)nl, 256, level_sh, C.stream>>>(F.all_leaves.data(), F.leaf_start.data(), F.self_off.data(),
+                                                                     F.self_inv.data(), F.leaf_row0.data(), F.ns_off.data(),
+                                                                     F.ns_M.data(), F.row_body.data(), F.lpos.data(),
+                                                                     F.strengths.data(), F.rhs.data(), F.old_infl.data(), cap);
     LAUNCH_CHECK();
     std::vector<double> sq(std::max<int64_t>(1, nl));
     // one iteration: farfield of the current strengths, the right-hand side's farfield term, the residual
@@ -542,10 +574,11 @@ void fgs_solve(Eval& E, const double* ext_dev, int max_iterations, double tolera
     auto part_b = [&]() {
         for (size_t l = 0; l + 1 < F.level_ptr.size(); ++l) {
             const int64_t a = F.level_ptr[l], cnt = F.level_ptr[l + 1] - a;
-            k_fgs_level<true><<<(unsigned)cnt, 256, 0, C.stream>>>(F.level_leaves.data() + a, F.leaf_start.data(), F.self_off.data(),
-                                                                   F.self_inv.data(), F.leaf_row0.data(), F.ns_off.data(),
-                                                                   F.ns_M.data(), F.row_body.data(), F.lpos.data(),
-                                                                   F.strengths.data(), F.rhs.data(), F.old_infl.data());
+            k_fgs_level<true><<<(unsigned)cnt, 256, level_sh, C.stream>>>(F.level_leaves.data() + a, F.leaf_start.data(),
+                                                                          F.self_off.data(), F.self_inv.data(), F.leaf_row0.data(),
+                                                                          F.ns_off.data(), F.ns_M.data(), F.row_body.data(),
+                                                                          F.lpos.data(), F.strengths.data(), F.rhs.data(),
+                                                                          F.old_infl.data(), cap);
         }
         LAUNCH_CHECK();
         k_fgs_to_sorted<<<grid_for(n, 256), 256, 0, C.stream>>>(F.lpos.data(), F.strengths.data(), n, st.strength.data());

@@ -225,7 +225,8 @@ enum { FMMHIP_INFLUENCE_POTENTIAL = 0, FMMHIP_INFLUENCE_NORMAL_GRADIENT = 1 };
 /* FastGaussSeidel(system; expansion_order, multipole_acceptance, leaf_size, shrink_recenter)
  * (src/solve.jl:538-645): trees, Barba lists, per-leaf self and non-self influence matrices in HBM.
  * Uses opt->expansion_order, leaf_size_source, multipole_acceptance, shrink_recenter; leaves hold at
- * most 101 bodies. */
+ * most 4096 bodies (FMMHIP_EINVAL otherwise). Self blocks of up to 101 bodies are inverted in LDS, larger
+ * ones in device memory. */
 fmmhip_status fmmhip_fgs_setup(fmmhip_eval* ev, const fmmhip_options* opt, int32_t influence);
 /* solve!(system, solver; max_iterations, tolerance) (src/solve.jl:797-946). external: the external
  * influence per body (the reference's target_influence_to_buffer! values), original order, host or

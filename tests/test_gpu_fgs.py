@@ -56,7 +56,19 @@ CASES = [
     ("quads-sphere-potential", None, lambda: W.sphere_panels(10, quads=True), dict(leaf_size=30, expansion_order=6)),
     ("tris-sphere-normal", None, lambda: W.sphere_panels(8, quads=False),
      dict(leaf_size=20, expansion_order=6, influence=_lib.INFLUENCE_NORMAL_GRADIENT)),
+    # leaves of more than 101 bodies: self blocks inverted in device memory, panels staged in chunks
+    ("quads-sphere-ls400", None, lambda: W.sphere_panels(20, quads=True), dict(leaf_size=400, expansion_order=6)),
+    ("tris-sphere-normal-ls350", None, lambda: W.sphere_panels(14, quads=False),
+     dict(leaf_size=350, expansion_order=6, influence=_lib.INFLUENCE_NORMAL_GRADIENT)),
 ]
+
+
+def test_fgs_large_leaf_cases_have_large_leaves():
+    for name, kind, gen, kw in CASES[3:]:
+        _, rows = _make(kind, gen)
+        og = O.GaussSeidel(_make(kind, gen)[0], rows, expansion_order=kw["expansion_order"], multipole_acceptance=0.5,
+                           leaf_size=kw["leaf_size"], influence=kw.get("influence", 0))
+        assert og.leaves()[:, 1].max() > 101, name
 
 
 def _make(kind, gen):
