@@ -265,10 +265,11 @@ class Evaluation:
     def set_tuple(self, kernels, source_buffers, on_device=False):
         """Several systems in one tree, each a target and a source of all of them (fmmhip_eval_set_tuple).
 
-        kernels: one point kernel per system; source_buffers: one (n_i, data_per_body) array per system, or
-        with on_device=True one device buffer of 7 x sum(n_i) rows [x, y, z, radius, s0, s1, s2] followed by
-        the list of counts."""
+        kernels: one point or panel kernel per system; source_buffers: one (n_i, data_per_body) array per
+        system, or with on_device=True one device buffer of R x sum(n_i) rows followed by the list of counts,
+        R the largest data_per_body of the kernels (each system's rows first, zeros after them)."""
         kernels = np.ascontiguousarray(kernels, dtype=np.int32)
+        R = max([7] + [L.data_per_body(int(k)) for k in kernels])
         if on_device:
             buf, counts = source_buffers
             counts = np.ascontiguousarray(counts, dtype=np.int64)
@@ -277,12 +278,14 @@ class Evaluation:
         else:
             counts = np.array([b.shape[0] for b in source_buffers], dtype=np.int64)
             rows = []
-            for b in source_buffers:
+            for k, b in zip(kernels, source_buffers):
                 b = np.asarray(b, dtype=np.float64)
-                r = np.zeros((b.shape[0], 7))
-                r[:, :min(7, b.shape[1])] = b[:, :7]
+                if b.ndim != 2 or b.shape[1] != L.data_per_body(int(k)):
+                    raise ValueError(f"a system of kernel {int(k)} takes {L.data_per_body(int(k))} rows per body")
+                r = np.zeros((b.shape[0], R))
+                r[:, :b.shape[1]] = b
                 rows.append(r)
-            sb = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros((0, 7)))
+            sb = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros((0, R)))
             self._keep = (sb, counts, kernels)
             sp = sb.ctypes.data
         n = int(counts.sum())
@@ -586,8 +589,8 @@ def _fmm_tuple(systems, leaf_size_source, leaf_size_target, expansion_order, mul
                shrink_recenter, interaction_list_method, scalar_potential, gradient, hessian, farfield, nearfield,
                self_induced, update_target_systems, precision, evaluation):
     """fmm!((sys1, sys2, ...)) (src/fmm.jl:807-884): one tree over the bodies of every system (src/tree.jl:3-122),
-    each system a target of all and a source with its own kernel. Point kernels, one leaf size, and
-    SelfTuningTreeStop lists (the default)."""
+    each system a target of all and a source with its own kernel (point or panel kernels). One leaf size,
+    and SelfTuningTreeStop lists (the default)."""
     if leaf_size_target != leaf_size_source:
         raise ValueError("a tuple of systems takes one leaf size")
     if interaction_list_method is None:

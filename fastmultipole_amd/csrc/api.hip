@@ -134,6 +134,9 @@ __global__ void k_sys_dir_counts(const int32_t* __restrict__ dir_s, int64_t n, c
 }
 
 static bool multi(const Eval& E) { return !E.sys_kind.empty(); }
+// rows per body of the source buffer, and strength rows of the source tree
+static int eval_rows(const Eval& E) { return multi(E) ? E.sys_rows : data_per_body(E.kernel); }
+static int eval_sdims(const Eval& E) { return multi(E) ? E.sys_sdims : strength_dims(E.kernel); }
 
 static void sys_branches(Eval& E) {
     if (!multi(E) || E.st.n_branches == 0) return;
@@ -223,20 +226,20 @@ struct Timer {
 
 static void eval_build_trees(Eval& E, const fmmhip_options& o) {
     Context& C = *E.ctx;
-    const int dpb = data_per_body(E.kernel);
+    const int dpb = eval_rows(E);
     const int p = o.expansion_order;
     E.tt.seg = E.seg;
     E.st.seg = E.seg;
     if (!E.t_xyz && o.leaf_size_target == o.leaf_size_source) {
         // the targets are the sources with one leaf size: the target tree is the source tree's copy
         build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
-                   strength_dims(E.kernel));
+                   eval_sdims(E));
         clone_target_tree(C, E.st, E.tt);
     } else {
         if (E.t_xyz) build_tree(C, E.tt, E.nt, E.t_xyz, 3, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
         else build_tree(C, E.tt, E.nt, E.s_buf, dpb, true, o.leaf_size_target, o.list_method, o.shrink_recenter != 0, p, 0);
         build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
-                   strength_dims(E.kernel));
+                   eval_sdims(E));
     }
     sys_branches(E);
     E.trees_built = true;
@@ -335,6 +338,8 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
     X.st.seg = E.seg;
     X.sys_kind = E.sys_kind;
     X.sys_off = E.sys_off;
+    X.sys_rows = E.sys_rows;
+    X.sys_sdims = E.sys_sdims;
     if (multi(E)) {
         X.sys_off_dev.resize(E.sys_off.size());
         HIP_CHECK(hipMemcpyAsync(X.sys_off_dev.data(), E.sys_off_dev.data(), E.sys_off.size() * sizeof(int64_t),
@@ -344,8 +349,8 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
     if (X.t_xyz) eval_build_trees(X, eo);
     else {   // the targets are the sources: one build gives both trees; the Morton keys and their
              // order are the main source tree's (same buffer), built on the evaluation's stream
-        build_tree(C, X.st, X.ns, X.s_buf, data_per_body(X.kernel), false, eo.leaf_size_source, eo.list_method, true,
-                   eo.expansion_order, strength_dims(X.kernel), E.keys_fresh ? tree_keys(*E.ctx, E.st) : nullptr);
+        build_tree(C, X.st, X.ns, X.s_buf, eval_rows(X), false, eo.leaf_size_source, eo.list_method, true,
+                   eo.expansion_order, eval_sdims(X), E.keys_fresh ? tree_keys(*E.ctx, E.st) : nullptr);
         clone_target_tree(C, X.st, X.tt);
         sys_branches(X);
         X.trees_built = true;
@@ -700,20 +705,24 @@ fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const in
         if (n_systems < 1 || !kernels || !counts) throw Error(FMMHIP_EINVAL, "a tuple needs at least one system");
         std::vector<int64_t> off(n_systems + 1, 0);
         bool lh = false;
+        int rows = 7;
         for (int64_t i = 0; i < n_systems; ++i) {
             const int k = kernels[i];
-            if (k != FMMHIP_POINT_SOURCE && k != FMMHIP_POINT_VORTEX && k != FMMHIP_POINT_DIPOLE && k != FMMHIP_VORTEX_GAUSSIAN)
-                throw Error(FMMHIP_EINVAL, "systems of a tuple take point kernels");
+            if (k < FMMHIP_POINT_SOURCE || k > FMMHIP_PANEL_DIPOLE_QUAD) throw Error(FMMHIP_EINVAL, "unknown kernel in a tuple");
             if (counts[i] < 0) throw Error(FMMHIP_EINVAL, "negative system size");
             lh = lh || has_vector_potential(k);
+            rows = std::max(rows, data_per_body(k));
             off[i + 1] = off[i] + counts[i];
         }
         const int64_t n = off[n_systems];
         if (n > 0 && !buffer) throw Error(FMMHIP_EINVAL, "missing system buffer");
         if (n >= (int64_t)1 << 31) throw Error(FMMHIP_EINVAL, "more than 2^31 bodies in one tuple");
-        // the tree's bodies carry 7 rows [x, y, z, radius, s0, s1, s2]; the kernel tag that sizes them, and the
-        // Lamb-Helmholtz terms if any system is a vortex system
+        // the tree's bodies carry the largest row count of the systems, [x, y, z, radius, strength rows...]
+        // (7 for point systems, 14 with triangles, 17 with quadrilaterals); the kernel tag switches the
+        // Lamb-Helmholtz terms on if any system is a vortex system
         E.kernel = lh ? FMMHIP_POINT_VORTEX : FMMHIP_POINT_DIPOLE;
+        E.sys_rows = rows;
+        E.sys_sdims = rows - 4;
         E.nt = E.ns = n;
         E.t_xyz = nullptr;
         E.seg.clear();
@@ -721,8 +730,8 @@ fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const in
         E.sys_ok.clear();
         if (on_device) E.s_buf = buffer;
         else {
-            E.s_buf_in.resize(std::max<int64_t>(1, 7 * n));
-            if (n) HIP_CHECK(hipMemcpyAsync(E.s_buf_in.data(), buffer, sizeof(double) * 7 * n, hipMemcpyHostToDevice, C->stream));
+            E.s_buf_in.resize(std::max<int64_t>(1, rows * n));
+            if (n) HIP_CHECK(hipMemcpyAsync(E.s_buf_in.data(), buffer, sizeof(double) * rows * n, hipMemcpyHostToDevice, C->stream));
             E.s_buf = E.s_buf_in.data();
         }
         E.sys_kind.assign(kernels, kernels + n_systems);

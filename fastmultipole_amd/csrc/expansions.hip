@@ -2361,7 +2361,7 @@ void run_p2m(Context& C, int kernel, Tree& st, int p, const Branch* sbr) {
     if (is_panel(kernel)) {
         const size_t sh = sizeof(double) * 8 * nh(p);
 #define FMMHIP_P2M_PANEL(K)                                                                                              \
-    k_p2m_panel<K><<<(unsigned)st.n_leaves, 64, sh, C.stream>>>(st.branches.data(), st.leaf_index.data(), st.strength.data(), \
+    k_p2m_panel<K><<<(unsigned)st.n_leaves, 64, sh, C.stream>>>(sbr, st.leaf_index.data(), st.strength.data(),             \
                                                                st.n, st.expansions.data(), p)
         switch (kernel) {
             case FMMHIP_PANEL_SOURCE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_TRI); break;

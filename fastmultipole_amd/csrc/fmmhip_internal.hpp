@@ -174,10 +174,12 @@ struct Eval {
     std::vector<int64_t> seg;                 // several independent systems (each its own target and source):
                                               // their body offsets; empty for one target and one source system
     DBuf<int64_t> seg_dev;                    // seg on the device
-    // several systems in one tree, each a target and a source of all (fmm!((s1, s2, ...))): their point
-    // kernels and body offsets (empty: one system of `kernel`)
+    // several systems in one tree, each a target and a source of all (fmm!((s1, s2, ...))): their kernels
+    // and body offsets (empty: one system of `kernel`), and the rows and strength rows of their shared
+    // buffer (the largest of the systems')
     std::vector<int32_t> sys_kind;
     std::vector<int64_t> sys_off;
+    int sys_rows = 0, sys_sdims = 0;
     DBuf<int64_t> sys_off_dev;
     DBuf<Branch> sys_branches;                // per system: the source tree's branches with its leaf ranges
     DBuf<int64_t> sys_dir_cum;                // per system: prefix of its counts over the direct list

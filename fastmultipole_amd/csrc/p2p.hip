@@ -554,7 +554,7 @@ static void launch_p2p(Context& C, const Tree& tt, const Tree& st, const Lists& 
 }
 
 void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
-                    bool GS, bool HS, int64_t b0, int64_t b1);
+                    bool GS, bool HS, int64_t b0, int64_t b1, SourceView sv);
 // targets b0 .. b1 - 1 of the direct list's CSR, none inside another
 static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
                           bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots, SourceView sv);
@@ -577,7 +577,7 @@ void run_p2p(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists
 
 static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt,
                           bool PS, bool GS, bool HS, int precision, int64_t b0, int64_t b1, int slots, SourceView sv) {
-    if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS, b0, b1); return; }
+    if (is_panel(kernel)) { run_p2p_panels(C, kernel, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); return; }
     switch (kernel) {
         case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
         case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
@@ -854,11 +854,13 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
 
 template <int NV, bool DIPOLE>
 static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
-                             bool GS, bool HS, int64_t b0, int64_t b1) {
+                             bool GS, bool HS, int64_t b0, int64_t b1, SourceView sv) {
     const int64_t nblk = b1 - b0;
     if (nblk <= 0) return;
+    const Branch* sbr = sv.branches ? sv.branches : st.branches.data();
+    const int64_t* cum = sv.dir_cum ? sv.dir_cum : L.dir_cum.data();
 #define PANEL_ARGS                                                                                                       \
-    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), L.dir_cum.data(), tt.branches.data(), st.branches.data(), \
+    L.dir_targets.data(), L.dir_offsets.data(), L.dir_s.data(), cum, tt.branches.data(), sbr,                            \
         tt.x.data(), tt.y.data(), tt.z.data(), st.x.data(), st.y.data(), st.z.data(), st.strength.data(), st.n, tout, nt, b0
     const int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);
     switch (code) {
@@ -876,12 +878,12 @@ static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const L
 }
 
 void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
-                    bool GS, bool HS, int64_t b0, int64_t b1) {
+                    bool GS, bool HS, int64_t b0, int64_t b1, SourceView sv) {
     switch (kernel) {
-        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
-        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
-        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
-        default: launch_p2p_panel<4, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1); break;
+        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        default: launch_p2p_panel<4, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
     }
 }
 

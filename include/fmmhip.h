@@ -150,13 +150,14 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
 fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_systems, const int64_t* counts,
                                      const double* source_buffer, int32_t on_device);
 /* Several systems in one tree, each a target and a source of all of them (fmm!((sys1, sys2, ...)),
- * src/fmm.jl:862-884 with the tuple trees of src/tree.jl:3-122): point kernels (FMMHIP_POINT_*), one
- * leaf size and SelfTuningTreeStop lists (with which the reference's per-system leaf test on equal leaf
- * sizes is the test on the branch's total count, so the tree is that of all bodies together). buffer:
- * 7 x sum(counts) column-major rows [x, y, z, radius, s0, s1, s2], the systems one after another in the
- * order of kernels (a point source's s1 and s2 are ignored). The nearfield runs once per source system
- * with its kernel; results follow the same concatenated order. Not combined with batches or
- * FastGaussSeidel. */
+ * src/fmm.jl:862-884 with the tuple trees of src/tree.jl:3-122): one leaf size and SelfTuningTreeStop lists
+ * (with which the reference's per-system leaf test on equal leaf sizes is the test on the branch's total
+ * count, so the tree is that of all bodies together). kernels: any FMMHIP_POINT_* or FMMHIP_PANEL_* kernel
+ * per system. buffer: R x sum(counts) column-major, R the largest data_per_body of the kernels (7 for point
+ * systems, 14 with triangles, 17 with quadrilaterals), each system's rows [x, y, z, radius, strength
+ * rows...] first and zeros after them, the systems one after another in the order of kernels. The
+ * nearfield and P2M run once per source system with its kernel; results follow the same concatenated
+ * order. Not combined with batches or FastGaussSeidel. */
 fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const int32_t* kernels, const int64_t* counts,
                                      const double* buffer, int32_t on_device);
 /* After an evaluation: per system of the batch, the expansion order and error_success its own fmm! call

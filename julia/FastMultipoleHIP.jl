@@ -417,23 +417,22 @@ function hip_fmm_batch!(hip::HipEval, systems::AbstractVector; update_target_sys
     return stats, [(Int(pmax[i]), ok[i] != 0) for i in eachindex(counts)]
 end
 
-const POINT_KERNELS = (FMMHIP_POINT_SOURCE, FMMHIP_POINT_VORTEX, FMMHIP_POINT_DIPOLE, FMMHIP_VORTEX_GAUSSIAN)
-
 """
     hip_fmm_tuple!(hip, systems::Tuple; update_target_systems=true, kwargs...)
 
-`fmm!(systems)` for a tuple of point systems that share one tree (src/fmm.jl:807-884, src/tree.jl:3-122):
-every system is a target of all of them and a source with its own kernel. One leaf size and
-`SelfTuningTreeStop` lists. Returns the reference's 8-tuple as `hip_fmm!` does, with one
-`DerivativesSwitch` per system.
+`fmm!(systems)` for a tuple of systems that share one tree, e.g. panels and vortex particles
+(src/fmm.jl:807-884, src/tree.jl:3-122): every system is a target of all of them and a source with its
+own kernel. One leaf size and `SelfTuningTreeStop` lists. Returns the reference's 8-tuple as `hip_fmm!`
+does, with one `DerivativesSwitch` per system.
 """
 function hip_fmm_tuple!(hip::HipEval, systems::Tuple; update_target_systems=true, kwargs...)
     ks = [hip_kernel(s) for s in systems]
-    all(k -> k in POINT_KERNELS, ks) || throw(ArgumentError("a tuple of systems takes point kernels"))
+    any(k -> k === nothing, ks) && throw(ArgumentError("every system of a tuple declares a hip_kernel"))
     kernels = Int32.(ks)
     bufs = [source_buffer(s) for s in systems]
     counts = Int64[size(b, 2) for b in bufs]
-    sbuf = zeros(Float64, 7, sum(counts))              # rows x, y, z, radius, s0, s1, s2
+    R = max(7, maximum(size(b, 1) for b in bufs))      # the largest data_per_body of the systems
+    sbuf = zeros(Float64, R, sum(counts))               # each system's rows first, zeros after them
     first = 1
     for b in bufs
         sbuf[1:size(b, 1), first:first+size(b, 2)-1] .= b
@@ -468,7 +467,7 @@ function fmm!(target_systems::Tuple, source_systems::Tuple, hip::HipEval; kwargs
     if length(target_systems) == 1 && length(source_systems) == 1 && hip_kernel(source_systems[1]) !== nothing
         return hip_fmm!(hip, target_systems[1], source_systems[1]; kwargs...)
     end
-    if target_systems === source_systems && all(s -> hip_kernel(s) in POINT_KERNELS, source_systems) &&
+    if target_systems === source_systems && all(s -> hip_kernel(s) !== nothing, source_systems) &&
        get(kwargs, :interaction_list_method, SelfTuningTreeStop()) isa SelfTuningTreeStop
         return hip_fmm_tuple!(hip, source_systems; kwargs...)
     end

@@ -146,13 +146,14 @@ def _options(p, leaf_size_source, leaf_size_target, method, shrink_recenter, err
 
 
 def pack_tuple(kinds, datas):
-    """Concatenate point systems system-major into one 7-row buffer [x, y, z, radius, s0, s1, s2]
-    (a source system's strength in s0, zeros after it)."""
+    """Concatenate systems system-major into one buffer of R rows per body, R the largest data_per_body of
+    the kinds and at least 7: each system's rows [x, y, z, radius, strength rows...] first, zeros after."""
+    R = max([7] + [data_per_body(k) for k in kinds])
     rows = []
     for k, d in zip(kinds, datas):
         d = np.asarray(d, dtype=np.float64)
-        assert k in (KIND_SOURCE, KIND_VORTEX, KIND_DIPOLE, KIND_VORTEX_GAUSS) and d.shape[1] == data_per_body(k)
-        r = np.zeros((len(d), 7))
+        assert d.shape[1] == data_per_body(k)
+        r = np.zeros((len(d), R))
         r[:, :d.shape[1]] = d
         rows.append(r)
     return np.ascontiguousarray(np.concatenate(rows, axis=0))
@@ -188,7 +189,7 @@ class Run:
     def tuple(cls, kinds, datas, p=5, leaf_size_source=20, leaf_size_target=None, method=METHOD_SELFTUNING_TREESTOP,
               shrink_recenter=True, error_kind=ERR_NONE, relative=False, eps=0.0, BE=True, scalar_potential=False,
               gradient=True, hessian=False, farfield=True, nearfield=True, self_induced=True, multipole_acceptance=0.4):
-        """fmm! on a tuple of point systems (each system both target and source): one tree over all bodies,
+        """fmm! on a tuple of systems (each system both target and source): one tree over all bodies,
         each source system's nearfield and P2M with its own kernel. Targets come back concatenated in the
         order of `datas`."""
         L = lib()

@@ -718,6 +718,7 @@ struct Run {
     // empty for one system
     std::vector<int> kinds;
     std::vector<int64_t> off;
+    int rows = 0;   // rows per body of a tuple's buffer: the largest data_per_body of its kinds
     int64_t nt, ns;
     std::vector<double> tfull, sfull;   // original order: 16 x nt, dpb x ns
     Options opt;
@@ -728,6 +729,8 @@ struct Run {
     std::vector<int32_t> m2l_P;   // expansion order chosen per M2L interaction
     std::vector<double> m2l_margin;   // last_m2l_margin of each M2L interaction
 };
+
+static int run_rows(const Run& R) { return R.kinds.empty() ? data_per_body(R.kind) : R.rows; }
 
 static BranchGeom geom(const OBranch& b) {
     BranchGeom g;
@@ -745,7 +748,7 @@ static void fmm_run(Run& R, bool estimate_pass);
 static void estimate_influence(Run& R) {
     Run E;
     E.kind = R.kind; E.nt = R.nt; E.ns = R.ns; E.tfull = R.tfull; E.sfull = R.sfull;
-    E.kinds = R.kinds; E.off = R.off;
+    E.kinds = R.kinds; E.off = R.off; E.rows = R.rows;
     E.opt = R.opt;
     E.opt.PS = true; E.opt.GS = true; E.opt.HS = false;
     E.opt.ls_source = 5; E.opt.ls_target = 5;
@@ -795,7 +798,7 @@ static void p2m_body(int kind, Exp M, Exp harm, const double* x, const double* c
 static void fmm_run(Run& R, bool estimate_pass) {
     const Options& o = R.opt;
     const bool LH = has_vector_potential(R.kind);
-    const int dpb = data_per_body(R.kind);
+    const int dpb = run_rows(R);
     build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p);
     build_tree(R.st, R.ns, dpb, R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p);
     R.m2l.clear(); R.direct.clear();
@@ -1282,21 +1285,27 @@ void* orc_run_create(int kind, int64_t nt, const double* tfull, int64_t ns, cons
     return R;
 }
 
-// several systems in one tree: n_sys point kernels (0 source, 1 vortex, 2 dipole, 3 Gaussian vortex) with
-// counts[s] bodies each, concatenated system-major in sfull (7 x sum(counts): x, y, z, radius, s0, s1, s2);
-// the targets are the same bodies (tfull 16 x sum(counts)). The tree's kernel tag is a vortex kernel when
-// any system has a vector potential (src/compatibility.jl:292-298), else a dipole kernel (7 rows).
+// several systems in one tree: n_sys kernels (0 source, 1 vortex, 2 dipole, 3 Gaussian vortex, 4..7 panels)
+// with counts[s] bodies each, concatenated system-major in sfull (rows x sum(counts), rows the largest
+// data_per_body of the kinds and at least 7; each system's rows first, zeros after them); the targets are
+// the same bodies (tfull 16 x sum(counts)). The Lamb-Helmholtz terms are on when any system has a vector
+// potential (src/compatibility.jl:292-298): the run's kernel tag is then a vortex kernel, else a dipole one.
 void* orc_run_create_tuple(int n_sys, const int* kinds, const int64_t* counts, const double* tfull, const double* sfull,
                            const int* oi, const double* od) {
     std::vector<int64_t> off(n_sys + 1, 0);
     bool lh = false;
+    int rows = 7;
     for (int s = 0; s < n_sys; ++s) {
         off[s + 1] = off[s] + counts[s];
         lh = lh || has_vector_potential(kinds[s]);
+        rows = std::max(rows, data_per_body(kinds[s]));
     }
-    Run* R = (Run*)orc_run_create(lh ? KIND_VORTEX : KIND_DIPOLE, off[n_sys], tfull, off[n_sys], sfull, oi, od);
+    Run* R = (Run*)orc_run_create(lh ? KIND_VORTEX : KIND_DIPOLE, off[n_sys], tfull, 0, sfull, oi, od);
+    R->ns = off[n_sys];
+    R->sfull.assign(sfull, sfull + (size_t)rows * R->ns);
     R->kinds.assign(kinds, kinds + n_sys);
     R->off = off;
+    R->rows = rows;
     return R;
 }
 
@@ -1312,7 +1321,7 @@ void orc_run_fmm(void* h) { fmm_run(*(Run*)h, false); }
 void orc_run_trees(void* h) {
     Run& R = *(Run*)h;
     build_tree(R.tt, R.nt, 16, R.tfull.data(), true, R.opt.ls_target, R.opt.method, R.opt.shrink, R.opt.p);
-    build_tree(R.st, R.ns, data_per_body(R.kind), R.sfull.data(), false, R.opt.ls_source, R.opt.method, R.opt.shrink, R.opt.p);
+    build_tree(R.st, R.ns, run_rows(R), R.sfull.data(), false, R.opt.ls_source, R.opt.method, R.opt.shrink, R.opt.p);
 }
 
 void orc_run_lists(void* h) {

@@ -20,7 +20,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 _ORC = {_lib.POINT_SOURCE: O.KIND_SOURCE, _lib.POINT_VORTEX: O.KIND_VORTEX, _lib.POINT_DIPOLE: O.KIND_DIPOLE,
-        _lib.VORTEX_GAUSSIAN: O.KIND_VORTEX_GAUSS}
+        _lib.VORTEX_GAUSSIAN: O.KIND_VORTEX_GAUSS, _lib.PANEL_SOURCE_TRI: O.KIND_PANEL_SOURCE_TRI,
+        _lib.PANEL_DIPOLE_TRI: O.KIND_PANEL_DIPOLE_TRI, _lib.PANEL_SOURCE_QUAD: O.KIND_PANEL_SOURCE_QUAD,
+        _lib.PANEL_DIPOLE_QUAD: O.KIND_PANEL_DIPOLE_QUAD}
 
 
 def _say(msg):
@@ -59,6 +61,16 @@ def _src(n, seed, shift):
     return d
 
 
+def _panels(quads):
+    def gen(n, seed, shift):   # panels tiling a sphere of radius 1.6 about the origin, about n of them
+        m = max(2, int(round((n / (6 if quads else 12)) ** 0.5)))
+        rows, _ = W.sphere_panels(m, quads=quads, seed=seed, radius=1.6)
+        rows = rows.copy()
+        rows[:, 4] *= 30.0
+        return rows
+    return gen
+
+
 def _ring(n, seed, shift):
     d = W.vortex_ring(n, seed=seed).copy()
     d[:, :3] += shift
@@ -73,6 +85,15 @@ CASES = [
      dict(expansion_order=8, leaf_size=24, multipole_acceptance=0.45, hessian=True)),
     ("source+dipole-p12-potential", [(_lib.POINT_SOURCE, _src, 7000), (_lib.POINT_DIPOLE, _dipoles, 2500)],
      dict(expansion_order=12, leaf_size=40, multipole_acceptance=0.5, scalar_potential=True)),
+    # the reference's motivating tuple: panels and vortex particles (17- and 14-row panel buffers, points padded)
+    ("quads+vortex-p10", [(_lib.PANEL_SOURCE_QUAD, _panels(True), 2400), (_lib.POINT_VORTEX, _ring, 6000)],
+     dict(expansion_order=10, leaf_size=24, multipole_acceptance=0.5, hessian=True)),
+    ("dipole-tris+gauss+source-p8", [(_lib.PANEL_DIPOLE_TRI, _panels(False), 2400), (_lib.VORTEX_GAUSSIAN, _gauss, 3000),
+                                     (_lib.POINT_SOURCE, _src, 2000)],
+     dict(expansion_order=8, leaf_size=20, multipole_acceptance=0.5, scalar_potential=True)),
+    ("dipole-quads+source-tris-p9", [(_lib.PANEL_DIPOLE_QUAD, _panels(True), 1500),
+                                     (_lib.PANEL_SOURCE_TRI, _panels(False), 1200)],
+     dict(expansion_order=9, leaf_size=16, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
 ]
 
 
@@ -175,7 +196,7 @@ def test_tuple_rejects_bad_input():
     rows = np.zeros((100, 7))
     rows[:, :3] = np.random.default_rng(3).random((100, 3))
     counts = np.array([60, 40], dtype=np.int64)
-    bad = np.array([_lib.POINT_SOURCE, _lib.PANEL_SOURCE_TRI], dtype=np.int32)
+    bad = np.array([_lib.POINT_SOURCE, 8], dtype=np.int32)   # no kernel 8
     assert lib.fmmhip_eval_set_tuple(ev._h, 2, bad.ctypes.data, counts.ctypes.data, rows.ctypes.data, 0) == _lib.FMMHIP_EINVAL
     good = np.array([_lib.POINT_SOURCE, _lib.POINT_VORTEX], dtype=np.int32)
     neg = np.array([60, -1], dtype=np.int64)

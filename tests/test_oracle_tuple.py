@@ -6,6 +6,8 @@ system's nearfield and P2M with its own kernel; src/fmm.jl:3-40, src/fmm.jl:331-
 * a tuple of a source, a vortex and a dipole system converges to the sum of the three direct sums over
   every body (a dropped system, a kernel applied to the wrong range or a wrong Lamb-Helmholtz switch fails
   this by orders of magnitude);
+* the same for quadrilateral panels (source and dipole) with vortex particles, the reference's motivating
+  case, whose buffer rows differ per system;
 * the error-controlled path with its influence estimate runs on the tuple and meets its tolerance.
 """
 import numpy as np
@@ -68,6 +70,51 @@ def test_mixed_tuple_converges_to_direct_sums(kinds, slots):
     for drop in range(len(kinds)):
         part = _direct_sum([k for i, k in enumerate(kinds) if i != drop], [d for i, d in enumerate(datas) if i != drop],
                            xyz, gradient=True)
+        rel = np.linalg.norm(part[:, 4:7] - ref[:, 4:7], axis=1) / np.linalg.norm(ref[:, 4:7], axis=1)
+        assert np.median(rel) > 0.05, (drop, np.median(rel))
+
+
+def _sphere_quads(m, rng):
+    """Source or dipole quadrilaterals tiling a cube's surface at distance 1 (rows of the panel kernels)."""
+    out = []
+    t = np.linspace(-1.0, 1.0, m + 1)
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            a, b = [k for k in range(3) if k != axis]
+            for i in range(m):
+                for j in range(m):
+                    v = np.zeros((4, 3))
+                    for q, (di, dj) in enumerate(((0, 0), (1, 0), (1, 1), (0, 1))):
+                        v[q, axis] = sgn
+                        v[q, a] = t[i + di]
+                        v[q, b] = t[j + dj]
+                    if sgn < 0:
+                        v = v[::-1]
+                    cen = v.mean(axis=0)
+                    rad = np.linalg.norm(v - cen, axis=1).max()
+                    out.append(np.concatenate([cen, [rad, rng.normal() / (6 * m * m)], v.ravel()]))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("panel", [O.KIND_PANEL_SOURCE_QUAD, O.KIND_PANEL_DIPOLE_QUAD])
+def test_panels_and_vortices_converge_to_direct_sums(panel):
+    """The reference's motivating tuple: panels and vortex particles in one tree (16-row panel buffer,
+    points padded)."""
+    rng = np.random.default_rng(24)
+    quads = _sphere_quads(12, rng)
+    quads[:, 4] *= 30.0   # a like share of the velocity for each system
+    vort = _system(O.KIND_VORTEX, 1500, rng, -0.5)
+    vort[:, :3] = rng.random((1500, 3)) * 1.4 - 0.7
+    kinds, datas = [panel, O.KIND_VORTEX], [quads, vort]
+    run = O.Run.tuple(kinds, datas, p=16, leaf_size_source=10, multipole_acceptance=0.5, gradient=True).fmm()
+    assert len(run.list(0)) > 0
+    got = run.targets()
+    xyz = np.concatenate([d[:, :3] for d in datas])
+    ref = _direct_sum(kinds, datas, xyz, gradient=True)
+    err = np.abs(got[:, 4:7] - ref[:, 4:7]).max() / np.abs(ref[:, 4:7]).max()
+    assert err < 1e-6, err
+    for drop in range(2):
+        part = O.direct(kinds[1 - drop], xyz, datas[1 - drop], gradient=True)
         rel = np.linalg.norm(part[:, 4:7] - ref[:, 4:7], axis=1) / np.linalg.norm(ref[:, 4:7], axis=1)
         assert np.median(rel) > 0.05, (drop, np.median(rel))
 
