@@ -27,8 +27,8 @@
 #include "fmmhip_internal.hpp"
 #include <cstdlib>
 #include "panels.hpp"
-#include <cstring>   // host memset for rocprim headers
-#include <rocprim/rocprim.hpp>
+#include "scan3.hpp"
+#include "radix_sort.hpp"
 #include <algorithm>
 #include <vector>
 
@@ -73,10 +73,7 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)std::ma
 
 template <class T, class O>
 static void excl_scan(Context& C, const T* in, O* out, int64_t n) {
-    size_t bytes = 0;
-    HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, O(0), (size_t)n, rocprim::plus<O>(), C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::exclusive_scan(C.temp.data(), bytes, in, out, O(0), (size_t)n, rocprim::plus<O>(), C.stream));
+    excl_scan3<T, O>(C, in, out, n);
 }
 
 // ---------------------------------------------------------------- set-up
@@ -420,12 +417,8 @@ void fgs_setup(Eval& E, const fmmhip_options& o, int influence) {
         DBuf<uint32_t> keys;
         keys.resize(np);
         HIP_CHECK(hipMemcpyAsync(keys.data(), E.lists.dfs_s.data(), sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, C.stream));
-        size_t bytes = 0;
-        HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys.data(), F.by_src_s.data(), (const int32_t*)E.lists.dfs_t.data(),
-                                            F.by_src_t.data(), (size_t)np, 0, 32, C.stream));
-        C.temp.resize(bytes + 16);
-        HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, keys.data(), F.by_src_s.data(), (const int32_t*)E.lists.dfs_t.data(),
-                                            F.by_src_t.data(), (size_t)np, 0, 32, C.stream));
+        radix_sort<uint32_t>(C, keys.data(), F.by_src_s.data(), (const uint32_t*)E.lists.dfs_t.data(),
+                             (uint32_t*)F.by_src_t.data(), np, 32);
         HIP_CHECK(hipStreamSynchronize(C.stream));
     }
     F.seg.resize(2 * std::max<int64_t>(1, nl));

@@ -130,6 +130,7 @@ struct Context {
     DBuf<double> direct_part;   // chunk sums of the split direct! form
     int n_cu = 0;               // compute units of the device (0 until first needed)
     DBuf<unsigned char> temp;   // scratch for sorts/scans
+    DBuf<unsigned char> sort_work;   // radix sort: digit counts, offsets, keys and values between passes
     void* pinned = nullptr;     // 64 bytes of pinned host memory for scalar read-backs
     // Launch sequences without host synchronisation may be captured in graphs and replayed. Off for
     // contexts that share the device with other threads (the request server's workers): a

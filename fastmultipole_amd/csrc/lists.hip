@@ -14,8 +14,7 @@
 // -ffp-contract=off (the MAC must match).
 #include "fmmhip_internal.hpp"
 #include "scan3.hpp"
-#include <cstring>   // host memset for rocprim headers
-#include <rocprim/rocprim.hpp>
+#include "radix_sort.hpp"
 #include <algorithm>
 #include <memory>
 #include <vector>
@@ -225,12 +224,7 @@ static void finish_list(Context& C, ListScratch& S, uint32_t* kt, int32_t* vs, i
     n_targets = 0;
     if (n_out == 0) { offsets.resize(1); targets.resize(1); HIP_CHECK(hipMemsetAsync(offsets.data(), 0, 8, C.stream)); return; }
     S.kt_sorted.resize(n_out);
-    size_t bytes = 0;
-    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kt, S.kt_sorted.data(), vs, out_s.data(), (size_t)n_out, 0, target_bits,
-                                        C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, kt, S.kt_sorted.data(), vs, out_s.data(), (size_t)n_out, 0,
-                                        target_bits, C.stream));
+    radix_sort<uint32_t>(C, kt, S.kt_sorted.data(), (const uint32_t*)vs, (uint32_t*)out_s.data(), n_out, target_bits);
     S.flag.resize(n_out + 1);
     S.rank.resize(n_out + 1);
     k_target_starts<<<grid_for(n_out, 256), 256, 0, C.stream>>>(S.kt_sorted.data(), n_out, S.flag.data());

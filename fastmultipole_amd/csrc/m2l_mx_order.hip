@@ -6,8 +6,7 @@
 // stable, so the result is deterministic) cut the modelled batch work by 12 %. The list itself keeps
 // the reference's order; only the launch reads the permutation.
 #include "expansions_common.hpp"
-#include <cstring>   // host memset for rocprim headers
-#include <rocprim/rocprim.hpp>
+#include "radix_sort.hpp"
 #include <algorithm>
 
 namespace fmmhip {
@@ -37,15 +36,9 @@ const int32_t* m2l_batch_order(Context& C, Lists& Ls, const Tree& tt, const Tree
                                                                        st.branches.data(), Ls.m2l_key.data(),
                                                                        Ls.m2l_iota.data());
     LAUNCH_CHECK();
-    size_t bytes = 0;
-    const int64_t* off = Ls.m2l_offsets.data();
-    HIP_CHECK(rocprim::segmented_radix_sort_pairs_desc(nullptr, bytes, Ls.m2l_key.data(), Ls.m2l_key_sorted.data(),
-                                                       Ls.m2l_iota.data(), Ls.m2l_perm.data(), (unsigned)n,
-                                                       (unsigned)Ls.n_m2l_targets, off, off + 1, 0, 32, C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::segmented_radix_sort_pairs_desc(C.temp.data(), bytes, Ls.m2l_key.data(), Ls.m2l_key_sorted.data(),
-                                                       Ls.m2l_iota.data(), Ls.m2l_perm.data(), (unsigned)n,
-                                                       (unsigned)Ls.n_m2l_targets, off, off + 1, 0, 32, C.stream));
+    // descending keys within each target box's range of the list (the values are list positions)
+    radix_sort<float>(C, Ls.m2l_key.data(), Ls.m2l_key_sorted.data(), (const uint32_t*)Ls.m2l_iota.data(),
+                      (uint32_t*)Ls.m2l_perm.data(), n, 32, true, Ls.m2l_offsets.data(), (int)Ls.n_m2l_targets);
     return Ls.m2l_perm.data();
 }
 

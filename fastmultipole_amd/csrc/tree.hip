@@ -22,8 +22,7 @@
 // reference's IEEE double arithmetic bit for bit.
 #include "fmmhip_internal.hpp"
 #include "scan3.hpp"
-#include <cstring>   // host memset for rocprim headers
-#include <rocprim/rocprim.hpp>
+#include "radix_sort.hpp"
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -349,27 +348,14 @@ static void exclusive_scan_i32(Context& C, const T* in, T* out, int64_t n) {
 }
 
 // stable sort by a 63-bit key, within each system's range when seg (device, n_seg + 1 offsets) is given
+// (the values are original body indices, so each lies in its system's range)
 static void sort_pairs_u64(Context& C, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, int64_t n,
                            const int64_t* seg = nullptr, int n_seg = 1) {
-    size_t bytes = 0;
-    if (seg && n_seg > 1) {
-        HIP_CHECK(rocprim::segmented_radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (unsigned)n, (unsigned)n_seg, seg,
-                                                      seg + 1, 0, 63, C.stream));
-        C.temp.resize(bytes + 16);
-        HIP_CHECK(rocprim::segmented_radix_sort_pairs(C.temp.data(), bytes, kin, kout, vin, vout, (unsigned)n, (unsigned)n_seg,
-                                                      seg, seg + 1, 0, 63, C.stream));
-        return;
-    }
-    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, 63, C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::radix_sort_pairs(C.temp.data(), bytes, kin, kout, vin, vout, (size_t)n, 0, 63, C.stream));
+    radix_sort<uint64_t>(C, kin, kout, vin, vout, n, 63, false, seg && n_seg > 1 ? seg : nullptr, n_seg);
 }
 
 static void sort_keys_u64(Context& C, const uint64_t* kin, uint64_t* kout, int64_t n) {
-    size_t bytes = 0;
-    HIP_CHECK(rocprim::radix_sort_keys(nullptr, bytes, kin, kout, (size_t)n, 0, 64, C.stream));
-    C.temp.resize(bytes + 16);
-    HIP_CHECK(rocprim::radix_sort_keys(C.temp.data(), bytes, kin, kout, (size_t)n, 0, 64, C.stream));
+    radix_sort<uint64_t>(C, kin, kout, nullptr, nullptr, n, 64);
 }
 
 struct TreeScratch {
