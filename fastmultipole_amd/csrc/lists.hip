@@ -42,16 +42,10 @@ __device__ inline bool mac_accept(const Branch& t, const Branch& s, double mac) 
     return sep2 * mac * mac > sr * sr;
 }
 
-// one step of build_interaction_lists! for every pair of a level (:23-166):
-// decision and number of children (0 unless split); entry n gets count 0 so
-// that the exclusive scan ends with the size of the next level
-__global__ void k_list_decide(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, int64_t n,
-                              const Branch* __restrict__ tb, const Branch* __restrict__ sb, ListParams P,
-                              int8_t* __restrict__ dec, int32_t* __restrict__ cnt) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i == n) cnt[i] = 0;
-    if (i >= n) return;
-    const int32_t it = ft[i], js = fs[i];
+// one step of build_interaction_lists! for one pair (:23-166): its decision and number of children (0
+// unless split)
+__device__ inline int32_t decide_pair(int32_t it, int32_t js, const Branch* __restrict__ tb, const Branch* __restrict__ sb,
+                                      const ListParams& P, int32_t& n_children) {
     const Branch& t = tb[it];
     const Branch& s = sb[js];
     const bool both_leaves = (s.n_branches == 0 && t.n_branches == 0);
@@ -75,24 +69,63 @@ __global__ void k_list_decide(const int32_t* __restrict__ ft, const int32_t* __r
             d = split_t ? D_SPLIT_T : D_SPLIT_S;
         }
     }
-    dec[i] = (int8_t)d;
-    cnt[i] = (d == D_SPLIT_T) ? (int32_t)t.n_branches : (d == D_SPLIT_S) ? (int32_t)s.n_branches : 0;
+    n_children = (d == D_SPLIT_T) ? (int32_t)t.n_branches : (d == D_SPLIT_S) ? (int32_t)s.n_branches : 0;
+    return d;
 }
 
-// children of the split pairs, in branch order, at their scanned positions
-__global__ void k_list_expand(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs, int64_t n,
-                              const Branch* __restrict__ tb, const Branch* __restrict__ sb, const int8_t* __restrict__ dec,
-                              const int64_t* __restrict__ pos, int32_t* __restrict__ ot, int32_t* __restrict__ os) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t o = pos[i];
-    const int32_t it = ft[i], js = fs[i];
-    if (dec[i] == D_SPLIT_T) {
-        const Branch& t = tb[it];
-        for (int64_t k = 0; k < t.n_branches; ++k) { ot[o + k] = (int32_t)(t.child_first + k); os[o + k] = js; }
-    } else if (dec[i] == D_SPLIT_S) {
-        const Branch& s = sb[js];
-        for (int64_t k = 0; k < s.n_branches; ++k) { ot[o + k] = it; os[o + k] = (int32_t)(s.child_first + k); }
+// The pairs of a level in tiles of scan3's size (SCAN_BLOCK threads x SCAN_ITEMS pairs, pair i of a tile at
+// round i / SCAN_BLOCK): each pair's decision and child count, and each tile's total count. Entry n counts
+// 0, so that the exclusive scan of the counts ends with the size of the next level.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_list_decide(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs,
+                                                            int64_t n, const Branch* __restrict__ tb,
+                                                            const Branch* __restrict__ sb, ListParams P,
+                                                            int8_t* __restrict__ dec, int32_t* __restrict__ cnt,
+                                                            int64_t* __restrict__ tile_total) {
+    __shared__ int64_t sh[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
+    int64_t sum = 0;
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const int64_t i = base + (int64_t)r * SCAN_BLOCK + threadIdx.x;
+        if (i < n) {
+            int32_t c;
+            dec[i] = (int8_t)decide_pair(ft[i], fs[i], tb, sb, P, c);
+            cnt[i] = c;
+            sum += c;
+        } else if (i == n) cnt[i] = 0;
+    }
+    int64_t total;
+    (void)block_excl_scan<int64_t>(sum, total, sh);
+    if (threadIdx.x == 0) tile_total[blockIdx.x] = total;
+}
+
+// With the tiles' offsets (the exclusive scan of their totals): each pair's position in the next level
+// (pos, n + 1 entries, the scan of the counts), and the children of the split pairs, in branch order, there.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_list_expand(const int32_t* __restrict__ ft, const int32_t* __restrict__ fs,
+                                                            int64_t n, const Branch* __restrict__ tb,
+                                                            const Branch* __restrict__ sb, const int8_t* __restrict__ dec,
+                                                            const int32_t* __restrict__ cnt,
+                                                            const int64_t* __restrict__ tile_off, int64_t* __restrict__ pos,
+                                                            int32_t* __restrict__ ot, int32_t* __restrict__ os) {
+    __shared__ int64_t sh[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
+    int64_t carry = tile_off[blockIdx.x];
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const int64_t i = base + (int64_t)r * SCAN_BLOCK + threadIdx.x;
+        const int64_t v = i <= n ? (int64_t)cnt[i] : 0;
+        int64_t total;
+        const int64_t o = carry + block_excl_scan<int64_t>(v, total, sh);
+        carry += total;
+        if (i > n) continue;
+        pos[i] = o;
+        if (i == n || v == 0) continue;
+        const int32_t it = ft[i], js = fs[i];
+        if (dec[i] == D_SPLIT_T) {
+            const Branch& t = tb[it];
+            for (int64_t k = 0; k < t.n_branches; ++k) { ot[o + k] = (int32_t)(t.child_first + k); os[o + k] = js; }
+        } else {
+            const Branch& s = sb[js];
+            for (int64_t k = 0; k < s.n_branches; ++k) { ot[o + k] = it; os[o + k] = (int32_t)(s.child_first + k); }
+        }
     }
 }
 
@@ -197,7 +230,7 @@ struct ListLevel {
 struct ListScratch {
     std::vector<std::unique_ptr<ListLevel>> levels;
     DBuf<int32_t> cnt, flag, rank;
-    DBuf<int64_t> pos;
+    DBuf<int64_t> pos, tile;
     DBuf<unsigned long long> counter;
     DBuf<uint32_t> kt_m2l, kt_dir, kt_sorted;
     DBuf<int32_t> vs_m2l, vs_dir;
@@ -267,17 +300,23 @@ void build_lists(Context& C, const Tree& tt, const Tree& st, Lists& L, int64_t l
         A.dec.resize(n);
         A.pos.resize(n + 1);
         S.cnt.resize(n + 1);
-        k_list_decide<<<grid_for(n + 1, 256), 256, 0, C.stream>>>(A.ft.data(), A.fs.data(), n, tt.branches.data(),
-                                                                  st.branches.data(), P, A.dec.data(), S.cnt.data());
+        // decisions with per-tile totals, the tiles' offsets, then positions and children: three launches
+        const int64_t tiles = (n + 1 + SCAN_BLOCK * SCAN_ITEMS - 1) / (SCAN_BLOCK * SCAN_ITEMS);
+        S.tile.resize(tiles + 1);
+        k_list_decide<<<(unsigned)tiles, SCAN_BLOCK, 0, C.stream>>>(A.ft.data(), A.fs.data(), n, tt.branches.data(),
+                                                                    st.branches.data(), P, A.dec.data(), S.cnt.data(),
+                                                                    S.tile.data());
         LAUNCH_CHECK();
-        excl_scan(C, S.cnt.data(), A.pos.data(), n + 1);
-        const int64_t next = read_back(C, A.pos.data() + n);
-        if (next == 0) break;
+        k_scan_offsets<int64_t><<<1, SCAN_BLOCK, 0, C.stream>>>(S.tile.data(), tiles);
+        LAUNCH_CHECK();
+        const int64_t next = read_back(C, S.tile.data() + tiles);
         ListLevel& B = S.level(k + 1);
-        B.ft.resize(next); B.fs.resize(next);
-        k_list_expand<<<grid_for(n, 256), 256, 0, C.stream>>>(A.ft.data(), A.fs.data(), n, tt.branches.data(), st.branches.data(),
-                                                             A.dec.data(), A.pos.data(), B.ft.data(), B.fs.data());
+        if (next) { B.ft.resize(next); B.fs.resize(next); }
+        k_list_expand<<<(unsigned)tiles, SCAN_BLOCK, 0, C.stream>>>(A.ft.data(), A.fs.data(), n, tt.branches.data(),
+                                                                    st.branches.data(), A.dec.data(), S.cnt.data(),
+                                                                    S.tile.data(), A.pos.data(), B.ft.data(), B.fs.data());
         LAUNCH_CHECK();
+        if (next == 0) break;
         n = next;
     }
     const int K = (int)sizes.size();

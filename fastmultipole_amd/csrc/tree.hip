@@ -236,6 +236,29 @@ __global__ void k_leaf_keys(const Branch* __restrict__ br, const int64_t* __rest
         key2[i] = ((uint64_t)B.first << 32) | (uint64_t)idx_sorted[i];
 }
 
+// In-leaf order = original index order, one wavefront per leaf: a body's place in its leaf is the number of
+// the leaf's bodies with a smaller original index (leaves of at most LEAF_ORDER_MAX bodies, staged in LDS)
+constexpr int LEAF_ORDER_MAX = 1024;
+__global__ __launch_bounds__(64) void k_leaf_order(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
+                                                   int64_t n_leaves, const uint32_t* __restrict__ idx_sorted,
+                                                   int64_t* __restrict__ sort_index, int64_t* __restrict__ inverse) {
+    __shared__ uint32_t v[LEAF_ORDER_MAX];
+    const int64_t l = blockIdx.x;
+    if (l >= n_leaves) return;
+    const Branch& B = br[leaf_index[l]];
+    const int m = (int)(B.n_bodies < LEAF_ORDER_MAX ? B.n_bodies : LEAF_ORDER_MAX);   // the host keeps leaves within
+    const int64_t f = B.first;
+    for (int i = threadIdx.x; i < m; i += 64) v[i] = idx_sorted[f + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += 64) {
+        const uint32_t x = v[i];
+        int r = 0;
+        for (int j = 0; j < m; ++j) r += v[j] < x ? 1 : 0;
+        sort_index[f + r] = (int64_t)x;
+        inverse[x] = f + r;
+    }
+}
+
 __global__ void k_finish_sort(const uint64_t* __restrict__ key2, int64_t n, int64_t* __restrict__ sort_index,
                               int64_t* __restrict__ inverse) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -261,12 +284,9 @@ __global__ void k_gather_bodies(const double* __restrict__ buf, int rows, int64_
 }
 
 // ---------------------------------------------------------- shrink_recenter!
-// one thread per branch of a level, deepest level first (src/tree.jl:1141-1188)
-__global__ void k_shrink_level(Branch* __restrict__ br, int64_t b0, int64_t b1, const double* __restrict__ x,
-                               const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ radius,
-                               bool use_radius) {
-    int64_t ib = b0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (ib > b1) return;
+// one branch from its bodies (a leaf) or its children (src/tree.jl:1141-1188)
+__device__ inline void shrink_branch(Branch* br, int64_t ib, const double* __restrict__ x, const double* __restrict__ y,
+                                     const double* __restrict__ z, const double* __restrict__ radius, bool use_radius) {
     Branch& b = br[ib];
     double tc[3], tb[3], sc[3], sbx[3], tr = 0.0, sr = 0.0;
     if (b.n_branches == 0) {
@@ -333,6 +353,28 @@ __global__ void k_shrink_level(Branch* __restrict__ br, int64_t b0, int64_t b1, 
     b.source_radius = sr;
 }
 
+// shrink_recenter! of the whole tree in one launch: a thread per leaf shrinks its leaf, then climbs: the
+// thread that finishes a branch's last child shrinks the branch (done[] counts finished children, zero at
+// launch). Each branch is computed once, from its finished children, as the level-by-level pass computes it.
+__global__ void k_shrink_up(Branch* br, const int64_t* __restrict__ leaf_index, int64_t n_leaves, const double* __restrict__ x,
+                            const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ radius,
+                            bool use_radius, int32_t* done) {
+    const int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (l >= n_leaves) return;
+    int64_t ib = leaf_index[l];
+    shrink_branch(br, ib, x, y, z, radius, use_radius);
+    for (;;) {
+        const int64_t pa = br[ib].parent;
+        if (pa < 0) return;
+        const int32_t last = (int32_t)br[pa].n_branches - 1;
+        __threadfence();   // this branch's fields, visible on every XCD before the count
+        if (atomicAdd(done + pa, 1) != last) return;
+        __threadfence();   // the siblings' fields, written before their counts
+        shrink_branch(br, pa, x, y, z, radius, use_radius);
+        ib = pa;
+    }
+}
+
 // the roots, one per system
 __global__ void k_init_roots(const Branch* __restrict__ roots, int n_roots, Branch* __restrict__ br) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -362,7 +404,7 @@ struct TreeScratch {
     DBuf<uint64_t> khi, klo, khi_s, klo_s, key2, key2_s;
     DBuf<uint32_t> idx, idx_s;
     DBuf<int64_t> act_branch, act_first, act_count, sub_first, sub_count;
-    DBuf<int32_t> nonempty, rank, split, srank, flag, frank;
+    DBuf<int32_t> nonempty, rank, split, srank, flag, frank, done;
     DBuf<double> part, roots;
     DBuf<int64_t> seg;
     DBuf<Branch> root_branches;
@@ -589,15 +631,25 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     T.leaf_index.resize(T.n_leaves);
     k_leaf_number<<<grid_for(nb, 256), 256, 0, C.stream>>>(T.branches.data(), nb, S.flag.data(), S.frank.data(), T.leaf_index.data());
     LAUNCH_CHECK();
-    // in-leaf order = original index order
-    S.key2.resize(n); S.key2_s.resize(n);
-    k_leaf_keys<<<(unsigned)T.n_leaves, 64, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves, S.idx_s.data(),
-                                                           S.key2.data());
-    LAUNCH_CHECK();
-    sort_keys_u64(C, S.key2.data(), S.key2_s.data(), n);
+    // in-leaf order = original index order. A leaf holds at most the largest count the leaf test keeps
+    // whole (a branch that exceeds it is split, down to the depth limit, past which the build fails):
+    // up to LEAF_ORDER_MAX bodies, each leaf ranks its bodies in LDS; beyond, a sort by (leaf, index)
     T.sort_index.resize(n); T.inverse_sort.resize(n);
-    k_finish_sort<<<grid_for(n, 256), 256, 0, C.stream>>>(S.key2_s.data(), n, T.sort_index.data(), T.inverse_sort.data());
-    LAUNCH_CHECK();
+    int64_t leaf_max = 1;
+    while (leaf_max <= LEAF_ORDER_MAX && !exceeds(leaf_max + 1, ls, method)) ++leaf_max;
+    if (leaf_max <= LEAF_ORDER_MAX) {
+        k_leaf_order<<<(unsigned)T.n_leaves, 64, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves,
+                                                                S.idx_s.data(), T.sort_index.data(), T.inverse_sort.data());
+        LAUNCH_CHECK();
+    } else {
+        S.key2.resize(n); S.key2_s.resize(n);
+        k_leaf_keys<<<(unsigned)T.n_leaves, 64, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves, S.idx_s.data(),
+                                                               S.key2.data());
+        LAUNCH_CHECK();
+        sort_keys_u64(C, S.key2.data(), S.key2_s.data(), n);
+        k_finish_sort<<<grid_for(n, 256), 256, 0, C.stream>>>(S.key2_s.data(), n, T.sort_index.data(), T.inverse_sort.data());
+        LAUNCH_CHECK();
+    }
     // sorted body data (target_to_buffer! / system_to_buffer!, src/compatibility.jl:340-399)
     T.x.resize(n); T.y.resize(n); T.z.resize(n);
     if (!target) { T.radius.resize(n); T.strength.resize((size_t)sdims * n); }
@@ -605,14 +657,14 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
                                                            T.y.data(), T.z.data(), target ? nullptr : T.radius.data(),
                                                            target ? nullptr : T.strength.data());
     LAUNCH_CHECK();
-    if (shrink)
-        for (int64_t l = (int64_t)T.level_first.size() - 1; l >= 0; --l) {
-            int64_t b0 = T.level_first[l], b1 = T.level_last[l];
-            k_shrink_level<<<grid_for(b1 - b0 + 1, 128), 128, 0, C.stream>>>(T.branches.data(), b0, b1, T.x.data(), T.y.data(),
-                                                                             T.z.data(), target ? nullptr : T.radius.data(),
-                                                                             !target);
-            LAUNCH_CHECK();
-        }
+    if (shrink) {   // bottom-up in one launch
+        S.done.resize(nb);
+        HIP_CHECK(hipMemsetAsync(S.done.data(), 0, sizeof(int32_t) * nb, C.stream));
+        k_shrink_up<<<grid_for(T.n_leaves, 64), 64, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves, T.x.data(),
+                                                                   T.y.data(), T.z.data(), target ? nullptr : T.radius.data(),
+                                                                   !target, S.done.data());
+        LAUNCH_CHECK();
+    }
     T.expansions.resize((size_t)nb * 4 * n_harmonics(p));
 }
 
