@@ -32,7 +32,7 @@ EXPORTED = [
     "fmmhip_options_default", "fmmhip_ctx_create", "fmmhip_ctx_destroy", "fmmhip_last_error", "fmmhip_sync",
     "fmmhip_ctx_set_p2p_free_slots",
     "fmmhip_eval_create", "fmmhip_eval_destroy", "fmmhip_eval_set_systems", "fmmhip_eval_set_batch",
-    "fmmhip_eval_get_batch_stats", "fmmhip_eval_set_tuple", "fmmhip_eval_build_trees",
+    "fmmhip_eval_get_batch_stats", "fmmhip_eval_set_tuple", "fmmhip_eval_set_tuple_leaf_sizes", "fmmhip_eval_build_trees",
     "fmmhip_eval_build_lists", "fmmhip_eval_run", "fmmhip_fmm", "fmmhip_eval_get_targets", "fmmhip_eval_n_branches",
     "fmmhip_eval_n_levels", "fmmhip_eval_n_leaves", "fmmhip_eval_get_branches", "fmmhip_eval_get_levels",
     "fmmhip_eval_get_leaf_index", "fmmhip_eval_get_sort_index", "fmmhip_eval_n_list", "fmmhip_eval_get_list",
@@ -133,6 +133,7 @@ def load():
     L.fmmhip_eval_set_batch.argtypes = [vp, i32, i64, dp, dp, i32]
     L.fmmhip_eval_get_batch_stats.argtypes = [vp, dp, dp]
     L.fmmhip_eval_set_tuple.argtypes = [vp, i64, dp, dp, dp, i32]
+    L.fmmhip_eval_set_tuple_leaf_sizes.argtypes = [vp, i64, dp]
     for f in ("fmmhip_eval_build_trees", "fmmhip_eval_build_lists"):
         getattr(L, f).argtypes = [vp, C.POINTER(Options)]
     for f in ("fmmhip_eval_run", "fmmhip_fmm"):

@@ -295,6 +295,11 @@ class Evaluation:
         self.ctx.check(self.ctx.lib.fmmhip_eval_set_tuple(self._h, len(counts), kernels.ctypes.data, counts.ctypes.data,
                                                           sp, int(on_device)))
 
+    def set_tuple_leaf_sizes(self, leaf_sizes):
+        """One leaf size per system of the tuple set last (the reference's SVector leaf_size)."""
+        ls = np.ascontiguousarray(leaf_sizes, dtype=np.int64)
+        self.ctx.check(self.ctx.lib.fmmhip_eval_set_tuple_leaf_sizes(self._h, len(ls), ls.ctypes.data))
+
     def batch_stats(self):
         """Per system of the last batch evaluation (one entry for a single system): (pmax, error_success)
         arrays, each system's values as its own fmm! call reports them."""
@@ -589,14 +594,21 @@ def _fmm_tuple(systems, leaf_size_source, leaf_size_target, expansion_order, mul
                shrink_recenter, interaction_list_method, scalar_potential, gradient, hessian, farfield, nearfield,
                self_induced, update_target_systems, precision, evaluation):
     """fmm!((sys1, sys2, ...)) (src/fmm.jl:807-884): one tree over the bodies of every system (src/tree.jl:3-122),
-    each system a target of all and a source with its own kernel (point or panel kernels). One leaf size,
-    and SelfTuningTreeStop lists (the default)."""
-    if leaf_size_target != leaf_size_source:
-        raise ValueError("a tuple of systems takes one leaf size")
+    each system a target of all and a source with its own kernel (point or panel kernels). leaf_size: one
+    for every system, or one per system (the reference's SVector); the same for targets and sources.
+    SelfTuningTreeStop (the default) or Barba lists."""
+    per_system = np.ndim(leaf_size_source) > 0
+    if (np.ndim(leaf_size_target) > 0) != per_system or not np.array_equal(leaf_size_target, leaf_size_source):
+        raise ValueError("a tuple of systems takes the same leaf sizes for targets and sources")
+    if per_system and len(leaf_size_source) != len(systems):
+        raise ValueError("one leaf size per system")
     if interaction_list_method is None:
         interaction_list_method = SelfTuningTreeStop()
     ev = evaluation or Evaluation()
     ev.set_tuple([_kernel_of(s) for s in systems], [s.source_buffer() for s in systems])
+    if per_system:
+        ev.set_tuple_leaf_sizes(leaf_size_source)
+        leaf_size_source = leaf_size_target = int(min(leaf_size_source))
     opt = make_options(expansion_order, leaf_size_source, leaf_size_target, multipole_acceptance,
                        interaction_list_method, shrink_recenter, error_tolerance, scalar_potential, gradient, hessian,
                        farfield, nearfield, self_induced, precision)

@@ -224,12 +224,25 @@ struct Timer {
     }
 };
 
+// several systems in one tree: the leaf test counts each system against its leaf size
+static void tree_systems(const Eval& E, Tree& T, int64_t leaf_size) {
+    if (!multi(E) || E.sys_kind.size() < 2) {
+        T.sys_off.clear();
+        T.sys_ls.clear();
+        return;
+    }
+    T.sys_off = E.sys_off;
+    T.sys_ls = E.sys_leaf.empty() ? std::vector<int64_t>(E.sys_kind.size(), leaf_size) : E.sys_leaf;
+}
+
 static void eval_build_trees(Eval& E, const fmmhip_options& o) {
     Context& C = *E.ctx;
     const int dpb = eval_rows(E);
     const int p = o.expansion_order;
     E.tt.seg = E.seg;
     E.st.seg = E.seg;
+    tree_systems(E, E.st, o.leaf_size_source);
+    tree_systems(E, E.tt, o.leaf_size_target);
     if (!E.t_xyz && o.leaf_size_target == o.leaf_size_source) {
         // the targets are the sources with one leaf size: the target tree is the source tree's copy
         build_tree(C, E.st, E.ns, E.s_buf, dpb, false, o.leaf_size_source, o.list_method, o.shrink_recenter != 0, p,
@@ -250,8 +263,8 @@ static void eval_build_trees(Eval& E, const fmmhip_options& o) {
 
 static void eval_build_lists(Eval& E, const fmmhip_options& o, bool count_pairs = true) {
     if (!E.trees_built) throw Error(FMMHIP_EINVAL, "trees are not built");
-    if (multi(E) && o.list_method != FMMHIP_SELFTUNING_TREESTOP)
-        throw Error(FMMHIP_EINVAL, "several systems in one tree take SelfTuningTreeStop lists");
+    if (multi(E) && o.list_method == FMMHIP_SELFTUNING)
+        throw Error(FMMHIP_EINVAL, "several systems in one tree take SelfTuningTreeStop or Barba lists");
     build_lists(*E.ctx, E.tt, E.st, E.lists, o.leaf_size_source, o.multipole_acceptance, o.list_method, o.farfield != 0,
                 o.nearfield != 0, o.self_induced != 0, count_pairs);
     sys_dir_cum(E);
@@ -340,6 +353,7 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
     X.sys_off = E.sys_off;
     X.sys_rows = E.sys_rows;
     X.sys_sdims = E.sys_sdims;
+    X.sys_leaf.clear();   // estimate_influence: leaf size 5 for every system
     if (multi(E)) {
         X.sys_off_dev.resize(E.sys_off.size());
         HIP_CHECK(hipMemcpyAsync(X.sys_off_dev.data(), E.sys_off_dev.data(), E.sys_off.size() * sizeof(int64_t),
@@ -349,6 +363,7 @@ static void estimate_build(Eval& E, const fmmhip_options& o, Context& C) {
     if (X.t_xyz) eval_build_trees(X, eo);
     else {   // the targets are the sources: one build gives both trees; the Morton keys and their
              // order are the main source tree's (same buffer), built on the evaluation's stream
+        tree_systems(X, X.st, eo.leaf_size_source);
         build_tree(C, X.st, X.ns, X.s_buf, eval_rows(X), false, eo.leaf_size_source, eo.list_method, true,
                    eo.expansion_order, eval_sdims(X), E.keys_fresh ? tree_keys(*E.ctx, E.st) : nullptr);
         clone_target_tree(C, X.st, X.tt);
@@ -736,9 +751,24 @@ fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const in
         }
         E.sys_kind.assign(kernels, kernels + n_systems);
         E.sys_off = off;
+        E.sys_leaf.clear();
         E.sys_off_dev.resize(off.size());
         HIP_CHECK(hipMemcpyAsync(E.sys_off_dev.data(), off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, C->stream));
         HIP_CHECK(hipStreamSynchronize(C->stream));
+        E.trees_built = E.lists_built = E.ran = false;
+    });
+}
+
+fmmhip_status fmmhip_eval_set_tuple_leaf_sizes(fmmhip_eval* ev, int64_t n_systems, const int64_t* leaf_sizes) {
+    if (!ev) return FMMHIP_EINVAL;
+    Eval& E = ev->e;
+    Context* C = E.ctx;
+    ABI_TRY(C, {
+        if (!multi(E)) throw Error(FMMHIP_EINVAL, "leaf sizes per system need a tuple (fmmhip_eval_set_tuple)");
+        if (n_systems != (int64_t)E.sys_kind.size() || !leaf_sizes) throw Error(FMMHIP_EINVAL, "one leaf size per system");
+        for (int64_t i = 0; i < n_systems; ++i)
+            if (leaf_sizes[i] < 1) throw Error(FMMHIP_EINVAL, "leaf sizes must be positive");
+        E.sys_leaf.assign(leaf_sizes, leaf_sizes + n_systems);
         E.trees_built = E.lists_built = E.ran = false;
     });
 }

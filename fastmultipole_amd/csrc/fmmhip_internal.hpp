@@ -82,6 +82,9 @@ struct Tree {
     int p = 0;
     uint64_t build_id = 0;           // identifies the build whose keys a context's scratch may still hold
     std::vector<int64_t> seg;        // forest of independent systems: their body offsets (n_systems + 1); else empty
+    // several systems in one tree (one root): their original index offsets (n_systems + 1) and leaf sizes, for
+    // the leaf test that counts each system; else empty
+    std::vector<int64_t> sys_off, sys_ls;
 };
 
 struct Lists {
@@ -181,6 +184,7 @@ struct Eval {
     std::vector<int32_t> sys_kind;
     std::vector<int64_t> sys_off;
     int sys_rows = 0, sys_sdims = 0;
+    std::vector<int64_t> sys_leaf;            // leaf size per system (empty: the options' leaf size for each)
     DBuf<int64_t> sys_off_dev;
     DBuf<Branch> sys_branches;                // per system: the source tree's branches with its leaf ranges
     DBuf<int64_t> sys_dir_cum;                // per system: prefix of its counts over the direct list

@@ -51,6 +51,39 @@ __device__ __host__ inline bool exceeds(int64_t n, int64_t ls, int method) {
     return fraction > 2.8284271247461903 && n > 1;
 }
 
+// exceeds for several systems in one tree (src/tree.jl:492-569): each system's count against its own leaf
+// size; count(i) gives system i's bodies in the branch
+template <class Count>
+__device__ __host__ inline bool exceeds_systems(int n_sys, Count count, const int64_t* ls, int method) {
+    if (method == FMMHIP_BARBA) {
+        bool not_over = true;
+        for (int i = 0; i < n_sys; ++i) not_over = not_over && count(i) <= ls[i];
+        return !not_over;
+    }
+    double fraction = 0.0;
+    int64_t n_bodies = 0, min_ls = ls[0];
+    for (int i = 0; i < n_sys; ++i) {
+        const int64_t n = count(i);
+        n_bodies += n;
+        fraction += (double)n / (double)(ls[i] * ls[i]);
+        min_ls = min_ls < ls[i] ? min_ls : ls[i];
+    }
+    fraction *= (double)min_ls;
+    if (method == FMMHIP_SELFTUNING) return fraction > 0.5 && n_bodies > 1;
+    return fraction > 2.8284271247461903 && n_bodies > 1;
+}
+
+// flag[i] = 1 where body i of the key order (original index idx[i]) belongs to system s; flag[n] = 0
+__global__ void k_sys_flag(const uint32_t* __restrict__ idx, int64_t n, const int64_t* __restrict__ off, int n_sys, int s,
+                           int32_t* __restrict__ flag) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { flag[i] = 0; return; }
+    const int64_t v = idx[i];
+    flag[i] = (v >= off[s] && v < off[s + 1]) ? 1 : 0;
+    (void)n_sys;
+}
+
 // ------------------------------------------------------------- bounding box
 // block (x, y) reduces its share of system y's bodies (seg null: one system, bodies 0 .. n - 1)
 __global__ void k_bbox_partial(const double* __restrict__ buf, int rows, int64_t n, const int64_t* __restrict__ seg,
@@ -148,7 +181,8 @@ __global__ void k_split(const uint64_t* __restrict__ khi, const uint64_t* __rest
 __global__ void k_make_children(int64_t n_active, const int64_t* __restrict__ act_branch, const int64_t* __restrict__ sub_first,
                                 const int64_t* __restrict__ sub_count, const int32_t* __restrict__ rank,
                                 int64_t child_base, Branch* __restrict__ br, int64_t ls, int method,
-                                int32_t* __restrict__ split_flag) {
+                                int32_t* __restrict__ split_flag, const int32_t* __restrict__ sys_prefix, int n_sys,
+                                int64_t n_bodies_all, const int64_t* __restrict__ sys_ls) {
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n_active * 8) return;
     int64_t cnt = sub_count[t];
@@ -178,7 +212,13 @@ __global__ void k_make_children(int64_t n_active, const int64_t* __restrict__ ac
     c.max_influence = 0.0;
     c.pad = 0.0;
     br[ci] = c;
-    split_flag[rank[t]] = exceeds(cnt, ls, method) ? 1 : 0;
+    bool split;
+    if (n_sys > 1) {   // several systems: their counts in the child's range from the per-system prefixes
+        const int64_t b = c.first, e = c.first + cnt, stride = n_bodies_all + 1;
+        split = exceeds_systems(
+            n_sys, [&](int i) { return (int64_t)(sys_prefix[i * stride + e] - sys_prefix[i * stride + b]); }, sys_ls, method);
+    } else split = exceeds(cnt, ls, method);
+    split_flag[rank[t]] = split ? 1 : 0;
 }
 
 // parents' child ranges: first nonempty octant's child index, number of children
@@ -405,6 +445,8 @@ struct TreeScratch {
     DBuf<uint32_t> idx, idx_s;
     DBuf<int64_t> act_branch, act_first, act_count, sub_first, sub_count;
     DBuf<int32_t> nonempty, rank, split, srank, flag, frank, done;
+    DBuf<int32_t> sys_prefix, sys_flag;   // several systems: per system, the prefix count over the key order
+    DBuf<int64_t> sys_off, sys_ls;
     DBuf<double> part, roots;
     DBuf<int64_t> seg;
     DBuf<Branch> root_branches;
@@ -436,6 +478,25 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, const std::vect
                             const std::vector<double>& roots_h, int64_t ls, int method, int max_level) {
     const int64_t n = T.n;
     const int n_roots = (int)seg.size() - 1;
+    // several systems in one tree: the leaf test counts each system (per-system prefix counts over the key
+    // order, which the branch ranges index)
+    const bool multi = T.sys_off.size() > 2;
+    const int n_sys = multi ? (int)T.sys_off.size() - 1 : 1;
+    if (multi) {
+        if (n_roots != 1 || (int)T.sys_ls.size() != n_sys || T.sys_off.front() != 0 || T.sys_off.back() != n)
+            throw Error(FMMHIP_EINVAL, "several systems need one root, offsets covering the bodies and a leaf size each");
+        S.sys_off.resize(n_sys + 1);
+        S.sys_ls.resize(n_sys);
+        HIP_CHECK(hipMemcpyAsync(S.sys_off.data(), T.sys_off.data(), (n_sys + 1) * sizeof(int64_t), hipMemcpyHostToDevice, C.stream));
+        HIP_CHECK(hipMemcpyAsync(S.sys_ls.data(), T.sys_ls.data(), n_sys * sizeof(int64_t), hipMemcpyHostToDevice, C.stream));
+        S.sys_flag.resize(n + 1);
+        S.sys_prefix.resize((size_t)n_sys * (n + 1));
+        for (int k = 0; k < n_sys; ++k) {
+            k_sys_flag<<<grid_for(n + 1, 256), 256, 0, C.stream>>>(S.idx_s.data(), n, S.sys_off.data(), n_sys, k, S.sys_flag.data());
+            LAUNCH_CHECK();
+            exclusive_scan_i32(C, S.sys_flag.data(), S.sys_prefix.data() + (size_t)k * (n + 1), n + 1);
+        }
+    }
     std::vector<Branch> rb(n_roots);
     std::vector<int64_t> act(3 * (size_t)n_roots);   // active roots: branch, first body, count
     int64_t n_active = 0;
@@ -448,7 +509,10 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, const std::vect
         }
         r.source_radius = r.target_radius = roots_h[5 * i + 4];
         r.max_influence = 0.0; r.pad = 0.0;
-        if (exceeds(r.n_bodies, ls, method)) {
+        const bool split = multi ? exceeds_systems(
+                                       n_sys, [&](int k) { return T.sys_off[k + 1] - T.sys_off[k]; }, T.sys_ls.data(), method)
+                                 : exceeds(r.n_bodies, ls, method);
+        if (split) {
             act[n_active] = i;
             act[n_roots + n_active] = r.first;
             act[2 * (size_t)n_roots + n_active] = r.n_bodies;
@@ -497,7 +561,8 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, const std::vect
         S.split.resize(n_children + 1); S.srank.resize(n_children + 1);
         k_make_children<<<grid_for(m, 256), 256, 0, C.stream>>>(n_active, S.act_branch.data(), S.sub_first.data(),
                                                                 S.sub_count.data(), S.rank.data(), child_base, T.branches.data(),
-                                                                ls, method, S.split.data());
+                                                                ls, method, S.split.data(), multi ? S.sys_prefix.data() : nullptr,
+                                                                n_sys, n, multi ? S.sys_ls.data() : nullptr);
         LAUNCH_CHECK();
         k_link_parents<<<grid_for(n_active, 256), 256, 0, C.stream>>>(n_active, S.act_branch.data(), S.nonempty.data(),
                                                                       S.rank.data(), child_base, T.branches.data());
@@ -635,8 +700,16 @@ void build_tree(Context& C, Tree& T, int64_t n, const double* d_buf, int rows, b
     // whole (a branch that exceeds it is split, down to the depth limit, past which the build fails):
     // up to LEAF_ORDER_MAX bodies, each leaf ranks its bodies in LDS; beyond, a sort by (leaf, index)
     T.sort_index.resize(n); T.inverse_sort.resize(n);
-    int64_t leaf_max = 1;
-    while (leaf_max <= LEAF_ORDER_MAX && !exceeds(leaf_max + 1, ls, method)) ++leaf_max;
+    int64_t leaf_max = 0;
+    if (T.sys_off.size() > 2) {   // a leaf's per-system terms each stay within the threshold: bound each count
+        const double min_ls = (double)*std::min_element(T.sys_ls.begin(), T.sys_ls.end());
+        const double thr = method == FMMHIP_SELFTUNING ? 0.5 : 2.8284271247461903;
+        for (int64_t l : T.sys_ls)
+            leaf_max += method == FMMHIP_BARBA ? l : (int64_t)(thr * (double)l * (double)l / min_ls) + 2;
+    } else {
+        leaf_max = 1;
+        while (leaf_max <= LEAF_ORDER_MAX && !exceeds(leaf_max + 1, ls, method)) ++leaf_max;
+    }
     if (leaf_max <= LEAF_ORDER_MAX) {
         k_leaf_order<<<(unsigned)T.n_leaves, 64, 0, C.stream>>>(T.branches.data(), T.leaf_index.data(), T.n_leaves,
                                                                 S.idx_s.data(), T.sort_index.data(), T.inverse_sort.data());

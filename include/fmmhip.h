@@ -150,9 +150,10 @@ fmmhip_status fmmhip_eval_set_systems(fmmhip_eval* ev, int32_t kernel, int64_t n
 fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_systems, const int64_t* counts,
                                      const double* source_buffer, int32_t on_device);
 /* Several systems in one tree, each a target and a source of all of them (fmm!((sys1, sys2, ...)),
- * src/fmm.jl:862-884 with the tuple trees of src/tree.jl:3-122): one leaf size and SelfTuningTreeStop lists
- * (with which the reference's per-system leaf test on equal leaf sizes is the test on the branch's total
- * count, so the tree is that of all bodies together). kernels: any FMMHIP_POINT_* or FMMHIP_PANEL_* kernel
+ * src/fmm.jl:862-884 with the tuple trees of src/tree.jl:3-122): one tree over the bodies of every system,
+ * whose leaf test counts each system against its leaf size (src/tree.jl:492-569; opt->leaf_size_source for
+ * each, or fmmhip_eval_set_tuple_leaf_sizes); SelfTuningTreeStop or Barba lists (SelfTuning lists are
+ * FMMHIP_EINVAL: their decisions weigh each system's count too). kernels: any FMMHIP_POINT_* or FMMHIP_PANEL_* kernel
  * per system. buffer: R x sum(counts) column-major, R the largest data_per_body of the kernels (7 for point
  * systems, 14 with triangles, 17 with quadrilaterals), each system's rows [x, y, z, radius, strength
  * rows...] first and zeros after them, the systems one after another in the order of kernels. The
@@ -160,6 +161,11 @@ fmmhip_status fmmhip_eval_set_batch(fmmhip_eval* ev, int32_t kernel, int64_t n_s
  * order. Not combined with batches or FastGaussSeidel. */
 fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const int32_t* kernels, const int64_t* counts,
                                      const double* buffer, int32_t on_device);
+/* Leaf sizes per system for the tuple set last (the reference's SVector leaf_size, src/tree.jl:492-569):
+ * a branch splits by the reference's test on each system's count against its own leaf size (SelfTuningTreeStop
+ * and Barba trees; without this call every system takes opt->leaf_size_source). Kept until the next
+ * fmmhip_eval_set_tuple. */
+fmmhip_status fmmhip_eval_set_tuple_leaf_sizes(fmmhip_eval* ev, int64_t n_systems, const int64_t* leaf_sizes);
 /* After an evaluation: per system of the batch, the expansion order and error_success its own fmm! call
  * reports (pmax: the largest order its M2L interactions used, plus one where the tolerance was not met).
  * Arrays of n_systems entries (one entry for an evaluation of a single system); either may be null.

@@ -422,10 +422,11 @@ end
 
 `fmm!(systems)` for a tuple of systems that share one tree, e.g. panels and vortex particles
 (src/fmm.jl:807-884, src/tree.jl:3-122): every system is a target of all of them and a source with its
-own kernel. One leaf size and `SelfTuningTreeStop` lists. Returns the reference's 8-tuple as `hip_fmm!`
-does, with one `DerivativesSwitch` per system.
+own kernel. `leaf_size_source`: one leaf size, or one per system (an `SVector`, as the reference takes it);
+`SelfTuningTreeStop` or `Barba` lists. Returns the reference's 8-tuple as `hip_fmm!` does, with one
+`DerivativesSwitch` per system.
 """
-function hip_fmm_tuple!(hip::HipEval, systems::Tuple; update_target_systems=true, kwargs...)
+function hip_fmm_tuple!(hip::HipEval, systems::Tuple; update_target_systems=true, leaf_size_source=20, kwargs...)
     ks = [hip_kernel(s) for s in systems]
     any(k -> k === nothing, ks) && throw(ArgumentError("every system of a tuple declares a hip_kernel"))
     kernels = Int32.(ks)
@@ -438,12 +439,16 @@ function hip_fmm_tuple!(hip::HipEval, systems::Tuple; update_target_systems=true
         sbuf[1:size(b, 1), first:first+size(b, 2)-1] .= b
         first += size(b, 2)
     end
-    opt = FmmHipOptions(; kwargs...)
+    per_system = leaf_size_source isa AbstractVector
+    ls = per_system ? Int64.(collect(leaf_size_source)) : Int64[]
+    opt = FmmHipOptions(; leaf_size_source=per_system ? minimum(ls) : leaf_size_source, kwargs...)
     stats = FmmHipStats()
-    st = GC.@preserve sbuf counts kernels begin
+    st = GC.@preserve sbuf counts kernels ls begin
         check(ccall((:fmmhip_eval_set_tuple, libfmmhip), Cint,
                     (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}, Cint),
                     hip.ev, length(counts), kernels, counts, sbuf, 0), hip.ctx)
+        per_system && check(ccall((:fmmhip_eval_set_tuple_leaf_sizes, libfmmhip), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}),
+                                  hip.ev, length(ls), ls), hip.ctx)
         check(ccall((:fmmhip_fmm, libfmmhip), Cint, (Ptr{Cvoid}, Ref{FmmHipOptions}, Ref{FmmHipStats}),
                     hip.ev, opt, stats), hip.ctx)
     end
@@ -468,7 +473,7 @@ function fmm!(target_systems::Tuple, source_systems::Tuple, hip::HipEval; kwargs
         return hip_fmm!(hip, target_systems[1], source_systems[1]; kwargs...)
     end
     if target_systems === source_systems && all(s -> hip_kernel(s) !== nothing, source_systems) &&
-       get(kwargs, :interaction_list_method, SelfTuningTreeStop()) isa SelfTuningTreeStop
+       !(get(kwargs, :interaction_list_method, SelfTuningTreeStop()) isa SelfTuning)
         return hip_fmm_tuple!(hip, source_systems; kwargs...)
     end
     return fmm!(target_systems, source_systems; kwargs...)

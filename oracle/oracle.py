@@ -58,7 +58,7 @@ def lib():
         L.orc_run_create.restype = C.c_void_p
         L.orc_run_create.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ic, _dp]
         L.orc_run_create_tuple.restype = C.c_void_p
-        L.orc_run_create_tuple.argtypes = [C.c_int, _ic, _ip64, _dp, _dp, _ic, _dp]
+        L.orc_run_create_tuple.argtypes = [C.c_int, _ic, _ip64, C.c_void_p, _dp, _dp, _ic, _dp]
         for f in ("orc_run_free", "orc_run_fmm", "orc_run_trees", "orc_run_lists"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = None
@@ -186,12 +186,14 @@ class Run:
         self.p = p
 
     @classmethod
-    def tuple(cls, kinds, datas, p=5, leaf_size_source=20, leaf_size_target=None, method=METHOD_SELFTUNING_TREESTOP,
+    def tuple(cls, kinds, datas, p=5, leaf_size_source=20, leaf_size_target=None, leaf_sizes=None,
+              method=METHOD_SELFTUNING_TREESTOP,
               shrink_recenter=True, error_kind=ERR_NONE, relative=False, eps=0.0, BE=True, scalar_potential=False,
               gradient=True, hessian=False, farfield=True, nearfield=True, self_induced=True, multipole_acceptance=0.4):
         """fmm! on a tuple of systems (each system both target and source): one tree over all bodies,
         each source system's nearfield and P2M with its own kernel. Targets come back concatenated in the
-        order of `datas`."""
+        order of `datas`. leaf_sizes: one leaf size per system (the reference's SVector leaf_size), else
+        leaf_size_source for each."""
         L = lib()
         self = cls.__new__(cls)
         s = pack_tuple(kinds, datas)
@@ -202,7 +204,10 @@ class Run:
         self.nt = self.ns = len(s)
         oi, od = _options(p, leaf_size_source, leaf_size_target, method, shrink_recenter, error_kind, relative, eps, BE,
                           scalar_potential, gradient, hessian, farfield, nearfield, self_induced, multipole_acceptance)
+        self._ls = None if leaf_sizes is None else np.ascontiguousarray(leaf_sizes, dtype=np.int64)
+        assert self._ls is None or len(self._ls) == len(kinds)
         self._h = L.orc_run_create_tuple(len(kinds), np.ascontiguousarray(kinds, dtype=np.intc), counts,
+                                         None if self._ls is None else self._ls.ctypes.data,
                                          np.ascontiguousarray(t.ravel()), np.ascontiguousarray(s.ravel()), oi, od)
         self.p = p
         return self

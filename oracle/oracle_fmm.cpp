@@ -56,6 +56,29 @@ static bool exceeds(int64_t n, int64_t ls, int method) {   // src/tree.jl:492-56
     return fraction > 2.8284271247461903 && n > 1;
 }
 
+// exceeds for several systems in one tree (src/tree.jl:492-569): the count of each system against its
+// own leaf size; count(i) gives system i's bodies in the branch
+template <class Count>
+static bool exceeds_systems(int n_sys, Count count, const int64_t* ls, int method) {
+    if (method == METHOD_BARBA) {
+        bool not_over = true;
+        for (int i = 0; i < n_sys; ++i) not_over = not_over && count(i) <= ls[i];
+        return !not_over;
+    }
+    double fraction = 0.0;
+    int64_t n_bodies = 0;
+    int64_t min_ls = ls[0];
+    for (int i = 0; i < n_sys; ++i) {
+        const int64_t n = count(i);
+        n_bodies += n;
+        fraction += (double)n / (double)(ls[i] * ls[i]);
+        min_ls = std::min(min_ls, ls[i]);
+    }
+    fraction *= (double)min_ls;
+    if (method == METHOD_SELFTUNING) return fraction > 0.5 && n_bodies > 1;
+    return fraction > 2.8284271247461903 && n_bodies > 1;
+}
+
 static inline int get_octant(const double* pos, const double* c) {   // src/tree.jl:474-480
     int o = 1;
     if (pos[0] > c[0]) o += 1;
@@ -87,6 +110,22 @@ struct TreeBuilder {
     TreeBuilder(OTree& t, int64_t n, int64_t l, int m) : T(t), N(n), ls(l), method(m), small(3 * n), sort_buf(n) {}
     double* pos(int64_t i) { return &T.buffer[(size_t)i * T.rows]; }
 
+    // several systems: original index ranges of the systems and their leaf sizes (null: one system)
+    const std::vector<int64_t>* sys_off = nullptr;
+    const std::vector<int64_t>* sys_ls = nullptr;
+
+    // the leaf test of the branch holding bodies first .. first + count - 1
+    bool splits(int64_t first, int64_t count) {
+        if (!sys_off) return exceeds(count, ls, method);
+        const int n_sys = (int)sys_off->size() - 1;
+        std::vector<int64_t> c(n_sys, 0);
+        for (int64_t i = first; i < first + count; ++i) {
+            const int64_t o = T.sort_index[i];
+            c[std::upper_bound(sys_off->begin(), sys_off->end(), o) - sys_off->begin() - 1] += 1;
+        }
+        return exceeds_systems(n_sys, [&](int i) { return c[i]; }, sys_ls->data(), method);
+    }
+
     void census(int64_t first, int64_t count, const double* c, int64_t* cum) {   // :608-625
         for (int k = 0; k < 8; ++k) cum[k] = 0;
         for (int64_t i = first; i < first + count; ++i) cum[get_octant(pos(i), c) - 1] += 1;
@@ -100,7 +139,7 @@ struct TreeBuilder {
         int64_t cum[8];
         census(first, count, tc, cum);
         n_children = 0;
-        if (exceeds(cum[7], ls, method))
+        if (splits(first, count))
             for (int o = 0; o < 8; ++o) if ((o == 0 ? cum[0] : cum[o] - cum[o - 1]) > 0) n_children += 1;
         if (n_children > 0) {
             int64_t oi[8];   // octant_beginning_index! (0-based)
@@ -145,7 +184,7 @@ struct TreeBuilder {
                 double child_box[3] = {parent.target_box[0] * 0.5, parent.target_box[1] * 0.5, parent.target_box[2] * 0.5};
                 int64_t cum[8];
                 census(parent.first, parent.n_bodies, parent.target_center, cum);
-                if (exceeds(cum[7], ls, method)) {
+                if (splits(parent.first, parent.n_bodies)) {
                     for (int o = 1; o <= 8; ++o) {
                         int64_t pop = (o == 1) ? cum[0] : cum[o - 1] - cum[o - 2];
                         if (pop > 0) {
@@ -241,7 +280,8 @@ static void shrink_recenter(OTree& T, bool use_radius) {   // src/tree.jl:1176-1
 // `full` is the system in its original order: rows x N (target: 16 rows with
 // positions only; source: data_per_body rows).
 static void build_tree(OTree& T, int64_t N, int rows, const double* full, bool target, int64_t ls, int method,
-                       bool shrink, int p, int n_divisions = 20) {
+                       bool shrink, int p, const std::vector<int64_t>* sys_off = nullptr,
+                       const std::vector<int64_t>* sys_ls = nullptr, int n_divisions = 20) {
     T = OTree();
     T.rows = rows;
     T.p = p;
@@ -266,6 +306,7 @@ static void build_tree(OTree& T, int64_t N, int rows, const double* full, bool t
     double radius = std::sqrt(bmax * bmax * 3.0);
     double cbox[3] = {bmax, bmax, bmax};
     TreeBuilder B(T, N, ls, method);
+    if (sys_off && sys_off->size() > 2) { B.sys_off = sys_off; B.sys_ls = sys_ls; }
     int64_t i_leaf = 0, n_children = 0;
     OBranch root = B.branch(1, 0, N, center, center, radius, radius, cbox, cbox, -1, i_leaf, n_children);
     T.branches.push_back(root);
@@ -719,6 +760,7 @@ struct Run {
     std::vector<int> kinds;
     std::vector<int64_t> off;
     int rows = 0;   // rows per body of a tuple's buffer: the largest data_per_body of its kinds
+    std::vector<int64_t> sys_ls;   // a tuple's leaf size per system (empty: the options' leaf size for each)
     int64_t nt, ns;
     std::vector<double> tfull, sfull;   // original order: 16 x nt, dpb x ns
     Options opt;
@@ -731,6 +773,20 @@ struct Run {
 };
 
 static int run_rows(const Run& R) { return R.kinds.empty() ? data_per_body(R.kind) : R.rows; }
+
+// both trees of a run; a tuple's trees take the leaf test of several systems
+static void build_trees(Run& R) {
+    const Options& o = R.opt;
+    if (R.kinds.empty()) {
+        build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p);
+        build_tree(R.st, R.ns, run_rows(R), R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p);
+        return;
+    }
+    std::vector<int64_t> ls = R.sys_ls;
+    if (ls.empty()) ls.assign(R.kinds.size(), o.ls_source);
+    build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p, &R.off, &ls);
+    build_tree(R.st, R.ns, run_rows(R), R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p, &R.off, &ls);
+}
 
 static BranchGeom geom(const OBranch& b) {
     BranchGeom g;
@@ -748,7 +804,7 @@ static void fmm_run(Run& R, bool estimate_pass);
 static void estimate_influence(Run& R) {
     Run E;
     E.kind = R.kind; E.nt = R.nt; E.ns = R.ns; E.tfull = R.tfull; E.sfull = R.sfull;
-    E.kinds = R.kinds; E.off = R.off; E.rows = R.rows;
+    E.kinds = R.kinds; E.off = R.off; E.rows = R.rows;   // leaf size 5 for every system
     E.opt = R.opt;
     E.opt.PS = true; E.opt.GS = true; E.opt.HS = false;
     E.opt.ls_source = 5; E.opt.ls_target = 5;
@@ -799,8 +855,7 @@ static void fmm_run(Run& R, bool estimate_pass) {
     const Options& o = R.opt;
     const bool LH = has_vector_potential(R.kind);
     const int dpb = run_rows(R);
-    build_tree(R.tt, R.nt, 16, R.tfull.data(), true, o.ls_target, o.method, o.shrink, o.p);
-    build_tree(R.st, R.ns, dpb, R.sfull.data(), false, o.ls_source, o.method, o.shrink, o.p);
+    build_trees(R);
     R.m2l.clear(); R.direct.clear();
     if (!R.tt.branches.empty() && !R.st.branches.empty()) {
         ListBuilder L{R.tt, R.st, o.ls_source, o.theta, o.ff, o.nf, o.si, o.method, {}, {}};
@@ -1290,8 +1345,9 @@ void* orc_run_create(int kind, int64_t nt, const double* tfull, int64_t ns, cons
 // data_per_body of the kinds and at least 7; each system's rows first, zeros after them); the targets are
 // the same bodies (tfull 16 x sum(counts)). The Lamb-Helmholtz terms are on when any system has a vector
 // potential (src/compatibility.jl:292-298): the run's kernel tag is then a vortex kernel, else a dipole one.
-void* orc_run_create_tuple(int n_sys, const int* kinds, const int64_t* counts, const double* tfull, const double* sfull,
-                           const int* oi, const double* od) {
+// leaf_sizes: one per system, or null for the options' leaf size for each
+void* orc_run_create_tuple(int n_sys, const int* kinds, const int64_t* counts, const int64_t* leaf_sizes, const double* tfull,
+                           const double* sfull, const int* oi, const double* od) {
     std::vector<int64_t> off(n_sys + 1, 0);
     bool lh = false;
     int rows = 7;
@@ -1306,6 +1362,7 @@ void* orc_run_create_tuple(int n_sys, const int* kinds, const int64_t* counts, c
     R->kinds.assign(kinds, kinds + n_sys);
     R->off = off;
     R->rows = rows;
+    if (leaf_sizes) R->sys_ls.assign(leaf_sizes, leaf_sizes + n_sys);
     return R;
 }
 
@@ -1318,11 +1375,7 @@ int orc_threads(int n) {
 void orc_run_fmm(void* h) { fmm_run(*(Run*)h, false); }
 
 // tree only (no lists, no evaluation): which = 0 target tree, 1 source tree
-void orc_run_trees(void* h) {
-    Run& R = *(Run*)h;
-    build_tree(R.tt, R.nt, 16, R.tfull.data(), true, R.opt.ls_target, R.opt.method, R.opt.shrink, R.opt.p);
-    build_tree(R.st, R.ns, run_rows(R), R.sfull.data(), false, R.opt.ls_source, R.opt.method, R.opt.shrink, R.opt.p);
-}
+void orc_run_trees(void* h) { build_trees(*(Run*)h); }
 
 void orc_run_lists(void* h) {
     Run& R = *(Run*)h;

@@ -5,7 +5,8 @@ the oracle relative to the largest target value of each slot, and the expansions
 coefficient (same expansion order, leaf size, MAC); with an error tolerance, pmax and error_success equal the
 oracle's, every M2L order decision that is not within 1e-9 (relative) of the tolerance equals the oracle's,
 and the targets match to 1e-11 away from flipped decisions. A tuple of one system equals the plain
-evaluation of that system bit for bit.
+evaluation of that system bit for bit. Cases cover point and panel kernels, a leaf size per system, and
+SelfTuningTreeStop and Barba lists.
 """
 import ctypes
 import sys
@@ -94,6 +95,14 @@ CASES = [
     ("dipole-quads+source-tris-p9", [(_lib.PANEL_DIPOLE_QUAD, _panels(True), 1500),
                                      (_lib.PANEL_SOURCE_TRI, _panels(False), 1200)],
      dict(expansion_order=9, leaf_size=16, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    # a leaf size per system (the reference's SVector leaf_size): the leaf test counts each system
+    ("source+vortex-leaf-sizes", [(_lib.POINT_SOURCE, _src, 6000), (_lib.POINT_VORTEX, _ring, 9000)],
+     dict(expansion_order=9, leaf_size=[40, 12], multipole_acceptance=0.5, hessian=True)),
+    ("quads+gauss-barba-leaf-sizes", [(_lib.PANEL_SOURCE_QUAD, _panels(True), 2400), (_lib.VORTEX_GAUSSIAN, _gauss, 4000)],
+     dict(expansion_order=8, leaf_size=[10, 30], multipole_acceptance=0.5, interaction_list_method=F.Barba())),
+    ("vortex+dipole+source-barba", [(_lib.POINT_VORTEX, _ring, 5000), (_lib.POINT_DIPOLE, _dipoles, 3000),
+                                    (_lib.POINT_SOURCE, _src, 3000)],
+     dict(expansion_order=8, leaf_size=24, multipole_acceptance=0.5, interaction_list_method=F.Barba())),
 ]
 
 
@@ -102,8 +111,13 @@ def _make(spec, seed):
 
 
 def _oracle(systems, kw):
-    okw = dict(p=kw["expansion_order"], leaf_size_source=kw["leaf_size"], multipole_acceptance=kw["multipole_acceptance"],
+    ls = kw["leaf_size"]
+    per_system = np.ndim(ls) > 0
+    okw = dict(p=kw["expansion_order"], leaf_size_source=int(min(ls)) if per_system else ls,
+               leaf_sizes=list(ls) if per_system else None, multipole_acceptance=kw["multipole_acceptance"],
                scalar_potential=kw.get("scalar_potential", False), gradient=True, hessian=kw.get("hessian", False))
+    if kw.get("interaction_list_method") is not None:
+        okw["method"] = kw["interaction_list_method"].code
     em = kw.get("error_tolerance")
     if em is not None:
         okw.update(error_kind=em.kind, relative=em.relative, eps=em.eps, BE=em.BE)
@@ -203,8 +217,12 @@ def test_tuple_rejects_bad_input():
     assert lib.fmmhip_eval_set_tuple(ev._h, 2, good.ctypes.data, neg.ctypes.data, rows.ctypes.data, 0) == _lib.FMMHIP_EINVAL
     assert lib.fmmhip_eval_set_tuple(ev._h, 0, good.ctypes.data, counts.ctypes.data, rows.ctypes.data, 0) == _lib.FMMHIP_EINVAL
     assert lib.fmmhip_eval_set_tuple(ev._h, 2, good.ctypes.data, counts.ctypes.data, rows.ctypes.data, 0) == 0
-    opt = F.make_options(4, 20, interaction_list_method=F.Barba())
+    opt = F.make_options(4, 20, interaction_list_method=F.SelfTuning())
     assert lib.fmmhip_eval_build_trees(ev._h, ctypes.byref(opt)) == 0
     assert lib.fmmhip_eval_build_lists(ev._h, ctypes.byref(opt)) == _lib.FMMHIP_EINVAL
+    bad_ls = np.array([10, 0], dtype=np.int64)
+    assert lib.fmmhip_eval_set_tuple_leaf_sizes(ev._h, 2, bad_ls.ctypes.data) == _lib.FMMHIP_EINVAL
+    assert lib.fmmhip_eval_set_tuple_leaf_sizes(ev._h, 3, counts.ctypes.data) == _lib.FMMHIP_EINVAL
+    assert lib.fmmhip_eval_set_tuple_leaf_sizes(ev._h, 2, counts.ctypes.data) == 0
     assert lib.fmmhip_fgs_setup(ev._h, ctypes.byref(opt), _lib.INFLUENCE_POTENTIAL) == _lib.FMMHIP_EINVAL
     ev.close()

@@ -132,3 +132,45 @@ def test_tuple_error_controlled_meets_tolerance():
     err = np.linalg.norm(got[:, 4:7] - ref[:, 4:7], axis=1)
     assert run.error_success
     assert np.max(err / v) < 1e-4
+
+
+# --------------------------------------------------------------- the leaf rule, per system
+def _rule(counts, ls, method):
+    """src/tree.jl:492-569: does a branch with these per-system counts split?"""
+    if method == O.METHOD_BARBA:
+        return any(c > l for c, l in zip(counts, ls))
+    frac = 0.0
+    for c, l in zip(counts, ls):
+        frac += c / (l * l)
+    frac *= min(ls)
+    thr = 0.5 if method == O.METHOD_SELFTUNING else 2.8284271247461903
+    return frac > thr and sum(counts) > 1
+
+
+@pytest.mark.parametrize("method", [O.METHOD_SELFTUNING_TREESTOP, O.METHOD_SELFTUNING, O.METHOD_BARBA])
+@pytest.mark.parametrize("leaf_sizes", [None, (30, 6, 14)])
+def test_every_branch_follows_the_leaf_rule(method, leaf_sizes):
+    """Every branch that has children passes the reference's leaf test on its per-system counts, and every
+    leaf fails it (for one system and for three with their own leaf sizes)."""
+    rng = np.random.default_rng(25)
+    kinds = [O.KIND_SOURCE, O.KIND_VORTEX, O.KIND_DIPOLE]
+    datas = [_system(k, n, rng, s) for k, n, s in zip(kinds, (1500, 900, 1200), (0.0, 0.4, 0.8))]
+    ls = list(leaf_sizes) if leaf_sizes else [12, 12, 12]
+    runs = [("tuple", O.Run.tuple(kinds, datas, p=4, leaf_size_source=12, leaf_sizes=leaf_sizes, method=method), None)]
+    if leaf_sizes is None:
+        runs.append(("one", O.Run(O.KIND_VORTEX, datas[1][:, :3], datas[1], p=4, leaf_size_source=12, method=method), 1))
+    for name, run, only in runs:
+        run.trees()
+        for which in (0, 1):
+            ints, _ = run.branches(which)
+            si = run.sort_index(which)
+            off = np.concatenate([[0], np.cumsum([len(d) for d in datas])]) if only is None else np.array([0, len(si)])
+            sys_of = np.searchsorted(off, si, side="right") - 1
+            n_split = 0
+            for b in ints:
+                members = sys_of[b[1]:b[1] + b[0]]
+                counts = [int(np.sum(members == i)) for i in range(len(off) - 1)]
+                rule_ls = ls if only is None else [12]
+                assert _rule(counts, rule_ls, method) == (b[2] > 0), (name, which, b, counts)
+                n_split += b[2] > 0
+            assert n_split > 3
