@@ -138,6 +138,96 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(const K* __restrict_
     }
 }
 
+// Every pass of a sort of at most RS_TILE pairs in one workgroup, the pairs in LDS between passes (the
+// same digits and stable ranking as the passes above, the tile's own digit counts giving the offsets).
+constexpr int RS_MAX_PASSES = 16;
+struct RadixPasses {
+    RadixPass p[RS_MAX_PASSES];
+    int n;
+};
+
+template <class K, bool HAS_V>
+__global__ __launch_bounds__(RS_BLOCK) void k_radix_small(const K* __restrict__ kin, const uint32_t* __restrict__ vin, int n,
+                                                          RadixPasses passes, const int64_t* __restrict__ seg, int n_seg,
+                                                          K* __restrict__ kout, uint32_t* __restrict__ vout) {
+    __shared__ K ks[2][RS_TILE];
+    __shared__ uint32_t vs[HAS_V ? 2 : 1][HAS_V ? RS_TILE : 1];
+    __shared__ int32_t cnt[RS_BLOCK / 64][256];
+    __shared__ int32_t run[256];
+    __shared__ int32_t tot[256];
+    __shared__ int64_t sh[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int i = tid; i < n; i += RS_BLOCK) {
+        ks[0][i] = kin[i];
+        if (HAS_V) vs[0][i] = vin[i];
+    }
+    int cur = 0;
+    for (int pi = 0; pi < passes.n; ++pi) {
+        const RadixPass P = passes.p[pi];
+        tot[tid] = 0;
+#pragma unroll
+        for (int q = 0; q < RS_BLOCK / 64; ++q) cnt[q][tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += RS_BLOCK)
+            atomicAdd(&tot[radix_digit<K>(ks[cur][i], HAS_V ? vs[cur][i] : 0u, P, seg, n_seg)], 1);
+        __syncthreads();
+        {   // each digit's first output position
+            int64_t total;
+            run[tid] = (int32_t)block_excl_scan<int64_t>((int64_t)tot[tid], total, sh);
+        }
+        __syncthreads();
+        for (int r = 0; r < RS_ROUNDS; ++r) {
+            const int i = r * RS_BLOCK + tid;
+            const bool valid = i < n;
+            K key = K();
+            uint32_t val = 0u;
+            unsigned d = 0u;
+            if (valid) {
+                key = ks[cur][i];
+                if (HAS_V) val = vs[cur][i];
+                d = radix_digit<K>(key, val, P, seg, n_seg);
+            }
+            unsigned long long peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const int rank = __popcll(peers & below);
+            if (valid && rank == 0) cnt[w][d] = __popcll(peers);
+            __syncthreads();
+            {
+                int s = 0;
+#pragma unroll
+                for (int q = 0; q < RS_BLOCK / 64; ++q) {
+                    const int c = cnt[q][tid];
+                    cnt[q][tid] = s;
+                    s += c;
+                }
+                tot[tid] = s;
+            }
+            __syncthreads();
+            if (valid) {
+                const int pos = run[d] + cnt[w][d] + rank;
+                ks[cur ^ 1][pos] = key;
+                if (HAS_V) vs[cur ^ 1][pos] = val;
+            }
+            __syncthreads();
+            run[tid] += tot[tid];
+#pragma unroll
+            for (int q = 0; q < RS_BLOCK / 64; ++q) cnt[q][tid] = 0;
+            __syncthreads();
+        }
+        cur ^= 1;
+    }
+    for (int i = tid; i < n; i += RS_BLOCK) {
+        kout[i] = ks[cur][i];
+        if (HAS_V) vout[i] = vs[cur][i];
+    }
+}
+
 // Sorts n pairs by bits 0 .. key_bits - 1 of the key (descending when desc), stably, on the context's
 // stream; vin/vout may be null (keys alone). With seg (device, n_seg + 1 offsets, seg[0] = 0 and
 // seg[n_seg] = n), within each range seg[s] .. seg[s + 1] - 1 (see the file's header for what the values
@@ -159,6 +249,15 @@ void radix_sort(Context& C, const K* kin, K* kout, const uint32_t* vin, uint32_t
     if (passes.empty()) {
         HIP_CHECK(hipMemcpyAsync(kout, kin, sizeof(K) * n, hipMemcpyDeviceToDevice, C.stream));
         if (has_v) HIP_CHECK(hipMemcpyAsync(vout, vin, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, C.stream));
+        return;
+    }
+    if (n <= RS_TILE && (int)passes.size() <= RS_MAX_PASSES) {   // one workgroup, one launch
+        RadixPasses rp;
+        rp.n = (int)passes.size();
+        for (int i = 0; i < rp.n; ++i) rp.p[i] = passes[i];
+        if (has_v) k_radix_small<K, true><<<1, RS_BLOCK, 0, C.stream>>>(kin, vin, (int)n, rp, seg, n_seg, kout, vout);
+        else k_radix_small<K, false><<<1, RS_BLOCK, 0, C.stream>>>(kin, nullptr, (int)n, rp, seg, n_seg, kout, nullptr);
+        LAUNCH_CHECK();
         return;
     }
     const int64_t nblk = (n + RS_TILE - 1) / RS_TILE;

@@ -157,8 +157,11 @@ __global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* _
 // one thread per (active node, octant): body sub-range with that digit
 __global__ void k_split(const uint64_t* __restrict__ khi, const uint64_t* __restrict__ klo, int level, int64_t n_active,
                         const int64_t* __restrict__ act_first, const int64_t* __restrict__ act_count,
-                        int64_t* __restrict__ sub_first, int64_t* __restrict__ sub_count, int32_t* __restrict__ nonempty) {
+                        int64_t* __restrict__ sub_first, int64_t* __restrict__ sub_count, int32_t* __restrict__ nonempty,
+                        int32_t* __restrict__ split_flag) {
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t <= n_active * 8) split_flag[t] = 0;   // entries past the level's children stay 0 for the scan
+    if (t == n_active * 8) nonempty[t] = 0;     // so that the scan's last entry is the number of children
     if (t >= n_active * 8) return;
     int64_t a = t >> 3;
     int o = (int)(t & 7);
@@ -243,7 +246,7 @@ __global__ void k_next_active(int64_t n_children, const int32_t* __restrict__ sp
                               int64_t child_base, const Branch* __restrict__ br, int64_t* __restrict__ act_branch,
                               int64_t* __restrict__ act_first, int64_t* __restrict__ act_count) {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= n_children || !split_flag[c]) return;
+    if (c >= n_children || !split_flag[c]) return;   // n_children: a bound (flags past the children are 0)
     int64_t k = srank[c];
     const Branch& B = br[child_base + c];
     act_branch[k] = child_base + c;
@@ -444,6 +447,7 @@ struct TreeScratch {
     DBuf<uint64_t> khi, klo, khi_s, klo_s, key2, key2_s;
     DBuf<uint32_t> idx, idx_s;
     DBuf<int64_t> act_branch, act_first, act_count, sub_first, sub_count;
+    DBuf<int64_t> nact_branch, nact_first, nact_count;   // the next level's active branches
     DBuf<int32_t> nonempty, rank, split, srank, flag, frank, done;
     DBuf<int32_t> sys_prefix, sys_flag;   // several systems: per system, the prefix count over the key order
     DBuf<int64_t> sys_off, sys_ls;
@@ -541,24 +545,27 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, const std::vect
     }
     HIP_CHECK(hipStreamSynchronize(C.stream));   // rb and act are read by the copies
     int level = 0;
+    // One host read-back per level: the children of a level are at most 8 per active branch, so every
+    // buffer of the level is sized by that bound; the next level's active branches go to a second set of
+    // arrays, swapped in afterwards.
     while (n_active > 0) {
         if (level >= max_level) return false;
         const int64_t m = n_active * 8;
-        S.sub_first.resize(m); S.sub_count.resize(m); S.nonempty.resize(m); S.rank.resize(m + 1);
-        k_split<<<grid_for(m, 256), 256, 0, C.stream>>>(S.khi_s.data(), S.klo_s.data(), level, n_active, S.act_first.data(),
-                                                        S.act_count.data(), S.sub_first.data(), S.sub_count.data(), S.nonempty.data());
+        S.sub_first.resize(m); S.sub_count.resize(m); S.nonempty.resize(m + 1); S.rank.resize(m + 1);
+        S.split.resize(m + 1); S.srank.resize(m + 1);
+        k_split<<<grid_for(m + 1, 256), 256, 0, C.stream>>>(S.khi_s.data(), S.klo_s.data(), level, n_active, S.act_first.data(),
+                                                            S.act_count.data(), S.sub_first.data(), S.sub_count.data(),
+                                                            S.nonempty.data(), S.split.data());
         LAUNCH_CHECK();
-        exclusive_scan_i32(C, S.nonempty.data(), S.rank.data(), m);
-        const int64_t n_children = read_back_sum(C, S.rank.data() + (m - 1), S.nonempty.data() + (m - 1));
+        exclusive_scan_i32(C, S.nonempty.data(), S.rank.data(), m + 1);   // rank[m]: the level's children
         const int64_t child_base = n_branches;
-        if ((size_t)(n_branches + n_children) > T.branches.cap) {
+        if ((size_t)(n_branches + m) > T.branches.cap) {
             DBuf<Branch> bigger;
-            bigger.resize(2 * (n_branches + n_children));
+            bigger.resize(2 * (n_branches + m));
             HIP_CHECK(hipMemcpyAsync(bigger.data(), T.branches.data(), n_branches * sizeof(Branch), hipMemcpyDeviceToDevice, C.stream));
             std::swap(bigger.p, T.branches.p);
             std::swap(bigger.cap, T.branches.cap);
         }
-        S.split.resize(n_children + 1); S.srank.resize(n_children + 1);
         k_make_children<<<grid_for(m, 256), 256, 0, C.stream>>>(n_active, S.act_branch.data(), S.sub_first.data(),
                                                                 S.sub_count.data(), S.rank.data(), child_base, T.branches.data(),
                                                                 ls, method, S.split.data(), multi ? S.sys_prefix.data() : nullptr,
@@ -567,15 +574,19 @@ static bool build_structure(Context& C, Tree& T, TreeScratch& S, const std::vect
         k_link_parents<<<grid_for(n_active, 256), 256, 0, C.stream>>>(n_active, S.act_branch.data(), S.nonempty.data(),
                                                                       S.rank.data(), child_base, T.branches.data());
         LAUNCH_CHECK();
-        exclusive_scan_i32(C, S.split.data(), S.srank.data(), n_children);
-        const int64_t next_active = read_back_sum(C, S.srank.data() + (n_children - 1), S.split.data() + (n_children - 1));
-        S.act_branch.resize(std::max<int64_t>(1, next_active));
-        S.act_first.resize(std::max<int64_t>(1, next_active));
-        S.act_count.resize(std::max<int64_t>(1, next_active));
-        k_next_active<<<grid_for(n_children, 256), 256, 0, C.stream>>>(n_children, S.split.data(), S.srank.data(), child_base,
-                                                                      T.branches.data(), S.act_branch.data(), S.act_first.data(),
-                                                                      S.act_count.data());
+        exclusive_scan_i32(C, S.split.data(), S.srank.data(), m + 1);   // srank[m]: the next level's active branches
+        S.nact_branch.resize(m); S.nact_first.resize(m); S.nact_count.resize(m);
+        k_next_active<<<grid_for(m, 256), 256, 0, C.stream>>>(m, S.split.data(), S.srank.data(), child_base, T.branches.data(),
+                                                             S.nact_branch.data(), S.nact_first.data(), S.nact_count.data());
         LAUNCH_CHECK();
+        int32_t* h = (int32_t*)pinned_scratch(C);
+        HIP_CHECK(hipMemcpyAsync(h, S.rank.data() + m, sizeof(int32_t), hipMemcpyDeviceToHost, C.stream));
+        HIP_CHECK(hipMemcpyAsync(h + 1, S.srank.data() + m, sizeof(int32_t), hipMemcpyDeviceToHost, C.stream));
+        HIP_CHECK(hipStreamSynchronize(C.stream));
+        const int64_t n_children = h[0], next_active = h[1];
+        std::swap(S.act_branch.p, S.nact_branch.p); std::swap(S.act_branch.cap, S.nact_branch.cap);
+        std::swap(S.act_first.p, S.nact_first.p); std::swap(S.act_first.cap, S.nact_first.cap);
+        std::swap(S.act_count.p, S.nact_count.p); std::swap(S.act_count.cap, S.nact_count.cap);
         n_branches += n_children;
         T.level_first.push_back(child_base);
         T.level_last.push_back(child_base + n_children - 1);
