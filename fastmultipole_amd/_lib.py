@@ -13,11 +13,14 @@ LIB_PATH = os.path.join(_HERE, "libfmmhip.so")
 FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMHIP_EDEPTH = range(6)
 POINT_SOURCE, POINT_VORTEX, POINT_DIPOLE, VORTEX_GAUSSIAN = 0, 1, 2, 3
 PANEL_SOURCE_TRI, PANEL_DIPOLE_TRI, PANEL_SOURCE_QUAD, PANEL_DIPOLE_QUAD = 4, 5, 6, 7
+POINT_SOURCE_VORTEX = 8
 
 
 def data_per_body(kernel):
     """Rows per body of a kernel's source buffer: [x, y, z, radius, strength rows...]; panels add their
     vertices (3 or 4 x 3 rows)."""
+    if kernel == POINT_SOURCE_VORTEX:
+        return 8
     if kernel >= PANEL_SOURCE_TRI:
         return 5 + 3 * (4 if kernel >= PANEL_SOURCE_QUAD else 3)
     return 5 if kernel == POINT_SOURCE else 7

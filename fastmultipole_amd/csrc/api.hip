@@ -723,7 +723,7 @@ fmmhip_status fmmhip_eval_set_tuple(fmmhip_eval* ev, int64_t n_systems, const in
         int rows = 7;
         for (int64_t i = 0; i < n_systems; ++i) {
             const int k = kernels[i];
-            if (k < FMMHIP_POINT_SOURCE || k > FMMHIP_PANEL_DIPOLE_QUAD) throw Error(FMMHIP_EINVAL, "unknown kernel in a tuple");
+            if (!valid_kernel(k)) throw Error(FMMHIP_EINVAL, "unknown kernel in a tuple");
             if (counts[i] < 0) throw Error(FMMHIP_EINVAL, "negative system size");
             lh = lh || has_vector_potential(k);
             rows = std::max(rows, data_per_body(k));

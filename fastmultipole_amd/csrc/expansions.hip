@@ -2374,12 +2374,17 @@ void run_p2m(Context& C, int kernel, Tree& st, int p, const Branch* sbr) {
         return;
     }
     const size_t p2m_sh = sizeof(double) * (size_t)(64 / (p + 1)) * (2 * nh(p) + 3);
-#define FMMHIP_P2M(K)                                                                                                  \
+#define FMMHIP_P2M(K, Q)                                                                                               \
     k_p2m<K><<<(unsigned)st.n_leaves, 64, p2m_sh, C.stream>>>(sbr, st.leaf_index.data(), st.x.data(), st.y.data(),             \
-                                                        st.z.data(), st.strength.data(), st.n, st.expansions.data(), p, tb.inv_fact)
-    if (kernel == FMMHIP_POINT_SOURCE) FMMHIP_P2M(FMMHIP_POINT_SOURCE);
-    else if (kernel == FMMHIP_POINT_DIPOLE) FMMHIP_P2M(FMMHIP_POINT_DIPOLE);
-    else FMMHIP_P2M(FMMHIP_POINT_VORTEX);   // the Gaussian core changes only the nearfield
+                                                        st.z.data(), Q, st.n, st.expansions.data(), p, tb.inv_fact)
+    const double* q = st.strength.data();
+    if (kernel == FMMHIP_POINT_SOURCE) FMMHIP_P2M(FMMHIP_POINT_SOURCE, q);
+    else if (kernel == FMMHIP_POINT_DIPOLE) FMMHIP_P2M(FMMHIP_POINT_DIPOLE, q);
+    else if (kernel == FMMHIP_POINT_SOURCE_VORTEX) {   // the source's expansion (row 0), then the vortex's (rows 1-3)
+        FMMHIP_P2M(FMMHIP_POINT_SOURCE, q);
+        LAUNCH_CHECK();
+        FMMHIP_P2M(FMMHIP_POINT_VORTEX, q + st.n);
+    } else FMMHIP_P2M(FMMHIP_POINT_VORTEX, q);   // the Gaussian core changes only the nearfield
 #undef FMMHIP_P2M
     LAUNCH_CHECK();
 }

@@ -51,9 +51,9 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
 // sources per LDS slot: strengths (pre-scaled by 1/(4 pi)) and, for the Gaussian
 // core, sigma
 template <int K> struct KernelTraits {
-    static constexpr int NS = K == FMMHIP_POINT_SOURCE ? 1 : (K == FMMHIP_VORTEX_GAUSSIAN ? 4 : 3);
+    static constexpr int NS = K == FMMHIP_POINT_SOURCE ? 1 : (K == FMMHIP_VORTEX_GAUSSIAN || K == FMMHIP_POINT_SOURCE_VORTEX ? 4 : 3);
     static constexpr bool HESS = K != FMMHIP_POINT_SOURCE;   // the reference's source kernel has no Hessian term
-    static constexpr bool POT = K == FMMHIP_POINT_SOURCE || K == FMMHIP_POINT_DIPOLE;
+    static constexpr bool POT = K == FMMHIP_POINT_SOURCE || K == FMMHIP_POINT_DIPOLE || K == FMMHIP_POINT_SOURCE_VORTEX;
     // outputs [phi, gradient (3), Hessian (9)]; in FP64 the point vortex keeps three
     // more sums, sum_j Gamma_j / r_j^3, whose antisymmetric part of the Hessian is
     // added once per target (fold_acc) instead of once per pair. FP32 sums keep the
@@ -83,6 +83,30 @@ __device__ inline void pair_eval(R dx, R dy, R dz, R r2, R ri, const R* g, R* a)
         const R q = g[0];
         if (PS) a[0] += q * ri;
         if (GS) { const R f = q * ri * ri2; a[1] += dx * f; a[2] += dy * f; a[3] += dz * f; }
+    } else if (K == FMMHIP_POINT_SOURCE_VORTEX) {
+        // the point source's terms (strength g[0], with the gradient's derivative), then the singular vortex's (g[1..3])
+        const R q = g[0], gx = g[1], gy = g[2], gz = g[3];
+        const R d3 = ri * ri2;
+        if (PS) a[0] += q * ri;
+        const R cx = dz * gy - dy * gz, cy = dx * gz - dz * gx, cz = dy * gx - dx * gy;
+        if (GS) {
+            const R f = q * d3;
+            a[1] += dx * f; a[2] += dy * f; a[3] += dz * f;
+            a[1] += cx * d3; a[2] += cy * d3; a[3] += cz * d3;
+        }
+        if (HS) {
+            const R d5 = d3 * ri2, t3 = R(-3) * d5;
+            // the source's gradient derivative q (delta_ij / r^3 - 3 d_i d_j / r^5)
+            const R qd = q * d3, qx = q * t3 * dx, qy = q * t3 * dy, qz = q * t3 * dz;
+            a[4] += qx * dx + qd; a[5] += qx * dy;      a[6] += qx * dz;
+            a[7] += qy * dx;      a[8] += qy * dy + qd; a[9] += qy * dz;
+            a[10] += qz * dx;     a[11] += qz * dy;     a[12] += qz * dz + qd;
+            const R ax = t3 * dx, ay = t3 * dy, az = t3 * dz;
+            const R g3x = gx * d3, g3y = gy * d3, g3z = gz * d3;
+            a[4] += ax * cx;       a[5] += ax * cy + g3z; a[6] += ax * cz - g3y;
+            a[7] += ay * cx - g3z; a[8] += ay * cy;       a[9] += ay * cz + g3x;
+            a[10] += az * cx + g3y; a[11] += az * cy - g3x; a[12] += az * cz;
+        }
     } else if (K == FMMHIP_POINT_DIPOLE) {
         const R qx = g[0], qy = g[1], qz = g[2];
         const R qd = qx * dx + qy * dy + qz * dz;
@@ -169,6 +193,7 @@ __device__ inline void stage_source(const double* __restrict__ sq, const double*
     g[0] = (R)(sq[j] * INV4PI);
     if (KernelTraits<K>::NS >= 3) { g[stride] = (R)(sq[ns + j] * INV4PI); g[2 * stride] = (R)(sq[2 * ns + j] * INV4PI); }
     if (K == FMMHIP_VORTEX_GAUSSIAN) g[3 * stride] = (R)srad[j];
+    if (K == FMMHIP_POINT_SOURCE_VORTEX) g[3 * stride] = (R)(sq[3 * ns + j] * INV4PI);
 }
 
 // Source body at position f of the flattened stream of a target leaf's source
@@ -582,6 +607,9 @@ static void run_p2p_range(Context& C, int kernel, const Tree& tt, const Tree& st
         case FMMHIP_POINT_SOURCE: launch_p2p<FMMHIP_POINT_SOURCE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
         case FMMHIP_POINT_VORTEX: launch_p2p<FMMHIP_POINT_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
         case FMMHIP_POINT_DIPOLE: launch_p2p<FMMHIP_POINT_DIPOLE>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
+        case FMMHIP_POINT_SOURCE_VORTEX:
+            launch_p2p<FMMHIP_POINT_SOURCE_VORTEX>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv);
+            break;
         default: launch_p2p<FMMHIP_VORTEX_GAUSSIAN>(C, tt, st, L, tout, nt, PS, GS, HS, precision, b0, b1, slots, sv); break;
     }
 }
@@ -623,6 +651,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_all(const double* __restri
             lg[threadIdx.x][0] = (R)(b[4] * INV4PI);
             if (NS >= 3) { lg[threadIdx.x][1] = (R)(b[5] * INV4PI); lg[threadIdx.x][2] = (R)(b[6] * INV4PI); }
             if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) lg[threadIdx.x][3] = (R)b[3];
+            if (KERNEL == FMMHIP_POINT_SOURCE_VORTEX) lg[threadIdx.x][NS - 1] = (R)(b[7] * INV4PI);
         }
         __syncthreads();
         if (!active) continue;
@@ -669,6 +698,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_part(const double* __restr
             lg[threadIdx.x][0] = (R)(b[4] * INV4PI);
             if (NS >= 3) { lg[threadIdx.x][1] = (R)(b[5] * INV4PI); lg[threadIdx.x][2] = (R)(b[6] * INV4PI); }
             if (KERNEL == FMMHIP_VORTEX_GAUSSIAN) lg[threadIdx.x][3] = (R)b[3];
+            if (KERNEL == FMMHIP_POINT_SOURCE_VORTEX) lg[threadIdx.x][NS - 1] = (R)(b[7] * INV4PI);
         }
         __syncthreads();
         if (!active) continue;
@@ -794,6 +824,9 @@ void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int
             break;
         case FMMHIP_POINT_DIPOLE:
             launch_direct_all<FMMHIP_POINT_DIPOLE>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);
+            break;
+        case FMMHIP_POINT_SOURCE_VORTEX:
+            launch_direct_all<FMMHIP_POINT_SOURCE_VORTEX>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);
             break;
         default:
             launch_direct_all<FMMHIP_VORTEX_GAUSSIAN>(C, nt, t_xyz, ns, s_buf, tout_aos, PS, GS, HS, precision, part);

@@ -170,6 +170,39 @@ class GaussianVortexParticles(VortexParticles):
     fmm_kernel = L.VORTEX_GAUSSIAN
 
 
+class SourceVortexPoints(VortexParticles):
+    """Bodies carrying a point source and a point vortex (Point{SourceVortex}, src/bodytomultipole.jl:491-496):
+    position (N,3), source strength q (N,), vortex strength (N,3), sigma (N,). Expansions: the source's then the
+    vortex's; nearfield: the point source's terms plus the singular vortex's."""
+
+    fmm_kernel = L.POINT_SOURCE_VORTEX
+
+    def __init__(self, position, source_strength, strength, sigma=None):
+        super().__init__(position, strength, sigma)
+        self.source_strength = np.ascontiguousarray(source_strength, dtype=np.float64).reshape(-1)
+        self.scalar_potential = np.zeros(self.get_n_bodies())
+
+    def data_per_body(self):
+        return 8
+
+    def strength_dims(self):
+        return 4
+
+    def source_buffer(self):
+        """rows [x, y, z, sigma, q, Gamma]"""
+        return np.column_stack([self.position, self.sigma, self.source_strength, self.strength])
+
+    def buffer_to_target_system(self, target_buffer, switch):
+        n = self.get_n_bodies()
+        if switch.scalar_potential:
+            self.scalar_potential += target_buffer[:n, 3]
+        super().buffer_to_target_system(target_buffer, switch)
+
+    def reset(self):
+        super().reset()
+        self.scalar_potential[:] = 0.0
+
+
 class _Panels:
     """Constant-strength planar panels (test/bodytomultipole_test.jl:80-120 SourcePanels,
     :221-272 DipolePanels): vertices (N, 3|4, 3) counter-clockwise about the normal."""

@@ -28,6 +28,17 @@ def uniform_vortices(n, seed=1):
     return b
 
 
+def source_vortex_points(n, seed=6):
+    """Point{SourceVortex} bodies: positions U[0,1]^3, source strengths N(0,1)/n, vortex strengths as
+    uniform_vortices. (n, 8) buffer [x, y, z, sigma, q, Gamma]."""
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 8))
+    b[:, :3] = rng.random((n, 3))
+    b[:, 4] = rng.normal(size=n) / n
+    b[:, 5:8] = (2.0 * rng.random((n, 3)) - 1.0 - 0.5) * 2.0 / n
+    return b
+
+
 def vortex_ring(n, seed=2, radius=1.0, core_sigma=0.1, noise=1e-3):
     """Vortex ring of radius `radius`: angle U[0, 2pi), cross-section offsets N(0, sigma^2) in the
     radial and axial directions (a Gaussian core), vector strength tangential with magnitude

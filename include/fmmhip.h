@@ -58,6 +58,11 @@ typedef enum {
  *                    (src/bodytomultipole.jl:643-831); triangles: rows [centroid(3), radius,
  *                    strength, v1(3), v2(3), v3(3)] (14 rows); quadrilaterals add v4(3) (17 rows);
  *                    vertices counter-clockwise about the normal; FP64 nearfield in both precisions
+ *   POINT_SOURCE_VORTEX  Point{SourceVortex} (src/bodytomultipole.jl:491-496): a point source and a point
+ *                    vortex on one body; expansions the source's, then the vortex's; nearfield the
+ *                    POINT_SOURCE pair terms (potential, gradient, and the gradient's derivative, which
+ *                    the far field carries) plus the POINT_VORTEX ones (velocity, its gradient), each
+ *                    written as the switches ask; rows [x,y,z,sigma,q,Gx,Gy,Gz]
  * Results use the expansion evaluation's convention: slot "gradient" = -grad(phi) for the
  * scalar kernels, the velocity for the vortex kernels; slot "hessian" = d(gradient slot). */
 typedef enum {
@@ -68,7 +73,8 @@ typedef enum {
     FMMHIP_PANEL_SOURCE_TRI = 4,
     FMMHIP_PANEL_DIPOLE_TRI = 5,
     FMMHIP_PANEL_SOURCE_QUAD = 6,
-    FMMHIP_PANEL_DIPOLE_QUAD = 7
+    FMMHIP_PANEL_DIPOLE_QUAD = 7,
+    FMMHIP_POINT_SOURCE_VORTEX = 8
 } fmmhip_kernel;
 typedef enum { FMMHIP_FP64 = 0, FMMHIP_MIXED = 1 } fmmhip_precision;
 typedef enum { FMMHIP_BARBA = 0, FMMHIP_SELFTUNING = 1, FMMHIP_SELFTUNING_TREESTOP = 2 } fmmhip_list_method;

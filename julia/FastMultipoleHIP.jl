@@ -18,6 +18,7 @@ const libfmmhip = get(ENV, "FMMHIP_LIB", "libfmmhip.so")
 const FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMHIP_EDEPTH = Cint.(0:5)
 const FMMHIP_POINT_SOURCE, FMMHIP_POINT_VORTEX, FMMHIP_POINT_DIPOLE, FMMHIP_VORTEX_GAUSSIAN = Cint.(0:3)
 const FMMHIP_PANEL_SOURCE_TRI, FMMHIP_PANEL_DIPOLE_TRI, FMMHIP_PANEL_SOURCE_QUAD, FMMHIP_PANEL_DIPOLE_QUAD = Cint.(4:7)
+const FMMHIP_POINT_SOURCE_VORTEX = Cint(8)   # Point{SourceVortex}: rows [x, y, z, sigma, q, Gx, Gy, Gz]
 const FMMHIP_FP64, FMMHIP_MIXED = Cint(0), Cint(1)
 const FMMHIP_INFLUENCE_POTENTIAL, FMMHIP_INFLUENCE_NORMAL_GRADIENT = Cint(0), Cint(1)
 const FMMHIP_REQUEST_FMM, FMMHIP_REQUEST_SOLVE = Cint(0), Cint(1)

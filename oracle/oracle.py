@@ -24,6 +24,9 @@ KIND_VORTEX_GAUSS = 3   # Gaussian-core vortex (sigma = buffer row 4), stated in
 # constant-strength panels (test/panels.jl; P2M src/bodytomultipole.jl:643-831);
 # rows [centroid(3), radius, strength, v1(3), v2(3), v3(3), v4(3) for quadrilaterals]
 KIND_PANEL_SOURCE_TRI, KIND_PANEL_DIPOLE_TRI, KIND_PANEL_SOURCE_QUAD, KIND_PANEL_DIPOLE_QUAD = 4, 5, 6, 7
+# Point{SourceVortex} (src/bodytomultipole.jl:491-496): rows [x, y, z, sigma, q, Gx, Gy, Gz]; nearfield stated
+# in oracle_fmm.cpp (the point source's terms plus the singular vortex's)
+KIND_SOURCE_VORTEX = 8
 METHOD_BARBA, METHOD_SELFTUNING, METHOD_SELFTUNING_TREESTOP = 0, 1, 2
 ERR_NONE, ERR_ROTATED_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
 
@@ -129,6 +132,8 @@ def length_Hs(p):
 
 
 def data_per_body(kind):
+    if kind == KIND_SOURCE_VORTEX:
+        return 8
     if kind >= KIND_PANEL_SOURCE_TRI:
         return 5 + 3 * (4 if kind >= KIND_PANEL_SOURCE_QUAD else 3)
     return 5 if kind == KIND_SOURCE else 7
@@ -200,7 +205,7 @@ class Run:
         counts = np.array([len(d) for d in datas], dtype=np.int64)
         t = np.zeros((len(s), 16))
         t[:, :3] = s[:, :3]
-        self.kind = KIND_VORTEX if any(k in (KIND_VORTEX, KIND_VORTEX_GAUSS) for k in kinds) else KIND_DIPOLE
+        self.kind = KIND_VORTEX if any(k in (KIND_VORTEX, KIND_VORTEX_GAUSS, KIND_SOURCE_VORTEX) for k in kinds) else KIND_DIPOLE
         self.nt = self.ns = len(s)
         oi, od = _options(p, leaf_size_source, leaf_size_target, method, shrink_recenter, error_kind, relative, eps, BE,
                           scalar_potential, gradient, hessian, farfield, nearfield, self_induced, multipole_acceptance)

@@ -31,8 +31,10 @@ enum { METHOD_BARBA = 0, METHOD_SELFTUNING = 1, METHOD_SELFTUNING_TREESTOP = 2 }
 enum { KIND_SOURCE = 0, KIND_VORTEX = 1, KIND_DIPOLE = 2, KIND_VORTEX_GAUSS = 3,
        // constant-strength planar panels (test/panels.jl), triangles and quadrilaterals;
        // buffer rows [centroid(3), radius, strength, v1(3), v2(3), v3(3), v4(3) for quads]
-       KIND_PANEL_SOURCE_TRI = 4, KIND_PANEL_DIPOLE_TRI = 5, KIND_PANEL_SOURCE_QUAD = 6, KIND_PANEL_DIPOLE_QUAD = 7 };
-static bool is_panel(int kind) { return kind >= KIND_PANEL_SOURCE_TRI; }
+       KIND_PANEL_SOURCE_TRI = 4, KIND_PANEL_DIPOLE_TRI = 5, KIND_PANEL_SOURCE_QUAD = 6, KIND_PANEL_DIPOLE_QUAD = 7,
+       // Point{SourceVortex} (src/bodytomultipole.jl:491-496): rows [x, y, z, sigma, q, Gx, Gy, Gz]
+       KIND_SOURCE_VORTEX = 8 };
+static bool is_panel(int kind) { return kind >= KIND_PANEL_SOURCE_TRI && kind <= KIND_PANEL_DIPOLE_QUAD; }
 static int panel_vertices(int kind) { return kind >= KIND_PANEL_SOURCE_QUAD ? 4 : 3; }
 static bool panel_dipole(int kind) { return kind == KIND_PANEL_DIPOLE_TRI || kind == KIND_PANEL_DIPOLE_QUAD; }
 
@@ -462,6 +464,58 @@ static void direct_vortex(double* tbuf, int64_t t0, int64_t nt, const double* sb
     }
 }
 
+// Point{SourceVortex}: the reference expands it as a point source plus a point vortex
+// (src/bodytomultipole.jl:491-496) and holds no direct! for it; its nearfield here is, per source body, the
+// point source's terms (test/gravitational.jl:80-104: potential and gradient, plus the gradient's
+// derivative q (delta_ij / r^3 - 3 d_i d_j / r^5) / (4 pi), which the far field carries) then the singular
+// vortex's (test/vortex.jl:83-128: velocity and its gradient), each written as the switches ask.
+static void direct_source_vortex(double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
+                                 bool PS, bool GS, bool HS) {
+    for (int64_t js = s0; js < s0 + ns; ++js) {
+        const double* s = sbuf + (size_t)js * sr;
+        const double sx = s[0], sy = s[1], sz = s[2], q = s[4], Gx = s[5], Gy = s[6], Gz = s[7];
+        for (int64_t it = t0; it < t0 + nt; ++it) {
+            double* t = tbuf + (size_t)it * 16;
+            const double dx = t[0] - sx, dy = t[1] - sy, dz = t[2] - sz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 > 0)) continue;
+            const double r = std::sqrt(r2);
+            if (PS) t[3] += q / r * ONE_OVER_4PI;
+            const double r3 = r2 * r;
+            if (GS) {
+                t[4] += dx * q / r3 * ONE_OVER_4PI;
+                t[5] += dy * q / r3 * ONE_OVER_4PI;
+                t[6] += dz * q / r3 * ONE_OVER_4PI;
+            }
+            const double rinv = 1 / r, rinv2 = rinv * rinv;
+            double denom = ONE_OVER_4PI * rinv * rinv2;
+            if (GS) {
+                t[4] += (dz * Gy - dy * Gz) * denom;
+                t[5] += (dx * Gz - dz * Gx) * denom;
+                t[6] += (dy * Gx - dx * Gy) * denom;
+            }
+            if (HS) {
+                const double d[3] = {dx, dy, dz};
+                const double qs = q * ONE_OVER_4PI / (r3 * r2);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) t[7 + 3 * i + j] += qs * ((i == j ? r2 : 0.0) - 3 * d[i] * d[j]);
+                denom *= rinv2;
+                double h[9];
+                h[0] = -3 * dx * (Gy * dz - Gz * dy) * denom;
+                h[1] = (-3 * dx * (Gz * dx - Gx * dz) + Gz * r2) * denom;
+                h[2] = (-3 * dx * (Gx * dy - Gy * dx) - Gy * r2) * denom;
+                h[3] = (-3 * dy * (Gy * dz - Gz * dy) - Gz * r2) * denom;
+                h[4] = -3 * dy * (Gz * dx - Gx * dz) * denom;
+                h[5] = (-3 * dy * (Gx * dy - Gy * dx) + Gx * r2) * denom;
+                h[6] = (-3 * dz * (Gy * dz - Gz * dy) + Gy * r2) * denom;
+                h[7] = (-3 * dz * (Gz * dx - Gx * dz) - Gx * r2) * denom;
+                h[8] = -3 * dz * (Gx * dy - Gy * dx) * denom;
+                for (int k = 0; k < 9; ++k) t[7 + k] += h[k];
+            }
+        }
+    }
+}
+
 // Point dipole of moment q at x_s. The potential is the one the reference's
 // dipole multipole expansion represents (body_to_multipole_point!(Point{Dipole}),
 // src/bodytomultipole.jl:431-463; checked analytically at
@@ -734,13 +788,16 @@ static void user_direct(int kind, double* tbuf, int64_t t0, int64_t nt, const do
                         bool PS, bool GS, bool HS) {
     if (is_panel(kind)) { direct_panels(kind, tbuf, t0, nt, sbuf, sr, s0, ns, PS, GS, HS); return; }
     if (kind == KIND_SOURCE) direct_gravitational(tbuf, t0, nt, sbuf, sr, s0, ns);
+    else if (kind == KIND_SOURCE_VORTEX) direct_source_vortex(tbuf, t0, nt, sbuf, sr, s0, ns, PS, GS, HS);
     else if (kind == KIND_VORTEX) direct_vortex(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
     else if (kind == KIND_DIPOLE) direct_dipole(tbuf, t0, nt, sbuf, sr, s0, ns, PS, GS, HS);
     else direct_vortex_gauss(tbuf, t0, nt, sbuf, sr, s0, ns, GS, HS);
 }
 
-static int data_per_body(int kind) { return kind == KIND_SOURCE ? 5 : is_panel(kind) ? 5 + 3 * panel_vertices(kind) : 7; }
-static bool has_vector_potential(int kind) { return kind == KIND_VORTEX || kind == KIND_VORTEX_GAUSS; }
+static int data_per_body(int kind) {
+    return kind == KIND_SOURCE ? 5 : is_panel(kind) ? 5 + 3 * panel_vertices(kind) : kind == KIND_SOURCE_VORTEX ? 8 : 7;
+}
+static bool has_vector_potential(int kind) { return kind == KIND_VORTEX || kind == KIND_VORTEX_GAUSS || kind == KIND_SOURCE_VORTEX; }
 
 // ------------------------------------------------------------ fmm driver
 struct Options {
@@ -847,6 +904,10 @@ static void p2m_body(int kind, Exp M, Exp harm, const double* x, const double* c
     double dx[3] = {x[0] - center[0], x[1] - center[1], x[2] - center[2]};
     if (is_panel(kind)) p2m_panel(kind, M, harm, x, center, p);
     else if (kind == KIND_SOURCE) p2m_point_source(M, harm, dx, x[4], p);
+    else if (kind == KIND_SOURCE_VORTEX) {   // src/bodytomultipole.jl:491-496: the source, then the vortex
+        p2m_point_source(M, harm, dx, x[4], p);
+        p2m_point_vortex(M, harm, dx, x + 5, p);
+    }
     else if (kind == KIND_DIPOLE) p2m_point_dipole(M, harm, dx, x + 4, p);
     else p2m_point_vortex(M, harm, dx, x + 4, p);
 }

@@ -161,6 +161,9 @@ FMM_CASES = [
      dict(expansion_order=8, leaf_size_source=40, multipole_acceptance=0.5, scalar_potential=True)),
     ("plummer-30k-p16", lambda: W.plummer_sources(30000, seed=83), O.KIND_SOURCE,
      dict(expansion_order=16, leaf_size_source=40, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    # Point{SourceVortex}: a source and a vortex on every body
+    ("source-vortex-20k-p10", lambda: W.source_vortex_points(20000, seed=84), O.KIND_SOURCE_VORTEX,
+     dict(expansion_order=10, leaf_size_source=32, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
     # constant-strength panels tiling a sphere (quadrilaterals, and triangles from splitting them)
     ("panel-src-tri-3k-p8", lambda: W.sphere_panels(16, quads=False)[0], O.KIND_PANEL_SOURCE_TRI,
      dict(expansion_order=8, leaf_size_source=20, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
@@ -214,6 +217,8 @@ ERR_CASES = [
     # PowerRelativePotential: estimate_influence! keeps the potential per target (src/fmm.jl:1185-1187)
     ("laplace-power-potential-rel", lambda: W.uniform_sources(5000, seed=40), O.KIND_SOURCE,
      F.PowerRelativePotential(1e-7), 20),
+    ("source-vortex-power-rel", lambda: W.source_vortex_points(6000, seed=45), O.KIND_SOURCE_VORTEX,
+     F.PowerRelativeGradient(1e-7), 24),
     # tolerances the order cap of 20 cannot meet everywhere (error_success false, pmax 21): orders 19-20
     # come from the deferred second M2L launch, and capped interactions keep the full order
     ("ring-power-rel-1e-10-mac0.55", lambda: W.vortex_ring(8000, seed=38), O.KIND_VORTEX,
@@ -326,6 +331,7 @@ def test_direct_fp64_matches_oracle():
     for kind, sbuf in ((O.KIND_SOURCE, W.uniform_sources(3000, seed=41)), (O.KIND_VORTEX, W.vortex_ring(3000, seed=42)),
                        (O.KIND_DIPOLE, W.uniform_dipoles(3000, seed=43)),
                        (O.KIND_VORTEX_GAUSS, W.gaussian_vortex_ring(3000, seed=44)),
+                       (O.KIND_SOURCE_VORTEX, W.source_vortex_points(3000, seed=46)),
                        (O.KIND_PANEL_SOURCE_TRI, W.sphere_panels(10, quads=False)[0]),
                        (O.KIND_PANEL_DIPOLE_QUAD, W.sphere_panels(12)[0])):
         ctx = F.default_context()
@@ -381,8 +387,9 @@ def test_fmm_api_mirror_and_edge_cases():
 
 
 @pytest.mark.parametrize("kind,gen", [(O.KIND_DIPOLE, lambda: W.uniform_dipoles(20000, seed=51)),
-                                      (O.KIND_VORTEX_GAUSS, lambda: W.gaussian_vortex_ring(20000, seed=52))],
-                         ids=["dipole", "gauss-ring"])
+                                      (O.KIND_VORTEX_GAUSS, lambda: W.gaussian_vortex_ring(20000, seed=52)),
+                                      (O.KIND_SOURCE_VORTEX, lambda: W.source_vortex_points(20000, seed=53))],
+                         ids=["dipole", "gauss-ring", "source-vortex"])
 def test_mixed_precision_new_kernels(kind, gen):
     """Mixed form of the dipole and Gaussian-core kernels against the FP64 form of the same
     evaluation: the FP32 nearfield's error stays below 1e-4 (max) and 1e-6 (median) per target."""

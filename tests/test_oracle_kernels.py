@@ -211,3 +211,51 @@ def test_panel_fmm_converges_to_direct(kind):
     for lo, hi in ((3, 4), (4, 7), (7, 16)):
         err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / np.abs(ref[:, lo:hi]).max()
         assert err < 1e-6, (kind, lo, err)
+
+
+# ------------------------------------------------------------ Point{SourceVortex}
+def _source_vortex(n, seed):
+    rng = np.random.default_rng(seed)
+    d = np.zeros((n, 8))
+    d[:, :3] = rng.random((n, 3))
+    d[:, 4] = rng.normal(size=n) / n
+    d[:, 5:8] = rng.normal(size=(n, 3)) / n
+    return d
+
+
+def test_source_vortex_direct_is_source_plus_vortex():
+    """The Point{SourceVortex} nearfield: the point source's test kernel plus the singular vortex's, with the
+    source's Hessian (which the source test kernel leaves out) equal to central differences of its gradient."""
+    d = _source_vortex(400, 31)
+    xt = np.random.default_rng(32).random((300, 3))
+    got = O.direct(O.KIND_SOURCE_VORTEX, xt, d, scalar_potential=True, gradient=True, hessian=True)
+    src = O.direct(O.KIND_SOURCE, xt, d[:, :5], scalar_potential=True, gradient=True, hessian=True)
+    vor = O.direct(O.KIND_VORTEX, xt, np.column_stack([d[:, :4], d[:, 5:8]]), scalar_potential=True, gradient=True,
+                   hessian=True)
+    sc = np.abs(got[:, 3:7]).max()
+    np.testing.assert_allclose(got[:, 3:7], src[:, 3:7] + vor[:, 3:7], rtol=1e-13, atol=1e-13 * sc)
+    h_src = got[:, 7:16] - vor[:, 7:16]
+    h = 1e-6
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = h
+        gp = O.direct(O.KIND_SOURCE, xt + e, d[:, :5], gradient=True)[:, 4:7]
+        gm = O.direct(O.KIND_SOURCE, xt - e, d[:, :5], gradient=True)[:, 4:7]
+        # slot (i, j) holds d(gradient_j)/dx_i
+        np.testing.assert_allclose(h_src[:, 3 * a:3 * a + 3], (gp - gm) / (2 * h), rtol=1e-5,
+                                   atol=1e-6 * np.abs(h_src).max())
+
+
+def test_source_vortex_fmm_converges_to_direct():
+    """Expansions of Point{SourceVortex} (src/bodytomultipole.jl:491-496: the source's, then the vortex's)."""
+    d = _source_vortex(3000, 33)
+    run = O.Run(O.KIND_SOURCE_VORTEX, d[:, :3], d, p=16, leaf_size_source=12, multipole_acceptance=0.5,
+                scalar_potential=True, gradient=True, hessian=True).fmm()
+    assert len(run.list(0)) > 0
+    got = run.targets()
+    ref = O.direct(O.KIND_SOURCE_VORTEX, d[:, :3], d, scalar_potential=True, gradient=True, hessian=True)
+    # velocity and its gradient; the far field's scalar potential is not the source's once a vector potential
+    # is present (the reference warns of it, src/fmm.jl:783)
+    for lo, hi in ((4, 7), (7, 16)):
+        err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / np.abs(ref[:, lo:hi]).max()
+        assert err < 1e-6, (lo, err)
