@@ -10,12 +10,13 @@ from .core import (Context, Evaluation, DerivativesSwitch, ErrorMethod, Interact
                    RotatedCoefficientsAbsoluteGradient, RotatedCoefficientsRelativeGradient, Barba, SelfTuning,
                    SelfTuningTreeStop, default_context, direct, fmm, fmm_batch, make_options, FastGaussSeidel, Server,
                    solve, tune_fmm, autotune)
-from .systems import (DipolePanels, DipolePoints, GaussianVortexParticles, Gravitational, SourcePanels, SourceVortexPoints,
+from .systems import (DipolePanels, DipolePoints, GaussianVortexParticles, Gravitational, SourceDipolePanels, SourcePanels,
+                      SourceVortexPoints,
                       VortexParticles)
 
 __all__ = ["Context", "Evaluation", "DerivativesSwitch", "ErrorMethod", "InteractionListMethod",
            "PowerAbsoluteGradient", "PowerAbsolutePotential", "PowerRelativeGradient", "PowerRelativePotential",
            "RotatedCoefficientsAbsoluteGradient", "RotatedCoefficientsRelativeGradient", "Barba", "SelfTuning",
            "SelfTuningTreeStop", "default_context", "direct", "fmm", "fmm_batch", "make_options", "Gravitational",
-           "VortexParticles", "DipolePoints", "GaussianVortexParticles", "SourceVortexPoints", "SourcePanels", "DipolePanels",
+           "VortexParticles", "DipolePoints", "GaussianVortexParticles", "SourceVortexPoints", "SourcePanels", "DipolePanels", "SourceDipolePanels",
            "FastGaussSeidel", "Server", "solve", "tune_fmm", "autotune"]

@@ -14,6 +14,7 @@ FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMHIP_EDE
 POINT_SOURCE, POINT_VORTEX, POINT_DIPOLE, VORTEX_GAUSSIAN = 0, 1, 2, 3
 PANEL_SOURCE_TRI, PANEL_DIPOLE_TRI, PANEL_SOURCE_QUAD, PANEL_DIPOLE_QUAD = 4, 5, 6, 7
 POINT_SOURCE_VORTEX = 8
+PANEL_SOURCE_DIPOLE_TRI, PANEL_SOURCE_DIPOLE_QUAD = 9, 10
 
 
 def data_per_body(kernel):
@@ -21,6 +22,8 @@ def data_per_body(kernel):
     vertices (3 or 4 x 3 rows)."""
     if kernel == POINT_SOURCE_VORTEX:
         return 8
+    if kernel in (PANEL_SOURCE_DIPOLE_TRI, PANEL_SOURCE_DIPOLE_QUAD):
+        return 6 + 3 * (4 if kernel == PANEL_SOURCE_DIPOLE_QUAD else 3)
     if kernel >= PANEL_SOURCE_TRI:
         return 5 + 3 * (4 if kernel >= PANEL_SOURCE_QUAD else 3)
     return 5 if kernel == POINT_SOURCE else 7

@@ -2322,7 +2322,8 @@ __device__ void p2m_triangle(const double* v1, const double* v2, const double* v
 
 template <int KERNEL>
 __global__ __launch_bounds__(64) void k_p2m_panel(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
-                                                  const double* __restrict__ rows, int64_t ns, double* __restrict__ ex, int P) {
+                                                  const double* __restrict__ rows, int64_t ns, double* __restrict__ ex, int P,
+                                                  int srow, int voff) {
     extern __shared__ double lds[];
     constexpr int NV = panel_vertices(KERNEL);
     constexpr bool DIPOLE = panel_dipole(KERNEL);
@@ -2340,8 +2341,8 @@ __global__ __launch_bounds__(64) void k_p2m_panel(const Branch* __restrict__ br,
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int a = 0; a < 3; ++a) v[k][a] = rows[(int64_t)(1 + 3 * k + a) * ns + j];
-        const double q = rows[j];
+            for (int a = 0; a < 3; ++a) v[k][a] = rows[(int64_t)(voff + 3 * k + a) * ns + j];
+        const double q = rows[(int64_t)srow * ns + j];   // strength row srow, vertices from row voff
 #pragma unroll
         for (int t = 0; t < NV - 2; ++t)
             p2m_triangle<DIPOLE>(v[0], v[t + 1], v[t + 2], B.source_center, q, P, Q, J, I, Macc);
@@ -2360,14 +2361,24 @@ void run_p2m(Context& C, int kernel, Tree& st, int p, const Branch* sbr) {
     TablePtrs tb = ptrs(C);
     if (is_panel(kernel)) {
         const size_t sh = sizeof(double) * 8 * nh(p);
-#define FMMHIP_P2M_PANEL(K)                                                                                              \
+#define FMMHIP_P2M_PANEL(K, SROW, VOFF)                                                                                \
     k_p2m_panel<K><<<(unsigned)st.n_leaves, 64, sh, C.stream>>>(sbr, st.leaf_index.data(), st.strength.data(),             \
-                                                               st.n, st.expansions.data(), p)
+                                                               st.n, st.expansions.data(), p, SROW, VOFF)
         switch (kernel) {
-            case FMMHIP_PANEL_SOURCE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_TRI); break;
-            case FMMHIP_PANEL_DIPOLE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_TRI); break;
-            case FMMHIP_PANEL_SOURCE_QUAD: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_QUAD); break;
-            default: FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_QUAD); break;
+            case FMMHIP_PANEL_SOURCE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_TRI, 0, 1); break;
+            case FMMHIP_PANEL_DIPOLE_TRI: FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_TRI, 0, 1); break;
+            case FMMHIP_PANEL_SOURCE_QUAD: FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_QUAD, 0, 1); break;
+            case FMMHIP_PANEL_SOURCE_DIPOLE_TRI:   // the source's expansion, then the dipole's (src/bodytomultipole.jl:717-720)
+                FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_TRI, 0, 2);
+                LAUNCH_CHECK();
+                FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_TRI, 1, 2);
+                break;
+            case FMMHIP_PANEL_SOURCE_DIPOLE_QUAD:
+                FMMHIP_P2M_PANEL(FMMHIP_PANEL_SOURCE_QUAD, 0, 2);
+                LAUNCH_CHECK();
+                FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_QUAD, 1, 2);
+                break;
+            default: FMMHIP_P2M_PANEL(FMMHIP_PANEL_DIPOLE_QUAD, 0, 1); break;
         }
 #undef FMMHIP_P2M_PANEL
         LAUNCH_CHECK();

@@ -177,8 +177,8 @@ __global__ __launch_bounds__(256) void k_fgs_blocks(const Branch* __restrict__ b
                 if (r2 > 0) v = 1.0 / sqrt(r2) * FGS_INV4PI;
             } else {
                 double a[13] = {0.0};
-                if (influence == 0) panel_pair<(NV > 0 ? NV : 3), false, true, false, false>(tile[cl], x[it], y[it], z[it], a);
-                else panel_pair<(NV > 0 ? NV : 3), false, false, true, false>(tile[cl], x[it], y[it], z[it], a);
+                if (influence == 0) panel_pair<(NV > 0 ? NV : 3), false, true, false, false>(tile[cl], x[it], y[it], z[it], a, 1.0);
+                else panel_pair<(NV > 0 ? NV : 3), false, false, true, false>(tile[cl], x[it], y[it], z[it], a, 1.0);
                 v = influence == 0 ? a[0] : a[1] * nrm[3 * it] + a[2] * nrm[3 * it + 1] + a[3] * nrm[3 * it + 2];
             }
             M[c * rows + r] = v;

@@ -208,22 +208,29 @@ struct Eval {
     ~Eval();
 };
 
-__host__ __device__ constexpr bool is_panel(int kernel) {
-    return kernel >= FMMHIP_PANEL_SOURCE_TRI && kernel <= FMMHIP_PANEL_DIPOLE_QUAD;
+__host__ __device__ constexpr bool panel_source_dipole(int kernel) {
+    return kernel == FMMHIP_PANEL_SOURCE_DIPOLE_TRI || kernel == FMMHIP_PANEL_SOURCE_DIPOLE_QUAD;
 }
-__host__ __device__ constexpr int panel_vertices(int kernel) { return kernel >= FMMHIP_PANEL_SOURCE_QUAD ? 4 : 3; }
+__host__ __device__ constexpr bool is_panel(int kernel) {
+    return (kernel >= FMMHIP_PANEL_SOURCE_TRI && kernel <= FMMHIP_PANEL_DIPOLE_QUAD) || panel_source_dipole(kernel);
+}
+__host__ __device__ constexpr int panel_vertices(int kernel) {
+    return kernel == FMMHIP_PANEL_SOURCE_QUAD || kernel == FMMHIP_PANEL_DIPOLE_QUAD || kernel == FMMHIP_PANEL_SOURCE_DIPOLE_QUAD ? 4 : 3;
+}
 __host__ __device__ constexpr bool panel_dipole(int kernel) {
     return kernel == FMMHIP_PANEL_DIPOLE_TRI || kernel == FMMHIP_PANEL_DIPOLE_QUAD;
 }
+// strength rows of a panel ahead of its vertices
+__host__ __device__ constexpr int panel_strengths(int kernel) { return panel_source_dipole(kernel) ? 2 : 1; }
 __host__ __device__ inline int data_per_body(int kernel) {
     return kernel == FMMHIP_POINT_SOURCE ? 5
-         : is_panel(kernel)                    ? 5 + 3 * panel_vertices(kernel)
+         : is_panel(kernel)                    ? 4 + panel_strengths(kernel) + 3 * panel_vertices(kernel)
          : kernel == FMMHIP_POINT_SOURCE_VORTEX ? 8
                                                 : 7;
 }
 // rows after [x, y, z, radius] kept with every sorted body: strengths (and panel vertices)
 inline int strength_dims(int kernel) { return kernel == FMMHIP_POINT_SOURCE ? 1 : data_per_body(kernel) - 4; }
-inline bool valid_kernel(int kernel) { return kernel >= FMMHIP_POINT_SOURCE && kernel <= FMMHIP_POINT_SOURCE_VORTEX; }
+inline bool valid_kernel(int kernel) { return kernel >= FMMHIP_POINT_SOURCE && kernel <= FMMHIP_PANEL_SOURCE_DIPOLE_QUAD; }
 // has_vector_potential (test/vortex.jl:145-147): Lamb-Helmholtz chi expansions
 inline bool has_vector_potential(int kernel) {
     return kernel == FMMHIP_POINT_VORTEX || kernel == FMMHIP_VORTEX_GAUSSIAN || kernel == FMMHIP_POINT_SOURCE_VORTEX;

@@ -841,7 +841,7 @@ void run_direct_all(Context& C, int kernel, int64_t nt, const double* t_xyz, int
 namespace fmmhip {
 
 
-template <int NV, bool DIPOLE, bool PS, bool GS, bool HS>
+template <int NV, int MODE, bool PS, bool GS, bool HS>
 __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restrict__ dir_targets,
                                                          const int64_t* __restrict__ dir_offsets,
                                                          const int32_t* __restrict__ dir_s, const int64_t* __restrict__ dir_cum,
@@ -870,10 +870,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
         for (int64_t f0 = base; f0 < end; f0 += PANEL_TILE) {
             const int64_t cnt = std::min<int64_t>(PANEL_TILE, end - f0);
             __syncthreads();
-            if (tid < cnt) stage_panel<NV>(sx, sy, sz, rows, ns, stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid), tile[tid]);
+            if (tid < cnt)
+                stage_panel<NV, MODE == 2 ? 2 : 1>(sx, sy, sz, rows, ns, stream_body(dir_s, dir_cum, sbr, l0, l1, f0 + tid), tile[tid]);
             __syncthreads();
             if (active)
-                for (int s = lane_k; s < cnt; s += tpt) panel_pair<NV, DIPOLE, PS, GS, HS>(tile[s], xt, yt, zt, a);
+                for (int s = lane_k; s < cnt; s += tpt) panel_pair_kind<NV, MODE, PS, GS, HS>(tile[s], xt, yt, zt, a);
         }
 #pragma unroll
         for (int k = 0; k < 13; ++k) a[k] = group_fold(a[k], LL);
@@ -885,7 +886,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_p2p_panel(const int32_t* __restri
     }
 }
 
-template <int NV, bool DIPOLE>
+template <int NV, int MODE>
 static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
                              bool GS, bool HS, int64_t b0, int64_t b1, SourceView sv) {
     const int64_t nblk = b1 - b0;
@@ -898,13 +899,13 @@ static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const L
     const int code = (PS ? 4 : 0) | (GS ? 2 : 0) | (HS ? 1 : 0);
     switch (code) {
         case 0: break;
-        case 1: k_p2p_panel<NV, DIPOLE, false, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
-        case 2: k_p2p_panel<NV, DIPOLE, false, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
-        case 3: k_p2p_panel<NV, DIPOLE, false, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
-        case 4: k_p2p_panel<NV, DIPOLE, true, false, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
-        case 5: k_p2p_panel<NV, DIPOLE, true, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
-        case 6: k_p2p_panel<NV, DIPOLE, true, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
-        default: k_p2p_panel<NV, DIPOLE, true, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 1: k_p2p_panel<NV, MODE, false, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 2: k_p2p_panel<NV, MODE, false, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 3: k_p2p_panel<NV, MODE, false, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 4: k_p2p_panel<NV, MODE, true, false, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 5: k_p2p_panel<NV, MODE, true, false, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        case 6: k_p2p_panel<NV, MODE, true, true, false><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
+        default: k_p2p_panel<NV, MODE, true, true, true><<<(unsigned)nblk, P2P_BLOCK, 0, C.stream>>>(PANEL_ARGS); break;
     }
 #undef PANEL_ARGS
     LAUNCH_CHECK();
@@ -913,20 +914,23 @@ static void launch_p2p_panel(Context& C, const Tree& tt, const Tree& st, const L
 void run_p2p_panels(Context& C, int kernel, const Tree& tt, const Tree& st, const Lists& L, double* tout, int64_t nt, bool PS,
                     bool GS, bool HS, int64_t b0, int64_t b1, SourceView sv) {
     switch (kernel) {
-        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
-        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
-        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, false>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
-        default: launch_p2p_panel<4, true>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_SOURCE_TRI: launch_p2p_panel<3, 0>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_DIPOLE_TRI: launch_p2p_panel<3, 1>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_SOURCE_QUAD: launch_p2p_panel<4, 0>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_SOURCE_DIPOLE_TRI: launch_p2p_panel<3, 2>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        case FMMHIP_PANEL_SOURCE_DIPOLE_QUAD: launch_p2p_panel<4, 2>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
+        default: launch_p2p_panel<4, 1>(C, tt, st, L, tout, nt, PS, GS, HS, b0, b1, sv); break;
     }
 }
 
 // direct! for panels: original-order AoS inputs, 16 x nt AoS output (+=)
-template <int NV, bool DIPOLE>
+template <int NV, int MODE>
 __global__ __launch_bounds__(P2P_BLOCK) void k_direct_panels(const double* __restrict__ txyz, int64_t nt,
                                                              const double* __restrict__ sbuf, int64_t ns,
                                                              double* __restrict__ out, int PS, int GS, int HS) {
     __shared__ PanelStage<NV> tile[PANEL_TILE];
-    constexpr int rows = 5 + 3 * NV;
+    constexpr int NSTR = MODE == 2 ? 2 : 1;
+    constexpr int rows = 4 + NSTR + 3 * NV;
     const int64_t it = blockIdx.x * (int64_t)P2P_BLOCK + threadIdx.x;
     const bool active = it < nt;
     const double xt = active ? txyz[3 * it] : 0.0, yt = active ? txyz[3 * it + 1] : 0.0, zt = active ? txyz[3 * it + 2] : 0.0;
@@ -937,11 +941,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void k_direct_panels(const double* __res
         __syncthreads();
         if ((int64_t)threadIdx.x < cnt) {
             const double* b = sbuf + (s0 + threadIdx.x) * rows;   // one panel's row as a 1-body SoA
-            stage_panel<NV>(b, b + 1, b + 2, b + 4, 1, 0, tile[threadIdx.x]);
+            stage_panel<NV, NSTR>(b, b + 1, b + 2, b + 4, 1, 0, tile[threadIdx.x]);
         }
         __syncthreads();
         if (!active) continue;
-        for (int s = 0; s < cnt; ++s) panel_pair<NV, DIPOLE, true, true, true>(tile[s], xt, yt, zt, a);
+        for (int s = 0; s < cnt; ++s) panel_pair_kind<NV, MODE, true, true, true>(tile[s], xt, yt, zt, a);
     }
     if (!active) return;
     double* o = out + it * 16;
@@ -954,10 +958,16 @@ void run_direct_panels(Context& C, int kernel, int64_t nt, const double* t_xyz, 
                        double* tout_aos, bool PS, bool GS, bool HS) {
     const unsigned grid = (unsigned)((nt + P2P_BLOCK - 1) / P2P_BLOCK);
     switch (kernel) {
-        case FMMHIP_PANEL_SOURCE_TRI: k_direct_panels<3, false><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
-        case FMMHIP_PANEL_DIPOLE_TRI: k_direct_panels<3, true><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
-        case FMMHIP_PANEL_SOURCE_QUAD: k_direct_panels<4, false><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
-        default: k_direct_panels<4, true><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        case FMMHIP_PANEL_SOURCE_TRI: k_direct_panels<3, 0><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        case FMMHIP_PANEL_DIPOLE_TRI: k_direct_panels<3, 1><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        case FMMHIP_PANEL_SOURCE_QUAD: k_direct_panels<4, 0><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
+        case FMMHIP_PANEL_SOURCE_DIPOLE_TRI:
+            k_direct_panels<3, 2><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS);
+            break;
+        case FMMHIP_PANEL_SOURCE_DIPOLE_QUAD:
+            k_direct_panels<4, 2><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS);
+            break;
+        default: k_direct_panels<4, 1><<<grid, P2P_BLOCK, 0, C.stream>>>(t_xyz, nt, s_buf, ns, tout_aos, PS, GS, HS); break;
     }
     LAUNCH_CHECK();
 }

@@ -21,19 +21,21 @@ template <int NV> struct PanelStage {
     double vx[NV + 1], vy[NV + 1];      // vertices in the panel frame (closed polygon)
     double dx[NV], dy[NV], ds[NV], ms[NV];
     double strength;
+    double strength2;   // a source-dipole panel's dipole strength
 };
 
-// panel j of the sorted arrays (rows after radius: strength, v1, v2, ...) into a stage
-template <int NV>
+// panel j of the sorted arrays (rows after radius: NSTR strengths, v1, v2, ...) into a stage
+template <int NV, int NSTR = 1>
 __device__ inline void stage_panel(const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
                                    const double* __restrict__ rows, int64_t n, int64_t j, PanelStage<NV>& P) {
     double v[NV][3];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[k][a] = rows[(int64_t)(1 + 3 * k + a) * n + j];
+        for (int a = 0; a < 3; ++a) v[k][a] = rows[(int64_t)(NSTR + 3 * k + a) * n + j];
     P.c[0] = px[j]; P.c[1] = py[j]; P.c[2] = pz[j];
     P.strength = rows[j];
+    P.strength2 = NSTR > 1 ? rows[n + j] : 0.0;
     // unit normal from the first three vertices (get_normal, src/compatibility.jl:65-72)
     const double a0 = v[1][0] - v[0][0], a1 = v[1][1] - v[0][1], a2 = v[1][2] - v[0][2];
     const double b0 = v[2][0] - v[0][0], b1 = v[2][1] - v[0][1], b2 = v[2][2] - v[0][2];
@@ -63,9 +65,9 @@ __device__ inline void stage_panel(const double* __restrict__ px, const double* 
     }
 }
 
-// one (target, panel) pair; accumulates [phi, velocity slot (3), Hessian slot (9)] into a
+// one (target, panel) pair of strength st; accumulates [phi, velocity slot (3), Hessian slot (9)] into a
 template <int NV, bool DIPOLE, bool PS, bool GS, bool HS>
-__device__ inline void panel_pair(const PanelStage<NV>& P, double tx, double ty, double tz, double* a) {
+__device__ inline void panel_pair(const PanelStage<NV>& P, double tx, double ty, double tz, double* a, double st) {
     const double EPS = 2.220446049250313e-16;   // the reference's eps() perturbation
     const double d0 = tx - P.c[0], d1 = ty - P.c[1], d2 = tz - P.c[2];
     const double tRx = P.ex[0] * d0 + P.ex[1] * d1 + P.ex[2] * d2;
@@ -140,7 +142,7 @@ __device__ inline void panel_pair(const PanelStage<NV>& P, double tx, double ty,
             }
         }
     }
-    const double s4 = P.strength * PANEL_INV4PI;
+    const double s4 = st * PANEL_INV4PI;
     if (PS) a[0] += pot * s4;
     if (GS) {   // velocity = (+-) s/(4 pi) R u, R = [ex ey ez] as columns
         const double sv = DIPOLE ? s4 : -s4;
@@ -160,6 +162,15 @@ __device__ inline void panel_pair(const PanelStage<NV>& P, double tx, double ty,
 #pragma unroll
             for (int c = 0; c < 3; ++c) a[4 + 3 * r + c] += -s4 * (RH[r][0] * E[0][c] + RH[r][1] * E[1][c] + RH[r][2] * E[2][c]);
     }
+}
+
+// one (target, panel) pair of a panel kind: MODE 0 a source panel, 1 a dipole panel, 2 a source-dipole panel
+// (the source's closed form with the first strength, then the dipole's with the second, test/panels.jl:323-429)
+template <int NV, int MODE, bool PS, bool GS, bool HS>
+__device__ inline void panel_pair_kind(const PanelStage<NV>& P, double tx, double ty, double tz, double* a) {
+    if (MODE == 1) panel_pair<NV, true, PS, GS, HS>(P, tx, ty, tz, a, P.strength);
+    else panel_pair<NV, false, PS, GS, HS>(P, tx, ty, tz, a, P.strength);
+    if (MODE == 2) panel_pair<NV, true, PS, GS, HS>(P, tx, ty, tz, a, P.strength2);
 }
 
 }  // namespace fmmhip

@@ -275,3 +275,29 @@ class DipolePanels(_Panels):
     @property
     def fmm_kernel(self):
         return L.PANEL_DIPOLE_QUAD if self.nv == 4 else L.PANEL_DIPOLE_TRI
+
+
+class SourceDipolePanels(_Panels):
+    """Panel{SourceDipole} (test/panels.jl:323-429): a constant source density and a constant normal-doublet
+    density on each triangle / planar quadrilateral; strength (N, 2) = [source, dipole]."""
+
+    def __init__(self, vertices, strength):
+        super().__init__(vertices, np.zeros(np.asarray(vertices).shape[0]))
+        self.strength = np.ascontiguousarray(strength, dtype=np.float64).reshape(-1, 2)
+
+    @property
+    def fmm_kernel(self):
+        return L.PANEL_SOURCE_DIPOLE_QUAD if self.nv == 4 else L.PANEL_SOURCE_DIPOLE_TRI
+
+    def data_per_body(self):
+        return 6 + 3 * self.nv
+
+    def strength_dims(self):
+        return 2
+
+    def source_buffer(self):
+        """rows [centroid(3), radius, source strength, dipole strength, vertices]"""
+        from .workloads import panel_rows
+        rows = panel_rows(self.vertices, self.strength[:, 0])
+        return np.ascontiguousarray(np.insert(rows, 5, self.strength[:, 1], axis=1))
+

@@ -116,6 +116,16 @@ def panel_rows(vertices, strength):
     return np.column_stack([cen, rad, np.asarray(strength, dtype=np.float64), v.reshape(len(v), -1)])
 
 
+def sphere_source_dipole_panels(m, quads=True, seed=5, radius=1.0):
+    """sphere_panels with a dipole strength N(0,1)/n beside each source strength (Panel{SourceDipole} rows
+    [centroid(3), radius, source, dipole, vertices]). Returns (rows, kind)."""
+    from . import _lib as L
+    rows, _ = sphere_panels(m, quads=quads, seed=seed, radius=radius)
+    dip = np.random.default_rng(seed + 1000).normal(size=len(rows)) / len(rows)
+    kind = L.PANEL_SOURCE_DIPOLE_QUAD if quads else L.PANEL_SOURCE_DIPOLE_TRI
+    return np.ascontiguousarray(np.insert(rows, 5, dip, axis=1)), kind
+
+
 def sphere_panels(m, quads=True, seed=5, radius=1.0):
     """Panels tiling a sphere: each cube face split into an m x m grid projected onto the sphere.
     Quadrilaterals are made planar (the 4th vertex is projected onto the plane of the first three);

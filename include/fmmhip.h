@@ -63,6 +63,10 @@ typedef enum {
  *                    POINT_SOURCE pair terms (potential, gradient, and the gradient's derivative, which
  *                    the far field carries) plus the POINT_VORTEX ones (velocity, its gradient), each
  *                    written as the switches ask; rows [x,y,z,sigma,q,Gx,Gy,Gz]
+ *   PANEL_SOURCE_DIPOLE_*  Panel{SourceDipole}: a source and a dipole strength on one panel; expansions the
+ *                    source's then the dipole's (src/bodytomultipole.jl:717-720); nearfield the two closed
+ *                    forms of test/panels.jl:323-429; rows [centroid(3), radius, source strength, dipole
+ *                    strength, v1(3), v2(3), v3(3)] (15 rows), quadrilaterals add v4(3) (18 rows)
  * Results use the expansion evaluation's convention: slot "gradient" = -grad(phi) for the
  * scalar kernels, the velocity for the vortex kernels; slot "hessian" = d(gradient slot). */
 typedef enum {
@@ -74,7 +78,9 @@ typedef enum {
     FMMHIP_PANEL_DIPOLE_TRI = 5,
     FMMHIP_PANEL_SOURCE_QUAD = 6,
     FMMHIP_PANEL_DIPOLE_QUAD = 7,
-    FMMHIP_POINT_SOURCE_VORTEX = 8
+    FMMHIP_POINT_SOURCE_VORTEX = 8,
+    FMMHIP_PANEL_SOURCE_DIPOLE_TRI = 9,
+    FMMHIP_PANEL_SOURCE_DIPOLE_QUAD = 10
 } fmmhip_kernel;
 typedef enum { FMMHIP_FP64 = 0, FMMHIP_MIXED = 1 } fmmhip_precision;
 typedef enum { FMMHIP_BARBA = 0, FMMHIP_SELFTUNING = 1, FMMHIP_SELFTUNING_TREESTOP = 2 } fmmhip_list_method;

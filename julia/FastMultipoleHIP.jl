@@ -19,6 +19,8 @@ const FMMHIP_OK, FMMHIP_EINVAL, FMMHIP_ENOMEM, FMMHIP_EDEVICE, FMMHIP_ETOL, FMMH
 const FMMHIP_POINT_SOURCE, FMMHIP_POINT_VORTEX, FMMHIP_POINT_DIPOLE, FMMHIP_VORTEX_GAUSSIAN = Cint.(0:3)
 const FMMHIP_PANEL_SOURCE_TRI, FMMHIP_PANEL_DIPOLE_TRI, FMMHIP_PANEL_SOURCE_QUAD, FMMHIP_PANEL_DIPOLE_QUAD = Cint.(4:7)
 const FMMHIP_POINT_SOURCE_VORTEX = Cint(8)   # Point{SourceVortex}: rows [x, y, z, sigma, q, Gx, Gy, Gz]
+# Panel{SourceDipole}: rows [centroid(3), radius, source strength, dipole strength, vertices]
+const FMMHIP_PANEL_SOURCE_DIPOLE_TRI, FMMHIP_PANEL_SOURCE_DIPOLE_QUAD = Cint(9), Cint(10)
 const FMMHIP_FP64, FMMHIP_MIXED = Cint(0), Cint(1)
 const FMMHIP_INFLUENCE_POTENTIAL, FMMHIP_INFLUENCE_NORMAL_GRADIENT = Cint(0), Cint(1)
 const FMMHIP_REQUEST_FMM, FMMHIP_REQUEST_SOLVE = Cint(0), Cint(1)

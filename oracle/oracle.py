@@ -27,6 +27,8 @@ KIND_PANEL_SOURCE_TRI, KIND_PANEL_DIPOLE_TRI, KIND_PANEL_SOURCE_QUAD, KIND_PANEL
 # Point{SourceVortex} (src/bodytomultipole.jl:491-496): rows [x, y, z, sigma, q, Gx, Gy, Gz]; nearfield stated
 # in oracle_fmm.cpp (the point source's terms plus the singular vortex's)
 KIND_SOURCE_VORTEX = 8
+# Panel{SourceDipole} (test/panels.jl:323-429): rows [centroid(3), radius, source strength, dipole strength, vertices]
+KIND_PANEL_SOURCE_DIPOLE_TRI, KIND_PANEL_SOURCE_DIPOLE_QUAD = 9, 10
 METHOD_BARBA, METHOD_SELFTUNING, METHOD_SELFTUNING_TREESTOP = 0, 1, 2
 ERR_NONE, ERR_ROTATED_GRADIENT, ERR_POWER_POTENTIAL, ERR_POWER_GRADIENT = 0, 1, 2, 3
 
@@ -134,6 +136,8 @@ def length_Hs(p):
 def data_per_body(kind):
     if kind == KIND_SOURCE_VORTEX:
         return 8
+    if kind in (KIND_PANEL_SOURCE_DIPOLE_TRI, KIND_PANEL_SOURCE_DIPOLE_QUAD):
+        return 6 + 3 * (4 if kind == KIND_PANEL_SOURCE_DIPOLE_QUAD else 3)
     if kind >= KIND_PANEL_SOURCE_TRI:
         return 5 + 3 * (4 if kind >= KIND_PANEL_SOURCE_QUAD else 3)
     return 5 if kind == KIND_SOURCE else 7

@@ -33,10 +33,19 @@ enum { KIND_SOURCE = 0, KIND_VORTEX = 1, KIND_DIPOLE = 2, KIND_VORTEX_GAUSS = 3,
        // buffer rows [centroid(3), radius, strength, v1(3), v2(3), v3(3), v4(3) for quads]
        KIND_PANEL_SOURCE_TRI = 4, KIND_PANEL_DIPOLE_TRI = 5, KIND_PANEL_SOURCE_QUAD = 6, KIND_PANEL_DIPOLE_QUAD = 7,
        // Point{SourceVortex} (src/bodytomultipole.jl:491-496): rows [x, y, z, sigma, q, Gx, Gy, Gz]
-       KIND_SOURCE_VORTEX = 8 };
-static bool is_panel(int kind) { return kind >= KIND_PANEL_SOURCE_TRI && kind <= KIND_PANEL_DIPOLE_QUAD; }
-static int panel_vertices(int kind) { return kind >= KIND_PANEL_SOURCE_QUAD ? 4 : 3; }
+       KIND_SOURCE_VORTEX = 8,
+       // Panel{SourceDipole} (test/panels.jl:323-429; src/bodytomultipole.jl:717-720): rows [centroid(3), radius,
+       // source strength, dipole strength, v1(3), v2(3), v3(3), v4(3) for quads]
+       KIND_PANEL_SOURCE_DIPOLE_TRI = 9, KIND_PANEL_SOURCE_DIPOLE_QUAD = 10 };
+static bool panel_source_dipole(int kind) { return kind == KIND_PANEL_SOURCE_DIPOLE_TRI || kind == KIND_PANEL_SOURCE_DIPOLE_QUAD; }
+static bool is_panel(int kind) {
+    return (kind >= KIND_PANEL_SOURCE_TRI && kind <= KIND_PANEL_DIPOLE_QUAD) || panel_source_dipole(kind);
+}
+static int panel_vertices(int kind) {
+    return kind == KIND_PANEL_SOURCE_QUAD || kind == KIND_PANEL_DIPOLE_QUAD || kind == KIND_PANEL_SOURCE_DIPOLE_QUAD ? 4 : 3;
+}
 static bool panel_dipole(int kind) { return kind == KIND_PANEL_DIPOLE_TRI || kind == KIND_PANEL_DIPOLE_QUAD; }
+static int panel_strengths(int kind) { return panel_source_dipole(kind) ? 2 : 1; }   // rows ahead of the vertices
 
 struct OTree {
     std::vector<OBranch> branches;
@@ -739,21 +748,27 @@ static void panel_normal(const double (*v)[3], double* n) {
 
 static void direct_panels(int kind, double* tbuf, int64_t t0, int64_t nt, const double* sbuf, int sr, int64_t s0, int64_t ns,
                           bool PS, bool GS, bool HS) {
-    const int NV = panel_vertices(kind);
+    const int NV = panel_vertices(kind), vo = 4 + panel_strengths(kind);
     for (int64_t js = s0; js < s0 + ns; ++js) {
         const double* s = sbuf + (size_t)js * sr;
         double v[4][3];
         for (int k = 0; k < NV; ++k)
-            for (int a = 0; a < 3; ++a) v[k][a] = s[5 + 3 * k + a];
+            for (int a = 0; a < 3; ++a) v[k][a] = s[vo + 3 * k + a];
         double n[3];
         panel_normal(v, n);
         for (int64_t it = t0; it < t0 + nt; ++it) {
             double* t = tbuf + (size_t)it * 16;
             double phi, vel[3], H[9];
-            panel_induced(t, v, NV, n, s[4], s, panel_dipole(kind), PS, GS, HS, phi, vel, H);
-            if (PS) t[3] += phi;
-            if (GS) for (int a = 0; a < 3; ++a) t[4 + a] += vel[a];
-            if (HS) for (int k = 0; k < 9; ++k) t[7 + k] += H[k];
+            // a source-dipole panel is the source's closed form with strength 1 plus the dipole's with strength 2,
+            // as test/panels.jl:323-429 sums them edge by edge
+            const int n_parts = panel_source_dipole(kind) ? 2 : 1;
+            for (int part = 0; part < n_parts; ++part) {
+                const bool dip = panel_source_dipole(kind) ? part == 1 : panel_dipole(kind);
+                panel_induced(t, v, NV, n, s[4 + part], s, dip, PS, GS, HS, phi, vel, H);
+                if (PS) t[3] += phi;
+                if (GS) for (int a = 0; a < 3; ++a) t[4 + a] += vel[a];
+                if (HS) for (int k = 0; k < 9; ++k) t[7 + k] += H[k];
+            }
         }
     }
 }
@@ -763,10 +778,10 @@ static void direct_panels(int kind, double* tbuf, int64_t t0, int64_t nt, const 
 // triangle's normal unnormalised (:705 discards the division); here it is normalised, so that the
 // dipole expansion of a quadrilateral matches its induced() field.
 static void p2m_panel(int kind, Exp M, Exp harm, const double* s, const double* center, int p) {
-    const int NV = panel_vertices(kind);
+    const int NV = panel_vertices(kind), vo = 4 + panel_strengths(kind);
     double v[4][3];
     for (int k = 0; k < NV; ++k)
-        for (int a = 0; a < 3; ++a) v[k][a] = s[5 + 3 * k + a];
+        for (int a = 0; a < 3; ++a) v[k][a] = s[vo + 3 * k + a];
     for (int tri = 0; tri < NV - 2; ++tri) {
         const double* a = v[0];
         const double* b = v[tri + 1];
@@ -775,11 +790,11 @@ static void p2m_panel(int kind, Exp M, Exp harm, const double* s, const double* 
         const double xu[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
         const double xv[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
         if (!panel_dipole(kind)) p2m_panel_source(M, harm, x0, xu, xv, s[4], p);
-        else {
+        if (panel_dipole(kind) || panel_source_dipole(kind)) {   // Panel{SourceDipole}: the source's, then the dipole's
             const double vv[3][3] = {{a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}};
             double n[3];
             panel_normal(vv, n);
-            p2m_panel_dipole(M, harm, x0, xu, xv, n, s[4], p);
+            p2m_panel_dipole(M, harm, x0, xu, xv, n, panel_source_dipole(kind) ? s[5] : s[4], p);
         }
     }
 }
@@ -795,7 +810,8 @@ static void user_direct(int kind, double* tbuf, int64_t t0, int64_t nt, const do
 }
 
 static int data_per_body(int kind) {
-    return kind == KIND_SOURCE ? 5 : is_panel(kind) ? 5 + 3 * panel_vertices(kind) : kind == KIND_SOURCE_VORTEX ? 8 : 7;
+    return kind == KIND_SOURCE ? 5 : is_panel(kind) ? 4 + panel_strengths(kind) + 3 * panel_vertices(kind)
+                                   : kind == KIND_SOURCE_VORTEX ? 8 : 7;
 }
 static bool has_vector_potential(int kind) { return kind == KIND_VORTEX || kind == KIND_VORTEX_GAUSS || kind == KIND_SOURCE_VORTEX; }
 

@@ -173,6 +173,11 @@ FMM_CASES = [
      dict(expansion_order=6, leaf_size_source=16, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
     ("panel-dip-quad-1.5k-p10", lambda: W.sphere_panels(16)[0], O.KIND_PANEL_DIPOLE_QUAD,
      dict(expansion_order=10, leaf_size_source=16, multipole_acceptance=0.4, scalar_potential=True, hessian=True)),
+    # Panel{SourceDipole}: a source and a dipole strength on every panel
+    ("panel-srcdip-tri-3k-p8", lambda: W.sphere_source_dipole_panels(16, quads=False)[0], O.KIND_PANEL_SOURCE_DIPOLE_TRI,
+     dict(expansion_order=8, leaf_size_source=20, multipole_acceptance=0.5, scalar_potential=True, hessian=True)),
+    ("panel-srcdip-quad-1.5k-p9", lambda: W.sphere_source_dipole_panels(16)[0], O.KIND_PANEL_SOURCE_DIPOLE_QUAD,
+     dict(expansion_order=9, leaf_size_source=16, multipole_acceptance=0.45, scalar_potential=True, hessian=True)),
 ]
 
 
@@ -333,7 +338,9 @@ def test_direct_fp64_matches_oracle():
                        (O.KIND_VORTEX_GAUSS, W.gaussian_vortex_ring(3000, seed=44)),
                        (O.KIND_SOURCE_VORTEX, W.source_vortex_points(3000, seed=46)),
                        (O.KIND_PANEL_SOURCE_TRI, W.sphere_panels(10, quads=False)[0]),
-                       (O.KIND_PANEL_DIPOLE_QUAD, W.sphere_panels(12)[0])):
+                       (O.KIND_PANEL_DIPOLE_QUAD, W.sphere_panels(12)[0]),
+                       (O.KIND_PANEL_SOURCE_DIPOLE_TRI, W.sphere_source_dipole_panels(10, quads=False)[0]),
+                       (O.KIND_PANEL_SOURCE_DIPOLE_QUAD, W.sphere_source_dipole_panels(12)[0])):
         ctx = F.default_context()
         txyz = np.ascontiguousarray(sbuf[:, :3])
         out = np.zeros((len(sbuf), 16))

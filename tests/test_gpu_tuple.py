@@ -23,7 +23,9 @@ pytestmark = pytest.mark.gpu
 _ORC = {_lib.POINT_SOURCE: O.KIND_SOURCE, _lib.POINT_VORTEX: O.KIND_VORTEX, _lib.POINT_DIPOLE: O.KIND_DIPOLE,
         _lib.VORTEX_GAUSSIAN: O.KIND_VORTEX_GAUSS, _lib.PANEL_SOURCE_TRI: O.KIND_PANEL_SOURCE_TRI,
         _lib.PANEL_DIPOLE_TRI: O.KIND_PANEL_DIPOLE_TRI, _lib.PANEL_SOURCE_QUAD: O.KIND_PANEL_SOURCE_QUAD,
-        _lib.PANEL_DIPOLE_QUAD: O.KIND_PANEL_DIPOLE_QUAD, _lib.POINT_SOURCE_VORTEX: O.KIND_SOURCE_VORTEX}
+        _lib.PANEL_DIPOLE_QUAD: O.KIND_PANEL_DIPOLE_QUAD, _lib.POINT_SOURCE_VORTEX: O.KIND_SOURCE_VORTEX,
+        _lib.PANEL_SOURCE_DIPOLE_TRI: O.KIND_PANEL_SOURCE_DIPOLE_TRI,
+        _lib.PANEL_SOURCE_DIPOLE_QUAD: O.KIND_PANEL_SOURCE_DIPOLE_QUAD}
 
 
 def _say(msg):
@@ -103,6 +105,10 @@ CASES = [
     ("source-vortex+quads-p8", [(_lib.POINT_SOURCE_VORTEX, lambda n, seed, shift: W.source_vortex_points(n, seed=seed), 6000),
                                 (_lib.PANEL_SOURCE_QUAD, _panels(True), 1500)],
      dict(expansion_order=8, leaf_size=[24, 16], multipole_acceptance=0.5, hessian=True)),
+    ("srcdip-quads+vortex-p8", [(_lib.PANEL_SOURCE_DIPOLE_QUAD,
+                                 lambda n, seed, shift: W.sphere_source_dipole_panels(16, seed=seed, radius=1.6)[0], 1536),
+                                (_lib.POINT_VORTEX, _ring, 5000)],
+     dict(expansion_order=8, leaf_size=20, multipole_acceptance=0.5, hessian=True)),
     ("vortex+dipole+source-barba", [(_lib.POINT_VORTEX, _ring, 5000), (_lib.POINT_DIPOLE, _dipoles, 3000),
                                     (_lib.POINT_SOURCE, _src, 3000)],
      dict(expansion_order=8, leaf_size=24, multipole_acceptance=0.5, interaction_list_method=F.Barba())),
@@ -213,7 +219,7 @@ def test_tuple_rejects_bad_input():
     rows = np.zeros((100, 7))
     rows[:, :3] = np.random.default_rng(3).random((100, 3))
     counts = np.array([60, 40], dtype=np.int64)
-    bad = np.array([_lib.POINT_SOURCE, 9], dtype=np.int32)   # no kernel 9
+    bad = np.array([_lib.POINT_SOURCE, 11], dtype=np.int32)   # no kernel 11
     assert lib.fmmhip_eval_set_tuple(ev._h, 2, bad.ctypes.data, counts.ctypes.data, rows.ctypes.data, 0) == _lib.FMMHIP_EINVAL
     good = np.array([_lib.POINT_SOURCE, _lib.POINT_VORTEX], dtype=np.int32)
     neg = np.array([60, -1], dtype=np.int64)

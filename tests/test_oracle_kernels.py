@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 from scipy.stats import chi
 
+from fastmultipole_amd import workloads as W
 from oracle import oracle as O
 
 
@@ -259,3 +260,46 @@ def test_source_vortex_fmm_converges_to_direct():
     for lo, hi in ((4, 7), (7, 16)):
         err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / np.abs(ref[:, lo:hi]).max()
         assert err < 1e-6, (lo, err)
+
+
+# ------------------------------------------------------------ Panel{SourceDipole}
+def _source_dipole_rows(quads, m=8):
+    """Source-dipole panels of the sphere's panels: the source strength, then a dipole strength."""
+    rows, kind = W.sphere_panels(m, quads=quads)
+    sd = np.insert(rows, 5, np.random.default_rng(34).normal(size=len(rows)) / len(rows), axis=1)
+    return rows, kind, sd, O.KIND_PANEL_SOURCE_DIPOLE_QUAD if quads else O.KIND_PANEL_SOURCE_DIPOLE_TRI
+
+
+@pytest.mark.parametrize("quads", [False, True])
+def test_source_dipole_panel_is_source_plus_dipole(quads):
+    """Panel{SourceDipole} (test/panels.jl:323-429): the source panel's closed form with the first strength plus
+    the dipole panel's with the second."""
+    rows, kind, sd, sdk = _source_dipole_rows(quads)
+    dip = np.delete(sd, 4, axis=1)
+    dip_kind = kind + 1   # the dipole panel kind of the same shape
+    xt = np.random.default_rng(35).random((200, 3)) * 2.4 - 1.2
+    got = O.direct(sdk, xt, sd, scalar_potential=True, gradient=True, hessian=True)
+    ref = (O.direct(kind, xt, rows, scalar_potential=True, gradient=True, hessian=True)[:, 3:]
+           + O.direct(dip_kind, xt, dip, scalar_potential=True, gradient=True, hessian=True)[:, 3:])
+    np.testing.assert_allclose(got[:, 3:], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("quads", [False, True])
+def test_source_dipole_panel_fmm_is_source_plus_dipole_fmm(quads):
+    """Expansions of Panel{SourceDipole} (src/bodytomultipole.jl:717-720: the source's, then the dipole's): the
+    source-dipole FMM equals the source-panel FMM plus the dipole-panel FMM on the same panels (same trees and
+    lists, linear passes), and for triangles it converges to the direct sum."""
+    rows, kind, sd, sdk = _source_dipole_rows(quads, m=16)
+    dip = np.delete(sd, 4, axis=1)
+    kw = dict(p=14, leaf_size_source=12, multipole_acceptance=0.5, scalar_potential=True, gradient=True)
+    run = O.Run(sdk, sd[:, :3], sd, **kw).fmm()
+    assert len(run.list(0)) > 0
+    got = run.targets()
+    parts = (O.Run(kind, rows[:, :3], rows, **kw).fmm().targets()[:, 3:7]
+             + O.Run(kind + 1, dip[:, :3], dip, **kw).fmm().targets()[:, 3:7])
+    np.testing.assert_allclose(got[:, 3:7], parts, rtol=1e-11, atol=1e-12 * np.abs(parts).max())
+    if not quads:
+        ref = O.direct(sdk, sd[:, :3], sd, scalar_potential=True, gradient=True)
+        for lo, hi in ((3, 4), (4, 7)):
+            err = np.abs(got[:, lo:hi] - ref[:, lo:hi]).max() / np.abs(ref[:, lo:hi]).max()
+            assert err < 1e-6, (lo, err)
