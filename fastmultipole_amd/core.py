@@ -805,18 +805,36 @@ def autotune(system, *, error_tolerance=None, expansion_order=20, leaf_sizes=(16
 
 def direct(target_systems, source_systems=None, *, scalar_potential=False, gradient=True, hessian=False, precision="fp64",
            ctx=None):
-    """direct!(targets, sources; scalar_potential, gradient, hessian) of src/direct.jl:20-75, on the GPU."""
+    """direct!(targets, sources; scalar_potential, gradient, hessian) of src/direct.jl:20-75, on the GPU.
+
+    Tuples of systems: every source system acts on every target system, sources outermost as in
+    src/direct.jl:64-69; the switches may be given per target system. Returns the (n, 16) target buffer, or
+    one per target system for a tuple."""
     if source_systems is None:
         source_systems = target_systems
-    tsys, ssys = _as_tuple(target_systems)[0], _as_tuple(source_systems)[0]
+    targets, sources = _as_tuple(target_systems), _as_tuple(source_systems)
     ctx = ctx or default_context()
-    kernel = _kernel_of(ssys)
-    txyz = np.ascontiguousarray(tsys.positions(), dtype=np.float64)
-    sbuf = np.ascontiguousarray(ssys.source_buffer(), dtype=np.float64)
-    out = np.zeros((txyz.shape[0], 16))
-    out[:, :3] = txyz
     prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
-    ctx.check(ctx.lib.fmmhip_direct(ctx._h, kernel, txyz.shape[0], txyz.ctypes.data, sbuf.shape[0], sbuf.ctypes.data,
-                                    int(scalar_potential), int(gradient), int(hessian), prec, out.ctypes.data, 0))
-    tsys.buffer_to_target_system(out, DerivativesSwitch(bool(scalar_potential), bool(gradient), bool(hessian)))
-    return out
+
+    def per_target(v):
+        v = list(v) if np.ndim(v) > 0 else [v] * len(targets)
+        if len(v) != len(targets):
+            raise ValueError("one derivative switch per target system")
+        return [bool(x) for x in v]
+
+    ps, gs, hs = per_target(scalar_potential), per_target(gradient), per_target(hessian)
+    txyz = [np.ascontiguousarray(t.positions(), dtype=np.float64) for t in targets]
+    outs = []
+    for x in txyz:
+        o = np.zeros((x.shape[0], 16))
+        o[:, :3] = x
+        outs.append(o)
+    for ssys in sources:
+        kernel = _kernel_of(ssys)
+        sbuf = np.ascontiguousarray(ssys.source_buffer(), dtype=np.float64)
+        for i, x in enumerate(txyz):
+            ctx.check(ctx.lib.fmmhip_direct(ctx._h, kernel, x.shape[0], x.ctypes.data, sbuf.shape[0], sbuf.ctypes.data,
+                                            int(ps[i]), int(gs[i]), int(hs[i]), prec, outs[i].ctypes.data, 0))
+    for i, t in enumerate(targets):
+        t.buffer_to_target_system(outs[i], DerivativesSwitch(ps[i], gs[i], hs[i]))
+    return outs[0] if len(targets) == 1 and not isinstance(target_systems, tuple) else outs

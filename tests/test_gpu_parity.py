@@ -459,3 +459,27 @@ def test_mixed_form_meets_bench_bar():
     ev_g = W.relative_error(got[idx, 7:16], ref[:, 7:16])
     _say(f"    mixed form: velocity max {ev_v[0]:.3e}, gradient max {ev_g[0]:.3e}")
     assert ev_v[0] <= 1e-6 and ev_g[0] <= 1e-6, (ev_v[0], ev_g[0])
+
+
+def test_direct_on_tuples_sums_every_source_system():
+    """direct!((t1, t2), (s1, s2)) (src/direct.jl:43-75): every source system acts on every target system, each
+    target system with its own switches; against the oracle's direct sums."""
+    _say("[gpu] direct on tuples")
+    src = F.Gravitational(W.uniform_sources(2000, seed=47))
+    vb = W.vortex_ring(1500, seed=48)
+    vor = F.VortexParticles(vb[:, :3], vb[:, 4:7], vb[:, 3])
+    outs = F.direct((src, vor), (src, vor), scalar_potential=[True, False], gradient=True, hessian=[False, True])
+    assert isinstance(outs, list) and len(outs) == 2
+    for i, t in enumerate((src, vor)):
+        ref = np.zeros((t.get_n_bodies(), 16))
+        for s in (src, vor):
+            kind = O.KIND_SOURCE if s is src else O.KIND_VORTEX
+            ref[:, 3:] += O.direct(kind, t.positions(), s.source_buffer(), scalar_potential=i == 0, gradient=True,
+                                   hessian=i == 1)[:, 3:]
+        for lo, hi in ((3, 4), (4, 7), (7, 16)):
+            sc = np.abs(ref[:, lo:hi]).max()
+            if sc:
+                assert np.abs(outs[i][:, lo:hi] - ref[:, lo:hi]).max() / sc < 1e-13, (i, lo)
+    # no Hessian where it was not asked for (the point-source kernel writes its potential whatever the switches,
+    # test/gravitational.jl:92-98, so the second target system's potential slot is not empty)
+    assert np.abs(outs[0][:, 7:16]).max() == 0 and np.abs(outs[1][:, 7:16]).max() > 0
