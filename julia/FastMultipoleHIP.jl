@@ -502,6 +502,18 @@ function hip_direct!(hip::HipEval, t, s=t; scalar_potential=false, gradient=true
     return t
 end
 
+# direct!(targets::Tuple, sources::Tuple) (src/direct.jl:43-75): every source system on every target system,
+# sources outermost, the switches per target system; systems without a hip_kernel take the reference's path
+function hip_direct!(hip::HipEval, targets::Tuple, sources::Tuple=targets; scalar_potential=false, gradient=true,
+                     hessian=false, precision=:fp64)
+    nt = length(targets)
+    ps, gs, hs = (v isa AbstractVector ? v : fill(v, nt) for v in (scalar_potential, gradient, hessian))
+    for s in sources, (i, t) in enumerate(targets)
+        hip_direct!(hip, t, s; scalar_potential=ps[i], gradient=gs[i], hessian=hs[i], precision)
+    end
+    return targets
+end
+
 #--- FastGaussSeidel / solve! (src/solve.jl:538-946) ---#
 
 # solved scalar strengths back into the user system: value_to_strength! into a source buffer, then
