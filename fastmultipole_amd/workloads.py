@@ -150,7 +150,8 @@ def sphere_panels(m, quads=True, seed=5, radius=1.0):
     n = np.cross(faces[:, 1] - faces[:, 0], faces[:, 2] - faces[:, 0])
     flip = np.einsum("ij,ij->i", n, faces.mean(axis=1)) < 0      # counter-clockwise about the outward normal
     faces[flip] = faces[flip, ::-1]
-    n[flip] = -n[flip]
+    # the plane of the (reordered) first three vertices, onto which the fourth is projected
+    n = np.cross(faces[:, 1] - faces[:, 0], faces[:, 2] - faces[:, 0])
     n /= np.linalg.norm(n, axis=1, keepdims=True)
     faces[:, 3] -= np.einsum("ij,ij->i", faces[:, 3] - faces[:, 0], n)[:, None] * n
     if quads:
