@@ -204,9 +204,10 @@ def batch_sample(F, W, _lib, args):
         t1 = time.perf_counter()
         st = ev.fmm(F.make_options(leaf_size_source=ls, leaf_size_target=ls, **kw))
         t2 = time.perf_counter()
-        ev.targets()    # results back to the host, as the server's wait delivers them
+        rows = ev.targets()    # results back to the host, as the server's wait delivers them
         t3 = time.perf_counter()
         _, sys_ok = ev.batch_stats()
+        outs[kind] = rows
         res[kind] = (len(counts), int(sys_ok.sum()), (t3 - t0) * 1e3,
                      {"copy_in_ms": round((t1 - t0) * 1e3, 1), "fmm_ms": round((t2 - t1) * 1e3, 1),
                       "copy_out_ms": round((t3 - t2) * 1e3, 1),
@@ -214,6 +215,7 @@ def batch_sample(F, W, _lib, args):
                                                               "ms_upward", "ms_m2l", "ms_downward")}})
 
     fused = None
+    outs = {}   # each kind's results of the last round, in its systems' order
     for rnd in range(2):
         res = {}
         t0 = time.perf_counter()
@@ -233,6 +235,27 @@ def batch_sample(F, W, _lib, args):
         fused = summary(dt, lat, status)
         fused["phases"] = {k: v[3] for k, v in res.items()}
         log(f"batch (fused) round {rnd}: {fused}")
+    # measured accuracy of the last round: each system's gradient slots (ring: velocity) on 32 sampled
+    # targets against the FP64 direct sum over that system's own sources (the status above is the error
+    # controller's own verdict, which reports a miss once an interaction's estimate reaches the order cap)
+    accuracy = {}
+    for kind in ("cube", "ring"):
+        kernel, buf, counts = staged[kind]
+        ctx = evs[kind].ctx
+        offs = np.concatenate([[0], np.cumsum(counts)])
+        errs = []
+        for k in range(len(counts)):
+            src = np.ascontiguousarray(buf[offs[k]:offs[k + 1]])
+            pick = np.sort(np.random.default_rng(k).choice(len(src), size=min(32, len(src)), replace=False))
+            txyz = np.ascontiguousarray(src[pick, :3])
+            ref = np.zeros((len(pick), 16))
+            ctx.check(ctx.lib.fmmhip_direct(ctx._h, kernel, len(pick), txyz.ctypes.data, len(src), src.ctypes.data, 0, 1, 0,
+                                            _lib.FP64, ref.ctypes.data, 0))
+            errs.append(W.relative_error(outs[kind][offs[k] + pick, 4:7], ref[:, 4:7])[0])
+        accuracy[kind] = {"slots": "velocity" if kind == "ring" else "gradient",
+                          "max_rel_err": float(np.max(errs)), "median_rel_err": float(np.median(errs)),
+                          "systems_within_1e-6": int(np.sum(np.array(errs) <= 1e-6)), "systems": len(errs)}
+    log(f"batch accuracy: {accuracy}")
     for ev in evs.values():
         ctx = ev.ctx
         ev.close()
@@ -257,6 +280,7 @@ def batch_sample(F, W, _lib, args):
                         "launches), concurrently on two streams; a system's latency is the end of its kind's batch",
                 "server": dict(server, slots=args.batch_slots,
                                mode="512 requests to the request server at once, overlapping on the device"),
+                "accuracy": dict(accuracy, sample="32 targets per system, FP64 direct sum over the system's sources"),
                 "tuned_leaf_size": {k: v["leaf_size"] for k, v in tuned.items()},
                 "tuned_on": f"autotune on one instance of each kind at N = {med} (body-weighted median)",
                 "tuned_error_success": {k: bool(v["error_success"]) for k, v in tuned.items()},
