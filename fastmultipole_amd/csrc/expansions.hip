@@ -2080,35 +2080,48 @@ __device__ inline void solid_pair(double x, double y, double z, int n, int k, co
     if (n - 1 >= k) finish(n - 1, pr, pi, R[2], R[3]);
 }
 
-// Column k of the regular solid harmonics R_j^k, j = k..P, of one position by the
-// Cartesian recurrence above (no trigonometry), into tab[(j(j+1)/2 + k)][re/im].
-__device__ inline void harmonic_column(double x, double y, double z, int k, int P, const double* inv_fact, double* tab) {
-    double qr = 1.0, qi = 0.0;
-    for (int j = 1; j <= k; ++j) {   // Q_k^k
+// Column k of the regular solid harmonics R_j^k, j = k..P, of two positions by the Cartesian
+// recurrence above (no trigonometry), into tab0 / tab1[(j(j+1)/2 + k)][re/im]; the two recursions are
+// independent, so their latencies overlap; the second is skipped when has1 is false
+__device__ inline void harmonic_column2(double x0, double y0, double z0, double x1, double y1, double z1, bool has1, int k,
+                                        int P, const double* inv_fact, double* tab0, double* tab1) {
+    double qr0 = 1.0, qi0 = 0.0, qr1 = 1.0, qi1 = 0.0;
+    for (int j = 1; j <= k; ++j) {
         const double f = -(double)(2 * j - 1);
-        const double nr = f * (x * qr - y * qi), ni = f * (x * qi + y * qr);
-        qr = nr; qi = ni;
+        const double nr0 = f * (x0 * qr0 - y0 * qi0), ni0 = f * (x0 * qi0 + y0 * qr0);
+        const double nr1 = f * (x1 * qr1 - y1 * qi1), ni1 = f * (x1 * qi1 + y1 * qr1);
+        qr0 = nr0; qi0 = ni0; qr1 = nr1; qi1 = ni1;
     }
-    const double rho2 = x * x + y * y + z * z;
-    double pr = 0.0, pi = 0.0, cr = qr, ci = qi;   // Q_{j-1}, Q_j
+    const double rho20 = x0 * x0 + y0 * y0 + z0 * z0, rho21 = x1 * x1 + y1 * y1 + z1 * z1;
+    double pr0 = 0.0, pi0 = 0.0, cr0 = qr0, ci0 = qi0;
+    double pr1 = 0.0, pi1 = 0.0, cr1 = qr1, ci1 = qi1;
     for (int j = k; j <= P; ++j) {
         if (j > k) {
-            const double a = (double)(2 * j - 1) * z, b = (double)(j + k - 1) * rho2, d = 1.0 / (double)(j - k);
-            const double nr = (a * cr - b * pr) * d, ni = (a * ci - b * pi) * d;
-            pr = cr; pi = ci; cr = nr; ci = ni;
+            const double d = 1.0 / (double)(j - k);
+            const double a0 = (double)(2 * j - 1) * z0, b0 = (double)(j + k - 1) * rho20;
+            const double nr0 = (a0 * cr0 - b0 * pr0) * d, ni0 = (a0 * ci0 - b0 * pi0) * d;
+            pr0 = cr0; pi0 = ci0; cr0 = nr0; ci0 = ni0;
+            const double a1 = (double)(2 * j - 1) * z1, b1 = (double)(j + k - 1) * rho21;
+            const double nr1 = (a1 * cr1 - b1 * pr1) * d, ni1 = (a1 * ci1 - b1 * pi1) * d;
+            pr1 = cr1; pi1 = ci1; cr1 = nr1; ci1 = ni1;
         }
-        const double s = ((j & 1) ? -1.0 : 1.0) * inv_fact[j + k];   // R = (-1)^j i^k Q / (j+k)!
-        const double ar = cr * s, ai = ci * s;
-        double o0, o1;
-        switch (k & 3) {
-            case 0: o0 = ar; o1 = ai; break;
-            case 1: o0 = -ai; o1 = ar; break;
-            case 2: o0 = -ar; o1 = -ai; break;
-            default: o0 = ai; o1 = -ar; break;
-        }
+        const double s = ((j & 1) ? -1.0 : 1.0) * inv_fact[j + k];
         const int i = ((j * (j + 1)) >> 1) + k;
-        tab[2 * i] = o0;
-        tab[2 * i + 1] = o1;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !has1) break;
+            const double ar = (h ? cr1 : cr0) * s, ai = (h ? ci1 : ci0) * s;
+            double o0, o1;
+            switch (k & 3) {
+                case 0: o0 = ar; o1 = ai; break;
+                case 1: o0 = -ai; o1 = ar; break;
+                case 2: o0 = -ar; o1 = -ai; break;
+                default: o0 = ai; o1 = -ar; break;
+            }
+            double* tab = h ? tab1 : tab0;
+            tab[2 * i] = o0;
+            tab[2 * i + 1] = o1;
+        }
     }
 }
 
@@ -2121,10 +2134,11 @@ __device__ inline void tab_get(const double* tab, int j, int k, double& r, doubl
 }
 
 // P2M (body_to_multipole!, src/bodytomultipole.jl:387-496): one wavefront per leaf. The
-// bodies go through in groups of 64/(P+1): lane l builds column l % (P+1) of the harmonic
-// table of body l / (P+1) (no barrier inside), then every lane adds the group's
-// contributions to the coefficients it owns (i = lane + 64 r), reading neighbours from
-// the tables. Point{Vortex} uses the mirrored position (mirrored_source_to_vortex!).
+// bodies go through in groups of 2 x 64/(P+1): lane l builds column l % (P+1) of the harmonic
+// tables of bodies l / (P+1) and G + l / (P+1) (no barrier inside; 1/k! from LDS), then every
+// lane adds the group's contributions, body by body, to the coefficients it owns
+// (i = lane + 64 r), reading neighbours from the tables. Point{Vortex} uses the mirrored
+// position (mirrored_source_to_vortex!).
 template <int KERNEL>
 __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const int64_t* __restrict__ leaf_index,
                                             const double* __restrict__ x, const double* __restrict__ y,
@@ -2133,9 +2147,12 @@ __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const
     extern __shared__ double lds[];
     constexpr int NR = 4;   // coefficients per lane: H(20) = 231 <= 4 x 64
     const int H = nh(P);
-    const int G = 64 / (P + 1);
-    double* tabs = lds;                               // G tables of 2H doubles
-    double* qs = lds + (size_t)G * 2 * H;             // G x 3 strengths
+    const int G = 64 / (P + 1);                       // bodies per column pass; two passes per group
+    double* tabs = lds;                               // 2G tables of 2H doubles
+    double* qs = lds + (size_t)2 * G * 2 * H;         // 2G x 3 strengths
+    double* fs = qs + (size_t)2 * G * 3;              // 1/k!, k = 0 .. 2P
+    for (int i = threadIdx.x; i <= 2 * P; i += 64) fs[i] = inv_fact[i];
+    __syncthreads();
     const int64_t ib = leaf_index[blockIdx.x];
     const Branch B = br[ib];
     const double cx = B.source_center[0], cy = B.source_center[1], cz = B.source_center[2];
@@ -2151,17 +2168,26 @@ __global__ __launch_bounds__(64) void k_p2m(const Branch* __restrict__ br, const
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0.0;
     const bool mirror = KERNEL == FMMHIP_POINT_VORTEX;
-    for (int64_t b0 = 0; b0 < B.n_bodies; b0 += G) {
-        const int cnt = (int)min<int64_t>(G, B.n_bodies - b0);
+    for (int64_t b0 = 0; b0 < B.n_bodies; b0 += 2 * G) {
+        const int cnt = (int)min<int64_t>(2 * G, B.n_bodies - b0);
         const int g = tid / (P + 1), k = tid % (P + 1);
-        if (g < cnt) {
-            const int64_t j = B.first + b0 + g;
-            double dx = x[j] - cx, dy = y[j] - cy, dz = z[j] - cz;
-            if (mirror) { dx = -dx; dy = -dy; dz = -dz; }
-            harmonic_column(dx, dy, dz, k, P, inv_fact, tabs + (size_t)g * 2 * H);
+        if (g < G && g < cnt) {
+            const bool has1 = G + g < cnt;
+            const int64_t j0 = B.first + b0 + g, j1 = has1 ? j0 + G : j0;
+            double dx0 = x[j0] - cx, dy0 = y[j0] - cy, dz0 = z[j0] - cz;
+            double dx1 = x[j1] - cx, dy1 = y[j1] - cy, dz1 = z[j1] - cz;
+            if (mirror) { dx0 = -dx0; dy0 = -dy0; dz0 = -dz0; dx1 = -dx1; dy1 = -dy1; dz1 = -dz1; }
+            harmonic_column2(dx0, dy0, dz0, dx1, dy1, dz1, has1, k, P, fs, tabs + (size_t)g * 2 * H,
+                             tabs + (size_t)(G + g) * 2 * H);
             if (k == 0) {
-                qs[3 * g] = q[j];
-                if (KERNEL != FMMHIP_POINT_SOURCE) { qs[3 * g + 1] = q[ns + j]; qs[3 * g + 2] = q[2 * ns + j]; }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (h == 1 && !has1) break;
+                    const int gg = g + h * G;
+                    const int64_t j = h ? j1 : j0;
+                    qs[3 * gg] = q[j];
+                    if (KERNEL != FMMHIP_POINT_SOURCE) { qs[3 * gg + 1] = q[ns + j]; qs[3 * gg + 2] = q[2 * ns + j]; }
+                }
             }
         }
         __syncthreads();
@@ -2384,7 +2410,7 @@ void run_p2m(Context& C, int kernel, Tree& st, int p, const Branch* sbr) {
         LAUNCH_CHECK();
         return;
     }
-    const size_t p2m_sh = sizeof(double) * (size_t)(64 / (p + 1)) * (2 * nh(p) + 3);
+    const size_t p2m_sh = sizeof(double) * ((size_t)2 * (64 / (p + 1)) * (2 * nh(p) + 3) + 2 * p + 1);
 #define FMMHIP_P2M(K, Q)                                                                                               \
     k_p2m<K><<<(unsigned)st.n_leaves, 64, p2m_sh, C.stream>>>(sbr, st.leaf_index.data(), st.x.data(), st.y.data(),             \
                                                         st.z.data(), Q, st.n, st.expansions.data(), p, tb.inv_fact)
